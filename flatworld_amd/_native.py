@@ -1,0 +1,351 @@
+"""ctypes binding of libflatworld_fem.so (include/flatworld_fem.h).
+
+The library is the only compute path of this package.  Loading fails loudly
+when it is missing or when no GPU is visible; there is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflatworld_fem.so")
+
+FW_OK = 0
+FW_E_INVALID = -1
+FW_E_CAPACITY = -2
+FW_E_HIP = -3
+FW_E_NOT_FOUND = -4
+FW_E_UNSUPPORTED = -5
+
+FW_STEP_DIAG = 1
+FW_STEP_NO_CONTACT = 2
+FW_STEP_PER_SCENE = 4
+
+FIELD_X, FIELD_V, FIELD_U, FIELD_FORCE, FIELD_MASS = 1, 2, 3, 4, 5
+FIELD_STRESS, FIELD_STRAIN, FIELD_EPS_P, FIELD_EQPS = 6, 7, 8, 9
+FIELD_AREA, FIELD_JINV, FIELD_YIELD, FIELD_CONTACT, FIELD_AABB = 10, 11, 12, 13, 14
+
+# (per_node, width, dtype)
+FIELD_SHAPES = {
+    FIELD_X: (True, 2, np.float32),
+    FIELD_V: (True, 2, np.float32),
+    FIELD_U: (True, 2, np.float32),
+    FIELD_FORCE: (True, 2, np.float32),
+    FIELD_MASS: (True, 1, np.float32),
+    FIELD_CONTACT: (True, 1, np.uint8),
+    FIELD_STRESS: (False, 3, np.float32),
+    FIELD_STRAIN: (False, 3, np.float32),
+    FIELD_EPS_P: (False, 3, np.float32),
+    FIELD_EQPS: (False, 1, np.float32),
+    FIELD_AREA: (False, 1, np.float32),
+    FIELD_JINV: (False, 4, np.float32),
+    FIELD_YIELD: (False, 1, np.uint8),
+}
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class GroundDesc(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("penalty", ctypes.c_float),
+        ("aabb_lo", ctypes.c_float * 2),
+        ("aabb_hi", ctypes.c_float * 2),
+        ("point", ctypes.c_float * 2),
+        ("normal", ctypes.c_float * 2),
+        ("heights", _f32p),
+        ("hf_nx", ctypes.c_int32),
+        ("hf_lb_x", ctypes.c_float),
+        ("hf_ub_x", ctypes.c_float),
+        ("reverse", ctypes.c_int32),
+        ("occ", _i32p),
+        ("vox_nx", ctypes.c_int32),
+        ("vox_ny", ctypes.c_int32),
+        ("vox_lb", ctypes.c_float * 2),
+        ("vox_dx", ctypes.c_float),
+        ("vox_dz", ctypes.c_float),
+    ]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", ctypes.c_int32),
+        ("n_elems", ctypes.c_int32),
+        ("n_grounds", ctypes.c_int32),
+        ("n_bc_values", ctypes.c_int32),
+        ("x0", _f32p),
+        ("v0", _f32p),
+        ("conn", _i32p),
+        ("flags", _u32p),
+        ("bc_values", _f32p),
+        ("mat_type", ctypes.c_int32),
+        ("E", ctypes.c_float),
+        ("nu", ctypes.c_float),
+        ("rho", ctypes.c_float),
+        ("sigma_y", ctypes.c_float),
+        ("H", ctypes.c_float),
+        ("dt", ctypes.c_float),
+        ("damping", ctypes.c_float),
+        ("horizon", ctypes.c_int32),
+        ("track_disp", ctypes.c_int32),
+        ("grounds", ctypes.POINTER(GroundDesc)),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("tile_nodes", ctypes.c_int32),
+        ("tile_slots", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("grid_per_cu", ctypes.c_int32),
+    ]
+
+
+class ArenaInfo(ctypes.Structure):
+    _fields_ = [
+        ("capacity_bytes", ctypes.c_int64),
+        ("used_bytes", ctypes.c_int64),
+        ("n_scenes", ctypes.c_int32),
+        ("n_tiles", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int64),
+        ("n_elems", ctypes.c_int64),
+        ("n_elems_j2", ctypes.c_int64),
+        ("global_step", ctypes.c_int64),
+        ("algo_bytes_per_substep", ctypes.c_double),
+        ("halo_slots", ctypes.c_int64),
+    ]
+
+
+# exported symbols and their signatures (restype, argtypes)
+SIGNATURES = {
+    "fw_last_error": (ctypes.c_char_p, []),
+    "fw_abi_version": (ctypes.c_int32, []),
+    "fw_arena_create": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_void_p)]),
+    "fw_arena_destroy": (ctypes.c_int32, [ctypes.c_void_p]),
+    "fw_arena_get_info": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ArenaInfo)]),
+    "fw_scene_footprint": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_int64)]),
+    "fw_scene_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_int32)]),
+    "fw_scene_remove": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_scene_progress": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "fw_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
+    "fw_scene_reset_gate": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_step_graph": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]),
+    "fw_sync": (ctypes.c_int32, [ctypes.c_void_p]),
+    "fw_scene_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
+    "fw_scene_write": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
+    "fw_tiling_check": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "fw_bandwidth_probe": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "fw_time_steps": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the shared library (no GPU needed just to load it)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is missing: build it with `python -m flatworld_amd.build` (hipcc, gfx950). "
+            "flatworld_amd has no CPU fallback."
+        )
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    """Map a C status to the reference's exception types."""
+    if rc == FW_OK:
+        return
+    msg = load().fw_last_error().decode(errors="replace")
+    if what:
+        msg = f"{what}: {msg}"
+    if rc == FW_E_INVALID:
+        raise ValueError(msg)
+    if rc == FW_E_CAPACITY:
+        raise RuntimeError(msg)
+    if rc == FW_E_NOT_FOUND:
+        raise KeyError(msg)
+    if rc == FW_E_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise NativeError(msg)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def scene_desc(spec, track_disp=True):
+    """Build the C description of a SceneSpec; returns (desc, keepalive list)."""
+    keep = []
+    x0 = np.ascontiguousarray(spec.x0, dtype=np.float32)
+    v0 = np.ascontiguousarray(spec.v0, dtype=np.float32)
+    conn = np.ascontiguousarray(spec.conn, dtype=np.int32)
+    flags, table = spec.node_flags()
+    keep += [x0, v0, conn, flags, table]
+    gs = []
+    for g in spec.grounds:
+        d = GroundDesc()
+        d.kind = int(g.kind)
+        d.penalty = float(g.penalty)
+        d.aabb_lo[:] = [float(v) for v in g.aabb_lo]
+        d.aabb_hi[:] = [float(v) for v in g.aabb_hi]
+        if g.kind == 1:
+            d.point[:] = [float(v) for v in g.point]
+            d.normal[:] = [float(v) for v in g.normal]
+        elif g.kind == 2:
+            h = np.ascontiguousarray(g.heights, dtype=np.float32)
+            keep.append(h)
+            d.heights = _ptr(h, _f32p)
+            d.hf_nx = int(h.shape[0])
+            d.hf_lb_x = float(g.hf_lb_x)
+            d.hf_ub_x = float(g.hf_ub_x)
+            d.reverse = int(g.reverse)
+        elif g.kind == 3:
+            occ = np.ascontiguousarray(g.occ, dtype=np.int32)
+            keep.append(occ)
+            d.occ = _ptr(occ, _i32p)
+            d.vox_nx, d.vox_ny = int(occ.shape[0]), int(occ.shape[1])
+            d.vox_lb[:] = [float(v) for v in g.vox_lb]
+            d.vox_dx = float(g.vox_dx)
+            d.vox_dz = float(g.vox_dz)
+        gs.append(d)
+    garr = (GroundDesc * max(len(gs), 1))(*gs)
+    keep.append(garr)
+    sd = SceneDesc()
+    sd.n_nodes = spec.n_nodes
+    sd.n_elems = spec.n_elems
+    sd.n_grounds = len(gs)
+    sd.n_bc_values = int(table.shape[0])
+    sd.x0 = _ptr(x0, _f32p)
+    sd.v0 = _ptr(v0, _f32p)
+    sd.conn = _ptr(conn, _i32p)
+    sd.flags = _ptr(flags, _u32p)
+    sd.bc_values = _ptr(table, _f32p)
+    sd.mat_type = int(spec.mat_type)
+    sd.E, sd.nu, sd.rho = float(spec.E), float(spec.nu), float(spec.rho)
+    sd.sigma_y, sd.H = float(spec.sigma_y), float(spec.H)
+    sd.dt = float(np.float32(spec.dt))
+    sd.damping = float(np.float32(spec.damping))
+    sd.horizon = int(spec.horizon)
+    sd.track_disp = 1 if track_disp else 0
+    sd.grounds = ctypes.cast(garr, ctypes.POINTER(GroundDesc))
+    return sd, keep
+
+
+def tiling_check(spec, tile_nodes=512, tile_slots=1024):
+    """Run the host-side tiler on a scene and verify it (no GPU)."""
+    d, keep = scene_desc(spec)
+    stats = (ctypes.c_int64 * 4)()
+    check(load().fw_tiling_check(ctypes.byref(d), int(tile_nodes), int(tile_slots), stats), "fw_tiling_check")
+    return dict(tiles=stats[0], halo_slots=stats[1], max_slots=stats[2], csr_entries=stats[3])
+
+
+def bandwidth_probe(device=0, nbytes=1 << 30, reps=20):
+    """Plain copy kernel: (read+write GB/s, average kernel ms)."""
+    g = ctypes.c_double()
+    k = ctypes.c_double()
+    check(load().fw_bandwidth_probe(int(device), int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(k)), "fw_bandwidth_probe")
+    return g.value, k.value
+
+
+class Arena:
+    """Owner of one fw_arena: a packed HBM region holding many scenes."""
+
+    def __init__(self, device=0, capacity_bytes=0, tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0):
+        L = load()
+        self._L = L
+        t = Tuning(tile_nodes, tile_slots, block, grid_per_cu)
+        h = ctypes.c_void_p()
+        check(L.fw_arena_create(int(device), int(capacity_bytes), ctypes.byref(t), ctypes.byref(h)), "fw_arena_create")
+        self.h = h
+        self.tuning = dict(tile_nodes=tile_nodes, tile_slots=tile_slots, block=block, grid_per_cu=grid_per_cu)
+        self._shapes = {}
+
+    def close(self):
+        if self.h:
+            self._L.fw_arena_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = ArenaInfo()
+        check(self._L.fw_arena_get_info(self.h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in ArenaInfo._fields_}
+
+    def footprint(self, spec, track_disp=True):
+        d, keep = scene_desc(spec, track_disp)
+        b = ctypes.c_int64()
+        check(self._L.fw_scene_footprint(self.h, ctypes.byref(d), ctypes.byref(b)), "fw_scene_footprint")
+        return int(b.value)
+
+    def add(self, spec, track_disp=True):
+        d, keep = scene_desc(spec, track_disp)
+        sid = ctypes.c_int32()
+        check(self._L.fw_scene_add(self.h, ctypes.byref(d), ctypes.byref(sid)), "fw_scene_add")
+        self._shapes[sid.value] = (spec.n_nodes, spec.n_elems)
+        return sid.value
+
+    def remove(self, sid):
+        check(self._L.fw_scene_remove(self.h, int(sid)), "fw_scene_remove")
+        self._shapes.pop(sid, None)
+
+    def progress(self, sid):
+        d = ctypes.c_int64()
+        f = ctypes.c_int32()
+        check(self._L.fw_scene_progress(self.h, int(sid), ctypes.byref(d), ctypes.byref(f)))
+        return int(d.value), bool(f.value)
+
+    def step(self, n, dt_override=-1.0, flags=0, damping_override=-1.0):
+        check(self._L.fw_step(self.h, int(n), float(dt_override), float(damping_override), int(flags)), "fw_step")
+
+    def reset_gate(self, sid):
+        check(self._L.fw_scene_reset_gate(self.h, int(sid)), "fw_scene_reset_gate")
+
+    def step_graph(self, n, flags=0):
+        check(self._L.fw_step_graph(self.h, int(n), int(flags)), "fw_step_graph")
+
+    def sync(self):
+        check(self._L.fw_sync(self.h), "fw_sync")
+
+    def read(self, sid, field):
+        if field == FIELD_AABB:
+            out = np.zeros(4, np.float32)
+            check(self._L.fw_scene_read(self.h, int(sid), field, out.ctypes.data, out.nbytes), "fw_scene_read")
+            return out
+        n_nodes, n_elems = self._shapes[sid]
+        per_node, width, dt = FIELD_SHAPES[field]
+        count = n_nodes if per_node else n_elems
+        out = np.zeros((count, width) if width > 1 else (count,), dtype=dt)
+        check(self._L.fw_scene_read(self.h, int(sid), int(field), out.ctypes.data, out.nbytes), "fw_scene_read")
+        return out
+
+    def write(self, sid, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        check(self._L.fw_scene_write(self.h, int(sid), int(field), a.ctypes.data, a.nbytes), "fw_scene_write")
+
+    def time_steps(self, n, mode=0):
+        tot = ctypes.c_double()
+        ker = ctypes.c_double()
+        check(self._L.fw_time_steps(self.h, int(n), int(mode), ctypes.byref(tot), ctypes.byref(ker)), "fw_time_steps")
+        return tot.value, ker.value

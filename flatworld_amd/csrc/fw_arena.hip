@@ -1,0 +1,1121 @@
+// fw_arena.hip -- host runtime behind include/flatworld_fem.h.
+//
+// Owns one HBM region per arena and carves one contiguous block per scene out
+// of it (first-fit free list).  At scene admission it
+//   1. orders nodes for locality (Morton order of the rest positions when the
+//      scene needs more than one tile) and cuts them into tiles of at most
+//      tile_nodes owned nodes whose incident elements fit tile_slots LDS slots;
+//   2. gives every element to the tile holding its lowest-ordered node and
+//      stores elements grouped by owner tile, so each tile's owned elements
+//      are one contiguous range and only the few elements it shares with
+//      neighbours need an index list (halo);
+//   3. builds, per tile, the node -> (slot, corner) incidence lists sorted by
+//      the caller's element id, which fixes every nodal sum to the order of
+//      the reference's serial atomic adds;
+//   4. writes the scene block image into pinned memory and copies it with one
+//      asynchronous DMA, then launches the prepare kernel.
+// Scenes are never serialised behind each other: a substep is one launch over
+// the tile table of the whole arena (or, for comparison, one launch per scene).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/flatworld_fem.h"
+#include "fw_types.h"
+
+extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
+                                        int tile_base, int tile_count, float dt_override, float damping_override,
+                                        uint32_t flags, int grid, int block, int slots, hipStream_t stream);
+extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
+                                        int slots, hipStream_t stream);
+extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
+extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(FW_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+constexpr int64_t ALIGN = 256;
+inline int64_t align_up(int64_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+
+// ------------------------------------------------------------------ tiling
+
+struct Tiling {
+    std::vector<int32_t> node_perm;  // internal -> caller node id
+    std::vector<int32_t> node_inv;   // caller -> internal
+    std::vector<int32_t> elem_perm;  // internal -> caller element id
+    std::vector<int32_t> elem_inv;
+    std::vector<TileDev> tiles;      // scene field left 0; offsets relative to the scene
+    std::vector<int32_t> halo;
+    std::vector<uint16_t> csr_ptr;
+    std::vector<uint16_t> csr_ent;
+    int64_t halo_slots = 0;
+};
+
+uint32_t spread16(uint32_t v) {
+    v &= 0xFFFF;
+    v = (v | (v << 8)) & 0x00FF00FF;
+    v = (v | (v << 4)) & 0x0F0F0F0F;
+    v = (v | (v << 2)) & 0x33333333;
+    v = (v | (v << 1)) & 0x55555555;
+    return v;
+}
+
+int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling& T) {
+    const int N = d->n_nodes, E = d->n_elems;
+    // incidence (caller ids): node -> (element, corner), element-major so lists are ascending
+    std::vector<int32_t> inc_ptr(N + 1, 0);
+    for (int64_t i = 0; i < 3 * (int64_t)E; ++i) {
+        int32_t n = d->conn[i];
+        if (n < 0 || n >= N) return fail(FW_E_INVALID, "connectivity references node " + std::to_string(n) + " outside [0, n_nodes)");
+        inc_ptr[n + 1]++;
+    }
+    for (int i = 0; i < N; ++i) inc_ptr[i + 1] += inc_ptr[i];
+    std::vector<int32_t> inc(3 * (int64_t)E);
+    {
+        std::vector<int32_t> fillp(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int32_t e = 0; e < E; ++e)
+            for (int c = 0; c < 3; ++c) inc[fillp[d->conn[3 * e + c]]++] = 3 * e + c;
+    }
+
+    // node order
+    T.node_perm.resize(N);
+    for (int i = 0; i < N; ++i) T.node_perm[i] = i;
+    const bool single = N <= tile_nodes && E <= tile_slots;
+    if (!single) {
+        float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+        for (int i = 0; i < N; ++i)
+            for (int k = 0; k < 2; ++k) {
+                float v = d->x0[2 * i + k];
+                if (std::isfinite(v)) {
+                    lo[k] = std::min(lo[k], v);
+                    hi[k] = std::max(hi[k], v);
+                }
+            }
+        std::vector<uint32_t> key(N);
+        for (int i = 0; i < N; ++i) {
+            uint32_t q[2];
+            for (int k = 0; k < 2; ++k) {
+                double span = (double)hi[k] - (double)lo[k];
+                double t = span > 0 ? ((double)d->x0[2 * i + k] - lo[k]) / span : 0.0;
+                if (!(t >= 0.0)) t = 0.0;
+                if (t > 1.0) t = 1.0;
+                q[k] = (uint32_t)(t * 65535.0);
+            }
+            key[i] = spread16(q[0]) | (spread16(q[1]) << 1);
+        }
+        std::stable_sort(T.node_perm.begin(), T.node_perm.end(),
+                         [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    }
+    T.node_inv.assign(N, 0);
+    for (int i = 0; i < N; ++i) T.node_inv[T.node_perm[i]] = i;
+
+    // greedy tiles over internal node order
+    std::vector<int32_t> stamp(E, -1), seen(E, -1);
+    std::vector<int32_t> tile_start;  // internal node index where each tile starts
+    {
+        int tid = 0, cur_nodes = 0, cur_slots = 0;
+        if (N > 0) tile_start.push_back(0);
+        for (int p = 0; p < N; ++p) {
+            const int n = T.node_perm[p];
+            int add = 0, all = 0;  // distinct incident elements: new to this tile / in total
+            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
+                const int e = inc[j] / 3;
+                if (seen[e] != p) {
+                    seen[e] = p;
+                    ++all;
+                    if (stamp[e] != tid) ++add;
+                }
+            }
+            if (cur_nodes > 0 && (cur_nodes + 1 > tile_nodes || cur_slots + add > tile_slots)) {
+                ++tid;
+                tile_start.push_back(p);
+                cur_nodes = 0;
+                cur_slots = 0;
+                add = all;
+            }
+            if (add > tile_slots)
+                return fail(FW_E_INVALID, "node " + std::to_string(n) + " touches " + std::to_string(add) +
+                                              " elements, more than the tile slot capacity");
+            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) stamp[inc[j] / 3] = tid;
+            cur_nodes += 1;
+            cur_slots += add;
+        }
+    }
+    const int ntiles = (int)tile_start.size();
+    std::vector<int32_t> node_tile(N);
+    for (int t = 0; t < ntiles; ++t) {
+        int p1 = t + 1 < ntiles ? tile_start[t + 1] : N;
+        for (int p = tile_start[t]; p < p1; ++p) node_tile[p] = t;
+    }
+
+    // element owner = tile of its lowest internal node; store grouped by owner
+    std::vector<int32_t> owner(E);
+    std::vector<int32_t> own_count(ntiles + 1, 0);
+    for (int e = 0; e < E; ++e) {
+        int m = std::min(T.node_inv[d->conn[3 * e]], std::min(T.node_inv[d->conn[3 * e + 1]], T.node_inv[d->conn[3 * e + 2]]));
+        owner[e] = node_tile[m];
+        own_count[owner[e] + 1]++;
+    }
+    for (int t = 0; t < ntiles; ++t) own_count[t + 1] += own_count[t];
+    T.elem_perm.resize(E);
+    {
+        std::vector<int32_t> fillp(own_count.begin(), own_count.end() - 1);
+        for (int e = 0; e < E; ++e) T.elem_perm[fillp[owner[e]]++] = e;  // ascending caller id inside a tile
+    }
+    T.elem_inv.assign(E, 0);
+    for (int i = 0; i < E; ++i) T.elem_inv[T.elem_perm[i]] = i;
+
+    // per tile: halo list and CSR
+    T.tiles.resize(ntiles);
+    std::fill(stamp.begin(), stamp.end(), -1);
+    std::vector<int32_t> slot_of(E, -1);
+    for (int t = 0; t < ntiles; ++t) {
+        TileDev& td = T.tiles[t];
+        std::memset(&td, 0, sizeof(td));
+        const int p0 = tile_start[t], p1 = t + 1 < ntiles ? tile_start[t + 1] : N;
+        td.node0 = p0;
+        td.n_nodes = p1 - p0;
+        td.elem0 = own_count[t];
+        td.n_own = own_count[t + 1] - own_count[t];
+        td.halo_off = (int32_t)T.halo.size();
+        td.multi = ntiles > 1;
+        for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) slot_of[T.elem_perm[i]] = i - td.elem0;
+        std::vector<int32_t> hl;
+        for (int p = p0; p < p1; ++p) {
+            const int n = T.node_perm[p];
+            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
+                int e = inc[j] / 3;
+                if (owner[e] != t && stamp[e] != t) {
+                    stamp[e] = t;
+                    hl.push_back(T.elem_inv[e]);
+                }
+            }
+        }
+        std::sort(hl.begin(), hl.end());
+        for (size_t h = 0; h < hl.size(); ++h) slot_of[T.elem_perm[hl[h]]] = td.n_own + (int)h;
+        td.n_halo = (int32_t)hl.size();
+        T.halo_slots += td.n_halo;
+        T.halo.insert(T.halo.end(), hl.begin(), hl.end());
+        if (td.n_own + td.n_halo > tile_slots) return fail(FW_E_INVALID, "internal tiling error: slot overflow");
+        td.ptr_off = (int32_t)T.csr_ptr.size();
+        td.ent_off = (int32_t)T.csr_ent.size();
+        int cnt = 0;
+        T.csr_ptr.push_back(0);
+        for (int p = p0; p < p1; ++p) {
+            const int n = T.node_perm[p];
+            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
+                int e = inc[j] / 3, c = inc[j] % 3;
+                T.csr_ent.push_back((uint16_t)(slot_of[e] * 3 + c));
+                ++cnt;
+            }
+            if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
+            T.csr_ptr.push_back((uint16_t)cnt);
+        }
+    }
+    return FW_OK;
+}
+
+// ------------------------------------------------------------------ scene layout
+
+struct Layout {
+    int64_t pos0, pos1, vel, mass, flag, bcv, disp, force, cflag;
+    int64_t c0, c1, c2, jinv, area, j2a, j2b, stress, strain, yflag;
+    int64_t halo, cptr, cent, tiles;
+    int64_t gdata[FW_MAX_GROUNDS];
+    int64_t total;
+};
+
+Layout make_layout(const fw_scene_desc* d, int64_t n_halo, int64_t n_ptr, int64_t n_ent, int64_t n_tiles) {
+    Layout L;
+    int64_t o = 0;
+    auto take = [&](int64_t bytes) {
+        int64_t r = o;
+        o += align_up(bytes);
+        return r;
+    };
+    const int64_t N = d->n_nodes, E = d->n_elems;
+    L.pos0 = take(8 * N);
+    L.pos1 = take(8 * N);
+    L.vel = take(8 * N);
+    L.mass = take(4 * N);
+    L.flag = take(4 * N);
+    L.bcv = take(8 * std::max<int64_t>(d->n_bc_values, 1));
+    L.disp = d->track_disp ? take(8 * N) : -1;
+    L.force = take(8 * N);
+    L.cflag = take(N);
+    L.c0 = take(4 * E);
+    L.c1 = take(4 * E);
+    L.c2 = take(4 * E);
+    L.jinv = take(16 * E);
+    L.area = take(4 * E);
+    if (d->mat_type == FW_MAT_MISES) {
+        L.j2a = take(16 * E);
+        L.j2b = take(16 * E);
+    } else {
+        L.j2a = L.j2b = -1;
+    }
+    L.stress = take(16 * E);
+    L.strain = take(16 * E);
+    L.yflag = take(E);
+    L.halo = take(4 * n_halo);
+    L.cptr = take(2 * n_ptr);
+    L.cent = take(2 * n_ent);
+    L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
+    for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
+        L.gdata[g] = -1;
+        if (g >= d->n_grounds) continue;
+        const fw_ground_desc& gd = d->grounds[g];
+        if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(4 * (int64_t)gd.hf_nx);
+        if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take((int64_t)gd.vox_nx * gd.vox_ny);
+    }
+    L.total = o;
+    return L;
+}
+
+int validate(const fw_scene_desc* d) {
+    if (!d) return fail(FW_E_INVALID, "null scene description");
+    if (d->n_nodes <= 0 || d->n_elems <= 0) return fail(FW_E_INVALID, "scene needs at least one node and one element");
+    if (!d->x0 || !d->v0 || !d->conn || !d->flags || !d->bc_values)
+        return fail(FW_E_INVALID, "scene arrays x0, v0, conn, flags and bc_values are required");
+    if (d->mat_type < FW_MAT_ELASTIC || d->mat_type > FW_MAT_MISES)
+        return fail(FW_E_INVALID, "unknown material type " + std::to_string(d->mat_type));
+    if (d->n_grounds < 0 || d->n_grounds > FW_MAX_GROUNDS)
+        return fail(FW_E_UNSUPPORTED, "a scene holds at most " + std::to_string(FW_MAX_GROUNDS) + " grounds");
+    if (d->n_grounds > 0 && !d->grounds) return fail(FW_E_INVALID, "grounds pointer is null");
+    if (d->n_bc_values <= 0 || d->n_bc_values > 65536) return fail(FW_E_INVALID, "n_bc_values must be in [1, 65536]");
+    for (int i = 0; i < d->n_nodes; ++i)
+        if ((int)(d->flags[i] >> FW_NODE_BCV_SHIFT) >= d->n_bc_values)
+            return fail(FW_E_INVALID, "node flag indexes a BC value beyond the table");
+    for (int g = 0; g < d->n_grounds; ++g) {
+        const fw_ground_desc& gd = d->grounds[g];
+        if (gd.kind == FW_GROUND_HEIGHTFIELD && (gd.hf_nx < 2 || !gd.heights))
+            return fail(FW_E_INVALID, "height field needs at least 2 samples");
+        if (gd.kind == FW_GROUND_VOXEL && (gd.vox_nx < 1 || gd.vox_ny < 1 || !gd.occ))
+            return fail(FW_E_INVALID, "voxel grid needs nx, ny >= 1 and an occupancy array");
+        if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_VOXEL)
+            return fail(FW_E_INVALID, "unknown ground kind " + std::to_string(gd.kind));
+    }
+    return FW_OK;
+}
+
+// ------------------------------------------------------------------ pinned staging
+
+struct Staging {
+    void* ptr = nullptr;
+    int64_t size = 0;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct SceneHost {
+    bool live = false;
+    int64_t block_off = 0, block_bytes = 0;
+    int32_t n_nodes = 0, n_elems = 0, mat_type = 0;
+    int32_t s0 = 0, horizon = INT32_MAX;
+    Tiling tiling;
+    std::vector<TileDev> tiles;  // scene-relative, scene id filled in
+    int tile_base = 0;           // position in the arena tile table
+    SceneDev dev;
+};
+
+struct fw_arena {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char* base = nullptr;
+    int64_t capacity = 0;
+    std::map<int64_t, int64_t> free_list;  // offset -> size
+    int64_t used = 0;
+    fw_tuning tune{};
+    int n_cu = 0;
+    // device tables
+    SceneDev* d_scenes = nullptr;
+    int max_scenes = 0;
+    TileDev* d_tiles = nullptr;
+    int tile_cap = 0;
+    ArenaCtl* d_ctl = nullptr;
+    // host state
+    std::vector<std::unique_ptr<SceneHost>> scenes;
+    std::vector<int> free_ids;
+    std::vector<TileDev> tile_table;
+    bool tiles_dirty = false;
+    int64_t clock = 0;  // host mirror of ctl->clock
+    std::vector<std::unique_ptr<Staging>> staging;
+    // graph cache
+    hipGraphExec_t graph = nullptr;
+    int graph_n = -1;
+    uint32_t graph_flags = 0;
+    int graph_grid = 0;
+};
+
+namespace {
+
+int64_t arena_alloc(fw_arena* a, int64_t bytes) {
+    for (auto it = a->free_list.begin(); it != a->free_list.end(); ++it) {
+        if (it->second >= bytes) {
+            int64_t off = it->first, sz = it->second;
+            a->free_list.erase(it);
+            if (sz > bytes) a->free_list[off + bytes] = sz - bytes;
+            a->used += bytes;
+            return off;
+        }
+    }
+    return -1;
+}
+
+void arena_free(fw_arena* a, int64_t off, int64_t bytes) {
+    a->used -= bytes;
+    auto it = a->free_list.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != a->free_list.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        a->free_list.erase(nx);
+    }
+    if (it != a->free_list.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            a->free_list.erase(it);
+        }
+    }
+}
+
+// pinned buffer whose previous copy has finished
+int get_staging(fw_arena* a, int64_t bytes, Staging** out) {
+    for (auto& s : a->staging) {
+        if (s->busy && hipEventQuery(s->ev) == hipSuccess) s->busy = false;
+    }
+    for (auto& s : a->staging) {
+        if (!s->busy && s->size >= bytes) {
+            *out = s.get();
+            return FW_OK;
+        }
+    }
+    auto s = std::make_unique<Staging>();
+    s->size = std::max<int64_t>(bytes, 4 << 20);
+    HIPCHK(hipHostMalloc(&s->ptr, s->size, hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming));
+    *out = s.get();
+    a->staging.push_back(std::move(s));
+    return FW_OK;
+}
+
+int upload(fw_arena* a, void* dst, const void* src, int64_t bytes) {
+    if (bytes <= 0) return FW_OK;
+    Staging* s = nullptr;
+    int rc = get_staging(a, bytes, &s);
+    if (rc) return rc;
+    std::memcpy(s->ptr, src, bytes);
+    HIPCHK(hipMemcpyAsync(dst, s->ptr, bytes, hipMemcpyHostToDevice, a->stream));
+    HIPCHK(hipEventRecord(s->ev, a->stream));
+    s->busy = true;
+    return FW_OK;
+}
+
+int ensure_tiles(fw_arena* a) {
+    if (!a->tiles_dirty) return FW_OK;
+    a->tile_table.clear();
+    for (auto& sp : a->scenes) {
+        if (!sp || !sp->live) continue;
+        sp->tile_base = (int)a->tile_table.size();
+        a->tile_table.insert(a->tile_table.end(), sp->tiles.begin(), sp->tiles.end());
+    }
+    const int nt = (int)a->tile_table.size();
+    if (nt > a->tile_cap) {
+        int cap = std::max(nt, std::max(1024, a->tile_cap * 2));
+        TileDev* nd = nullptr;
+        HIPCHK(hipStreamSynchronize(a->stream));
+        HIPCHK(hipMalloc(&nd, sizeof(TileDev) * (size_t)cap));
+        if (a->d_tiles) HIPCHK(hipFree(a->d_tiles));
+        a->d_tiles = nd;
+        a->tile_cap = cap;
+        if (a->graph) {  // graph holds the old table pointer
+            (void)hipGraphExecDestroy(a->graph);
+            a->graph = nullptr;
+        }
+    }
+    int rc = upload(a, a->d_tiles, a->tile_table.data(), (int64_t)sizeof(TileDev) * nt);
+    if (rc) return rc;
+    ArenaCtl ctl{(int32_t)a->clock, nt};
+    rc = upload(a, &a->d_ctl->n_tiles, &ctl.n_tiles, sizeof(int32_t));
+    if (rc) return rc;
+    a->tiles_dirty = false;
+    return FW_OK;
+}
+
+int launch_grid(fw_arena* a, int ntiles) {
+    int g = ntiles;
+    if (a->tune.grid_per_cu > 0) g = std::min(ntiles, a->tune.grid_per_cu * a->n_cu);
+    g = std::max(g, 1);
+    return (g + 7) / 8 * 8;
+}
+
+SceneHost* get_scene(fw_arena* a, int id) {
+    if (id < 0 || id >= (int)a->scenes.size() || !a->scenes[id] || !a->scenes[id]->live) return nullptr;
+    return a->scenes[id].get();
+}
+
+// substeps this scene has completed (warm-up excluded), and the buffer holding its state
+void scene_state(const fw_arena* a, const SceneHost* s, int64_t* done, int* buf, bool* warmed) {
+    int64_t ran = a->clock - ((int64_t)s->s0 - 1);  // launches that covered this scene incl. warm-up
+    if (ran <= 0) {
+        *done = 0;
+        *buf = 0;
+        *warmed = false;
+        return;
+    }
+    int64_t c = std::min<int64_t>(ran - 1, s->horizon);
+    *done = c;
+    *buf = (int)((c + 1) & 1);
+    *warmed = true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* fw_last_error(void) { return g_err.c_str(); }
+
+int32_t fw_abi_version(void) { return FW_ABI_VERSION; }
+
+int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning* tuning, fw_arena** out) {
+    if (!out) return fail(FW_E_INVALID, "out is null");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(FW_E_INVALID, "device " + std::to_string(device) + " not present");
+    HIPCHK(hipSetDevice(device));
+    auto a = std::make_unique<fw_arena>();
+    a->device = device;
+    a->tune.tile_nodes = 512;
+    a->tune.tile_slots = 1024;
+    a->tune.block = 256;
+    a->tune.grid_per_cu = 0;
+    if (tuning) {
+        if (tuning->tile_slots != 1024 && tuning->tile_slots != 2048)
+            return fail(FW_E_INVALID, "tile_slots must be 1024 or 2048");
+        if (tuning->block != 256 && tuning->block != 512) return fail(FW_E_INVALID, "block must be 256 or 512");
+        if (tuning->block == 512 && tuning->tile_slots != 2048)
+            return fail(FW_E_INVALID, "block 512 needs tile_slots 2048");
+        if (tuning->tile_nodes < 1 || tuning->tile_nodes > 8192) return fail(FW_E_INVALID, "tile_nodes out of range");
+        a->tune = *tuning;
+    }
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    a->n_cu = prop.multiProcessorCount;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    if (capacity_bytes <= 0) capacity_bytes = (int64_t)((double)fr * 0.8);
+    capacity_bytes = align_up(capacity_bytes);
+    HIPCHK(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&a->base, capacity_bytes));
+    a->capacity = capacity_bytes;
+    a->free_list[0] = capacity_bytes;
+    a->max_scenes = 16384;
+    HIPCHK(hipMalloc(&a->d_scenes, sizeof(SceneDev) * (size_t)a->max_scenes));
+    HIPCHK(hipMalloc(&a->d_ctl, sizeof(ArenaCtl)));
+    ArenaCtl c0{0, 0};
+    HIPCHK(hipMemcpy(a->d_ctl, &c0, sizeof(c0), hipMemcpyHostToDevice));
+    *out = a.release();
+    return FW_OK;
+}
+
+int32_t fw_arena_destroy(fw_arena* a) {
+    if (!a) return FW_OK;
+    (void)hipSetDevice(a->device);
+    (void)hipStreamSynchronize(a->stream);
+    if (a->graph) (void)hipGraphExecDestroy(a->graph);
+    for (auto& s : a->staging) {
+        (void)hipHostFree(s->ptr);
+        (void)hipEventDestroy(s->ev);
+    }
+    (void)hipFree(a->base);
+    (void)hipFree(a->d_scenes);
+    (void)hipFree(a->d_tiles);
+    (void)hipFree(a->d_ctl);
+    (void)hipStreamDestroy(a->stream);
+    delete a;
+    return FW_OK;
+}
+
+int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
+    if (!a || !out) return fail(FW_E_INVALID, "null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->capacity_bytes = a->capacity;
+    out->used_bytes = a->used;
+    out->global_step = a->clock;
+    double algo = 0;
+    for (auto& sp : a->scenes) {
+        if (!sp || !sp->live) continue;
+        out->n_scenes++;
+        out->n_tiles += (int32_t)sp->tiles.size();
+        out->n_nodes += sp->n_nodes;
+        out->n_elems += sp->n_elems;
+        if (sp->mat_type == FW_MAT_MISES) out->n_elems_j2 += sp->n_elems;
+        out->halo_slots += sp->tiling.halo_slots;
+        algo += (double)sp->n_elems * (sp->mat_type == FW_MAT_MISES ? 64.0 : 32.0) + (double)sp->n_nodes * 40.0;
+    }
+    out->algo_bytes_per_substep = algo;
+    return FW_OK;
+}
+
+int32_t fw_scene_footprint(const fw_arena* a, const fw_scene_desc* d, int64_t* bytes) {
+    if (!a || !bytes) return fail(FW_E_INVALID, "null argument");
+    int rc = validate(d);
+    if (rc) return rc;
+    // upper bound without tiling: halo <= E, csr ptr <= 2N, tiles <= N
+    const int64_t N = d->n_nodes, E = d->n_elems;
+    Layout L = make_layout(d, E, 2 * N + 1, 3 * E, N);
+    *bytes = L.total;
+    return FW_OK;
+}
+
+int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
+    if (!a || !scene_id) return fail(FW_E_INVALID, "null argument");
+    int rc = validate(d);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(a->device));
+    auto sh = std::make_unique<SceneHost>();
+    rc = build_tiling(d, a->tune.tile_nodes, a->tune.tile_slots, sh->tiling);
+    if (rc) return rc;
+    Tiling& T = sh->tiling;
+    const int N = d->n_nodes, E = d->n_elems;
+    const int ntiles = (int)T.tiles.size();
+    Layout L = make_layout(d, (int64_t)T.halo.size(), (int64_t)T.csr_ptr.size(), (int64_t)T.csr_ent.size(), ntiles);
+    int id;
+    if (!a->free_ids.empty()) {
+        id = a->free_ids.back();
+    } else {
+        id = (int)a->scenes.size();
+        if (id >= a->max_scenes)
+            return fail(FW_E_CAPACITY, "arena holds at most " + std::to_string(a->max_scenes) + " scenes");
+    }
+    int64_t off = arena_alloc(a, L.total);
+    if (off < 0)
+        return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
+                                       std::to_string(a->capacity - a->used) + " free (fragmented)");
+    if (!a->free_ids.empty()) a->free_ids.pop_back();
+
+    // host image of the block
+    Staging* st = nullptr;
+    rc = get_staging(a, L.total, &st);
+    if (rc) {
+        arena_free(a, off, L.total);
+        return rc;
+    }
+    char* img = (char*)st->ptr;
+    std::memset(img, 0, L.total);
+    float2* pos0 = (float2*)(img + L.pos0);
+    float2* vel = (float2*)(img + L.vel);
+    uint32_t* flag = (uint32_t*)(img + L.flag);
+    for (int i = 0; i < N; ++i) {
+        int o = T.node_perm[i];
+        pos0[i] = make_float2(d->x0[2 * o], d->x0[2 * o + 1]);
+        vel[i] = make_float2(d->v0[2 * o], d->v0[2 * o + 1]);
+        flag[i] = d->flags[o];
+    }
+    std::memcpy(img + L.pos1, img + L.pos0, 8 * (size_t)N);
+    std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
+    int32_t* c0 = (int32_t*)(img + L.c0);
+    int32_t* c1 = (int32_t*)(img + L.c1);
+    int32_t* c2 = (int32_t*)(img + L.c2);
+    for (int i = 0; i < E; ++i) {
+        int o = T.elem_perm[i];
+        c0[i] = T.node_inv[d->conn[3 * o]];
+        c1[i] = T.node_inv[d->conn[3 * o + 1]];
+        c2[i] = T.node_inv[d->conn[3 * o + 2]];
+    }
+    std::memcpy(img + L.halo, T.halo.data(), 4 * T.halo.size());
+    std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
+    std::memcpy(img + L.cent, T.csr_ent.data(), 2 * T.csr_ent.size());
+    for (auto& td : T.tiles) td.scene = id;
+    std::memcpy(img + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
+    char* blk = a->base + off;
+    SceneDev& D = sh->dev;
+    std::memset(&D, 0, sizeof(D));
+    for (int g = 0; g < d->n_grounds; ++g) {
+        const fw_ground_desc& gd = d->grounds[g];
+        GroundDev& G = D.ground[g];
+        G.kind = gd.kind;
+        G.penalty = gd.penalty;
+        for (int k = 0; k < 2; ++k) {
+            G.lo[k] = gd.aabb_lo[k];
+            G.hi[k] = gd.aabb_hi[k];
+            float b = 0.1f * (gd.aabb_hi[k] - gd.aabb_lo[k]);
+            G.plo[k] = gd.aabb_lo[k] - b;
+            G.phi[k] = gd.aabb_hi[k] + b;
+        }
+        if (gd.kind == FW_GROUND_PLANE) {
+            G.a[0] = gd.point[0];
+            G.a[1] = gd.point[1];
+            G.a[2] = gd.normal[0];
+            G.a[3] = gd.normal[1];
+        } else if (gd.kind == FW_GROUND_HEIGHTFIELD) {
+            G.a[0] = gd.hf_lb_x;
+            G.a[1] = gd.hf_ub_x;
+            G.nx = gd.hf_nx;
+            G.reverse = gd.reverse;
+            std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
+            G.data = blk + L.gdata[g];
+        } else {
+            G.a[0] = gd.vox_lb[0];
+            G.a[1] = gd.vox_lb[1];
+            G.a[2] = gd.vox_dx;
+            G.a[3] = gd.vox_dz;
+            G.nx = gd.vox_nx;
+            G.ny = gd.vox_ny;
+            int8_t* occ = (int8_t*)(img + L.gdata[g]);
+            for (int64_t c = 0; c < (int64_t)gd.vox_nx * gd.vox_ny; ++c) {
+                int32_t v = gd.occ[c];
+                occ[c] = v == 0 ? 0 : (v == 1 ? 1 : 2);
+            }
+            G.data = blk + L.gdata[g];
+        }
+    }
+    HIPCHK(hipMemcpyAsync(blk, img, L.total, hipMemcpyHostToDevice, a->stream));
+    HIPCHK(hipEventRecord(st->ev, a->stream));
+    st->busy = true;
+
+    D.pos[0] = (float2*)(blk + L.pos0);
+    D.pos[1] = (float2*)(blk + L.pos1);
+    D.vel = (float2*)(blk + L.vel);
+    D.mass = (float*)(blk + L.mass);
+    D.flag = (const uint32_t*)(blk + L.flag);
+    D.bcv = (const float2*)(blk + L.bcv);
+    D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
+    D.force = (float2*)(blk + L.force);
+    D.cflag = (uint8_t*)(blk + L.cflag);
+    D.c0 = (const int32_t*)(blk + L.c0);
+    D.c1 = (const int32_t*)(blk + L.c1);
+    D.c2 = (const int32_t*)(blk + L.c2);
+    D.jinv = (float4*)(blk + L.jinv);
+    D.area = (float*)(blk + L.area);
+    D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
+    D.j2[1] = L.j2b >= 0 ? (float4*)(blk + L.j2b) : nullptr;
+    D.stress = (float4*)(blk + L.stress);
+    D.strain = (float4*)(blk + L.strain);
+    D.yflag = (uint8_t*)(blk + L.yflag);
+    D.halo = (const int32_t*)(blk + L.halo);
+    D.csr_ptr = (const uint16_t*)(blk + L.cptr);
+    D.csr_ent = (const uint16_t*)(blk + L.cent);
+    D.n_nodes = N;
+    D.n_elems = E;
+    D.mat_type = d->mat_type;
+    D.n_grounds = d->n_grounds;
+    D.E = d->E;
+    D.nu = d->nu;
+    D.rho = d->rho;
+    D.sigma_y = d->sigma_y;
+    D.H = d->H;
+    D.dt = d->dt;
+    D.damping = d->damping;
+    // warm-up runs at the next substep launched (local step -1)
+    D.s0 = (int32_t)(a->clock + 1);
+    D.horizon = d->horizon > 0 ? d->horizon : INT32_MAX;
+    D.live = 1;
+    for (int b = 0; b < 3; ++b) {
+        D.aabb[b][0] = fw_f2o(1e30f);
+        D.aabb[b][1] = fw_f2o(1e30f);
+        D.aabb[b][2] = fw_f2o(-1e30f);
+        D.aabb[b][3] = fw_f2o(-1e30f);
+    }
+    rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
+    if (rc) return rc;
+    HIPCHK(fw_launch_prepare(a->d_scenes, (const TileDev*)(blk + L.tiles), 0, ntiles, a->tune.tile_slots, a->stream));
+
+    sh->live = true;
+    sh->block_off = off;
+    sh->block_bytes = L.total;
+    sh->n_nodes = N;
+    sh->n_elems = E;
+    sh->mat_type = d->mat_type;
+    sh->s0 = D.s0;
+    sh->horizon = D.horizon;
+    sh->tiles = T.tiles;
+    if (id == (int)a->scenes.size())
+        a->scenes.push_back(std::move(sh));
+    else
+        a->scenes[id] = std::move(sh);
+    a->tiles_dirty = true;
+    *scene_id = id;
+    return FW_OK;
+}
+
+int32_t fw_scene_remove(fw_arena* a, int32_t id) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    // stream order keeps in-flight launches valid; the block is reused only after they finish
+    HIPCHK(hipStreamSynchronize(a->stream));
+    SceneDev dead = s->dev;
+    dead.live = 0;
+    HIPCHK(hipMemcpy(a->d_scenes + id, &dead, sizeof(SceneDev), hipMemcpyHostToDevice));
+    arena_free(a, s->block_off, s->block_bytes);
+    a->scenes[id].reset();
+    a->free_ids.push_back(id);
+    a->tiles_dirty = true;
+    return FW_OK;
+}
+
+int32_t fw_scene_progress(fw_arena* a, int32_t id, int64_t* steps_done, int32_t* finished) {
+    SceneHost* s = a ? get_scene(a, id) : nullptr;
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    int64_t done;
+    int buf;
+    bool warmed;
+    scene_state(a, s, &done, &buf, &warmed);
+    if (steps_done) *steps_done = done;
+    if (finished) *finished = (s->horizon != INT32_MAX && done >= s->horizon) ? 1 : 0;
+    return FW_OK;
+}
+
+static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
+    int rc = ensure_tiles(a);
+    if (rc) return rc;
+    const int nt = (int)a->tile_table.size();
+    for (int k = 0; k < n; ++k) {
+        uint32_t f = flags & ~FW_STEP_PER_SCENE;
+        if (k != n - 1) f &= ~FW_STEP_DIAG;
+        if (nt > 0) {
+            if (flags & FW_STEP_PER_SCENE) {
+                for (auto& sp : a->scenes) {
+                    if (!sp || !sp->live) continue;
+                    const int cnt = (int)sp->tiles.size();
+                    HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, sp->tile_base, cnt, dt_override, damping_override, f,
+                                             launch_grid(a, cnt), a->tune.block, a->tune.tile_slots, a->stream));
+                }
+            } else {
+                HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
+                                         launch_grid(a, nt), a->tune.block, a->tune.tile_slots, a->stream));
+            }
+        }
+    }
+    HIPCHK(fw_launch_advance_clock(a->d_ctl, n, a->stream));
+    a->clock += n;
+    return FW_OK;
+}
+
+int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
+    if (!a || n < 0) return fail(FW_E_INVALID, "bad arguments");
+    if (n == 0) return FW_OK;
+    HIPCHK(hipSetDevice(a->device));
+    return step_impl(a, n, dt_override, damping_override, flags);
+}
+
+int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile_slots, int64_t* stats) {
+    // Host-only: build the tiling a scene would get and verify it, without touching the GPU.
+    //   every element owned by exactly one tile, owned ranges contiguous and disjoint;
+    //   slots (owned + halo) within capacity; every tile's node CSR lists exactly the
+    //   node's (element, corner) incidences in ascending caller element id.
+    int rc = validate(d);
+    if (rc) return rc;
+    Tiling T;
+    rc = build_tiling(d, tile_nodes, tile_slots, T);
+    if (rc) return rc;
+    const int N = d->n_nodes, E = d->n_elems;
+    std::vector<int> owned(E, 0);
+    int64_t max_slots = 0, entries = 0;
+    for (const TileDev& td : T.tiles) {
+        if (td.n_own + td.n_halo > tile_slots || td.n_nodes > tile_nodes) return fail(FW_E_INVALID, "tile over capacity");
+        max_slots = std::max<int64_t>(max_slots, td.n_own + td.n_halo);
+        for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) owned[T.elem_perm[i]]++;
+        for (int i = 0; i < td.n_nodes; ++i) {
+            const int p = td.node0 + i;
+            const int n = T.node_perm[p];
+            std::vector<int64_t> want, got;
+            for (int e = 0; e < E; ++e)  // brute force: fine for the small meshes this is used on
+                for (int c = 0; c < 3; ++c)
+                    if (d->conn[3 * e + c] == n) want.push_back(3 * (int64_t)e + c);
+            const uint16_t* cp = T.csr_ptr.data() + td.ptr_off;
+            for (int j = cp[i]; j < cp[i + 1]; ++j) {
+                const int ent = T.csr_ent[td.ent_off + j];
+                const int slot = ent / 3, c = ent % 3;
+                const int eint = slot < td.n_own ? td.elem0 + slot : T.halo[td.halo_off + slot - td.n_own];
+                got.push_back(3 * (int64_t)T.elem_perm[eint] + c);
+            }
+            entries += (int64_t)got.size();
+            if (want != got) return fail(FW_E_INVALID, "CSR of node " + std::to_string(n) + " is wrong");
+        }
+    }
+    for (int e = 0; e < E; ++e)
+        if (owned[e] != 1) return fail(FW_E_INVALID, "element " + std::to_string(e) + " owned " + std::to_string(owned[e]) + " times");
+    for (int i = 0; i < N; ++i)
+        if (T.node_perm[T.node_inv[i]] != i) return fail(FW_E_INVALID, "node permutation is not a bijection");
+    if (stats) {
+        stats[0] = (int64_t)T.tiles.size();
+        stats[1] = T.halo_slots;
+        stats[2] = max_slots;
+        stats[3] = entries;
+    }
+    return FW_OK;
+}
+
+int32_t fw_scene_reset_gate(fw_arena* a, int32_t id) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    HIPCHK(hipSetDevice(a->device));
+    // next launch runs local step clock - s0
+    s->dev.gate_base = (int32_t)(a->clock - s->s0);
+    return upload(a, &a->d_scenes[id].gate_base, &s->dev.gate_base, sizeof(int32_t));
+}
+
+int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
+    if (!a || n <= 0) return fail(FW_E_INVALID, "bad arguments");
+    if (flags & FW_STEP_PER_SCENE) return fail(FW_E_INVALID, "graph replay covers the batched launch only");
+    HIPCHK(hipSetDevice(a->device));
+    int rc = ensure_tiles(a);
+    if (rc) return rc;
+    const int nt = (int)a->tile_table.size();
+    const int grid = launch_grid(a, std::max(nt, 1));
+    if (!a->graph || a->graph_n != n || a->graph_flags != flags || grid > a->graph_grid) {
+        if (a->graph) {
+            (void)hipGraphExecDestroy(a->graph);
+            a->graph = nullptr;
+        }
+        const int cap_grid = std::max(grid, launch_grid(a, a->tile_cap));
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(a->stream, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < n; ++k) {
+            uint32_t f = flags;
+            if (k != n - 1) f &= ~FW_STEP_DIAG;
+            hipError_t e = fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, -1.0f, -1.0f, f, cap_grid,
+                                             a->tune.block, a->tune.tile_slots, a->stream);
+            if (e != hipSuccess) {
+                (void)hipStreamEndCapture(a->stream, &g);
+                return fail(FW_E_HIP, std::string("capture launch: ") + hipGetErrorString(e));
+            }
+        }
+        hipError_t e = fw_launch_advance_clock(a->d_ctl, n, a->stream);
+        hipError_t e2 = hipStreamEndCapture(a->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) return fail(FW_E_HIP, "graph capture failed");
+        HIPCHK(hipGraphInstantiate(&a->graph, g, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(g));
+        a->graph_n = n;
+        a->graph_flags = flags;
+        a->graph_grid = cap_grid;
+    }
+    HIPCHK(hipGraphLaunch(a->graph, a->stream));
+    a->clock += n;
+    return FW_OK;
+}
+
+int32_t fw_sync(fw_arena* a) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    HIPCHK(hipSetDevice(a->device));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    return FW_OK;
+}
+
+static int field_geometry(const SceneHost* s, int field, bool* per_node, int* width, int* elsize) {
+    *per_node = true;
+    *width = 2;
+    *elsize = 4;
+    switch (field) {
+        case FW_FIELD_X: case FW_FIELD_V: case FW_FIELD_U: case FW_FIELD_FORCE: break;
+        case FW_FIELD_MASS: *width = 1; break;
+        case FW_FIELD_CONTACT: *width = 1; *elsize = 1; break;
+        case FW_FIELD_STRESS: case FW_FIELD_STRAIN: case FW_FIELD_EPS_P: *per_node = false; *width = 3; break;
+        case FW_FIELD_EQPS: case FW_FIELD_AREA: *per_node = false; *width = 1; break;
+        case FW_FIELD_JINV: *per_node = false; *width = 4; break;
+        case FW_FIELD_YIELD: *per_node = false; *width = 1; *elsize = 1; break;
+        case FW_FIELD_AABB: *per_node = false; *width = 4; break;
+        default: return fail(FW_E_INVALID, "unknown field " + std::to_string(field));
+    }
+    (void)s;
+    return FW_OK;
+}
+
+int32_t fw_scene_read(fw_arena* a, int32_t id, int32_t field, void* out, int64_t out_bytes) {
+    if (!a || !out) return fail(FW_E_INVALID, "null argument");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    HIPCHK(hipSetDevice(a->device));
+    bool per_node;
+    int width, elsize;
+    int rc = field_geometry(s, field, &per_node, &width, &elsize);
+    if (rc) return rc;
+    const SceneDev& D = s->dev;
+    int64_t done;
+    int buf;
+    bool warmed;
+    scene_state(a, s, &done, &buf, &warmed);
+    HIPCHK(hipStreamSynchronize(a->stream));
+    if (field == FW_FIELD_AABB) {
+        if (out_bytes < 16) return fail(FW_E_INVALID, "output buffer too small");
+        // the AABB at the current x was written by the last completed local step
+        int32_t enc[3][4];
+        HIPCHK(hipMemcpy(enc, &a->d_scenes[id].aabb[0][0], sizeof(enc), hipMemcpyDeviceToHost));
+        int slot = warmed ? (int)(((done % 3) + 3) % 3) : 0;
+        float* o = (float*)out;
+        for (int k = 0; k < 4; ++k) o[k] = fw_o2f(enc[slot][k]);
+        return FW_OK;
+    }
+    const int64_t count = per_node ? s->n_nodes : s->n_elems;
+    const int64_t need = count * width * elsize;
+    if (out_bytes < need) return fail(FW_E_INVALID, "output buffer too small: need " + std::to_string(need));
+    const void* src = nullptr;
+    int dev_width = width;  // components stored per item on the device
+    switch (field) {
+        case FW_FIELD_X: src = D.pos[buf]; break;
+        case FW_FIELD_V: src = D.vel; break;
+        case FW_FIELD_U: src = D.disp; break;
+        case FW_FIELD_FORCE: src = D.force; break;
+        case FW_FIELD_MASS: src = D.mass; break;
+        case FW_FIELD_CONTACT: src = D.cflag; break;
+        case FW_FIELD_STRESS: src = D.stress; dev_width = 4; break;
+        case FW_FIELD_STRAIN: src = D.strain; dev_width = 4; break;
+        case FW_FIELD_EPS_P: src = D.j2[buf]; dev_width = 4; break;
+        case FW_FIELD_EQPS: src = D.j2[buf]; dev_width = 4; break;
+        case FW_FIELD_AREA: src = D.area; break;
+        case FW_FIELD_JINV: src = D.jinv; break;
+        case FW_FIELD_YIELD: src = D.yflag; break;
+    }
+    if (!src) {  // field not stored for this scene (no displacement tracking / not J2): zeros
+        std::memset(out, 0, need);
+        return FW_OK;
+    }
+    std::vector<char> tmp((size_t)(count * dev_width * elsize));
+    HIPCHK(hipMemcpy(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost));
+    const std::vector<int32_t>& perm = per_node ? s->tiling.node_perm : s->tiling.elem_perm;
+    char* o = (char*)out;
+    const int comp0 = field == FW_FIELD_EQPS ? 3 : 0;
+    for (int64_t i = 0; i < count; ++i) {
+        const char* srow = tmp.data() + i * dev_width * elsize + comp0 * elsize;
+        std::memcpy(o + (int64_t)perm[i] * width * elsize, srow, (size_t)width * elsize);
+    }
+    return FW_OK;
+}
+
+int32_t fw_scene_write(fw_arena* a, int32_t id, int32_t field, const void* in, int64_t in_bytes) {
+    if (!a || !in) return fail(FW_E_INVALID, "null argument");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    if (field != FW_FIELD_X && field != FW_FIELD_V)
+        return fail(FW_E_UNSUPPORTED, "only positions and velocities can be written");
+    HIPCHK(hipSetDevice(a->device));
+    const int64_t need = (int64_t)s->n_nodes * 8;
+    if (in_bytes < need) return fail(FW_E_INVALID, "input buffer too small");
+    int64_t done;
+    int buf;
+    bool warmed;
+    scene_state(a, s, &done, &buf, &warmed);
+    std::vector<float2> tmp(s->n_nodes);
+    const float* src = (const float*)in;
+    for (int i = 0; i < s->n_nodes; ++i) {
+        int o = s->tiling.node_perm[i];
+        tmp[i] = make_float2(src[2 * o], src[2 * o + 1]);
+    }
+    HIPCHK(hipStreamSynchronize(a->stream));
+    void* dst = field == FW_FIELD_X ? (void*)s->dev.pos[buf] : (void*)s->dev.vel;
+    HIPCHK(hipMemcpy(dst, tmp.data(), need, hipMemcpyHostToDevice));
+    return FW_OK;
+}
+
+int32_t fw_time_steps(fw_arena* a, int32_t n, int32_t mode, double* total_ms, double* kernel_ms) {
+    if (!a || n <= 0 || !total_ms || !kernel_ms) return fail(FW_E_INVALID, "bad arguments");
+    HIPCHK(hipSetDevice(a->device));
+    int rc = ensure_tiles(a);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(a->stream));
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    double ksum = 0.0;
+    if (mode == 0) {
+        // per-launch events bracket each substep kernel on the launch stream
+        std::vector<hipEvent_t> ev(2 * (size_t)n);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        const int nt = (int)a->tile_table.size();
+        HIPCHK(hipEventRecord(t0, a->stream));
+        for (int k = 0; k < n; ++k) {
+            HIPCHK(hipEventRecord(ev[2 * k], a->stream));
+            HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, -1.0f, -1.0f, 0, launch_grid(a, nt),
+                                     a->tune.block, a->tune.tile_slots, a->stream));
+            HIPCHK(hipEventRecord(ev[2 * k + 1], a->stream));
+        }
+        HIPCHK(fw_launch_advance_clock(a->d_ctl, n, a->stream));
+        a->clock += n;
+        HIPCHK(hipEventRecord(t1, a->stream));
+        HIPCHK(hipEventSynchronize(t1));
+        for (int k = 0; k < n; ++k) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+            ksum += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *kernel_ms = ksum / n;
+    } else {
+        HIPCHK(hipEventRecord(t0, a->stream));
+        if (mode == 1)
+            rc = fw_step_graph(a, n, 0);
+        else
+            rc = step_impl(a, n, -1.0f, -1.0f, FW_STEP_PER_SCENE);
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(t1, a->stream));
+        HIPCHK(hipEventSynchronize(t1));
+    }
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, t0, t1));
+    *total_ms = tot;
+    if (mode != 0) *kernel_ms = tot / n;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return FW_OK;
+}
+
+int32_t fw_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double* gbs, double* kernel_ms) {
+    if (bytes < 4096 || reps < 1 || !gbs || !kernel_ms) return fail(FW_E_INVALID, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    const int64_t n4 = bytes / 16;
+    void *src = nullptr, *dst = nullptr;
+    HIPCHK(hipMalloc(&src, n4 * 16));
+    HIPCHK(hipMalloc(&dst, n4 * 16));
+    HIPCHK(hipMemset(src, 0, n4 * 16));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    const int grid = prop.multiProcessorCount * 8;
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(fw_launch_copy_probe(src, dst, n4, grid, nullptr));  // first touch
+    HIPCHK(hipEventRecord(t0, nullptr));
+    for (int r = 0; r < reps; ++r) HIPCHK(fw_launch_copy_probe(src, dst, n4, grid, nullptr));
+    HIPCHK(hipEventRecord(t1, nullptr));
+    HIPCHK(hipEventSynchronize(t1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+    *kernel_ms = ms / reps;
+    *gbs = 2.0 * (double)(n4 * 16) / (*kernel_ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return FW_OK;
+}
+
+}  // extern "C"

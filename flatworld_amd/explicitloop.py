@@ -1,0 +1,161 @@
+"""ExplicitLoop drop-in for FEM bodies on static grounds (flatworld/explicitloop.py:71-589).
+
+Setup follows flatworld/explicitloop.py:72-398 for the FEM-vs-ground subset:
+
+* loop stable time = min(1.0, 1e-3 when a plane exists (RigidManager default,
+  flatworld/rigidmanager.py:1780), every FEM domain's stable time, every
+  contact pair's stable time);
+* contact pairs are formed over domains with ``considerContact`` in the
+  reference's O(D^2) order and split into the batched plane / height-field /
+  voxel work lists (flatworld/mixedcontact.py:1756-2269).  As in the reference,
+  a loop with more than one distinct height field (or voxel grid) leaves those
+  contacts unbatched, which means they never run; a warning says so.
+
+``advance`` is one fused substep launch: activation gate, ground forces,
+element forces, symplectic Euler and AABB, all on the device.
+``advanceWithTime`` computes the reference's float64 substep sequence on the
+host and enqueues it without any device synchronisation.
+"""
+
+import warnings
+
+import numpy as np
+
+from .definitions import ContactType, DomainType
+from .femcontact import contactHelper
+from .femspringmanager import FemSpringManager
+
+_GROUND_TYPES = (DomainType.ANALYTICAL, DomainType.HEIGHTFIELD, DomainType.VOXELMAP)
+
+
+class ExplicitLoop:
+    def __init__(
+        self,
+        dt,
+        domains,
+        joints=[],
+        ties=None,
+        damping=0.0,
+        useAdapativeDT=True,
+        skip_spatial_hash=False,
+        considerRigidRigidContact=True,
+        use_pd=0,
+        mass_scaling_dt=0.0,
+        max_contact_velocity=50.0,
+        device=0,
+        arena_kwargs=None,
+    ):
+        if len(joints) > 0 or ties:
+            raise NotImplementedError("joints and ties act on rigid bodies, which are not part of this package")
+        if mass_scaling_dt > 0.0:
+            raise NotImplementedError(
+                "selective mass scaling (flatworld/femspringmanager.py:141-201) is not part of this package"
+            )
+        for d in domains:
+            if d.type not in (DomainType.FEM,) + _GROUND_TYPES:
+                raise NotImplementedError(f"domain type {d.type:#b} (rigid / spring-mass) is not part of this package")
+        self.domains = domains
+        self.d = domains[0].d
+        self.numDomains = len(domains)
+        self._max_domains = self.numDomains + 10
+        self.damping = damping
+        self.useAdapative = useAdapativeDT
+        self._counter_py = 0
+
+        self.stableTime = 1.0
+        if any(d.type == DomainType.ANALYTICAL for d in domains):
+            self.stableTime = min(1.0 / 1000.0, self.stableTime)
+
+        # contact pairs in the reference's order (flatworld/explicitloop.py:237-279)
+        self.contacts = []
+        self.contactPairIds = []
+        contact_domains = [(i, d) for i, d in enumerate(domains) if d.considerContact]
+        for idx_a, (i, dom_i) in enumerate(contact_domains):
+            for idx_b in range(idx_a):
+                j, dom_j = contact_domains[idx_b]
+                ctype = dom_i.type | dom_j.type
+                if dom_i.category_bits & dom_j.collide_bits == 0:
+                    continue
+                if dom_i.collide_bits & dom_j.category_bits == 0:
+                    continue
+                if ctype in (ContactType.RIGIDRIGID, ContactType.RIGIDANLAYTICAL, ContactType.ANALYTICALANALYTICAL):
+                    continue
+                if hasattr(dom_i, "env_id") and hasattr(dom_j, "env_id") and dom_i.env_id != dom_j.env_id:
+                    continue
+                if dom_i.type != DomainType.FEM and dom_j.type != DomainType.FEM:
+                    continue  # ground-ground: no contact object in the reference either
+                c = contactHelper(dom_i, dom_j, ctype)
+                if c is not None:
+                    self.contacts.append(c)
+                    self.contactPairIds.append((i, j))
+
+        # batched work lists: planes, then one height field, then one voxel grid
+        grounds_of = {}
+        anal = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.ANALYTICAL]
+        hf = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.HEIGHTFIELD]
+        vox = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.VOXELMAP]
+        self._unbatched_contacts = []
+        for name, group in (("HeightField", hf), ("VoxelMap", vox)):
+            if len({id(c.domain2) for c, _ in group}) > 1:
+                warnings.warn(
+                    f"{name}: {len({id(c.domain2) for c, _ in group})} distinct grounds in one loop; as in "
+                    "flatworld/mixedcontact.py:2140-2261 these contacts stay unbatched and are never applied"
+                )
+                self._unbatched_contacts += [c for c, _ in group]
+                group.clear()
+        for c, _ in anal + hf + vox:
+            fem_idx = self.domains.index(c.domain1)
+            grounds_of.setdefault(fem_idx, []).append((c.domain2, c))
+
+        self.femSpringManager = FemSpringManager(self.d, domains, grounds_of=grounds_of, device=device,
+                                                 arena_kwargs=arena_kwargs)
+        self.stableTime = min(self.femSpringManager.stableTime, self.stableTime)
+        self.femSpringManager.setGlobalAABB(None)
+        self.numContacts = len(self.contacts)
+        for c in self.contacts:
+            self.stableTime = min(self.stableTime, c.stableTime)
+
+        if useAdapativeDT:
+            self.dt = self.stableTime
+        else:
+            self.dt = dt
+        self.default_dt = dt
+
+    # ------------------------------------------------------------ stepping
+    def advance(self):
+        """One substep: gate, ground forces, element forces, integrate (flatworld/explicitloop.py:473-555)."""
+        self.femSpringManager.advance_substeps(1, self.dt, self.damping, diag=True)
+        self._counter_py += 1
+
+    def substep_sequence(self, frame_dt):
+        """The float64 dt sequence advanceWithTime runs (flatworld/explicitloop.py:573-586)."""
+        seq = []
+        remaining = float(frame_dt)
+        eps = 1e-12
+        while remaining > eps:
+            current_stable = self.stableTime if self.useAdapative else self.default_dt
+            sub_dt = max(min(current_stable, remaining), 1e-8)
+            seq.append(sub_dt)
+            remaining -= sub_dt
+        return seq
+
+    def advanceWithTime(self, frame_dt=1.0 / 60.0, verbose=False):
+        """Advance by ``frame_dt`` of simulated time (flatworld/explicitloop.py:558-589)."""
+        seq = self.substep_sequence(frame_dt)
+        if not seq:
+            return
+        dts = np.asarray(seq, dtype=np.float64).astype(np.float32)
+        # runs of equal float32 dt become one multi-substep enqueue
+        starts = [0] + [k for k in range(1, len(dts)) if dts[k] != dts[k - 1]]
+        ends = starts[1:] + [len(dts)]
+        for r, (s, e) in enumerate(zip(starts, ends)):
+            self.dt = seq[e - 1]
+            self.femSpringManager.advance_substeps(e - s, float(dts[s]), self.damping, diag=(r == len(starts) - 1))
+        self._counter_py += len(seq)
+        if verbose:
+            print(f"Completed {len(seq)} substeps for {frame_dt} seconds of simulation time.")
+
+    def reset(self):
+        """Restart the step counter (FEM state is kept, as in flatworld/explicitloop.py:635-679)."""
+        self._counter_py = 0
+        self.femSpringManager.reset_gate()

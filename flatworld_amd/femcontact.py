@@ -1,0 +1,102 @@
+"""FEM-vs-ground contact pair records: penalty stiffness and contact stable time.
+
+Host scalars only, following flatworld/femcontact.py:94-197 and 283-303:
+
+* bulk modulus ``K = E / (3 (1 - 2 min(nu, 0.4999)))``;
+* 2D segment penalty ``k = 2 K`` (independent of mesh size);
+* contact stable time ``0.9 sqrt(m_node / k)`` with
+  ``m_node = rho * (L_min^2 / 2) / 3``.
+
+The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip).
+FEM-FEM, rigid and spring contacts are outside this package: constructing one
+raises ``NotImplementedError`` instead of silently dropping it.
+"""
+
+from .definitions import ContactType, DomainType
+
+
+class ContactBase:
+    def __init__(self, domain1, domain2, type, is_tied=False):
+        self.domain1 = domain1
+        self.domain2 = domain2
+        self.penalty = 1.0
+        self.type = type
+        self.tied = is_tied
+
+    @staticmethod
+    def _bulk_modulus(mat):
+        nu = min(mat.nu, 0.4999)
+        return mat.E / (3.0 * (1.0 - 2.0 * nu))
+
+    @staticmethod
+    def _segment_penalty(dom, slsfac=1.0):
+        mat = dom.prop.mat
+        K = ContactBase._bulk_modulus(mat)
+        L = dom.mesh.charAverageLength
+        if dom.d == 2:
+            return 2.0 * slsfac * K
+        return 1.5 * slsfac * K * L
+
+    def calStableTime(self, penalty, femdomain):
+        mat = femdomain.prop.mat
+        L = femdomain.mesh.charLength
+        if femdomain.d == 2:
+            m_node = mat.rho * (L * L * 0.5) / 3.0
+        else:
+            m_node = mat.rho * (L * L * L / 6.0) / 4.0
+        return 0.9 * (m_node / penalty) ** 0.5
+
+    def update(self, dt):
+        pass
+
+    def calculate(self, dt):
+        pass
+
+
+class _FlexGround(ContactBase):
+    """FEM body against a static ground; ``domain1`` is always the FEM body."""
+
+    ground_type = None
+
+    def __init__(self, domain1, domain2, type, is_tied=False):
+        super().__init__(domain1, domain2, type, is_tied)
+        if self.domain1.type == self.ground_type:
+            self.domain1, self.domain2 = self.domain2, self.domain1
+        self.penalty = self._segment_penalty(self.domain1) * self.penalty
+        self.stableTime = self.calStableTime(self.penalty, self.domain1)
+
+
+class ContactFlexAnalytical(_FlexGround):
+    """FEM vs plane (flatworld/femcontact.py:155-161)."""
+
+    ground_type = DomainType.ANALYTICAL
+
+
+class ContactFlexHeightField(_FlexGround):
+    """FEM vs height field (flatworld/femcontact.py:182-188)."""
+
+    ground_type = DomainType.HEIGHTFIELD
+
+
+class ContactFlexVoxelMap(_FlexGround):
+    """FEM vs voxel grid (flatworld/femcontact.py:191-197)."""
+
+    ground_type = DomainType.VOXELMAP
+
+
+def contactHelper(domain1, domain2, type):
+    """Pair factory (flatworld/femcontact.py:283-303) restricted to FEM-vs-ground."""
+    if type == ContactType.FLEXANLAYTICAL:
+        return ContactFlexAnalytical(domain1, domain2, type)
+    if type == ContactType.FEXHEIGHTFIELD:
+        return ContactFlexHeightField(domain1, domain2, type)
+    if type == ContactType.FLEXVOXELMAP:
+        return ContactFlexVoxelMap(domain1, domain2, type)
+    if type == ContactType.FLEXFLEX:
+        raise NotImplementedError(
+            "FEM-FEM penalty contact (flatworld/mixedcontact.py:687-1025) is not part of this "
+            "package; give the FEM domains considerContact=False or keep one FEM body per loop"
+        )
+    if type in (ContactType.RIGIDANLAYTICAL, ContactType.RIGIDRIGID, ContactType.ANALYTICALANALYTICAL):
+        return None
+    raise NotImplementedError(f"contact type {type:#b} is not part of the explicit-FEM path")

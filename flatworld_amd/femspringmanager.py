@@ -1,0 +1,185 @@
+"""FemSpringManager drop-in over the HIP arena (flatworld/femspringmanager.py:495-1033).
+
+The reference packs every FEM domain of a loop into global Warp arrays and
+launches force -> integrate -> AABB -> clear kernels per substep.  Here every
+FEM domain is one scene of a ``flatworld_amd._native.Arena`` and a substep is
+one fused kernel launch over all of them.  Readbacks keep the reference's
+attribute names and global layout: ``coords.numpy()`` returns the
+``MAX_NODES x 2`` array with each domain at its node offset, exactly as the
+Warp array would hold it.
+"""
+
+import numpy as np
+
+from . import _native
+from .definitions import DomainType
+from .scene import pack_fem_domain
+
+
+class DeviceField:
+    """Read-only view with the ``.numpy()`` accessor of a Warp array."""
+
+    def __init__(self, manager, field, capacity, width, dtype, per_node=True):
+        self._m = manager
+        self._field = field
+        self._capacity = capacity
+        self._width = width
+        self._dtype = dtype
+        self._per_node = per_node
+
+    @property
+    def shape(self):
+        return (self._capacity, self._width) if self._width > 1 else (self._capacity,)
+
+    def numpy(self):
+        return self._m._gather(self._field, self._capacity, self._width, self._dtype, self._per_node)
+
+
+class ScalarField:
+    """Length-1 counter array read as ``arr.numpy()[0]`` (flatworld/femspringmanager.py:558-563)."""
+
+    def __init__(self, value):
+        self._v = int(value)
+
+    def numpy(self):
+        return np.array([self._v], dtype=np.int32)
+
+
+class FemSpringManager:
+    """FEM domains of one loop, resident in one HIP arena.
+
+    ``grounds_of`` maps a FEM domain index (position in ``domains``) to its
+    ordered list of ``(ground_domain, contact)`` pairs; ExplicitLoop fills it.
+    """
+
+    def __init__(self, d, domains, spatial_hash=None, grounds_of=None, device=0, track_disp=True, arena_kwargs=None):
+        if d != 2:
+            raise ValueError(f"FemSpringManager supports d=2 only (got {d})")
+        self.d = d
+        self.aabb = None
+        fem = []
+        for idx, dom in enumerate(domains):
+            if dom.type == DomainType.FEM:
+                fem.append((idx, dom))
+            elif dom.type == DomainType.SPRINGMASS:
+                raise NotImplementedError(
+                    "spring-mass domains (flatworld/femspringmanager.py:283-312, springmass.py) are not part of this package"
+                )
+        count_nodes = sum(dom.nnodes for _, dom in fem)
+        count_elements = sum(dom.nelements for _, dom in fem)
+        self.MAX_NODES = max(count_nodes + 1024, 2048)
+        self.MAX_ELEMENTS = max(count_elements + 2048, 4096)
+        self.MAX_DOMAINS = max(len(domains), 128)
+        self.domains = []
+        self.domainMeshes = []
+        self.stableTime = 1.0
+        self._arena = _native.Arena(device=device, **(arena_kwargs or {}))
+        self._scene_ids = []
+        node_off, elem_off = 0, 0
+        self._node_offsets = []
+        self._elem_offsets = []
+        grounds_of = grounds_of or {}
+        for local_idx, (idx, dom) in enumerate(fem):
+            spec = pack_fem_domain(dom, grounds_of.get(idx, []))
+            sid = self._arena.add(spec, track_disp=track_disp)
+            self._scene_ids.append(sid)
+            self._node_offsets.append(node_off)
+            self._elem_offsets.append(elem_off)
+            node_off += dom.nnodes
+            elem_off += dom.nelements
+            self.domains.append(dom)
+            self.domainMeshes.append(dom.mesh)
+            dom.attach(self, local_idx)
+            self.stableTime = min(dom.stableTime, self.stableTime)
+        self._n_nodes = node_off
+        self._n_elems = elem_off
+        self.femDomainIds = ScalarField(0)
+        self.numFemDomains = ScalarField(len(fem))
+        self.numSpringDomains = ScalarField(0)
+        self.totalNodes = ScalarField(node_off)
+        self.totalElements = ScalarField(elem_off)
+        self.totalFEMElements = ScalarField(elem_off)
+        self.totalFEMNodes = ScalarField(node_off)
+        self.domainNodeOffset = _OffsetField(self._node_offsets, self.MAX_DOMAINS)
+        self.domainElemOffset = _OffsetField(self._elem_offsets, self.MAX_DOMAINS)
+        self._global_ids = [idx for idx, _ in fem]
+
+        N, E = self.MAX_NODES, self.MAX_ELEMENTS
+        self.coords = DeviceField(self, _native.FIELD_X, N, 2, np.float32)
+        self.V = DeviceField(self, _native.FIELD_V, N, 2, np.float32)
+        self.U = DeviceField(self, _native.FIELD_U, N, 2, np.float32)
+        self.Fext = DeviceField(self, _native.FIELD_FORCE, N, 2, np.float32)
+        self.mass = DeviceField(self, _native.FIELD_MASS, N, 1, np.float32)
+        self.elemStress = DeviceField(self, _native.FIELD_STRESS, E, 3, np.float32, per_node=False)
+        self.elemStrain = DeviceField(self, _native.FIELD_STRAIN, E, 3, np.float32, per_node=False)
+        self.plasticStrain = DeviceField(self, _native.FIELD_EPS_P, E, 3, np.float32, per_node=False)
+        self.eqPlasticStrain = DeviceField(self, _native.FIELD_EQPS, E, 1, np.float32, per_node=False)
+        self.elemVolume = DeviceField(self, _native.FIELD_AREA, E, 1, np.float32, per_node=False)
+        self.J_inv = DeviceField(self, _native.FIELD_JINV, E, 4, np.float32, per_node=False)
+        self.yieldFlag = DeviceField(self, _native.FIELD_YIELD, E, 1, np.uint8, per_node=False)
+        self.contactFlag = DeviceField(self, _native.FIELD_CONTACT, N, 1, np.uint8)
+        self.A = _ZeroField((N, 2))
+
+    # ------------------------------------------------------------ stepping
+    def setGlobalAABB(self, aabb=None):
+        """Attach the loop's AABB table and run the zero-dt warm-up femStep (flatworld/femspringmanager.py:872-879)."""
+        self.aabb = aabb
+        self._arena.step(1, 0.0, _native.FW_STEP_NO_CONTACT | _native.FW_STEP_DIAG, damping_override=0.0)
+
+    def substep(self, dt, damping):
+        """One femStep without ground contact (flatworld/femspringmanager.py:938-939)."""
+        self.femStep(float(dt), float(damping))
+
+    def femStep(self, dt, damping):
+        self._arena.step(1, float(np.float32(dt)), _native.FW_STEP_NO_CONTACT | _native.FW_STEP_DIAG,
+                         damping_override=float(np.float32(damping)))
+
+    def advance_substeps(self, n, dt, damping, diag=True):
+        """n full substeps (ground contact + femStep) with one fused launch each."""
+        flags = _native.FW_STEP_DIAG if diag else 0
+        self._arena.step(int(n), float(np.float32(dt)), flags, damping_override=float(np.float32(damping)))
+
+    def reset_gate(self):
+        for sid in self._scene_ids:
+            self._arena.reset_gate(sid)
+
+    def sync(self):
+        self._arena.sync()
+
+    # ------------------------------------------------------------ readback
+    def _gather(self, field, capacity, width, dtype, per_node):
+        out = np.zeros((capacity, width) if width > 1 else (capacity,), dtype=dtype)
+        offs = self._node_offsets if per_node else self._elem_offsets
+        for sid, off in zip(self._scene_ids, offs):
+            a = self._arena.read(sid, field)
+            out[off : off + a.shape[0]] = a
+        return out
+
+    def domain_coords(self, local_idx):
+        return self._arena.read(self._scene_ids[local_idx], _native.FIELD_X)
+
+    def domain_aabb(self, local_idx):
+        bb = self._arena.read(self._scene_ids[local_idx], _native.FIELD_AABB)
+        return bb[:2].copy(), bb[2:].copy()
+
+    def arena_info(self):
+        return self._arena.info()
+
+
+class _OffsetField:
+    def __init__(self, offsets, cap):
+        self._a = np.zeros(cap, dtype=np.int32)
+        self._a[: len(offsets)] = offsets
+
+    def numpy(self):
+        return self._a.copy()
+
+
+class _ZeroField:
+    """The reference's acceleration array A, which no kernel ever writes (always zero)."""
+
+    def __init__(self, shape):
+        self.shape = shape
+
+    def numpy(self):
+        return np.zeros(self.shape, dtype=np.float32)

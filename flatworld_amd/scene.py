@@ -1,0 +1,210 @@
+"""Packed scene description: the flat arrays handed across the C-ABI.
+
+A *scene* is one FEM body (one ``FemDomain``) with the ordered list of grounds it
+touches.  The packing mirrors what the reference manager and contact builder
+upload to the device:
+
+* node/element arrays and per-element material
+  (flatworld/femspringmanager.py:706-813),
+* BC bits ORed per node with the last-written value vector, then initial
+  velocities (flatworld/femspringmanager.py:660-704),
+* one ground record per (FEM, ground) contact pair with the float32 penalty,
+  plane point/normal, height samples or occupancy grid, and the ground AABB the
+  activation gate compares against (flatworld/mixedcontact.py:1794-1856,
+  2131-2261; flatworld/explicitloop.py:198-216).
+
+Every float that reaches the device is rounded to float32 here exactly once, in
+the same place the reference rounds it.
+"""
+
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .definitions import (
+    MAX_BC_VALUES,
+    NODE_BC_MASK,
+    NODE_BCV_SHIFT,
+    NODE_BOUNDARY_BIT,
+    DomainType,
+    MaterialType,
+)
+
+GROUND_PLANE = 1
+GROUND_HEIGHTFIELD = 2
+GROUND_VOXEL = 3
+
+
+@dataclass
+class GroundSpec:
+    """One (FEM body, ground) contact pair."""
+
+    kind: int
+    penalty: np.float32
+    aabb_lo: np.ndarray  # float32[2], the ground AABB as stored for the gate
+    aabb_hi: np.ndarray
+    point: Optional[np.ndarray] = None  # plane
+    normal: Optional[np.ndarray] = None
+    heights: Optional[np.ndarray] = None  # height field
+    hf_lb_x: np.float32 = np.float32(0.0)
+    hf_ub_x: np.float32 = np.float32(0.0)
+    reverse: int = 0
+    occ: Optional[np.ndarray] = None  # voxel grid int32[nx, ny]
+    vox_lb: Optional[np.ndarray] = None
+    vox_dx: np.float32 = np.float32(0.0)
+    vox_dz: np.float32 = np.float32(0.0)
+
+
+@dataclass
+class SceneSpec:
+    """Flat arrays of one FEM body and its ground contacts."""
+
+    x0: np.ndarray  # float32[N, 2]
+    v0: np.ndarray  # float32[N, 2]
+    conn: np.ndarray  # int32[E, 3]
+    bc: np.ndarray  # int32[N] BC bits
+    bcv: np.ndarray  # float32[N, 2] last-written BC value
+    bnodes: np.ndarray  # int32[Nb] sorted boundary nodes
+    mat_type: int
+    E: np.float32
+    nu: np.float32
+    rho: np.float32
+    sigma_y: np.float32
+    H: np.float32
+    grounds: List[GroundSpec] = field(default_factory=list)
+    dt: float = 0.0  # host float64; rounded to float32 on upload
+    damping: float = 0.0
+    horizon: int = 0  # substeps to run when served; 0 = unbounded
+
+    @property
+    def n_nodes(self):
+        return int(self.x0.shape[0])
+
+    @property
+    def n_elems(self):
+        return int(self.conn.shape[0])
+
+    def node_flags(self):
+        """Device flag word per node and the BC value table it indexes.
+
+        bits 0..8: BC bits; bit 15: boundary node; bits 16..31: row of the
+        returned float32[K, 2] value table holding that node's BC vector.
+        """
+        table, inv = np.unique(self.bcv.astype(np.float32), axis=0, return_inverse=True)
+        inv = np.asarray(inv).reshape(-1)
+        if table.shape[0] > MAX_BC_VALUES:
+            raise ValueError(f"scene has {table.shape[0]} distinct BC vectors; at most {MAX_BC_VALUES} are supported")
+        flags = (self.bc.astype(np.int64) & NODE_BC_MASK) | (inv.astype(np.int64) << NODE_BCV_SHIFT)
+        flags[self.bnodes] |= NODE_BOUNDARY_BIT
+        return flags.astype(np.uint32), np.ascontiguousarray(table, dtype=np.float32)
+
+
+def pack_bcs(domain):
+    """Per-node BC bits, values and initial velocity (flatworld/femspringmanager.py:660-704)."""
+    n = domain.nnodes
+    bc = np.zeros(n, dtype=np.int32)
+    bcv = np.zeros((n, 2), dtype=np.float32)
+    v0 = np.zeros((n, 2), dtype=np.float32)
+    for b in domain.bcs:
+        type_, nodes, values = b.processData()
+        if values is None:
+            values = [0.0] * domain.d
+        idx = np.arange(n) if nodes is None else np.asarray(nodes, dtype=np.int64).reshape(-1)
+        bc[idx] |= int(type_)
+        bcv[idx, 0] = np.float32(float(values[0]))
+        bcv[idx, 1] = np.float32(float(values[1]))
+    for ic in domain.initials:
+        vx, vy = float(ic.vel[0]), float(ic.vel[1])
+        if getattr(ic, "is_all_nodes", False):
+            v0[:, 0] = vx
+            v0[:, 1] = vy
+        else:
+            nds = np.asarray(ic.nds, dtype=np.int64).reshape(-1)
+            v0[nds, 0] = vx
+            v0[nds, 1] = vy
+    return bc, bcv, v0
+
+
+def ground_spec(ground, contact):
+    """Ground record for one FEM-vs-ground pair (flatworld/mixedcontact.py:1794-1856, 2131-2261)."""
+    lo, hi = ground.getBBox()
+    lo = np.asarray(lo, dtype=np.float32).reshape(2).copy()
+    hi = np.asarray(hi, dtype=np.float32).reshape(2).copy()
+    pen = np.float32(float(contact.penalty))
+    if ground.type == DomainType.ANALYTICAL:
+        return GroundSpec(
+            GROUND_PLANE,
+            pen,
+            lo,
+            hi,
+            point=np.asarray(ground.point, dtype=np.float32).reshape(2).copy(),
+            normal=np.asarray(ground.normal, dtype=np.float32).reshape(2).copy(),
+        )
+    if ground.type == DomainType.HEIGHTFIELD:
+        return GroundSpec(
+            GROUND_HEIGHTFIELD,
+            pen,
+            lo,
+            hi,
+            heights=np.ascontiguousarray(ground.height, dtype=np.float32),
+            hf_lb_x=np.float32(float(ground.lb[0])),
+            hf_ub_x=np.float32(float(ground.ub[0])),
+            reverse=1 if ground.reverse else 0,
+        )
+    if ground.type == DomainType.VOXELMAP:
+        return GroundSpec(
+            GROUND_VOXEL,
+            pen,
+            lo,
+            hi,
+            occ=np.ascontiguousarray(ground.occ, dtype=np.int32),
+            vox_lb=np.asarray(ground.lb, dtype=np.float32).reshape(2).copy(),
+            vox_dx=np.float32(float(ground.dx)),
+            vox_dz=np.float32(float(ground.dz)),
+        )
+    raise ValueError(f"not a ground domain: type {ground.type}")
+
+
+def pack_fem_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
+    """SceneSpec for one FemDomain and its (ground, contact) pairs in work-list order."""
+    mesh = domain.mesh
+    if mesh.d != 2:
+        raise ValueError(f"only d=2 meshes are supported (got {mesh.d})")
+    mat = domain.prop.getMaterial()
+    x0 = np.ascontiguousarray(np.asarray(mesh.coords, dtype=np.float32)[:, :2])
+    conn = np.ascontiguousarray(np.asarray(mesh.connectivity, dtype=np.int32)[:, :3])
+    bc, bcv, v0 = pack_bcs(domain)
+    bn = np.asarray(mesh.boundaryNodes).astype(np.int32) if mesh.numBoundNodes else np.zeros(0, np.int32)
+    return SceneSpec(
+        x0=x0,
+        v0=v0,
+        conn=conn,
+        bc=bc,
+        bcv=bcv,
+        bnodes=np.ascontiguousarray(bn),
+        mat_type=int(mat.type),
+        E=np.float32(float(mat.E)),
+        nu=np.float32(float(mat.nu)),
+        rho=np.float32(float(mat.rho)),
+        sigma_y=np.float32(float(getattr(mat, "sigma_y", 0.0))),
+        H=np.float32(float(getattr(mat, "H", 0.0))),
+        grounds=[ground_spec(g, c) for g, c in grounds],
+        dt=float(dt),
+        damping=float(damping),
+        horizon=int(horizon),
+    )
+
+
+def scene_stable_time(domain, contacts, has_plane):
+    """Loop stable time for one FEM body (flatworld/explicitloop.py:145-170, 312-318)."""
+    st = 1.0
+    if has_plane:
+        st = min(1.0 / 1000.0, st)
+    st = min(st, min(domain.stableTime, 1.0))
+    for c in contacts:
+        st = min(st, c.stableTime)
+    return st
+
+
+MATERIAL_NAMES = {MaterialType.ELASTIC: "elastic", MaterialType.NEOHOOKEAN: "neohookean", MaterialType.MISES: "j2"}

@@ -1,0 +1,156 @@
+"""Synthetic scenes for the benchmark configurations and the parity tests.
+
+Every generator is seeded through ``numpy.random.Generator(PCG64(seed))`` and
+builds its scenes through the public API (Mesh -> FemDomain -> contact pair ->
+SceneSpec), so the packed arrays are exactly what ``ExplicitLoop`` would
+upload for a one-body loop with the same domains.
+
+Configurations (sizes as in the benchmark definition):
+  small     20x20 grid on [0.3,0.7]^2, Elastic(2e4, .2, 40), g=(0,-1), plane y=0, dt=1e-4
+  large     1000x1000 jittered grid, Neo-Hookean(1e5, .3, 1000), g=(0,-9.8), plane y=0, CFL dt
+  j2voxel   512x512 block, J2(1e5, .3, 1000, sy=500, H=1e3), v0=(0,-2), 256x256 value-noise voxels
+  batched   4096 scenes, side log-uniform in [8,128], Elastic/NH/J2 mix, one height field each
+  serving   Poisson stream, side in [16,256], plane / height field / voxel grounds, horizon 60..600
+"""
+
+import math
+
+import numpy as np
+
+from .bcs import Gravity, InitialVel
+from .definitions import MaterialType
+from .domains import FemDomain, GroundDomain, HeightFieldDomain, VoxelGridDomain
+from .femcontact import contactHelper
+from .femesher import structured_grid
+from .materials import Elastic, HyperElastic, MisesPlastic, SolidProp
+from .mesh import Mesh
+from .scene import pack_fem_domain, scene_stable_time
+
+
+def rng_for(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def one_body_spec(domain, ground=None, dt=None, damping=0.0, horizon=0):
+    """SceneSpec of a one-body loop [ground, domain] with the loop's stable dt."""
+    pairs = []
+    if ground is not None:
+        c = contactHelper(ground, domain, ground.type | domain.type)
+        pairs.append((ground, c))
+    if dt is None:
+        dt = scene_stable_time(domain, [c for _, c in pairs], has_plane=isinstance(ground, GroundDomain))
+    return pack_fem_domain(domain, pairs, dt=dt, damping=damping, horizon=horizon)
+
+
+def make_material(kind, E, nu, rho, sigma_y=None, H=None):
+    if kind == MaterialType.ELASTIC:
+        return Elastic(E, nu, rho)
+    if kind == MaterialType.NEOHOOKEAN:
+        return HyperElastic(E, nu, rho)
+    return MisesPlastic(E, nu, rho, sigma_y, H)
+
+
+def sinusoid_heightfield(rng, n=1024, lo=-0.5, hi=1.5, amplitude=0.05):
+    """Sum of 3 sinusoids, total amplitude bound ``amplitude``, wavelengths U[0.1, 1]."""
+    x = np.linspace(lo, hi, n)
+    lam = rng.uniform(0.1, 1.0, size=3)
+    phi = rng.uniform(0.0, 2.0 * math.pi, size=3)
+    h = sum((amplitude / 3.0) * np.sin(2.0 * math.pi * x / lam[i] + phi[i]) for i in range(3))
+    return HeightFieldDomain(2, h.astype(np.float32), lb=[lo, 0.0], ub=[hi, 1.0])
+
+
+def value_noise_voxels(rng, nx, ny, lb, ub, spacing=8, threshold=0.5, floor_rows=0):
+    """Random lattice every ``spacing`` cells, bilinear interpolation, thresholded."""
+    gx = nx // spacing + 2
+    gy = ny // spacing + 2
+    lat = rng.uniform(0.0, 1.0, size=(gx, gy))
+    ii = np.arange(nx) / spacing
+    jj = np.arange(ny) / spacing
+    i0 = np.floor(ii).astype(int)
+    j0 = np.floor(jj).astype(int)
+    ti = (ii - i0)[:, None]
+    tj = (jj - j0)[None, :]
+    v = (
+        lat[i0][:, j0] * (1 - ti) * (1 - tj)
+        + lat[i0 + 1][:, j0] * ti * (1 - tj)
+        + lat[i0][:, j0 + 1] * (1 - ti) * tj
+        + lat[i0 + 1][:, j0 + 1] * ti * tj
+    )
+    occ = (v > threshold).astype(np.int32)
+    if floor_rows > 0:
+        occ[:, :floor_rows] = 1
+    return VoxelGridDomain(2, nx, ny, lb, ub, occupancy_np=occ)
+
+
+def grid_domain(side, lower, extent, material, bcs, initials=(), jitter=0.0, seed=0):
+    conn, coords = structured_grid(side, lower=lower, extent=extent, jitter=jitter, seed=seed)
+    return FemDomain(Mesh(2, conn, coords), SolidProp(material), list(bcs), initials=list(initials))
+
+
+# ------------------------------------------------------------------ configurations
+
+
+def small_config():
+    """20x20 Elastic block over a plane; fixed dt 1e-4, run with advance() 600 times."""
+    dom = grid_domain(20, (0.3, 0.3), 0.4, Elastic(2e4, 0.2, 40.0), [Gravity([0.0, -1.0])])
+    ground = GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    return one_body_spec(dom, ground, dt=1e-4)
+
+
+def large_config(side=1000, seed=0):
+    """Jittered 1000x1000 Neo-Hookean sheet resting on a plane at y=0 (bottom row penetrates)."""
+    dom = grid_domain(side, (0.0, 0.0), 1.0, HyperElastic(1e5, 0.3, 1000.0), [Gravity([0.0, -9.8])],
+                      jitter=0.1, seed=seed)
+    ground = GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    return one_body_spec(dom, ground)
+
+
+def j2_voxel_config(side=512, vox=256, seed=0):
+    """J2 block falling at 2 m/s onto a value-noise voxel ground."""
+    rng = rng_for(seed)
+    mat = MisesPlastic(1e5, 0.3, 1000.0, 500.0, 1e3)
+    dom = grid_domain(side, (0.25, 0.55), 0.5, mat, [Gravity([0.0, -9.8])], initials=[InitialVel("ALL", [0.0, -2.0])])
+    ground = value_noise_voxels(rng, vox, vox, [0.0, 0.0], [1.0, 0.55])
+    return one_body_spec(dom, ground)
+
+
+def random_material(rng):
+    kind = [MaterialType.ELASTIC, MaterialType.NEOHOOKEAN, MaterialType.MISES][int(rng.integers(0, 3))]
+    E = float(rng.uniform(1e4, 1e5))
+    nu = float(rng.uniform(0.2, 0.4))
+    return make_material(kind, E, nu, 1000.0, 5e-3 * E, 1e-2 * E)
+
+
+def batched_config(n_scenes=4096, seed=0, side_lo=8, side_hi=128):
+    """Uneven scenes: side log-uniform in [side_lo, side_hi], one height field each."""
+    rng = rng_for(seed)
+    specs = []
+    for _ in range(n_scenes):
+        s = int(round(side_lo * (side_hi / side_lo) ** rng.uniform()))
+        mat = random_material(rng)
+        dom = grid_domain(s, (0.3, 0.3), 0.4, mat, [Gravity([0.0, -9.8])])
+        hf = sinusoid_heightfield(rng)
+        specs.append(one_body_spec(dom, hf))
+    return specs
+
+
+def serving_job(rng, side_lo=16, side_hi=256, horizon_lo=60, horizon_hi=600):
+    """One request: random size, material, ground kind and horizon."""
+    s = int(rng.integers(side_lo, side_hi + 1))
+    mat = random_material(rng)
+    dom = grid_domain(s, (0.3, 0.3), 0.4, mat, [Gravity([0.0, -9.8])])
+    kind = int(rng.integers(0, 3))
+    if kind == 0:
+        ground = GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    elif kind == 1:
+        ground = sinusoid_heightfield(rng)
+    else:
+        ground = value_noise_voxels(rng, 128, 128, [-0.5, 0.0], [1.5, 0.3], floor_rows=4)
+    horizon = int(rng.integers(horizon_lo, horizon_hi + 1))
+    return one_body_spec(dom, ground, horizon=horizon)
+
+
+def algorithmic_bytes(spec):
+    """E*B_e + N*B_n with B_e = 32 (64 for J2) and B_n = 40 bytes."""
+    be = 64 if spec.mat_type == MaterialType.MISES else 32
+    return spec.n_elems * be + spec.n_nodes * 40

@@ -1,0 +1,206 @@
+/*
+ * flatworld_fem.h -- C ABI of the MI355X explicit-FEM substep.
+ *
+ * One library, libflatworld_fem.so, built from flatworld_amd/csrc/ for gfx950.
+ * Plain C types only: pointers, sizes, int status codes.  Every function
+ * returns FW_OK (0) or a negative FW_E* code; fw_last_error() gives the text
+ * of the most recent failure on the calling thread.  No C++ exception crosses
+ * this boundary.  Host pointers passed in are borrowed for the duration of the
+ * call only; all device memory belongs to the arena.
+ *
+ * Model: an *arena* is a packed region of HBM on one GPU holding any number of
+ * *scenes*.  A scene is one FEM body (linear triangles, one material) with up
+ * to FW_MAX_GROUNDS ground contacts.  fw_step advances every resident scene by
+ * n substeps, one fused kernel launch per substep; each scene carries its own
+ * dt, damping, step counter and horizon.
+ *
+ * Reference interfaces this replaces (FlatWorld, paths relative to its root):
+ *   fw_scene_add      FemSpringManager.__init__/addFemDomain/processConditions/
+ *                     prepareFEMData_ + MixedContact._build_batched_contacts
+ *                     (flatworld/femspringmanager.py:509-638, 660-813, 881-897;
+ *                      flatworld/mixedcontact.py:1756-2269)
+ *   fw_step           ExplicitLoop.advance: BVH gate, run_batched_contacts,
+ *                     FemSpringManager.substep -> femStep
+ *                     (flatworld/explicitloop.py:473-555;
+ *                      flatworld/femspringmanager.py:938-1033)
+ *   fw_scene_read     the .numpy() reads of coords/V/U/Fext/elemStress/...
+ *                     (flatworld/femspringmanager.py:566-614; femdomain.py:47-50)
+ *   fw_scene_remove   no counterpart: the reference never removes a domain.
+ */
+#ifndef FLATWORLD_FEM_H
+#define FLATWORLD_FEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FW_ABI_VERSION 1
+#define FW_MAX_GROUNDS 4
+
+/* status codes */
+#define FW_OK 0
+#define FW_E_INVALID (-1)   /* bad argument (maps to ValueError) */
+#define FW_E_CAPACITY (-2)  /* arena full / too many scenes (maps to RuntimeError) */
+#define FW_E_HIP (-3)       /* HIP runtime failure */
+#define FW_E_NOT_FOUND (-4) /* unknown scene id */
+#define FW_E_UNSUPPORTED (-5)
+
+/* material codes (flatworld/definitions.py:80-83) */
+#define FW_MAT_ELASTIC 1
+#define FW_MAT_NEOHOOKEAN 2
+#define FW_MAT_MISES 3
+
+/* ground kinds */
+#define FW_GROUND_PLANE 1
+#define FW_GROUND_HEIGHTFIELD 2
+#define FW_GROUND_VOXEL 3
+
+/* node flag word: bits 0..8 BC bits (flatworld/definitions.py:95-104),
+ * bit 15 boundary node, bits 16..31 row of the scene's BC value table */
+#define FW_NODE_BC_MASK 0x1FFu
+#define FW_NODE_BOUNDARY 0x8000u
+#define FW_NODE_BCV_SHIFT 16
+
+/* fw_step flags */
+#define FW_STEP_DIAG 1u      /* last substep writes element stress/strain, nodal force, flags */
+#define FW_STEP_NO_CONTACT 2u /* skip ground contact (femStep alone) */
+#define FW_STEP_PER_SCENE 4u  /* one launch per scene per substep (the unbatched launch pattern) */
+
+/* fields for fw_scene_read / fw_scene_write (node fields in the caller's node order) */
+#define FW_FIELD_X 1        /* float32[N][2] positions */
+#define FW_FIELD_V 2        /* float32[N][2] velocities */
+#define FW_FIELD_U 3        /* float32[N][2] accumulated displacement */
+#define FW_FIELD_FORCE 4    /* float32[N][2] nodal force of the last DIAG substep (after BCs) */
+#define FW_FIELD_MASS 5     /* float32[N] lumped mass */
+#define FW_FIELD_STRESS 6   /* float32[E][3] Cauchy stress (xx, yy, xy), last DIAG substep */
+#define FW_FIELD_STRAIN 7   /* float32[E][3] Green strain (xx, yy, xy), last DIAG substep */
+#define FW_FIELD_EPS_P 8    /* float32[E][3] plastic strain */
+#define FW_FIELD_EQPS 9     /* float32[E] equivalent plastic strain */
+#define FW_FIELD_AREA 10    /* float32[E] rest area */
+#define FW_FIELD_JINV 11    /* float32[E][4] inverse rest Jacobian, row-major */
+#define FW_FIELD_YIELD 12   /* uint8[E] f_trial > 0 in the last DIAG substep */
+#define FW_FIELD_CONTACT 13 /* uint8[N] a ground pushed the node in the last DIAG substep */
+#define FW_FIELD_AABB 14    /* float32[4] boundary-node AABB lb.x lb.y ub.x ub.y at the current x */
+
+/* One FEM-vs-ground contact pair (one work-list entry group of
+ * flatworld/mixedcontact.py:1794-1856, 2131-2261). */
+typedef struct {
+    int32_t kind;       /* FW_GROUND_* */
+    float penalty;      /* float32 penalty k = 2K (flatworld/femcontact.py:100-119) */
+    float aabb_lo[2];   /* ground AABB used by the activation gate */
+    float aabb_hi[2];
+    float point[2];     /* plane */
+    float normal[2];
+    const float* heights; /* height field samples [hf_nx] */
+    int32_t hf_nx;
+    float hf_lb_x;
+    float hf_ub_x;
+    int32_t reverse;
+    const int32_t* occ; /* voxel occupancy occ[i * vox_ny + j], i along x */
+    int32_t vox_nx;
+    int32_t vox_ny;
+    float vox_lb[2];
+    float vox_dx;
+    float vox_dz;
+} fw_ground_desc;
+
+/* One FEM body; arrays in the caller's node/element order. */
+typedef struct {
+    int32_t n_nodes;
+    int32_t n_elems;
+    int32_t n_grounds;
+    int32_t n_bc_values;
+    const float* x0;        /* [2N] rest = initial positions */
+    const float* v0;        /* [2N] initial velocities */
+    const int32_t* conn;    /* [3E] triangle node ids */
+    const uint32_t* flags;  /* [N] node flag words (FW_NODE_*) */
+    const float* bc_values; /* [2 * n_bc_values] BC value table */
+    int32_t mat_type;       /* FW_MAT_* */
+    float E, nu, rho, sigma_y, H;
+    float dt;               /* substep dt (float32), used when fw_step gets dt_override < 0 */
+    float damping;
+    int32_t horizon;        /* substeps to run after the warm-up; 0 = unbounded */
+    int32_t track_disp;     /* 1: accumulate U every substep (the reference always does) */
+    const fw_ground_desc* grounds;
+} fw_scene_desc;
+
+typedef struct {
+    int32_t tile_nodes;   /* max owned nodes per tile */
+    int32_t tile_slots;   /* max element slots (owned + halo) per tile: 1024 or 2048 */
+    int32_t block;        /* threads per workgroup: 256 */
+    int32_t grid_per_cu;  /* persistent workgroups per CU; 0 = one workgroup per tile */
+} fw_tuning;
+
+typedef struct {
+    int64_t capacity_bytes;
+    int64_t used_bytes;
+    int32_t n_scenes;
+    int32_t n_tiles;
+    int64_t n_nodes;
+    int64_t n_elems;
+    int64_t n_elems_j2;
+    int64_t global_step;       /* substeps launched so far on this arena */
+    double algo_bytes_per_substep; /* sum of E*B_e + N*B_n over resident scenes */
+    int64_t halo_slots;        /* element slots evaluated on top of owned ones */
+} fw_arena_info;
+
+typedef struct fw_arena fw_arena;
+
+const char* fw_last_error(void);
+int32_t fw_abi_version(void);
+
+/* Arena on `device` with `capacity_bytes` of HBM (0 = 80% of free memory). */
+int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning* tuning, fw_arena** out);
+int32_t fw_arena_destroy(fw_arena* a);
+int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out);
+
+/* Bytes of arena memory a scene needs (admission control). */
+int32_t fw_scene_footprint(const fw_arena* a, const fw_scene_desc* desc, int64_t* bytes);
+/* Copy a scene in, compute area / J^-1 / lumped mass on the device and schedule
+ * its zero-dt warm-up for the next substep.  Returns FW_E_CAPACITY when the
+ * arena cannot hold it (nothing is changed then). */
+int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* desc, int32_t* scene_id);
+int32_t fw_scene_remove(fw_arena* a, int32_t scene_id);
+/* Restart the scene's BVH-rebuild schedule: the next substep counts as step 0
+ * of the activation gate (ExplicitLoop.reset, flatworld/explicitloop.py:635-679). */
+int32_t fw_scene_reset_gate(fw_arena* a, int32_t scene_id);
+/* Substeps completed by the scene (warm-up excluded) and whether it reached its horizon. */
+int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, int32_t* finished);
+
+/* Enqueue n substeps of every resident scene on the arena stream (asynchronous).
+ * dt_override >= 0 replaces every scene's dt and damping_override >= 0 every
+ * scene's damping for these substeps (FemSpringManager.substep(dt, damping),
+ * flatworld/femspringmanager.py:938-939).  With
+ * FW_STEP_PER_SCENE each substep is issued as one launch per scene instead of
+ * one launch for the whole arena; results are identical. */
+int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags);
+/* n substeps captured once as a HIP graph (per n and flags) and replayed. */
+int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags);
+int32_t fw_sync(fw_arena* a);
+
+/* Host-only self-check of the tiling fw_scene_add would build (no GPU needed):
+ * stats[0] tiles, [1] halo slots, [2] largest tile slot count, [3] CSR entries.
+ * Returns FW_E_INVALID with a message if any invariant fails. */
+int32_t fw_tiling_check(const fw_scene_desc* desc, int32_t tile_nodes, int32_t tile_slots, int64_t* stats);
+
+/* Synchronous copies between a scene and host memory (caller's order). */
+int32_t fw_scene_read(fw_arena* a, int32_t scene_id, int32_t field, void* out, int64_t out_bytes);
+int32_t fw_scene_write(fw_arena* a, int32_t scene_id, int32_t field, const void* in, int64_t in_bytes);
+
+/* Time n substeps of all scenes with HIP events on the arena stream.
+ * mode 0: fw_step, 1: fw_step_graph, 2: fw_step with FW_STEP_PER_SCENE.
+ * Returns total milliseconds and the average duration of one substep kernel. */
+int32_t fw_time_steps(fw_arena* a, int32_t n, int32_t mode, double* total_ms, double* kernel_ms);
+
+/* Calibration: a plain float4 copy of `bytes` bytes, `reps` times; reports the
+ * read+write bandwidth and the average kernel time.  No reference counterpart:
+ * it measures the achievable HBM bandwidth the substep is compared against. */
+int32_t fw_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double* gbs, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

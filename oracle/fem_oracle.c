@@ -1,0 +1,506 @@
+/*
+ * fem_oracle.c -- serial float32 restatement of FlatWorld's explicit-FEM
+ * substep.  TEST INFRASTRUCTURE ONLY (see fem_oracle.h).
+ *
+ * Line references are to the reference package (flatworld/...).
+ */
+#include "fem_oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ 2x2 algebra */
+
+/* Row-major 2x2: m[0]=m00 m[1]=m01 m[2]=m10 m[3]=m11. */
+static void mat_mul(const float a[4], const float b[4], float c[4]) {
+    c[0] = a[0] * b[0] + a[1] * b[2];
+    c[1] = a[0] * b[1] + a[1] * b[3];
+    c[2] = a[2] * b[0] + a[3] * b[2];
+    c[3] = a[2] * b[1] + a[3] * b[3];
+}
+
+static float mat_det(const float m[4]) { return m[0] * m[3] - m[1] * m[2]; }
+
+/* Textbook adjugate inverse, adj(m) * (1/det). */
+static void mat_inv(const float m[4], float r[4]) {
+    float inv_det = 1.0f / mat_det(m);
+    r[0] = m[3] * inv_det;
+    r[1] = -m[1] * inv_det;
+    r[2] = -m[2] * inv_det;
+    r[3] = m[0] * inv_det;
+}
+
+/* C = F^T F */
+static void mat_ftf(const float F[4], float C[4]) {
+    C[0] = F[0] * F[0] + F[2] * F[2];
+    C[1] = F[0] * F[1] + F[2] * F[3];
+    C[2] = F[1] * F[0] + F[3] * F[2];
+    C[3] = F[1] * F[1] + F[3] * F[3];
+}
+
+static float log_f32(float x) { return (float)log((double)x); }
+
+/* ------------------------------------------------------------------ materials */
+
+/* flatworld/materials/materialfunctions.py:10-29 (getStress2D). */
+void ora_stress2d(int32_t type, float E, float nu, const float F[4], float stress[3], float strain[3]) {
+    float lam = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    float mu = E / (2.0f * (1.0f + nu));
+    float C[4];
+    mat_ftf(F, C);
+    /* strain = 0.5 * (F^T F - I) */
+    float e00 = 0.5f * (C[0] - 1.0f);
+    float e01 = 0.5f * (C[1] - 0.0f);
+    float e11 = 0.5f * (C[3] - 1.0f);
+    strain[0] = e00;
+    strain[1] = e11;
+    strain[2] = e01;
+    stress[0] = 0.0f;
+    stress[1] = 0.0f;
+    stress[2] = 0.0f;
+    if (type == ORA_ELASTIC) {
+        /* (lam * tr(E)) * I + 2 mu E */
+        float ltr = lam * (e00 + e11);
+        float tmu = 2.0f * mu;
+        stress[0] = ltr * 1.0f + tmu * e00;
+        stress[1] = ltr * 1.0f + tmu * e11;
+        stress[2] = ltr * 0.0f + tmu * e01;
+    } else if (type == ORA_NEOHOOKEAN) {
+        /* (lam ln J - mu) C^-1 + mu I */
+        float Ci[4];
+        mat_inv(C, Ci);
+        float J = mat_det(F);
+        float c = lam * log_f32(J) - mu;
+        stress[0] = c * Ci[0] + mu * 1.0f;
+        stress[1] = c * Ci[3] + mu * 1.0f;
+        stress[2] = c * Ci[1] + mu * 0.0f;
+    }
+}
+
+/* flatworld/materials/materialfunctions.py:35-101 (misesReturnMap2D).
+ * Returns 1 when the trial state is above yield (f_trial > 0). */
+int32_t ora_mises2d(float E, float nu, float sy, float H, const float F[4], const float eps_p_old[3],
+                    float eqps_old, float stress[3], float eps_p_new[3], float* eqps_new, float strain[3]) {
+    float lam = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    float mu = E / (2.0f * (1.0f + nu));
+    float C[4];
+    mat_ftf(F, C);
+    float et0 = 0.5f * (C[0] - 1.0f);
+    float et1 = 0.5f * (C[3] - 1.0f);
+    float et2 = 0.5f * (C[1] - 0.0f);
+    strain[0] = et0;
+    strain[1] = et1;
+    strain[2] = et2;
+    float ee0 = et0 - eps_p_old[0];
+    float ee1 = et1 - eps_p_old[1];
+    float ee2 = et2 - eps_p_old[2];
+    float tr = ee0 + ee1;
+    float s0 = lam * tr + 2.0f * mu * ee0;
+    float s1 = lam * tr + 2.0f * mu * ee1;
+    float s2 = 2.0f * mu * ee2;
+    float p = (s0 + s1) / 3.0f;
+    float d0 = s0 - p;
+    float d1 = s1 - p;
+    float d2 = s2;
+    float vm = sqrtf(d0 * d0 + d1 * d1 - d0 * d1 + 3.0f * d2 * d2);
+    float f_trial = vm - (sy + H * eqps_old);
+    eps_p_new[0] = eps_p_old[0];
+    eps_p_new[1] = eps_p_old[1];
+    eps_p_new[2] = eps_p_old[2];
+    *eqps_new = eqps_old;
+    stress[0] = s0;
+    stress[1] = s1;
+    stress[2] = s2;
+    if (f_trial > 0.0f) {
+        float denom = 3.0f * mu + H;
+        float dgamma = f_trial / denom;
+        *eqps_new = eqps_old + dgamma;
+        float scale = 1.0f - 3.0f * mu * dgamma / vm;
+        stress[0] = d0 * scale + p;
+        stress[1] = d1 * scale + p;
+        stress[2] = d2 * scale;
+        float n0 = d0 / vm, n1 = d1 / vm, n2 = d2 / vm;
+        eps_p_new[0] = eps_p_old[0] + dgamma * n0;
+        eps_p_new[1] = eps_p_old[1] + dgamma * n1;
+        eps_p_new[2] = eps_p_old[2] + dgamma * n2;
+        return 1;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ element */
+
+/* flatworld/femspringmanager.py:217-280 with get_b_matrix_2d (71-105).
+ * f holds el_fint = area * B^T S for corners 0..2 (before the minus sign of
+ * the scatter). */
+void ora_element_force(const ora_scene* s, int32_t e, float f[6], float cauchy[3], float strain[3],
+                       float eps_p_new[3], float* eqps_new, int32_t* yielded) {
+    const int32_t n1 = s->conn[3 * e + 0], n2 = s->conn[3 * e + 1], n3 = s->conn[3 * e + 2];
+    const float* x = s->x;
+    float Jac[4] = {x[2 * n1] - x[2 * n3], x[2 * n2] - x[2 * n3], x[2 * n1 + 1] - x[2 * n3 + 1],
+                    x[2 * n2 + 1] - x[2 * n3 + 1]};
+    const float* Ji = s->jinv + 4 * e;
+    float F[4];
+    mat_mul(Jac, Ji, F);
+
+    /* B matrix entries (3x6), flatworld/femspringmanager.py:71-105 */
+    float d00 = Ji[0], d01 = Ji[1], d10 = Ji[2], d11 = Ji[3];
+    float d20 = -Ji[0] - Ji[2];
+    float d21 = -Ji[1] - Ji[3];
+    float B[3][6];
+    B[0][0] = d00 * F[0];
+    B[0][1] = d00 * F[2];
+    B[1][0] = d01 * F[1];
+    B[1][1] = d01 * F[3];
+    B[2][0] = d01 * F[0] + d00 * F[1];
+    B[2][1] = d00 * F[3] + d01 * F[2];
+    B[0][2] = d10 * F[0];
+    B[0][3] = d10 * F[2];
+    B[1][2] = d11 * F[1];
+    B[1][3] = d11 * F[3];
+    B[2][2] = d11 * F[0] + d10 * F[1];
+    B[2][3] = d10 * F[3] + d11 * F[2];
+    B[0][4] = d20 * F[0];
+    B[0][5] = d20 * F[2];
+    B[1][4] = d21 * F[1];
+    B[1][5] = d21 * F[3];
+    B[2][4] = d21 * F[0] + d20 * F[1];
+    B[2][5] = d20 * F[3] + d21 * F[2];
+
+    float sig[3];
+    *yielded = 0;
+    if (s->mat_type == ORA_MISES) {
+        *yielded = ora_mises2d(s->E, s->nu, s->sigma_y, s->H, F, s->eps_p + 3 * e, s->eqps[e], sig, eps_p_new,
+                               eqps_new, strain);
+    } else {
+        ora_stress2d(s->mat_type, s->E, s->nu, F, sig, strain);
+    }
+
+    /* PK2 -> Cauchy (diagnostic), flatworld/femspringmanager.py:267-272 */
+    float S[4] = {sig[0], sig[2], sig[2], sig[1]};
+    float J = mat_det(F);
+    float FS[4], FSFt[4], Ft[4] = {F[0], F[2], F[1], F[3]};
+    mat_mul(F, S, FS);
+    mat_mul(FS, Ft, FSFt);
+    float jd = J > 1e-12f ? J : 1e-12f;
+    cauchy[0] = FSFt[0] / jd;
+    cauchy[1] = FSFt[3] / jd;
+    cauchy[2] = FSFt[1] / jd;
+
+    float a = s->area[e];
+    for (int k = 0; k < 6; ++k) {
+        float t = B[0][k] * sig[0] + B[1][k] * sig[1] + B[2][k] * sig[2];
+        f[k] = t * a;
+    }
+}
+
+/* ------------------------------------------------------------------ grounds */
+
+/* flatworld/mixedcontact.py:166-217 (_nearest_on_curve_2d). */
+static float hf_signed(const ora_ground* g, float x, float z, float n[2]) {
+    const int nx = g->hf_nx;
+    float span = g->hf_ub_x - g->hf_lb_x;
+    float u = 0.0f;
+    if (span > 1e-12f) u = (x - g->hf_lb_x) / span;
+    u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+    float sgrid = u * (float)(nx - 1);
+    int i0 = (int)floorf(sgrid);
+    int i1 = i0 + 1;
+    if (i1 >= nx) i1 = nx - 1;
+    if (i0 < 0) i0 = 0;
+    float t = sgrid - (float)i0;
+    float h = g->heights[i0] + (g->heights[i1] - g->heights[i0]) * t;
+    int ir = (int)roundf(sgrid);
+    if (ir < 0) ir = 0;
+    if (ir >= nx) ir = nx - 1;
+    int j0 = ir - 1, j1 = ir + 1;
+    if (j0 < 0) j0 = 0;
+    if (j1 >= nx) j1 = nx - 1;
+    float dxw = span / (float)(nx - 1);
+    float denom = dxw * (float)(j1 - j0);
+    float dhdx = 0.0f;
+    if (denom > 1e-6f) dhdx = (g->heights[j1] - g->heights[j0]) / denom;
+    float n0 = -dhdx, n1 = 1.0f;
+    if (g->reverse == 1) {
+        n0 = -n0;
+        n1 = -n1;
+    }
+    float nlen = sqrtf(n0 * n0 + n1 * n1);
+    if (nlen > 1e-12f) {
+        n0 = n0 / nlen;
+        n1 = n1 / nlen;
+    }
+    n[0] = n0;
+    n[1] = n1;
+    return (x - x) * n0 + (z - h) * n1;
+}
+
+/* flatworld/mixedcontact.py:220-302 (_voxel_signed_distance_2d). */
+static float voxel_signed(const ora_ground* g, const float p[2], float n[2]) {
+    const int nx = g->vox_nx, ny = g->vox_ny;
+    const int32_t* occ = g->occ;
+    float best_d = 1e9f;
+    n[0] = 0.0f;
+    n[1] = 1.0f;
+    int i = (int)floorf((p[0] - g->vox_lb[0]) / g->vox_dx);
+    int j = (int)floorf((p[1] - g->vox_lb[1]) / g->vox_dz);
+    if (0 <= i && i < nx && 0 <= j && j < ny && occ[i * ny + j] == 1) {
+        float dnx = 1e9f, dpx = 1e9f, dny = 1e9f, dpy = 1e9f;
+        for (int step = 0; step < nx; ++step) {
+            int idx = i - 1 - step;
+            if (idx < 0 || occ[idx * ny + j] == 0) {
+                float bx = g->vox_lb[0] + (float)(idx + 1) * g->vox_dx;
+                dnx = p[0] - bx;
+                break;
+            }
+        }
+        for (int step = 0; step < nx; ++step) {
+            int idx = i + 1 + step;
+            if (idx >= nx || occ[idx * ny + j] == 0) {
+                float bx = g->vox_lb[0] + (float)idx * g->vox_dx;
+                dpx = bx - p[0];
+                break;
+            }
+        }
+        for (int step = 0; step < ny; ++step) {
+            int idx = j - 1 - step;
+            if (idx < 0 || occ[i * ny + idx] == 0) {
+                float by = g->vox_lb[1] + (float)(idx + 1) * g->vox_dz;
+                dny = p[1] - by;
+                break;
+            }
+        }
+        for (int step = 0; step < ny; ++step) {
+            int idx = j + 1 + step;
+            if (idx >= ny || occ[i * ny + idx] == 0) {
+                float by = g->vox_lb[1] + (float)idx * g->vox_dz;
+                dpy = by - p[1];
+                break;
+            }
+        }
+        float mv = dnx;
+        int face = 0;
+        if (dpx < mv) { mv = dpx; face = 1; }
+        if (dny < mv) { mv = dny; face = 2; }
+        if (dpy < mv) { mv = dpy; face = 3; }
+        if (face == 0) { best_d = -dnx; n[0] = -1.0f; n[1] = 0.0f; }
+        else if (face == 1) { best_d = -dpx; n[0] = 1.0f; n[1] = 0.0f; }
+        else if (face == 2) { best_d = -dny; n[0] = 0.0f; n[1] = -1.0f; }
+        else { best_d = -dpy; n[0] = 0.0f; n[1] = 1.0f; }
+    }
+    return best_d;
+}
+
+/* Force of one ground on one boundary node; returns 1 when it pushes.
+ * Plane: flatworld/mixedcontact.py:660-684 (no damping).
+ * Height field: flatworld/mixedcontact.py:1566-1602.
+ * Voxel: flatworld/mixedcontact.py:1605-1643. */
+int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[2], float dt, float force[2]) {
+    const float k = g->penalty;
+    const float damping_coeff = 0.5f;
+    force[0] = 0.0f;
+    force[1] = 0.0f;
+    if (g->kind == ORA_PLANE) {
+        float pen = (p[0] - g->point[0]) * g->normal[0] + (p[1] - g->point[1]) * g->normal[1];
+        if (pen < 0.0f) {
+            force[0] = -((g->normal[0] * k) * pen);
+            force[1] = -((g->normal[1] * k) * pen);
+            return 1;
+        }
+        return 0;
+    }
+    if (g->kind == ORA_HEIGHTFIELD) {
+        float n[2];
+        float sd = hf_signed(g, p[0], p[1], n);
+        if (sd < 0.0f) {
+            float c0 = ((-n[0]) * k) * sd;
+            float c1 = ((-n[1]) * k) * sd;
+            float vn = vel[0] * n[0] + vel[1] * n[1];
+            if (vn < 0.0f) {
+                c0 = c0 + (((((-n[0]) * damping_coeff) * k) * vn) * dt);
+                c1 = c1 + (((((-n[1]) * damping_coeff) * k) * vn) * dt);
+            }
+            force[0] = c0;
+            force[1] = c1;
+            return 1;
+        }
+        return 0;
+    }
+    if (g->kind == ORA_VOXEL) {
+        float n[2];
+        float sd = voxel_signed(g, p, n);
+        if (sd < 0.0f) {
+            float pen = -sd;
+            float c0 = (n[0] * k) * pen;
+            float c1 = (n[1] * k) * pen;
+            float vn = vel[0] * n[0] + vel[1] * n[1];
+            if (vn < 0.0f) {
+                c0 = c0 + (((((-n[0]) * damping_coeff) * k) * vn) * dt);
+                c1 = c1 + (((((-n[1]) * damping_coeff) * k) * vn) * dt);
+            }
+            force[0] = c0;
+            force[1] = c1;
+            return 1;
+        }
+        return 0;
+    }
+    return 0;
+}
+
+/* Pair activation: inclusive AABB overlap of the FEM boundary AABB and the
+ * ground AABB, both widened by 0.1 * extent on full BVH rebuilds
+ * (counter % 100 == 0) and raw on refits.
+ * flatworld/explicitloop.py:477-487, flatworld/bvh.py:39-46, 321-342, 393-477. */
+int32_t ora_gate(const float fem_lb[2], const float fem_ub[2], const ora_ground* g, int32_t counter) {
+    float alb[2] = {fem_lb[0], fem_lb[1]}, aub[2] = {fem_ub[0], fem_ub[1]};
+    float blb[2] = {g->aabb_lo[0], g->aabb_lo[1]}, bub[2] = {g->aabb_hi[0], g->aabb_hi[1]};
+    if (counter % 100 == 0) {
+        for (int k = 0; k < 2; ++k) {
+            float ba = 0.1f * (aub[k] - alb[k]);
+            float la = alb[k] - ba, ua = aub[k] + ba;
+            float bb = 0.1f * (bub[k] - blb[k]);
+            float lb = blb[k] - bb, ub = bub[k] + bb;
+            alb[k] = la;
+            aub[k] = ua;
+            blb[k] = lb;
+            bub[k] = ub;
+        }
+    }
+    return alb[0] <= bub[0] && aub[0] >= blb[0] && alb[1] <= bub[1] && aub[1] >= blb[1];
+}
+
+/* flatworld/femspringmanager.py:361-408 (one FEM domain). */
+void ora_update_bbox(ora_scene* s) {
+    float lb0 = 1e30f, lb1 = 1e30f, ub0 = -1e30f, ub1 = -1e30f;
+    for (int32_t k = 0; k < s->n_bnodes; ++k) {
+        int32_t n = s->bnodes[k];
+        float cx = s->x[2 * n], cy = s->x[2 * n + 1];
+        lb0 = lb0 < cx ? lb0 : cx;
+        lb1 = lb1 < cy ? lb1 : cy;
+        ub0 = ub0 > cx ? ub0 : cx;
+        ub1 = ub1 > cy ? ub1 : cy;
+    }
+    s->aabb[0] = lb0;
+    s->aabb[1] = lb1;
+    s->aabb[2] = ub0;
+    s->aabb[3] = ub1;
+}
+
+/* ------------------------------------------------------------------ driver */
+
+/* flatworld/femspringmanager.py:113-138 */
+void ora_prepare(ora_scene* s) {
+    for (int32_t i = 0; i < s->n_nodes; ++i) s->mass[i] = 0.0f;
+    for (int32_t e = 0; e < s->n_elems; ++e) {
+        const int32_t n1 = s->conn[3 * e], n2 = s->conn[3 * e + 1], n3 = s->conn[3 * e + 2];
+        const float* x = s->x;
+        float Jac[4] = {x[2 * n1] - x[2 * n3], x[2 * n2] - x[2 * n3], x[2 * n1 + 1] - x[2 * n3 + 1],
+                        x[2 * n2 + 1] - x[2 * n3 + 1]};
+        float vol = fabsf(Jac[0] * Jac[3] - Jac[1] * Jac[2]) * 0.5f;
+        s->area[e] = vol;
+        mat_inv(Jac, s->jinv + 4 * e);
+        float nd_mass = vol * s->rho / 3.0f;
+        s->mass[n1] += nd_mass;
+        s->mass[n2] += nd_mass;
+        s->mass[n3] += nd_mass;
+    }
+    ora_update_bbox(s);
+}
+
+void ora_substep(ora_scene* s, float dt, float damping, int32_t counter, int32_t with_contact) {
+    const int32_t N = s->n_nodes;
+    float* fx = s->fext;
+    for (int32_t i = 0; i < 2 * N; ++i) fx[i] = 0.0f; /* _clear_fext_kernel of the previous substep */
+    if (s->contact_flag) memset(s->contact_flag, 0, (size_t)N);
+
+    /* ground forces at (x_n, v_n) in work-list order: pairs, then boundary nodes
+       (flatworld/mixedcontact.py:1654-1754) */
+    if (with_contact) {
+        for (int32_t gi = 0; gi < s->n_grounds; ++gi) {
+            const ora_ground* g = s->grounds + gi;
+            if (!ora_gate(s->aabb, s->aabb + 2, g, counter)) continue;
+            for (int32_t k = 0; k < s->n_bnodes; ++k) {
+                int32_t n = s->bnodes[k];
+                float f[2];
+                if (ora_ground_force(g, s->x + 2 * n, s->v + 2 * n, dt, f)) {
+                    fx[2 * n] += f[0];
+                    fx[2 * n + 1] += f[1];
+                    if (s->contact_flag) s->contact_flag[n] = 1;
+                }
+            }
+        }
+    }
+
+    /* element forces, ascending element id (flatworld/femspringmanager.py:947-967) */
+    for (int32_t e = 0; e < s->n_elems; ++e) {
+        float f[6], cauchy[3], strain[3], epn[3], eqn = 0.0f;
+        int32_t yielded = 0;
+        ora_element_force(s, e, f, cauchy, strain, epn, &eqn, &yielded);
+        if (s->mat_type == ORA_MISES) {
+            s->eps_p[3 * e] = epn[0];
+            s->eps_p[3 * e + 1] = epn[1];
+            s->eps_p[3 * e + 2] = epn[2];
+            s->eqps[e] = eqn;
+        }
+        if (s->yield_flag) s->yield_flag[e] = (uint8_t)yielded;
+        if (s->stress) {
+            s->stress[3 * e] = cauchy[0];
+            s->stress[3 * e + 1] = cauchy[1];
+            s->stress[3 * e + 2] = cauchy[2];
+        }
+        if (s->strain) {
+            s->strain[3 * e] = strain[0];
+            s->strain[3 * e + 1] = strain[1];
+            s->strain[3 * e + 2] = strain[2];
+        }
+        for (int c = 0; c < 3; ++c) {
+            int32_t n = s->conn[3 * e + c];
+            fx[2 * n] += -f[2 * c];
+            fx[2 * n + 1] += -f[2 * c + 1];
+        }
+    }
+
+    /* symplectic Euler, flatworld/femspringmanager.py:315-358 */
+    for (int32_t i = 0; i < N; ++i) {
+        const int32_t bc = s->bc[i];
+        const float m = s->mass[i];
+        const float bx = s->bcv[2 * i], by = s->bcv[2 * i + 1];
+        float f0 = fx[2 * i], f1 = fx[2 * i + 1];
+        if (bc & 1) { /* UTYPE */
+            f0 = 0.0f;
+            f1 = 0.0f;
+        } else if (bc & 4) { /* ATYPE */
+            f0 = m * bx;
+            f1 = m * by;
+        } else if (bc & 16) { /* GRAVITY */
+            f0 = f0 + m * bx;
+            f1 = f1 + m * by;
+        } else if (bc & 8) { /* FORCETYPE */
+            f0 = f0 + bx;
+            f1 = f1 + by;
+        }
+        fx[2 * i] = f0;
+        fx[2 * i + 1] = f1;
+        float rm = 1.0f / m;
+        float a0 = 0.0f + (f0 - damping * s->v[2 * i]) * rm;
+        float a1 = 0.0f + (f1 - damping * s->v[2 * i + 1]) * rm;
+        float v0 = s->v[2 * i] + a0 * dt;
+        float v1 = s->v[2 * i + 1] + a1 * dt;
+        if (bc & 2) { /* VTYPE */
+            v0 = bx;
+            v1 = by;
+        } else if (bc & 1) {
+            v0 = 0.0f;
+            v1 = 0.0f;
+        }
+        s->v[2 * i] = v0;
+        s->v[2 * i + 1] = v1;
+        float du0 = v0 * dt, du1 = v1 * dt;
+        s->u[2 * i] = s->u[2 * i] + du0;
+        s->u[2 * i + 1] = s->u[2 * i + 1] + du1;
+        s->x[2 * i] = s->x[2 * i] + du0;
+        s->x[2 * i + 1] = s->x[2 * i + 1] + du1;
+    }
+
+    ora_update_bbox(s);
+}

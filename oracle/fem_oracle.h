@@ -1,0 +1,101 @@
+/*
+ * fem_oracle.h -- CPU restatement of FlatWorld's explicit-FEM substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product path in flatworld_amd never calls
+ * it.
+ *
+ * Each routine runs serially over "thread ids" in ascending order, which is how
+ * Warp executes a kernel launch on its CPU device, so float accumulations
+ * (Fext atomics, lumped mass) happen in the reference's order.  All arithmetic
+ * is IEEE float32 without contraction (built with -ffp-contract=off);
+ * log(J) is evaluated as log((double)J) rounded to float.
+ */
+#ifndef FW_FEM_ORACLE_H
+#define FW_FEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_ELASTIC = 1, ORA_NEOHOOKEAN = 2, ORA_MISES = 3 };
+enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3 };
+
+typedef struct {
+    int32_t kind;
+    float penalty;
+    float aabb_lo[2];
+    float aabb_hi[2];
+    /* plane */
+    float point[2];
+    float normal[2];
+    /* height field */
+    const float* heights;
+    int32_t hf_nx;
+    float hf_lb_x;
+    float hf_ub_x;
+    int32_t reverse;
+    /* voxel grid, occ[i * ny + j] */
+    const int32_t* occ;
+    int32_t vox_nx;
+    int32_t vox_ny;
+    float vox_lb[2];
+    float vox_dx;
+    float vox_dz;
+} ora_ground;
+
+typedef struct {
+    int32_t n_nodes;
+    int32_t n_elems;
+    int32_t n_bnodes;
+    int32_t n_grounds;
+    float* x;        /* [2N] positions (x, y) */
+    float* v;        /* [2N] velocities */
+    float* u;        /* [2N] accumulated displacement */
+    float* fext;     /* [2N] nodal force of the last substep (after BCs) */
+    float* mass;     /* [N] lumped mass, filled by ora_prepare */
+    const int32_t* conn;   /* [3E] */
+    const int32_t* bc;     /* [N] BC bits */
+    const float* bcv;      /* [2N] BC value vector */
+    const int32_t* bnodes; /* [Nb] boundary nodes */
+    float* jinv;     /* [4E] row-major inverse reference Jacobian */
+    float* area;     /* [E] */
+    int32_t mat_type;
+    float E, nu, rho, sigma_y, H;
+    float* eps_p;    /* [3E] plastic strain (xx, yy, xy) */
+    float* eqps;     /* [E] equivalent plastic strain */
+    float* stress;   /* [3E] Cauchy stress (xx, yy, xy) of the last substep */
+    float* strain;   /* [3E] Green strain (xx, yy, xy) of the last substep */
+    uint8_t* yield_flag;   /* [E] f_trial > 0 in the last substep (J2 only) */
+    uint8_t* contact_flag; /* [N] a ground pushed this node in the last substep */
+    float aabb[4];   /* boundary-node AABB lb.x, lb.y, ub.x, ub.y at the current x */
+    const ora_ground* grounds;
+} ora_scene;
+
+/* flatworld/femspringmanager.py:113-138 (area, J^-1, lumped mass by atomics). */
+void ora_prepare(ora_scene* s);
+
+/* One ExplicitLoop.advance() for a one-FEM-body loop
+ * (flatworld/explicitloop.py:473-555): gate, ground forces, element forces,
+ * integrate, AABB.  with_contact=0 gives FemSpringManager.femStep alone (the
+ * zero-dt warm-up of flatworld/femspringmanager.py:872-879). */
+void ora_substep(ora_scene* s, float dt, float damping, int32_t counter, int32_t with_contact);
+
+/* Element-level building blocks, exported for unit tests. */
+void ora_stress2d(int32_t type, float E, float nu, const float F[4], float stress[3], float strain[3]);
+int32_t ora_mises2d(float E, float nu, float sy, float H, const float F[4], const float eps_p_old[3],
+                    float eqps_old, float stress[3], float eps_p_new[3], float* eqps_new, float strain[3]);
+void ora_element_force(const ora_scene* s, int32_t e, float f[6], float cauchy[3], float strain[3],
+                       float eps_p_new[3], float* eqps_new, int32_t* yielded);
+int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[2], float dt, float force[2]);
+int32_t ora_gate(const float fem_lb[2], const float fem_ub[2], const ora_ground* g, int32_t counter);
+void ora_update_bbox(ora_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

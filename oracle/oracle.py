@@ -1,0 +1,272 @@
+"""Python side of the CPU oracle.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and the cpu_baseline leg of bench.py
+import this module, as the checker or the timed CPU baseline.  The product
+package (flatworld_amd) never imports it.
+
+Contents
+* ``OracleScene``: ctypes driver of oracle/liboracle.so (fem_oracle.c), the
+  serial float32 restatement of the substep, fed the same ``SceneSpec`` arrays
+  the device receives.
+* ``boundary_edges_dict``: a straightforward restatement of
+  flatworld/mesh.py:53-71 (a dict over element edges), used to pin the
+  vectorized topology of flatworld_amd.mesh bit for bit.
+"""
+
+import ctypes
+import os
+import subprocess
+from collections import defaultdict
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class OraGround(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("penalty", ctypes.c_float),
+        ("aabb_lo", ctypes.c_float * 2),
+        ("aabb_hi", ctypes.c_float * 2),
+        ("point", ctypes.c_float * 2),
+        ("normal", ctypes.c_float * 2),
+        ("heights", _f32p),
+        ("hf_nx", ctypes.c_int32),
+        ("hf_lb_x", ctypes.c_float),
+        ("hf_ub_x", ctypes.c_float),
+        ("reverse", ctypes.c_int32),
+        ("occ", _i32p),
+        ("vox_nx", ctypes.c_int32),
+        ("vox_ny", ctypes.c_int32),
+        ("vox_lb", ctypes.c_float * 2),
+        ("vox_dx", ctypes.c_float),
+        ("vox_dz", ctypes.c_float),
+    ]
+
+
+class OraScene(ctypes.Structure):
+    _fields_ = [
+        ("n_nodes", ctypes.c_int32),
+        ("n_elems", ctypes.c_int32),
+        ("n_bnodes", ctypes.c_int32),
+        ("n_grounds", ctypes.c_int32),
+        ("x", _f32p),
+        ("v", _f32p),
+        ("u", _f32p),
+        ("fext", _f32p),
+        ("mass", _f32p),
+        ("conn", _i32p),
+        ("bc", _i32p),
+        ("bcv", _f32p),
+        ("bnodes", _i32p),
+        ("jinv", _f32p),
+        ("area", _f32p),
+        ("mat_type", ctypes.c_int32),
+        ("E", ctypes.c_float),
+        ("nu", ctypes.c_float),
+        ("rho", ctypes.c_float),
+        ("sigma_y", ctypes.c_float),
+        ("H", ctypes.c_float),
+        ("eps_p", _f32p),
+        ("eqps", _f32p),
+        ("stress", _f32p),
+        ("strain", _f32p),
+        ("yield_flag", _u8p),
+        ("contact_flag", _u8p),
+        ("aabb", ctypes.c_float * 4),
+        ("grounds", ctypes.POINTER(OraGround)),
+    ]
+
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.ora_prepare.argtypes = [ctypes.POINTER(OraScene)]
+        L.ora_substep.argtypes = [ctypes.POINTER(OraScene), ctypes.c_float, ctypes.c_float, ctypes.c_int32, ctypes.c_int32]
+        L.ora_stress2d.argtypes = [ctypes.c_int32, ctypes.c_float, ctypes.c_float, _f32p, _f32p, _f32p]
+        L.ora_mises2d.argtypes = [ctypes.c_float] * 4 + [_f32p, _f32p, ctypes.c_float, _f32p, _f32p, _f32p, _f32p]
+        L.ora_mises2d.restype = ctypes.c_int32
+        L.ora_ground_force.argtypes = [ctypes.POINTER(OraGround), _f32p, _f32p, ctypes.c_float, _f32p]
+        L.ora_ground_force.restype = ctypes.c_int32
+        L.ora_gate.argtypes = [_f32p, _f32p, ctypes.POINTER(OraGround), ctypes.c_int32]
+        L.ora_gate.restype = ctypes.c_int32
+        L.ora_update_bbox.argtypes = [ctypes.POINTER(OraScene)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def make_ground(gs, keep):
+    """OraGround from a flatworld_amd.scene.GroundSpec; ``keep`` retains numpy buffers."""
+    g = OraGround()
+    g.kind = int(gs.kind)
+    g.penalty = float(gs.penalty)
+    g.aabb_lo[:] = [float(v) for v in gs.aabb_lo]
+    g.aabb_hi[:] = [float(v) for v in gs.aabb_hi]
+    if gs.kind == 1:
+        g.point[:] = [float(v) for v in gs.point]
+        g.normal[:] = [float(v) for v in gs.normal]
+    elif gs.kind == 2:
+        h = np.ascontiguousarray(gs.heights, dtype=np.float32)
+        keep.append(h)
+        g.heights = _p(h, _f32p)
+        g.hf_nx = h.shape[0]
+        g.hf_lb_x = float(gs.hf_lb_x)
+        g.hf_ub_x = float(gs.hf_ub_x)
+        g.reverse = int(gs.reverse)
+    elif gs.kind == 3:
+        occ = np.ascontiguousarray(gs.occ, dtype=np.int32)
+        keep.append(occ)
+        g.occ = _p(occ, _i32p)
+        g.vox_nx, g.vox_ny = occ.shape
+        g.vox_lb[:] = [float(v) for v in gs.vox_lb]
+        g.vox_dx = float(gs.vox_dx)
+        g.vox_dz = float(gs.vox_dz)
+    return g
+
+
+class OracleScene:
+    """Serial CPU run of one packed scene (one FEM body + grounds)."""
+
+    def __init__(self, spec):
+        L = lib()
+        self.spec = spec
+        N, E = spec.n_nodes, spec.n_elems
+        self.x = np.ascontiguousarray(spec.x0, dtype=np.float32).copy()
+        self.v = np.ascontiguousarray(spec.v0, dtype=np.float32).copy()
+        self.u = np.zeros((N, 2), np.float32)
+        self.fext = np.zeros((N, 2), np.float32)
+        self.mass = np.zeros(N, np.float32)
+        self.conn = np.ascontiguousarray(spec.conn, dtype=np.int32)
+        self.bc = np.ascontiguousarray(spec.bc, dtype=np.int32)
+        self.bcv = np.ascontiguousarray(spec.bcv, dtype=np.float32)
+        self.bnodes = np.ascontiguousarray(spec.bnodes, dtype=np.int32)
+        self.jinv = np.zeros((E, 4), np.float32)
+        self.area = np.zeros(E, np.float32)
+        self.eps_p = np.zeros((E, 3), np.float32)
+        self.eqps = np.zeros(E, np.float32)
+        self.stress = np.zeros((E, 3), np.float32)
+        self.strain = np.zeros((E, 3), np.float32)
+        self.yield_flag = np.zeros(E, np.uint8)
+        self.contact_flag = np.zeros(N, np.uint8)
+        self._keep = []
+        gs = [make_ground(g, self._keep) for g in spec.grounds]
+        self._grounds = (OraGround * max(len(gs), 1))(*gs) if gs else (OraGround * 1)()
+        s = OraScene()
+        s.n_nodes, s.n_elems, s.n_bnodes, s.n_grounds = N, E, self.bnodes.shape[0], len(gs)
+        s.x, s.v, s.u, s.fext, s.mass = (_p(a, _f32p) for a in (self.x, self.v, self.u, self.fext, self.mass))
+        s.conn, s.bc, s.bnodes = _p(self.conn, _i32p), _p(self.bc, _i32p), _p(self.bnodes, _i32p)
+        s.bcv, s.jinv, s.area = _p(self.bcv, _f32p), _p(self.jinv, _f32p), _p(self.area, _f32p)
+        s.mat_type = int(spec.mat_type)
+        s.E, s.nu, s.rho = float(spec.E), float(spec.nu), float(spec.rho)
+        s.sigma_y, s.H = float(spec.sigma_y), float(spec.H)
+        s.eps_p, s.eqps = _p(self.eps_p, _f32p), _p(self.eqps, _f32p)
+        s.stress, s.strain = _p(self.stress, _f32p), _p(self.strain, _f32p)
+        s.yield_flag, s.contact_flag = _p(self.yield_flag, _u8p), _p(self.contact_flag, _u8p)
+        s.grounds = ctypes.cast(self._grounds, ctypes.POINTER(OraGround))
+        self._s = s
+        self._lib = L
+        self.counter = 0
+        L.ora_prepare(ctypes.byref(s))
+
+    @property
+    def aabb(self):
+        return np.array(self._s.aabb[:], dtype=np.float32)
+
+    def warmup(self):
+        """Zero-dt femStep run by setGlobalAABB (flatworld/femspringmanager.py:872-879)."""
+        self._lib.ora_substep(ctypes.byref(self._s), 0.0, 0.0, 0, 0)
+
+    def substep(self, dt, damping=0.0):
+        """One ExplicitLoop.advance(); ``dt`` is rounded to float32 like a Warp float argument."""
+        self._lib.ora_substep(ctypes.byref(self._s), float(np.float32(dt)), float(np.float32(damping)), self.counter, 1)
+        self.counter += 1
+
+    def run(self, n, dt=None, damping=None):
+        dt = self.spec.dt if dt is None else dt
+        damping = self.spec.damping if damping is None else damping
+        for _ in range(int(n)):
+            self.substep(dt, damping)
+
+
+def stress2d(mat_type, E, nu, F):
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float32).reshape(4))
+    s = np.zeros(3, np.float32)
+    e = np.zeros(3, np.float32)
+    lib().ora_stress2d(int(mat_type), float(np.float32(E)), float(np.float32(nu)), _p(F, _f32p), _p(s, _f32p), _p(e, _f32p))
+    return s, e
+
+
+def mises2d(E, nu, sy, H, F, eps_p_old, eqps_old):
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.float32).reshape(4))
+    ep = np.ascontiguousarray(np.asarray(eps_p_old, dtype=np.float32).reshape(3))
+    s = np.zeros(3, np.float32)
+    epn = np.zeros(3, np.float32)
+    eqn = np.zeros(1, np.float32)
+    st = np.zeros(3, np.float32)
+    y = lib().ora_mises2d(
+        float(np.float32(E)), float(np.float32(nu)), float(np.float32(sy)), float(np.float32(H)),
+        _p(F, _f32p), _p(ep, _f32p), float(np.float32(eqps_old)),
+        _p(s, _f32p), _p(epn, _f32p), _p(eqn, _f32p), _p(st, _f32p),
+    )
+    return s, epn, float(eqn[0]), st, bool(y)
+
+
+def ground_force(gs, p, vel, dt):
+    keep = []
+    g = make_ground(gs, keep)
+    p = np.ascontiguousarray(np.asarray(p, dtype=np.float32).reshape(2))
+    v = np.ascontiguousarray(np.asarray(vel, dtype=np.float32).reshape(2))
+    f = np.zeros(2, np.float32)
+    hit = lib().ora_ground_force(ctypes.byref(g), _p(p, _f32p), _p(v, _f32p), float(np.float32(dt)), _p(f, _f32p))
+    return f, bool(hit)
+
+
+def gate(fem_lb, fem_ub, gs, counter):
+    keep = []
+    g = make_ground(gs, keep)
+    lb = np.ascontiguousarray(np.asarray(fem_lb, dtype=np.float32).reshape(2))
+    ub = np.ascontiguousarray(np.asarray(fem_ub, dtype=np.float32).reshape(2))
+    return bool(lib().ora_gate(_p(lb, _f32p), _p(ub, _f32p), ctypes.byref(g), int(counter)))
+
+
+def boundary_edges_dict(conns):
+    """Straight restatement of flatworld/mesh.py:53-71 (dict insertion order)."""
+    edge_count = defaultdict(int)
+    for cell in conns:
+        for i in range(3):
+            v1, v2 = int(cell[i]), int(cell[(i + 1) % 3])
+            if (v2, v1) in edge_count:
+                edge_count[(v2, v1)] += 1
+            else:
+                edge_count[(v1, v2)] += 1
+    return np.array([e for e, c in edge_count.items() if c == 1])
+
+
+def char_lengths_loop(conns, coords):
+    """Per-edge float32 lengths with np.linalg.norm (flatworld/mesh.py:90-112)."""
+    out = []
+    for c in conns:
+        for ia, ib in ((0, 1), (1, 2), (2, 0)):
+            out.append(np.linalg.norm(coords[c[ia]] - coords[c[ib]]))
+    return np.asarray(out, dtype=np.float32)

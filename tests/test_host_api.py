@@ -1,0 +1,214 @@
+"""Host-side logic and the C ABI surface (no GPU needed).
+
+* Mesh topology and lengths vs the dict restatement of flatworld/mesh.py:53-112
+  and the committed fixtures (bit-exact);
+* BC / IC packing vs a straight restatement of
+  flatworld/femspringmanager.py:660-704;
+* stable-time and penalty scalars (flatworld/femdomain.py:24,
+  flatworld/femcontact.py:94-141);
+* libflatworld_fem.so loads and exports every function declared in
+  include/flatworld_fem.h; the host tiler's invariants hold on regular and
+  irregular meshes; argument errors map to the reference's exception types.
+"""
+
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import flatworld_amd as fw
+from flatworld_amd import _native
+from flatworld_amd import workloads as W
+from flatworld_amd.definitions import ATYPE, FORCETYPE, GRAVITY, UTYPE, VTYPE
+from oracle.oracle import boundary_edges_dict, char_lengths_loop
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _topology_meshes():
+    from tests.golden.make_golden import circle_mesh, grid100_mesh, quad_mesh
+
+    return {"quad": quad_mesh(), "circle": circle_mesh(), "grid100": grid100_mesh()}
+
+
+def test_mesh_topology_matches_fixture():
+    topo = json.load(open(os.path.join(GOLDEN, "topology.json")))
+    for name, (conn, coords) in _topology_meshes().items():
+        m = fw.Mesh(2, conn, coords)
+        assert m.boundaryElements.tolist() == topo[name]["boundary_edges"], name
+        assert m.boundaryNodes.tolist() == topo[name]["boundary_nodes"], name
+        assert m.numBoundNodes == len(topo[name]["boundary_nodes"])
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_mesh_boundary_random_orientations(seed):
+    """Random grids with shuffled elements and flipped orientations: vectorized == dict order."""
+    rng = np.random.default_rng(seed)
+    s = int(rng.integers(2, 9))
+    conn, coords = fw.structured_grid(s, jitter=0.2, seed=seed)
+    conn = conn[rng.permutation(conn.shape[0])]
+    flip = rng.random(conn.shape[0]) < 0.5
+    conn[flip] = conn[flip][:, [0, 2, 1]]
+    rot = rng.integers(0, 3, conn.shape[0])
+    conn = np.stack([np.roll(c, r) for c, r in zip(conn, rot)])
+    # drop some elements to create holes and notches
+    keep = rng.random(conn.shape[0]) > 0.2
+    conn = np.ascontiguousarray(conn[keep])
+    m = fw.Mesh(2, conn, coords)
+    ref = boundary_edges_dict(conn)
+    assert np.array_equal(m.boundaryElements, ref)
+    assert np.array_equal(m.boundaryNodes, np.array(sorted(set(ref.reshape(-1).tolist()))))
+
+
+def test_char_lengths_bitexact():
+    for name, (conn, coords) in _topology_meshes().items():
+        if name == "grid100":
+            conn = conn[:400]
+        m = fw.Mesh(2, conn, coords)
+        ref = char_lengths_loop(conn, coords)
+        assert np.array_equal(m._collect_edge_lengths(), ref)
+        assert m.charLength == float(np.min(ref))
+        assert m.charAverageLength == float(np.mean(ref))
+
+
+def _process_conditions_loop(domain):
+    """Straight restatement of flatworld/femspringmanager.py:660-704 for one domain."""
+    n = domain.nnodes
+    bc = np.zeros(n, np.int32)
+    bcv = np.zeros((n, 2), np.float32)
+    V = np.zeros((n, 2), np.float32)
+    for b in domain.bcs:
+        t, nodes, values = b.processData()
+        if values is None:
+            values = [0.0] * 2
+        if nodes is None:
+            nodes = range(n)
+        for i in nodes:
+            bc[int(i)] |= int(t)
+            bcv[int(i), 0] = float(values[0])
+            bcv[int(i), 1] = float(values[1])
+    for ic in domain.initials:
+        if ic.is_all_nodes:
+            V[:, 0], V[:, 1] = float(ic.vel[0]), float(ic.vel[1])
+        else:
+            for i in ic.nds:
+                V[int(i)] = [float(ic.vel[0]), float(ic.vel[1])]
+    return bc, bcv, V
+
+
+def test_bc_packing_matches_restatement():
+    conn, coords = fw.structured_grid(4)
+    bcs = [fw.Gravity([0.0, -9.8]), fw.Force([0, 3, 7], [1.0, 2.0]), fw.Fixed([7, 8]), fw.EnforceVel([8, 9], [0.5, -0.5]),
+           fw.EnforceAcc([10], [0.0, 3.0])]
+    ics = [fw.InitialVel("ALL", [0.1, 0.2]), fw.InitialVel([3, 4], [-1.0, 0.0])]
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), bcs, initials=ics)
+    spec = fw.pack_fem_domain(dom)
+    bc, bcv, V = _process_conditions_loop(dom)
+    assert np.array_equal(spec.bc, bc)
+    assert np.array_equal(spec.bcv, bcv)
+    assert np.array_equal(spec.v0, V)
+    flags, table = spec.node_flags()
+    assert np.array_equal(table[flags >> 16], bcv)
+    assert np.array_equal(flags & 0x1FF, bc.astype(np.uint32))
+    bn = np.zeros(spec.n_nodes, bool)
+    bn[spec.bnodes] = True
+    assert np.array_equal((flags & 0x8000) != 0, bn)
+    assert bc[7] == FORCETYPE | GRAVITY | UTYPE and bc[8] == GRAVITY | UTYPE | VTYPE and bc[10] == GRAVITY | ATYPE
+
+
+def test_stable_time_and_penalty_scalars():
+    conn, coords = fw.structured_grid(10, extent=0.4)
+    mat = fw.Elastic(5e4, 0.35, 900.0)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(mat), [])
+    L = dom.mesh.charLength
+    assert dom.stableTime == 0.4 * L / (5e4 / 900.0) ** 0.5
+    g = fw.GroundDomain(2, [0, 0], [0, 1])
+    spec = W.one_body_spec(dom, g)
+    K = 5e4 / (3.0 * (1.0 - 2.0 * 0.35))
+    assert spec.grounds[0].penalty == np.float32(2.0 * K)
+    m_node = 900.0 * (L * L * 0.5) / 3.0
+    dtc = 0.9 * (m_node / (2.0 * K)) ** 0.5
+    assert spec.dt == min(1e-3, dom.stableTime, dtc)
+    # nu is clamped at 0.4999 in the bulk modulus
+    from flatworld_amd.femcontact import ContactBase
+
+    assert ContactBase._bulk_modulus(fw.Elastic(1.0, 0.6, 1.0)) == 1.0 / (3.0 * (1.0 - 2.0 * 0.4999))
+
+
+def test_ground_aabbs():
+    g = fw.GroundDomain(2, [0.0, 0.3], [0.0, 2.0])
+    lo, hi = g.getBBox()
+    assert np.allclose(lo, [-10000.0, 0.2]) and np.allclose(hi, [10000.0, 0.4])
+    hf = fw.HeightFieldDomain(2, np.array([0.1, 0.3, 0.2], np.float32), [0.0, 0.0], [2.0, 1.0])
+    lo, hi = hf.getBBox()
+    assert np.allclose(lo, [0.0, 0.0]) and np.allclose(hi, [2.0, 0.4])
+
+
+# ------------------------------------------------------------------ C ABI
+
+
+def test_library_exports_declared_symbols():
+    header = open(os.path.join(ROOT, "include", "flatworld_fem.h")).read()
+    declared = set(re.findall(r"\b(fw_[a-z_]+)\s*\(", header))
+    assert declared, "no declarations found"
+    L = _native.load()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+        assert name in _native.SIGNATURES, f"{name} not bound in _native.SIGNATURES"
+    assert L.fw_abi_version() == 1
+
+
+@pytest.mark.parametrize("tile_nodes,tile_slots", [(512, 1024), (64, 1024), (1024, 2048), (7, 1024)])
+def test_tiler_invariants_grids(tile_nodes, tile_slots):
+    for side in (3, 17, 40):
+        dom = W.grid_domain(side, (0, 0), 1.0, fw.Elastic(1e4, 0.3, 1.0), [], jitter=0.1, seed=side)
+        st = _native.tiling_check(W.one_body_spec(dom, None, dt=1e-4), tile_nodes, tile_slots)
+        assert st["csr_entries"] == 3 * dom.nelements
+        assert st["max_slots"] <= tile_slots
+
+
+def test_tiler_invariants_irregular():
+    """Shuffled numbering, a fan node of degree 16 and a degenerate (repeated-node) element."""
+    rng = np.random.default_rng(3)
+    conn, coords = fw.structured_grid(12, jitter=0.2, seed=3)
+    perm = rng.permutation(coords.shape[0])
+    inv = np.argsort(perm)
+    coords = coords[perm]
+    conn = inv[conn].astype(np.int32)[rng.permutation(conn.shape[0])]
+    conn = np.vstack([conn, [[0, 0, 5]]]).astype(np.int32)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
+    for tn in (16, 100, 1000):
+        st = _native.tiling_check(W.one_body_spec(dom, None, dt=1e-4), tn, 1024)
+        assert st["csr_entries"] == 3 * conn.shape[0]
+    m = fw.FEMesher(2).createCircle([0.5, 0.5], 0.1)
+    dom = fw.FemDomain(m, fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
+    assert _native.tiling_check(W.one_body_spec(dom, None, dt=1e-4), 3, 1024)["tiles"] >= 6
+
+
+def test_abi_errors_map_to_exceptions():
+    conn, coords = fw.structured_grid(3)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
+    spec = W.one_body_spec(dom, None, dt=1e-4)
+    spec.conn = spec.conn.copy()
+    spec.conn[0, 0] = 999
+    with pytest.raises(ValueError, match="outside"):
+        _native.tiling_check(spec)
+    spec = W.one_body_spec(dom, None, dt=1e-4)
+    spec.mat_type = 7
+    with pytest.raises(ValueError, match="material"):
+        _native.tiling_check(spec)
+
+
+def test_explicitloop_rejects_out_of_scope_domains():
+    from flatworld_amd.explicitloop import ExplicitLoop
+
+    conn, coords = fw.structured_grid(2)
+    a = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
+    b = fw.FemDomain(fw.Mesh(2, conn, coords + 2.0), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
+    with pytest.raises(NotImplementedError, match="FEM-FEM"):
+        ExplicitLoop(0.0, [a, b])
+    with pytest.raises(NotImplementedError, match="mass scaling"):
+        ExplicitLoop(0.0, [a], mass_scaling_dt=1e-3)
