@@ -68,7 +68,7 @@ struct Tiling {
     std::vector<TileDev> tiles;      // scene field left 0; offsets relative to the scene
     std::vector<int32_t> halo;
     std::vector<uint16_t> csr_ptr;
-    std::vector<uint16_t> csr_ent;
+    std::vector<uint16_t> dest;  // 4 per slot: node-major LDS position of corners 0..2, pad
     int64_t halo_slots = 0;
 };
 
@@ -218,15 +218,20 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
         T.halo_slots += td.n_halo;
         T.halo.insert(T.halo.end(), hl.begin(), hl.end());
         if (td.n_own + td.n_halo > tile_slots) return fail(FW_E_INVALID, "internal tiling error: slot overflow");
+        // node-major incidence positions: node i of the tile owns LDS entries
+        // [ptr[i], ptr[i+1]) in ascending (caller element id, corner); each slot
+        // records where its three corner forces go (0xFFFF: node not owned here)
         td.ptr_off = (int32_t)T.csr_ptr.size();
-        td.ent_off = (int32_t)T.csr_ent.size();
+        td.dest_off = (int32_t)(T.dest.size() / 4);
+        T.dest.resize(T.dest.size() + 4 * (size_t)(td.n_own + td.n_halo), 0xFFFF);
+        uint16_t* dst = T.dest.data() + 4 * (size_t)td.dest_off;
         int cnt = 0;
         T.csr_ptr.push_back(0);
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
                 int e = inc[j] / 3, c = inc[j] % 3;
-                T.csr_ent.push_back((uint16_t)(slot_of[e] * 3 + c));
+                dst[4 * slot_of[e] + c] = (uint16_t)cnt;
                 ++cnt;
             }
             if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
@@ -241,7 +246,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
 struct Layout {
     int64_t pos0, pos1, vel, mass, flag, bcv, disp, force, cflag;
     int64_t c0, c1, c2, jinv, area, j2a, j2b, stress, strain, yflag;
-    int64_t halo, cptr, cent, tiles;
+    int64_t halo, cptr, dest, tiles;
     int64_t gdata[FW_MAX_GROUNDS];
     int64_t total;
 };
@@ -280,7 +285,7 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_halo, int64_t n_ptr, int64_
     L.yflag = take(E);
     L.halo = take(4 * n_halo);
     L.cptr = take(2 * n_ptr);
-    L.cent = take(2 * n_ent);
+    L.dest = take(2 * n_ent);
     L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
     for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
         L.gdata[g] = -1;
@@ -605,7 +610,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     Tiling& T = sh->tiling;
     const int N = d->n_nodes, E = d->n_elems;
     const int ntiles = (int)T.tiles.size();
-    Layout L = make_layout(d, (int64_t)T.halo.size(), (int64_t)T.csr_ptr.size(), (int64_t)T.csr_ent.size(), ntiles);
+    Layout L = make_layout(d, (int64_t)T.halo.size(), (int64_t)T.csr_ptr.size(), (int64_t)T.dest.size(), ntiles);
     int id;
     if (!a->free_ids.empty()) {
         id = a->free_ids.back();
@@ -651,7 +656,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     }
     std::memcpy(img + L.halo, T.halo.data(), 4 * T.halo.size());
     std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
-    std::memcpy(img + L.cent, T.csr_ent.data(), 2 * T.csr_ent.size());
+    std::memcpy(img + L.dest, T.dest.data(), 2 * T.dest.size());
     for (auto& td : T.tiles) td.scene = id;
     std::memcpy(img + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
     char* blk = a->base + off;
@@ -721,7 +726,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     D.yflag = (uint8_t*)(blk + L.yflag);
     D.halo = (const int32_t*)(blk + L.halo);
     D.csr_ptr = (const uint16_t*)(blk + L.cptr);
-    D.csr_ent = (const uint16_t*)(blk + L.cent);
+    D.dest = (const uint2*)(blk + L.dest);
     D.n_nodes = N;
     D.n_elems = E;
     D.mat_type = d->mat_type;
@@ -843,6 +848,19 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
         if (td.n_own + td.n_halo > tile_slots || td.n_nodes > tile_nodes) return fail(FW_E_INVALID, "tile over capacity");
         max_slots = std::max<int64_t>(max_slots, td.n_own + td.n_halo);
         for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) owned[T.elem_perm[i]]++;
+        // invert the per-slot destinations: LDS position -> (caller element, corner)
+        const int nslots = td.n_own + td.n_halo;
+        const uint16_t* cp = T.csr_ptr.data() + td.ptr_off;
+        std::vector<int64_t> at(cp[td.n_nodes], -1);
+        for (int sl = 0; sl < nslots; ++sl) {
+            const int eint = sl < td.n_own ? td.elem0 + sl : T.halo[td.halo_off + sl - td.n_own];
+            for (int c = 0; c < 3; ++c) {
+                const uint16_t pos = T.dest[4 * ((size_t)td.dest_off + sl) + c];
+                if (pos == 0xFFFF) continue;
+                if (pos >= at.size() || at[pos] != -1) return fail(FW_E_INVALID, "slot destinations collide");
+                at[pos] = 3 * (int64_t)T.elem_perm[eint] + c;
+            }
+        }
         for (int i = 0; i < td.n_nodes; ++i) {
             const int p = td.node0 + i;
             const int n = T.node_perm[p];
@@ -850,15 +868,9 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
             for (int e = 0; e < E; ++e)  // brute force: fine for the small meshes this is used on
                 for (int c = 0; c < 3; ++c)
                     if (d->conn[3 * e + c] == n) want.push_back(3 * (int64_t)e + c);
-            const uint16_t* cp = T.csr_ptr.data() + td.ptr_off;
-            for (int j = cp[i]; j < cp[i + 1]; ++j) {
-                const int ent = T.csr_ent[td.ent_off + j];
-                const int slot = ent / 3, c = ent % 3;
-                const int eint = slot < td.n_own ? td.elem0 + slot : T.halo[td.halo_off + slot - td.n_own];
-                got.push_back(3 * (int64_t)T.elem_perm[eint] + c);
-            }
+            for (int j = cp[i]; j < cp[i + 1]; ++j) got.push_back(at[j]);
             entries += (int64_t)got.size();
-            if (want != got) return fail(FW_E_INVALID, "CSR of node " + std::to_string(n) + " is wrong");
+            if (want != got) return fail(FW_E_INVALID, "incidence order of node " + std::to_string(n) + " is wrong");
         }
     }
     for (int e = 0; e < E; ++e)

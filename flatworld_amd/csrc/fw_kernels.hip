@@ -296,6 +296,58 @@ __device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
     }
 }
 
+// One element slot: F = Jac(x) J^-1, stress, corner forces area * B^T S
+// (flatworld/femspringmanager.py:71-105, 240-276), stored at the LDS entries
+// of the owning tile's node-major incidence lists.
+__device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, float2 p3, float4 ji, float ar, float4 st,
+                                           uint2 dd, bool own, int e, bool diag, float4* __restrict__ j2_wr,
+                                           float4* __restrict__ stress, float4* __restrict__ strain,
+                                           uint8_t* __restrict__ yflag, float2* lds_f) {
+    const float J0 = p1.x - p3.x, J1 = p2.x - p3.x, J2 = p1.y - p3.y, J3 = p2.y - p3.y;
+    float F[4];
+    F[0] = J0 * ji.x + J1 * ji.z;
+    F[1] = J0 * ji.y + J1 * ji.w;
+    F[2] = J2 * ji.x + J3 * ji.z;
+    F[3] = J2 * ji.y + J3 * ji.w;
+    float s[3], g[3];
+    bool yielded = false;
+    if (m.type == 3) {
+        float4 stn;
+        yielded = stress_j2(m, F, st, s, g, stn);
+        if (own) j2_wr[e] = stn;
+    } else {
+        stress_hyper(m, F, s, g);
+    }
+    const float d00 = ji.x, d01 = ji.y, d10 = ji.z, d11 = ji.w;
+    const float d20 = -ji.x - ji.z;
+    const float d21 = -ji.y - ji.w;
+    const float f0 = ((d00 * F[0]) * s[0] + (d01 * F[1]) * s[1] + (d01 * F[0] + d00 * F[1]) * s[2]) * ar;
+    const float f1 = ((d00 * F[2]) * s[0] + (d01 * F[3]) * s[1] + (d00 * F[3] + d01 * F[2]) * s[2]) * ar;
+    const float f2 = ((d10 * F[0]) * s[0] + (d11 * F[1]) * s[1] + (d11 * F[0] + d10 * F[1]) * s[2]) * ar;
+    const float f3 = ((d10 * F[2]) * s[0] + (d11 * F[3]) * s[1] + (d10 * F[3] + d11 * F[2]) * s[2]) * ar;
+    const float f4 = ((d20 * F[0]) * s[0] + (d21 * F[1]) * s[1] + (d21 * F[0] + d20 * F[1]) * s[2]) * ar;
+    const float f5 = ((d20 * F[2]) * s[0] + (d21 * F[3]) * s[1] + (d20 * F[3] + d21 * F[2]) * s[2]) * ar;
+    const uint32_t q0 = dd.x & 0xFFFFu, q1 = dd.x >> 16, q2 = dd.y & 0xFFFFu;
+    if (q0 != 0xFFFFu) lds_f[q0] = make_float2(f0, f1);
+    if (q1 != 0xFFFFu) lds_f[q1] = make_float2(f2, f3);
+    if (q2 != 0xFFFFu) lds_f[q2] = make_float2(f4, f5);
+    if (diag && own) {
+        // PK2 -> Cauchy F S F^T / max(J, 1e-12) (flatworld/femspringmanager.py:267-272)
+        const float FS0 = F[0] * s[0] + F[1] * s[2];
+        const float FS1 = F[0] * s[2] + F[1] * s[1];
+        const float FS2 = F[2] * s[0] + F[3] * s[2];
+        const float FS3 = F[2] * s[2] + F[3] * s[1];
+        const float c00 = FS0 * F[0] + FS1 * F[1];
+        const float c01 = FS0 * F[2] + FS1 * F[3];
+        const float c11 = FS2 * F[2] + FS3 * F[3];
+        const float J = F[0] * F[3] - F[1] * F[2];
+        const float jd = J > 1e-12f ? J : 1e-12f;
+        stress[e] = make_float4(c00 / jd, c11 / jd, c01 / jd, 0.0f);
+        strain[e] = make_float4(g[0], g[1], g[2], 0.0f);
+        yflag[e] = yielded ? 1 : 0;
+    }
+}
+
 // ------------------------------------------------------------------ substep kernel
 
 struct StepArgs {
@@ -343,58 +395,34 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         m.sy = S.sigma_y;
         m.H = S.H;
 
-        // ---------------- phase A: element slots
+        // ---------------- phase A: element slots, two per lane per iteration with
+        // every independent load of both issued before the dependent gathers
         const int nslots = T.n_own + T.n_halo;
-        for (int sl = threadIdx.x; sl < nslots; sl += BLOCK) {
-            const bool own = sl < T.n_own;
-            const int e = own ? T.elem0 + sl : S.halo[T.halo_off + sl - T.n_own];
-            const int n1 = S.c0[e], n2 = S.c1[e], n3 = S.c2[e];
-            const float2 p1 = pos_rd[n1], p2 = pos_rd[n2], p3 = pos_rd[n3];
-            const float4 ji = S.jinv[e];
-            const float ar = S.area[e];
-            const float J0 = p1.x - p3.x, J1 = p2.x - p3.x, J2 = p1.y - p3.y, J3 = p2.y - p3.y;
-            float F[4];
-            F[0] = J0 * ji.x + J1 * ji.z;
-            F[1] = J0 * ji.y + J1 * ji.w;
-            F[2] = J2 * ji.x + J3 * ji.z;
-            F[3] = J2 * ji.y + J3 * ji.w;
-            float s[3], g[3];
-            bool yielded = false;
-            if (m.type == 3) {
-                float4 stn;
-                yielded = stress_j2(m, F, S.j2[rd][e], s, g, stn);
-                if (own) S.j2[wr][e] = stn;
-            } else {
-                stress_hyper(m, F, s, g);
+        const uint2* __restrict__ dest = S.dest + T.dest_off;
+        const bool j2 = m.type == 3;
+        for (int slA = threadIdx.x; slA < nslots; slA += 2 * BLOCK) {
+            const int slB = slA + BLOCK;
+            const bool vB = slB < nslots;
+            const bool ownA = slA < T.n_own, ownB = slB < T.n_own;
+            const int eA = ownA ? T.elem0 + slA : S.halo[T.halo_off + slA - T.n_own];
+            const int eB = !vB ? eA : (ownB ? T.elem0 + slB : S.halo[T.halo_off + slB - T.n_own]);
+            const int a1 = S.c0[eA], a2 = S.c1[eA], a3 = S.c2[eA];
+            const int b1 = S.c0[eB], b2 = S.c1[eB], b3 = S.c2[eB];
+            const float4 jiA = S.jinv[eA], jiB = S.jinv[eB];
+            const float arA = S.area[eA], arB = S.area[eB];
+            const uint2 ddA = dest[slA];
+            const uint2 ddB = vB ? dest[slB] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+            float4 stA = make_float4(0.0f, 0.0f, 0.0f, 0.0f), stB = stA;
+            if (j2) {
+                stA = S.j2[rd][eA];
+                stB = S.j2[rd][eB];
             }
-            // corner forces area * B^T S (flatworld/femspringmanager.py:71-105, 275-276)
-            const float d00 = ji.x, d01 = ji.y, d10 = ji.z, d11 = ji.w;
-            const float d20 = -ji.x - ji.z;
-            const float d21 = -ji.y - ji.w;
-            const float f0 = ((d00 * F[0]) * s[0] + (d01 * F[1]) * s[1] + (d01 * F[0] + d00 * F[1]) * s[2]) * ar;
-            const float f1 = ((d00 * F[2]) * s[0] + (d01 * F[3]) * s[1] + (d00 * F[3] + d01 * F[2]) * s[2]) * ar;
-            const float f2 = ((d10 * F[0]) * s[0] + (d11 * F[1]) * s[1] + (d11 * F[0] + d10 * F[1]) * s[2]) * ar;
-            const float f3 = ((d10 * F[2]) * s[0] + (d11 * F[3]) * s[1] + (d10 * F[3] + d11 * F[2]) * s[2]) * ar;
-            const float f4 = ((d20 * F[0]) * s[0] + (d21 * F[1]) * s[1] + (d21 * F[0] + d20 * F[1]) * s[2]) * ar;
-            const float f5 = ((d20 * F[2]) * s[0] + (d21 * F[3]) * s[1] + (d20 * F[3] + d21 * F[2]) * s[2]) * ar;
-            lds_f[sl * 3 + 0] = make_float2(f0, f1);
-            lds_f[sl * 3 + 1] = make_float2(f2, f3);
-            lds_f[sl * 3 + 2] = make_float2(f4, f5);
-            if (diag && own) {
-                // PK2 -> Cauchy F S F^T / max(J, 1e-12) (flatworld/femspringmanager.py:267-272)
-                const float FS0 = F[0] * s[0] + F[1] * s[2];
-                const float FS1 = F[0] * s[2] + F[1] * s[1];
-                const float FS2 = F[2] * s[0] + F[3] * s[2];
-                const float FS3 = F[2] * s[2] + F[3] * s[1];
-                const float c00 = FS0 * F[0] + FS1 * F[1];
-                const float c01 = FS0 * F[2] + FS1 * F[3];
-                const float c11 = FS2 * F[2] + FS3 * F[3];
-                const float J = F[0] * F[3] - F[1] * F[2];
-                const float jd = J > 1e-12f ? J : 1e-12f;
-                S.stress[e] = make_float4(c00 / jd, c11 / jd, c01 / jd, 0.0f);
-                S.strain[e] = make_float4(g[0], g[1], g[2], 0.0f);
-                S.yflag[e] = yielded ? 1 : 0;
-            }
+            const float2 pA1 = pos_rd[a1], pA2 = pos_rd[a2], pA3 = pos_rd[a3];
+            const float2 pB1 = pos_rd[b1], pB2 = pos_rd[b2], pB3 = pos_rd[b3];
+            slot_force(m, pA1, pA2, pA3, jiA, arA, stA, ddA, ownA, eA, diag, S.j2[wr], S.stress, S.strain, S.yflag, lds_f);
+            if (vB)
+                slot_force(m, pB1, pB2, pB3, jiB, arB, stB, ddB, ownB, eB, diag, S.j2[wr], S.stress, S.strain, S.yflag,
+                           lds_f);
         }
 
         // ---------------- activation gate (flatworld/explicitloop.py:477-487, bvh.py:39-46, 321-342, 393-416)
@@ -423,22 +451,22 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // ---------------- phase B: owned nodes
         int32_t bb[4] = {fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)};
         const uint16_t* __restrict__ cptr = S.csr_ptr + T.ptr_off;
-        const uint16_t* __restrict__ cent = S.csr_ent + T.ent_off;
         for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
             const int n = T.node0 + i;
             const float2 p = pos_rd[n];
             float2 v = S.vel[n];
             const uint32_t fl = S.flag[n];
             const float mass = S.mass[n];
+            const int jb = cptr[i], je = cptr[i + 1];
             float fx = 0.0f, fy = 0.0f;
             bool hit = false;
             if (active && (fl & NODE_BOUNDARY)) {
                 for (int gi = 0; gi < S.n_grounds; ++gi)
                     if (active & (1u << gi)) hit |= ground_force(S.ground[gi], p.x, p.y, v.x, v.y, dt, fx, fy);
             }
-            const int jb = cptr[i], je = cptr[i + 1];
+            // element forces in ascending element id: a contiguous run of LDS entries
             for (int j = jb; j < je; ++j) {
-                const float2 f = lds_f[cent[j]];
+                const float2 f = lds_f[j];
                 fx = fx + (-f.x);
                 fy = fy + (-f.y);
             }
@@ -522,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 template <int BLOCK, int SLOTS>
 __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, const TileDev* tiles, int tile_base,
                                                         int tile_count) {
-    __shared__ float lds_m[SLOTS];
+    __shared__ float lds_m[SLOTS * 3];
     const int ti = blockIdx.x;
     if (ti >= tile_count) return;
     const TileDev T = tiles[tile_base + ti];
@@ -541,14 +569,18 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
             S.area[e] = vol;
             S.jinv[e] = make_float4(J3 * inv_det, -J1 * inv_det, -J2 * inv_det, J0 * inv_det);
         }
-        lds_m[sl] = vol * S.rho / 3.0f;
+        const float nd = vol * S.rho / 3.0f;
+        const uint2 dd = S.dest[T.dest_off + sl];
+        const uint32_t q0 = dd.x & 0xFFFFu, q1 = dd.x >> 16, q2 = dd.y & 0xFFFFu;
+        if (q0 != 0xFFFFu) lds_m[q0] = nd;
+        if (q1 != 0xFFFFu) lds_m[q1] = nd;
+        if (q2 != 0xFFFFu) lds_m[q2] = nd;
     }
     __syncthreads();
     const uint16_t* cptr = S.csr_ptr + T.ptr_off;
-    const uint16_t* cent = S.csr_ent + T.ent_off;
     for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
         float m = 0.0f;
-        for (int j = cptr[i]; j < cptr[i + 1]; ++j) m += lds_m[cent[j] / 3];
+        for (int j = cptr[i]; j < cptr[i + 1]; ++j) m += lds_m[j];
         S.mass[T.node0 + i] = m;
     }
 }

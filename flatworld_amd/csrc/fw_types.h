@@ -7,7 +7,7 @@
 //   elements c0/c1/c2 int32 (internal node ids), jinv float4, area float,
 //            j2[2] float4 (eps_xx, eps_yy, eps_xy, eqps; ping-pong, J2 scenes only),
 //            stress float4 / strain float4 (diag), yflag uint8 (diag)
-//   tiles    halo int32 list, csr_ptr uint16 per owned node (+1), csr entries uint16
+//   tiles    halo int32 list, csr_ptr uint16 per owned node (+1), dest uint2 per element slot
 //   ground   height samples float / occupancy int8
 #pragma once
 #include <hip/hip_runtime.h>
@@ -47,8 +47,8 @@ struct SceneDev {
     uint8_t* yflag;  // diag
     // tile data
     const int32_t* halo;
-    const uint16_t* csr_ptr;
-    const uint16_t* csr_ent;
+    const uint16_t* csr_ptr;  // per tile: n_nodes + 1 offsets into the tile's node-major LDS entries
+    const uint2* dest;        // per slot: LDS entry of corners 0..2 (uint16 each, 0xFFFF = not owned)
     // scalars
     int32_t n_nodes, n_elems;
     int32_t mat_type, n_grounds;
@@ -68,7 +68,7 @@ struct TileDev {
     int32_t elem0, n_own;     // owned elements [elem0, elem0 + n_own)
     int32_t halo_off, n_halo; // halo elements halo[halo_off .. +n_halo)
     int32_t ptr_off;          // csr_ptr[ptr_off .. ptr_off + n_nodes]
-    int32_t ent_off;          // base of this tile's csr entries
+    int32_t dest_off;         // dest[dest_off + slot]
     int32_t multi;            // scene has more than one tile (AABB via atomics)
     int32_t pad[2];
 };

@@ -40,6 +40,16 @@ static void mat_ftf(const float F[4], float C[4]) {
 
 static float log_f32(float x) { return (float)log((double)x); }
 
+/* float -> int32 with the device conversion's semantics: NaN -> 0, saturating
+ * at the int32 range.  Identical to a C cast for finite in-range values; the
+ * reference leaves the other cases undefined. */
+static int cvt_i32(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return 2147483647;
+    if (v <= -2147483648.0f) return (int)(-2147483647 - 1);
+    return (int)v;
+}
+
 /* ------------------------------------------------------------------ materials */
 
 /* flatworld/materials/materialfunctions.py:10-29 (getStress2D). */
@@ -204,13 +214,13 @@ static float hf_signed(const ora_ground* g, float x, float z, float n[2]) {
     if (span > 1e-12f) u = (x - g->hf_lb_x) / span;
     u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
     float sgrid = u * (float)(nx - 1);
-    int i0 = (int)floorf(sgrid);
+    int i0 = cvt_i32(floorf(sgrid));
     int i1 = i0 + 1;
     if (i1 >= nx) i1 = nx - 1;
     if (i0 < 0) i0 = 0;
     float t = sgrid - (float)i0;
     float h = g->heights[i0] + (g->heights[i1] - g->heights[i0]) * t;
-    int ir = (int)roundf(sgrid);
+    int ir = cvt_i32(roundf(sgrid));
     if (ir < 0) ir = 0;
     if (ir >= nx) ir = nx - 1;
     int j0 = ir - 1, j1 = ir + 1;
@@ -242,8 +252,8 @@ static float voxel_signed(const ora_ground* g, const float p[2], float n[2]) {
     float best_d = 1e9f;
     n[0] = 0.0f;
     n[1] = 1.0f;
-    int i = (int)floorf((p[0] - g->vox_lb[0]) / g->vox_dx);
-    int j = (int)floorf((p[1] - g->vox_lb[1]) / g->vox_dz);
+    int i = cvt_i32(floorf((p[0] - g->vox_lb[0]) / g->vox_dx));
+    int j = cvt_i32(floorf((p[1] - g->vox_lb[1]) / g->vox_dz));
     if (0 <= i && i < nx && 0 <= j && j < ny && occ[i * ny + j] == 1) {
         float dnx = 1e9f, dpx = 1e9f, dny = 1e9f, dpy = 1e9f;
         for (int step = 0; step < nx; ++step) {

@@ -57,9 +57,9 @@ def test_elastic_multitile_bitexact():
 
 def _j2_voxel_spec():
     rng = W.rng_for(7)
-    mat = fw.MisesPlastic(1e5, 0.3, 1000.0, 500.0, 1e3)
-    dom = W.grid_domain(32, (0.25, 0.3), 0.5, mat, [fw.Gravity([0.0, -9.8])], initials=[fw.InitialVel("ALL", [0.0, -2.0])])
-    ground = W.value_noise_voxels(rng, 64, 64, [0.0, 0.0], [1.0, 0.35], floor_rows=3)
+    mat = fw.MisesPlastic(1e5, 0.3, 300.0, 500.0, 1e3)
+    dom = W.grid_domain(32, (0.25, 0.56), 0.5, mat, [fw.Gravity([0.0, -9.8])], initials=[fw.InitialVel("ALL", [0.0, -1.0])])
+    ground = W.value_noise_voxels(rng, 64, 64, [0.0, 0.0], [1.0, 0.55], floor_rows=2)
     return W.one_body_spec(dom, ground)
 
 
