@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tile-nodes", type=int, default=512)
     ap.add_argument("--tile-slots", type=int, default=1024)
+    ap.add_argument("--block", type=int, default=256, help="threads per tile workgroup")
     ap.add_argument("--per-scene-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -112,7 +113,7 @@ def main():
     E = sum(s.n_elems for s in specs)
     N = sum(s.n_nodes for s in specs)
     algo = float(sum(W.algorithmic_bytes(s) for s in specs))
-    arena = _native.Arena(tile_nodes=args.tile_nodes, tile_slots=args.tile_slots)
+    arena = _native.Arena(tile_nodes=args.tile_nodes, tile_slots=args.tile_slots, block=args.block)
     for s in specs:
         arena.add(s, track_disp=True)
     arena.step(1, -1.0, 0)  # zero-dt warm-up of every scene

@@ -66,9 +66,9 @@ struct Tiling {
     std::vector<int32_t> elem_perm;  // internal -> caller element id
     std::vector<int32_t> elem_inv;
     std::vector<TileDev> tiles;      // scene field left 0; offsets relative to the scene
-    std::vector<int32_t> halo;
+    std::vector<uint32_t> rec;       // 4 per slot (one uint4): local corner ids, LDS destinations, element
+    std::vector<int32_t> hnode;      // per tile: storage ids of the halo nodes, ascending
     std::vector<uint16_t> csr_ptr;
-    std::vector<uint16_t> dest;  // 4 per slot: node-major LDS position of corners 0..2, pad
     int64_t halo_slots = 0;
 };
 
@@ -81,7 +81,7 @@ uint32_t spread16(uint32_t v) {
     return v;
 }
 
-int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling& T) {
+int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int node_cap, Tiling& T) {
     const int N = d->n_nodes, E = d->n_elems;
     // incidence (caller ids): node -> (element, corner), element-major so lists are ascending
     std::vector<int32_t> inc_ptr(N + 1, 0);
@@ -101,7 +101,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
     // node order
     T.node_perm.resize(N);
     for (int i = 0; i < N; ++i) T.node_perm[i] = i;
-    const bool single = N <= tile_nodes && E <= tile_slots;
+    const bool single = N <= tile_nodes && E <= tile_slots && N <= node_cap;
     if (!single) {
         float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
         for (int i = 0; i < N; ++i)
@@ -130,36 +130,58 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
     T.node_inv.assign(N, 0);
     for (int i = 0; i < N; ++i) T.node_inv[T.node_perm[i]] = i;
 
-    // greedy tiles over internal node order
-    std::vector<int32_t> stamp(E, -1), seen(E, -1);
+    // greedy tiles over internal node order: at most tile_nodes owned nodes, tile_slots
+    // incident elements and node_cap nodes in LDS (owned + nodes of incident elements)
+    std::vector<int32_t> stamp(E, -1), seen(E, -1), nstamp(N, -1), nseen(N, -1);
     std::vector<int32_t> tile_start;  // internal node index where each tile starts
     {
-        int tid = 0, cur_nodes = 0, cur_slots = 0;
+        int tid = 0, cur_nodes = 0, cur_slots = 0, cur_lds = 0;
         if (N > 0) tile_start.push_back(0);
         for (int p = 0; p < N; ++p) {
             const int n = T.node_perm[p];
-            int add = 0, all = 0;  // distinct incident elements: new to this tile / in total
+            // what adding node n costs, relative to the current tile (add*) and to an empty one (all*)
+            int add = 0, all = 0, addn = 0, alln = 0;
+            if (nseen[n] != p) {
+                nseen[n] = p;
+                ++alln;
+                if (nstamp[n] != tid) ++addn;
+            }
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
                 const int e = inc[j] / 3;
-                if (seen[e] != p) {
-                    seen[e] = p;
-                    ++all;
-                    if (stamp[e] != tid) ++add;
+                if (seen[e] == p) continue;
+                seen[e] = p;
+                ++all;
+                if (stamp[e] != tid) ++add;
+                for (int c = 0; c < 3; ++c) {
+                    const int q = d->conn[3 * e + c];
+                    if (nseen[q] == p) continue;
+                    nseen[q] = p;
+                    ++alln;
+                    if (nstamp[q] != tid) ++addn;
                 }
             }
-            if (cur_nodes > 0 && (cur_nodes + 1 > tile_nodes || cur_slots + add > tile_slots)) {
+            if (cur_nodes > 0 &&
+                (cur_nodes + 1 > tile_nodes || cur_slots + add > tile_slots || cur_lds + addn > node_cap)) {
                 ++tid;
                 tile_start.push_back(p);
                 cur_nodes = 0;
                 cur_slots = 0;
+                cur_lds = 0;
                 add = all;
+                addn = alln;
             }
-            if (add > tile_slots)
+            if (add > tile_slots || addn > node_cap)
                 return fail(FW_E_INVALID, "node " + std::to_string(n) + " touches " + std::to_string(add) +
-                                              " elements, more than the tile slot capacity");
-            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) stamp[inc[j] / 3] = tid;
+                                              " elements, more than one tile holds");
+            nstamp[n] = tid;
+            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
+                const int e = inc[j] / 3;
+                stamp[e] = tid;
+                for (int c = 0; c < 3; ++c) nstamp[d->conn[3 * e + c]] = tid;
+            }
             cur_nodes += 1;
             cur_slots += add;
+            cur_lds += addn;
         }
     }
     const int ntiles = (int)tile_start.size();
@@ -186,10 +208,10 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
     T.elem_inv.assign(E, 0);
     for (int i = 0; i < E; ++i) T.elem_inv[T.elem_perm[i]] = i;
 
-    // per tile: halo list and CSR
+    // per tile: slot records, halo node list, node CSR
     T.tiles.resize(ntiles);
     std::fill(stamp.begin(), stamp.end(), -1);
-    std::vector<int32_t> slot_of(E, -1);
+    std::vector<int32_t> slot_of(E, -1), lid(N, -1), nst(N, -1);
     for (int t = 0; t < ntiles; ++t) {
         TileDev& td = T.tiles[t];
         std::memset(&td, 0, sizeof(td));
@@ -198,9 +220,10 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
         td.n_nodes = p1 - p0;
         td.elem0 = own_count[t];
         td.n_own = own_count[t + 1] - own_count[t];
-        td.halo_off = (int32_t)T.halo.size();
         td.multi = ntiles > 1;
-        for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) slot_of[T.elem_perm[i]] = i - td.elem0;
+        // slots: owned elements in storage order, then halo elements (ascending storage index)
+        std::vector<int32_t> slots;  // internal element index per slot
+        for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) slots.push_back(i);
         std::vector<int32_t> hl;
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
@@ -213,29 +236,60 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
             }
         }
         std::sort(hl.begin(), hl.end());
-        for (size_t h = 0; h < hl.size(); ++h) slot_of[T.elem_perm[hl[h]]] = td.n_own + (int)h;
+        slots.insert(slots.end(), hl.begin(), hl.end());
         td.n_halo = (int32_t)hl.size();
         T.halo_slots += td.n_halo;
-        T.halo.insert(T.halo.end(), hl.begin(), hl.end());
-        if (td.n_own + td.n_halo > tile_slots) return fail(FW_E_INVALID, "internal tiling error: slot overflow");
+        const int nslots = (int)slots.size();
+        if (nslots > tile_slots) return fail(FW_E_INVALID, "internal tiling error: slot overflow");
+        for (int s = 0; s < nslots; ++s) slot_of[T.elem_perm[slots[s]]] = s;
+        // local node ids: owned nodes first (p - p0), then halo nodes in ascending storage order
+        for (int p = p0; p < p1; ++p) lid[T.node_perm[p]] = p - p0;
+        std::vector<int32_t> hn;
+        for (int s = 0; s < nslots; ++s) {
+            const int e = T.elem_perm[slots[s]];
+            for (int c = 0; c < 3; ++c) {
+                const int n = d->conn[3 * e + c];
+                const int q = T.node_inv[n];
+                if ((q < p0 || q >= p1) && nst[n] != t) {
+                    nst[n] = t;
+                    hn.push_back(q);
+                }
+            }
+        }
+        std::sort(hn.begin(), hn.end());
+        for (size_t k = 0; k < hn.size(); ++k) lid[T.node_perm[hn[k]]] = td.n_nodes + (int)k;
+        if (td.n_nodes + (int)hn.size() > node_cap) return fail(FW_E_INVALID, "internal tiling error: node overflow");
+        td.hnode_off = (int32_t)T.hnode.size();
+        td.n_hnodes = (int32_t)hn.size();
+        T.hnode.insert(T.hnode.end(), hn.begin(), hn.end());
+        // slot records: local ids of the three corners, LDS destinations, storage element index
+        td.rec_off = (int32_t)(T.rec.size() / 4);
+        T.rec.resize(T.rec.size() + 4 * (size_t)nslots);
+        uint32_t* rec = T.rec.data() + 4 * (size_t)td.rec_off;
+        std::vector<uint16_t> dst(3 * (size_t)nslots, 0xFFFF);
         // node-major incidence positions: node i of the tile owns LDS entries
-        // [ptr[i], ptr[i+1]) in ascending (caller element id, corner); each slot
-        // records where its three corner forces go (0xFFFF: node not owned here)
+        // [ptr[i], ptr[i+1]) in ascending (caller element id, corner)
         td.ptr_off = (int32_t)T.csr_ptr.size();
-        td.dest_off = (int32_t)(T.dest.size() / 4);
-        T.dest.resize(T.dest.size() + 4 * (size_t)(td.n_own + td.n_halo), 0xFFFF);
-        uint16_t* dst = T.dest.data() + 4 * (size_t)td.dest_off;
         int cnt = 0;
         T.csr_ptr.push_back(0);
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
                 int e = inc[j] / 3, c = inc[j] % 3;
-                dst[4 * slot_of[e] + c] = (uint16_t)cnt;
+                dst[3 * (size_t)slot_of[e] + c] = (uint16_t)cnt;
                 ++cnt;
             }
             if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
             T.csr_ptr.push_back((uint16_t)cnt);
+        }
+        for (int s = 0; s < nslots; ++s) {
+            const int e = T.elem_perm[slots[s]];
+            const uint32_t l0 = (uint32_t)lid[d->conn[3 * e]], l1 = (uint32_t)lid[d->conn[3 * e + 1]],
+                           l2 = (uint32_t)lid[d->conn[3 * e + 2]];
+            rec[4 * s + 0] = l0 | (l1 << 16);
+            rec[4 * s + 1] = l2 | ((uint32_t)dst[3 * s] << 16);
+            rec[4 * s + 2] = (uint32_t)dst[3 * s + 1] | ((uint32_t)dst[3 * s + 2] << 16);
+            rec[4 * s + 3] = (uint32_t)slots[s];
         }
     }
     return FW_OK;
@@ -244,14 +298,14 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, Tiling&
 // ------------------------------------------------------------------ scene layout
 
 struct Layout {
-    int64_t pos0, pos1, vel, mass, flag, bcv, disp, force, cflag;
-    int64_t c0, c1, c2, jinv, area, j2a, j2b, stress, strain, yflag;
-    int64_t halo, cptr, dest, tiles;
+    int64_t pos0, pos1, rest, vel, mass, flag, bcv, disp, force, cflag;
+    int64_t jinv, area, j2a, j2b, stress, strain, yflag;
+    int64_t rec, hnode, cptr, tiles;
     int64_t gdata[FW_MAX_GROUNDS];
     int64_t total;
 };
 
-Layout make_layout(const fw_scene_desc* d, int64_t n_halo, int64_t n_ptr, int64_t n_ent, int64_t n_tiles) {
+Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hnode, int64_t n_ptr, int64_t n_tiles) {
     Layout L;
     int64_t o = 0;
     auto take = [&](int64_t bytes) {
@@ -262,6 +316,7 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_halo, int64_t n_ptr, int64_
     const int64_t N = d->n_nodes, E = d->n_elems;
     L.pos0 = take(8 * N);
     L.pos1 = take(8 * N);
+    L.rest = take(8 * N);
     L.vel = take(8 * N);
     L.mass = take(4 * N);
     L.flag = take(4 * N);
@@ -269,9 +324,6 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_halo, int64_t n_ptr, int64_
     L.disp = d->track_disp ? take(8 * N) : -1;
     L.force = take(8 * N);
     L.cflag = take(N);
-    L.c0 = take(4 * E);
-    L.c1 = take(4 * E);
-    L.c2 = take(4 * E);
     L.jinv = take(16 * E);
     L.area = take(4 * E);
     if (d->mat_type == FW_MAT_MISES) {
@@ -283,9 +335,9 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_halo, int64_t n_ptr, int64_
     L.stress = take(16 * E);
     L.strain = take(16 * E);
     L.yflag = take(E);
-    L.halo = take(4 * n_halo);
+    L.rec = take(16 * n_slots);
+    L.hnode = take(4 * n_hnode);
     L.cptr = take(2 * n_ptr);
-    L.dest = take(2 * n_ent);
     L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
     for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
         L.gdata[g] = -1;
@@ -373,9 +425,51 @@ struct fw_arena {
     int graph_n = -1;
     uint32_t graph_flags = 0;
     int graph_grid = 0;
+    // one-entry tiling cache: fw_scene_footprint followed by fw_scene_add (or a retry of
+    // an add that found the arena full) reuses the tiling instead of rebuilding it
+    struct {
+        const void* x0 = nullptr;
+        const void* conn = nullptr;
+        int32_t n_nodes = -1, n_elems = -1;
+        int64_t conn_sum = 0, x_sum = 0;
+        std::unique_ptr<Tiling> tiling;
+    } cache;
 };
 
 namespace {
+
+int node_cap_of(const fw_tuning& t) { return t.tile_slots; }
+
+// Tiling of a scene description, from the cache when the same arrays are presented again.
+int tiling_for(fw_arena* a, const fw_scene_desc* d, std::unique_ptr<Tiling>* out) {
+    int64_t cs = 0, xs = 0;
+    const int64_t E3 = 3 * (int64_t)d->n_elems;
+    for (int64_t i = 0; i < E3; i += std::max<int64_t>(1, E3 / 64)) cs = cs * 31 + d->conn[i];
+    for (int64_t i = 0; i < 2 * (int64_t)d->n_nodes; i += std::max<int64_t>(1, d->n_nodes / 32)) {
+        int32_t b;
+        std::memcpy(&b, d->x0 + i, 4);
+        xs = xs * 31 + b;
+    }
+    auto& c = a->cache;
+    if (c.tiling && c.x0 == d->x0 && c.conn == d->conn && c.n_nodes == d->n_nodes && c.n_elems == d->n_elems &&
+        c.conn_sum == cs && c.x_sum == xs) {
+        *out = std::move(c.tiling);
+        return FW_OK;
+    }
+    auto T = std::make_unique<Tiling>();
+    int rc = build_tiling(d, a->tune.tile_nodes, a->tune.tile_slots, node_cap_of(a->tune), *T);
+    if (rc) return rc;
+    c.x0 = d->x0;
+    c.conn = d->conn;
+    c.n_nodes = d->n_nodes;
+    c.n_elems = d->n_elems;
+    c.conn_sum = cs;
+    c.x_sum = xs;
+    *out = std::move(T);
+    return FW_OK;
+}
+
+void cache_keep(fw_arena* a, std::unique_ptr<Tiling> T) { a->cache.tiling = std::move(T); }
 
 int64_t arena_alloc(fw_arena* a, int64_t bytes) {
     for (auto it = a->free_list.begin(); it != a->free_list.end(); ++it) {
@@ -521,12 +615,18 @@ int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning*
     a->tune.block = 256;
     a->tune.grid_per_cu = 0;
     if (tuning) {
-        if (tuning->tile_slots != 1024 && tuning->tile_slots != 2048)
-            return fail(FW_E_INVALID, "tile_slots must be 1024 or 2048");
-        if (tuning->block != 256 && tuning->block != 512) return fail(FW_E_INVALID, "block must be 256 or 512");
-        if (tuning->block == 512 && tuning->tile_slots != 2048)
-            return fail(FW_E_INVALID, "block 512 needs tile_slots 2048");
-        if (tuning->tile_nodes < 1 || tuning->tile_nodes > 8192) return fail(FW_E_INVALID, "tile_nodes out of range");
+        // kernel shapes compiled in fw_kernels.hip: (threads, slots) -> owned nodes per thread
+        int npt = 0;
+        if (tuning->block == 512 && tuning->tile_slots == 1024) npt = 1;
+        if (tuning->block == 256 && tuning->tile_slots == 1024) npt = 2;
+        if (tuning->block == 256 && tuning->tile_slots == 512) npt = 1;
+        if (tuning->block == 512 && tuning->tile_slots == 2048) npt = 2;
+        if (tuning->block == 1024 && tuning->tile_slots == 2048) npt = 1;
+        if (!npt)
+            return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (512,1024) (256,1024) "
+                                      "(256,512) (512,2048) (1024,2048)");
+        if (tuning->tile_nodes < 1 || tuning->tile_nodes > npt * tuning->block)
+            return fail(FW_E_INVALID, "tile_nodes must be in [1, " + std::to_string(npt * tuning->block) + "]");
         a->tune = *tuning;
     }
     hipDeviceProp_t prop;
@@ -588,14 +688,21 @@ int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
     return FW_OK;
 }
 
-int32_t fw_scene_footprint(const fw_arena* a, const fw_scene_desc* d, int64_t* bytes) {
-    if (!a || !bytes) return fail(FW_E_INVALID, "null argument");
+static Layout layout_of(const fw_scene_desc* d, const Tiling& T) {
+    return make_layout(d, (int64_t)(T.rec.size() / 4), (int64_t)T.hnode.size(), (int64_t)T.csr_ptr.size(),
+                       (int64_t)T.tiles.size());
+}
+
+int32_t fw_scene_footprint(const fw_arena* ca, const fw_scene_desc* d, int64_t* bytes) {
+    if (!ca || !bytes) return fail(FW_E_INVALID, "null argument");
     int rc = validate(d);
     if (rc) return rc;
-    // upper bound without tiling: halo <= E, csr ptr <= 2N, tiles <= N
-    const int64_t N = d->n_nodes, E = d->n_elems;
-    Layout L = make_layout(d, E, 2 * N + 1, 3 * E, N);
-    *bytes = L.total;
+    fw_arena* a = const_cast<fw_arena*>(ca);  // only the tiling cache changes
+    std::unique_ptr<Tiling> T;
+    rc = tiling_for(a, d, &T);
+    if (rc) return rc;
+    *bytes = layout_of(d, *T).total;
+    cache_keep(a, std::move(T));
     return FW_OK;
 }
 
@@ -605,19 +712,32 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     if (rc) return rc;
     HIPCHK(hipSetDevice(a->device));
     auto sh = std::make_unique<SceneHost>();
-    rc = build_tiling(d, a->tune.tile_nodes, a->tune.tile_slots, sh->tiling);
-    if (rc) return rc;
+    {
+        std::unique_ptr<Tiling> Tp;
+        rc = tiling_for(a, d, &Tp);
+        if (rc) return rc;
+        const int64_t need = layout_of(d, *Tp).total;
+        int id_probe = !a->free_ids.empty() ? a->free_ids.back() : (int)a->scenes.size();
+        if (id_probe >= a->max_scenes) {
+            cache_keep(a, std::move(Tp));
+            return fail(FW_E_CAPACITY, "arena holds at most " + std::to_string(a->max_scenes) + " scenes");
+        }
+        if (need > a->capacity - a->used) {
+            cache_keep(a, std::move(Tp));
+            return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(need) + " bytes, " +
+                                           std::to_string(a->capacity - a->used) + " free");
+        }
+        sh->tiling = std::move(*Tp);
+    }
     Tiling& T = sh->tiling;
     const int N = d->n_nodes, E = d->n_elems;
     const int ntiles = (int)T.tiles.size();
-    Layout L = make_layout(d, (int64_t)T.halo.size(), (int64_t)T.csr_ptr.size(), (int64_t)T.dest.size(), ntiles);
+    Layout L = layout_of(d, T);
     int id;
     if (!a->free_ids.empty()) {
         id = a->free_ids.back();
     } else {
         id = (int)a->scenes.size();
-        if (id >= a->max_scenes)
-            return fail(FW_E_CAPACITY, "arena holds at most " + std::to_string(a->max_scenes) + " scenes");
     }
     int64_t off = arena_alloc(a, L.total);
     if (off < 0)
@@ -644,19 +764,11 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
         flag[i] = d->flags[o];
     }
     std::memcpy(img + L.pos1, img + L.pos0, 8 * (size_t)N);
+    std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
     std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
-    int32_t* c0 = (int32_t*)(img + L.c0);
-    int32_t* c1 = (int32_t*)(img + L.c1);
-    int32_t* c2 = (int32_t*)(img + L.c2);
-    for (int i = 0; i < E; ++i) {
-        int o = T.elem_perm[i];
-        c0[i] = T.node_inv[d->conn[3 * o]];
-        c1[i] = T.node_inv[d->conn[3 * o + 1]];
-        c2[i] = T.node_inv[d->conn[3 * o + 2]];
-    }
-    std::memcpy(img + L.halo, T.halo.data(), 4 * T.halo.size());
+    std::memcpy(img + L.rec, T.rec.data(), 4 * T.rec.size());
+    std::memcpy(img + L.hnode, T.hnode.data(), 4 * T.hnode.size());
     std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
-    std::memcpy(img + L.dest, T.dest.data(), 2 * T.dest.size());
     for (auto& td : T.tiles) td.scene = id;
     std::memcpy(img + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
     char* blk = a->base + off;
@@ -714,9 +826,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
     D.force = (float2*)(blk + L.force);
     D.cflag = (uint8_t*)(blk + L.cflag);
-    D.c0 = (const int32_t*)(blk + L.c0);
-    D.c1 = (const int32_t*)(blk + L.c1);
-    D.c2 = (const int32_t*)(blk + L.c2);
+    D.rest = (const float2*)(blk + L.rest);
     D.jinv = (float4*)(blk + L.jinv);
     D.area = (float*)(blk + L.area);
     D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
@@ -724,9 +834,9 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     D.stress = (float4*)(blk + L.stress);
     D.strain = (float4*)(blk + L.strain);
     D.yflag = (uint8_t*)(blk + L.yflag);
-    D.halo = (const int32_t*)(blk + L.halo);
+    D.rec = (const uint4*)(blk + L.rec);
+    D.hnode = (const int32_t*)(blk + L.hnode);
     D.csr_ptr = (const uint16_t*)(blk + L.cptr);
-    D.dest = (const uint2*)(blk + L.dest);
     D.n_nodes = N;
     D.n_elems = E;
     D.mat_type = d->mat_type;
@@ -774,11 +884,14 @@ int32_t fw_scene_remove(fw_arena* a, int32_t id) {
     if (!a) return fail(FW_E_INVALID, "null arena");
     SceneHost* s = get_scene(a, id);
     if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
-    // stream order keeps in-flight launches valid; the block is reused only after they finish
-    HIPCHK(hipStreamSynchronize(a->stream));
-    SceneDev dead = s->dev;
-    dead.live = 0;
-    HIPCHK(hipMemcpy(a->d_scenes + id, &dead, sizeof(SceneDev), hipMemcpyHostToDevice));
+    // No synchronisation: launches already in flight may still read this scene's
+    // table entry and arrays, but every later write to them -- the live=0 flag
+    // below, a reused block's upload, a reused id's entry -- is a copy on the
+    // same stream and so lands only after those launches have finished.
+    HIPCHK(hipSetDevice(a->device));
+    const int32_t dead = 0;
+    int rc = upload(a, &a->d_scenes[id].live, &dead, sizeof(int32_t));
+    if (rc) return rc;
     arena_free(a, s->block_off, s->block_bytes);
     a->scenes[id].reset();
     a->free_ids.push_back(id);
@@ -834,31 +947,49 @@ int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_overrid
 int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile_slots, int64_t* stats) {
     // Host-only: build the tiling a scene would get and verify it, without touching the GPU.
     //   every element owned by exactly one tile, owned ranges contiguous and disjoint;
-    //   slots (owned + halo) within capacity; every tile's node CSR lists exactly the
-    //   node's (element, corner) incidences in ascending caller element id.
+    //   slots (owned + halo) and LDS nodes (owned + halo) within capacity; each slot
+    //   record's local corner ids name the element's nodes; every tile's node CSR lists
+    //   exactly the node's (element, corner) incidences in ascending caller element id.
     int rc = validate(d);
     if (rc) return rc;
     Tiling T;
-    rc = build_tiling(d, tile_nodes, tile_slots, T);
+    const int node_cap = tile_slots;
+    rc = build_tiling(d, tile_nodes, tile_slots, node_cap, T);
     if (rc) return rc;
     const int N = d->n_nodes, E = d->n_elems;
     std::vector<int> owned(E, 0);
     int64_t max_slots = 0, entries = 0;
     for (const TileDev& td : T.tiles) {
-        if (td.n_own + td.n_halo > tile_slots || td.n_nodes > tile_nodes) return fail(FW_E_INVALID, "tile over capacity");
+        if (td.n_own + td.n_halo > tile_slots || td.n_nodes > tile_nodes || td.n_nodes + td.n_hnodes > node_cap)
+            return fail(FW_E_INVALID, "tile over capacity");
         max_slots = std::max<int64_t>(max_slots, td.n_own + td.n_halo);
         for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) owned[T.elem_perm[i]]++;
-        // invert the per-slot destinations: LDS position -> (caller element, corner)
+        // storage node id of each local id
+        std::vector<int32_t> local(td.n_nodes + td.n_hnodes);
+        for (int i = 0; i < td.n_nodes; ++i) local[i] = td.node0 + i;
+        for (int h = 0; h < td.n_hnodes; ++h) {
+            local[td.n_nodes + h] = T.hnode[td.hnode_off + h];
+            if (local[td.n_nodes + h] >= td.node0 && local[td.n_nodes + h] < td.node0 + td.n_nodes)
+                return fail(FW_E_INVALID, "halo node list holds an owned node");
+        }
+        // invert the slot records: LDS position -> (caller element, corner)
         const int nslots = td.n_own + td.n_halo;
         const uint16_t* cp = T.csr_ptr.data() + td.ptr_off;
         std::vector<int64_t> at(cp[td.n_nodes], -1);
         for (int sl = 0; sl < nslots; ++sl) {
-            const int eint = sl < td.n_own ? td.elem0 + sl : T.halo[td.halo_off + sl - td.n_own];
+            const uint32_t* r = T.rec.data() + 4 * ((size_t)td.rec_off + sl);
+            const int eint = (int)r[3];
+            if (sl < td.n_own && eint != td.elem0 + sl) return fail(FW_E_INVALID, "owned slot out of order");
+            const int ecaller = T.elem_perm[eint];
+            const uint32_t lids[3] = {r[0] & 0xFFFFu, r[0] >> 16, r[1] & 0xFFFFu};
+            const uint32_t dsts[3] = {r[1] >> 16, r[2] & 0xFFFFu, r[2] >> 16};
             for (int c = 0; c < 3; ++c) {
-                const uint16_t pos = T.dest[4 * ((size_t)td.dest_off + sl) + c];
+                if (lids[c] >= local.size() || T.node_perm[local[lids[c]]] != d->conn[3 * ecaller + c])
+                    return fail(FW_E_INVALID, "slot record names the wrong node");
+                const uint32_t pos = dsts[c];
                 if (pos == 0xFFFF) continue;
                 if (pos >= at.size() || at[pos] != -1) return fail(FW_E_INVALID, "slot destinations collide");
-                at[pos] = 3 * (int64_t)T.elem_perm[eint] + c;
+                at[pos] = 3 * (int64_t)ecaller + c;
             }
         }
         for (int i = 0; i < td.n_nodes; ++i) {

@@ -1,0 +1,310 @@
+// fw_device.h -- device helpers of the substep kernels: material laws, ground contact
+// laws, tile scheduling and block reductions.  Included by fw_kernels.hip only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fw_types.h"
+
+namespace fw {
+
+constexpr uint32_t BC_U = 1u, BC_V = 2u, BC_A = 4u, BC_F = 8u, BC_G = 16u;
+constexpr uint32_t NODE_BOUNDARY = 0x8000u;
+constexpr int NODE_BCV_SHIFT = 16;
+constexpr uint32_t STEP_DIAG = 1u, STEP_NO_CONTACT = 2u;
+
+__device__ __forceinline__ int mod3(int k) {
+    int r = k % 3;
+    return r < 0 ? r + 3 : r;
+}
+
+// ------------------------------------------------------------------ materials
+
+// Natural log of a float, evaluated with IEEE double operations only and rounded
+// to float once: x = m * 2^e with m in [sqrt(1/2), sqrt(2)), then
+// log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.172, summed to s^13
+// (truncation < 2e-12 relative).  Every operation is correctly rounded and none
+// is contracted, so the CPU oracle's copy of this routine gives the same bits.
+__device__ __forceinline__ float fw_logf(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (x == __builtin_inff()) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) {  // subnormal: scale into the normal range (exact)
+        x = x * 8388608.0f;
+        e = -23;
+    }
+    uint32_t b = __float_as_uint(x);
+    e += (int)((b >> 23) & 0xFFu) - 127;
+    float m = __uint_as_float((b & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const double f = (double)m - 1.0;
+    const double s = f / (2.0 + f);
+    const double s2 = s * s;
+    double p = 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    p = p * s2 + 1.0;
+    const double r = (double)e * 0.69314718055994530942 + (2.0 * s) * p;
+    return (float)r;
+}
+
+struct Mat {
+    int type;
+    float lam, mu, sy, H;
+};
+
+// flatworld/materials/materialfunctions.py:10-29 (Elastic / Neo-Hookean PK2).
+__device__ __forceinline__ void stress_hyper(const Mat& m, const float F[4], float s[3], float g[3]) {
+    float C0 = F[0] * F[0] + F[2] * F[2];
+    float C1 = F[0] * F[1] + F[2] * F[3];
+    float C2 = F[1] * F[0] + F[3] * F[2];
+    float C3 = F[1] * F[1] + F[3] * F[3];
+    float e00 = 0.5f * (C0 - 1.0f);
+    float e01 = 0.5f * (C1 - 0.0f);
+    float e11 = 0.5f * (C3 - 1.0f);
+    g[0] = e00;
+    g[1] = e11;
+    g[2] = e01;
+    if (m.type == 1) {
+        float ltr = m.lam * (e00 + e11);
+        float tmu = 2.0f * m.mu;
+        s[0] = ltr * 1.0f + tmu * e00;
+        s[1] = ltr * 1.0f + tmu * e11;
+        s[2] = ltr * 0.0f + tmu * e01;
+    } else {
+        float inv_det = 1.0f / (C0 * C3 - C1 * C2);
+        float Ci0 = C3 * inv_det;
+        float Ci1 = -C1 * inv_det;
+        float Ci3 = C0 * inv_det;
+        float J = F[0] * F[3] - F[1] * F[2];
+        float c = m.lam * fw_logf(J) - m.mu;
+        s[0] = c * Ci0 + m.mu * 1.0f;
+        s[1] = c * Ci3 + m.mu * 1.0f;
+        s[2] = c * Ci1 + m.mu * 0.0f;
+    }
+}
+
+// flatworld/materials/materialfunctions.py:35-101 (radial return, taken verbatim
+// including the plane-strain mean stress p = (s_xx + s_yy) / 3).
+__device__ __forceinline__ bool stress_j2(const Mat& m, const float F[4], float4 st, float s[3], float g[3],
+                                          float4& st_new) {
+    float C0 = F[0] * F[0] + F[2] * F[2];
+    float C1 = F[0] * F[1] + F[2] * F[3];
+    float C3 = F[1] * F[1] + F[3] * F[3];
+    float et0 = 0.5f * (C0 - 1.0f);
+    float et1 = 0.5f * (C3 - 1.0f);
+    float et2 = 0.5f * (C1 - 0.0f);
+    g[0] = et0;
+    g[1] = et1;
+    g[2] = et2;
+    float ee0 = et0 - st.x;
+    float ee1 = et1 - st.y;
+    float ee2 = et2 - st.z;
+    float tr = ee0 + ee1;
+    float tmu = 2.0f * m.mu;
+    float s0 = m.lam * tr + tmu * ee0;
+    float s1 = m.lam * tr + tmu * ee1;
+    float s2 = tmu * ee2;
+    float p = (s0 + s1) / 3.0f;
+    float d0 = s0 - p;
+    float d1 = s1 - p;
+    float d2 = s2;
+    float vm = sqrtf(d0 * d0 + d1 * d1 - d0 * d1 + 3.0f * d2 * d2);
+    float f_trial = vm - (m.sy + m.H * st.w);
+    st_new = st;
+    s[0] = s0;
+    s[1] = s1;
+    s[2] = s2;
+    if (f_trial > 0.0f) {
+        float denom = 3.0f * m.mu + m.H;
+        float dg = f_trial / denom;
+        st_new.w = st.w + dg;
+        float scale = 1.0f - 3.0f * m.mu * dg / vm;
+        s[0] = d0 * scale + p;
+        s[1] = d1 * scale + p;
+        s[2] = d2 * scale;
+        st_new.x = st.x + dg * (d0 / vm);
+        st_new.y = st.y + dg * (d1 / vm);
+        st_new.z = st.z + dg * (d2 / vm);
+        return true;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------------ grounds
+
+// flatworld/mixedcontact.py:166-217 (_nearest_on_curve_2d); returns the signed distance.
+__device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z, float& n0o, float& n1o) {
+    const float* hgt = (const float*)g.data;
+    const int nx = g.nx;
+    const float lbx = g.a[0], ubx = g.a[1];
+    float span = ubx - lbx;
+    float u = 0.0f;
+    if (span > 1e-12f) u = (x - lbx) / span;
+    u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+    float sg = u * (float)(nx - 1);
+    int i0 = (int)floorf(sg);
+    int i1 = i0 + 1;
+    if (i1 >= nx) i1 = nx - 1;
+    if (i0 < 0) i0 = 0;
+    float t = sg - (float)i0;
+    float h0 = hgt[i0], h1 = hgt[i1];
+    float h = h0 + (h1 - h0) * t;
+    int ir = (int)roundf(sg);
+    if (ir < 0) ir = 0;
+    if (ir >= nx) ir = nx - 1;
+    int j0 = ir - 1, j1 = ir + 1;
+    if (j0 < 0) j0 = 0;
+    if (j1 >= nx) j1 = nx - 1;
+    float dxw = span / (float)(nx - 1);
+    float denom = dxw * (float)(j1 - j0);
+    float dhdx = 0.0f;
+    if (denom > 1e-6f) dhdx = (hgt[j1] - hgt[j0]) / denom;
+    float n0 = -dhdx, n1 = 1.0f;
+    if (g.reverse == 1) {
+        n0 = -n0;
+        n1 = -n1;
+    }
+    float nlen = sqrtf(n0 * n0 + n1 * n1);
+    if (nlen > 1e-12f) {
+        n0 = n0 / nlen;
+        n1 = n1 / nlen;
+    }
+    n0o = n0;
+    n1o = n1;
+    return (x - x) * n0 + (z - h) * n1;
+}
+
+// flatworld/mixedcontact.py:220-302 (_voxel_signed_distance_2d, interior cells only).
+__device__ __forceinline__ float voxel_signed(const GroundDev& g, float px, float py, float& n0o, float& n1o) {
+    const int8_t* occ = (const int8_t*)g.data;
+    const int nx = g.nx, ny = g.ny;
+    const float lb0 = g.a[0], lb1 = g.a[1], dx = g.a[2], dz = g.a[3];
+    float best = 1e9f;
+    n0o = 0.0f;
+    n1o = 1.0f;
+    int i = (int)floorf((px - lb0) / dx);
+    int j = (int)floorf((py - lb1) / dz);
+    if (0 <= i && i < nx && 0 <= j && j < ny && occ[(int64_t)i * ny + j] == 1) {
+        float dnx = 1e9f, dpx = 1e9f, dny = 1e9f, dpy = 1e9f;
+        int idx = i - 1;
+        while (idx >= 0 && occ[(int64_t)idx * ny + j] != 0) --idx;
+        dnx = px - (lb0 + (float)(idx + 1) * dx);
+        idx = i + 1;
+        while (idx < nx && occ[(int64_t)idx * ny + j] != 0) ++idx;
+        dpx = (lb0 + (float)idx * dx) - px;
+        idx = j - 1;
+        while (idx >= 0 && occ[(int64_t)i * ny + idx] != 0) --idx;
+        dny = py - (lb1 + (float)(idx + 1) * dz);
+        idx = j + 1;
+        while (idx < ny && occ[(int64_t)i * ny + idx] != 0) ++idx;
+        dpy = (lb1 + (float)idx * dz) - py;
+        float mv = dnx;
+        int face = 0;
+        if (dpx < mv) { mv = dpx; face = 1; }
+        if (dny < mv) { mv = dny; face = 2; }
+        if (dpy < mv) { mv = dpy; face = 3; }
+        if (face == 0) { best = -dnx; n0o = -1.0f; n1o = 0.0f; }
+        else if (face == 1) { best = -dpx; n0o = 1.0f; n1o = 0.0f; }
+        else if (face == 2) { best = -dny; n0o = 0.0f; n1o = -1.0f; }
+        else { best = -dpy; n0o = 0.0f; n1o = 1.0f; }
+    }
+    return best;
+}
+
+// One ground on one boundary node; adds into (fx, fy) only when it pushes.
+// Plane flatworld/mixedcontact.py:660-684, height field 1566-1602, voxel 1605-1643.
+__device__ __forceinline__ bool ground_force(const GroundDev& g, float px, float py, float vx, float vy, float dt,
+                                             float& fx, float& fy) {
+    const float k = g.penalty;
+    if (g.kind == 1) {
+        float pen = (px - g.a[0]) * g.a[2] + (py - g.a[1]) * g.a[3];
+        if (pen < 0.0f) {
+            fx = fx + (-((g.a[2] * k) * pen));
+            fy = fy + (-((g.a[3] * k) * pen));
+            return true;
+        }
+        return false;
+    }
+    float n0, n1, sd;
+    if (g.kind == 2) {
+        sd = hf_signed(g, px, py, n0, n1);
+        if (!(sd < 0.0f)) return false;
+        float c0 = ((-n0) * k) * sd;
+        float c1 = ((-n1) * k) * sd;
+        float vn = vx * n0 + vy * n1;
+        if (vn < 0.0f) {
+            c0 = c0 + (((((-n0) * 0.5f) * k) * vn) * dt);
+            c1 = c1 + (((((-n1) * 0.5f) * k) * vn) * dt);
+        }
+        fx = fx + c0;
+        fy = fy + c1;
+        return true;
+    }
+    sd = voxel_signed(g, px, py, n0, n1);
+    if (!(sd < 0.0f)) return false;
+    float pen = -sd;
+    float c0 = (n0 * k) * pen;
+    float c1 = (n1 * k) * pen;
+    float vn = vx * n0 + vy * n1;
+    if (vn < 0.0f) {
+        c0 = c0 + (((((-n0) * 0.5f) * k) * vn) * dt);
+        c1 = c1 + (((((-n1) * 0.5f) * k) * vn) * dt);
+    }
+    fx = fx + c0;
+    fy = fy + c1;
+    return true;
+}
+
+// ------------------------------------------------------------------ tile loop helpers
+
+__device__ __forceinline__ int tile_of_block(int round, int ntiles_total_grid) {
+    // XCD-aware mapping: hardware places block b on XCD b % 8; give each XCD a
+    // contiguous run of tiles so halo elements and nodes shared by neighbouring
+    // tiles are reused from that XCD's L2.
+    const int G = gridDim.x;
+    const int b = blockIdx.x;
+    int v = b;
+    if ((G & 7) == 0) v = (b & 7) * (G >> 3) + (b >> 3);
+    return v + round * G;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
+    // v: lb.x, lb.y (min), ub.x, ub.y (max) as ordered ints
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        int32_t a0 = __shfl_xor(v[0], off, 64);
+        int32_t a1 = __shfl_xor(v[1], off, 64);
+        int32_t a2 = __shfl_xor(v[2], off, 64);
+        int32_t a3 = __shfl_xor(v[3], off, 64);
+        v[0] = min(v[0], a0);
+        v[1] = min(v[1], a1);
+        v[2] = max(v[2], a2);
+        v[3] = max(v[3], a3);
+    }
+    if (lane == 0) {
+        red[wid * 4 + 0] = v[0];
+        red[wid * 4 + 1] = v[1];
+        red[wid * 4 + 2] = v[2];
+        red[wid * 4 + 3] = v[3];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            v[0] = min(v[0], red[w * 4 + 0]);
+            v[1] = min(v[1], red[w * 4 + 1]);
+            v[2] = max(v[2], red[w * 4 + 2]);
+            v[3] = max(v[3], red[w * 4 + 3]);
+        }
+    }
+}
+
+}  // namespace fw

@@ -1,19 +1,24 @@
 // fw_kernels.hip -- gfx950 kernels of the explicit-FEM substep.
 //
 // One fused kernel per substep.  A workgroup takes one tile: a contiguous range
-// of owned nodes of one scene plus every element touching them (owned elements
-// are a contiguous range, halo elements come from a short list).
-//   phase A  one element slot per lane: F = Jac(x) J^-1, PK2 stress
-//            (Elastic / Neo-Hookean / J2 return map), corner forces
-//            area * B^T S -> LDS.  J2 state is read from one buffer and the
-//            owner tile writes the other (ping-pong), so halo tiles never see a
-//            half-updated state.
-//   phase B  one owned node per lane: ground penalty forces (plane, height
-//            field, voxel grid) behind the AABB activation gate, then the
-//            element forces gathered from LDS in ascending element id through
-//            the tile's node CSR -- the order in which Warp's CPU backend
-//            applies its atomic adds -- then BCs and symplectic Euler, then the
-//            boundary AABB for the next substep's gate.
+// of owned nodes of one scene plus every element touching them.
+//   stage    every independent load of the tile goes out at once: the owned
+//            nodes' positions and rest positions (coalesced) and those of the
+//            tile's few halo nodes, one 16-byte record per element slot
+//            (local corner ids, LDS destinations, element id), J2 state, and
+//            the node data phase B needs.  Positions land in LDS.
+//   phase A  element slots: rest frame (J^-1, area) recomputed from the rest
+//            positions -- bit-identical to storing it, 16 B less per element --
+//            F = Jac(x) J^-1, PK2 stress (Elastic / Neo-Hookean / J2 return
+//            map), corner forces area * B^T S written straight to the LDS
+//            entry of the owning node's incidence list.  The owner tile writes
+//            the J2 state to the other buffer (ping-pong), so halo tiles never
+//            see a half-updated state.
+//   phase B  owned nodes: ground penalty forces (plane, height field, voxel)
+//            behind the AABB activation gate, then the element forces from a
+//            contiguous LDS run in ascending caller element id -- the order of
+//            Warp-CPU's serial atomic adds -- then BCs and symplectic Euler,
+//            then the boundary AABB for the next substep's gate.
 // No float atomics: every nodal sum has a fixed order, so a scene's result does
 // not depend on what else is resident in the arena.  Positions are
 // double-buffered because neighbouring tiles read them; velocities, mass and
@@ -26,283 +31,40 @@
 // rounding.  log(J) is log((double)J) rounded to float.
 #include <hip/hip_runtime.h>
 
+#include "fw_device.h"
 #include "fw_types.h"
 
 namespace fw {
 
-constexpr uint32_t BC_U = 1u, BC_V = 2u, BC_A = 4u, BC_F = 8u, BC_G = 16u;
-constexpr uint32_t NODE_BOUNDARY = 0x8000u;
-constexpr int NODE_BCV_SHIFT = 16;
-constexpr uint32_t STEP_DIAG = 1u, STEP_NO_CONTACT = 2u;
-
-__device__ __forceinline__ int mod3(int k) {
-    int r = k % 3;
-    return r < 0 ? r + 3 : r;
+// Inverse rest Jacobian and area (flatworld/femspringmanager.py:59-68, 113-133):
+// Jac = [[x1-x3, x2-x3], [y1-y3, y2-y3]], area = |det| / 2, J^-1 = adj(Jac) / det.
+__device__ __forceinline__ void rest_frame(float2 X1, float2 X2, float2 X3, float4& ji, float& area) {
+    const float J0 = X1.x - X3.x, J1 = X2.x - X3.x, J2 = X1.y - X3.y, J3 = X2.y - X3.y;
+    const float det = J0 * J3 - J1 * J2;
+    area = fabsf(det) * 0.5f;
+    const float inv_det = 1.0f / det;
+    ji = make_float4(J3 * inv_det, -J1 * inv_det, -J2 * inv_det, J0 * inv_det);
 }
 
-// ------------------------------------------------------------------ materials
-
-struct Mat {
-    int type;
-    float lam, mu, sy, H;
-};
-
-// flatworld/materials/materialfunctions.py:10-29 (Elastic / Neo-Hookean PK2).
-__device__ __forceinline__ void stress_hyper(const Mat& m, const float F[4], float s[3], float g[3]) {
-    float C0 = F[0] * F[0] + F[2] * F[2];
-    float C1 = F[0] * F[1] + F[2] * F[3];
-    float C2 = F[1] * F[0] + F[3] * F[2];
-    float C3 = F[1] * F[1] + F[3] * F[3];
-    float e00 = 0.5f * (C0 - 1.0f);
-    float e01 = 0.5f * (C1 - 0.0f);
-    float e11 = 0.5f * (C3 - 1.0f);
-    g[0] = e00;
-    g[1] = e11;
-    g[2] = e01;
-    if (m.type == 1) {
-        float ltr = m.lam * (e00 + e11);
-        float tmu = 2.0f * m.mu;
-        s[0] = ltr * 1.0f + tmu * e00;
-        s[1] = ltr * 1.0f + tmu * e11;
-        s[2] = ltr * 0.0f + tmu * e01;
-    } else {
-        float inv_det = 1.0f / (C0 * C3 - C1 * C2);
-        float Ci0 = C3 * inv_det;
-        float Ci1 = -C1 * inv_det;
-        float Ci3 = C0 * inv_det;
-        float J = F[0] * F[3] - F[1] * F[2];
-        float c = m.lam * (float)log((double)J) - m.mu;
-        s[0] = c * Ci0 + m.mu * 1.0f;
-        s[1] = c * Ci3 + m.mu * 1.0f;
-        s[2] = c * Ci1 + m.mu * 0.0f;
-    }
-}
-
-// flatworld/materials/materialfunctions.py:35-101 (radial return, taken verbatim
-// including the plane-strain mean stress p = (s_xx + s_yy) / 3).
-__device__ __forceinline__ bool stress_j2(const Mat& m, const float F[4], float4 st, float s[3], float g[3],
-                                          float4& st_new) {
-    float C0 = F[0] * F[0] + F[2] * F[2];
-    float C1 = F[0] * F[1] + F[2] * F[3];
-    float C3 = F[1] * F[1] + F[3] * F[3];
-    float et0 = 0.5f * (C0 - 1.0f);
-    float et1 = 0.5f * (C3 - 1.0f);
-    float et2 = 0.5f * (C1 - 0.0f);
-    g[0] = et0;
-    g[1] = et1;
-    g[2] = et2;
-    float ee0 = et0 - st.x;
-    float ee1 = et1 - st.y;
-    float ee2 = et2 - st.z;
-    float tr = ee0 + ee1;
-    float tmu = 2.0f * m.mu;
-    float s0 = m.lam * tr + tmu * ee0;
-    float s1 = m.lam * tr + tmu * ee1;
-    float s2 = tmu * ee2;
-    float p = (s0 + s1) / 3.0f;
-    float d0 = s0 - p;
-    float d1 = s1 - p;
-    float d2 = s2;
-    float vm = sqrtf(d0 * d0 + d1 * d1 - d0 * d1 + 3.0f * d2 * d2);
-    float f_trial = vm - (m.sy + m.H * st.w);
-    st_new = st;
-    s[0] = s0;
-    s[1] = s1;
-    s[2] = s2;
-    if (f_trial > 0.0f) {
-        float denom = 3.0f * m.mu + m.H;
-        float dg = f_trial / denom;
-        st_new.w = st.w + dg;
-        float scale = 1.0f - 3.0f * m.mu * dg / vm;
-        s[0] = d0 * scale + p;
-        s[1] = d1 * scale + p;
-        s[2] = d2 * scale;
-        st_new.x = st.x + dg * (d0 / vm);
-        st_new.y = st.y + dg * (d1 / vm);
-        st_new.z = st.z + dg * (d2 / vm);
-        return true;
-    }
-    return false;
-}
-
-// ------------------------------------------------------------------ grounds
-
-// flatworld/mixedcontact.py:166-217 (_nearest_on_curve_2d); returns the signed distance.
-__device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z, float& n0o, float& n1o) {
-    const float* hgt = (const float*)g.data;
-    const int nx = g.nx;
-    const float lbx = g.a[0], ubx = g.a[1];
-    float span = ubx - lbx;
-    float u = 0.0f;
-    if (span > 1e-12f) u = (x - lbx) / span;
-    u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
-    float sg = u * (float)(nx - 1);
-    int i0 = (int)floorf(sg);
-    int i1 = i0 + 1;
-    if (i1 >= nx) i1 = nx - 1;
-    if (i0 < 0) i0 = 0;
-    float t = sg - (float)i0;
-    float h0 = hgt[i0], h1 = hgt[i1];
-    float h = h0 + (h1 - h0) * t;
-    int ir = (int)roundf(sg);
-    if (ir < 0) ir = 0;
-    if (ir >= nx) ir = nx - 1;
-    int j0 = ir - 1, j1 = ir + 1;
-    if (j0 < 0) j0 = 0;
-    if (j1 >= nx) j1 = nx - 1;
-    float dxw = span / (float)(nx - 1);
-    float denom = dxw * (float)(j1 - j0);
-    float dhdx = 0.0f;
-    if (denom > 1e-6f) dhdx = (hgt[j1] - hgt[j0]) / denom;
-    float n0 = -dhdx, n1 = 1.0f;
-    if (g.reverse == 1) {
-        n0 = -n0;
-        n1 = -n1;
-    }
-    float nlen = sqrtf(n0 * n0 + n1 * n1);
-    if (nlen > 1e-12f) {
-        n0 = n0 / nlen;
-        n1 = n1 / nlen;
-    }
-    n0o = n0;
-    n1o = n1;
-    return (x - x) * n0 + (z - h) * n1;
-}
-
-// flatworld/mixedcontact.py:220-302 (_voxel_signed_distance_2d, interior cells only).
-__device__ __forceinline__ float voxel_signed(const GroundDev& g, float px, float py, float& n0o, float& n1o) {
-    const int8_t* occ = (const int8_t*)g.data;
-    const int nx = g.nx, ny = g.ny;
-    const float lb0 = g.a[0], lb1 = g.a[1], dx = g.a[2], dz = g.a[3];
-    float best = 1e9f;
-    n0o = 0.0f;
-    n1o = 1.0f;
-    int i = (int)floorf((px - lb0) / dx);
-    int j = (int)floorf((py - lb1) / dz);
-    if (0 <= i && i < nx && 0 <= j && j < ny && occ[(int64_t)i * ny + j] == 1) {
-        float dnx = 1e9f, dpx = 1e9f, dny = 1e9f, dpy = 1e9f;
-        int idx = i - 1;
-        while (idx >= 0 && occ[(int64_t)idx * ny + j] != 0) --idx;
-        dnx = px - (lb0 + (float)(idx + 1) * dx);
-        idx = i + 1;
-        while (idx < nx && occ[(int64_t)idx * ny + j] != 0) ++idx;
-        dpx = (lb0 + (float)idx * dx) - px;
-        idx = j - 1;
-        while (idx >= 0 && occ[(int64_t)i * ny + idx] != 0) --idx;
-        dny = py - (lb1 + (float)(idx + 1) * dz);
-        idx = j + 1;
-        while (idx < ny && occ[(int64_t)i * ny + idx] != 0) ++idx;
-        dpy = (lb1 + (float)idx * dz) - py;
-        float mv = dnx;
-        int face = 0;
-        if (dpx < mv) { mv = dpx; face = 1; }
-        if (dny < mv) { mv = dny; face = 2; }
-        if (dpy < mv) { mv = dpy; face = 3; }
-        if (face == 0) { best = -dnx; n0o = -1.0f; n1o = 0.0f; }
-        else if (face == 1) { best = -dpx; n0o = 1.0f; n1o = 0.0f; }
-        else if (face == 2) { best = -dny; n0o = 0.0f; n1o = -1.0f; }
-        else { best = -dpy; n0o = 0.0f; n1o = 1.0f; }
-    }
-    return best;
-}
-
-// One ground on one boundary node; adds into (fx, fy) only when it pushes.
-// Plane flatworld/mixedcontact.py:660-684, height field 1566-1602, voxel 1605-1643.
-__device__ __forceinline__ bool ground_force(const GroundDev& g, float px, float py, float vx, float vy, float dt,
-                                             float& fx, float& fy) {
-    const float k = g.penalty;
-    if (g.kind == 1) {
-        float pen = (px - g.a[0]) * g.a[2] + (py - g.a[1]) * g.a[3];
-        if (pen < 0.0f) {
-            fx = fx + (-((g.a[2] * k) * pen));
-            fy = fy + (-((g.a[3] * k) * pen));
-            return true;
-        }
-        return false;
-    }
-    float n0, n1, sd;
-    if (g.kind == 2) {
-        sd = hf_signed(g, px, py, n0, n1);
-        if (!(sd < 0.0f)) return false;
-        float c0 = ((-n0) * k) * sd;
-        float c1 = ((-n1) * k) * sd;
-        float vn = vx * n0 + vy * n1;
-        if (vn < 0.0f) {
-            c0 = c0 + (((((-n0) * 0.5f) * k) * vn) * dt);
-            c1 = c1 + (((((-n1) * 0.5f) * k) * vn) * dt);
-        }
-        fx = fx + c0;
-        fy = fy + c1;
-        return true;
-    }
-    sd = voxel_signed(g, px, py, n0, n1);
-    if (!(sd < 0.0f)) return false;
-    float pen = -sd;
-    float c0 = (n0 * k) * pen;
-    float c1 = (n1 * k) * pen;
-    float vn = vx * n0 + vy * n1;
-    if (vn < 0.0f) {
-        c0 = c0 + (((((-n0) * 0.5f) * k) * vn) * dt);
-        c1 = c1 + (((((-n1) * 0.5f) * k) * vn) * dt);
-    }
-    fx = fx + c0;
-    fy = fy + c1;
-    return true;
-}
-
-// ------------------------------------------------------------------ tile loop helpers
-
-__device__ __forceinline__ int tile_of_block(int round, int ntiles_total_grid) {
-    // XCD-aware mapping: hardware places block b on XCD b % 8; give each XCD a
-    // contiguous run of tiles so halo elements and nodes shared by neighbouring
-    // tiles are reused from that XCD's L2.
-    const int G = gridDim.x;
-    const int b = blockIdx.x;
-    int v = b;
-    if ((G & 7) == 0) v = (b & 7) * (G >> 3) + (b >> 3);
-    return v + round * G;
-}
-
-template <int BLOCK>
-__device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
-    // v: lb.x, lb.y (min), ub.x, ub.y (max) as ordered ints
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int32_t a0 = __shfl_xor(v[0], off, 64);
-        int32_t a1 = __shfl_xor(v[1], off, 64);
-        int32_t a2 = __shfl_xor(v[2], off, 64);
-        int32_t a3 = __shfl_xor(v[3], off, 64);
-        v[0] = min(v[0], a0);
-        v[1] = min(v[1], a1);
-        v[2] = max(v[2], a2);
-        v[3] = max(v[3], a3);
-    }
-    if (lane == 0) {
-        red[wid * 4 + 0] = v[0];
-        red[wid * 4 + 1] = v[1];
-        red[wid * 4 + 2] = v[2];
-        red[wid * 4 + 3] = v[3];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < BLOCK / 64; ++w) {
-            v[0] = min(v[0], red[w * 4 + 0]);
-            v[1] = min(v[1], red[w * 4 + 1]);
-            v[2] = max(v[2], red[w * 4 + 2]);
-            v[3] = max(v[3], red[w * 4 + 3]);
-        }
-    }
+__device__ __forceinline__ void unpack_rec(uint4 r, uint32_t l[3], uint32_t q[3]) {
+    l[0] = r.x & 0xFFFFu;
+    l[1] = r.x >> 16;
+    l[2] = r.y & 0xFFFFu;
+    q[0] = r.y >> 16;
+    q[1] = r.z & 0xFFFFu;
+    q[2] = r.z >> 16;
 }
 
 // One element slot: F = Jac(x) J^-1, stress, corner forces area * B^T S
-// (flatworld/femspringmanager.py:71-105, 240-276), stored at the LDS entries
-// of the owning tile's node-major incidence lists.
-__device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, float2 p3, float4 ji, float ar, float4 st,
-                                           uint2 dd, bool own, int e, bool diag, float4* __restrict__ j2_wr,
-                                           float4* __restrict__ stress, float4* __restrict__ strain,
-                                           uint8_t* __restrict__ yflag, float2* lds_f) {
+// (flatworld/femspringmanager.py:71-105, 240-276), stored at the LDS entries of
+// the owning tile's node-major incidence lists.
+__device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, float2 p3, float2 X1, float2 X2,
+                                           float2 X3, float4 st, const uint32_t q[3], bool own, int e, bool diag,
+                                           float4* __restrict__ j2_wr, float4* __restrict__ stress,
+                                           float4* __restrict__ strain, uint8_t* __restrict__ yflag, float2* lds_f) {
+    float4 ji;
+    float ar;
+    rest_frame(X1, X2, X3, ji, ar);
     const float J0 = p1.x - p3.x, J1 = p2.x - p3.x, J2 = p1.y - p3.y, J3 = p2.y - p3.y;
     float F[4];
     F[0] = J0 * ji.x + J1 * ji.z;
@@ -327,10 +89,9 @@ __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, f
     const float f3 = ((d10 * F[2]) * s[0] + (d11 * F[3]) * s[1] + (d10 * F[3] + d11 * F[2]) * s[2]) * ar;
     const float f4 = ((d20 * F[0]) * s[0] + (d21 * F[1]) * s[1] + (d21 * F[0] + d20 * F[1]) * s[2]) * ar;
     const float f5 = ((d20 * F[2]) * s[0] + (d21 * F[3]) * s[1] + (d20 * F[3] + d21 * F[2]) * s[2]) * ar;
-    const uint32_t q0 = dd.x & 0xFFFFu, q1 = dd.x >> 16, q2 = dd.y & 0xFFFFu;
-    if (q0 != 0xFFFFu) lds_f[q0] = make_float2(f0, f1);
-    if (q1 != 0xFFFFu) lds_f[q1] = make_float2(f2, f3);
-    if (q2 != 0xFFFFu) lds_f[q2] = make_float2(f4, f5);
+    if (q[0] != 0xFFFFu) lds_f[q[0]] = make_float2(f0, f1);
+    if (q[1] != 0xFFFFu) lds_f[q[1]] = make_float2(f2, f3);
+    if (q[2] != 0xFFFFu) lds_f[q[2]] = make_float2(f4, f5);
     if (diag && own) {
         // PK2 -> Cauchy F S F^T / max(J, 1e-12) (flatworld/femspringmanager.py:267-272)
         const float FS0 = F[0] * s[0] + F[1] * s[2];
@@ -362,13 +123,28 @@ struct StepArgs {
     uint32_t flags;
 };
 
-template <int BLOCK, int SLOTS>
+// Node data phase B needs, loaded before the tile's barrier.
+struct NodeIn {
+    float2 v;
+    float2 u;
+    float m;
+    uint32_t fl;
+    int jb, je;
+};
+
+// BLOCK threads; at most SLOTS element slots, NODES LDS nodes (owned + halo) and
+// NPT * BLOCK owned nodes per tile.
+template <int BLOCK, int SLOTS, int NODES, int NPT>
 __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
+    constexpr int SPT = (SLOTS + BLOCK - 1) / BLOCK;
+    __shared__ float2 lds_x[NODES];
+    __shared__ float2 lds_X[NODES];
     __shared__ float2 lds_f[SLOTS * 3];
     __shared__ int32_t lds_red[(BLOCK / 64) * 4];
 
     const int G = A.ctl->clock + A.k;
     const int ntiles = A.tile_count >= 0 ? A.tile_count : (A.ctl->n_tiles - A.tile_base);
+    const int tid = threadIdx.x;
 
     for (int round = 0;; ++round) {
         const int ti = tile_of_block(round, ntiles);
@@ -387,6 +163,11 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         const int wr = local & 1;
         const float2* __restrict__ pos_rd = S.pos[rd];
         float2* __restrict__ pos_wr = S.pos[wr];
+        const float2* __restrict__ rest = S.rest;
+        const uint4* __restrict__ rec = S.rec + T.rec_off;
+        const int32_t* __restrict__ hnode = S.hnode + T.hnode_off;
+        const uint16_t* __restrict__ cptr = S.csr_ptr + T.ptr_off;
+        const int nslots = T.n_own + T.n_halo;
 
         Mat m;
         m.type = S.mat_type;
@@ -394,35 +175,56 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         m.mu = S.E / (2.0f * (1.0f + S.nu));
         m.sy = S.sigma_y;
         m.H = S.H;
-
-        // ---------------- phase A: element slots, two per lane per iteration with
-        // every independent load of both issued before the dependent gathers
-        const int nslots = T.n_own + T.n_halo;
-        const uint2* __restrict__ dest = S.dest + T.dest_off;
         const bool j2 = m.type == 3;
-        for (int slA = threadIdx.x; slA < nslots; slA += 2 * BLOCK) {
-            const int slB = slA + BLOCK;
-            const bool vB = slB < nslots;
-            const bool ownA = slA < T.n_own, ownB = slB < T.n_own;
-            const int eA = ownA ? T.elem0 + slA : S.halo[T.halo_off + slA - T.n_own];
-            const int eB = !vB ? eA : (ownB ? T.elem0 + slB : S.halo[T.halo_off + slB - T.n_own]);
-            const int a1 = S.c0[eA], a2 = S.c1[eA], a3 = S.c2[eA];
-            const int b1 = S.c0[eB], b2 = S.c1[eB], b3 = S.c2[eB];
-            const float4 jiA = S.jinv[eA], jiB = S.jinv[eB];
-            const float arA = S.area[eA], arB = S.area[eB];
-            const uint2 ddA = dest[slA];
-            const uint2 ddB = vB ? dest[slB] : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
-            float4 stA = make_float4(0.0f, 0.0f, 0.0f, 0.0f), stB = stA;
-            if (j2) {
-                stA = S.j2[rd][eA];
-                stB = S.j2[rd][eB];
+
+        // ---------------- stage: issue every independent load of the tile
+        uint4 rc[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int sl = tid + k * BLOCK;
+            rc[k] = sl < nslots ? rec[sl] : make_uint4(0u, 0u, 0u, 0u);
+        }
+        NodeIn nd[NPT];
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * BLOCK;
+            if (i < T.n_nodes) {
+                const int n = T.node0 + i;
+                nd[k].v = S.vel[n];
+                nd[k].u = S.disp ? S.disp[n] : make_float2(0.0f, 0.0f);
+                nd[k].m = S.mass[n];
+                nd[k].fl = S.flag[n];
+                nd[k].jb = cptr[i];
+                nd[k].je = cptr[i + 1];
             }
-            const float2 pA1 = pos_rd[a1], pA2 = pos_rd[a2], pA3 = pos_rd[a3];
-            const float2 pB1 = pos_rd[b1], pB2 = pos_rd[b2], pB3 = pos_rd[b3];
-            slot_force(m, pA1, pA2, pA3, jiA, arA, stA, ddA, ownA, eA, diag, S.j2[wr], S.stress, S.strain, S.yflag, lds_f);
-            if (vB)
-                slot_force(m, pB1, pB2, pB3, jiB, arB, stB, ddB, ownB, eB, diag, S.j2[wr], S.stress, S.strain, S.yflag,
-                           lds_f);
+        }
+        for (int i = tid; i < T.n_nodes; i += BLOCK) {
+            lds_x[i] = pos_rd[T.node0 + i];
+            lds_X[i] = rest[T.node0 + i];
+        }
+        for (int h = tid; h < T.n_hnodes; h += BLOCK) {
+            const int g = hnode[h];
+            lds_x[T.n_nodes + h] = pos_rd[g];
+            lds_X[T.n_nodes + h] = rest[g];
+        }
+        float4 st[SPT];
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            st[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (j2 && tid + k * BLOCK < nslots) st[k] = S.j2[rd][rc[k].w];
+        }
+        __syncthreads();
+
+        // ---------------- phase A: element slots from LDS
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) {
+            const int sl = tid + k * BLOCK;
+            if (sl < nslots) {
+                uint32_t l[3], q[3];
+                unpack_rec(rc[k], l, q);
+                slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st[k], q,
+                           sl < T.n_own, (int)rc[k].w, diag, S.j2[wr], S.stress, S.strain, S.yflag, lds_f);
+            }
         }
 
         // ---------------- activation gate (flatworld/explicitloop.py:477-487, bvh.py:39-46, 321-342, 393-416)
@@ -450,14 +252,15 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 
         // ---------------- phase B: owned nodes
         int32_t bb[4] = {fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)};
-        const uint16_t* __restrict__ cptr = S.csr_ptr + T.ptr_off;
-        for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * BLOCK;
+            if (i >= T.n_nodes) continue;
             const int n = T.node0 + i;
-            const float2 p = pos_rd[n];
-            float2 v = S.vel[n];
-            const uint32_t fl = S.flag[n];
-            const float mass = S.mass[n];
-            const int jb = cptr[i], je = cptr[i + 1];
+            const float2 p = lds_x[i];
+            const float2 v = nd[k].v;
+            const uint32_t fl = nd[k].fl;
+            const float mass = nd[k].m;
             float fx = 0.0f, fy = 0.0f;
             bool hit = false;
             if (active && (fl & NODE_BOUNDARY)) {
@@ -465,7 +268,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                     if (active & (1u << gi)) hit |= ground_force(S.ground[gi], p.x, p.y, v.x, v.y, dt, fx, fy);
             }
             // element forces in ascending element id: a contiguous run of LDS entries
-            for (int j = jb; j < je; ++j) {
+            for (int j = nd[k].jb; j < nd[k].je; ++j) {
                 const float2 f = lds_f[j];
                 fx = fx + (-f.x);
                 fy = fy + (-f.y);
@@ -499,10 +302,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             }
             S.vel[n] = make_float2(vx, vy);
             const float dux = vx * dt, duy = vy * dt;
-            if (S.disp) {
-                const float2 u = S.disp[n];
-                S.disp[n] = make_float2(u.x + dux, u.y + duy);
-            }
+            if (S.disp) S.disp[n] = make_float2(nd[k].u.x + dux, nd[k].u.y + duy);
             const float nx_ = p.x + dux, ny_ = p.y + duy;
             pos_wr[n] = make_float2(nx_, ny_);
             if (diag) {
@@ -520,7 +320,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 
         // ---------------- boundary AABB at x_{n+1} (flatworld/femspringmanager.py:361-408)
         block_minmax<BLOCK>(bb, lds_red);
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             int32_t* aw = const_cast<int32_t*>(S.aabb[mod3(local + 1)]);
             int32_t* az = const_cast<int32_t*>(S.aabb[mod3(local + 2)]);
             if (T.multi) {
@@ -547,34 +347,36 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 
 // flatworld/femspringmanager.py:113-138: area = |det Jac| / 2, J^-1 = inverse(Jac),
 // lumped mass = sum over incident elements (ascending id) of area * rho / 3.
-template <int BLOCK, int SLOTS>
+template <int BLOCK, int SLOTS, int NODES>
 __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, const TileDev* tiles, int tile_base,
                                                         int tile_count) {
+    __shared__ float2 lds_X[NODES];
     __shared__ float lds_m[SLOTS * 3];
     const int ti = blockIdx.x;
     if (ti >= tile_count) return;
     const TileDev T = tiles[tile_base + ti];
     const SceneDev& S = scenes[T.scene];
-    const float2* __restrict__ x0 = S.pos[0];
+    const uint4* rec = S.rec + T.rec_off;
+    const int32_t* hnode = S.hnode + T.hnode_off;
+    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = S.rest[T.node0 + i];
+    for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = S.rest[hnode[h]];
+    __syncthreads();
     const int nslots = T.n_own + T.n_halo;
     for (int sl = threadIdx.x; sl < nslots; sl += BLOCK) {
-        const bool own = sl < T.n_own;
-        const int e = own ? T.elem0 + sl : S.halo[T.halo_off + sl - T.n_own];
-        const float2 p1 = x0[S.c0[e]], p2 = x0[S.c1[e]], p3 = x0[S.c2[e]];
-        const float J0 = p1.x - p3.x, J1 = p2.x - p3.x, J2 = p1.y - p3.y, J3 = p2.y - p3.y;
-        const float det = J0 * J3 - J1 * J2;
-        const float vol = fabsf(det) * 0.5f;
-        if (own) {
-            const float inv_det = 1.0f / det;
-            S.area[e] = vol;
-            S.jinv[e] = make_float4(J3 * inv_det, -J1 * inv_det, -J2 * inv_det, J0 * inv_det);
+        const uint4 r = rec[sl];
+        uint32_t l[3], q[3];
+        unpack_rec(r, l, q);
+        float4 ji;
+        float vol;
+        rest_frame(lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], ji, vol);
+        if (sl < T.n_own) {
+            S.area[r.w] = vol;
+            S.jinv[r.w] = ji;
         }
         const float nd = vol * S.rho / 3.0f;
-        const uint2 dd = S.dest[T.dest_off + sl];
-        const uint32_t q0 = dd.x & 0xFFFFu, q1 = dd.x >> 16, q2 = dd.y & 0xFFFFu;
-        if (q0 != 0xFFFFu) lds_m[q0] = nd;
-        if (q1 != 0xFFFFu) lds_m[q1] = nd;
-        if (q2 != 0xFFFFu) lds_m[q2] = nd;
+        if (q[0] != 0xFFFFu) lds_m[q[0]] = nd;
+        if (q[1] != 0xFFFFu) lds_m[q[1]] = nd;
+        if (q[2] != 0xFFFFu) lds_m[q[2]] = nd;
     }
     __syncthreads();
     const uint16_t* cptr = S.csr_ptr + T.ptr_off;
@@ -600,16 +402,22 @@ __global__ void advance_clock_kernel(ArenaCtl* ctl, int n) {
 
 // ------------------------------------------------------------------ host launchers
 
+// Kernel shapes: (threads, element slots, LDS nodes, owned nodes per thread).
+// The host keeps tile_nodes <= NPT * threads and node_cap == slots.
 extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
                                         int tile_base, int tile_count, float dt_override, float damping_override,
                                         uint32_t flags, int grid, int block, int slots, hipStream_t stream) {
     fw::StepArgs A{scenes, tiles, ctl, k, tile_base, tile_count, dt_override, damping_override, flags};
-    if (block == 256 && slots == 1024)
-        hipLaunchKernelGGL((fw::substep_kernel<256, 1024>), dim3(grid), dim3(256), 0, stream, A);
-    else if (block == 256 && slots == 2048)
-        hipLaunchKernelGGL((fw::substep_kernel<256, 2048>), dim3(grid), dim3(256), 0, stream, A);
+    if (block == 512 && slots == 1024)
+        hipLaunchKernelGGL((fw::substep_kernel<512, 1024, 1024, 1>), dim3(grid), dim3(512), 0, stream, A);
+    else if (block == 256 && slots == 1024)
+        hipLaunchKernelGGL((fw::substep_kernel<256, 1024, 1024, 2>), dim3(grid), dim3(256), 0, stream, A);
+    else if (block == 256 && slots == 512)
+        hipLaunchKernelGGL((fw::substep_kernel<256, 512, 512, 1>), dim3(grid), dim3(256), 0, stream, A);
     else if (block == 512 && slots == 2048)
-        hipLaunchKernelGGL((fw::substep_kernel<512, 2048>), dim3(grid), dim3(512), 0, stream, A);
+        hipLaunchKernelGGL((fw::substep_kernel<512, 2048, 2048, 2>), dim3(grid), dim3(512), 0, stream, A);
+    else if (block == 1024 && slots == 2048)
+        hipLaunchKernelGGL((fw::substep_kernel<1024, 2048, 2048, 1>), dim3(grid), dim3(1024), 0, stream, A);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -618,12 +426,15 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
                                         int slots, hipStream_t stream) {
     if (tile_count <= 0) return hipSuccess;
-    if (slots == 1024)
-        hipLaunchKernelGGL((fw::prepare_kernel<256, 1024>), dim3(tile_count), dim3(256), 0, stream, scenes, tiles,
+    if (slots == 512)
+        hipLaunchKernelGGL((fw::prepare_kernel<256, 512, 512>), dim3(tile_count), dim3(256), 0, stream, scenes, tiles,
                            tile_base, tile_count);
+    else if (slots == 1024)
+        hipLaunchKernelGGL((fw::prepare_kernel<256, 1024, 1024>), dim3(tile_count), dim3(256), 0, stream, scenes,
+                           tiles, tile_base, tile_count);
     else if (slots == 2048)
-        hipLaunchKernelGGL((fw::prepare_kernel<256, 2048>), dim3(tile_count), dim3(256), 0, stream, scenes, tiles,
-                           tile_base, tile_count);
+        hipLaunchKernelGGL((fw::prepare_kernel<256, 2048, 2048>), dim3(tile_count), dim3(256), 0, stream, scenes,
+                           tiles, tile_base, tile_count);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

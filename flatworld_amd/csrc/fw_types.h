@@ -2,12 +2,14 @@
 //
 // HBM layout per scene (all SoA, internal node/element order, 256-byte aligned blocks):
 //   nodes    pos[2] float2 (ping-pong: substep k reads pos[(k+1)&1], writes pos[k&1]),
-//            vel float2, mass float, flag uint32, disp float2 (optional), force float2 (diag),
-//            cflag uint8 (diag)
-//   elements c0/c1/c2 int32 (internal node ids), jinv float4, area float,
-//            j2[2] float4 (eps_xx, eps_yy, eps_xy, eqps; ping-pong, J2 scenes only),
-//            stress float4 / strain float4 (diag), yflag uint8 (diag)
-//   tiles    halo int32 list, csr_ptr uint16 per owned node (+1), dest uint2 per element slot
+//            rest float2 (reference positions), vel float2, mass float, flag uint32,
+//            disp float2 (optional), force float2 (diag), cflag uint8 (diag)
+//   elements jinv float4 + area float (written once, for readback only: the substep
+//            recomputes both from the rest positions), j2[2] float4 (eps_xx, eps_yy,
+//            eps_xy, eqps; ping-pong, J2 scenes only), stress / strain float4 (diag),
+//            yflag uint8 (diag)
+//   tiles    rec uint4 per element slot, hnode int32 per halo node, csr_ptr uint16 per
+//            owned node (+1)
 //   ground   height samples float / occupancy int8
 #pragma once
 #include <hip/hip_runtime.h>
@@ -28,6 +30,7 @@ struct GroundDev {
 struct SceneDev {
     // node arrays
     float2* pos[2];
+    const float2* rest;
     float2* vel;
     float* mass;
     const uint32_t* flag;
@@ -36,9 +39,6 @@ struct SceneDev {
     float2* force;  // diag
     uint8_t* cflag; // diag
     // element arrays
-    const int32_t* c0;
-    const int32_t* c1;
-    const int32_t* c2;
     float4* jinv;
     float* area;
     float4* j2[2];
@@ -46,9 +46,12 @@ struct SceneDev {
     float4* strain;  // diag
     uint8_t* yflag;  // diag
     // tile data
-    const int32_t* halo;
+    // rec per element slot: .x = l0 | l1 << 16, .y = l2 | d0 << 16, .z = d1 | d2 << 16, .w = element
+    //   l* local node ids of the corners (owned nodes first, then the tile's halo nodes),
+    //   d* LDS entry receiving that corner's force (0xFFFF: node not owned by the tile)
+    const uint4* rec;
+    const int32_t* hnode;     // per tile: storage ids of the halo nodes
     const uint16_t* csr_ptr;  // per tile: n_nodes + 1 offsets into the tile's node-major LDS entries
-    const uint2* dest;        // per slot: LDS entry of corners 0..2 (uint16 each, 0xFFFF = not owned)
     // scalars
     int32_t n_nodes, n_elems;
     int32_t mat_type, n_grounds;
@@ -64,13 +67,14 @@ struct SceneDev {
 
 struct TileDev {
     int32_t scene;
-    int32_t node0, n_nodes;   // owned nodes [node0, node0 + n_nodes)
-    int32_t elem0, n_own;     // owned elements [elem0, elem0 + n_own)
-    int32_t halo_off, n_halo; // halo elements halo[halo_off .. +n_halo)
-    int32_t ptr_off;          // csr_ptr[ptr_off .. ptr_off + n_nodes]
-    int32_t dest_off;         // dest[dest_off + slot]
-    int32_t multi;            // scene has more than one tile (AABB via atomics)
-    int32_t pad[2];
+    int32_t node0, n_nodes;    // owned nodes [node0, node0 + n_nodes)
+    int32_t elem0, n_own;      // owned elements [elem0, elem0 + n_own)
+    int32_t n_halo;            // halo element slots after the owned ones
+    int32_t rec_off;           // rec[rec_off + slot]
+    int32_t hnode_off, n_hnodes;  // hnode[hnode_off .. +n_hnodes)
+    int32_t ptr_off;           // csr_ptr[ptr_off .. ptr_off + n_nodes]
+    int32_t multi;             // scene has more than one tile (AABB via atomics)
+    int32_t pad;
 };
 
 // Arena control block in device memory.

@@ -1,0 +1,151 @@
+"""Served form: a request loop that continuously batches simulation jobs.
+
+The reference's ExplicitLoop drives one fixed set of domains from construction
+to exit (flatworld/explicitloop.py:71-589); nothing can join or leave.  Here a
+job -- one FEM body, its ground, its material and a horizon in substeps --
+arrives at any time and is:
+
+1. queued in arrival order;
+2. admitted at the next frame boundary once its HBM footprint fits in the
+   arena (head-of-line jobs wait; nothing is evicted);
+3. advanced together with every other resident job, one fused launch per
+   substep, frames replayed as a hipGraph (its zero-dt warm-up is simply its
+   first substep in the arena);
+4. retired when it reaches its horizon: its final state is copied out and its
+   memory is released (stream-ordered, without stalling the running frame).
+
+Per-job latency is arrival -> result copied to the host.  A job's result is
+bit-identical whether it runs alone or among others, because no device sum
+depends on the resident set.  A job whose state turns non-finite (for
+example a Neo-Hookean element inverting: log of a negative J) is reported as
+failed at retirement without touching the other jobs.
+"""
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .batch import FemBatch
+
+
+@dataclass
+class Job:
+    job_id: int
+    spec: object  # SceneSpec with horizon > 0
+    arrival: float = 0.0  # seconds after the server clock starts
+
+
+@dataclass
+class JobResult:
+    job_id: int
+    arrival: float
+    admitted: float
+    finished: float
+    horizon: int
+    n_elems: int
+    ok: bool
+    positions: np.ndarray = field(repr=False, default=None)
+    velocities: np.ndarray = field(repr=False, default=None)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+class SimServer:
+    def __init__(self, batch=None, frame=60, keep_results=True, clock=time.perf_counter, **batch_kw):
+        self.batch = batch or FemBatch(frame=frame, **batch_kw)
+        self.frame = int(frame)
+        self.keep_results = keep_results
+        self.clock = clock
+        self.reset_stats()
+
+    def reset_stats(self):
+        self.results = []
+        self.frames = 0
+        self.busy_time = 0.0
+        self.elem_substeps = 0
+
+    def run(self, jobs, max_wall=None):
+        """Serve ``jobs`` (arrival times relative to the start of this call); returns the results."""
+        pending = collections.deque(sorted(jobs, key=lambda j: j.arrival))
+        waiting = collections.deque()
+        resident = {}  # scene id -> (job, admitted time)
+        t0 = self.clock()
+        b = self.batch
+        for job in pending:
+            if job.spec.horizon <= 0:
+                raise ValueError(f"job {job.job_id}: served jobs need a positive horizon")
+        while pending or waiting or resident:
+            now = self.clock() - t0
+            if max_wall is not None and now > max_wall:
+                break
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # admission by footprint, arrival order
+            free = b.free_bytes()
+            while waiting:
+                job = waiting[0]
+                need = b.footprint(job.spec)
+                if need > free:
+                    if not resident:
+                        raise RuntimeError(f"job {job.job_id} needs {need} bytes; the empty arena has {free}")
+                    break
+                try:
+                    sid = b.add(job.spec)
+                except RuntimeError:
+                    break  # fragmented: wait for retirements
+                waiting.popleft()
+                resident[sid] = (job, self.clock() - t0)
+                free = b.free_bytes()
+            if not resident:
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - (self.clock() - t0), 0.01)))
+                continue
+            # one frame for everyone resident
+            fs = self.clock()
+            b.step(self.frame, graph=True)
+            b.sync()
+            self.busy_time += self.clock() - fs
+            self.frames += 1
+            # retire jobs that reached their horizon
+            done_now = self.clock() - t0
+            for sid in list(resident.keys()):
+                steps, fin = b.progress(sid)
+                if not fin:
+                    continue
+                job, adm = resident.pop(sid)
+                x = b.positions(sid)
+                v = b.velocities(sid) if self.keep_results else None
+                ok = bool(np.isfinite(x).all())
+                b.remove(sid)
+                self.elem_substeps += job.spec.n_elems * job.spec.horizon
+                self.results.append(
+                    JobResult(job.job_id, job.arrival, adm, self.clock() - t0, job.spec.horizon, job.spec.n_elems, ok,
+                              x if self.keep_results else None, v)
+                )
+            del done_now
+        self.wall = self.clock() - t0
+        return self.results
+
+    def stats(self):
+        lat = np.array([r.latency for r in self.results]) if self.results else np.zeros(1)
+        return {
+            "jobs": len(self.results),
+            "failed": sum(1 for r in self.results if not r.ok),
+            "wall_s": self.wall,
+            "frames": self.frames,
+            "gpu_busy_frac": self.busy_time / max(self.wall, 1e-9),
+            "elem_substeps_per_s": self.elem_substeps / max(self.wall, 1e-9),
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_mean_s": float(lat.mean()),
+        }
+
+
+def poisson_arrivals(n, rate, rng):
+    """n arrival times of a Poisson process with ``rate`` jobs per second."""
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

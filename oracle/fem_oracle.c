@@ -38,7 +38,43 @@ static void mat_ftf(const float F[4], float C[4]) {
     C[3] = F[1] * F[1] + F[3] * F[3];
 }
 
-static float log_f32(float x) { return (float)log((double)x); }
+/* Natural log of a float from IEEE double operations only, rounded to float
+ * once.  The reference evaluates wp.log in float32 (warp/native/builtin.h,
+ * logf); any faithful float log fits the restated algorithm, and this one is
+ * chosen because the HIP kernel evaluates the identical operation sequence,
+ * which makes the Neo-Hookean stress reproducible bit for bit:
+ *   x = m * 2^e, m in [sqrt(1/2), sqrt(2)),  log m = 2 atanh(s),
+ *   s = (m - 1) / (m + 1) = f / (2 + f),  atanh(s)/s = sum s^2k / (2k + 1), k <= 6.
+ * |s| <= 0.1716, so the dropped tail is below 2e-12 relative. */
+static float log_f32(float x)
+{
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (isinf(x)) return x;
+    int e = 0;
+    if (x < 1.17549435e-38f) { x = x * 8388608.0f; e = -23; }
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    e += (int)((b >> 23) & 0xFFu) - 127;
+    uint32_t mb = (b & 0x007FFFFFu) | 0x3F800000u;
+    float m;
+    memcpy(&m, &mb, 4);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    const double f = (double)m - 1.0;
+    const double s = f / (2.0 + f);
+    const double s2 = s * s;
+    double p = 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    p = p * s2 + 1.0;
+    const double r = (double)e * 0.69314718055994530942 + (2.0 * s) * p;
+    return (float)r;
+}
+
+/* Exported for the pinning tests of the log above. */
+float ora_log_f32(float x) { return log_f32(x); }
 
 /* float -> int32 with the device conversion's semantics: NaN -> 0, saturating
  * at the int32 range.  Identical to a C cast for finite in-range values; the

@@ -10,7 +10,8 @@
  * Warp executes a kernel launch on its CPU device, so float accumulations
  * (Fext atomics, lumped mass) happen in the reference's order.  All arithmetic
  * is IEEE float32 without contraction (built with -ffp-contract=off);
- * log(J) is evaluated as log((double)J) rounded to float.
+ * log(J) is a float log built from double operations (ora_log_f32), faithful
+ * to within one float ulp and pinned against numpy's double log.
  */
 #ifndef FW_FEM_ORACLE_H
 #define FW_FEM_ORACLE_H
@@ -93,6 +94,7 @@ void ora_element_force(const ora_scene* s, int32_t e, float f[6], float cauchy[3
 int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[2], float dt, float force[2]);
 int32_t ora_gate(const float fem_lb[2], const float fem_ub[2], const ora_ground* g, int32_t counter);
 void ora_update_bbox(ora_scene* s);
+float ora_log_f32(float x);
 
 #ifdef __cplusplus
 }

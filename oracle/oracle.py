@@ -108,6 +108,8 @@ def lib():
         L.ora_gate.argtypes = [_f32p, _f32p, ctypes.POINTER(OraGround), ctypes.c_int32]
         L.ora_gate.restype = ctypes.c_int32
         L.ora_update_bbox.argtypes = [ctypes.POINTER(OraScene)]
+        L.ora_log_f32.argtypes = [ctypes.c_float]
+        L.ora_log_f32.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -215,6 +217,11 @@ def stress2d(mat_type, E, nu, F):
     e = np.zeros(3, np.float32)
     lib().ora_stress2d(int(mat_type), float(np.float32(E)), float(np.float32(nu)), _p(F, _f32p), _p(s, _f32p), _p(e, _f32p))
     return s, e
+
+
+def log_f32(x):
+    """The oracle's float32 natural log (the one the Neo-Hookean stress uses)."""
+    return np.float32(lib().ora_log_f32(float(np.float32(x))))
 
 
 def mises2d(E, nu, sy, H, F, eps_p_old, eqps_old):

@@ -1,10 +1,12 @@
 """HIP substep vs the CPU oracle on identical seeded inputs.
 
-Elastic and J2 scenes must agree bit for bit (same float32 operations, same
-order, no contraction); Neo-Hookean scenes within the stated tolerance, since
-the device and host log(double) may differ in the last double ulp:
-nodal forces within 1e-5 of the largest nodal force and positions after 100
-substeps within 1e-4 of the minimum edge length.
+Every material must agree bit for bit: both sides evaluate the same float32
+operations in the same order without contraction, and the Neo-Hookean log is
+one shared double-operation routine (correctly rounded in practice, pinned
+against numpy in the oracle tests).  The Neo-Hookean case also reports the
+tolerance figures (nodal forces within 1e-5 of the largest nodal force,
+positions within 1e-4 of the minimum edge length) for a reader comparing a
+build whose log differs.
 """
 
 import numpy as np
@@ -39,14 +41,23 @@ def test_small_block_bitexact():
     run_both(spec, [1, 99, 500], True, "small 20x20 elastic, plane")
 
 
-def test_neohookean_multitile_tolerance():
+def test_neohookean_multitile_bitexact():
     dom = W.grid_domain(48, (0.0, 0.0), 1.0, fw.HyperElastic(1e5, 0.3, 1000.0), [fw.Gravity([0.0, -9.8])], jitter=0.1, seed=3)
     spec = W.one_body_spec(dom, fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0]))
-    g, o, res = run_both(spec, [1, 99], False, "nh 48x48 jittered, plane", tile_nodes=256)
+    g, o, _ = run_both(spec, [1, 99, 200], True, "nh 48x48 jittered, plane", tile_nodes=256)
+    res = compare(g, o, False, "nh 48x48 tolerance figures")
     log(f"  {res}")
-    assert res[0]["force_rel"] < FORCE_RTOL
-    assert res[1]["force_rel"] < FORCE_RTOL
-    assert res[1]["pos_err_edges"] < POS_TOL_EDGE
+    assert res["force_rel"] < FORCE_RTOL
+    assert res["pos_err_edges"] < POS_TOL_EDGE
+
+
+def test_neohookean_heightfield_bitexact():
+    rng = W.rng_for(17)
+    dom = W.grid_domain(28, (0.3, 0.1), 0.4, fw.HyperElastic(4e4, 0.35, 200.0), [fw.Gravity([0.0, -9.8])],
+                        initials=[fw.InitialVel("ALL", [0.3, -0.5])])
+    spec = W.one_body_spec(dom, W.sinusoid_heightfield(rng))
+    _, o, _ = run_both(spec, [1, 300, 300], True, "nh 28x28 on height field", tile_nodes=200, tile_slots=512)
+    assert np.count_nonzero(o.contact_flag) > 0, "scenario should touch the height field"
 
 
 def test_elastic_multitile_bitexact():
@@ -55,17 +66,18 @@ def test_elastic_multitile_bitexact():
     run_both(spec, [1, 150], True, "elastic 40x40 multi-tile, tilted plane", tile_nodes=200)
 
 
-def _j2_voxel_spec():
+def _j2_voxel_spec(side=32):
     rng = W.rng_for(7)
     mat = fw.MisesPlastic(1e5, 0.3, 300.0, 500.0, 1e3)
-    dom = W.grid_domain(32, (0.25, 0.56), 0.5, mat, [fw.Gravity([0.0, -9.8])], initials=[fw.InitialVel("ALL", [0.0, -1.0])])
+    dom = W.grid_domain(side, (0.25, 0.56), 0.5, mat, [fw.Gravity([0.0, -9.8])], initials=[fw.InitialVel("ALL", [0.0, -1.0])])
     ground = W.value_noise_voxels(rng, 64, 64, [0.0, 0.0], [1.0, 0.55], floor_rows=2)
     return W.one_body_spec(dom, ground)
 
 
 def test_j2_voxel_single_tile_bitexact():
-    g, o, _ = run_both(_j2_voxel_spec(), [1, 100, 100, 100, 300], True, "j2 32x32 on voxels, one tile", tile_nodes=2048,
-                       tile_slots=2048)
+    # 30x30 cells: 961 nodes and 1800 elements fit one 1024-thread, 2048-slot tile
+    g, o, _ = run_both(_j2_voxel_spec(30), [1, 100, 100, 100, 300], True, "j2 30x30 on voxels, one tile", tile_nodes=1024,
+                       tile_slots=2048, block=1024)
     assert o.eqps.max() > 0.0, "scenario should yield"
 
 

@@ -8,7 +8,10 @@ wrong sign or index, transposed operand, wrong accumulation order) fails it:
                  test2D/test_2Dmaterial_functions.py:85-105;
 * J2          -- the return-map invariants (below yield == elastic, consistency
                  vm(sigma) = sigma_y + H * eqps after return, radial return,
-                 pressure kept, flag flips exactly at f_trial = 0);
+                 flag flips exactly at f_trial = 0);
+* log         -- the float log inside the Neo-Hookean stress against numpy's
+                 double log (within one ulp, correctly rounded almost always)
+                 and its special values;
 * assembly    -- internal forces from an independent formulation
                  f_a = A * (F S) grad N_a in float64, translation and rotation
                  invariance, zero force under rigid motion;
@@ -64,6 +67,40 @@ def test_material_reference_assertions():
     s, _ = O.stress2d(2, E, nu, np.eye(2))
     assert abs(s[0]) < 10.0 and abs(s[1]) < 10.0
     assert O.stress2d(2, E, nu, [[1.1, 0.0], [0.0, 1.0]])[0][0] != 0.0
+
+
+def test_log_f32_is_faithful():
+    """The oracle's float log against numpy's float64 log: within 1 ulp everywhere,
+    correctly rounded for all but a vanishing fraction, over normal and subnormal
+    inputs, both sides of the sqrt(2) mantissa split and the neighbourhood of 1."""
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([
+        rng.uniform(0.0, 1.0, 1500),
+        np.exp(rng.uniform(-80.0, 80.0, 1500)),
+        1.0 + rng.uniform(-1e-3, 1e-3, 500),
+        np.float32(math.sqrt(2.0)) * (1.0 + rng.uniform(-1e-6, 1e-6, 200)),
+        2.0 ** np.arange(-149, 128, dtype=np.float64),
+        [1e-45, 1.1754942e-38, 1.1754944e-38, 3.4028235e38],
+    ]).astype(np.float32)
+    xs = xs[xs > 0]
+    exact = 0
+    for x in xs:
+        got = O.log_f32(x)
+        ref = np.log(np.float64(x))
+        want = np.float32(ref)
+        ulp = np.spacing(np.abs(want)) if want != 0 else np.float32(1e-45)
+        assert abs(np.float64(got) - ref) <= 1.0 * ulp, (x, got, ref)
+        exact += int(got == want)
+    assert exact >= 0.999 * len(xs)
+
+
+def test_log_f32_special_values():
+    assert O.log_f32(1.0) == 0.0
+    assert O.log_f32(2.0) == np.float32(math.log(2.0))
+    assert O.log_f32(0.0) == -np.inf
+    assert O.log_f32(np.inf) == np.inf
+    assert np.isnan(O.log_f32(-1.0))
+    assert np.isnan(O.log_f32(np.nan))
 
 
 def test_material_rest_state_is_stress_free():

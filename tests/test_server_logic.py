@@ -1,0 +1,110 @@
+"""Scheduling logic of SimServer on the CPU, against a simulated batch.
+
+The fake batch has the FemBatch surface (footprint / free_bytes / add / remove /
+step / sync / progress / positions / velocities) but only counts substeps, so
+admission order, footprint waits, horizon retirement and failure isolation are
+checked without a GPU.
+"""
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from flatworld_amd.server import Job, SimServer, poisson_arrivals
+
+
+class FakeBatch:
+    def __init__(self, capacity):
+        self.capacity = capacity
+        self.used = 0
+        self.scenes = {}
+        self.next_id = 0
+        self.max_resident = 0
+        self.admit_order = []
+
+    def footprint(self, spec):
+        return spec.bytes
+
+    def free_bytes(self):
+        return self.capacity - self.used
+
+    def add(self, spec):
+        if spec.bytes > self.free_bytes():
+            raise RuntimeError("arena full")
+        sid = self.next_id
+        self.next_id += 1
+        self.used += spec.bytes
+        self.scenes[sid] = {"spec": spec, "ran": 0}
+        self.max_resident = max(self.max_resident, len(self.scenes))
+        self.admit_order.append(spec.name)
+        return sid
+
+    def remove(self, sid):
+        self.used -= self.scenes.pop(sid)["spec"].bytes
+
+    def step(self, n, graph=True):
+        for s in self.scenes.values():
+            s["ran"] += n
+
+    def sync(self):
+        pass
+
+    def progress(self, sid):
+        s = self.scenes[sid]
+        done = min(max(s["ran"] - 1, 0), s["spec"].horizon)  # first substep is the warm-up
+        return done, done >= s["spec"].horizon
+
+    def positions(self, sid):
+        s = self.scenes[sid]
+        v = np.nan if s["spec"].poison else float(s["spec"].name)
+        return np.full((4, 2), v, np.float32)
+
+    def velocities(self, sid):
+        return np.zeros((4, 2), np.float32)
+
+
+def spec(name, horizon, nbytes, poison=False):
+    return SimpleNamespace(name=name, horizon=horizon, bytes=nbytes, n_elems=10, poison=poison)
+
+
+def test_fifo_admission_and_retirement():
+    b = FakeBatch(capacity=100)
+    srv = SimServer(batch=b, frame=60)
+    jobs = [Job(i, spec(i, h, 40), 0.0) for i, h in enumerate([30, 200, 59, 60, 500])]
+    res = srv.run(jobs)
+    assert sorted(r.job_id for r in res) == list(range(5))
+    assert b.admit_order == [0, 1, 2, 3, 4]  # arrival order
+    assert b.max_resident == 2  # 100 bytes hold two 40-byte jobs
+    assert all(r.ok for r in res)
+    # horizon 59 needs warm-up + 59 substeps = 60: one frame once admitted
+    by = {r.job_id: r for r in res}
+    assert by[0].finished <= by[1].finished
+
+
+def test_failed_job_is_isolated():
+    b = FakeBatch(capacity=1000)
+    srv = SimServer(batch=b, frame=60)
+    jobs = [Job(0, spec(0, 60, 10)), Job(1, spec(1, 60, 10, poison=True)), Job(2, spec(2, 60, 10))]
+    res = {r.job_id: r for r in srv.run(jobs)}
+    assert not res[1].ok and res[0].ok and res[2].ok
+    assert srv.stats()["failed"] == 1
+
+
+def test_oversized_job_rejected():
+    b = FakeBatch(capacity=10)
+    srv = SimServer(batch=b, frame=60)
+    with pytest.raises(RuntimeError, match="needs"):
+        srv.run([Job(0, spec(0, 60, 11))])
+
+
+def test_horizon_required():
+    srv = SimServer(batch=FakeBatch(10), frame=60)
+    with pytest.raises(ValueError, match="horizon"):
+        srv.run([Job(0, spec(0, 0, 1))])
+
+
+def test_poisson_arrivals_rate():
+    t = poisson_arrivals(20000, 500.0, np.random.default_rng(0))
+    assert np.all(np.diff(t) > 0)
+    assert abs(20000 / t[-1] - 500.0) < 15.0
