@@ -65,10 +65,11 @@ struct Tiling {
     std::vector<int32_t> node_inv;   // caller -> internal
     std::vector<int32_t> elem_perm;  // internal -> caller element id
     std::vector<int32_t> elem_inv;
-    std::vector<TileDev> tiles;      // scene field left 0; offsets relative to the scene
+    std::vector<TileDev> tiles;      // pointers and scene fields left 0; offsets relative to the scene
     std::vector<uint32_t> rec;       // 4 per slot (one uint4): local corner ids, LDS destinations, element
     std::vector<int32_t> hnode;      // per tile: storage ids of the halo nodes, ascending
     std::vector<uint16_t> csr_ptr;
+    std::vector<int32_t> push;       // 2 per halo copy, grouped by source tile: (local owned node, copy index)
     int64_t halo_slots = 0;
 };
 
@@ -292,6 +293,24 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
             rec[4 * s + 3] = (uint32_t)slots[s];
         }
     }
+    // push lists: every halo copy is written by the tile owning its node, right after that
+    // tile computes the node's new position
+    std::vector<std::vector<int32_t>> by_src(ntiles);
+    for (int t = 0; t < ntiles; ++t) {
+        const TileDev& td = T.tiles[t];
+        for (int h = 0; h < td.n_hnodes; ++h) {
+            const int q = T.hnode[td.hnode_off + h];
+            const int o = node_tile[q];
+            by_src[o].push_back(q - T.tiles[o].node0);
+            by_src[o].push_back(td.hnode_off + h);
+        }
+    }
+    for (int t = 0; t < ntiles; ++t) {
+        TileDev& td = T.tiles[t];
+        td.push_off = (int32_t)(T.push.size() / 2);
+        td.n_push = (int32_t)(by_src[t].size() / 2);
+        T.push.insert(T.push.end(), by_src[t].begin(), by_src[t].end());
+    }
     return FW_OK;
 }
 
@@ -300,7 +319,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
 struct Layout {
     int64_t pos0, pos1, rest, vel, mass, flag, bcv, disp, force, cflag;
     int64_t jinv, area, j2a, j2b, stress, strain, yflag;
-    int64_t rec, hnode, cptr, tiles;
+    int64_t rec, hpos0, hpos1, hrest, push, cptr, tiles;
     int64_t gdata[FW_MAX_GROUNDS];
     int64_t total;
 };
@@ -336,7 +355,10 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hnode, int
     L.strain = take(16 * E);
     L.yflag = take(E);
     L.rec = take(16 * n_slots);
-    L.hnode = take(4 * n_hnode);
+    L.hpos0 = take(8 * n_hnode);
+    L.hpos1 = take(8 * n_hnode);
+    L.hrest = take(8 * n_hnode);
+    L.push = take(8 * n_hnode);
     L.cptr = take(2 * n_ptr);
     L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
     for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
@@ -745,6 +767,73 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
                                        std::to_string(a->capacity - a->used) + " free (fragmented)");
     if (!a->free_ids.empty()) a->free_ids.pop_back();
 
+    // device addresses of the block's arrays
+    char* blk = a->base + off;
+    SceneDev& D = sh->dev;
+    std::memset(&D, 0, sizeof(D));
+    D.pos[0] = (float2*)(blk + L.pos0);
+    D.pos[1] = (float2*)(blk + L.pos1);
+    D.vel = (float2*)(blk + L.vel);
+    D.mass = (float*)(blk + L.mass);
+    D.flag = (const uint32_t*)(blk + L.flag);
+    D.bcv = (const float2*)(blk + L.bcv);
+    D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
+    D.force = (float2*)(blk + L.force);
+    D.cflag = (uint8_t*)(blk + L.cflag);
+    D.rest = (const float2*)(blk + L.rest);
+    D.hpos[0] = (float2*)(blk + L.hpos0);
+    D.hpos[1] = (float2*)(blk + L.hpos1);
+    D.jinv = (float4*)(blk + L.jinv);
+    D.area = (float*)(blk + L.area);
+    D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
+    D.j2[1] = L.j2b >= 0 ? (float4*)(blk + L.j2b) : nullptr;
+    D.stress = (float4*)(blk + L.stress);
+    D.strain = (float4*)(blk + L.strain);
+    D.yflag = (uint8_t*)(blk + L.yflag);
+    D.n_nodes = N;
+    D.n_elems = E;
+    D.mat_type = d->mat_type;
+    D.n_grounds = d->n_grounds;
+    D.E = d->E;
+    D.nu = d->nu;
+    D.rho = d->rho;
+    D.sigma_y = d->sigma_y;
+    D.H = d->H;
+    D.dt = d->dt;
+    D.damping = d->damping;
+    // warm-up runs at the next substep launched (local step -1)
+    D.s0 = (int32_t)(a->clock + 1);
+    D.horizon = d->horizon > 0 ? d->horizon : INT32_MAX;
+    D.live = 1;
+    for (int b = 0; b < 3; ++b) {
+        D.aabb[b][0] = fw_f2o(1e30f);
+        D.aabb[b][1] = fw_f2o(1e30f);
+        D.aabb[b][2] = fw_f2o(-1e30f);
+        D.aabb[b][3] = fw_f2o(-1e30f);
+    }
+    auto o8 = [](int64_t off) { return (uint32_t)(off >> 8); };  // layout offsets are 256-byte aligned
+    for (auto& td : T.tiles) {
+        td.base = blk;
+        td.o_rec = o8(L.rec);
+        td.o_pos[0] = o8(L.pos0);
+        td.o_pos[1] = o8(L.pos1);
+        td.o_hpos[0] = o8(L.hpos0);
+        td.o_hpos[1] = o8(L.hpos1);
+        td.o_rest = o8(L.rest);
+        td.o_hrest = o8(L.hrest);
+        td.o_vel = o8(L.vel);
+        td.o_mass = o8(L.mass);
+        td.o_flag = o8(L.flag);
+        td.o_disp = L.disp >= 0 ? o8(L.disp) : 0;
+        td.o_cptr = o8(L.cptr);
+        td.o_push = o8(L.push);
+        td.o_j2[0] = L.j2a >= 0 ? o8(L.j2a) : 0;
+        td.o_j2[1] = L.j2b >= 0 ? o8(L.j2b) : 0;
+        td.scene = id;
+        td.s0 = D.s0;
+        td.horizon = D.horizon;
+    }
+
     // host image of the block
     Staging* st = nullptr;
     rc = get_staging(a, L.total, &st);
@@ -765,15 +854,17 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     }
     std::memcpy(img + L.pos1, img + L.pos0, 8 * (size_t)N);
     std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
+    {
+        float2* hp = (float2*)(img + L.hpos0);
+        for (size_t j = 0; j < T.hnode.size(); ++j) hp[j] = pos0[T.hnode[j]];
+        std::memcpy(img + L.hpos1, img + L.hpos0, 8 * T.hnode.size());
+        std::memcpy(img + L.hrest, img + L.hpos0, 8 * T.hnode.size());
+    }
     std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
     std::memcpy(img + L.rec, T.rec.data(), 4 * T.rec.size());
-    std::memcpy(img + L.hnode, T.hnode.data(), 4 * T.hnode.size());
+    std::memcpy(img + L.push, T.push.data(), 4 * T.push.size());
     std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
-    for (auto& td : T.tiles) td.scene = id;
     std::memcpy(img + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
-    char* blk = a->base + off;
-    SceneDev& D = sh->dev;
-    std::memset(&D, 0, sizeof(D));
     for (int g = 0; g < d->n_grounds; ++g) {
         const fw_ground_desc& gd = d->grounds[g];
         GroundDev& G = D.ground[g];
@@ -817,47 +908,6 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     HIPCHK(hipEventRecord(st->ev, a->stream));
     st->busy = true;
 
-    D.pos[0] = (float2*)(blk + L.pos0);
-    D.pos[1] = (float2*)(blk + L.pos1);
-    D.vel = (float2*)(blk + L.vel);
-    D.mass = (float*)(blk + L.mass);
-    D.flag = (const uint32_t*)(blk + L.flag);
-    D.bcv = (const float2*)(blk + L.bcv);
-    D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
-    D.force = (float2*)(blk + L.force);
-    D.cflag = (uint8_t*)(blk + L.cflag);
-    D.rest = (const float2*)(blk + L.rest);
-    D.jinv = (float4*)(blk + L.jinv);
-    D.area = (float*)(blk + L.area);
-    D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
-    D.j2[1] = L.j2b >= 0 ? (float4*)(blk + L.j2b) : nullptr;
-    D.stress = (float4*)(blk + L.stress);
-    D.strain = (float4*)(blk + L.strain);
-    D.yflag = (uint8_t*)(blk + L.yflag);
-    D.rec = (const uint4*)(blk + L.rec);
-    D.hnode = (const int32_t*)(blk + L.hnode);
-    D.csr_ptr = (const uint16_t*)(blk + L.cptr);
-    D.n_nodes = N;
-    D.n_elems = E;
-    D.mat_type = d->mat_type;
-    D.n_grounds = d->n_grounds;
-    D.E = d->E;
-    D.nu = d->nu;
-    D.rho = d->rho;
-    D.sigma_y = d->sigma_y;
-    D.H = d->H;
-    D.dt = d->dt;
-    D.damping = d->damping;
-    // warm-up runs at the next substep launched (local step -1)
-    D.s0 = (int32_t)(a->clock + 1);
-    D.horizon = d->horizon > 0 ? d->horizon : INT32_MAX;
-    D.live = 1;
-    for (int b = 0; b < 3; ++b) {
-        D.aabb[b][0] = fw_f2o(1e30f);
-        D.aabb[b][1] = fw_f2o(1e30f);
-        D.aabb[b][2] = fw_f2o(-1e30f);
-        D.aabb[b][3] = fw_f2o(-1e30f);
-    }
     rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
     if (rc) return rc;
     HIPCHK(fw_launch_prepare(a->d_scenes, (const TileDev*)(blk + L.tiles), 0, ntiles, a->tune.tile_slots, a->stream));
@@ -949,7 +999,8 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
     //   every element owned by exactly one tile, owned ranges contiguous and disjoint;
     //   slots (owned + halo) and LDS nodes (owned + halo) within capacity; each slot
     //   record's local corner ids name the element's nodes; every tile's node CSR lists
-    //   exactly the node's (element, corner) incidences in ascending caller element id.
+    //   exactly the node's (element, corner) incidences in ascending caller element id;
+    //   every halo copy is pushed exactly once, by its node's owner.
     int rc = validate(d);
     if (rc) return rc;
     Tiling T;
@@ -1004,6 +1055,20 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
             if (want != got) return fail(FW_E_INVALID, "incidence order of node " + std::to_string(n) + " is wrong");
         }
     }
+    // push lists: each halo copy written exactly once, by the tile owning its node,
+    // from that node's local id
+    std::vector<int> written(T.hnode.size(), 0);
+    for (const TileDev& td : T.tiles) {
+        for (int q = 0; q < td.n_push; ++q) {
+            const int lo = T.push[2 * ((size_t)td.push_off + q)], j = T.push[2 * ((size_t)td.push_off + q) + 1];
+            if (lo < 0 || lo >= td.n_nodes || j < 0 || j >= (int)T.hnode.size())
+                return fail(FW_E_INVALID, "push entry out of range");
+            if (T.hnode[j] != td.node0 + lo) return fail(FW_E_INVALID, "push entry copies the wrong node");
+            written[j]++;
+        }
+    }
+    for (size_t j = 0; j < written.size(); ++j)
+        if (written[j] != 1) return fail(FW_E_INVALID, "halo copy " + std::to_string(j) + " written " + std::to_string(written[j]) + " times");
     for (int e = 0; e < E; ++e)
         if (owned[e] != 1) return fail(FW_E_INVALID, "element " + std::to_string(e) + " owned " + std::to_string(owned[e]) + " times");
     for (int i = 0; i < N; ++i)
@@ -1176,6 +1241,12 @@ int32_t fw_scene_write(fw_arena* a, int32_t id, int32_t field, const void* in, i
     HIPCHK(hipStreamSynchronize(a->stream));
     void* dst = field == FW_FIELD_X ? (void*)s->dev.pos[buf] : (void*)s->dev.vel;
     HIPCHK(hipMemcpy(dst, tmp.data(), need, hipMemcpyHostToDevice));
+    if (field == FW_FIELD_X && !s->tiling.hnode.empty()) {
+        // the halo copies the next substep reads hold the same positions
+        std::vector<float2> hp(s->tiling.hnode.size());
+        for (size_t j = 0; j < hp.size(); ++j) hp[j] = tmp[s->tiling.hnode[j]];
+        HIPCHK(hipMemcpy(s->dev.hpos[buf], hp.data(), 8 * hp.size(), hipMemcpyHostToDevice));
+    }
     return FW_OK;
 }
 

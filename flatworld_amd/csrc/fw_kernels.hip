@@ -28,7 +28,8 @@
 // flatworld/materials/materialfunctions.py:10-101 and
 // flatworld/mixedcontact.py:166-302, 660-684, 1566-1643 operation by
 // operation in float32; the build uses -ffp-contract=off so no FMA changes a
-// rounding.  log(J) is log((double)J) rounded to float.
+// rounding.  log(J) is fw_logf, a double-operation series shared with the
+// CPU restatement.
 #include <hip/hip_runtime.h>
 
 #include "fw_device.h"
@@ -129,18 +130,21 @@ struct NodeIn {
     float2 u;
     float m;
     uint32_t fl;
-    int jb, je;
+    uint32_t jr;  // CSR range: begin | end << 16 (one register)
 };
 
 // BLOCK threads; at most SLOTS element slots, NODES LDS nodes (owned + halo) and
-// NPT * BLOCK owned nodes per tile.
+// NPT * BLOCK owned nodes per tile.  LDS: 16 B per node + 24 B per slot, nothing
+// else (the AABB reduction reuses the rest-position array once phase A is done),
+// so 1024 slots take exactly 40 KiB and four tiles fit one CU's 160 KiB.
 template <int BLOCK, int SLOTS, int NODES, int NPT>
 __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     constexpr int SPT = (SLOTS + BLOCK - 1) / BLOCK;
     __shared__ float2 lds_x[NODES];
     __shared__ float2 lds_X[NODES];
     __shared__ float2 lds_f[SLOTS * 3];
-    __shared__ int32_t lds_red[(BLOCK / 64) * 4];
+    static_assert((BLOCK / 64) * 4 * sizeof(int32_t) <= NODES * sizeof(float2), "reduction scratch");
+    int32_t* lds_red = reinterpret_cast<int32_t*>(lds_X);
 
     const int G = A.ctl->clock + A.k;
     const int ntiles = A.tile_count >= 0 ? A.tile_count : (A.ctl->n_tiles - A.tile_base);
@@ -149,70 +153,80 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     for (int round = 0;; ++round) {
         const int ti = tile_of_block(round, ntiles);
         if (ti >= ntiles) break;
-        const TileDev T = A.tiles[A.tile_base + ti];
-        const SceneDev& S = A.scenes[T.scene];
-        const int local = G - S.s0;
-        if (!S.live || local < -1 || local >= S.horizon) continue;  // uniform per block
+        const TileDev T = A.tiles[A.tile_base + ti];  // a copy: stores below cannot alias it
+        const int local = G - T.s0;
+        if (local < -1 || local >= T.horizon) continue;  // uniform per block
 
-        const bool warm = local == -1;
-        const float dt = warm ? 0.0f : (A.dt_override >= 0.0f ? A.dt_override : S.dt);
-        const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : S.damping);
-        const bool diag = (A.flags & STEP_DIAG) != 0;
-        const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && S.n_grounds > 0;
         const int rd = (local + 1) & 1;
         const int wr = local & 1;
-        const float2* __restrict__ pos_rd = S.pos[rd];
-        float2* __restrict__ pos_wr = S.pos[wr];
-        const float2* __restrict__ rest = S.rest;
-        const uint4* __restrict__ rec = S.rec + T.rec_off;
-        const int32_t* __restrict__ hnode = S.hnode + T.hnode_off;
-        const uint16_t* __restrict__ cptr = S.csr_ptr + T.ptr_off;
         const int nslots = T.n_own + T.n_halo;
+        const int n_nodes = T.n_nodes;
+        const bool j2 = T.o_j2[0] != 0;
 
-        Mat m;
-        m.type = S.mat_type;
-        m.lam = S.E * S.nu / ((1.0f + S.nu) * (1.0f - 2.0f * S.nu));
-        m.mu = S.E / (2.0f * (1.0f + S.nu));
-        m.sy = S.sigma_y;
-        m.H = S.H;
-        const bool j2 = m.type == 3;
-
-        // ---------------- stage: issue every independent load of the tile
+        // ---------------- stage: every load below depends on the tile record only
+        const uint4* __restrict__ rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
         uint4 rc[SPT];
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int sl = tid + k * BLOCK;
             rc[k] = sl < nslots ? rec[sl] : make_uint4(0u, 0u, 0u, 0u);
         }
-        NodeIn nd[NPT];
-#pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int i = tid + k * BLOCK;
-            if (i < T.n_nodes) {
-                const int n = T.node0 + i;
-                nd[k].v = S.vel[n];
-                nd[k].u = S.disp ? S.disp[n] : make_float2(0.0f, 0.0f);
-                nd[k].m = S.mass[n];
-                nd[k].fl = S.flag[n];
-                nd[k].jb = cptr[i];
-                nd[k].je = cptr[i + 1];
-            }
-        }
-        for (int i = tid; i < T.n_nodes; i += BLOCK) {
-            lds_x[i] = pos_rd[T.node0 + i];
-            lds_X[i] = rest[T.node0 + i];
-        }
-        for (int h = tid; h < T.n_hnodes; h += BLOCK) {
-            const int g = hnode[h];
-            lds_x[T.n_nodes + h] = pos_rd[g];
-            lds_X[T.n_nodes + h] = rest[g];
-        }
         float4 st[SPT];
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             st[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (j2 && tid + k * BLOCK < nslots) st[k] = S.j2[rd][rc[k].w];
+            if (j2 && tid + k * BLOCK < T.n_own) st[k] = fw_at<const float4>(T.base, (rd ? T.o_j2[1] : T.o_j2[0]))[T.elem0 + tid + k * BLOCK];
         }
+        NodeIn nd[NPT];
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            const int i = tid + k * BLOCK;
+            if (i < n_nodes) {
+                const int n = T.node0 + i;
+                nd[k].v = fw_at<const float2>(T.base, T.o_vel)[n];
+                nd[k].u = T.o_disp ? fw_at<const float2>(T.base, T.o_disp)[n] : make_float2(0.0f, 0.0f);
+                nd[k].m = fw_at<const float>(T.base, T.o_mass)[n];
+                nd[k].fl = fw_at<const uint32_t>(T.base, T.o_flag)[n];
+                const uint16_t* cp = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off + i;
+                nd[k].jr = (uint32_t)cp[0] | ((uint32_t)cp[1] << 16);
+            }
+        }
+        {
+            const float2* __restrict__ pos_rd = fw_at<const float2>(T.base, (rd ? T.o_pos[1] : T.o_pos[0])) + T.node0;
+            const float2* __restrict__ rest = fw_at<const float2>(T.base, T.o_rest) + T.node0;
+            for (int i = tid; i < n_nodes; i += BLOCK) {
+                lds_x[i] = pos_rd[i];
+                lds_X[i] = rest[i];
+            }
+            const float2* __restrict__ hpos_rd = fw_at<const float2>(T.base, (rd ? T.o_hpos[1] : T.o_hpos[0])) + T.hnode_off;
+            const float2* __restrict__ hrest = fw_at<const float2>(T.base, T.o_hrest) + T.hnode_off;
+            for (int h = tid; h < T.n_hnodes; h += BLOCK) {
+                lds_x[n_nodes + h] = hpos_rd[h];
+                lds_X[n_nodes + h] = hrest[h];
+            }
+        }
+        // halo slots of J2 scenes: the element id comes from the record
+        if (j2) {
+#pragma unroll
+            for (int k = 0; k < SPT; ++k) {
+                const int sl = tid + k * BLOCK;
+                if (sl >= T.n_own && sl < nslots) st[k] = fw_at<const float4>(T.base, (rd ? T.o_j2[1] : T.o_j2[0]))[rc[k].w];
+            }
+        }
+
+        // scene scalars (needed from phase A on)
+        const SceneDev& S = A.scenes[T.scene];
+        const bool warm = local == -1;
+        const float dt = warm ? 0.0f : (A.dt_override >= 0.0f ? A.dt_override : S.dt);
+        const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : S.damping);
+        const bool diag = (A.flags & STEP_DIAG) != 0;
+        const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && S.n_grounds > 0;
+        Mat m;
+        m.type = S.mat_type;
+        m.lam = S.E * S.nu / ((1.0f + S.nu) * (1.0f - 2.0f * S.nu));
+        m.mu = S.E / (2.0f * (1.0f + S.nu));
+        m.sy = S.sigma_y;
+        m.H = S.H;
         __syncthreads();
 
         // ---------------- phase A: element slots from LDS
@@ -223,7 +237,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 uint32_t l[3], q[3];
                 unpack_rec(rc[k], l, q);
                 slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st[k], q,
-                           sl < T.n_own, (int)rc[k].w, diag, S.j2[wr], S.stress, S.strain, S.yflag, lds_f);
+                           sl < T.n_own, (int)rc[k].w, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])), S.stress, S.strain, S.yflag,
+                           lds_f);
             }
         }
 
@@ -248,14 +263,19 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 if (lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1]) active |= 1u << gi;
             }
         }
-        __syncthreads();
+        __syncthreads();  // lds_f complete; lds_X free from here on
 
         // ---------------- phase B: owned nodes
+        // first push entry per thread, loaded now so that it lands during phase B
+        const int2 pe = tid < T.n_push ? fw_at<const int2>(T.base, T.o_push)[T.push_off + tid] : make_int2(0, 0);
         int32_t bb[4] = {fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)};
+        float2* __restrict__ pos_wr = fw_at<float2>(T.base, (wr ? T.o_pos[1] : T.o_pos[0]));
+        float2* __restrict__ vel = fw_at<float2>(T.base, T.o_vel);
+        float2* __restrict__ disp = T.o_disp ? fw_at<float2>(T.base, T.o_disp) : nullptr;
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
             const int i = tid + k * BLOCK;
-            if (i >= T.n_nodes) continue;
+            if (i >= n_nodes) continue;
             const int n = T.node0 + i;
             const float2 p = lds_x[i];
             const float2 v = nd[k].v;
@@ -268,7 +288,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                     if (active & (1u << gi)) hit |= ground_force(S.ground[gi], p.x, p.y, v.x, v.y, dt, fx, fy);
             }
             // element forces in ascending element id: a contiguous run of LDS entries
-            for (int j = nd[k].jb; j < nd[k].je; ++j) {
+            for (int j = (int)(nd[k].jr & 0xFFFFu); j < (int)(nd[k].jr >> 16); ++j) {
                 const float2 f = lds_f[j];
                 fx = fx + (-f.x);
                 fy = fy + (-f.y);
@@ -300,11 +320,12 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 vx = 0.0f;
                 vy = 0.0f;
             }
-            S.vel[n] = make_float2(vx, vy);
+            vel[n] = make_float2(vx, vy);
             const float dux = vx * dt, duy = vy * dt;
-            if (S.disp) S.disp[n] = make_float2(nd[k].u.x + dux, nd[k].u.y + duy);
+            if (disp) disp[n] = make_float2(nd[k].u.x + dux, nd[k].u.y + duy);
             const float nx_ = p.x + dux, ny_ = p.y + duy;
             pos_wr[n] = make_float2(nx_, ny_);
+            lds_x[i] = make_float2(nx_, ny_);  // source of the halo copies pushed below
             if (diag) {
                 S.force[n] = make_float2(fx, fy);
                 S.cflag[n] = hit ? 1 : 0;
@@ -319,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         }
 
         // ---------------- boundary AABB at x_{n+1} (flatworld/femspringmanager.py:361-408)
-        block_minmax<BLOCK>(bb, lds_red);
+        block_minmax<BLOCK>(bb, lds_red);  // its barrier also publishes the lds_x updates
         if (tid == 0) {
             int32_t* aw = const_cast<int32_t*>(S.aabb[mod3(local + 1)]);
             int32_t* az = const_cast<int32_t*>(S.aabb[mod3(local + 2)]);
@@ -339,6 +360,16 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 aw[3] = bb[3];
             }
         }
+
+        // ---------------- halo copies of this tile's nodes for the neighbouring tiles
+        {
+            float2* __restrict__ hpos_wr = fw_at<float2>(T.base, (wr ? T.o_hpos[1] : T.o_hpos[0]));
+            if (tid < T.n_push) hpos_wr[pe.y] = lds_x[pe.x];
+            for (int q = tid + BLOCK; q < T.n_push; q += BLOCK) {
+                const int2 e = fw_at<const int2>(T.base, T.o_push)[T.push_off + q];
+                hpos_wr[e.y] = lds_x[e.x];
+            }
+        }
         __syncthreads();  // LDS reuse by the next tile of this block
     }
 }
@@ -356,10 +387,11 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
     if (ti >= tile_count) return;
     const TileDev T = tiles[tile_base + ti];
     const SceneDev& S = scenes[T.scene];
-    const uint4* rec = S.rec + T.rec_off;
-    const int32_t* hnode = S.hnode + T.hnode_off;
-    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = S.rest[T.node0 + i];
-    for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = S.rest[hnode[h]];
+    const uint4* rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
+    const float2* rest = fw_at<const float2>(T.base, T.o_rest);
+    const float2* hrest = fw_at<const float2>(T.base, T.o_hrest);
+    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = rest[T.node0 + i];
+    for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = hrest[T.hnode_off + h];
     __syncthreads();
     const int nslots = T.n_own + T.n_halo;
     for (int sl = threadIdx.x; sl < nslots; sl += BLOCK) {
@@ -379,11 +411,12 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
         if (q[2] != 0xFFFFu) lds_m[q[2]] = nd;
     }
     __syncthreads();
-    const uint16_t* cptr = S.csr_ptr + T.ptr_off;
+    const uint16_t* cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
+    float* mass = fw_at<float>(T.base, T.o_mass);
     for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
         float m = 0.0f;
         for (int j = cptr[i]; j < cptr[i + 1]; ++j) m += lds_m[j];
-        S.mass[T.node0 + i] = m;
+        mass[T.node0 + i] = m;
     }
 }
 

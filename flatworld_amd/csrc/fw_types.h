@@ -4,12 +4,15 @@
 //   nodes    pos[2] float2 (ping-pong: substep k reads pos[(k+1)&1], writes pos[k&1]),
 //            rest float2 (reference positions), vel float2, mass float, flag uint32,
 //            disp float2 (optional), force float2 (diag), cflag uint8 (diag)
+//   halo     hpos[2] float2 per (tile, halo node): copies of neighbour-owned positions,
+//            written by the owner tile in the same substep as pos (same ping-pong);
+//            hrest float2 per (tile, halo node): their rest positions
 //   elements jinv float4 + area float (written once, for readback only: the substep
 //            recomputes both from the rest positions), j2[2] float4 (eps_xx, eps_yy,
 //            eps_xy, eqps; ping-pong, J2 scenes only), stress / strain float4 (diag),
 //            yflag uint8 (diag)
-//   tiles    rec uint4 per element slot, hnode int32 per halo node, csr_ptr uint16 per
-//            owned node (+1)
+//   tiles    rec uint4 per element slot, csr_ptr uint16 per owned node (+1),
+//            push int2 per halo copy, grouped by the tile owning the source node
 //   ground   height samples float / occupancy int8
 #pragma once
 #include <hip/hip_runtime.h>
@@ -38,6 +41,7 @@ struct SceneDev {
     float2* disp;   // nullptr when displacement tracking is off
     float2* force;  // diag
     uint8_t* cflag; // diag
+    float2* hpos[2];       // halo copies (host bookkeeping; the substep uses the tile's pointers)
     // element arrays
     float4* jinv;
     float* area;
@@ -45,13 +49,6 @@ struct SceneDev {
     float4* stress;  // diag (xx, yy, xy, 0)
     float4* strain;  // diag
     uint8_t* yflag;  // diag
-    // tile data
-    // rec per element slot: .x = l0 | l1 << 16, .y = l2 | d0 << 16, .z = d1 | d2 << 16, .w = element
-    //   l* local node ids of the corners (owned nodes first, then the tile's halo nodes),
-    //   d* LDS entry receiving that corner's force (0xFFFF: node not owned by the tile)
-    const uint4* rec;
-    const int32_t* hnode;     // per tile: storage ids of the halo nodes
-    const uint16_t* csr_ptr;  // per tile: n_nodes + 1 offsets into the tile's node-major LDS entries
     // scalars
     int32_t n_nodes, n_elems;
     int32_t mat_type, n_grounds;
@@ -59,23 +56,47 @@ struct SceneDev {
     float dt, damping;
     int32_t s0;       // global step at which the scene runs its warm-up (local step -1 is s0 - 1)
     int32_t horizon;  // substeps after the warm-up; INT32_MAX when unbounded
-    int32_t live;     // 0 once removed
+    int32_t live;     // 0 once removed (host bookkeeping: the tile table holds live scenes only)
     int32_t gate_base; // local step that counts as 0 for the gate's rebuild schedule
     int32_t aabb[3][4];  // boundary AABB (ordered-int encoded), triple buffered by local step
     GroundDev ground[FW_MAX_GROUNDS_DEV];
 };
 
+// One tile of one scene.  It locates every array the substep's first loads touch
+// (one base pointer + 32-bit offsets, to keep the record in few scalar registers),
+// so those loads depend on this record alone; the scene entry is read in parallel,
+// for material, ground and diagnostics.
 struct TileDev {
-    int32_t scene;
-    int32_t node0, n_nodes;    // owned nodes [node0, node0 + n_nodes)
-    int32_t elem0, n_own;      // owned elements [elem0, elem0 + n_own)
-    int32_t n_halo;            // halo element slots after the owned ones
-    int32_t rec_off;           // rec[rec_off + slot]
-    int32_t hnode_off, n_hnodes;  // hnode[hnode_off .. +n_hnodes)
-    int32_t ptr_off;           // csr_ptr[ptr_off .. ptr_off + n_nodes]
-    int32_t multi;             // scene has more than one tile (AABB via atomics)
-    int32_t pad;
+    char* base;              // the scene's block; array = base + (offset << 8) (blocks are 256-byte aligned)
+    uint32_t o_rec;          // uint4 per element slot:
+    //   .x = l0 | l1 << 16, .y = l2 | d0 << 16, .z = d1 | d2 << 16, .w = element
+    //   l* local node ids of the corners (owned nodes first, then the tile's halo nodes),
+    //   d* LDS entry receiving that corner's force (0xFFFF: node not owned by the tile)
+    uint32_t o_pos[2];       // float2 node positions (ping-pong)
+    uint32_t o_hpos[2];      // float2 halo copies (ping-pong)
+    uint32_t o_rest;         // float2 rest positions
+    uint32_t o_hrest;        // float2 halo rest positions
+    uint32_t o_vel;          // float2
+    uint32_t o_mass;         // float
+    uint32_t o_flag;         // uint32
+    uint32_t o_disp;         // float2; 0 when displacement tracking is off
+    uint32_t o_cptr;         // uint16 node CSR: n_nodes + 1 offsets per tile into its node-major LDS entries
+    uint32_t o_push;         // int2 push list: (local owned node, halo copy index in the scene)
+    uint32_t o_j2[2];        // float4 J2 state (ping-pong); 0 unless J2
+    int32_t scene, s0, horizon, multi;  // s0 / horizon as in SceneDev; multi: scene has > 1 tile
+    int32_t node0, n_nodes;             // owned nodes [node0, node0 + n_nodes)
+    int32_t elem0, n_own;               // owned elements [elem0, elem0 + n_own)
+    int32_t n_halo;                     // halo element slots after the owned ones
+    int32_t rec_off;                    // rec[rec_off + slot]
+    int32_t hnode_off, n_hnodes;        // halo copies hpos[.][hnode_off .. +n_hnodes)
+    int32_t ptr_off;                    // cptr[ptr_off .. ptr_off + n_nodes]
+    int32_t push_off, n_push;           // push[push_off .. +n_push)
 };
+
+template <typename P>
+__host__ __device__ inline P* fw_at(char* base, uint32_t off) {
+    return reinterpret_cast<P*>(base + ((size_t)off << 8));
+}
 
 // Arena control block in device memory.
 struct ArenaCtl {

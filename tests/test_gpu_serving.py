@@ -50,17 +50,24 @@ def test_served_results_equal_solo_runs():
 
 def test_admission_waits_for_memory():
     jobs = small_jobs(8, seed=22, side_lo=10, side_hi=12)
-    probe = FemBatch(capacity_bytes=1 << 26)
-    need = max(probe.footprint(j.spec) for j in jobs)
+    probe = FemBatch(capacity_bytes=1 << 26, tile_nodes=256)
+    fp = {j.job_id: probe.footprint(j.spec) for j in jobs}
     probe.close()
-    log(f"[case] admission with room for 2 jobs ({need} bytes each)")
-    srv = SimServer(frame=30, tile_nodes=256, capacity_bytes=2 * need + 4096)
+    cap = 2 * max(fp.values()) + 4096
+    log(f"[case] admission into {cap} bytes, footprints {sorted(fp.values())}")
+    srv = SimServer(frame=30, tile_nodes=256, capacity_bytes=cap)
     res = srv.run(jobs)
     assert len(res) == 8 and all(r.ok for r in res)
-    # never more than two resident: each admission waits for a retirement
-    spans = sorted((r.admitted, r.finished) for r in res)
-    for t, _ in spans:
-        assert sum(1 for a, f in spans if a <= t < f) <= 2
+    # at every admission the resident footprints fit the arena, and the arena
+    # was small enough that some admissions had to wait for a retirement
+    spans = [(r.admitted, r.finished, fp[r.job_id]) for r in res]
+    most = 0
+    for t, _, _ in spans:
+        live = [b for a, f, b in spans if a <= t < f]
+        assert sum(live) <= cap
+        most = max(most, len(live))
+    assert most < len(jobs)
+    assert sum(fp.values()) > cap
 
 
 def test_failed_job_is_isolated():
