@@ -117,6 +117,8 @@ class ArenaInfo(ctypes.Structure):
         ("global_step", ctypes.c_int64),
         ("algo_bytes_per_substep", ctypes.c_double),
         ("halo_slots", ctypes.c_int64),
+        ("n_cu", ctypes.c_int32),
+        ("tiles_per_cu", ctypes.c_int32),
     ]
 
 

@@ -37,6 +37,7 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
                                         int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
+extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
 
 namespace {
@@ -366,7 +367,7 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hnode, int
         if (g >= d->n_grounds) continue;
         const fw_ground_desc& gd = d->grounds[g];
         if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(4 * (int64_t)gd.hf_nx);
-        if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take((int64_t)gd.vox_nx * gd.vox_ny);
+        if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
     }
     L.total = o;
     return L;
@@ -392,6 +393,8 @@ int validate(const fw_scene_desc* d) {
             return fail(FW_E_INVALID, "height field needs at least 2 samples");
         if (gd.kind == FW_GROUND_VOXEL && (gd.vox_nx < 1 || gd.vox_ny < 1 || !gd.occ))
             return fail(FW_E_INVALID, "voxel grid needs nx, ny >= 1 and an occupancy array");
+        if (gd.kind == FW_GROUND_VOXEL && (gd.vox_nx > 65534 || gd.vox_ny > 65534))
+            return fail(FW_E_INVALID, "voxel grid is limited to 65534 cells per axis");
         if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_VOXEL)
             return fail(FW_E_INVALID, "unknown ground kind " + std::to_string(gd.kind));
     }
@@ -399,6 +402,37 @@ int validate(const fw_scene_desc* d) {
 }
 
 // ------------------------------------------------------------------ pinned staging
+
+// Per cell of a voxel grid, the four scan results of the signed-distance query
+// (flatworld/mixedcontact.py:220-302): scanning from an occupied cell (i, j) along
+// -x, +x, -z, +z for the first empty cell ends at i0 - 1, i1, j0 - 1, j1, stored as
+// (i0, i1, j0, j1); a cell that is not occupied (value != 1) stores i0 = 0xFFFF.
+// The device reads one 8-byte entry instead of walking the grid, and evaluates
+// the same float expressions on the same integers.
+void build_voxel_extents(const fw_ground_desc& gd, uint16_t* ext) {
+    const int nx = gd.vox_nx, ny = gd.vox_ny;
+    auto at = [&](int i, int j) { return gd.occ[(int64_t)i * ny + j]; };
+    std::vector<uint16_t> lo(nx), hi(nx);
+    for (int j = 0; j < ny; ++j) {
+        for (int i = 0; i < nx; ++i) lo[i] = (i > 0 && at(i - 1, j) != 0) ? lo[i - 1] : (uint16_t)i;
+        for (int i = nx - 1; i >= 0; --i) hi[i] = (i + 1 < nx && at(i + 1, j) != 0) ? hi[i + 1] : (uint16_t)(i + 1);
+        for (int i = 0; i < nx; ++i) {
+            uint16_t* e = ext + 4 * ((int64_t)i * ny + j);
+            e[0] = at(i, j) == 1 ? lo[i] : (uint16_t)0xFFFF;
+            e[1] = hi[i];
+        }
+    }
+    std::vector<uint16_t> lo2(ny), hi2(ny);
+    for (int i = 0; i < nx; ++i) {
+        for (int j = 0; j < ny; ++j) lo2[j] = (j > 0 && at(i, j - 1) != 0) ? lo2[j - 1] : (uint16_t)j;
+        for (int j = ny - 1; j >= 0; --j) hi2[j] = (j + 1 < ny && at(i, j + 1) != 0) ? hi2[j + 1] : (uint16_t)(j + 1);
+        for (int j = 0; j < ny; ++j) {
+            uint16_t* e = ext + 4 * ((int64_t)i * ny + j);
+            e[2] = lo2[j];
+            e[3] = hi2[j];
+        }
+    }
+}
 
 struct Staging {
     void* ptr = nullptr;
@@ -707,6 +741,9 @@ int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
         algo += (double)sp->n_elems * (sp->mat_type == FW_MAT_MISES ? 64.0 : 32.0) + (double)sp->n_nodes * 40.0;
     }
     out->algo_bytes_per_substep = algo;
+    out->n_cu = a->n_cu;
+    HIPCHK(hipSetDevice(a->device));
+    HIPCHK(fw_substep_occupancy(a->tune.block, a->tune.tile_slots, &out->tiles_per_cu));
     return FW_OK;
 }
 
@@ -896,11 +933,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
             G.a[3] = gd.vox_dz;
             G.nx = gd.vox_nx;
             G.ny = gd.vox_ny;
-            int8_t* occ = (int8_t*)(img + L.gdata[g]);
-            for (int64_t c = 0; c < (int64_t)gd.vox_nx * gd.vox_ny; ++c) {
-                int32_t v = gd.occ[c];
-                occ[c] = v == 0 ? 0 : (v == 1 ? 1 : 2);
-            }
+            build_voxel_extents(gd, (uint16_t*)(img + L.gdata[g]));
             G.data = blk + L.gdata[g];
         }
     }

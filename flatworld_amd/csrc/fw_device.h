@@ -152,19 +152,20 @@ __device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z,
     int i1 = i0 + 1;
     if (i1 >= nx) i1 = nx - 1;
     if (i0 < 0) i0 = 0;
-    float t = sg - (float)i0;
-    float h0 = hgt[i0], h1 = hgt[i1];
-    float h = h0 + (h1 - h0) * t;
     int ir = (int)roundf(sg);
     if (ir < 0) ir = 0;
     if (ir >= nx) ir = nx - 1;
     int j0 = ir - 1, j1 = ir + 1;
     if (j0 < 0) j0 = 0;
     if (j1 >= nx) j1 = nx - 1;
+    // the four samples in one round trip
+    const float h0 = fw_ld(hgt + i0), h1 = fw_ld(hgt + i1), hj0 = fw_ld(hgt + j0), hj1 = fw_ld(hgt + j1);
+    float t = sg - (float)i0;
+    float h = h0 + (h1 - h0) * t;
     float dxw = span / (float)(nx - 1);
     float denom = dxw * (float)(j1 - j0);
     float dhdx = 0.0f;
-    if (denom > 1e-6f) dhdx = (hgt[j1] - hgt[j0]) / denom;
+    if (denom > 1e-6f) dhdx = (hj1 - hj0) / denom;
     float n0 = -dhdx, n1 = 1.0f;
     if (g.reverse == 1) {
         n0 = -n0;
@@ -181,38 +182,34 @@ __device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z,
 }
 
 // flatworld/mixedcontact.py:220-302 (_voxel_signed_distance_2d, interior cells only).
+// The reference scans from an occupied cell along -x, +x, -z, +z for the first empty
+// cell; the scan end points come precomputed per cell (host build_voxel_extents), so
+// one 8-byte load replaces the walk and the distances below are the same float
+// expressions on the same integers.
 __device__ __forceinline__ float voxel_signed(const GroundDev& g, float px, float py, float& n0o, float& n1o) {
-    const int8_t* occ = (const int8_t*)g.data;
-    const int nx = g.nx, ny = g.ny;
     const float lb0 = g.a[0], lb1 = g.a[1], dx = g.a[2], dz = g.a[3];
     float best = 1e9f;
     n0o = 0.0f;
     n1o = 1.0f;
-    int i = (int)floorf((px - lb0) / dx);
-    int j = (int)floorf((py - lb1) / dz);
-    if (0 <= i && i < nx && 0 <= j && j < ny && occ[(int64_t)i * ny + j] == 1) {
-        float dnx = 1e9f, dpx = 1e9f, dny = 1e9f, dpy = 1e9f;
-        int idx = i - 1;
-        while (idx >= 0 && occ[(int64_t)idx * ny + j] != 0) --idx;
-        dnx = px - (lb0 + (float)(idx + 1) * dx);
-        idx = i + 1;
-        while (idx < nx && occ[(int64_t)idx * ny + j] != 0) ++idx;
-        dpx = (lb0 + (float)idx * dx) - px;
-        idx = j - 1;
-        while (idx >= 0 && occ[(int64_t)i * ny + idx] != 0) --idx;
-        dny = py - (lb1 + (float)(idx + 1) * dz);
-        idx = j + 1;
-        while (idx < ny && occ[(int64_t)i * ny + idx] != 0) ++idx;
-        dpy = (lb1 + (float)idx * dz) - py;
-        float mv = dnx;
-        int face = 0;
-        if (dpx < mv) { mv = dpx; face = 1; }
-        if (dny < mv) { mv = dny; face = 2; }
-        if (dpy < mv) { mv = dpy; face = 3; }
-        if (face == 0) { best = -dnx; n0o = -1.0f; n1o = 0.0f; }
-        else if (face == 1) { best = -dpx; n0o = 1.0f; n1o = 0.0f; }
-        else if (face == 2) { best = -dny; n0o = 0.0f; n1o = -1.0f; }
-        else { best = -dpy; n0o = 0.0f; n1o = 1.0f; }
+    const int i = (int)floorf((px - lb0) / dx);
+    const int j = (int)floorf((py - lb1) / dz);
+    if (0 <= i && i < g.nx && 0 <= j && j < g.ny) {
+        const uint2 e = fw_ld((const uint2*)g.data + (int64_t)i * g.ny + j);
+        if ((e.x & 0xFFFFu) != 0xFFFFu) {
+            const float dnx = px - (lb0 + (float)(int)(e.x & 0xFFFFu) * dx);
+            const float dpx = (lb0 + (float)(int)(e.x >> 16) * dx) - px;
+            const float dny = py - (lb1 + (float)(int)(e.y & 0xFFFFu) * dz);
+            const float dpy = (lb1 + (float)(int)(e.y >> 16) * dz) - py;
+            float mv = dnx;
+            int face = 0;
+            if (dpx < mv) { mv = dpx; face = 1; }
+            if (dny < mv) { mv = dny; face = 2; }
+            if (dpy < mv) { mv = dpy; face = 3; }
+            if (face == 0) { best = -dnx; n0o = -1.0f; n1o = 0.0f; }
+            else if (face == 1) { best = -dpx; n0o = 1.0f; n1o = 0.0f; }
+            else if (face == 2) { best = -dny; n0o = 0.0f; n1o = -1.0f; }
+            else { best = -dpy; n0o = 0.0f; n1o = 1.0f; }
+        }
     }
     return best;
 }

@@ -62,7 +62,8 @@ __device__ __forceinline__ void unpack_rec(uint4 r, uint32_t l[3], uint32_t q[3]
 __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, float2 p3, float2 X1, float2 X2,
                                            float2 X3, float4 st, const uint32_t q[3], bool own, int e, bool diag,
                                            float4* __restrict__ j2_wr, float4* __restrict__ stress,
-                                           float4* __restrict__ strain, uint8_t* __restrict__ yflag, float2* lds_f) {
+                                           float4* __restrict__ strain, uint8_t* __restrict__ yflag,
+                                           float2* lds_f) {
     float4 ji;
     float ar;
     rest_frame(X1, X2, X3, ji, ar);
@@ -77,7 +78,7 @@ __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, f
     if (m.type == 3) {
         float4 stn;
         yielded = stress_j2(m, F, st, s, g, stn);
-        if (own) j2_wr[e] = stn;
+        if (own) fw_st(j2_wr + e, stn);
     } else {
         stress_hyper(m, F, s, g);
     }
@@ -104,9 +105,9 @@ __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, f
         const float c11 = FS2 * F[2] + FS3 * F[3];
         const float J = F[0] * F[3] - F[1] * F[2];
         const float jd = J > 1e-12f ? J : 1e-12f;
-        stress[e] = make_float4(c00 / jd, c11 / jd, c01 / jd, 0.0f);
-        strain[e] = make_float4(g[0], g[1], g[2], 0.0f);
-        yflag[e] = yielded ? 1 : 0;
+        fw_st(stress + e, make_float4(c00 / jd, c11 / jd, c01 / jd, 0.0f));
+        fw_st(strain + e, make_float4(g[0], g[1], g[2], 0.0f));
+        fw_st(yflag + e, (uint8_t)(yielded ? 1 : 0));
     }
 }
 
@@ -169,50 +170,29 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int sl = tid + k * BLOCK;
-            rc[k] = sl < nslots ? rec[sl] : make_uint4(0u, 0u, 0u, 0u);
-        }
-        float4 st[SPT];
-#pragma unroll
-        for (int k = 0; k < SPT; ++k) {
-            st[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (j2 && tid + k * BLOCK < T.n_own) st[k] = fw_at<const float4>(T.base, (rd ? T.o_j2[1] : T.o_j2[0]))[T.elem0 + tid + k * BLOCK];
+            rc[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (sl < nslots) rc[k] = fw_ld(rec + sl);
         }
         NodeIn nd[NPT];
-#pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            const int i = tid + k * BLOCK;
-            if (i < n_nodes) {
-                const int n = T.node0 + i;
-                nd[k].v = fw_at<const float2>(T.base, T.o_vel)[n];
-                nd[k].u = T.o_disp ? fw_at<const float2>(T.base, T.o_disp)[n] : make_float2(0.0f, 0.0f);
-                nd[k].m = fw_at<const float>(T.base, T.o_mass)[n];
-                nd[k].fl = fw_at<const uint32_t>(T.base, T.o_flag)[n];
-                const uint16_t* cp = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off + i;
-                nd[k].jr = (uint32_t)cp[0] | ((uint32_t)cp[1] << 16);
-            }
-        }
         {
             const float2* __restrict__ pos_rd = fw_at<const float2>(T.base, (rd ? T.o_pos[1] : T.o_pos[0])) + T.node0;
             const float2* __restrict__ rest = fw_at<const float2>(T.base, T.o_rest) + T.node0;
             for (int i = tid; i < n_nodes; i += BLOCK) {
-                lds_x[i] = pos_rd[i];
-                lds_X[i] = rest[i];
+                lds_x[i] = fw_ld(pos_rd + i);
+                lds_X[i] = fw_ld(rest + i);
             }
             const float2* __restrict__ hpos_rd = fw_at<const float2>(T.base, (rd ? T.o_hpos[1] : T.o_hpos[0])) + T.hnode_off;
             const float2* __restrict__ hrest = fw_at<const float2>(T.base, T.o_hrest) + T.hnode_off;
             for (int h = tid; h < T.n_hnodes; h += BLOCK) {
-                lds_x[n_nodes + h] = hpos_rd[h];
-                lds_X[n_nodes + h] = hrest[h];
+                lds_x[n_nodes + h] = fw_ld(hpos_rd + h);
+                lds_X[n_nodes + h] = fw_ld(hrest + h);
             }
         }
-        // halo slots of J2 scenes: the element id comes from the record
-        if (j2) {
-#pragma unroll
-            for (int k = 0; k < SPT; ++k) {
-                const int sl = tid + k * BLOCK;
-                if (sl >= T.n_own && sl < nslots) st[k] = fw_at<const float4>(T.base, (rd ? T.o_j2[1] : T.o_j2[0]))[rc[k].w];
-            }
-        }
+        // J2 state of slot 0 (halo slots take the element id from the record); the
+        // other slots' states are fetched one slot ahead inside phase A
+        const float4* __restrict__ j2_rd = fw_at<const float4>(T.base, (rd ? T.o_j2[1] : T.o_j2[0]));
+        float4 st_cur = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (j2 && tid < nslots) st_cur = fw_ld(j2_rd + (tid < T.n_own ? T.elem0 + tid : (int)rc[0].w));
 
         // scene scalars (needed from phase A on)
         const SceneDev& S = A.scenes[T.scene];
@@ -230,16 +210,40 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         __syncthreads();
 
         // ---------------- phase A: element slots from LDS
+        // Registers are the occupancy limit here: the next slot's J2 state is
+        // fetched one slot ahead, and the nodes' inputs for phase B are fetched two
+        // slots before the end, so that they land during the last slots and the gate.
+        constexpr int ND_AT = SPT >= 2 ? SPT - 2 : 0;
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int sl = tid + k * BLOCK;
+            float4 st_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (k + 1 < SPT && j2 && sl + BLOCK < nslots)
+                st_next = fw_ld(j2_rd + (sl + BLOCK < T.n_own ? T.elem0 + sl + BLOCK : (int)rc[k + 1 < SPT ? k + 1 : k].w));
+            if (k == ND_AT) {
+#pragma unroll
+                for (int kn = 0; kn < NPT; ++kn) {
+                    const int i = tid + kn * BLOCK;
+                    if (i < n_nodes) {
+                        const int n = T.node0 + i;
+                        nd[kn].v = fw_ld(fw_at<const float2>(T.base, T.o_vel) + n);
+                        nd[kn].u = make_float2(0.0f, 0.0f);
+                        if (T.o_disp) nd[kn].u = fw_ld(fw_at<const float2>(T.base, T.o_disp) + n);
+                        nd[kn].m = fw_ld(fw_at<const float>(T.base, T.o_mass) + n);
+                        nd[kn].fl = fw_ld(fw_at<const uint32_t>(T.base, T.o_flag) + n);
+                        const uint16_t* cp = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off + i;
+                        nd[kn].jr = (uint32_t)fw_ld(cp) | ((uint32_t)fw_ld(cp + 1) << 16);
+                    }
+                }
+            }
             if (sl < nslots) {
                 uint32_t l[3], q[3];
                 unpack_rec(rc[k], l, q);
-                slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st[k], q,
-                           sl < T.n_own, (int)rc[k].w, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])), S.stress, S.strain, S.yflag,
-                           lds_f);
+                slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st_cur, q,
+                           sl < T.n_own, T.elem0 + sl, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])),
+                           S.stress, S.strain, S.yflag, lds_f);
             }
+            st_cur = st_next;
         }
 
         // ---------------- activation gate (flatworld/explicitloop.py:477-487, bvh.py:39-46, 321-342, 393-416)
@@ -267,11 +271,13 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 
         // ---------------- phase B: owned nodes
         // first push entry per thread, loaded now so that it lands during phase B
-        const int2 pe = tid < T.n_push ? fw_at<const int2>(T.base, T.o_push)[T.push_off + tid] : make_int2(0, 0);
+        int2 pe = make_int2(0, 0);
+        if (tid < T.n_push) pe = fw_ld(fw_at<const int2>(T.base, T.o_push) + T.push_off + tid);
         int32_t bb[4] = {fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)};
         float2* __restrict__ pos_wr = fw_at<float2>(T.base, (wr ? T.o_pos[1] : T.o_pos[0]));
         float2* __restrict__ vel = fw_at<float2>(T.base, T.o_vel);
         float2* __restrict__ disp = T.o_disp ? fw_at<float2>(T.base, T.o_disp) : nullptr;
+        const float2* __restrict__ bcv = S.bcv;
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
             const int i = tid + k * BLOCK;
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 fy = fy + (-f.y);
             }
             // symplectic Euler with BC precedence (flatworld/femspringmanager.py:315-358)
-            const float2 bv = S.bcv[fl >> NODE_BCV_SHIFT];
+            const float2 bv = fw_ld(bcv + (fl >> NODE_BCV_SHIFT));
             if (fl & BC_U) {
                 fx = 0.0f;
                 fy = 0.0f;
@@ -320,15 +326,15 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 vx = 0.0f;
                 vy = 0.0f;
             }
-            vel[n] = make_float2(vx, vy);
+            fw_st(vel + n, make_float2(vx, vy));
             const float dux = vx * dt, duy = vy * dt;
-            if (disp) disp[n] = make_float2(nd[k].u.x + dux, nd[k].u.y + duy);
+            if (disp) fw_st(disp + n, make_float2(nd[k].u.x + dux, nd[k].u.y + duy));
             const float nx_ = p.x + dux, ny_ = p.y + duy;
-            pos_wr[n] = make_float2(nx_, ny_);
+            fw_st(pos_wr + n, make_float2(nx_, ny_));
             lds_x[i] = make_float2(nx_, ny_);  // source of the halo copies pushed below
             if (diag) {
-                S.force[n] = make_float2(fx, fy);
-                S.cflag[n] = hit ? 1 : 0;
+                fw_st(S.force + n, make_float2(fx, fy));
+                fw_st(S.cflag + n, (uint8_t)(hit ? 1 : 0));
             }
             if (fl & NODE_BOUNDARY) {
                 const int32_t ox = fw_f2o(nx_), oy = fw_f2o(ny_);
@@ -349,25 +355,25 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 atomicMin(aw + 1, bb[1]);
                 atomicMax(aw + 2, bb[2]);
                 atomicMax(aw + 3, bb[3]);
-                az[0] = fw_f2o(1e30f);
-                az[1] = fw_f2o(1e30f);
-                az[2] = fw_f2o(-1e30f);
-                az[3] = fw_f2o(-1e30f);
+                fw_st(az + 0, fw_f2o(1e30f));
+                fw_st(az + 1, fw_f2o(1e30f));
+                fw_st(az + 2, fw_f2o(-1e30f));
+                fw_st(az + 3, fw_f2o(-1e30f));
             } else {
-                aw[0] = bb[0];
-                aw[1] = bb[1];
-                aw[2] = bb[2];
-                aw[3] = bb[3];
+                fw_st(aw + 0, bb[0]);
+                fw_st(aw + 1, bb[1]);
+                fw_st(aw + 2, bb[2]);
+                fw_st(aw + 3, bb[3]);
             }
         }
 
         // ---------------- halo copies of this tile's nodes for the neighbouring tiles
         {
             float2* __restrict__ hpos_wr = fw_at<float2>(T.base, (wr ? T.o_hpos[1] : T.o_hpos[0]));
-            if (tid < T.n_push) hpos_wr[pe.y] = lds_x[pe.x];
+            if (tid < T.n_push) fw_st(hpos_wr + pe.y, lds_x[pe.x]);
             for (int q = tid + BLOCK; q < T.n_push; q += BLOCK) {
-                const int2 e = fw_at<const int2>(T.base, T.o_push)[T.push_off + q];
-                hpos_wr[e.y] = lds_x[e.x];
+                const int2 e = fw_ld(fw_at<const int2>(T.base, T.o_push) + T.push_off + q);
+                fw_st(hpos_wr + e.y, lds_x[e.x]);
             }
         }
         __syncthreads();  // LDS reuse by the next tile of this block
@@ -390,20 +396,20 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
     const uint4* rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
     const float2* rest = fw_at<const float2>(T.base, T.o_rest);
     const float2* hrest = fw_at<const float2>(T.base, T.o_hrest);
-    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = rest[T.node0 + i];
-    for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = hrest[T.hnode_off + h];
+    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = fw_ld(rest + T.node0 + i);
+    for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = fw_ld(hrest + T.hnode_off + h);
     __syncthreads();
     const int nslots = T.n_own + T.n_halo;
     for (int sl = threadIdx.x; sl < nslots; sl += BLOCK) {
-        const uint4 r = rec[sl];
+        const uint4 r = fw_ld(rec + sl);
         uint32_t l[3], q[3];
         unpack_rec(r, l, q);
         float4 ji;
         float vol;
         rest_frame(lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], ji, vol);
         if (sl < T.n_own) {
-            S.area[r.w] = vol;
-            S.jinv[r.w] = ji;
+            fw_st(S.area + r.w, vol);
+            fw_st(S.jinv + r.w, ji);
         }
         const float nd = vol * S.rho / 3.0f;
         if (q[0] != 0xFFFFu) lds_m[q[0]] = nd;
@@ -415,8 +421,8 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
     float* mass = fw_at<float>(T.base, T.o_mass);
     for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
         float m = 0.0f;
-        for (int j = cptr[i]; j < cptr[i + 1]; ++j) m += lds_m[j];
-        mass[T.node0 + i] = m;
+        for (int j = fw_ld(cptr + i); j < fw_ld(cptr + i + 1); ++j) m += lds_m[j];
+        fw_st(mass + T.node0 + i, m);
     }
 }
 
@@ -454,6 +460,23 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+// Workgroups of the substep kernel resident per CU for a shape (HIP's occupancy
+// calculator: LDS, registers and wave slots).
+extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu) {
+    *per_cu = 0;
+    if (block == 512 && slots == 1024)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<512, 1024, 1024, 1>, 512, 0);
+    if (block == 256 && slots == 1024)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<256, 1024, 1024, 2>, 256, 0);
+    if (block == 256 && slots == 512)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<256, 512, 512, 1>, 256, 0);
+    if (block == 512 && slots == 2048)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<512, 2048, 2048, 2>, 512, 0);
+    if (block == 1024 && slots == 2048)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<1024, 2048, 2048, 1>, 1024, 0);
+    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,

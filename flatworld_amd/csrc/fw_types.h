@@ -13,7 +13,7 @@
 //            yflag uint8 (diag)
 //   tiles    rec uint4 per element slot, csr_ptr uint16 per owned node (+1),
 //            push int2 per halo copy, grouped by the tile owning the source node
-//   ground   height samples float / occupancy int8
+//   ground   height samples float / voxel cell extents uint16 x4
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,7 +27,8 @@ struct GroundDev {
     float plo[2], phi[2];  // padded by 0.1 * extent (BVH rebuild substeps)
     float a[4];            // plane: p.x p.y n.x n.y | HF: lb_x ub_x | voxel: lb.x lb.y dx dz
     int32_t nx, ny, reverse, pad;
-    const void* data;      // HF heights (float) | voxel occupancy (int8: 0 empty, 1 occupied, 2 other)
+    const void* data;      // HF heights (float) | voxel cell extents (uint16 x4 per cell: scan end points
+                           // along -x, +x, -z, +z; first = 0xFFFF when the cell is not occupied)
 };
 
 struct SceneDev {
@@ -93,8 +94,41 @@ struct TileDev {
     int32_t push_off, n_push;           // push[push_off .. +n_push)
 };
 
+// HBM accesses of the kernels go through fw_ld / fw_st: they address global
+// memory explicitly.  A plain access through a pointer the compiler cannot prove
+// global (one read from a table or computed from a char*) is emitted as a flat
+// access, which counts against the LDS counter too and so makes every later LDS
+// wait also wait for the outstanding HBM traffic.
+#define FW_GLOBAL __attribute__((address_space(1)))
+typedef float fw_v2f __attribute__((ext_vector_type(2)));
+typedef float fw_v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t fw_v2u __attribute__((ext_vector_type(2)));
+typedef uint32_t fw_v4u __attribute__((ext_vector_type(4)));
+typedef int32_t fw_v2i __attribute__((ext_vector_type(2)));
+
+__device__ inline float2 fw_ld(const float2* p) { const fw_v2f v = *(const FW_GLOBAL fw_v2f*)p; return make_float2(v.x, v.y); }
+__device__ inline float4 fw_ld(const float4* p) {
+    const fw_v4f v = *(const FW_GLOBAL fw_v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ inline uint4 fw_ld(const uint4* p) {
+    const fw_v4u v = *(const FW_GLOBAL fw_v4u*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline uint2 fw_ld(const uint2* p) { const fw_v2u v = *(const FW_GLOBAL fw_v2u*)p; return make_uint2(v.x, v.y); }
+__device__ inline int2 fw_ld(const int2* p) { const fw_v2i v = *(const FW_GLOBAL fw_v2i*)p; return make_int2(v.x, v.y); }
+__device__ inline float fw_ld(const float* p) { return *(const FW_GLOBAL float*)p; }
+__device__ inline uint32_t fw_ld(const uint32_t* p) { return *(const FW_GLOBAL uint32_t*)p; }
+__device__ inline uint16_t fw_ld(const uint16_t* p) { return *(const FW_GLOBAL uint16_t*)p; }
+
+__device__ inline void fw_st(float2* p, float2 a) { *(FW_GLOBAL fw_v2f*)p = fw_v2f{a.x, a.y}; }
+__device__ inline void fw_st(float4* p, float4 a) { *(FW_GLOBAL fw_v4f*)p = fw_v4f{a.x, a.y, a.z, a.w}; }
+__device__ inline void fw_st(float* p, float a) { *(FW_GLOBAL float*)p = a; }
+__device__ inline void fw_st(int32_t* p, int32_t a) { *(FW_GLOBAL int32_t*)p = a; }
+__device__ inline void fw_st(uint8_t* p, uint8_t a) { *(FW_GLOBAL uint8_t*)p = a; }
+
 template <typename P>
-__host__ __device__ inline P* fw_at(char* base, uint32_t off) {
+__device__ inline P* fw_at(char* base, uint32_t off) {
     return reinterpret_cast<P*>(base + ((size_t)off << 8));
 }
 

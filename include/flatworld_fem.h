@@ -144,6 +144,8 @@ typedef struct {
     int64_t global_step;       /* substeps launched so far on this arena */
     double algo_bytes_per_substep; /* sum of E*B_e + N*B_n over resident scenes */
     int64_t halo_slots;        /* element slots evaluated on top of owned ones */
+    int32_t n_cu;              /* compute units of the device */
+    int32_t tiles_per_cu;      /* substep workgroups resident per CU for the tuning (HIP occupancy) */
 } fw_arena_info;
 
 typedef struct fw_arena fw_arena;
