@@ -271,6 +271,16 @@ __device__ __forceinline__ int tile_of_block(int round, int ntiles_total_grid) {
     return v + round * G;
 }
 
+// Workgroup barrier that orders LDS only.  Threads of a tile never exchange data
+// through global memory within a launch (positions, halo copies and J2 state are
+// double-buffered and read in the next launch), so global loads issued before
+// the barrier may stay in flight across it; __syncthreads would wait for them.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int BLOCK>
 __device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
     // v: lb.x, lb.y (min), ub.x, ub.y (max) as ordered ints
@@ -293,7 +303,7 @@ __device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
         red[wid * 4 + 2] = v[2];
         red[wid * 4 + 3] = v[3];
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         for (int w = 1; w < BLOCK / 64; ++w) {
             v[0] = min(v[0], red[w * 4 + 0]);

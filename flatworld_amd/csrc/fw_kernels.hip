@@ -134,92 +134,151 @@ struct NodeIn {
     uint32_t jr;  // CSR range: begin | end << 16 (one register)
 };
 
+// What a tile's stage loads into registers before its data goes to LDS: the
+// element slot records and the positions / rest positions of every LDS node
+// (owned nodes first, then the tile's halo copies).
+template <int SPT, int LPT>
+struct Stage {
+    uint4 rc[SPT];
+    float2 x[LPT];
+    float2 X[LPT];
+    float4 j2s;  // J2 state of slot 0 when it is an owned slot
+};
+
+__device__ __forceinline__ bool tile_live(const TileDev& T, int G) {
+    const int local = G - T.s0;
+    return local >= -1 && local < T.horizon;
+}
+
+// Issue every stage load of tile T at global step G; none depends on another, so
+// they are in flight together (one HBM round trip for the whole stage).  Lanes
+// past the end re-read the last valid entry instead of branching around the
+// load: the loads stay unconditional, which lets the compiler track each one
+// precisely instead of waiting for all of them at the first control-flow join.
+template <int BLOCK, int SPT, int LPT>
+__device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, LPT>& st) {
+    const int tid = threadIdx.x;
+    const int rd = (G - T.s0 + 1) & 1;
+    const int nslots = T.n_own + T.n_halo;
+    const uint4* __restrict__ rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
+    if (nslots > 0) {
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) st.rc[k] = fw_ld(rec + min(tid + k * BLOCK, nslots - 1));
+    } else {
+#pragma unroll
+        for (int k = 0; k < SPT; ++k) st.rc[k] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    const int nn = T.n_nodes;
+    const int nl = nn + T.n_hnodes;  // >= 1: a tile owns at least one node
+    const float2* __restrict__ pos_rd = fw_at<const float2>(T.base, (rd ? T.o_pos[1] : T.o_pos[0])) + T.node0;
+    const float2* __restrict__ rest = fw_at<const float2>(T.base, T.o_rest) + T.node0;
+    const float2* __restrict__ hpos_rd = fw_at<const float2>(T.base, (rd ? T.o_hpos[1] : T.o_hpos[0])) + T.hnode_off - nn;
+    const float2* __restrict__ hrest = fw_at<const float2>(T.base, T.o_hrest) + T.hnode_off - nn;
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+        const int j = min(tid + k * BLOCK, nl - 1);
+        const bool own = j < nn;
+        st.x[k] = fw_ld((own ? pos_rd : hpos_rd) + j);
+        st.X[k] = fw_ld((own ? rest : hrest) + j);
+    }
+    st.j2s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (T.o_j2[0] != 0 && T.n_own > 0)
+        st.j2s = fw_ld(fw_at<const float4>(T.base, rd ? T.o_j2[1] : T.o_j2[0]) + T.elem0 + min(tid, T.n_own - 1));
+}
+
 // BLOCK threads; at most SLOTS element slots, NODES LDS nodes (owned + halo) and
 // NPT * BLOCK owned nodes per tile.  LDS: 16 B per node + 24 B per slot, nothing
 // else (the AABB reduction reuses the rest-position array once phase A is done),
 // so 1024 slots take exactly 40 KiB and four tiles fit one CU's 160 KiB.
+//
+// A workgroup loops over tiles (grid-stride, XCD-aware).  Each tile starts with
+// one batch of independent loads (records, positions, rest positions) whose
+// addresses come from the tile record alone, read with scalar loads.
 template <int BLOCK, int SLOTS, int NODES, int NPT>
 __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     constexpr int SPT = (SLOTS + BLOCK - 1) / BLOCK;
+    constexpr int LPT = (NODES + BLOCK - 1) / BLOCK;
     __shared__ float2 lds_x[NODES];
     __shared__ float2 lds_X[NODES];
     __shared__ float2 lds_f[SLOTS * 3];
     static_assert((BLOCK / 64) * 4 * sizeof(int32_t) <= NODES * sizeof(float2), "reduction scratch");
     int32_t* lds_red = reinterpret_cast<int32_t*>(lds_X);
 
-    const int G = A.ctl->clock + A.k;
-    const int ntiles = A.tile_count >= 0 ? A.tile_count : (A.ctl->n_tiles - A.tile_base);
+    const int G = fw_ldc1(&A.ctl->clock) + A.k;
+    const int ntiles = A.tile_count >= 0 ? A.tile_count : (fw_ldc1(&A.ctl->n_tiles) - A.tile_base);
     const int tid = threadIdx.x;
+    const bool diag = (A.flags & STEP_DIAG) != 0;
 
     for (int round = 0;; ++round) {
         const int ti = tile_of_block(round, ntiles);
         if (ti >= ntiles) break;
-        const TileDev T = A.tiles[A.tile_base + ti];  // a copy: stores below cannot alias it
+        const TileDev T = fw_ldc(A.tiles + A.tile_base + ti);  // scalar loads; stores below cannot alias it
+        if (!tile_live(T, G)) continue;  // uniform per block
+        Stage<SPT, LPT> st;
+        stage_load<BLOCK>(T, G, st);
         const int local = G - T.s0;
-        if (local < -1 || local >= T.horizon) continue;  // uniform per block
-
-        const int rd = (local + 1) & 1;
         const int wr = local & 1;
         const int nslots = T.n_own + T.n_halo;
         const int n_nodes = T.n_nodes;
         const bool j2 = T.o_j2[0] != 0;
+        const float4* __restrict__ j2_rd = fw_at<const float4>(T.base, (local + 1) & 1 ? T.o_j2[1] : T.o_j2[0]);
 
-        // ---------------- stage: every load below depends on the tile record only
-        const uint4* __restrict__ rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
-        uint4 rc[SPT];
+        // ---------------- stage: registers -> LDS
 #pragma unroll
-        for (int k = 0; k < SPT; ++k) {
-            const int sl = tid + k * BLOCK;
-            rc[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (sl < nslots) rc[k] = fw_ld(rec + sl);
-        }
-        NodeIn nd[NPT];
-        {
-            const float2* __restrict__ pos_rd = fw_at<const float2>(T.base, (rd ? T.o_pos[1] : T.o_pos[0])) + T.node0;
-            const float2* __restrict__ rest = fw_at<const float2>(T.base, T.o_rest) + T.node0;
-            for (int i = tid; i < n_nodes; i += BLOCK) {
-                lds_x[i] = fw_ld(pos_rd + i);
-                lds_X[i] = fw_ld(rest + i);
-            }
-            const float2* __restrict__ hpos_rd = fw_at<const float2>(T.base, (rd ? T.o_hpos[1] : T.o_hpos[0])) + T.hnode_off;
-            const float2* __restrict__ hrest = fw_at<const float2>(T.base, T.o_hrest) + T.hnode_off;
-            for (int h = tid; h < T.n_hnodes; h += BLOCK) {
-                lds_x[n_nodes + h] = fw_ld(hpos_rd + h);
-                lds_X[n_nodes + h] = fw_ld(hrest + h);
+        for (int k = 0; k < LPT; ++k) {
+            const int j = tid + k * BLOCK;
+            if (j < n_nodes + T.n_hnodes) {
+                lds_x[j] = st.x[k];
+                lds_X[j] = st.X[k];
             }
         }
-        // J2 state of slot 0 (halo slots take the element id from the record); the
-        // other slots' states are fetched one slot ahead inside phase A
-        const float4* __restrict__ j2_rd = fw_at<const float4>(T.base, (rd ? T.o_j2[1] : T.o_j2[0]));
-        float4 st_cur = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (j2 && tid < nslots) st_cur = fw_ld(j2_rd + (tid < T.n_own ? T.elem0 + tid : (int)rc[0].w));
+        // J2 state of slot 0: owned slots loaded it with the stage batch; a halo
+        // slot (tiles with fewer owned elements than threads) takes the element id
+        // from its record.  The other slots' states are fetched one slot ahead
+        // inside phase A.
+        float4 st_cur = st.j2s;  // lanes past the owned slots hold a copy; unused there
+        if (j2 && tid >= T.n_own && tid < nslots) {
+            // an opaque copy keeps the compiler from widening the element id right
+            // after its load, which would wait for the first record before the
+            // rest of the stage batch is issued
+            uint32_t e0 = st.rc[0].w;
+            __asm__ volatile("" : "+v"(e0));
+            st_cur = fw_ld(j2_rd + e0);
+        }
 
-        // scene scalars (needed from phase A on)
-        const SceneDev& S = A.scenes[T.scene];
+        // scene scalars (needed from phase A on), scalar loads
+        const SceneDev* Sp = A.scenes + T.scene;
         const bool warm = local == -1;
-        const float dt = warm ? 0.0f : (A.dt_override >= 0.0f ? A.dt_override : S.dt);
-        const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : S.damping);
-        const bool diag = (A.flags & STEP_DIAG) != 0;
-        const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && S.n_grounds > 0;
+        const float dt = warm ? 0.0f : (A.dt_override >= 0.0f ? A.dt_override : fw_ldc1(&Sp->dt));
+        const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : fw_ldc1(&Sp->damping));
+        const int n_grounds = fw_ldc1(&Sp->n_grounds);
+        const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && n_grounds > 0;
         Mat m;
-        m.type = S.mat_type;
-        m.lam = S.E * S.nu / ((1.0f + S.nu) * (1.0f - 2.0f * S.nu));
-        m.mu = S.E / (2.0f * (1.0f + S.nu));
-        m.sy = S.sigma_y;
-        m.H = S.H;
-        __syncthreads();
+        {
+            const float E = fw_ldc1(&Sp->E), nu = fw_ldc1(&Sp->nu);
+            m.type = fw_ldc1(&Sp->mat_type);
+            m.lam = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+            m.mu = E / (2.0f * (1.0f + nu));
+            m.sy = fw_ldc1(&Sp->sigma_y);
+            m.H = fw_ldc1(&Sp->H);
+        }
+        float4* const st_stress = diag ? fw_ldc1(&Sp->stress) : nullptr;
+        float4* const st_strain = diag ? fw_ldc1(&Sp->strain) : nullptr;
+        uint8_t* const st_yflag = diag ? fw_ldc1(&Sp->yflag) : nullptr;
+        lds_barrier();
 
         // ---------------- phase A: element slots from LDS
         // Registers are the occupancy limit here: the next slot's J2 state is
         // fetched one slot ahead, and the nodes' inputs for phase B are fetched two
         // slots before the end, so that they land during the last slots and the gate.
         constexpr int ND_AT = SPT >= 2 ? SPT - 2 : 0;
+        NodeIn nd[NPT];
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int sl = tid + k * BLOCK;
             float4 st_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (k + 1 < SPT && j2 && sl + BLOCK < nslots)
-                st_next = fw_ld(j2_rd + (sl + BLOCK < T.n_own ? T.elem0 + sl + BLOCK : (int)rc[k + 1 < SPT ? k + 1 : k].w));
+                st_next = fw_ld(j2_rd + (sl + BLOCK < T.n_own ? T.elem0 + sl + BLOCK : (int)st.rc[k + 1 < SPT ? k + 1 : k].w));
             if (k == ND_AT) {
 #pragma unroll
                 for (int kn = 0; kn < NPT; ++kn) {
@@ -238,10 +297,10 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             }
             if (sl < nslots) {
                 uint32_t l[3], q[3];
-                unpack_rec(rc[k], l, q);
+                unpack_rec(st.rc[k], l, q);
                 slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st_cur, q,
                            sl < T.n_own, T.elem0 + sl, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])),
-                           S.stress, S.strain, S.yflag, lds_f);
+                           st_stress, st_strain, st_yflag, lds_f);
             }
             st_cur = st_next;
         }
@@ -250,9 +309,9 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         uint32_t active = 0;
         if (contact) {
             const int ar3 = mod3(local);
-            float lbx = fw_o2f(S.aabb[ar3][0]), lby = fw_o2f(S.aabb[ar3][1]);
-            float ubx = fw_o2f(S.aabb[ar3][2]), uby = fw_o2f(S.aabb[ar3][3]);
-            const bool pad = ((local - S.gate_base) % 100) == 0;
+            float lbx = fw_o2f(fw_ldc1(&Sp->aabb[ar3][0])), lby = fw_o2f(fw_ldc1(&Sp->aabb[ar3][1]));
+            float ubx = fw_o2f(fw_ldc1(&Sp->aabb[ar3][2])), uby = fw_o2f(fw_ldc1(&Sp->aabb[ar3][3]));
+            const bool pad = ((local - fw_ldc1(&Sp->gate_base)) % 100) == 0;
             if (pad) {
                 const float bx = 0.1f * (ubx - lbx), by = 0.1f * (uby - lby);
                 lbx = lbx - bx;
@@ -260,24 +319,32 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 lby = lby - by;
                 uby = uby + by;
             }
-            for (int gi = 0; gi < S.n_grounds; ++gi) {
-                const GroundDev& gd = S.ground[gi];
+            for (int gi = 0; gi < n_grounds; ++gi) {
+                const GroundDev gd = fw_ldc(&Sp->ground[gi]);
                 const float* lo = pad ? gd.plo : gd.lo;
                 const float* hi = pad ? gd.phi : gd.hi;
                 if (lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1]) active |= 1u << gi;
             }
         }
-        __syncthreads();  // lds_f complete; lds_X free from here on
+        lds_barrier();  // lds_f complete; lds_X free from here on
 
         // ---------------- phase B: owned nodes
         // first push entry per thread, loaded now so that it lands during phase B
         int2 pe = make_int2(0, 0);
         if (tid < T.n_push) pe = fw_ld(fw_at<const int2>(T.base, T.o_push) + T.push_off + tid);
+        const float2* __restrict__ bcv = fw_ldc1(&Sp->bcv);
+        float2 bv[NPT];
+#pragma unroll
+        for (int k = 0; k < NPT; ++k) {
+            bv[k] = make_float2(0.0f, 0.0f);
+            if (tid + k * BLOCK < n_nodes) bv[k] = fw_ld(bcv + (nd[k].fl >> NODE_BCV_SHIFT));
+        }
+        float2* const st_force = diag ? fw_ldc1(&Sp->force) : nullptr;
+        uint8_t* const st_cflag = diag ? fw_ldc1(&Sp->cflag) : nullptr;
         int32_t bb[4] = {fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)};
         float2* __restrict__ pos_wr = fw_at<float2>(T.base, (wr ? T.o_pos[1] : T.o_pos[0]));
         float2* __restrict__ vel = fw_at<float2>(T.base, T.o_vel);
         float2* __restrict__ disp = T.o_disp ? fw_at<float2>(T.base, T.o_disp) : nullptr;
-        const float2* __restrict__ bcv = S.bcv;
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
             const int i = tid + k * BLOCK;
@@ -290,8 +357,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             float fx = 0.0f, fy = 0.0f;
             bool hit = false;
             if (active && (fl & NODE_BOUNDARY)) {
-                for (int gi = 0; gi < S.n_grounds; ++gi)
-                    if (active & (1u << gi)) hit |= ground_force(S.ground[gi], p.x, p.y, v.x, v.y, dt, fx, fy);
+                for (int gi = 0; gi < n_grounds; ++gi)
+                    if (active & (1u << gi)) hit |= ground_force(fw_ldc(&Sp->ground[gi]), p.x, p.y, v.x, v.y, dt, fx, fy);
             }
             // element forces in ascending element id: a contiguous run of LDS entries
             for (int j = (int)(nd[k].jr & 0xFFFFu); j < (int)(nd[k].jr >> 16); ++j) {
@@ -300,19 +367,18 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 fy = fy + (-f.y);
             }
             // symplectic Euler with BC precedence (flatworld/femspringmanager.py:315-358)
-            const float2 bv = fw_ld(bcv + (fl >> NODE_BCV_SHIFT));
             if (fl & BC_U) {
                 fx = 0.0f;
                 fy = 0.0f;
             } else if (fl & BC_A) {
-                fx = mass * bv.x;
-                fy = mass * bv.y;
+                fx = mass * bv[k].x;
+                fy = mass * bv[k].y;
             } else if (fl & BC_G) {
-                fx = fx + mass * bv.x;
-                fy = fy + mass * bv.y;
+                fx = fx + mass * bv[k].x;
+                fy = fy + mass * bv[k].y;
             } else if (fl & BC_F) {
-                fx = fx + bv.x;
-                fy = fy + bv.y;
+                fx = fx + bv[k].x;
+                fy = fy + bv[k].y;
             }
             const float rm = 1.0f / mass;
             const float ax = 0.0f + (fx - damping * v.x) * rm;
@@ -320,8 +386,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             float vx = v.x + ax * dt;
             float vy = v.y + ay * dt;
             if (fl & BC_V) {
-                vx = bv.x;
-                vy = bv.y;
+                vx = bv[k].x;
+                vy = bv[k].y;
             } else if (fl & BC_U) {
                 vx = 0.0f;
                 vy = 0.0f;
@@ -333,8 +399,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             fw_st(pos_wr + n, make_float2(nx_, ny_));
             lds_x[i] = make_float2(nx_, ny_);  // source of the halo copies pushed below
             if (diag) {
-                fw_st(S.force + n, make_float2(fx, fy));
-                fw_st(S.cflag + n, (uint8_t)(hit ? 1 : 0));
+                fw_st(st_force + n, make_float2(fx, fy));
+                fw_st(st_cflag + n, (uint8_t)(hit ? 1 : 0));
             }
             if (fl & NODE_BOUNDARY) {
                 const int32_t ox = fw_f2o(nx_), oy = fw_f2o(ny_);
@@ -348,8 +414,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // ---------------- boundary AABB at x_{n+1} (flatworld/femspringmanager.py:361-408)
         block_minmax<BLOCK>(bb, lds_red);  // its barrier also publishes the lds_x updates
         if (tid == 0) {
-            int32_t* aw = const_cast<int32_t*>(S.aabb[mod3(local + 1)]);
-            int32_t* az = const_cast<int32_t*>(S.aabb[mod3(local + 2)]);
+            int32_t* aw = const_cast<SceneDev*>(Sp)->aabb[mod3(local + 1)];
+            int32_t* az = const_cast<SceneDev*>(Sp)->aabb[mod3(local + 2)];
             if (T.multi) {
                 atomicMin(aw + 0, bb[0]);
                 atomicMin(aw + 1, bb[1]);
@@ -376,7 +442,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 fw_st(hpos_wr + e.y, lds_x[e.x]);
             }
         }
-        __syncthreads();  // LDS reuse by the next tile of this block
+        lds_barrier();  // LDS reuse by the next tile of this block
     }
 }
 

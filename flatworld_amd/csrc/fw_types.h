@@ -127,6 +127,27 @@ __device__ inline void fw_st(float* p, float a) { *(FW_GLOBAL float*)p = a; }
 __device__ inline void fw_st(int32_t* p, int32_t a) { *(FW_GLOBAL int32_t*)p = a; }
 __device__ inline void fw_st(uint8_t* p, uint8_t a) { *(FW_GLOBAL uint8_t*)p = a; }
 
+// Tile and scene tables are never written while a kernel runs (the host edits
+// them between launches, and a kernel start invalidates the scalar cache), so
+// the kernels read them through the constant address space: with a uniform
+// address that is one batch of scalar loads instead of vector loads followed by
+// v_readfirstlane.  fw_ldc copies a whole record, fw_ldc1 one field.
+#define FW_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ inline T fw_ldc(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "record of whole dwords");
+    T out;
+    const FW_CONST uint32_t* s = (const FW_CONST uint32_t*)p;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = s[i];
+    return out;
+}
+template <typename T>
+__device__ inline T fw_ldc1(const T* p) {
+    return *(const FW_CONST T*)p;
+}
+
 template <typename P>
 __device__ inline P* fw_at(char* base, uint32_t off) {
     return reinterpret_cast<P*>(base + ((size_t)off << 8));
