@@ -35,7 +35,7 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
                                         int tile_base, int tile_count, float dt_override, float damping_override,
                                         uint32_t flags, int grid, int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
-                                        int slots, hipStream_t stream);
+                                        int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
 extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
@@ -67,7 +67,8 @@ struct Tiling {
     std::vector<int32_t> elem_perm;  // internal -> caller element id
     std::vector<int32_t> elem_inv;
     std::vector<TileDev> tiles;      // pointers and scene fields left 0; offsets relative to the scene
-    std::vector<uint32_t> rec;       // 4 per slot (one uint4): local corner ids, LDS destinations, element
+    std::vector<uint32_t> rec;       // 2 per slot (one uint2): corner words (local node id | incidence rank)
+    std::vector<int32_t> hel;        // per halo slot: storage element index
     std::vector<int32_t> hnode;      // per tile: storage ids of the halo nodes, ascending
     std::vector<uint16_t> csr_ptr;
     std::vector<int32_t> push;       // 2 per halo copy, grouped by source tile: (local owned node, copy index)
@@ -264,35 +265,45 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         td.hnode_off = (int32_t)T.hnode.size();
         td.n_hnodes = (int32_t)hn.size();
         T.hnode.insert(T.hnode.end(), hn.begin(), hn.end());
-        // slot records: local ids of the three corners, LDS destinations, storage element index
-        td.rec_off = (int32_t)(T.rec.size() / 4);
-        T.rec.resize(T.rec.size() + 4 * (size_t)nslots);
-        uint32_t* rec = T.rec.data() + 4 * (size_t)td.rec_off;
-        std::vector<uint16_t> dst(3 * (size_t)nslots, 0xFFFF);
         // node-major incidence positions: node i of the tile owns LDS entries
-        // [ptr[i], ptr[i+1]) in ascending (caller element id, corner)
+        // [ptr[i], ptr[i+1]) in ascending (caller element id, corner); a corner's
+        // rank is its position in its node's list
+        std::vector<uint8_t> rank(3 * (size_t)nslots, 0);
         td.ptr_off = (int32_t)T.csr_ptr.size();
         int cnt = 0;
         T.csr_ptr.push_back(0);
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
+            if (inc_ptr[n + 1] - inc_ptr[n] > FW_REC_MAX_RANK + 1)
+                return fail(FW_E_UNSUPPORTED, "node " + std::to_string(n) + " has " +
+                                                  std::to_string(inc_ptr[n + 1] - inc_ptr[n]) + " element corners; the substep kernel "
+                                                  "supports at most " + std::to_string(FW_REC_MAX_RANK + 1));
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
                 int e = inc[j] / 3, c = inc[j] % 3;
-                dst[3 * (size_t)slot_of[e] + c] = (uint16_t)cnt;
+                rank[3 * (size_t)slot_of[e] + c] = (uint8_t)(j - inc_ptr[n]);
                 ++cnt;
             }
             if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
             T.csr_ptr.push_back((uint16_t)cnt);
         }
+        // slot records: corner words (local id | rank << FW_REC_ID_BITS); halo slots'
+        // storage element indices in their own list
+        td.rec_off = (int32_t)(T.rec.size() / 2);
+        T.rec.resize(T.rec.size() + 2 * (size_t)nslots);
+        uint32_t* rec = T.rec.data() + 2 * (size_t)td.rec_off;
         for (int s = 0; s < nslots; ++s) {
             const int e = T.elem_perm[slots[s]];
-            const uint32_t l0 = (uint32_t)lid[d->conn[3 * e]], l1 = (uint32_t)lid[d->conn[3 * e + 1]],
-                           l2 = (uint32_t)lid[d->conn[3 * e + 2]];
-            rec[4 * s + 0] = l0 | (l1 << 16);
-            rec[4 * s + 1] = l2 | ((uint32_t)dst[3 * s] << 16);
-            rec[4 * s + 2] = (uint32_t)dst[3 * s + 1] | ((uint32_t)dst[3 * s + 2] << 16);
-            rec[4 * s + 3] = (uint32_t)slots[s];
+            uint32_t w[3];
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t l = (uint32_t)lid[d->conn[3 * e + c]];
+                if (l > FW_REC_ID_MASK) return fail(FW_E_INVALID, "internal tiling error: local id overflow");
+                w[c] = l | ((uint32_t)rank[3 * (size_t)s + c] << FW_REC_ID_BITS);
+            }
+            rec[2 * s + 0] = w[0] | (w[1] << 16);
+            rec[2 * s + 1] = w[2];
         }
+        td.hel_off = (int32_t)T.hel.size();
+        for (int s = td.n_own; s < nslots; ++s) T.hel.push_back(slots[s]);
     }
     // push lists: every halo copy is written by the tile owning its node, right after that
     // tile computes the node's new position
@@ -320,12 +331,13 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
 struct Layout {
     int64_t pos0, pos1, rest, vel, mass, flag, bcv, disp, force, cflag;
     int64_t jinv, area, j2a, j2b, stress, strain, yflag;
-    int64_t rec, hpos0, hpos1, hrest, push, cptr, tiles;
+    int64_t rec, hel, hpos0, hpos1, hrest, push, cptr, tiles;
     int64_t gdata[FW_MAX_GROUNDS];
     int64_t total;
 };
 
-Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hnode, int64_t n_ptr, int64_t n_tiles) {
+Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, int64_t n_hnode, int64_t n_ptr,
+                   int64_t n_tiles) {
     Layout L;
     int64_t o = 0;
     auto take = [&](int64_t bytes) {
@@ -355,7 +367,8 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hnode, int
     L.stress = take(16 * E);
     L.strain = take(16 * E);
     L.yflag = take(E);
-    L.rec = take(16 * n_slots);
+    L.rec = take(8 * n_slots);
+    L.hel = take(4 * std::max<int64_t>(n_hslots, 1));
     L.hpos0 = take(8 * n_hnode);
     L.hpos1 = take(8 * n_hnode);
     L.hrest = take(8 * n_hnode);
@@ -494,7 +507,16 @@ struct fw_arena {
 
 namespace {
 
-int node_cap_of(const fw_tuning& t) { return t.tile_slots; }
+// LDS nodes (owned + halo) a tile of the compiled shapes holds: FW_SHAPES gives the
+// same count for every thread count of a slot count.
+int node_cap_of_slots(int slots) {
+#define FW_CAP(B, S, N, P) \
+    if (slots == S) return N;
+    FW_SHAPES(FW_CAP)
+#undef FW_CAP
+    return 0;
+}
+int node_cap_of(const fw_tuning& t) { return node_cap_of_slots(t.tile_slots); }
 
 // Tiling of a scene description, from the cache when the same arrays are presented again.
 int tiling_for(fw_arena* a, const fw_scene_desc* d, std::unique_ptr<Tiling>* out) {
@@ -673,11 +695,10 @@ int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning*
     if (tuning) {
         // kernel shapes compiled in fw_kernels.hip: (threads, slots) -> owned nodes per thread
         int npt = 0;
-        if (tuning->block == 512 && tuning->tile_slots == 1024) npt = 1;
-        if (tuning->block == 256 && tuning->tile_slots == 1024) npt = 2;
-        if (tuning->block == 256 && tuning->tile_slots == 512) npt = 1;
-        if (tuning->block == 512 && tuning->tile_slots == 2048) npt = 2;
-        if (tuning->block == 1024 && tuning->tile_slots == 2048) npt = 1;
+#define FW_NPT(B, S, N, P) \
+    if (tuning->block == B && tuning->tile_slots == S) npt = P;
+        FW_SHAPES(FW_NPT)
+#undef FW_NPT
         if (!npt)
             return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (512,1024) (256,1024) "
                                       "(256,512) (512,2048) (1024,2048)");
@@ -748,7 +769,7 @@ int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
 }
 
 static Layout layout_of(const fw_scene_desc* d, const Tiling& T) {
-    return make_layout(d, (int64_t)(T.rec.size() / 4), (int64_t)T.hnode.size(), (int64_t)T.csr_ptr.size(),
+    return make_layout(d, (int64_t)(T.rec.size() / 2), (int64_t)T.hel.size(), (int64_t)T.hnode.size(), (int64_t)T.csr_ptr.size(),
                        (int64_t)T.tiles.size());
 }
 
@@ -852,6 +873,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     for (auto& td : T.tiles) {
         td.base = blk;
         td.o_rec = o8(L.rec);
+        td.o_hel = o8(L.hel);
         td.o_pos[0] = o8(L.pos0);
         td.o_pos[1] = o8(L.pos1);
         td.o_hpos[0] = o8(L.hpos0);
@@ -899,6 +921,7 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     }
     std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
     std::memcpy(img + L.rec, T.rec.data(), 4 * T.rec.size());
+    std::memcpy(img + L.hel, T.hel.data(), 4 * T.hel.size());
     std::memcpy(img + L.push, T.push.data(), 4 * T.push.size());
     std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
     std::memcpy(img + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
@@ -943,7 +966,8 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
 
     rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
     if (rc) return rc;
-    HIPCHK(fw_launch_prepare(a->d_scenes, (const TileDev*)(blk + L.tiles), 0, ntiles, a->tune.tile_slots, a->stream));
+    HIPCHK(fw_launch_prepare(a->d_scenes, (const TileDev*)(blk + L.tiles), 0, ntiles, a->tune.block, a->tune.tile_slots,
+                             a->stream));
 
     sh->live = true;
     sh->block_off = off;
@@ -1037,7 +1061,8 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
     int rc = validate(d);
     if (rc) return rc;
     Tiling T;
-    const int node_cap = tile_slots;
+    const int node_cap = node_cap_of_slots(tile_slots);
+    if (node_cap == 0) return fail(FW_E_INVALID, "tile_slots must be one of the compiled shapes (512, 1024, 2048)");
     rc = build_tiling(d, tile_nodes, tile_slots, node_cap, T);
     if (rc) return rc;
     const int N = d->n_nodes, E = d->n_elems;
@@ -1060,18 +1085,26 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
         const int nslots = td.n_own + td.n_halo;
         const uint16_t* cp = T.csr_ptr.data() + td.ptr_off;
         std::vector<int64_t> at(cp[td.n_nodes], -1);
+        int prev_halo = -1;
         for (int sl = 0; sl < nslots; ++sl) {
-            const uint32_t* r = T.rec.data() + 4 * ((size_t)td.rec_off + sl);
-            const int eint = (int)r[3];
-            if (sl < td.n_own && eint != td.elem0 + sl) return fail(FW_E_INVALID, "owned slot out of order");
+            const uint32_t* r = T.rec.data() + 2 * ((size_t)td.rec_off + sl);
+            int eint = td.elem0 + sl;
+            if (sl >= td.n_own) {
+                eint = T.hel[(size_t)td.hel_off + sl - td.n_own];
+                if (eint <= prev_halo || (eint >= td.elem0 && eint < td.elem0 + td.n_own))
+                    return fail(FW_E_INVALID, "halo slots not ascending or name an owned element");
+                prev_halo = eint;
+            }
             const int ecaller = T.elem_perm[eint];
-            const uint32_t lids[3] = {r[0] & 0xFFFFu, r[0] >> 16, r[1] & 0xFFFFu};
-            const uint32_t dsts[3] = {r[1] >> 16, r[2] & 0xFFFFu, r[2] >> 16};
+            if (r[1] >> 16) return fail(FW_E_INVALID, "slot record has stray bits");
+            const uint32_t words[3] = {r[0] & 0xFFFFu, r[0] >> 16, r[1] & 0xFFFFu};
             for (int c = 0; c < 3; ++c) {
-                if (lids[c] >= local.size() || T.node_perm[local[lids[c]]] != d->conn[3 * ecaller + c])
+                const uint32_t lidc = words[c] & FW_REC_ID_MASK;
+                if (lidc >= local.size() || T.node_perm[local[lidc]] != d->conn[3 * ecaller + c])
                     return fail(FW_E_INVALID, "slot record names the wrong node");
-                const uint32_t pos = dsts[c];
-                if (pos == 0xFFFF) continue;
+                if ((int)lidc >= td.n_nodes) continue;  // halo node: its owner tile assembles it
+                const uint32_t pos = (uint32_t)cp[lidc] + (words[c] >> FW_REC_ID_BITS);
+                if (pos >= (uint32_t)cp[lidc + 1]) return fail(FW_E_INVALID, "corner rank beyond its node's list");
                 if (pos >= at.size() || at[pos] != -1) return fail(FW_E_INVALID, "slot destinations collide");
                 at[pos] = 3 * (int64_t)ecaller + c;
             }

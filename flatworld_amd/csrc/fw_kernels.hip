@@ -47,13 +47,16 @@ __device__ __forceinline__ void rest_frame(float2 X1, float2 X2, float2 X3, floa
     ji = make_float4(J3 * inv_det, -J1 * inv_det, -J2 * inv_det, J0 * inv_det);
 }
 
-__device__ __forceinline__ void unpack_rec(uint4 r, uint32_t l[3], uint32_t q[3]) {
-    l[0] = r.x & 0xFFFFu;
-    l[1] = r.x >> 16;
-    l[2] = r.y & 0xFFFFu;
-    q[0] = r.y >> 16;
-    q[1] = r.z & 0xFFFFu;
-    q[2] = r.z >> 16;
+// Corner words of a slot record: local node ids, and for corners at owned nodes
+// the LDS entry of the corner force, cptr[l] + rank (0xFFFF for halo nodes).
+__device__ __forceinline__ void unpack_rec(uint2 r, int n_own_nodes, const uint16_t* lds_cp, uint32_t l[3],
+                                           uint32_t q[3]) {
+    const uint32_t w[3] = {r.x & 0xFFFFu, r.x >> 16, r.y & 0xFFFFu};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        l[c] = w[c] & FW_REC_ID_MASK;
+        q[c] = (int)l[c] < n_own_nodes ? (uint32_t)lds_cp[l[c]] + (w[c] >> FW_REC_ID_BITS) : 0xFFFFu;
+    }
 }
 
 // One element slot: F = Jac(x) J^-1, stress, corner forces area * B^T S
@@ -131,18 +134,20 @@ struct NodeIn {
     float2 u;
     float m;
     uint32_t fl;
-    uint32_t jr;  // CSR range: begin | end << 16 (one register)
 };
 
 // What a tile's stage loads into registers before its data goes to LDS: the
-// element slot records and the positions / rest positions of every LDS node
-// (owned nodes first, then the tile's halo copies).
-template <int SPT, int LPT>
+// element slot records, the positions / rest positions of every LDS node (owned
+// nodes first, then the tile's halo copies), the owned nodes' CSR offsets, and
+// for J2 tiles the state of slot 0 and the element ids of halo slots.
+template <int SPT, int LPT, int CPT>
 struct Stage {
-    uint4 rc[SPT];
+    uint2 rc[SPT];
     float2 x[LPT];
     float2 X[LPT];
-    float4 j2s;  // J2 state of slot 0 when it is an owned slot
+    uint32_t cp[CPT];
+    float4 j2s;       // J2 state of slot 0 when it is an owned slot
+    int32_t he[SPT];  // J2 tiles: element of each halo slot
 };
 
 __device__ __forceinline__ bool tile_live(const TileDev& T, int G) {
@@ -155,18 +160,18 @@ __device__ __forceinline__ bool tile_live(const TileDev& T, int G) {
 // past the end re-read the last valid entry instead of branching around the
 // load: the loads stay unconditional, which lets the compiler track each one
 // precisely instead of waiting for all of them at the first control-flow join.
-template <int BLOCK, int SPT, int LPT>
-__device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, LPT>& st) {
+template <int BLOCK, int SPT, int LPT, int CPT>
+__device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, LPT, CPT>& st) {
     const int tid = threadIdx.x;
     const int rd = (G - T.s0 + 1) & 1;
     const int nslots = T.n_own + T.n_halo;
-    const uint4* __restrict__ rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
+    const uint2* __restrict__ rec = fw_at<const uint2>(T.base, T.o_rec) + T.rec_off;
     if (nslots > 0) {
 #pragma unroll
         for (int k = 0; k < SPT; ++k) st.rc[k] = fw_ld(rec + min(tid + k * BLOCK, nslots - 1));
     } else {
 #pragma unroll
-        for (int k = 0; k < SPT; ++k) st.rc[k] = make_uint4(0u, 0u, 0u, 0u);
+        for (int k = 0; k < SPT; ++k) st.rc[k] = make_uint2(0u, 0u);
     }
     const int nn = T.n_nodes;
     const int nl = nn + T.n_hnodes;  // >= 1: a tile owns at least one node
@@ -181,9 +186,21 @@ __device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, L
         st.x[k] = fw_ld((own ? pos_rd : hpos_rd) + j);
         st.X[k] = fw_ld((own ? rest : hrest) + j);
     }
+    const uint16_t* __restrict__ cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) st.cp[k] = fw_ld(cptr + min(tid + k * BLOCK, nn));
     st.j2s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (T.o_j2[0] != 0 && T.n_own > 0)
-        st.j2s = fw_ld(fw_at<const float4>(T.base, rd ? T.o_j2[1] : T.o_j2[0]) + T.elem0 + min(tid, T.n_own - 1));
+#pragma unroll
+    for (int k = 0; k < SPT; ++k) st.he[k] = 0;
+    if (T.o_j2[0] != 0) {
+        if (T.n_own > 0)
+            st.j2s = fw_ld(fw_at<const float4>(T.base, rd ? T.o_j2[1] : T.o_j2[0]) + T.elem0 + min(tid, T.n_own - 1));
+        if (T.n_halo > 0) {
+            const int32_t* __restrict__ hel = fw_at<const int32_t>(T.base, T.o_hel) + T.hel_off;
+#pragma unroll
+            for (int k = 0; k < SPT; ++k) st.he[k] = fw_ld(hel + min(max(tid + k * BLOCK - T.n_own, 0), T.n_halo - 1));
+        }
+    }
 }
 
 // BLOCK threads; at most SLOTS element slots, NODES LDS nodes (owned + halo) and
@@ -198,10 +215,13 @@ template <int BLOCK, int SLOTS, int NODES, int NPT>
 __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     constexpr int SPT = (SLOTS + BLOCK - 1) / BLOCK;
     constexpr int LPT = (NODES + BLOCK - 1) / BLOCK;
+    constexpr int CPT = NPT + 1;  // CSR offsets per thread: owned nodes + 1
     __shared__ float2 lds_x[NODES];
     __shared__ float2 lds_X[NODES];
     __shared__ float2 lds_f[SLOTS * 3];
+    __shared__ uint16_t lds_cp[NPT * BLOCK + 1];
     static_assert((BLOCK / 64) * 4 * sizeof(int32_t) <= NODES * sizeof(float2), "reduction scratch");
+    static_assert(NODES <= (int)FW_REC_ID_MASK + 1, "local node ids fit the record");
     int32_t* lds_red = reinterpret_cast<int32_t*>(lds_X);
 
     const int G = fw_ldc1(&A.ctl->clock) + A.k;
@@ -214,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         if (ti >= ntiles) break;
         const TileDev T = fw_ldc(A.tiles + A.tile_base + ti);  // scalar loads; stores below cannot alias it
         if (!tile_live(T, G)) continue;  // uniform per block
-        Stage<SPT, LPT> st;
+        Stage<SPT, LPT, CPT> st;
         stage_load<BLOCK>(T, G, st);
         const int local = G - T.s0;
         const int wr = local & 1;
@@ -232,19 +252,17 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 lds_X[j] = st.X[k];
             }
         }
-        // J2 state of slot 0: owned slots loaded it with the stage batch; a halo
-        // slot (tiles with fewer owned elements than threads) takes the element id
-        // from its record.  The other slots' states are fetched one slot ahead
-        // inside phase A.
-        float4 st_cur = st.j2s;  // lanes past the owned slots hold a copy; unused there
-        if (j2 && tid >= T.n_own && tid < nslots) {
-            // an opaque copy keeps the compiler from widening the element id right
-            // after its load, which would wait for the first record before the
-            // rest of the stage batch is issued
-            uint32_t e0 = st.rc[0].w;
-            __asm__ volatile("" : "+v"(e0));
-            st_cur = fw_ld(j2_rd + e0);
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+            const int j = tid + k * BLOCK;
+            if (j <= n_nodes) lds_cp[j] = (uint16_t)st.cp[k];
         }
+        // J2 state of slot 0: owned slots loaded it with the stage batch, a halo
+        // slot (tiles with fewer owned elements than threads) loads it by the
+        // element id the stage brought.  The other slots' states are fetched one
+        // slot ahead inside phase A.
+        float4 st_cur = st.j2s;  // lanes past the owned slots hold a copy; unused there
+        if (j2 && tid >= T.n_own && tid < nslots) st_cur = fw_ld(j2_rd + st.he[0]);
 
         // scene scalars (needed from phase A on), scalar loads
         const SceneDev* Sp = A.scenes + T.scene;
@@ -278,7 +296,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             const int sl = tid + k * BLOCK;
             float4 st_next = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (k + 1 < SPT && j2 && sl + BLOCK < nslots)
-                st_next = fw_ld(j2_rd + (sl + BLOCK < T.n_own ? T.elem0 + sl + BLOCK : (int)st.rc[k + 1 < SPT ? k + 1 : k].w));
+                st_next = fw_ld(j2_rd + (sl + BLOCK < T.n_own ? T.elem0 + sl + BLOCK : st.he[k + 1 < SPT ? k + 1 : k]));
             if (k == ND_AT) {
 #pragma unroll
                 for (int kn = 0; kn < NPT; ++kn) {
@@ -290,14 +308,12 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                         if (T.o_disp) nd[kn].u = fw_ld(fw_at<const float2>(T.base, T.o_disp) + n);
                         nd[kn].m = fw_ld(fw_at<const float>(T.base, T.o_mass) + n);
                         nd[kn].fl = fw_ld(fw_at<const uint32_t>(T.base, T.o_flag) + n);
-                        const uint16_t* cp = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off + i;
-                        nd[kn].jr = (uint32_t)fw_ld(cp) | ((uint32_t)fw_ld(cp + 1) << 16);
                     }
                 }
             }
             if (sl < nslots) {
                 uint32_t l[3], q[3];
-                unpack_rec(st.rc[k], l, q);
+                unpack_rec(st.rc[k], n_nodes, lds_cp, l, q);
                 slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st_cur, q,
                            sl < T.n_own, T.elem0 + sl, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])),
                            st_stress, st_strain, st_yflag, lds_f);
@@ -361,7 +377,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                     if (active & (1u << gi)) hit |= ground_force(fw_ldc(&Sp->ground[gi]), p.x, p.y, v.x, v.y, dt, fx, fy);
             }
             // element forces in ascending element id: a contiguous run of LDS entries
-            for (int j = (int)(nd[k].jr & 0xFFFFu); j < (int)(nd[k].jr >> 16); ++j) {
+            const int j_end = lds_cp[i + 1];
+            for (int j = lds_cp[i]; j < j_end; ++j) {
                 const float2 f = lds_f[j];
                 fx = fx + (-f.x);
                 fy = fy + (-f.y);
@@ -450,32 +467,35 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 
 // flatworld/femspringmanager.py:113-138: area = |det Jac| / 2, J^-1 = inverse(Jac),
 // lumped mass = sum over incident elements (ascending id) of area * rho / 3.
-template <int BLOCK, int SLOTS, int NODES>
+// One workgroup per tile; MAXN = most owned nodes a tile of this shape holds.
+template <int BLOCK, int SLOTS, int NODES, int MAXN>
 __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, const TileDev* tiles, int tile_base,
                                                         int tile_count) {
     __shared__ float2 lds_X[NODES];
     __shared__ float lds_m[SLOTS * 3];
+    __shared__ uint16_t lds_cp[MAXN + 1];
     const int ti = blockIdx.x;
     if (ti >= tile_count) return;
     const TileDev T = tiles[tile_base + ti];
     const SceneDev& S = scenes[T.scene];
-    const uint4* rec = fw_at<const uint4>(T.base, T.o_rec) + T.rec_off;
+    const uint2* rec = fw_at<const uint2>(T.base, T.o_rec) + T.rec_off;
     const float2* rest = fw_at<const float2>(T.base, T.o_rest);
     const float2* hrest = fw_at<const float2>(T.base, T.o_hrest);
+    const uint16_t* cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
     for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = fw_ld(rest + T.node0 + i);
     for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = fw_ld(hrest + T.hnode_off + h);
+    for (int i = threadIdx.x; i <= T.n_nodes; i += BLOCK) lds_cp[i] = fw_ld(cptr + i);
     __syncthreads();
     const int nslots = T.n_own + T.n_halo;
     for (int sl = threadIdx.x; sl < nslots; sl += BLOCK) {
-        const uint4 r = fw_ld(rec + sl);
         uint32_t l[3], q[3];
-        unpack_rec(r, l, q);
+        unpack_rec(fw_ld(rec + sl), T.n_nodes, lds_cp, l, q);
         float4 ji;
         float vol;
         rest_frame(lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], ji, vol);
         if (sl < T.n_own) {
-            fw_st(S.area + r.w, vol);
-            fw_st(S.jinv + r.w, ji);
+            fw_st(S.area + T.elem0 + sl, vol);
+            fw_st(S.jinv + T.elem0 + sl, ji);
         }
         const float nd = vol * S.rho / 3.0f;
         if (q[0] != 0xFFFFu) lds_m[q[0]] = nd;
@@ -483,11 +503,10 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
         if (q[2] != 0xFFFFu) lds_m[q[2]] = nd;
     }
     __syncthreads();
-    const uint16_t* cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
     float* mass = fw_at<float>(T.base, T.o_mass);
     for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
         float m = 0.0f;
-        for (int j = fw_ld(cptr + i); j < fw_ld(cptr + i + 1); ++j) m += lds_m[j];
+        for (int j = lds_cp[i]; j < lds_cp[i + 1]; ++j) m += lds_m[j];
         fw_st(mass + T.node0 + i, m);
     }
 }
@@ -507,59 +526,46 @@ __global__ void advance_clock_kernel(ArenaCtl* ctl, int n) {
 
 // ------------------------------------------------------------------ host launchers
 
-// Kernel shapes: (threads, element slots, LDS nodes, owned nodes per thread).
-// The host keeps tile_nodes <= NPT * threads and node_cap == slots.
+// Kernel shapes (FW_SHAPES): (threads, element slots, LDS nodes, owned nodes per
+// thread).  The host keeps tile_nodes <= NPT * threads and the LDS node count of a
+// tile <= the shape's LDS nodes.
 extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
                                         int tile_base, int tile_count, float dt_override, float damping_override,
                                         uint32_t flags, int grid, int block, int slots, hipStream_t stream) {
     fw::StepArgs A{scenes, tiles, ctl, k, tile_base, tile_count, dt_override, damping_override, flags};
-    if (block == 512 && slots == 1024)
-        hipLaunchKernelGGL((fw::substep_kernel<512, 1024, 1024, 1>), dim3(grid), dim3(512), 0, stream, A);
-    else if (block == 256 && slots == 1024)
-        hipLaunchKernelGGL((fw::substep_kernel<256, 1024, 1024, 2>), dim3(grid), dim3(256), 0, stream, A);
-    else if (block == 256 && slots == 512)
-        hipLaunchKernelGGL((fw::substep_kernel<256, 512, 512, 1>), dim3(grid), dim3(256), 0, stream, A);
-    else if (block == 512 && slots == 2048)
-        hipLaunchKernelGGL((fw::substep_kernel<512, 2048, 2048, 2>), dim3(grid), dim3(512), 0, stream, A);
-    else if (block == 1024 && slots == 2048)
-        hipLaunchKernelGGL((fw::substep_kernel<1024, 2048, 2048, 1>), dim3(grid), dim3(1024), 0, stream, A);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+#define FW_LAUNCH(B, S, N, P)                                                                              \
+    if (block == B && slots == S) {                                                                       \
+        hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P>), dim3(grid), dim3(B), 0, stream, A);         \
+        return hipGetLastError();                                                                         \
+    }
+    FW_SHAPES(FW_LAUNCH)
+#undef FW_LAUNCH
+    return hipErrorInvalidValue;
 }
 
 // Workgroups of the substep kernel resident per CU for a shape (HIP's occupancy
 // calculator: LDS, registers and wave slots).
 extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu) {
     *per_cu = 0;
-    if (block == 512 && slots == 1024)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<512, 1024, 1024, 1>, 512, 0);
-    if (block == 256 && slots == 1024)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<256, 1024, 1024, 2>, 256, 0);
-    if (block == 256 && slots == 512)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<256, 512, 512, 1>, 256, 0);
-    if (block == 512 && slots == 2048)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<512, 2048, 2048, 2>, 512, 0);
-    if (block == 1024 && slots == 2048)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<1024, 2048, 2048, 1>, 1024, 0);
+#define FW_OCC(B, S, N, P) \
+    if (block == B && slots == S) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<B, S, N, P>, B, 0);
+    FW_SHAPES(FW_OCC)
+#undef FW_OCC
     return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
-                                        int slots, hipStream_t stream) {
+                                        int block, int slots, hipStream_t stream) {
     if (tile_count <= 0) return hipSuccess;
-    if (slots == 512)
-        hipLaunchKernelGGL((fw::prepare_kernel<256, 512, 512>), dim3(tile_count), dim3(256), 0, stream, scenes, tiles,
-                           tile_base, tile_count);
-    else if (slots == 1024)
-        hipLaunchKernelGGL((fw::prepare_kernel<256, 1024, 1024>), dim3(tile_count), dim3(256), 0, stream, scenes,
-                           tiles, tile_base, tile_count);
-    else if (slots == 2048)
-        hipLaunchKernelGGL((fw::prepare_kernel<256, 2048, 2048>), dim3(tile_count), dim3(256), 0, stream, scenes,
-                           tiles, tile_base, tile_count);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+#define FW_PREP(B, S, N, P)                                                                                \
+    if (block == B && slots == S) {                                                                       \
+        hipLaunchKernelGGL((fw::prepare_kernel<256, S, N, B * P>), dim3(tile_count), dim3(256), 0, stream, \
+                           scenes, tiles, tile_base, tile_count);                                         \
+        return hipGetLastError();                                                                         \
+    }
+    FW_SHAPES(FW_PREP)
+#undef FW_PREP
+    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream) {

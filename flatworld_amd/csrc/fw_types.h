@@ -11,8 +11,9 @@
 //            recomputes both from the rest positions), j2[2] float4 (eps_xx, eps_yy,
 //            eps_xy, eqps; ping-pong, J2 scenes only), stress / strain float4 (diag),
 //            yflag uint8 (diag)
-//   tiles    rec uint4 per element slot, csr_ptr uint16 per owned node (+1),
-//            push int2 per halo copy, grouped by the tile owning the source node
+//   tiles    rec uint2 per element slot, hel int32 per halo slot, csr_ptr uint16 per
+//            owned node (+1), push int2 per halo copy, grouped by the tile owning the
+//            source node
 //   ground   height samples float / voxel cell extents uint16 x4
 #pragma once
 #include <hip/hip_runtime.h>
@@ -69,10 +70,12 @@ struct SceneDev {
 // for material, ground and diagnostics.
 struct TileDev {
     char* base;              // the scene's block; array = base + (offset << 8) (blocks are 256-byte aligned)
-    uint32_t o_rec;          // uint4 per element slot:
-    //   .x = l0 | l1 << 16, .y = l2 | d0 << 16, .z = d1 | d2 << 16, .w = element
-    //   l* local node ids of the corners (owned nodes first, then the tile's halo nodes),
-    //   d* LDS entry receiving that corner's force (0xFFFF: node not owned by the tile)
+    uint32_t o_rec;          // uint2 per element slot: .x = w0 | w1 << 16, .y = w2 (bits 16-31 zero);
+    //   corner word w = l | r << FW_REC_ID_BITS: l the local id of the corner's node (owned
+    //   nodes first, then the tile's halo nodes), r the rank of this (element, corner)
+    //   in the node's incidence list.  An owned node's corner force goes to LDS entry
+    //   cptr[l] + r; a halo node's (l >= n_nodes) is its owner tile's business.
+    uint32_t o_hel;          // int32 per halo slot: storage index of the element (J2 state)
     uint32_t o_pos[2];       // float2 node positions (ping-pong)
     uint32_t o_hpos[2];      // float2 halo copies (ping-pong)
     uint32_t o_rest;         // float2 rest positions
@@ -92,7 +95,25 @@ struct TileDev {
     int32_t hnode_off, n_hnodes;        // halo copies hpos[.][hnode_off .. +n_hnodes)
     int32_t ptr_off;                    // cptr[ptr_off .. ptr_off + n_nodes]
     int32_t push_off, n_push;           // push[push_off .. +n_push)
+    int32_t hel_off;                    // hel[hel_off + slot - n_own] for halo slots
+    int32_t pad_;
 };
+
+// Slot record corner word: local node id in the low bits, incidence rank above.
+#define FW_REC_ID_BITS 11
+#define FW_REC_ID_MASK 0x7FFu
+#define FW_REC_MAX_RANK 31
+
+// Compiled substep kernel shapes: (threads, element slots, LDS nodes, owned nodes per
+// thread).  LDS per workgroup = 16 B per LDS node (positions, rest positions) + 24 B
+// per slot (three corner forces) + 2 B per CSR offset (owned nodes + 1), sized so
+// that 4 (1024 slots), 8 (512) or 2 (2048) workgroups share a CU's 160 KiB.
+#define FW_SHAPES(X)         \
+    X(256, 1024, 952, 2)     \
+    X(512, 1024, 952, 1)     \
+    X(256, 512, 472, 1)      \
+    X(512, 2048, 1912, 2)    \
+    X(1024, 2048, 1912, 1)
 
 // HBM accesses of the kernels go through fw_ld / fw_st: they address global
 // memory explicitly.  A plain access through a pointer the compiler cannot prove
@@ -119,6 +140,7 @@ __device__ inline uint2 fw_ld(const uint2* p) { const fw_v2u v = *(const FW_GLOB
 __device__ inline int2 fw_ld(const int2* p) { const fw_v2i v = *(const FW_GLOBAL fw_v2i*)p; return make_int2(v.x, v.y); }
 __device__ inline float fw_ld(const float* p) { return *(const FW_GLOBAL float*)p; }
 __device__ inline uint32_t fw_ld(const uint32_t* p) { return *(const FW_GLOBAL uint32_t*)p; }
+__device__ inline int32_t fw_ld(const int32_t* p) { return *(const FW_GLOBAL int32_t*)p; }
 __device__ inline uint16_t fw_ld(const uint16_t* p) { return *(const FW_GLOBAL uint16_t*)p; }
 
 __device__ inline void fw_st(float2* p, float2 a) { *(FW_GLOBAL fw_v2f*)p = fw_v2f{a.x, a.y}; }
