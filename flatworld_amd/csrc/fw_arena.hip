@@ -850,30 +850,31 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     D.yflag = (uint8_t*)(blk + L.yflag);
     D.n_nodes = N;
     D.n_elems = E;
-    D.mat_type = d->mat_type;
-    D.n_grounds = d->n_grounds;
-    D.E = d->E;
-    D.nu = d->nu;
+    D.hot.mat_type = d->mat_type;
+    D.hot.n_grounds = d->n_grounds;
+    D.hot.E = d->E;
+    D.hot.nu = d->nu;
     D.rho = d->rho;
-    D.sigma_y = d->sigma_y;
-    D.H = d->H;
-    D.dt = d->dt;
-    D.damping = d->damping;
+    D.hot.sigma_y = d->sigma_y;
+    D.hot.H = d->H;
+    D.hot.dt = d->dt;
+    D.hot.damping = d->damping;
     // warm-up runs at the next substep launched (local step -1)
     D.s0 = (int32_t)(a->clock + 1);
     D.horizon = d->horizon > 0 ? d->horizon : INT32_MAX;
     D.live = 1;
     for (int b = 0; b < 3; ++b) {
-        D.aabb[b][0] = fw_f2o(1e30f);
-        D.aabb[b][1] = fw_f2o(1e30f);
-        D.aabb[b][2] = fw_f2o(-1e30f);
-        D.aabb[b][3] = fw_f2o(-1e30f);
+        D.hot.aabb[b][0] = fw_f2o(1e30f);
+        D.hot.aabb[b][1] = fw_f2o(1e30f);
+        D.hot.aabb[b][2] = fw_f2o(-1e30f);
+        D.hot.aabb[b][3] = fw_f2o(-1e30f);
     }
     auto o8 = [](int64_t off) { return (uint32_t)(off >> 8); };  // layout offsets are 256-byte aligned
     for (auto& td : T.tiles) {
         td.base = blk;
         td.o_rec = o8(L.rec);
         td.o_hel = o8(L.hel);
+        td.o_bcv = o8(L.bcv);
         td.o_pos[0] = o8(L.pos0);
         td.o_pos[1] = o8(L.pos1);
         td.o_hpos[0] = o8(L.hpos0);
@@ -946,6 +947,11 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
             G.a[0] = gd.hf_lb_x;
             G.a[1] = gd.hf_ub_x;
             G.nx = gd.hf_nx;
+            {
+                // sample spacing as the device query computes it: float32 subtraction, then division
+                volatile float span = gd.hf_ub_x - gd.hf_lb_x;
+                G.hf_dxw = span / (float)(gd.hf_nx - 1);
+            }
             G.reverse = gd.reverse;
             std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
             G.data = blk + L.gdata[g];
@@ -1154,8 +1160,8 @@ int32_t fw_scene_reset_gate(fw_arena* a, int32_t id) {
     if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
     HIPCHK(hipSetDevice(a->device));
     // next launch runs local step clock - s0
-    s->dev.gate_base = (int32_t)(a->clock - s->s0);
-    return upload(a, &a->d_scenes[id].gate_base, &s->dev.gate_base, sizeof(int32_t));
+    s->dev.hot.gate_base = (int32_t)(a->clock - s->s0);
+    return upload(a, &a->d_scenes[id].hot.gate_base, &s->dev.hot.gate_base, sizeof(int32_t));
 }
 
 int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
@@ -1243,7 +1249,7 @@ int32_t fw_scene_read(fw_arena* a, int32_t id, int32_t field, void* out, int64_t
         if (out_bytes < 16) return fail(FW_E_INVALID, "output buffer too small");
         // the AABB at the current x was written by the last completed local step
         int32_t enc[3][4];
-        HIPCHK(hipMemcpy(enc, &a->d_scenes[id].aabb[0][0], sizeof(enc), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(enc, &a->d_scenes[id].hot.aabb[0][0], sizeof(enc), hipMemcpyDeviceToHost));
         int slot = warmed ? (int)(((done % 3) + 3) % 3) : 0;
         float* o = (float*)out;
         for (int k = 0; k < 4; ++k) o[k] = fw_o2f(enc[slot][k]);

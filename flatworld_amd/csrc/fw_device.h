@@ -162,7 +162,7 @@ __device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z,
     const float h0 = fw_ld(hgt + i0), h1 = fw_ld(hgt + i1), hj0 = fw_ld(hgt + j0), hj1 = fw_ld(hgt + j1);
     float t = sg - (float)i0;
     float h = h0 + (h1 - h0) * t;
-    float dxw = span / (float)(nx - 1);
+    float dxw = g.hf_dxw;  // span / (nx - 1), rounded once in float32 on the host
     float denom = dxw * (float)(j1 - j0);
     float dhdx = 0.0f;
     if (denom > 1e-6f) dhdx = (hj1 - hj0) / denom;

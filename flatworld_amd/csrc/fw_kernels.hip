@@ -55,7 +55,9 @@ __device__ __forceinline__ void unpack_rec(uint2 r, int n_own_nodes, const uint1
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         l[c] = w[c] & FW_REC_ID_MASK;
-        q[c] = (int)l[c] < n_own_nodes ? (uint32_t)lds_cp[l[c]] + (w[c] >> FW_REC_ID_BITS) : 0xFFFFu;
+        // branch-free: halo corners read a valid entry and discard it
+        const uint32_t base = lds_cp[min((int)l[c], n_own_nodes)];
+        q[c] = (int)l[c] < n_own_nodes ? base + (w[c] >> FW_REC_ID_BITS) : 0xFFFFu;
     }
 }
 
@@ -264,21 +266,23 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         float4 st_cur = st.j2s;  // lanes past the owned slots hold a copy; unused there
         if (j2 && tid >= T.n_own && tid < nslots) st_cur = fw_ld(j2_rd + st.he[0]);
 
-        // scene scalars (needed from phase A on), scalar loads
+        // scene scalars (needed from phase A on), scalar loads of single fields: a
+        // whole-record copy would hold every field in SGPRs at once and spill
         const SceneDev* Sp = A.scenes + T.scene;
+        const SceneHot* sh = &Sp->hot;
         const bool warm = local == -1;
-        const float dt = warm ? 0.0f : (A.dt_override >= 0.0f ? A.dt_override : fw_ldc1(&Sp->dt));
-        const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : fw_ldc1(&Sp->damping));
-        const int n_grounds = fw_ldc1(&Sp->n_grounds);
+        const float dt = warm ? 0.0f : (A.dt_override >= 0.0f ? A.dt_override : fw_ldc1(&sh->dt));
+        const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : fw_ldc1(&sh->damping));
+        const int n_grounds = fw_ldc1(&sh->n_grounds);
         const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && n_grounds > 0;
         Mat m;
         {
-            const float E = fw_ldc1(&Sp->E), nu = fw_ldc1(&Sp->nu);
-            m.type = fw_ldc1(&Sp->mat_type);
+            const float E = fw_ldc1(&sh->E), nu = fw_ldc1(&sh->nu);
+            m.type = fw_ldc1(&sh->mat_type);
             m.lam = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
             m.mu = E / (2.0f * (1.0f + nu));
-            m.sy = fw_ldc1(&Sp->sigma_y);
-            m.H = fw_ldc1(&Sp->H);
+            m.sy = fw_ldc1(&sh->sigma_y);
+            m.H = fw_ldc1(&sh->H);
         }
         float4* const st_stress = diag ? fw_ldc1(&Sp->stress) : nullptr;
         float4* const st_strain = diag ? fw_ldc1(&Sp->strain) : nullptr;
@@ -325,9 +329,9 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         uint32_t active = 0;
         if (contact) {
             const int ar3 = mod3(local);
-            float lbx = fw_o2f(fw_ldc1(&Sp->aabb[ar3][0])), lby = fw_o2f(fw_ldc1(&Sp->aabb[ar3][1]));
-            float ubx = fw_o2f(fw_ldc1(&Sp->aabb[ar3][2])), uby = fw_o2f(fw_ldc1(&Sp->aabb[ar3][3]));
-            const bool pad = ((local - fw_ldc1(&Sp->gate_base)) % 100) == 0;
+            float lbx = fw_o2f(fw_ldc1(&sh->aabb[ar3][0])), lby = fw_o2f(fw_ldc1(&sh->aabb[ar3][1]));
+            float ubx = fw_o2f(fw_ldc1(&sh->aabb[ar3][2])), uby = fw_o2f(fw_ldc1(&sh->aabb[ar3][3]));
+            const bool pad = ((local - fw_ldc1(&sh->gate_base)) % 100) == 0;
             if (pad) {
                 const float bx = 0.1f * (ubx - lbx), by = 0.1f * (uby - lby);
                 lbx = lbx - bx;
@@ -348,7 +352,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // first push entry per thread, loaded now so that it lands during phase B
         int2 pe = make_int2(0, 0);
         if (tid < T.n_push) pe = fw_ld(fw_at<const int2>(T.base, T.o_push) + T.push_off + tid);
-        const float2* __restrict__ bcv = fw_ldc1(&Sp->bcv);
+        const float2* __restrict__ bcv = fw_at<const float2>(T.base, T.o_bcv);
         float2 bv[NPT];
 #pragma unroll
         for (int k = 0; k < NPT; ++k) {
@@ -431,8 +435,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // ---------------- boundary AABB at x_{n+1} (flatworld/femspringmanager.py:361-408)
         block_minmax<BLOCK>(bb, lds_red);  // its barrier also publishes the lds_x updates
         if (tid == 0) {
-            int32_t* aw = const_cast<SceneDev*>(Sp)->aabb[mod3(local + 1)];
-            int32_t* az = const_cast<SceneDev*>(Sp)->aabb[mod3(local + 2)];
+            int32_t* aw = const_cast<SceneDev*>(Sp)->hot.aabb[mod3(local + 1)];
+            int32_t* az = const_cast<SceneDev*>(Sp)->hot.aabb[mod3(local + 2)];
             if (T.multi) {
                 atomicMin(aw + 0, bb[0]);
                 atomicMin(aw + 1, bb[1]);

@@ -27,12 +27,25 @@ struct GroundDev {
     float lo[2], hi[2];    // raw ground AABB
     float plo[2], phi[2];  // padded by 0.1 * extent (BVH rebuild substeps)
     float a[4];            // plane: p.x p.y n.x n.y | HF: lb_x ub_x | voxel: lb.x lb.y dx dz
-    int32_t nx, ny, reverse, pad;
+    int32_t nx, ny, reverse;
+    float hf_dxw;          // HF: sample spacing (ub_x - lb_x) / (nx - 1), rounded once in float32 on the host
     const void* data;      // HF heights (float) | voxel cell extents (uint16 x4 per cell: scan end points
                            // along -x, +x, -z, +z; first = 0xFFFF when the cell is not occupied)
 };
 
+// The scene fields every tile reads (material, dt, gate state), kept together at
+// the front of the scene entry.
+struct SceneHot {
+    int32_t mat_type, n_grounds;
+    float E, nu, sigma_y, H;
+    float dt, damping;
+    int32_t gate_base;  // local step that counts as 0 for the gate's rebuild schedule
+    int32_t pad_;
+    int32_t aabb[3][4];  // boundary AABB (ordered-int encoded), triple buffered by local step
+};
+
 struct SceneDev {
+    SceneHot hot;
     // node arrays
     float2* pos[2];
     const float2* rest;
@@ -53,14 +66,10 @@ struct SceneDev {
     uint8_t* yflag;  // diag
     // scalars
     int32_t n_nodes, n_elems;
-    int32_t mat_type, n_grounds;
-    float E, nu, rho, sigma_y, H;
-    float dt, damping;
+    float rho;
     int32_t s0;       // global step at which the scene runs its warm-up (local step -1 is s0 - 1)
     int32_t horizon;  // substeps after the warm-up; INT32_MAX when unbounded
     int32_t live;     // 0 once removed (host bookkeeping: the tile table holds live scenes only)
-    int32_t gate_base; // local step that counts as 0 for the gate's rebuild schedule
-    int32_t aabb[3][4];  // boundary AABB (ordered-int encoded), triple buffered by local step
     GroundDev ground[FW_MAX_GROUNDS_DEV];
 };
 
@@ -76,6 +85,7 @@ struct TileDev {
     //   in the node's incidence list.  An owned node's corner force goes to LDS entry
     //   cptr[l] + r; a halo node's (l >= n_nodes) is its owner tile's business.
     uint32_t o_hel;          // int32 per halo slot: storage index of the element (J2 state)
+    uint32_t o_bcv;          // float2 BC value table of the scene (indexed by the node flag's row)
     uint32_t o_pos[2];       // float2 node positions (ping-pong)
     uint32_t o_hpos[2];      // float2 halo copies (ping-pong)
     uint32_t o_rest;         // float2 rest positions
@@ -96,7 +106,6 @@ struct TileDev {
     int32_t ptr_off;                    // cptr[ptr_off .. ptr_off + n_nodes]
     int32_t push_off, n_push;           // push[push_off .. +n_push)
     int32_t hel_off;                    // hel[hel_off + slot - n_own] for halo slots
-    int32_t pad_;
 };
 
 // Slot record corner word: local node id in the low bits, incidence rank above.
