@@ -4,14 +4,21 @@
 Default workload: the batched mix -- 4096 independent scenes, grid side
 log-uniform in [8, 128], Elastic / Neo-Hookean / J2 mix, one height field per
 scene (24.08 M triangles, 12.40 M nodes).  A "step" is one substep of every
-scene; the timed region replays hipGraph-captured 60-substep frames.  Inputs
-are resident in HBM before timing starts.
+scene; the timed region replays hipGraph-captured frames.  Inputs are resident
+in HBM before timing starts.
 
 Printed JSON line:
   value            element-substeps/s of the batched, graph-replayed form
   per_scene        the same scenes advanced with one launch per scene per substep
   roofline         algorithmic bytes per substep / average substep-kernel time,
-                   measured live with HIP events on the launch stream
+                   measured live with HIP events on the launch stream; traffic =
+                   HBM bytes per launch from the committed rocprof counter profile
+                   of the same kernel shape and workload
+  tuning           kernel shape, workgroups per CU and substeps per graph chosen
+                   by the autotuner (flatworld_amd.autotune) for this workload
+  serving          a Poisson stream of jobs (grid side 16-256, random material and
+                   ground, horizon 60-600 substeps) at 80% of the capacity measured
+                   on a closed-loop burst: p50 / p99 request latency, throughput
   cpu_baseline     the oracle (serial C restatement) on a bounded sample
 """
 
@@ -27,6 +34,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "substep_batched4096_traffic.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def build_specs(config, seed, n_scenes):
@@ -74,19 +86,78 @@ def cpu_baseline(specs, budget_s=12.0):
     }
 
 
+def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
+    """Capacity on a closed-loop burst, then a Poisson stream at 80% of it."""
+    from flatworld_amd import workloads as W
+    from flatworld_amd.server import Job, SimServer, poisson_arrivals
+
+    rng = W.rng_for(seed + 7919)
+    t = time.perf_counter()
+    specs = [W.serving_job(rng) for _ in range(n_jobs)]  # the requests' payloads, built before the clock starts
+    log(f"[bench] serving: {n_jobs} job payloads built in {time.perf_counter() - t:.1f}s")
+
+    nb = min(burst_jobs, n_jobs)
+    srv = SimServer(frame=frame, keep_results=False, **arena_kw)
+    srv.run([Job(i, specs[i], 0.0) for i in range(nb)])
+    burst = srv.stats()
+    srv.batch.close()
+    capacity = burst["jobs"] / burst["wall_s"]
+    rate = 0.8 * capacity
+    log(f"[bench] serving: burst of {nb} jobs in {burst['wall_s']:.2f}s -> capacity {capacity:.1f} jobs/s")
+
+    arrivals = poisson_arrivals(n_jobs, rate, rng)
+    srv = SimServer(frame=frame, keep_results=False, **arena_kw)
+    srv.run([Job(i, specs[i], float(arrivals[i])) for i in range(n_jobs)])
+    st = srv.stats()
+    srv.batch.close()
+    log(f"[bench] serving: {st['jobs']} jobs at {rate:.1f}/s, p50 {st['latency_p50_s'] * 1e3:.1f} ms, "
+        f"p99 {st['latency_p99_s'] * 1e3:.1f} ms, {st['elem_substeps_per_s']:.3e} elem-substeps/s")
+    return {
+        "jobs": st["jobs"],
+        "failed": st["failed"],
+        "arrival_rate_jobs_per_s": rate,
+        "capacity_jobs_per_s": capacity,
+        "latency_p50_ms": st["latency_p50_s"] * 1e3,
+        "latency_p99_ms": st["latency_p99_s"] * 1e3,
+        "latency_mean_ms": st["latency_mean_s"] * 1e3,
+        "elem_substeps_per_s": st["elem_substeps_per_s"],
+        "burst_elem_substeps_per_s": burst["elem_substeps_per_s"],
+        "gpu_busy_frac": st["gpu_busy_frac"],
+        "frame_substeps": frame,
+        "workload": "Poisson stream: grid side U{16..256}, Elastic/Neo-Hookean/J2, plane/height field/voxel ground, "
+                    "horizon U{60..600}; latency = arrival -> final positions on the host",
+    }
+
+
+def traffic_from_profile(shape, config):
+    """HBM bytes per substep launch from the committed counter profile of this shape, if any."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            prof = json.load(fh)
+    except (OSError, ValueError):
+        return None, None
+    if prof.get("workload") != config or any(prof.get("shape", {}).get(k) != v for k, v in shape.items()):
+        return None, None
+    return prof.get("bytes_per_launch"), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1200, help="timed substeps (multiple of --frame is replayed as graphs)")
+    ap.add_argument("--steps", type=int, default=1200, help="timed substeps (whole frames replay as hipGraphs)")
     ap.add_argument("--warmup", type=int, default=120)
-    ap.add_argument("--frame", type=int, default=60, help="substeps per hipGraph frame")
+    ap.add_argument("--frame", type=int, default=0, help="substeps per hipGraph frame (0: the autotuner's choice)")
     ap.add_argument("--config", default="batched", choices=["batched", "large", "j2voxel", "small"])
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--tile-nodes", type=int, default=512)
+    ap.add_argument("--tile-nodes", type=int, default=0, help="explicit kernel shape (skips the autotuner)")
     ap.add_argument("--tile-slots", type=int, default=1024)
     ap.add_argument("--block", type=int, default=256, help="threads per tile workgroup")
+    ap.add_argument("--grid-per-cu", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="default kernel shape instead of the autotuner")
     ap.add_argument("--per-scene-steps", type=int, default=6)
+    ap.add_argument("--serve-jobs", type=int, default=300, help="jobs in the served Poisson stream (0: skip)")
+    ap.add_argument("--serve-burst", type=int, default=150, help="jobs of the closed-loop capacity burst")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph frames")
@@ -106,6 +177,7 @@ def main():
         sys.exit(2)
 
     from flatworld_amd import _native
+    from flatworld_amd import autotune
     from flatworld_amd import workloads as W
 
     t_setup = time.perf_counter()
@@ -113,17 +185,33 @@ def main():
     E = sum(s.n_elems for s in specs)
     N = sum(s.n_nodes for s in specs)
     algo = float(sum(W.algorithmic_bytes(s) for s in specs))
-    arena = _native.Arena(tile_nodes=args.tile_nodes, tile_slots=args.tile_slots, block=args.block)
+
+    # kernel shape, workgroups per CU, substeps per graph
+    if args.tile_nodes > 0 or args.no_tune:
+        tuning = {"tile_nodes": args.tile_nodes or 512, "tile_slots": args.tile_slots, "block": args.block,
+                  "grid_per_cu": args.grid_per_cu, "frame": args.frame or 60, "source": "explicit"}
+    else:
+        t = time.perf_counter()
+        res = autotune.tune(specs, log=log)
+        tuning = {k: res[k] for k in ("tile_nodes", "tile_slots", "block", "grid_per_cu", "frame")}
+        tuning["source"] = "autotuner (cached)" if res.get("cached") else "autotuner"
+        tuning["tune_s"] = time.perf_counter() - t
+        if args.frame:
+            tuning["frame"] = args.frame
+    arena_kw = autotune.arena_kwargs(tuning)
+    log(f"[bench] tuning: {tuning}")
+
+    arena = _native.Arena(**arena_kw)
     for s in specs:
         arena.add(s, track_disp=True)
     arena.step(1, -1.0, 0)  # zero-dt warm-up of every scene
     arena.sync()
     info = arena.info()
     t_setup = time.perf_counter() - t_setup
-    print(f"[bench] {args.config}: {len(specs)} scenes, E={E}, N={N}, tiles={info['n_tiles']}, "
-          f"halo slots={info['halo_slots']}, setup {t_setup:.1f}s", file=sys.stderr, flush=True)
+    log(f"[bench] {args.config}: {len(specs)} scenes, E={E}, N={N}, tiles={info['n_tiles']}, "
+        f"halo slots={info['halo_slots']}, setup {t_setup:.1f}s")
 
-    frame = max(1, args.frame)
+    frame = max(1, tuning["frame"])
 
     def run(n):
         if args.no_graph:
@@ -143,7 +231,7 @@ def main():
     t1 = time.perf_counter()
     total_s = t1 - t0
     value = E * args.steps / total_s
-    print(f"[bench] batched: {args.steps} substeps in {total_s*1e3:.1f} ms", file=sys.stderr, flush=True)
+    log(f"[bench] batched: {args.steps} substeps in {total_s * 1e3:.1f} ms")
 
     # live kernel duration: HIP events around each substep kernel on the arena stream
     n_k = min(args.steps, 240)
@@ -161,13 +249,20 @@ def main():
             "ms_per_step": ps_total_ms / args.per_scene_steps,
             "batched_speedup": value / ps_value,
         }
-        print(f"[bench] per-scene launches: {ps_total_ms / args.per_scene_steps:.2f} ms/substep", file=sys.stderr, flush=True)
+        log(f"[bench] per-scene launches: {ps_total_ms / args.per_scene_steps:.2f} ms/substep")
+    arena.close()
+
+    serving = None
+    if args.serve_jobs > 0 and args.config == "batched":
+        serving = serving_bench(args.serve_jobs, args.seed, arena_kw, frame, args.serve_burst)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget)
-        print(f"[bench] cpu baseline {cpu['value']:.3e} elem-substeps/s", file=sys.stderr, flush=True)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} elem-substeps/s")
 
+    shape = {k: tuning[k] for k in ("tile_nodes", "tile_slots", "block")}
+    traffic, traffic_src = traffic_from_profile(shape, f"{args.config}-{len(specs)}")
     out = {
         "metric": "element-substeps/s, batched vs per-scene; p50/p99 request latency @1e-4 tol",
         "value": value,
@@ -182,26 +277,27 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": args.config,
+            "workload": f"{args.config}-{len(specs)}",
             "scenes": len(specs),
             "triangles": E,
             "nodes": N,
             "frame_substeps": frame,
-            "tile_nodes": args.tile_nodes,
-            "tile_slots": args.tile_slots,
             "seed": args.seed,
         },
+        "tuning": tuning,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel_ms": kernel_ms,
             "algo_bytes_per_substep": algo,
         },
         "per_scene": per_scene,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

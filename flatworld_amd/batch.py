@@ -15,10 +15,10 @@ from . import _native
 
 
 class FemBatch:
-    def __init__(self, device=0, capacity_bytes=0, tile_nodes=512, tile_slots=1024, block=256, frame=60,
-                 track_disp=True):
+    def __init__(self, device=0, capacity_bytes=0, tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0,
+                 frame=60, track_disp=True):
         self.arena = _native.Arena(device=device, capacity_bytes=capacity_bytes, tile_nodes=tile_nodes,
-                                   tile_slots=tile_slots, block=block)
+                                   tile_slots=tile_slots, block=block, grid_per_cu=grid_per_cu)
         self.frame = int(frame)
         self.track_disp = track_disp
         self.specs = {}

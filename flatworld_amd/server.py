@@ -6,8 +6,9 @@ job -- one FEM body, its ground, its material and a horizon in substeps --
 arrives at any time and is:
 
 1. queued in arrival order;
-2. admitted at the next frame boundary once its HBM footprint fits in the
-   arena (head-of-line jobs wait; nothing is evicted);
+2. admitted once its HBM footprint fits in the arena (head-of-line jobs
+   wait; nothing is evicted) -- the host does this while the current frame
+   runs on the device, and the job joins at the next frame;
 3. advanced together with every other resident job, one fused launch per
    substep, frames replayed as a hipGraph (its zero-dt warm-up is simply its
    first substep in the arena);
@@ -70,7 +71,13 @@ class SimServer:
         self.elem_substeps = 0
 
     def run(self, jobs, max_wall=None):
-        """Serve ``jobs`` (arrival times relative to the start of this call); returns the results."""
+        """Serve ``jobs`` (arrival times relative to the start of this call); returns the results.
+
+        One frame is in flight at a time.  While it runs, the host admits the
+        jobs that arrived meanwhile (tiling, packing, uploads queued behind the
+        frame on the arena stream), so admission work overlaps device work; the
+        admitted jobs join at the next frame.
+        """
         pending = collections.deque(sorted(jobs, key=lambda j: j.arrival))
         waiting = collections.deque()
         resident = {}  # scene id -> (job, admitted time)
@@ -79,13 +86,11 @@ class SimServer:
         for job in pending:
             if job.spec.horizon <= 0:
                 raise ValueError(f"job {job.job_id}: served jobs need a positive horizon")
-        while pending or waiting or resident:
+
+        def admit():
             now = self.clock() - t0
-            if max_wall is not None and now > max_wall:
-                break
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            # admission by footprint, arrival order
             free = b.free_bytes()
             while waiting:
                 job = waiting[0]
@@ -101,18 +106,8 @@ class SimServer:
                 waiting.popleft()
                 resident[sid] = (job, self.clock() - t0)
                 free = b.free_bytes()
-            if not resident:
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - (self.clock() - t0), 0.01)))
-                continue
-            # one frame for everyone resident
-            fs = self.clock()
-            b.step(self.frame, graph=True)
-            b.sync()
-            self.busy_time += self.clock() - fs
-            self.frames += 1
-            # retire jobs that reached their horizon
-            done_now = self.clock() - t0
+
+        def retire():
             for sid in list(resident.keys()):
                 steps, fin = b.progress(sid)
                 if not fin:
@@ -127,7 +122,29 @@ class SimServer:
                     JobResult(job.job_id, job.arrival, adm, self.clock() - t0, job.spec.horizon, job.spec.n_elems, ok,
                               x if self.keep_results else None, v)
                 )
-            del done_now
+
+        in_flight = False
+        launched = 0.0
+        while pending or waiting or resident:
+            if max_wall is not None and self.clock() - t0 > max_wall:
+                break
+            if in_flight:
+                admit()  # overlaps the running frame
+                b.sync()
+                self.busy_time += self.clock() - launched
+                in_flight = False
+                retire()
+            admit()
+            if resident:
+                launched = self.clock()
+                b.step(self.frame, graph=True)
+                self.frames += 1
+                in_flight = True
+            elif pending:
+                time.sleep(max(0.0, min(pending[0].arrival - (self.clock() - t0), 0.01)))
+        if in_flight:
+            b.sync()
+            self.busy_time += self.clock() - launched
         self.wall = self.clock() - t0
         return self.results
 

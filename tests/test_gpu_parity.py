@@ -14,6 +14,7 @@ import pytest
 
 import flatworld_amd as fw
 from flatworld_amd import _native
+from flatworld_amd import autotune
 from flatworld_amd import workloads as W
 from oracle.oracle import OracleScene
 from tests.parity import FORCE_RTOL, POS_TOL_EDGE, GpuScene, compare, log
@@ -138,3 +139,67 @@ def test_graph_replay_equals_plain():
         b.sync()
         for x, y in zip(ia, ib):
             assert np.array_equal(a.read(x, _native.FIELD_X), b.read(y, _native.FIELD_X)), f"replay {rep}"
+
+
+# ---------------------------------------------------------------- full sizes of the benchmark configurations
+
+
+def test_large_sheet_full_size_bitexact():
+    """1000x1000 jittered Neo-Hookean sheet (2 M triangles) on a plane, CFL dt, 100 substeps."""
+    spec = W.large_config()
+    g, o, _ = run_both(spec, [1, 99], True, "large 1000x1000 nh, plane")
+    res = compare(g, o, False, "large tolerance figures")
+    log(f"  {res}")
+    assert res["force_rel"] < FORCE_RTOL and res["pos_err_edges"] < POS_TOL_EDGE
+    assert np.count_nonzero(o.contact_flag) > 0, "jittered bottom row should touch the plane"
+
+
+def test_j2_voxel_full_size_bitexact():
+    """512x512 J2 block (524 k triangles) dropped at 2 m/s onto 256x256 value-noise voxels, 200 substeps."""
+    spec = W.j2_voxel_config()
+    run_both(spec, [1, 199], True, "j2 512x512 on 256x256 voxels")
+
+
+def test_batched_4096_sampled_scenes_bitexact():
+    """The full 4096-scene arena of the benchmark: 48 sampled scenes against the oracle after 30 substeps."""
+    specs = W.batched_config(n_scenes=4096, seed=0)
+    log(f"[case] batched 4096 scenes, {sum(s.n_elems for s in specs)} triangles")
+    arena = _native.Arena()
+    ids = [arena.add(s) for s in specs]
+    arena.step(1, -1.0, 0)
+    arena.step(30, -1.0, _native.FW_STEP_DIAG)
+    pick = W.rng_for(99).choice(len(specs), size=48, replace=False)
+    for k in pick:
+        o = OracleScene(specs[k])
+        o.warmup()
+        o.run(30)
+        for fld, ref in ((_native.FIELD_X, o.x), (_native.FIELD_V, o.v), (_native.FIELD_FORCE, o.fext)):
+            got = arena.read(ids[k], fld)
+            assert np.isfinite(ref).all()
+            assert np.array_equal(got, ref), f"scene {k} field {fld}"
+    arena.close()
+
+
+def test_every_kernel_shape_gives_identical_results():
+    """Tiling, kernel shape and launch mode change no bit of the result."""
+    specs = W.batched_config(n_scenes=10, seed=31, side_lo=6, side_hi=48)
+    log("[case] all compiled shapes, one workgroup per tile and persistent")
+    ref = None
+    for block, slots, nodes in autotune.SHAPES:
+        for gpc in (0, 4):
+            a = _native.Arena(capacity_bytes=1 << 30, tile_nodes=nodes, tile_slots=slots, block=block, grid_per_cu=gpc)
+            ids = [a.add(s) for s in specs]
+            a.step(1, -1.0, 0)
+            a.step(80, -1.0, _native.FW_STEP_DIAG)
+            got = [(a.read(i, _native.FIELD_X), a.read(i, _native.FIELD_V), a.read(i, _native.FIELD_FORCE)) for i in ids]
+            a.close()
+            if ref is None:
+                ref = got
+                o = OracleScene(specs[3])
+                o.warmup()
+                o.run(80)
+                assert np.array_equal(got[3][0], o.x) and np.array_equal(got[3][2], o.fext)
+                continue
+            for k, (x, v, f) in enumerate(got):
+                assert np.array_equal(x, ref[k][0]) and np.array_equal(v, ref[k][1]) and np.array_equal(f, ref[k][2]), \
+                    f"shape ({block},{slots},{nodes}) grid_per_cu={gpc} scene {k}"
