@@ -142,6 +142,10 @@ SIGNATURES = {
     "fw_tiling_check": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "fw_bandwidth_probe": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "fw_time_steps": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "fw_scene_prepare": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_void_p)]),
+    "fw_prepared_footprint": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "fw_scene_add_prepared": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    "fw_prepared_free": (ctypes.c_int32, [ctypes.c_void_p]),
 }
 
 _lib = None
@@ -266,6 +270,39 @@ def bandwidth_probe(device=0, nbytes=1 << 30, reps=20):
     return g.value, k.value
 
 
+class Prepared:
+    """A scene tiled and packed on the host for one kernel shape (fw_scene_prepare).
+
+    Building one needs no GPU and releases the GIL inside the C call, so servers
+    prepare arriving scenes on worker threads while the device runs.
+    """
+
+    def __init__(self, spec, tuning=None, track_disp=True):
+        L = load()
+        self._L = L
+        t = Tuning(**{k: int(v) for k, v in (tuning or {}).items() if k in ("tile_nodes", "tile_slots", "block", "grid_per_cu")}) \
+            if tuning else Tuning(512, 1024, 256, 0)
+        d, keep = scene_desc(spec, track_disp)
+        h = ctypes.c_void_p()
+        check(L.fw_scene_prepare(ctypes.byref(d), ctypes.byref(t), ctypes.byref(h)), "fw_scene_prepare")
+        self.h = h
+        self.spec = spec
+        b = ctypes.c_int64()
+        check(L.fw_prepared_footprint(self.h, ctypes.byref(b)), "fw_prepared_footprint")
+        self.footprint = int(b.value)
+
+    def close(self):
+        if self.h:
+            self._L.fw_prepared_free(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Arena:
     """Owner of one fw_arena: a packed HBM region holding many scenes."""
 
@@ -306,6 +343,18 @@ class Arena:
         sid = ctypes.c_int32()
         check(self._L.fw_scene_add(self.h, ctypes.byref(d), ctypes.byref(sid)), "fw_scene_add")
         self._shapes[sid.value] = (spec.n_nodes, spec.n_elems)
+        return sid.value
+
+    def prepare(self, spec, track_disp=True):
+        """Host half of add() for this arena's kernel shape; thread-safe, no GPU work."""
+        return Prepared(spec, self.tuning, track_disp)
+
+    def add_prepared(self, prep):
+        """Device half of add(): copy a Prepared scene in (it can be added once)."""
+        sid = ctypes.c_int32()
+        check(self._L.fw_scene_add_prepared(self.h, prep.h, ctypes.byref(sid)), "fw_scene_add_prepared")
+        self._shapes[sid.value] = (prep.spec.n_nodes, prep.spec.n_elems)
+        prep.close()
         return sid.value
 
     def remove(self, sid):

@@ -38,6 +38,16 @@ class FemBatch:
         self.specs[sid] = spec
         return sid
 
+    def prepare(self, spec):
+        """Host half of add(): tiling and packing, thread-safe, no GPU work."""
+        return self.arena.prepare(spec, track_disp=self.track_disp)
+
+    def add_prepared(self, prep):
+        """Device half of add(); raises RuntimeError (arena full) without consuming ``prep``."""
+        sid = self.arena.add_prepared(prep)
+        self.specs[sid] = prep.spec
+        return sid
+
     def remove(self, sid):
         self.arena.remove(sid)
         self.specs.pop(sid, None)

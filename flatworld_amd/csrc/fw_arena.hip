@@ -467,6 +467,17 @@ struct SceneHost {
     SceneDev dev;
 };
 
+// A scene prepared on the host for an arena tuning (fw_scene_prepare).
+struct fw_prepared {
+    fw_tuning tune{};
+    Tiling T;               // tiles carry offsets; base pointer, scene id and s0 are set at the commit
+    Layout L{};
+    std::vector<char> img;  // the block's bytes, tile table excluded (written at the commit)
+    SceneDev D{};           // scene entry without device addresses
+    int32_t n_nodes = 0, n_elems = 0, mat_type = 0, horizon = 0;
+    bool consumed = false;
+};
+
 struct fw_arena {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -669,6 +680,271 @@ void scene_state(const fw_arena* a, const SceneHost* s, int64_t* done, int* buf,
     *warmed = true;
 }
 
+Layout layout_of(const fw_scene_desc* d, const Tiling& T) {
+    return make_layout(d, (int64_t)(T.rec.size() / 2), (int64_t)T.hel.size(), (int64_t)T.hnode.size(),
+                       (int64_t)T.csr_ptr.size(), (int64_t)T.tiles.size());
+}
+
+// Host-only half of adding a scene: validation, tiling and the packed image of
+// the scene's block (everything but the device addresses).  Uses no arena state
+// except the optional tiling cache, so scenes can be prepared on other threads.
+int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache_arena,
+                  std::unique_ptr<fw_prepared>* out) {
+    int rc = validate(d);
+    if (rc) return rc;
+    auto P = std::make_unique<fw_prepared>();
+    P->tune = tune;
+    {
+        std::unique_ptr<Tiling> Tp;
+        if (cache_arena) {
+            rc = tiling_for(cache_arena, d, &Tp);
+        } else {
+            Tp = std::make_unique<Tiling>();
+            rc = build_tiling(d, tune.tile_nodes, tune.tile_slots, node_cap_of(tune), *Tp);
+        }
+        if (rc) return rc;
+        P->T = std::move(*Tp);
+    }
+    Tiling& T = P->T;
+    const int N = d->n_nodes, E = d->n_elems;
+    P->L = layout_of(d, T);
+    const Layout& L = P->L;
+    P->n_nodes = N;
+    P->n_elems = E;
+    P->mat_type = d->mat_type;
+    P->horizon = d->horizon > 0 ? d->horizon : INT32_MAX;
+
+    // scene entry, device addresses left for the commit
+    SceneDev& D = P->D;
+    std::memset(&D, 0, sizeof(D));
+    D.n_nodes = N;
+    D.n_elems = E;
+    D.hot.mat_type = d->mat_type;
+    D.hot.n_grounds = d->n_grounds;
+    D.hot.E = d->E;
+    D.hot.nu = d->nu;
+    D.rho = d->rho;
+    D.hot.sigma_y = d->sigma_y;
+    D.hot.H = d->H;
+    D.hot.dt = d->dt;
+    D.hot.damping = d->damping;
+    D.horizon = P->horizon;
+    D.live = 1;
+    for (int b = 0; b < 3; ++b) {
+        D.hot.aabb[b][0] = fw_f2o(1e30f);
+        D.hot.aabb[b][1] = fw_f2o(1e30f);
+        D.hot.aabb[b][2] = fw_f2o(-1e30f);
+        D.hot.aabb[b][3] = fw_f2o(-1e30f);
+    }
+    auto o8 = [](int64_t off) { return (uint32_t)(off >> 8); };  // layout offsets are 256-byte aligned
+    for (auto& td : T.tiles) {
+        td.base = nullptr;
+        td.o_rec = o8(L.rec);
+        td.o_hel = o8(L.hel);
+        td.o_bcv = o8(L.bcv);
+        td.o_pos[0] = o8(L.pos0);
+        td.o_pos[1] = o8(L.pos1);
+        td.o_hpos[0] = o8(L.hpos0);
+        td.o_hpos[1] = o8(L.hpos1);
+        td.o_rest = o8(L.rest);
+        td.o_hrest = o8(L.hrest);
+        td.o_vel = o8(L.vel);
+        td.o_mass = o8(L.mass);
+        td.o_flag = o8(L.flag);
+        td.o_disp = L.disp >= 0 ? o8(L.disp) : 0;
+        td.o_cptr = o8(L.cptr);
+        td.o_push = o8(L.push);
+        td.o_j2[0] = L.j2a >= 0 ? o8(L.j2a) : 0;
+        td.o_j2[1] = L.j2b >= 0 ? o8(L.j2b) : 0;
+        td.horizon = D.horizon;
+    }
+
+    // host image of the block
+    P->img.assign((size_t)L.total, 0);
+    char* img = P->img.data();
+    float2* pos0 = (float2*)(img + L.pos0);
+    float2* vel = (float2*)(img + L.vel);
+    uint32_t* flag = (uint32_t*)(img + L.flag);
+    for (int i = 0; i < N; ++i) {
+        int o = T.node_perm[i];
+        pos0[i] = make_float2(d->x0[2 * o], d->x0[2 * o + 1]);
+        vel[i] = make_float2(d->v0[2 * o], d->v0[2 * o + 1]);
+        flag[i] = d->flags[o];
+    }
+    std::memcpy(img + L.pos1, img + L.pos0, 8 * (size_t)N);
+    std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
+    {
+        float2* hp = (float2*)(img + L.hpos0);
+        for (size_t j = 0; j < T.hnode.size(); ++j) hp[j] = pos0[T.hnode[j]];
+        std::memcpy(img + L.hpos1, img + L.hpos0, 8 * T.hnode.size());
+        std::memcpy(img + L.hrest, img + L.hpos0, 8 * T.hnode.size());
+    }
+    std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
+    std::memcpy(img + L.rec, T.rec.data(), 4 * T.rec.size());
+    std::memcpy(img + L.hel, T.hel.data(), 4 * T.hel.size());
+    std::memcpy(img + L.push, T.push.data(), 4 * T.push.size());
+    std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
+    for (int g = 0; g < d->n_grounds; ++g) {
+        const fw_ground_desc& gd = d->grounds[g];
+        GroundDev& G = D.ground[g];
+        G.kind = gd.kind;
+        G.penalty = gd.penalty;
+        for (int k = 0; k < 2; ++k) {
+            G.lo[k] = gd.aabb_lo[k];
+            G.hi[k] = gd.aabb_hi[k];
+            float b = 0.1f * (gd.aabb_hi[k] - gd.aabb_lo[k]);
+            G.plo[k] = gd.aabb_lo[k] - b;
+            G.phi[k] = gd.aabb_hi[k] + b;
+        }
+        if (gd.kind == FW_GROUND_PLANE) {
+            G.a[0] = gd.point[0];
+            G.a[1] = gd.point[1];
+            G.a[2] = gd.normal[0];
+            G.a[3] = gd.normal[1];
+        } else if (gd.kind == FW_GROUND_HEIGHTFIELD) {
+            G.a[0] = gd.hf_lb_x;
+            G.a[1] = gd.hf_ub_x;
+            G.nx = gd.hf_nx;
+            {
+                // sample spacing as the device query computes it: float32 subtraction, then division
+                volatile float span = gd.hf_ub_x - gd.hf_lb_x;
+                G.hf_dxw = span / (float)(gd.hf_nx - 1);
+            }
+            G.reverse = gd.reverse;
+            std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
+        } else {
+            G.a[0] = gd.vox_lb[0];
+            G.a[1] = gd.vox_lb[1];
+            G.a[2] = gd.vox_dx;
+            G.a[3] = gd.vox_dz;
+            G.nx = gd.vox_nx;
+            G.ny = gd.vox_ny;
+            build_voxel_extents(gd, (uint16_t*)(img + L.gdata[g]));
+        }
+    }
+    *out = std::move(P);
+    return FW_OK;
+}
+
+// Device half: allocate the block, patch the device addresses into the image,
+// upload it and the scene entry, and compute area / J^-1 / lumped mass on the
+// device.  The prepared scene's tiling moves into the arena.
+int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
+    const Layout& L = p->L;
+    int id_probe = !a->free_ids.empty() ? a->free_ids.back() : (int)a->scenes.size();
+    if (id_probe >= a->max_scenes)
+        return fail(FW_E_CAPACITY, "arena holds at most " + std::to_string(a->max_scenes) + " scenes");
+    if (L.total > a->capacity - a->used)
+        return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
+                                       std::to_string(a->capacity - a->used) + " free");
+    Staging* st = nullptr;
+    int rc = get_staging(a, L.total, &st);
+    if (rc) return rc;
+    int64_t off = arena_alloc(a, L.total);
+    if (off < 0)
+        return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
+                                       std::to_string(a->capacity - a->used) + " free (fragmented)");
+    int id;
+    if (!a->free_ids.empty()) {
+        id = a->free_ids.back();
+        a->free_ids.pop_back();
+    } else {
+        id = (int)a->scenes.size();
+    }
+    char* blk = a->base + off;
+
+    SceneDev D = p->D;
+    D.pos[0] = (float2*)(blk + L.pos0);
+    D.pos[1] = (float2*)(blk + L.pos1);
+    D.vel = (float2*)(blk + L.vel);
+    D.mass = (float*)(blk + L.mass);
+    D.flag = (const uint32_t*)(blk + L.flag);
+    D.bcv = (const float2*)(blk + L.bcv);
+    D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
+    D.force = (float2*)(blk + L.force);
+    D.cflag = (uint8_t*)(blk + L.cflag);
+    D.rest = (const float2*)(blk + L.rest);
+    D.hpos[0] = (float2*)(blk + L.hpos0);
+    D.hpos[1] = (float2*)(blk + L.hpos1);
+    D.jinv = (float4*)(blk + L.jinv);
+    D.area = (float*)(blk + L.area);
+    D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
+    D.j2[1] = L.j2b >= 0 ? (float4*)(blk + L.j2b) : nullptr;
+    D.stress = (float4*)(blk + L.stress);
+    D.strain = (float4*)(blk + L.strain);
+    D.yflag = (uint8_t*)(blk + L.yflag);
+    for (int g = 0; g < D.hot.n_grounds; ++g)
+        if (L.gdata[g] >= 0) D.ground[g].data = blk + L.gdata[g];
+    // warm-up runs at the next substep launched (local step -1)
+    D.s0 = (int32_t)(a->clock + 1);
+
+    Tiling& T = p->T;
+    for (auto& td : T.tiles) {
+        td.base = blk;
+        td.scene = id;
+        td.s0 = D.s0;
+    }
+    std::memcpy(st->ptr, p->img.data(), (size_t)L.total);
+    std::memcpy((char*)st->ptr + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
+    HIPCHK(hipMemcpyAsync(blk, st->ptr, L.total, hipMemcpyHostToDevice, a->stream));
+    HIPCHK(hipEventRecord(st->ev, a->stream));
+    st->busy = true;
+
+    rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
+    if (rc) return rc;
+    HIPCHK(fw_launch_prepare(a->d_scenes, (const TileDev*)(blk + L.tiles), 0, (int)T.tiles.size(), a->tune.block,
+                             a->tune.tile_slots, a->stream));
+
+    auto sh = std::make_unique<SceneHost>();
+    sh->live = true;
+    sh->block_off = off;
+    sh->block_bytes = L.total;
+    sh->n_nodes = p->n_nodes;
+    sh->n_elems = p->n_elems;
+    sh->mat_type = p->mat_type;
+    sh->s0 = D.s0;
+    sh->horizon = D.horizon;
+    sh->tiles = T.tiles;
+    sh->dev = D;
+    sh->tiling = std::move(T);
+    p->consumed = true;
+    p->img.clear();
+    p->img.shrink_to_fit();
+    if (id == (int)a->scenes.size())
+        a->scenes.push_back(std::move(sh));
+    else
+        a->scenes[id] = std::move(sh);
+    a->tiles_dirty = true;
+    *scene_id = id;
+    return FW_OK;
+}
+
+// The arena's tuning for a caller's fw_tuning (NULL: the defaults), checked
+// against the compiled kernel shapes.
+int tuning_of(const fw_tuning* tuning, fw_tuning* out) {
+    fw_tuning t{};
+    t.tile_nodes = 512;
+    t.tile_slots = 1024;
+    t.block = 256;
+    t.grid_per_cu = 0;
+    if (tuning) {
+        int npt = 0;
+#define FW_NPT(B, S, N, P) \
+    if (tuning->block == B && tuning->tile_slots == S) npt = P;
+        FW_SHAPES(FW_NPT)
+#undef FW_NPT
+        if (!npt)
+            return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (256,1024) (512,1024) "
+                                      "(256,512) (512,2048) (1024,2048)");
+        if (tuning->tile_nodes < 1 || tuning->tile_nodes > npt * tuning->block)
+            return fail(FW_E_INVALID, "tile_nodes must be in [1, " + std::to_string(npt * tuning->block) + "]");
+        if (tuning->grid_per_cu < 0) return fail(FW_E_INVALID, "grid_per_cu must be >= 0");
+        t = *tuning;
+    }
+    *out = t;
+    return FW_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -688,23 +964,9 @@ int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning*
     HIPCHK(hipSetDevice(device));
     auto a = std::make_unique<fw_arena>();
     a->device = device;
-    a->tune.tile_nodes = 512;
-    a->tune.tile_slots = 1024;
-    a->tune.block = 256;
-    a->tune.grid_per_cu = 0;
-    if (tuning) {
-        // kernel shapes compiled in fw_kernels.hip: (threads, slots) -> owned nodes per thread
-        int npt = 0;
-#define FW_NPT(B, S, N, P) \
-    if (tuning->block == B && tuning->tile_slots == S) npt = P;
-        FW_SHAPES(FW_NPT)
-#undef FW_NPT
-        if (!npt)
-            return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (512,1024) (256,1024) "
-                                      "(256,512) (512,2048) (1024,2048)");
-        if (tuning->tile_nodes < 1 || tuning->tile_nodes > npt * tuning->block)
-            return fail(FW_E_INVALID, "tile_nodes must be in [1, " + std::to_string(npt * tuning->block) + "]");
-        a->tune = *tuning;
+    {
+        int rc = tuning_of(tuning, &a->tune);
+        if (rc) return rc;
     }
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -768,11 +1030,6 @@ int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
     return FW_OK;
 }
 
-static Layout layout_of(const fw_scene_desc* d, const Tiling& T) {
-    return make_layout(d, (int64_t)(T.rec.size() / 2), (int64_t)T.hel.size(), (int64_t)T.hnode.size(), (int64_t)T.csr_ptr.size(),
-                       (int64_t)T.tiles.size());
-}
-
 int32_t fw_scene_footprint(const fw_arena* ca, const fw_scene_desc* d, int64_t* bytes) {
     if (!ca || !bytes) return fail(FW_E_INVALID, "null argument");
     int rc = validate(d);
@@ -786,211 +1043,48 @@ int32_t fw_scene_footprint(const fw_arena* ca, const fw_scene_desc* d, int64_t* 
     return FW_OK;
 }
 
+int32_t fw_scene_prepare(const fw_scene_desc* d, const fw_tuning* tuning, fw_prepared** out) {
+    if (!out) return fail(FW_E_INVALID, "null argument");
+    *out = nullptr;
+    fw_tuning tune;
+    int rc = tuning_of(tuning, &tune);
+    if (rc) return rc;
+    std::unique_ptr<fw_prepared> P;
+    rc = prepare_scene(d, tune, nullptr, &P);
+    if (rc) return rc;
+    *out = P.release();
+    return FW_OK;
+}
+
+int32_t fw_prepared_footprint(const fw_prepared* p, int64_t* bytes) {
+    if (!p || !bytes) return fail(FW_E_INVALID, "null argument");
+    if (p->consumed) return fail(FW_E_INVALID, "prepared scene was already added");
+    *bytes = p->L.total;
+    return FW_OK;
+}
+
+int32_t fw_prepared_free(fw_prepared* p) {
+    delete p;
+    return FW_OK;
+}
+
+int32_t fw_scene_add_prepared(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
+    if (!a || !p || !scene_id) return fail(FW_E_INVALID, "null argument");
+    if (p->consumed) return fail(FW_E_INVALID, "prepared scene was already added");
+    if (p->tune.tile_nodes != a->tune.tile_nodes || p->tune.tile_slots != a->tune.tile_slots ||
+        p->tune.block != a->tune.block)
+        return fail(FW_E_INVALID, "prepared scene was tiled for another kernel shape than this arena's");
+    HIPCHK(hipSetDevice(a->device));
+    return commit_scene(a, p, scene_id);
+}
+
 int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     if (!a || !scene_id) return fail(FW_E_INVALID, "null argument");
-    int rc = validate(d);
-    if (rc) return rc;
     HIPCHK(hipSetDevice(a->device));
-    auto sh = std::make_unique<SceneHost>();
-    {
-        std::unique_ptr<Tiling> Tp;
-        rc = tiling_for(a, d, &Tp);
-        if (rc) return rc;
-        const int64_t need = layout_of(d, *Tp).total;
-        int id_probe = !a->free_ids.empty() ? a->free_ids.back() : (int)a->scenes.size();
-        if (id_probe >= a->max_scenes) {
-            cache_keep(a, std::move(Tp));
-            return fail(FW_E_CAPACITY, "arena holds at most " + std::to_string(a->max_scenes) + " scenes");
-        }
-        if (need > a->capacity - a->used) {
-            cache_keep(a, std::move(Tp));
-            return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(need) + " bytes, " +
-                                           std::to_string(a->capacity - a->used) + " free");
-        }
-        sh->tiling = std::move(*Tp);
-    }
-    Tiling& T = sh->tiling;
-    const int N = d->n_nodes, E = d->n_elems;
-    const int ntiles = (int)T.tiles.size();
-    Layout L = layout_of(d, T);
-    int id;
-    if (!a->free_ids.empty()) {
-        id = a->free_ids.back();
-    } else {
-        id = (int)a->scenes.size();
-    }
-    int64_t off = arena_alloc(a, L.total);
-    if (off < 0)
-        return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
-                                       std::to_string(a->capacity - a->used) + " free (fragmented)");
-    if (!a->free_ids.empty()) a->free_ids.pop_back();
-
-    // device addresses of the block's arrays
-    char* blk = a->base + off;
-    SceneDev& D = sh->dev;
-    std::memset(&D, 0, sizeof(D));
-    D.pos[0] = (float2*)(blk + L.pos0);
-    D.pos[1] = (float2*)(blk + L.pos1);
-    D.vel = (float2*)(blk + L.vel);
-    D.mass = (float*)(blk + L.mass);
-    D.flag = (const uint32_t*)(blk + L.flag);
-    D.bcv = (const float2*)(blk + L.bcv);
-    D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
-    D.force = (float2*)(blk + L.force);
-    D.cflag = (uint8_t*)(blk + L.cflag);
-    D.rest = (const float2*)(blk + L.rest);
-    D.hpos[0] = (float2*)(blk + L.hpos0);
-    D.hpos[1] = (float2*)(blk + L.hpos1);
-    D.jinv = (float4*)(blk + L.jinv);
-    D.area = (float*)(blk + L.area);
-    D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
-    D.j2[1] = L.j2b >= 0 ? (float4*)(blk + L.j2b) : nullptr;
-    D.stress = (float4*)(blk + L.stress);
-    D.strain = (float4*)(blk + L.strain);
-    D.yflag = (uint8_t*)(blk + L.yflag);
-    D.n_nodes = N;
-    D.n_elems = E;
-    D.hot.mat_type = d->mat_type;
-    D.hot.n_grounds = d->n_grounds;
-    D.hot.E = d->E;
-    D.hot.nu = d->nu;
-    D.rho = d->rho;
-    D.hot.sigma_y = d->sigma_y;
-    D.hot.H = d->H;
-    D.hot.dt = d->dt;
-    D.hot.damping = d->damping;
-    // warm-up runs at the next substep launched (local step -1)
-    D.s0 = (int32_t)(a->clock + 1);
-    D.horizon = d->horizon > 0 ? d->horizon : INT32_MAX;
-    D.live = 1;
-    for (int b = 0; b < 3; ++b) {
-        D.hot.aabb[b][0] = fw_f2o(1e30f);
-        D.hot.aabb[b][1] = fw_f2o(1e30f);
-        D.hot.aabb[b][2] = fw_f2o(-1e30f);
-        D.hot.aabb[b][3] = fw_f2o(-1e30f);
-    }
-    auto o8 = [](int64_t off) { return (uint32_t)(off >> 8); };  // layout offsets are 256-byte aligned
-    for (auto& td : T.tiles) {
-        td.base = blk;
-        td.o_rec = o8(L.rec);
-        td.o_hel = o8(L.hel);
-        td.o_bcv = o8(L.bcv);
-        td.o_pos[0] = o8(L.pos0);
-        td.o_pos[1] = o8(L.pos1);
-        td.o_hpos[0] = o8(L.hpos0);
-        td.o_hpos[1] = o8(L.hpos1);
-        td.o_rest = o8(L.rest);
-        td.o_hrest = o8(L.hrest);
-        td.o_vel = o8(L.vel);
-        td.o_mass = o8(L.mass);
-        td.o_flag = o8(L.flag);
-        td.o_disp = L.disp >= 0 ? o8(L.disp) : 0;
-        td.o_cptr = o8(L.cptr);
-        td.o_push = o8(L.push);
-        td.o_j2[0] = L.j2a >= 0 ? o8(L.j2a) : 0;
-        td.o_j2[1] = L.j2b >= 0 ? o8(L.j2b) : 0;
-        td.scene = id;
-        td.s0 = D.s0;
-        td.horizon = D.horizon;
-    }
-
-    // host image of the block
-    Staging* st = nullptr;
-    rc = get_staging(a, L.total, &st);
-    if (rc) {
-        arena_free(a, off, L.total);
-        return rc;
-    }
-    char* img = (char*)st->ptr;
-    std::memset(img, 0, L.total);
-    float2* pos0 = (float2*)(img + L.pos0);
-    float2* vel = (float2*)(img + L.vel);
-    uint32_t* flag = (uint32_t*)(img + L.flag);
-    for (int i = 0; i < N; ++i) {
-        int o = T.node_perm[i];
-        pos0[i] = make_float2(d->x0[2 * o], d->x0[2 * o + 1]);
-        vel[i] = make_float2(d->v0[2 * o], d->v0[2 * o + 1]);
-        flag[i] = d->flags[o];
-    }
-    std::memcpy(img + L.pos1, img + L.pos0, 8 * (size_t)N);
-    std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
-    {
-        float2* hp = (float2*)(img + L.hpos0);
-        for (size_t j = 0; j < T.hnode.size(); ++j) hp[j] = pos0[T.hnode[j]];
-        std::memcpy(img + L.hpos1, img + L.hpos0, 8 * T.hnode.size());
-        std::memcpy(img + L.hrest, img + L.hpos0, 8 * T.hnode.size());
-    }
-    std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
-    std::memcpy(img + L.rec, T.rec.data(), 4 * T.rec.size());
-    std::memcpy(img + L.hel, T.hel.data(), 4 * T.hel.size());
-    std::memcpy(img + L.push, T.push.data(), 4 * T.push.size());
-    std::memcpy(img + L.cptr, T.csr_ptr.data(), 2 * T.csr_ptr.size());
-    std::memcpy(img + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
-    for (int g = 0; g < d->n_grounds; ++g) {
-        const fw_ground_desc& gd = d->grounds[g];
-        GroundDev& G = D.ground[g];
-        G.kind = gd.kind;
-        G.penalty = gd.penalty;
-        for (int k = 0; k < 2; ++k) {
-            G.lo[k] = gd.aabb_lo[k];
-            G.hi[k] = gd.aabb_hi[k];
-            float b = 0.1f * (gd.aabb_hi[k] - gd.aabb_lo[k]);
-            G.plo[k] = gd.aabb_lo[k] - b;
-            G.phi[k] = gd.aabb_hi[k] + b;
-        }
-        if (gd.kind == FW_GROUND_PLANE) {
-            G.a[0] = gd.point[0];
-            G.a[1] = gd.point[1];
-            G.a[2] = gd.normal[0];
-            G.a[3] = gd.normal[1];
-        } else if (gd.kind == FW_GROUND_HEIGHTFIELD) {
-            G.a[0] = gd.hf_lb_x;
-            G.a[1] = gd.hf_ub_x;
-            G.nx = gd.hf_nx;
-            {
-                // sample spacing as the device query computes it: float32 subtraction, then division
-                volatile float span = gd.hf_ub_x - gd.hf_lb_x;
-                G.hf_dxw = span / (float)(gd.hf_nx - 1);
-            }
-            G.reverse = gd.reverse;
-            std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
-            G.data = blk + L.gdata[g];
-        } else {
-            G.a[0] = gd.vox_lb[0];
-            G.a[1] = gd.vox_lb[1];
-            G.a[2] = gd.vox_dx;
-            G.a[3] = gd.vox_dz;
-            G.nx = gd.vox_nx;
-            G.ny = gd.vox_ny;
-            build_voxel_extents(gd, (uint16_t*)(img + L.gdata[g]));
-            G.data = blk + L.gdata[g];
-        }
-    }
-    HIPCHK(hipMemcpyAsync(blk, img, L.total, hipMemcpyHostToDevice, a->stream));
-    HIPCHK(hipEventRecord(st->ev, a->stream));
-    st->busy = true;
-
-    rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
+    std::unique_ptr<fw_prepared> P;
+    int rc = prepare_scene(d, a->tune, a, &P);
     if (rc) return rc;
-    HIPCHK(fw_launch_prepare(a->d_scenes, (const TileDev*)(blk + L.tiles), 0, ntiles, a->tune.block, a->tune.tile_slots,
-                             a->stream));
-
-    sh->live = true;
-    sh->block_off = off;
-    sh->block_bytes = L.total;
-    sh->n_nodes = N;
-    sh->n_elems = E;
-    sh->mat_type = d->mat_type;
-    sh->s0 = D.s0;
-    sh->horizon = D.horizon;
-    sh->tiles = T.tiles;
-    if (id == (int)a->scenes.size())
-        a->scenes.push_back(std::move(sh));
-    else
-        a->scenes[id] = std::move(sh);
-    a->tiles_dirty = true;
-    *scene_id = id;
-    return FW_OK;
+    return commit_scene(a, P.get(), scene_id);
 }
 
 int32_t fw_scene_remove(fw_arena* a, int32_t id) {

@@ -91,8 +91,17 @@ class SceneSpec:
         bits 0..8: BC bits; bit 15: boundary node; bits 16..31: row of the
         returned float32[K, 2] value table holding that node's BC vector.
         """
-        table, inv = np.unique(self.bcv.astype(np.float32), axis=0, return_inverse=True)
-        inv = np.asarray(inv).reshape(-1)
+        bcv = np.ascontiguousarray(self.bcv, dtype=np.float32).reshape(-1, 2)
+        # rows are deduplicated by their bit pattern (one uint64 per row), so -0.0 and
+        # +0.0 stay distinct exactly as in the reference's per-node bc_values
+        keys = bcv.view(np.uint64).reshape(-1)
+        if keys.size == 0 or np.all(keys == keys[0]):
+            table = bcv[:1] if keys.size else np.zeros((1, 2), np.float32)
+            inv = np.zeros(keys.size, np.int64)
+        else:
+            uk, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+            table = bcv[first]
+            inv = np.asarray(inv).reshape(-1)
         if table.shape[0] > MAX_BC_VALUES:
             raise ValueError(f"scene has {table.shape[0]} distinct BC vectors; at most {MAX_BC_VALUES} are supported")
         flags = (self.bc.astype(np.int64) & NODE_BC_MASK) | (inv.astype(np.int64) << NODE_BCV_SHIFT)

@@ -23,7 +23,9 @@ failed at retirement without touching the other jobs.
 """
 
 import collections
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -57,11 +59,22 @@ class JobResult:
 
 
 class SimServer:
-    def __init__(self, batch=None, frame=60, keep_results=True, clock=time.perf_counter, **batch_kw):
+    """Continuous-batching request loop over one FemBatch.
+
+    ``prep_workers`` host threads tile and pack arriving jobs (the batch's
+    ``prepare``) in the background; admission then only copies the packed block
+    into the arena.  A batch without ``prepare`` is admitted with ``add``.
+    """
+
+    def __init__(self, batch=None, frame=60, keep_results=True, clock=time.perf_counter, prep_workers=None,
+                 **batch_kw):
         self.batch = batch or FemBatch(frame=frame, **batch_kw)
         self.frame = int(frame)
         self.keep_results = keep_results
         self.clock = clock
+        if prep_workers is None:
+            prep_workers = min(8, os.cpu_count() or 1)
+        self.prep_workers = prep_workers if hasattr(self.batch, "prepare") else 0
         self.reset_stats()
 
     def reset_stats(self):
@@ -87,23 +100,42 @@ class SimServer:
             if job.spec.horizon <= 0:
                 raise ValueError(f"job {job.job_id}: served jobs need a positive horizon")
 
+        pool = ThreadPoolExecutor(self.prep_workers) if self.prep_workers > 0 else None
+        prepared = {}  # job id -> Future of the job's prepared scene
+
         def admit():
             now = self.clock() - t0
             while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
+                job = pending.popleft()
+                waiting.append(job)
+                if pool is not None:
+                    prepared[job.job_id] = pool.submit(b.prepare, job.spec)
             free = b.free_bytes()
             while waiting:
                 job = waiting[0]
-                need = b.footprint(job.spec)
+                fut = prepared.get(job.job_id)
+                if fut is not None and not fut.done() and resident:
+                    break  # head of the line still being tiled; admit it at a later call
+                try:
+                    prep = fut.result() if fut is not None else None
+                except (RuntimeError, ValueError):
+                    # the job's mesh cannot be tiled (e.g. a node with too many elements): it fails alone
+                    waiting.popleft()
+                    prepared.pop(job.job_id, None)
+                    t = self.clock() - t0
+                    self.results.append(JobResult(job.job_id, job.arrival, t, t, job.spec.horizon, job.spec.n_elems, False))
+                    continue
+                need = prep.footprint if prep is not None else b.footprint(job.spec)
                 if need > free:
                     if not resident:
                         raise RuntimeError(f"job {job.job_id} needs {need} bytes; the empty arena has {free}")
                     break
                 try:
-                    sid = b.add(job.spec)
+                    sid = b.add_prepared(prep) if prep is not None else b.add(job.spec)
                 except RuntimeError:
                     break  # fragmented: wait for retirements
                 waiting.popleft()
+                prepared.pop(job.job_id, None)
                 resident[sid] = (job, self.clock() - t0)
                 free = b.free_bytes()
 
@@ -123,6 +155,18 @@ class SimServer:
                               x if self.keep_results else None, v)
                 )
 
+        self._t0 = t0
+        try:
+            self._loop(pending, waiting, resident, admit, retire, max_wall)
+        finally:
+            if pool is not None:
+                pool.shutdown(wait=True)
+        self.wall = self.clock() - t0
+        return self.results
+
+    def _loop(self, pending, waiting, resident, admit, retire, max_wall=None):
+        b = self.batch
+        t0 = self._t0
         in_flight = False
         launched = 0.0
         while pending or waiting or resident:
@@ -145,8 +189,6 @@ class SimServer:
         if in_flight:
             b.sync()
             self.busy_time += self.clock() - launched
-        self.wall = self.clock() - t0
-        return self.results
 
     def stats(self):
         lat = np.array([r.latency for r in self.results]) if self.results else np.zeros(1)

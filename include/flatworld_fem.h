@@ -149,6 +149,7 @@ typedef struct {
 } fw_arena_info;
 
 typedef struct fw_arena fw_arena;
+typedef struct fw_prepared fw_prepared;
 
 const char* fw_last_error(void);
 int32_t fw_abi_version(void);
@@ -164,6 +165,22 @@ int32_t fw_scene_footprint(const fw_arena* a, const fw_scene_desc* desc, int64_t
  * its zero-dt warm-up for the next substep.  Returns FW_E_CAPACITY when the
  * arena cannot hold it (nothing is changed then). */
 int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* desc, int32_t* scene_id);
+
+/* fw_scene_add in two halves, so that a server can do the host work of a scene
+ * (validation, tiling, packing its block) on worker threads while the GPU runs:
+ *   fw_scene_prepare      host only, thread-safe, no GPU and no arena needed; the
+ *                         scene is tiled for `tuning` (NULL: the defaults) and the
+ *                         description's arrays are copied, so they can be freed;
+ *   fw_prepared_footprint arena bytes the prepared scene will take (admission);
+ *   fw_scene_add_prepared copies it into an arena of the same kernel shape
+ *                         (FW_E_CAPACITY, nothing changed, when it does not fit;
+ *                         it can be retried later); a prepared scene is added once;
+ *   fw_prepared_free      releases it (added or not).
+ * Same reference interfaces as fw_scene_add. */
+int32_t fw_scene_prepare(const fw_scene_desc* desc, const fw_tuning* tuning, fw_prepared** out);
+int32_t fw_prepared_footprint(const fw_prepared* p, int64_t* bytes);
+int32_t fw_scene_add_prepared(fw_arena* a, fw_prepared* p, int32_t* scene_id);
+int32_t fw_prepared_free(fw_prepared* p);
 int32_t fw_scene_remove(fw_arena* a, int32_t scene_id);
 /* Restart the scene's BVH-rebuild schedule: the next substep counts as step 0
  * of the activation gate (ExplicitLoop.reset, flatworld/explicitloop.py:635-679). */

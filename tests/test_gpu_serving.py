@@ -83,3 +83,29 @@ def test_failed_job_is_isolated():
     assert not res[1].ok
     for k in (0, 2, 3):
         assert res[k].ok and np.array_equal(res[k].positions, solo(jobs[k].spec))
+
+
+def test_prepared_scenes_equal_direct_adds():
+    """Two-phase admission (host prepare on a worker thread, then add) gives the same bits as add()."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from flatworld_amd import _native
+
+    jobs = small_jobs(6, seed=24)
+    log("[case] prepared admission vs direct add")
+    a = FemBatch(tile_nodes=256)
+    b = FemBatch(tile_nodes=256)
+    ia = [a.add(j.spec) for j in jobs]
+    with ThreadPoolExecutor(3) as ex:
+        preps = list(ex.map(lambda j: b.prepare(j.spec), jobs))
+    ib = [b.add_prepared(p) for p in preps]
+    a.step(61, graph=False)
+    b.step(61, graph=False)
+    for x, y in zip(ia, ib):
+        assert np.array_equal(a.positions(x), b.positions(y))
+        assert np.array_equal(a.velocities(x), b.velocities(y))
+    other = _native.Prepared(jobs[0].spec, dict(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0))
+    with pytest.raises(ValueError, match="another kernel shape"):
+        b.add_prepared(other)
+    a.close()
+    b.close()

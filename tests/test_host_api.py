@@ -11,6 +11,7 @@
   irregular meshes; argument errors map to the reference's exception types.
 """
 
+import dataclasses
 import json
 import os
 import re
@@ -119,6 +120,23 @@ def test_bc_packing_matches_restatement():
     assert bc[7] == FORCETYPE | GRAVITY | UTYPE and bc[8] == GRAVITY | UTYPE | VTYPE and bc[10] == GRAVITY | ATYPE
 
 
+def test_bc_table_keeps_signed_zeros_distinct():
+    """The BC value table is keyed by bit pattern: a node whose value is -0.0 keeps -0.0."""
+    conn, coords = fw.structured_grid(3)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)),
+                       [fw.Gravity([0.0, -9.8]), fw.Force([1, 2], [-0.0, 1.0]), fw.Force([5], [0.0, 1.0])])
+    spec = fw.pack_fem_domain(dom)
+    flags, table = spec.node_flags()
+    got = table[flags >> 16]
+    assert np.array_equal(got.view(np.uint32), spec.bcv.astype(np.float32).view(np.uint32))
+    assert np.signbit(got[1, 0]) and not np.signbit(got[5, 0])
+    assert table.shape[0] == 3
+    # one distinct row: the single-row table
+    dom1 = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
+    f1, t1 = fw.pack_fem_domain(dom1).node_flags()
+    assert t1.shape == (1, 2) and np.all((f1 >> 16) == 0) and t1[0, 1] == np.float32(-9.8)
+
+
 def test_stable_time_and_penalty_scalars():
     conn, coords = fw.structured_grid(10, extent=0.4)
     mat = fw.Elastic(5e4, 0.35, 900.0)
@@ -212,3 +230,24 @@ def test_explicitloop_rejects_out_of_scope_domains():
         ExplicitLoop(0.0, [a, b])
     with pytest.raises(NotImplementedError, match="mass scaling"):
         ExplicitLoop(0.0, [a], mass_scaling_dt=1e-3)
+
+
+def test_prepare_is_host_only_and_thread_safe():
+    """fw_scene_prepare needs no GPU; concurrent preparations agree with a serial one."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from flatworld_amd import workloads as W
+
+    rng = W.rng_for(4)
+    specs = [W.serving_job(rng, 8, 40, 60, 90) for _ in range(12)]
+    tune = dict(tile_nodes=256, tile_slots=512, block=256, grid_per_cu=0)
+    serial = [_native.Prepared(s, tune).footprint for s in specs]
+    with ThreadPoolExecutor(6) as ex:
+        par = list(ex.map(lambda s: _native.Prepared(s, tune).footprint, specs))
+    assert serial == par and all(f > 0 for f in serial)
+    with pytest.raises(ValueError, match="unsupported"):
+        _native.Prepared(specs[0], dict(tile_nodes=256, tile_slots=777, block=256, grid_per_cu=0))
+    bad = dataclasses.replace(specs[0], conn=specs[0].conn.copy())
+    bad.conn[0, 0] = bad.n_nodes + 5
+    with pytest.raises(ValueError, match="outside"):
+        _native.Prepared(bad, tune)

@@ -108,3 +108,37 @@ def test_poisson_arrivals_rate():
     t = poisson_arrivals(20000, 500.0, np.random.default_rng(0))
     assert np.all(np.diff(t) > 0)
     assert abs(20000 / t[-1] - 500.0) < 15.0
+
+
+class FakePrepBatch(FakeBatch):
+    """FakeBatch with the two-phase admission of FemBatch (prepare on worker threads)."""
+
+    def __init__(self, capacity):
+        super().__init__(capacity)
+        self.prepared_on = set()
+
+    def prepare(self, spec):
+        import threading
+
+        if getattr(spec, "untileable", False):
+            raise ValueError("node has too many element corners")
+        self.prepared_on.add(threading.get_ident())
+        return SimpleNamespace(spec=spec, footprint=spec.bytes)
+
+    def add_prepared(self, prep):
+        return self.add(prep.spec)
+
+
+def test_prepared_admission_keeps_fifo_and_isolates_untileable_jobs():
+    b = FakePrepBatch(capacity=100)
+    srv = SimServer(batch=b, frame=60, prep_workers=3)
+    jobs = [Job(i, spec(i, h, 40), 0.0) for i, h in enumerate([30, 200, 59, 60, 500])]
+    bad = spec(9, 60, 10)
+    bad.untileable = True
+    jobs.append(Job(9, bad, 0.0))
+    res = {r.job_id: r for r in srv.run(jobs)}
+    assert sorted(res) == [0, 1, 2, 3, 4, 9]
+    assert not res[9].ok and all(res[i].ok for i in range(5))
+    assert b.admit_order == [0, 1, 2, 3, 4]
+    assert b.max_resident == 2
+    assert len(b.prepared_on) >= 1
