@@ -30,6 +30,7 @@ from . import _native
 # (threads, element slots, owned nodes per tile) of the compiled kernel shapes
 SHAPES = [
     (256, 1024, 512),
+    (256, 768, 384),
     (512, 1024, 512),
     (256, 512, 256),
     (512, 2048, 1024),

@@ -326,6 +326,40 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
     return FW_OK;
 }
 
+// Height-field block: nx float heights, then (8-byte aligned) nx float2 unit normals.
+int64_t hf_normals_offset(int nx) { return (4 * (int64_t)nx + 7) / 8 * 8; }
+int64_t hf_bytes(int nx) { return hf_normals_offset(nx) + 8 * (int64_t)nx; }
+
+// The height-field query's normal (flatworld/mixedcontact.py:193-214) depends on the
+// query point only through the rounded sample index ir: a central difference over
+// the clamped samples ir - 1, ir + 1, guarded by denom > 1e-6, negated when
+// `reverse`, normalised when its length exceeds 1e-12.  Evaluated here once per
+// sample with the same float32 operations (no contraction, IEEE division and square
+// root), so the device looks the normal up instead of recomputing it per node.
+void build_hf_normals(const GroundDev& G, const float* h, float2* nrm) {
+    const int nx = G.nx;
+    for (int ir = 0; ir < nx; ++ir) {
+        int j0 = ir - 1, j1 = ir + 1;
+        if (j0 < 0) j0 = 0;
+        if (j1 >= nx) j1 = nx - 1;
+        volatile float denom = G.hf_dxw * (float)(j1 - j0);
+        float dhdx = 0.0f;
+        if (denom > 1e-6f) dhdx = (h[j1] - h[j0]) / denom;
+        float n0 = -dhdx, n1 = 1.0f;
+        if (G.reverse == 1) {
+            n0 = -n0;
+            n1 = -n1;
+        }
+        volatile float sq = n0 * n0;
+        volatile float nlen = std::sqrt(sq + n1 * n1);
+        if (nlen > 1e-12f) {
+            n0 = n0 / nlen;
+            n1 = n1 / nlen;
+        }
+        nrm[ir] = make_float2(n0, n1);
+    }
+}
+
 // ------------------------------------------------------------------ scene layout
 
 struct Layout {
@@ -379,7 +413,7 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
         L.gdata[g] = -1;
         if (g >= d->n_grounds) continue;
         const fw_ground_desc& gd = d->grounds[g];
-        if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(4 * (int64_t)gd.hf_nx);
+        if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(hf_bytes(gd.hf_nx));
         if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
     }
     L.total = o;
@@ -812,6 +846,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             }
             G.reverse = gd.reverse;
             std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
+            build_hf_normals(G, gd.heights, (float2*)(img + L.gdata[g] + hf_normals_offset(gd.hf_nx)));
         } else {
             G.a[0] = gd.vox_lb[0];
             G.a[1] = gd.vox_lb[1];
@@ -934,8 +969,8 @@ int tuning_of(const fw_tuning* tuning, fw_tuning* out) {
         FW_SHAPES(FW_NPT)
 #undef FW_NPT
         if (!npt)
-            return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (256,1024) (512,1024) "
-                                      "(256,512) (512,2048) (1024,2048)");
+            return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (256,1024) (256,768) "
+                                      "(512,1024) (256,512) (512,2048) (1024,2048)");
         if (tuning->tile_nodes < 1 || tuning->tile_nodes > npt * tuning->block)
             return fail(FW_E_INVALID, "tile_nodes must be in [1, " + std::to_string(npt * tuning->block) + "]");
         if (tuning->grid_per_cu < 0) return fail(FW_E_INVALID, "grid_per_cu must be >= 0");
@@ -1162,7 +1197,7 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
     if (rc) return rc;
     Tiling T;
     const int node_cap = node_cap_of_slots(tile_slots);
-    if (node_cap == 0) return fail(FW_E_INVALID, "tile_slots must be one of the compiled shapes (512, 1024, 2048)");
+    if (node_cap == 0) return fail(FW_E_INVALID, "tile_slots must be one of the compiled shapes (512, 768, 1024, 2048)");
     rc = build_tiling(d, tile_nodes, tile_slots, node_cap, T);
     if (rc) return rc;
     const int N = d->n_nodes, E = d->n_elems;

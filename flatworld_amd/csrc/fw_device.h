@@ -139,9 +139,12 @@ __device__ __forceinline__ bool stress_j2(const Mat& m, const float F[4], float4
 // ------------------------------------------------------------------ grounds
 
 // flatworld/mixedcontact.py:166-217 (_nearest_on_curve_2d); returns the signed distance.
+// The unit normal at the rounded sample index comes from the per-sample table the
+// host evaluated with the same float32 operations (nx float2 after the heights).
 __device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z, float& n0o, float& n1o) {
     const float* hgt = (const float*)g.data;
     const int nx = g.nx;
+    const float2* nrm = (const float2*)(hgt + ((nx + 1) & ~1));
     const float lbx = g.a[0], ubx = g.a[1];
     float span = ubx - lbx;
     float u = 0.0f;
@@ -155,30 +158,14 @@ __device__ __forceinline__ float hf_signed(const GroundDev& g, float x, float z,
     int ir = (int)roundf(sg);
     if (ir < 0) ir = 0;
     if (ir >= nx) ir = nx - 1;
-    int j0 = ir - 1, j1 = ir + 1;
-    if (j0 < 0) j0 = 0;
-    if (j1 >= nx) j1 = nx - 1;
-    // the four samples in one round trip
-    const float h0 = fw_ld(hgt + i0), h1 = fw_ld(hgt + i1), hj0 = fw_ld(hgt + j0), hj1 = fw_ld(hgt + j1);
+    // the three samples in one round trip
+    const float h0 = fw_ld(hgt + i0), h1 = fw_ld(hgt + i1);
+    const float2 n = fw_ld(nrm + ir);
     float t = sg - (float)i0;
     float h = h0 + (h1 - h0) * t;
-    float dxw = g.hf_dxw;  // span / (nx - 1), rounded once in float32 on the host
-    float denom = dxw * (float)(j1 - j0);
-    float dhdx = 0.0f;
-    if (denom > 1e-6f) dhdx = (hj1 - hj0) / denom;
-    float n0 = -dhdx, n1 = 1.0f;
-    if (g.reverse == 1) {
-        n0 = -n0;
-        n1 = -n1;
-    }
-    float nlen = sqrtf(n0 * n0 + n1 * n1);
-    if (nlen > 1e-12f) {
-        n0 = n0 / nlen;
-        n1 = n1 / nlen;
-    }
-    n0o = n0;
-    n1o = n1;
-    return (x - x) * n0 + (z - h) * n1;
+    n0o = n.x;
+    n1o = n.y;
+    return (x - x) * n.x + (z - h) * n.y;
 }
 
 // flatworld/mixedcontact.py:220-302 (_voxel_signed_distance_2d, interior cells only).

@@ -381,8 +381,17 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                     if (active & (1u << gi)) hit |= ground_force(fw_ldc(&Sp->ground[gi]), p.x, p.y, v.x, v.y, dt, fx, fy);
             }
             // element forces in ascending element id: a contiguous run of LDS entries
+            // (two entries per iteration, so that their LDS reads overlap; same order)
+            int j = lds_cp[i];
             const int j_end = lds_cp[i + 1];
-            for (int j = lds_cp[i]; j < j_end; ++j) {
+            for (; j + 1 < j_end; j += 2) {
+                const float2 f0 = lds_f[j], f1 = lds_f[j + 1];
+                fx = fx + (-f0.x);
+                fy = fy + (-f0.y);
+                fx = fx + (-f1.x);
+                fy = fy + (-f1.y);
+            }
+            if (j < j_end) {
                 const float2 f = lds_f[j];
                 fx = fx + (-f.x);
                 fy = fy + (-f.y);

@@ -116,9 +116,10 @@ struct TileDev {
 // Compiled substep kernel shapes: (threads, element slots, LDS nodes, owned nodes per
 // thread).  LDS per workgroup = 16 B per LDS node (positions, rest positions) + 24 B
 // per slot (three corner forces) + 2 B per CSR offset (owned nodes + 1), sized so
-// that 4 (1024 slots), 8 (512) or 2 (2048) workgroups share a CU's 160 KiB.
+// that 4 (1024 slots), 5 (768), 8 (512) or 2 (2048) workgroups share a CU's 160 KiB.
 #define FW_SHAPES(X)         \
     X(256, 1024, 952, 2)     \
+    X(256, 768, 704, 2)      \
     X(512, 1024, 952, 1)     \
     X(256, 512, 472, 1)      \
     X(512, 2048, 1912, 2)    \

@@ -128,7 +128,7 @@ typedef struct {
 
 typedef struct {
     int32_t tile_nodes;   /* max owned nodes per tile */
-    int32_t tile_slots;   /* max element slots (owned + halo) per tile: 1024 or 2048 */
+    int32_t tile_slots;   /* max element slots (owned + halo) per tile: 512, 768, 1024 or 2048 */
     int32_t block;        /* threads per workgroup: 256 */
     int32_t grid_per_cu;  /* persistent workgroups per CU; 0 = one workgroup per tile */
 } fw_tuning;
