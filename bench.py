@@ -156,8 +156,8 @@ def main():
     ap.add_argument("--grid-per-cu", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="default kernel shape instead of the autotuner")
     ap.add_argument("--per-scene-steps", type=int, default=6)
-    ap.add_argument("--serve-jobs", type=int, default=300, help="jobs in the served Poisson stream (0: skip)")
-    ap.add_argument("--serve-burst", type=int, default=150, help="jobs of the closed-loop capacity burst")
+    ap.add_argument("--serve-jobs", type=int, default=600, help="jobs in the served Poisson stream (0: skip)")
+    ap.add_argument("--serve-burst", type=int, default=300, help="jobs of the closed-loop capacity burst")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph frames")

@@ -116,19 +116,32 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
                 }
             }
         std::vector<uint32_t> key(N);
+        double scale[2];
+        for (int k = 0; k < 2; ++k) {
+            const double span = (double)hi[k] - (double)lo[k];
+            scale[k] = span > 0 ? 65535.0 / span : 0.0;
+        }
         for (int i = 0; i < N; ++i) {
             uint32_t q[2];
             for (int k = 0; k < 2; ++k) {
-                double span = (double)hi[k] - (double)lo[k];
-                double t = span > 0 ? ((double)d->x0[2 * i + k] - lo[k]) / span : 0.0;
+                double t = ((double)d->x0[2 * i + k] - lo[k]) * scale[k];
                 if (!(t >= 0.0)) t = 0.0;
-                if (t > 1.0) t = 1.0;
-                q[k] = (uint32_t)(t * 65535.0);
+                if (t > 65535.0) t = 65535.0;
+                q[k] = (uint32_t)t;
             }
             key[i] = spread16(q[0]) | (spread16(q[1]) << 1);
         }
-        std::stable_sort(T.node_perm.begin(), T.node_perm.end(),
-                         [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+        // stable LSD radix sort of the node ids by key, two 16-bit passes
+        std::vector<int32_t> tmp(N);
+        std::vector<uint32_t> cnt(65537);
+        for (int pass = 0; pass < 2; ++pass) {
+            const int sh = 16 * pass;
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (int i = 0; i < N; ++i) cnt[((key[T.node_perm[i]] >> sh) & 0xFFFFu) + 1]++;
+            for (int b = 0; b < 65536; ++b) cnt[b + 1] += cnt[b];
+            for (int i = 0; i < N; ++i) tmp[cnt[(key[T.node_perm[i]] >> sh) & 0xFFFFu]++] = T.node_perm[i];
+            T.node_perm.swap(tmp);
+        }
     }
     T.node_inv.assign(N, 0);
     for (int i = 0; i < N; ++i) T.node_inv[T.node_perm[i]] = i;
@@ -367,6 +380,7 @@ struct Layout {
     int64_t jinv, area, j2a, j2b, stress, strain, yflag;
     int64_t rec, hel, hpos0, hpos1, hrest, push, cptr, tiles;
     int64_t gdata[FW_MAX_GROUNDS];
+    int64_t host_bytes;  // [0, host_bytes) is built on the host; the rest is cleared on the device
     int64_t total;
 };
 
@@ -380,13 +394,33 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
         return r;
     };
     const int64_t N = d->n_nodes, E = d->n_elems;
+    // arrays the host fills, uploaded at admission
     L.pos0 = take(8 * N);
-    L.pos1 = take(8 * N);
     L.rest = take(8 * N);
     L.vel = take(8 * N);
-    L.mass = take(4 * N);
     L.flag = take(4 * N);
     L.bcv = take(8 * std::max<int64_t>(d->n_bc_values, 1));
+    L.rec = take(8 * n_slots);
+    L.hel = take(4 * std::max<int64_t>(n_hslots, 1));
+    L.hpos0 = take(8 * n_hnode);
+    L.hrest = take(8 * n_hnode);
+    L.push = take(8 * n_hnode);
+    L.cptr = take(2 * n_ptr);
+    L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
+    for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
+        L.gdata[g] = -1;
+        if (g >= d->n_grounds) continue;
+        const fw_ground_desc& gd = d->grounds[g];
+        if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(hf_bytes(gd.hf_nx));
+        if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
+    }
+    L.host_bytes = o;
+    // arrays that start at zero (cleared on the device) or that the device writes
+    // before reading: the second position and halo buffers (the warm-up substep
+    // writes them), mass / J^-1 / area (prepare kernel), U, J2 state, diagnostics
+    L.pos1 = take(8 * N);
+    L.hpos1 = take(8 * n_hnode);
+    L.mass = take(4 * N);
     L.disp = d->track_disp ? take(8 * N) : -1;
     L.force = take(8 * N);
     L.cflag = take(N);
@@ -401,21 +435,6 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
     L.stress = take(16 * E);
     L.strain = take(16 * E);
     L.yflag = take(E);
-    L.rec = take(8 * n_slots);
-    L.hel = take(4 * std::max<int64_t>(n_hslots, 1));
-    L.hpos0 = take(8 * n_hnode);
-    L.hpos1 = take(8 * n_hnode);
-    L.hrest = take(8 * n_hnode);
-    L.push = take(8 * n_hnode);
-    L.cptr = take(2 * n_ptr);
-    L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
-    for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
-        L.gdata[g] = -1;
-        if (g >= d->n_grounds) continue;
-        const fw_ground_desc& gd = d->grounds[g];
-        if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(hf_bytes(gd.hf_nx));
-        if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
-    }
     L.total = o;
     return L;
 }
@@ -794,7 +813,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
     }
 
     // host image of the block
-    P->img.assign((size_t)L.total, 0);
+    P->img.assign((size_t)L.host_bytes, 0);
     char* img = P->img.data();
     float2* pos0 = (float2*)(img + L.pos0);
     float2* vel = (float2*)(img + L.vel);
@@ -805,12 +824,10 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
         vel[i] = make_float2(d->v0[2 * o], d->v0[2 * o + 1]);
         flag[i] = d->flags[o];
     }
-    std::memcpy(img + L.pos1, img + L.pos0, 8 * (size_t)N);
     std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
     {
         float2* hp = (float2*)(img + L.hpos0);
         for (size_t j = 0; j < T.hnode.size(); ++j) hp[j] = pos0[T.hnode[j]];
-        std::memcpy(img + L.hpos1, img + L.hpos0, 8 * T.hnode.size());
         std::memcpy(img + L.hrest, img + L.hpos0, 8 * T.hnode.size());
     }
     std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
@@ -873,7 +890,7 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
         return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
                                        std::to_string(a->capacity - a->used) + " free");
     Staging* st = nullptr;
-    int rc = get_staging(a, L.total, &st);
+    int rc = get_staging(a, L.host_bytes, &st);
     if (rc) return rc;
     int64_t off = arena_alloc(a, L.total);
     if (off < 0)
@@ -919,11 +936,12 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
         td.scene = id;
         td.s0 = D.s0;
     }
-    std::memcpy(st->ptr, p->img.data(), (size_t)L.total);
+    std::memcpy(st->ptr, p->img.data(), (size_t)L.host_bytes);
     std::memcpy((char*)st->ptr + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
-    HIPCHK(hipMemcpyAsync(blk, st->ptr, L.total, hipMemcpyHostToDevice, a->stream));
+    HIPCHK(hipMemcpyAsync(blk, st->ptr, L.host_bytes, hipMemcpyHostToDevice, a->stream));
     HIPCHK(hipEventRecord(st->ev, a->stream));
     st->busy = true;
+    HIPCHK(hipMemsetAsync(blk + L.host_bytes, 0, (size_t)(L.total - L.host_bytes), a->stream));
 
     rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
     if (rc) return rc;

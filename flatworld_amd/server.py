@@ -73,7 +73,7 @@ class SimServer:
         self.keep_results = keep_results
         self.clock = clock
         if prep_workers is None:
-            prep_workers = min(8, os.cpu_count() or 1)
+            prep_workers = min(16, os.cpu_count() or 1)
         self.prep_workers = prep_workers if hasattr(self.batch, "prepare") else 0
         self.reset_stats()
 
