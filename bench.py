@@ -10,6 +10,7 @@ in HBM before timing starts.
 Printed JSON line:
   value            element-substeps/s of the batched, graph-replayed form
   per_scene        the same scenes advanced with one launch per scene per substep
+                   (hbm_frac: algorithmic bytes per second over the HBM peak)
   roofline         algorithmic bytes per substep / average substep-kernel time,
                    measured live with HIP events on the launch stream; traffic =
                    HBM bytes per launch from the committed rocprof counter profile
@@ -18,7 +19,8 @@ Printed JSON line:
                    by the autotuner (flatworld_amd.autotune) for this workload
   serving          a Poisson stream of jobs (grid side 16-256, random material and
                    ground, horizon 60-600 substeps) at 80% of the capacity measured
-                   on a closed-loop burst: p50 / p99 request latency, throughput
+                   on a closed-loop burst: p50 / p99 request latency, throughput,
+                   hbm_frac of the served work over the wall clock
   cpu_baseline     the oracle (serial C restatement) on a bounded sample
 """
 
@@ -110,6 +112,9 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
     srv.run([Job(i, specs[i], float(arrivals[i])) for i in range(n_jobs)])
     st = srv.stats()
     srv.batch.close()
+    # algorithmic bytes the served stream moved per second (each job: its per-substep bytes x horizon)
+    done = {r.job_id for r in srv.results}
+    served_bytes = sum(W.algorithmic_bytes(specs[i]) * specs[i].horizon for i in done)
     log(f"[bench] serving: {st['jobs']} jobs at {rate:.1f}/s, p50 {st['latency_p50_s'] * 1e3:.1f} ms, "
         f"p99 {st['latency_p99_s'] * 1e3:.1f} ms, {st['elem_substeps_per_s']:.3e} elem-substeps/s")
     return {
@@ -121,6 +126,7 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
         "latency_p99_ms": st["latency_p99_s"] * 1e3,
         "latency_mean_ms": st["latency_mean_s"] * 1e3,
         "elem_substeps_per_s": st["elem_substeps_per_s"],
+        "hbm_frac": served_bytes / st["wall_s"] / 1e9 / HBM_PEAK_GBS,
         "burst_elem_substeps_per_s": burst["elem_substeps_per_s"],
         "gpu_busy_frac": st["gpu_busy_frac"],
         "frame_substeps": frame,
@@ -247,6 +253,7 @@ def main():
             "unit": "element-substeps/s",
             "launches_per_substep": len(specs),
             "ms_per_step": ps_total_ms / args.per_scene_steps,
+            "hbm_frac": algo / (ps_total_ms / args.per_scene_steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "batched_speedup": value / ps_value,
         }
         log(f"[bench] per-scene launches: {ps_total_ms / args.per_scene_steps:.2f} ms/substep")
