@@ -203,3 +203,69 @@ def test_every_kernel_shape_gives_identical_results():
             for k, (x, v, f) in enumerate(got):
                 assert np.array_equal(x, ref[k][0]) and np.array_equal(v, ref[k][1]) and np.array_equal(f, ref[k][2]), \
                     f"shape ({block},{slots},{nodes}) grid_per_cu={gpc} scene {k}"
+
+
+# ---------------------------------------------------------------- edge cases
+
+
+def _nan_equal(a, b):
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(np.where(na, 0.0, a), np.where(nb, 0.0, b))
+
+
+def test_ragged_arena_bitexact():
+    """One- and two-triangle scenes next to a multi-tile 64x64 scene, three materials, three ground kinds."""
+    rng = W.rng_for(41)
+    tri = fw.FemDomain(fw.Mesh(2, np.array([[0, 1, 2]], np.int32), np.array([[0.4, 0.0005], [0.6, 0.0005], [0.5, 0.2]], np.float32)),
+                       fw.SolidProp(fw.Elastic(2e4, 0.2, 40.0)), [fw.Gravity([0.0, -9.8])],
+                       initials=[fw.InitialVel("ALL", [0.0, -0.5])])
+    hf = W.sinusoid_heightfield(rng)
+    y0 = float(hf.height.max()) + 0.001
+    two = fw.FemDomain(fw.Mesh(2, np.array([[0, 1, 3], [0, 3, 2]], np.int32),
+                               np.array([[0.5, y0], [0.7, y0], [0.5, y0 + 0.2], [0.7, y0 + 0.2]], np.float32)),
+                       fw.SolidProp(fw.HyperElastic(3e4, 0.3, 50.0)), [fw.Gravity([0.0, -9.8])],
+                       initials=[fw.InitialVel("ALL", [0.0, -0.5])])
+    big = W.grid_domain(64, (0.25, 0.105), 0.5, fw.MisesPlastic(1e5, 0.3, 300.0, 500.0, 1e3), [fw.Gravity([0.0, -9.8])],
+                        initials=[fw.InitialVel("ALL", [0.0, -1.0])])
+    mid = W.grid_domain(8, (0.45, 0.2), 0.1, fw.Elastic(1e4, 0.25, 100.0), [fw.Gravity([0.0, -9.8])])
+    specs = [
+        W.one_body_spec(tri, fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])),
+        W.one_body_spec(two, hf),
+        W.one_body_spec(big, W.value_noise_voxels(rng, 64, 32, [0.0, 0.0], [1.0, 0.1], floor_rows=2)),
+        W.one_body_spec(mid, fw.GroundDomain(2, [0.0, 0.19], [0.05, 1.0])),
+    ]
+    log("[case] ragged arena: 1, 2, 128, 8192 triangles")
+    arena = _native.Arena(capacity_bytes=1 << 30, tile_nodes=512)
+    ids = [arena.add(s) for s in specs]
+    arena.step(1, -1.0, 0)
+    arena.step(120, -1.0, _native.FW_STEP_DIAG)
+    assert arena.info()["n_tiles"] > len(specs)
+    for k, s in enumerate(specs):
+        o = OracleScene(s)
+        o.warmup()
+        o.run(120)
+        assert np.isfinite(o.x).all()
+        for fld, ref in ((_native.FIELD_X, o.x), (_native.FIELD_V, o.v), (_native.FIELD_FORCE, o.fext)):
+            assert np.array_equal(arena.read(ids[k], fld), ref), f"scene {k} field {fld}"
+    arena.close()
+
+
+def test_degenerate_element_and_isolated_node_match_oracle():
+    """A zero-area element (repeated node) and a node no element references: NaN appears where the oracle's does."""
+    conn, coords = fw.structured_grid(5, lower=(0.3, 0.2), extent=0.3)
+    coords = np.concatenate([coords, [[0.9, 0.9]]]).astype(np.float32)
+    conn = np.concatenate([conn, [[0, 1, 1]]]).astype(np.int32)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(2e4, 0.3, 100.0)), [fw.Gravity([0.0, -9.8])])
+    spec = W.one_body_spec(dom, fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0]), dt=1e-4)
+    log("[case] degenerate element + isolated node")
+    for tile_nodes, tile_slots, block in ((512, 1024, 256), (8, 512, 256)):
+        g = GpuScene(spec, tile_nodes=tile_nodes, tile_slots=tile_slots, block=block)
+        o = OracleScene(spec)
+        g.warmup()
+        o.warmup()
+        g.run(20)
+        o.run(20)
+        for name, a, b in (("x", g.read(_native.FIELD_X), o.x), ("v", g.read(_native.FIELD_V), o.v)):
+            assert _nan_equal(a, b), f"{name} (tile_nodes {tile_nodes})"
+        assert np.isnan(o.x[-1]).any(), "the massless isolated node should be NaN, as in the reference"
+        g.arena.close()

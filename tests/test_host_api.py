@@ -251,3 +251,35 @@ def test_prepare_is_host_only_and_thread_safe():
     bad.conn[0, 0] = bad.n_nodes + 5
     with pytest.raises(ValueError, match="outside"):
         _native.Prepared(bad, tune)
+
+
+def _fan(n_tri, r=0.1):
+    """n_tri triangles around one centre node (the centre has n_tri element corners)."""
+    ang = np.linspace(0.0, 2.0 * np.pi, n_tri, endpoint=False)
+    coords = np.concatenate([[[0.5, 0.5]], 0.5 + r * np.stack([np.cos(ang), np.sin(ang)], 1)]).astype(np.float32)
+    conn = np.array([[0, 1 + k, 1 + (k + 1) % n_tri] for k in range(n_tri)], dtype=np.int32)
+    return conn, coords
+
+
+def test_node_degree_limit_is_reported():
+    """A node with more than 32 element corners cannot be encoded in a slot record: a clear error."""
+    tune = dict(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
+    ok_conn, ok_coords = _fan(32)
+    dom = fw.FemDomain(fw.Mesh(2, ok_conn, ok_coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
+    assert _native.Prepared(fw.pack_fem_domain(dom), tune).footprint > 0
+    conn, coords = _fan(33)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
+    with pytest.raises(NotImplementedError, match="at most 32"):
+        _native.Prepared(fw.pack_fem_domain(dom), tune)
+
+
+def test_isolated_node_and_degenerate_element_tile():
+    """A node that no element references and a zero-area element still give a valid tiling."""
+    conn, coords = fw.structured_grid(5)
+    coords = np.concatenate([coords, [[2.0, 2.0]]]).astype(np.float32)  # isolated node
+    conn = np.concatenate([conn, [[0, 1, 1]]]).astype(np.int32)  # repeated node: zero area
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
+    spec = fw.pack_fem_domain(dom)
+    for tile_nodes, tile_slots in ((512, 1024), (8, 512)):
+        stats = _native.tiling_check(spec, tile_nodes, tile_slots)
+        assert stats["tiles"] >= 1 and stats["csr_entries"] == 3 * spec.n_elems
