@@ -191,6 +191,7 @@ def main():
     ap.add_argument("--per-scene-steps", type=int, default=6)
     ap.add_argument("--serve-jobs", type=int, default=600, help="jobs in the served Poisson stream (0: skip)")
     ap.add_argument("--serve-burst", type=int, default=300, help="jobs of the closed-loop capacity burst")
+    ap.add_argument("--serve-frame", type=int, default=60, help="substeps per hipGraph frame of the served stream")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph frames")
@@ -288,7 +289,7 @@ def main():
 
     serving = None
     if args.serve_jobs > 0 and args.config == "batched":
-        serving = serving_bench(args.serve_jobs, args.seed, arena_kw, frame, args.serve_burst)
+        serving = serving_bench(args.serve_jobs, args.seed, arena_kw, args.serve_frame, args.serve_burst)
 
     cpu = None
     if not args.no_cpu:
