@@ -1,7 +1,8 @@
 """flatworld_amd -- FlatWorld's explicit-FEM substep, native to MI355X (gfx950).
 
-Drop-in for the reference's FEM-vs-ground path: the same ``Mesh / FemDomain /
-SolidProp / ExplicitLoop.advanceWithTime`` API (flatworld/__init__.py), with
+Drop-in for the reference's FEM- and spring-vs-ground path: the same ``Mesh /
+FemDomain / SpringMassDomain / SolidProp / ExplicitLoop.advanceWithTime`` API
+(flatworld/__init__.py), with
 the substep running as hand-written HIP kernels behind the C ABI in
 include/flatworld_fem.h.  ``FemBatch`` and ``SimServer`` add the batched and
 served forms: thousands of uneven scenes per launch, hipGraph frames,
@@ -14,7 +15,8 @@ from .domains import FemDomain, GroundDomain, HeightFieldDomain, VoxelGridDomain
 from .femesher import FEMesher, structured_grid
 from .materials import Elastic, HyperElastic, MisesPlastic, SolidProp
 from .mesh import Mesh
-from .scene import GroundSpec, SceneSpec, pack_fem_domain
+from .scene import GroundSpec, SceneSpec, pack_domain, pack_fem_domain, pack_spring_domain
+from .springmass import SpringMassDomain
 
 
 def __getattr__(name):

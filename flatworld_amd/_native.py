@@ -93,6 +93,8 @@ class SceneDesc(ctypes.Structure):
         ("horizon", ctypes.c_int32),
         ("track_disp", ctypes.c_int32),
         ("grounds", ctypes.POINTER(GroundDesc)),
+        ("rest_len", _f32p),
+        ("spring_c", ctypes.c_float),
     ]
 
 
@@ -146,6 +148,7 @@ SIGNATURES = {
     "fw_prepared_footprint": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "fw_scene_add_prepared": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "fw_prepared_free": (ctypes.c_int32, [ctypes.c_void_p]),
+    "fw_scene_mass_scaling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float]),
 }
 
 _lib = None
@@ -251,6 +254,13 @@ def scene_desc(spec, track_disp=True):
     sd.horizon = int(spec.horizon)
     sd.track_disp = 1 if track_disp else 0
     sd.grounds = ctypes.cast(garr, ctypes.POINTER(GroundDesc))
+    if getattr(spec, "rest_len", None) is not None:
+        rl = np.ascontiguousarray(spec.rest_len, dtype=np.float32)
+        if rl.shape != (spec.n_elems,):
+            raise ValueError(f"rest_len must hold one length per spring ({spec.n_elems}), got shape {rl.shape}")
+        keep.append(rl)
+        sd.rest_len = _ptr(rl, _f32p)
+        sd.spring_c = float(np.float32(spec.spring_c))
     return sd, keep
 
 
@@ -372,6 +382,10 @@ class Arena:
 
     def reset_gate(self, sid):
         check(self._L.fw_scene_reset_gate(self.h, int(sid)), "fw_scene_reset_gate")
+
+    def mass_scaling(self, sid, dt_target):
+        """Selective mass scaling of one scene at its current positions (asynchronous)."""
+        check(self._L.fw_scene_mass_scaling(self.h, int(sid), float(np.float32(dt_target))), "fw_scene_mass_scaling")
 
     def step_graph(self, n, flags=0):
         check(self._L.fw_step_graph(self.h, int(n), int(flags)), "fw_step_graph")

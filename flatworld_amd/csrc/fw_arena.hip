@@ -39,6 +39,8 @@ extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* t
 extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
 extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
+extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
+                                             float dt_target, int block, int slots, hipStream_t stream);
 
 namespace {
 
@@ -84,21 +86,25 @@ uint32_t spread16(uint32_t v) {
     return v;
 }
 
+// Corners per element: three for a triangle, two for a spring.
+inline int corners_of(const fw_scene_desc* d) { return d->mat_type == FW_MAT_SPRING ? 2 : 3; }
+
 int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int node_cap, Tiling& T) {
     const int N = d->n_nodes, E = d->n_elems;
+    const int K = corners_of(d);
     // incidence (caller ids): node -> (element, corner), element-major so lists are ascending
     std::vector<int32_t> inc_ptr(N + 1, 0);
-    for (int64_t i = 0; i < 3 * (int64_t)E; ++i) {
+    for (int64_t i = 0; i < K * (int64_t)E; ++i) {
         int32_t n = d->conn[i];
         if (n < 0 || n >= N) return fail(FW_E_INVALID, "connectivity references node " + std::to_string(n) + " outside [0, n_nodes)");
         inc_ptr[n + 1]++;
     }
     for (int i = 0; i < N; ++i) inc_ptr[i + 1] += inc_ptr[i];
-    std::vector<int32_t> inc(3 * (int64_t)E);
+    std::vector<int32_t> inc(K * (int64_t)E);
     {
         std::vector<int32_t> fillp(inc_ptr.begin(), inc_ptr.end() - 1);
         for (int32_t e = 0; e < E; ++e)
-            for (int c = 0; c < 3; ++c) inc[fillp[d->conn[3 * e + c]]++] = 3 * e + c;
+            for (int c = 0; c < K; ++c) inc[fillp[d->conn[K * e + c]]++] = K * e + c;
     }
 
     // node order
@@ -163,13 +169,13 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
                 if (nstamp[n] != tid) ++addn;
             }
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
-                const int e = inc[j] / 3;
+                const int e = inc[j] / K;
                 if (seen[e] == p) continue;
                 seen[e] = p;
                 ++all;
                 if (stamp[e] != tid) ++add;
-                for (int c = 0; c < 3; ++c) {
-                    const int q = d->conn[3 * e + c];
+                for (int c = 0; c < K; ++c) {
+                    const int q = d->conn[K * e + c];
                     if (nseen[q] == p) continue;
                     nseen[q] = p;
                     ++alln;
@@ -191,9 +197,9 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
                                               " elements, more than one tile holds");
             nstamp[n] = tid;
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
-                const int e = inc[j] / 3;
+                const int e = inc[j] / K;
                 stamp[e] = tid;
-                for (int c = 0; c < 3; ++c) nstamp[d->conn[3 * e + c]] = tid;
+                for (int c = 0; c < K; ++c) nstamp[d->conn[K * e + c]] = tid;
             }
             cur_nodes += 1;
             cur_slots += add;
@@ -211,7 +217,8 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
     std::vector<int32_t> owner(E);
     std::vector<int32_t> own_count(ntiles + 1, 0);
     for (int e = 0; e < E; ++e) {
-        int m = std::min(T.node_inv[d->conn[3 * e]], std::min(T.node_inv[d->conn[3 * e + 1]], T.node_inv[d->conn[3 * e + 2]]));
+        int m = T.node_inv[d->conn[K * e]];
+        for (int c = 1; c < K; ++c) m = std::min(m, T.node_inv[d->conn[K * e + c]]);
         owner[e] = node_tile[m];
         own_count[owner[e] + 1]++;
     }
@@ -244,7 +251,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
-                int e = inc[j] / 3;
+                int e = inc[j] / K;
                 if (owner[e] != t && stamp[e] != t) {
                     stamp[e] = t;
                     hl.push_back(T.elem_inv[e]);
@@ -263,8 +270,8 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         std::vector<int32_t> hn;
         for (int s = 0; s < nslots; ++s) {
             const int e = T.elem_perm[slots[s]];
-            for (int c = 0; c < 3; ++c) {
-                const int n = d->conn[3 * e + c];
+            for (int c = 0; c < K; ++c) {
+                const int n = d->conn[K * e + c];
                 const int q = T.node_inv[n];
                 if ((q < p0 || q >= p1) && nst[n] != t) {
                     nst[n] = t;
@@ -281,7 +288,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         // node-major incidence positions: node i of the tile owns LDS entries
         // [ptr[i], ptr[i+1]) in ascending (caller element id, corner); a corner's
         // rank is its position in its node's list
-        std::vector<uint8_t> rank(3 * (size_t)nslots, 0);
+        std::vector<uint8_t> rank(K * (size_t)nslots, 0);
         td.ptr_off = (int32_t)T.csr_ptr.size();
         int cnt = 0;
         T.csr_ptr.push_back(0);
@@ -292,26 +299,28 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
                                                   std::to_string(inc_ptr[n + 1] - inc_ptr[n]) + " element corners; the substep kernel "
                                                   "supports at most " + std::to_string(FW_REC_MAX_RANK + 1));
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
-                int e = inc[j] / 3, c = inc[j] % 3;
-                rank[3 * (size_t)slot_of[e] + c] = (uint8_t)(j - inc_ptr[n]);
+                int e = inc[j] / K, c = inc[j] % K;
+                rank[K * (size_t)slot_of[e] + c] = (uint8_t)(j - inc_ptr[n]);
                 ++cnt;
             }
             if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
             T.csr_ptr.push_back((uint16_t)cnt);
         }
-        // slot records: corner words (local id | rank << FW_REC_ID_BITS); halo slots'
-        // storage element indices in their own list
+        // slot records: corner words (local id | rank << FW_REC_ID_BITS), for a spring the
+        // rest length's bits in place of the third word; halo slots' storage element
+        // indices in their own list
         td.rec_off = (int32_t)(T.rec.size() / 2);
         T.rec.resize(T.rec.size() + 2 * (size_t)nslots);
         uint32_t* rec = T.rec.data() + 2 * (size_t)td.rec_off;
         for (int s = 0; s < nslots; ++s) {
             const int e = T.elem_perm[slots[s]];
-            uint32_t w[3];
-            for (int c = 0; c < 3; ++c) {
-                const uint32_t l = (uint32_t)lid[d->conn[3 * e + c]];
+            uint32_t w[3] = {0u, 0u, 0u};
+            for (int c = 0; c < K; ++c) {
+                const uint32_t l = (uint32_t)lid[d->conn[K * e + c]];
                 if (l > FW_REC_ID_MASK) return fail(FW_E_INVALID, "internal tiling error: local id overflow");
-                w[c] = l | ((uint32_t)rank[3 * (size_t)s + c] << FW_REC_ID_BITS);
+                w[c] = l | ((uint32_t)rank[K * (size_t)s + c] << FW_REC_ID_BITS);
             }
+            if (K == 2) std::memcpy(&w[2], d->rest_len + e, 4);
             rec[2 * s + 0] = w[0] | (w[1] << 16);
             rec[2 * s + 1] = w[2];
         }
@@ -378,7 +387,7 @@ void build_hf_normals(const GroundDev& G, const float* h, float2* nrm) {
 struct Layout {
     int64_t pos0, pos1, rest, vel, mass, flag, bcv, disp, force, cflag;
     int64_t jinv, area, j2a, j2b, stress, strain, yflag;
-    int64_t rec, hel, hpos0, hpos1, hrest, push, cptr, tiles;
+    int64_t rec, hel, hpos0, hpos1, hrest, hvel0, hvel1, push, cptr, tiles;
     int64_t gdata[FW_MAX_GROUNDS];
     int64_t host_bytes;  // [0, host_bytes) is built on the host; the rest is cleared on the device
     int64_t total;
@@ -394,16 +403,20 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
         return r;
     };
     const int64_t N = d->n_nodes, E = d->n_elems;
+    // springs need velocities (and halo velocity copies) where triangles need rest
+    // positions, and no element arrays: the rest length is in the slot record
+    const bool spr = d->mat_type == FW_MAT_SPRING;
     // arrays the host fills, uploaded at admission
     L.pos0 = take(8 * N);
-    L.rest = take(8 * N);
+    L.rest = spr ? -1 : take(8 * N);
     L.vel = take(8 * N);
     L.flag = take(4 * N);
     L.bcv = take(8 * std::max<int64_t>(d->n_bc_values, 1));
     L.rec = take(8 * n_slots);
     L.hel = take(4 * std::max<int64_t>(n_hslots, 1));
     L.hpos0 = take(8 * n_hnode);
-    L.hrest = take(8 * n_hnode);
+    L.hrest = spr ? -1 : take(8 * n_hnode);
+    L.hvel0 = spr ? take(8 * n_hnode) : -1;
     L.push = take(8 * n_hnode);
     L.cptr = take(2 * n_ptr);
     L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
@@ -420,21 +433,22 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
     // writes them), mass / J^-1 / area (prepare kernel), U, J2 state, diagnostics
     L.pos1 = take(8 * N);
     L.hpos1 = take(8 * n_hnode);
+    L.hvel1 = spr ? take(8 * n_hnode) : -1;
     L.mass = take(4 * N);
     L.disp = d->track_disp ? take(8 * N) : -1;
     L.force = take(8 * N);
     L.cflag = take(N);
-    L.jinv = take(16 * E);
-    L.area = take(4 * E);
+    L.jinv = spr ? -1 : take(16 * E);
+    L.area = spr ? -1 : take(4 * E);
     if (d->mat_type == FW_MAT_MISES) {
         L.j2a = take(16 * E);
         L.j2b = take(16 * E);
     } else {
         L.j2a = L.j2b = -1;
     }
-    L.stress = take(16 * E);
-    L.strain = take(16 * E);
-    L.yflag = take(E);
+    L.stress = spr ? -1 : take(16 * E);
+    L.strain = spr ? -1 : take(16 * E);
+    L.yflag = spr ? -1 : take(E);
     L.total = o;
     return L;
 }
@@ -444,8 +458,9 @@ int validate(const fw_scene_desc* d) {
     if (d->n_nodes <= 0 || d->n_elems <= 0) return fail(FW_E_INVALID, "scene needs at least one node and one element");
     if (!d->x0 || !d->v0 || !d->conn || !d->flags || !d->bc_values)
         return fail(FW_E_INVALID, "scene arrays x0, v0, conn, flags and bc_values are required");
-    if (d->mat_type < FW_MAT_ELASTIC || d->mat_type > FW_MAT_MISES)
+    if (d->mat_type < FW_MAT_ELASTIC || d->mat_type > FW_MAT_SPRING)
         return fail(FW_E_INVALID, "unknown material type " + std::to_string(d->mat_type));
+    if (d->mat_type == FW_MAT_SPRING && !d->rest_len) return fail(FW_E_INVALID, "spring scene needs rest_len");
     if (d->n_grounds < 0 || d->n_grounds > FW_MAX_GROUNDS)
         return fail(FW_E_UNSUPPORTED, "a scene holds at most " + std::to_string(FW_MAX_GROUNDS) + " grounds");
     if (d->n_grounds > 0 && !d->grounds) return fail(FW_E_INVALID, "grounds pointer is null");
@@ -512,6 +527,7 @@ struct Staging {
 struct SceneHost {
     bool live = false;
     int64_t block_off = 0, block_bytes = 0;
+    const TileDev* tiles_dev = nullptr;  // the scene's own copy of its tile records (in its block)
     int32_t n_nodes = 0, n_elems = 0, mat_type = 0;
     int32_t s0 = 0, horizon = INT32_MAX;
     Tiling tiling;
@@ -585,8 +601,9 @@ int node_cap_of(const fw_tuning& t) { return node_cap_of_slots(t.tile_slots); }
 // Tiling of a scene description, from the cache when the same arrays are presented again.
 int tiling_for(fw_arena* a, const fw_scene_desc* d, std::unique_ptr<Tiling>* out) {
     int64_t cs = 0, xs = 0;
-    const int64_t E3 = 3 * (int64_t)d->n_elems;
+    const int64_t E3 = corners_of(d) * (int64_t)d->n_elems;
     for (int64_t i = 0; i < E3; i += std::max<int64_t>(1, E3 / 64)) cs = cs * 31 + d->conn[i];
+    cs = cs * 31 + d->mat_type;
     for (int64_t i = 0; i < 2 * (int64_t)d->n_nodes; i += std::max<int64_t>(1, d->n_nodes / 32)) {
         int32_t b;
         std::memcpy(&b, d->x0 + i, 4);
@@ -799,8 +816,14 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
         td.o_pos[1] = o8(L.pos1);
         td.o_hpos[0] = o8(L.hpos0);
         td.o_hpos[1] = o8(L.hpos1);
-        td.o_rest = o8(L.rest);
-        td.o_hrest = o8(L.hrest);
+        if (d->mat_type == FW_MAT_SPRING) {
+            td.o_aux = o8(L.vel);
+            td.o_haux[0] = o8(L.hvel0);
+            td.o_haux[1] = o8(L.hvel1);
+        } else {
+            td.o_aux = o8(L.rest);
+            td.o_haux[0] = td.o_haux[1] = o8(L.hrest);
+        }
         td.o_vel = o8(L.vel);
         td.o_mass = o8(L.mass);
         td.o_flag = o8(L.flag);
@@ -824,11 +847,15 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
         vel[i] = make_float2(d->v0[2 * o], d->v0[2 * o + 1]);
         flag[i] = d->flags[o];
     }
-    std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
+    if (L.rest >= 0) std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
     {
         float2* hp = (float2*)(img + L.hpos0);
         for (size_t j = 0; j < T.hnode.size(); ++j) hp[j] = pos0[T.hnode[j]];
-        std::memcpy(img + L.hrest, img + L.hpos0, 8 * T.hnode.size());
+        if (L.hrest >= 0) std::memcpy(img + L.hrest, img + L.hpos0, 8 * T.hnode.size());
+        if (L.hvel0 >= 0) {
+            float2* hv = (float2*)(img + L.hvel0);
+            for (size_t j = 0; j < T.hnode.size(); ++j) hv[j] = vel[T.hnode[j]];
+        }
     }
     std::memcpy(img + L.bcv, d->bc_values, 8 * (size_t)d->n_bc_values);
     std::memcpy(img + L.rec, T.rec.data(), 4 * T.rec.size());
@@ -915,16 +942,19 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
     D.disp = L.disp >= 0 ? (float2*)(blk + L.disp) : nullptr;
     D.force = (float2*)(blk + L.force);
     D.cflag = (uint8_t*)(blk + L.cflag);
-    D.rest = (const float2*)(blk + L.rest);
+    auto at = [&](int64_t off) -> char* { return off >= 0 ? blk + off : nullptr; };
+    D.rest = (const float2*)at(L.rest);
     D.hpos[0] = (float2*)(blk + L.hpos0);
     D.hpos[1] = (float2*)(blk + L.hpos1);
-    D.jinv = (float4*)(blk + L.jinv);
-    D.area = (float*)(blk + L.area);
-    D.j2[0] = L.j2a >= 0 ? (float4*)(blk + L.j2a) : nullptr;
-    D.j2[1] = L.j2b >= 0 ? (float4*)(blk + L.j2b) : nullptr;
-    D.stress = (float4*)(blk + L.stress);
-    D.strain = (float4*)(blk + L.strain);
-    D.yflag = (uint8_t*)(blk + L.yflag);
+    D.hvel[0] = (float2*)at(L.hvel0);
+    D.hvel[1] = (float2*)at(L.hvel1);
+    D.jinv = (float4*)at(L.jinv);
+    D.area = (float*)at(L.area);
+    D.j2[0] = (float4*)at(L.j2a);
+    D.j2[1] = (float4*)at(L.j2b);
+    D.stress = (float4*)at(L.stress);
+    D.strain = (float4*)at(L.strain);
+    D.yflag = (uint8_t*)at(L.yflag);
     for (int g = 0; g < D.hot.n_grounds; ++g)
         if (L.gdata[g] >= 0) D.ground[g].data = blk + L.gdata[g];
     // warm-up runs at the next substep launched (local step -1)
@@ -952,6 +982,7 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
     sh->live = true;
     sh->block_off = off;
     sh->block_bytes = L.total;
+    sh->tiles_dev = (const TileDev*)(blk + L.tiles);
     sh->n_nodes = p->n_nodes;
     sh->n_elems = p->n_elems;
     sh->mat_type = p->mat_type;
@@ -1074,7 +1105,8 @@ int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
         out->n_elems += sp->n_elems;
         if (sp->mat_type == FW_MAT_MISES) out->n_elems_j2 += sp->n_elems;
         out->halo_slots += sp->tiling.halo_slots;
-        algo += (double)sp->n_elems * (sp->mat_type == FW_MAT_MISES ? 64.0 : 32.0) + (double)sp->n_nodes * 40.0;
+        const double be = sp->mat_type == FW_MAT_MISES ? 64.0 : (sp->mat_type == FW_MAT_SPRING ? 12.0 : 32.0);
+        algo += (double)sp->n_elems * be + (double)sp->n_nodes * 40.0;
     }
     out->algo_bytes_per_substep = algo;
     out->n_cu = a->n_cu;
@@ -1219,6 +1251,7 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
     rc = build_tiling(d, tile_nodes, tile_slots, node_cap, T);
     if (rc) return rc;
     const int N = d->n_nodes, E = d->n_elems;
+    const int K = corners_of(d);
     std::vector<int> owned(E, 0);
     int64_t max_slots = 0, entries = 0;
     for (const TileDev& td : T.tiles) {
@@ -1249,17 +1282,19 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
                 prev_halo = eint;
             }
             const int ecaller = T.elem_perm[eint];
-            if (r[1] >> 16) return fail(FW_E_INVALID, "slot record has stray bits");
+            if (K == 3 && (r[1] >> 16)) return fail(FW_E_INVALID, "slot record has stray bits");
+            if (K == 2 && std::memcmp(&r[1], d->rest_len + ecaller, 4) != 0)
+                return fail(FW_E_INVALID, "spring slot record holds the wrong rest length");
             const uint32_t words[3] = {r[0] & 0xFFFFu, r[0] >> 16, r[1] & 0xFFFFu};
-            for (int c = 0; c < 3; ++c) {
+            for (int c = 0; c < K; ++c) {
                 const uint32_t lidc = words[c] & FW_REC_ID_MASK;
-                if (lidc >= local.size() || T.node_perm[local[lidc]] != d->conn[3 * ecaller + c])
+                if (lidc >= local.size() || T.node_perm[local[lidc]] != d->conn[K * ecaller + c])
                     return fail(FW_E_INVALID, "slot record names the wrong node");
                 if ((int)lidc >= td.n_nodes) continue;  // halo node: its owner tile assembles it
                 const uint32_t pos = (uint32_t)cp[lidc] + (words[c] >> FW_REC_ID_BITS);
                 if (pos >= (uint32_t)cp[lidc + 1]) return fail(FW_E_INVALID, "corner rank beyond its node's list");
                 if (pos >= at.size() || at[pos] != -1) return fail(FW_E_INVALID, "slot destinations collide");
-                at[pos] = 3 * (int64_t)ecaller + c;
+                at[pos] = K * (int64_t)ecaller + c;
             }
         }
         for (int i = 0; i < td.n_nodes; ++i) {
@@ -1267,8 +1302,8 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
             const int n = T.node_perm[p];
             std::vector<int64_t> want, got;
             for (int e = 0; e < E; ++e)  // brute force: fine for the small meshes this is used on
-                for (int c = 0; c < 3; ++c)
-                    if (d->conn[3 * e + c] == n) want.push_back(3 * (int64_t)e + c);
+                for (int c = 0; c < K; ++c)
+                    if (d->conn[K * e + c] == n) want.push_back(K * (int64_t)e + c);
             for (int j = cp[i]; j < cp[i + 1]; ++j) got.push_back(at[j]);
             entries += (int64_t)got.size();
             if (want != got) return fail(FW_E_INVALID, "incidence order of node " + std::to_string(n) + " is wrong");
@@ -1460,12 +1495,30 @@ int32_t fw_scene_write(fw_arena* a, int32_t id, int32_t field, const void* in, i
     HIPCHK(hipStreamSynchronize(a->stream));
     void* dst = field == FW_FIELD_X ? (void*)s->dev.pos[buf] : (void*)s->dev.vel;
     HIPCHK(hipMemcpy(dst, tmp.data(), need, hipMemcpyHostToDevice));
-    if (field == FW_FIELD_X && !s->tiling.hnode.empty()) {
-        // the halo copies the next substep reads hold the same positions
+    // the halo copies the next substep reads hold the same positions (velocities of
+    // spring scenes: their force reads both ends' velocities)
+    float2* hdst = field == FW_FIELD_X ? s->dev.hpos[buf] : s->dev.hvel[buf];
+    if (hdst && !s->tiling.hnode.empty()) {
         std::vector<float2> hp(s->tiling.hnode.size());
         for (size_t j = 0; j < hp.size(); ++j) hp[j] = tmp[s->tiling.hnode[j]];
-        HIPCHK(hipMemcpy(s->dev.hpos[buf], hp.data(), 8 * hp.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(hdst, hp.data(), 8 * hp.size(), hipMemcpyHostToDevice));
     }
+    return FW_OK;
+}
+
+int32_t fw_scene_mass_scaling(fw_arena* a, int32_t id, float dt_target) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    if (!(dt_target > 0.0f)) return fail(FW_E_INVALID, "dt_target must be > 0");
+    HIPCHK(hipSetDevice(a->device));
+    int64_t done;
+    int buf;
+    bool warmed;
+    scene_state(a, s, &done, &buf, &warmed);
+    // enqueued behind every substep launched so far, so it sees the current positions
+    HIPCHK(fw_launch_mass_scaling(a->d_scenes, s->tiles_dev, (int)s->tiles.size(), buf, dt_target, a->tune.block,
+                                  a->tune.tile_slots, a->stream));
     return FW_OK;
 }
 

@@ -13,7 +13,9 @@
 //            map), corner forces area * B^T S written straight to the LDS
 //            entry of the owning node's incidence list.  The owner tile writes
 //            the J2 state to the other buffer (ping-pong), so halo tiles never
-//            see a half-updated state.
+//            see a half-updated state.  Spring tiles stage velocities where
+//            triangle tiles stage rest positions and evaluate one spring per
+//            slot (stiffness and damping along the spring).
 //   phase B  owned nodes: ground penalty forces (plane, height field, voxel)
 //            behind the AABB activation gate, then the element forces from a
 //            contiguous LDS run in ascending caller element id -- the order of
@@ -24,7 +26,7 @@
 // double-buffered because neighbouring tiles read them; velocities, mass and
 // flags are touched by the owner tile only.
 //
-// Arithmetic follows flatworld/femspringmanager.py:59-138, 217-280, 315-418,
+// Arithmetic follows flatworld/femspringmanager.py:59-138, 141-201, 217-418,
 // flatworld/materials/materialfunctions.py:10-101 and
 // flatworld/mixedcontact.py:166-302, 660-684, 1566-1643 operation by
 // operation in float32; the build uses -ffp-contract=off so no FMA changes a
@@ -116,6 +118,38 @@ __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, f
     }
 }
 
+// Corner words of a spring's slot record (.x: both ends, .y: the rest length's bits).
+__device__ __forceinline__ void unpack_spring(uint2 r, int n_own_nodes, const uint16_t* lds_cp, uint32_t l[2],
+                                              uint32_t q[2], float& rest) {
+    const uint32_t w[2] = {r.x & 0xFFFFu, r.x >> 16};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        l[c] = w[c] & FW_REC_ID_MASK;
+        const uint32_t base = lds_cp[min((int)l[c], n_own_nodes)];
+        q[c] = (int)l[c] < n_own_nodes ? base + (w[c] >> FW_REC_ID_BITS) : 0xFFFFu;
+    }
+    rest = __uint_as_float(r.y);
+}
+
+// One spring slot (flatworld/femspringmanager.py:283-312): xab = x_a - x_b,
+// lnew = |xab| (Warp's dot: a zero-initialised sum), dir = xab / lnew,
+// f = k (lnew / l0 - 1) dir + c (v_ab . dir) dir.  Node a receives -f and node b
+// +f; the node sums subtract their entries, so a's entry holds f and b's -f.
+__device__ __forceinline__ void spring_slot(float k, float c, float2 pa, float2 pb, float2 va, float2 vb, float l0,
+                                            const uint32_t q[2], float2* lds_f) {
+    const float xab0 = pa.x - pb.x, xab1 = pa.y - pb.y;
+    const float lnew = sqrtf((0.0f + xab0 * xab0) + xab1 * xab1);
+    const float d0 = xab0 / lnew, d1 = xab1 / lnew;
+    const float a = k * (lnew / l0 - 1.0f);
+    float f0 = a * d0, f1 = a * d1;
+    const float vab0 = va.x - vb.x, vab1 = va.y - vb.y;
+    const float b = c * ((0.0f + vab0 * d0) + vab1 * d1);
+    f0 = f0 + b * d0;
+    f1 = f1 + b * d1;
+    if (q[0] != 0xFFFFu) lds_f[q[0]] = make_float2(f0, f1);
+    if (q[1] != 0xFFFFu) lds_f[q[1]] = make_float2(-f0, -f1);
+}
+
 // ------------------------------------------------------------------ substep kernel
 
 struct StepArgs {
@@ -139,9 +173,10 @@ struct NodeIn {
 };
 
 // What a tile's stage loads into registers before its data goes to LDS: the
-// element slot records, the positions / rest positions of every LDS node (owned
-// nodes first, then the tile's halo copies), the owned nodes' CSR offsets, and
-// for J2 tiles the state of slot 0 and the element ids of halo slots.
+// element slot records, the positions and rest positions (springs: velocities)
+// of every LDS node (owned nodes first, then the tile's halo copies), the owned
+// nodes' CSR offsets, and for J2 tiles the state of slot 0 and the element ids
+// of halo slots.
 template <int SPT, int LPT, int CPT>
 struct Stage {
     uint2 rc[SPT];
@@ -178,15 +213,15 @@ __device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, L
     const int nn = T.n_nodes;
     const int nl = nn + T.n_hnodes;  // >= 1: a tile owns at least one node
     const float2* __restrict__ pos_rd = fw_at<const float2>(T.base, (rd ? T.o_pos[1] : T.o_pos[0])) + T.node0;
-    const float2* __restrict__ rest = fw_at<const float2>(T.base, T.o_rest) + T.node0;
+    const float2* __restrict__ aux = fw_at<const float2>(T.base, T.o_aux) + T.node0;
     const float2* __restrict__ hpos_rd = fw_at<const float2>(T.base, (rd ? T.o_hpos[1] : T.o_hpos[0])) + T.hnode_off - nn;
-    const float2* __restrict__ hrest = fw_at<const float2>(T.base, T.o_hrest) + T.hnode_off - nn;
+    const float2* __restrict__ haux_rd = fw_at<const float2>(T.base, (rd ? T.o_haux[1] : T.o_haux[0])) + T.hnode_off - nn;
 #pragma unroll
     for (int k = 0; k < LPT; ++k) {
         const int j = min(tid + k * BLOCK, nl - 1);
         const bool own = j < nn;
         st.x[k] = fw_ld((own ? pos_rd : hpos_rd) + j);
-        st.X[k] = fw_ld((own ? rest : hrest) + j);
+        st.X[k] = fw_ld((own ? aux : haux_rd) + j);
     }
     const uint16_t* __restrict__ cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
 #pragma unroll
@@ -208,7 +243,9 @@ __device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, L
 // BLOCK threads; at most SLOTS element slots, NODES LDS nodes (owned + halo) and
 // NPT * BLOCK owned nodes per tile.  LDS: 16 B per node + 24 B per slot, nothing
 // else (the AABB reduction reuses the rest-position array once phase A is done),
-// so 1024 slots take exactly 40 KiB and four tiles fit one CU's 160 KiB.
+// so 1024 slots take exactly 40 KiB and four tiles fit one CU's 160 KiB.  A
+// spring slot has two corners, so the last third of the force entries is free in
+// spring tiles: it holds the owned nodes' new velocities for the halo push.
 //
 // A workgroup loops over tiles (grid-stride, XCD-aware).  Each tile starts with
 // one batch of independent loads (records, positions, rest positions) whose
@@ -224,7 +261,9 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     __shared__ uint16_t lds_cp[NPT * BLOCK + 1];
     static_assert((BLOCK / 64) * 4 * sizeof(int32_t) <= NODES * sizeof(float2), "reduction scratch");
     static_assert(NODES <= (int)FW_REC_ID_MASK + 1, "local node ids fit the record");
+    static_assert(NPT * BLOCK <= SLOTS, "spring tiles keep new velocities in the free force entries");
     int32_t* lds_red = reinterpret_cast<int32_t*>(lds_X);
+    float2* const lds_vn = lds_f + 2 * SLOTS;
 
     const int G = fw_ldc1(&A.ctl->clock) + A.k;
     const int ntiles = A.tile_count >= 0 ? A.tile_count : (fw_ldc1(&A.ctl->n_tiles) - A.tile_base);
@@ -275,12 +314,13 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         const float damping = warm ? 0.0f : (A.damping_override >= 0.0f ? A.damping_override : fw_ldc1(&sh->damping));
         const int n_grounds = fw_ldc1(&sh->n_grounds);
         const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && n_grounds > 0;
-        Mat m;
+        Mat m;  // spring tiles: lam = stiffness, mu = damping coefficient
+        m.type = fw_ldc1(&sh->mat_type);
+        const bool spring = m.type == FW_MAT_SPRING_DEV;
         {
             const float E = fw_ldc1(&sh->E), nu = fw_ldc1(&sh->nu);
-            m.type = fw_ldc1(&sh->mat_type);
-            m.lam = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
-            m.mu = E / (2.0f * (1.0f + nu));
+            m.lam = spring ? E : E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+            m.mu = spring ? nu : E / (2.0f * (1.0f + nu));
             m.sy = fw_ldc1(&sh->sigma_y);
             m.H = fw_ldc1(&sh->H);
         }
@@ -316,11 +356,18 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 }
             }
             if (sl < nslots) {
-                uint32_t l[3], q[3];
-                unpack_rec(st.rc[k], n_nodes, lds_cp, l, q);
-                slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st_cur, q,
-                           sl < T.n_own, T.elem0 + sl, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])),
-                           st_stress, st_strain, st_yflag, lds_f);
+                if (spring) {
+                    uint32_t l[2], q[2];
+                    float l0;
+                    unpack_spring(st.rc[k], n_nodes, lds_cp, l, q, l0);
+                    spring_slot(m.lam, m.mu, lds_x[l[0]], lds_x[l[1]], lds_X[l[0]], lds_X[l[1]], l0, q, lds_f);
+                } else {
+                    uint32_t l[3], q[3];
+                    unpack_rec(st.rc[k], n_nodes, lds_cp, l, q);
+                    slot_force(m, lds_x[l[0]], lds_x[l[1]], lds_x[l[2]], lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], st_cur, q,
+                               sl < T.n_own, T.elem0 + sl, diag, fw_at<float4>(T.base, (wr ? T.o_j2[1] : T.o_j2[0])),
+                               st_stress, st_strain, st_yflag, lds_f);
+                }
             }
             st_cur = st_next;
         }
@@ -423,6 +470,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 vy = 0.0f;
             }
             fw_st(vel + n, make_float2(vx, vy));
+            if (spring) lds_vn[i] = make_float2(vx, vy);  // source of the halo velocity copies
             const float dux = vx * dt, duy = vy * dt;
             if (disp) fw_st(disp + n, make_float2(nd[k].u.x + dux, nd[k].u.y + duy));
             const float nx_ = p.x + dux, ny_ = p.y + duy;
@@ -466,10 +514,15 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // ---------------- halo copies of this tile's nodes for the neighbouring tiles
         {
             float2* __restrict__ hpos_wr = fw_at<float2>(T.base, (wr ? T.o_hpos[1] : T.o_hpos[0]));
-            if (tid < T.n_push) fw_st(hpos_wr + pe.y, lds_x[pe.x]);
+            float2* __restrict__ hvel_wr = fw_at<float2>(T.base, (wr ? T.o_haux[1] : T.o_haux[0]));
+            if (tid < T.n_push) {
+                fw_st(hpos_wr + pe.y, lds_x[pe.x]);
+                if (spring) fw_st(hvel_wr + pe.y, lds_vn[pe.x]);
+            }
             for (int q = tid + BLOCK; q < T.n_push; q += BLOCK) {
                 const int2 e = fw_ld(fw_at<const int2>(T.base, T.o_push) + T.push_off + q);
                 fw_st(hpos_wr + e.y, lds_x[e.x]);
+                if (spring) fw_st(hvel_wr + e.y, lds_vn[e.x]);
             }
         }
         lds_barrier();  // LDS reuse by the next tile of this block
@@ -479,7 +532,8 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 // ------------------------------------------------------------------ prepare kernel
 
 // flatworld/femspringmanager.py:113-138: area = |det Jac| / 2, J^-1 = inverse(Jac),
-// lumped mass = sum over incident elements (ascending id) of area * rho / 3.
+// lumped mass = sum over incident elements (ascending id) of area * rho / 3.  A
+// spring scene's nodes all take the domain mass (femspringmanager.py:852-853).
 // One workgroup per tile; MAXN = most owned nodes a tile of this shape holds.
 template <int BLOCK, int SLOTS, int NODES, int MAXN>
 __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, const TileDev* tiles, int tile_base,
@@ -491,9 +545,14 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
     if (ti >= tile_count) return;
     const TileDev T = tiles[tile_base + ti];
     const SceneDev& S = scenes[T.scene];
+    if (S.hot.mat_type == FW_MAT_SPRING_DEV) {
+        float* mass = fw_at<float>(T.base, T.o_mass);
+        for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) fw_st(mass + T.node0 + i, S.rho);
+        return;
+    }
     const uint2* rec = fw_at<const uint2>(T.base, T.o_rec) + T.rec_off;
-    const float2* rest = fw_at<const float2>(T.base, T.o_rest);
-    const float2* hrest = fw_at<const float2>(T.base, T.o_hrest);
+    const float2* rest = fw_at<const float2>(T.base, T.o_aux);
+    const float2* hrest = fw_at<const float2>(T.base, T.o_haux[0]);
     const uint16_t* cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
     for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) lds_X[i] = fw_ld(rest + T.node0 + i);
     for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) lds_X[T.n_nodes + h] = fw_ld(hrest + T.hnode_off + h);
@@ -521,6 +580,92 @@ __global__ __launch_bounds__(BLOCK) void prepare_kernel(const SceneDev* scenes, 
         float m = 0.0f;
         for (int j = lds_cp[i]; j < lds_cp[i + 1]; ++j) m += lds_m[j];
         fw_st(mass + T.node0 + i, m);
+    }
+}
+
+// Selective mass scaling, flatworld/femspringmanager.py:141-201.  The reference runs
+// it on one thread: for each triangle in ascending id, L = the shortest edge at the
+// current positions, dt_e = 0.4 L / sqrt(E / rho); when dt_e < dt_target and
+// L > 1e-15 each corner node gains dm = (E (dt / (0.4 L))^2 - rho) * area / 3; then
+// for each spring, both end nodes are raised to at least 4 k dt^2.  Per node that is
+// its incidence list in ascending element id, the order of the CSR entries, so one
+// workgroup per tile writes the dm of each (element, corner) to LDS and each owned
+// node adds its run.  An element that gains nothing contributes +0, which leaves a
+// (non-negative) mass unchanged.  `buf` selects the position buffer holding the
+// scene's current positions.
+template <int BLOCK, int SLOTS, int NODES, int MAXN>
+__global__ __launch_bounds__(BLOCK) void mass_scale_kernel(const SceneDev* scenes, const TileDev* tiles, int tile_count,
+                                                           int buf, float dt_target) {
+    __shared__ float2 lds_x[NODES];
+    __shared__ float2 lds_X[NODES];
+    __shared__ float lds_m[SLOTS * 3];
+    __shared__ uint16_t lds_cp[MAXN + 1];
+    const int ti = blockIdx.x;
+    if (ti >= tile_count) return;
+    const TileDev T = tiles[ti];
+    const SceneDev& S = scenes[T.scene];
+    const float E = S.hot.E;
+    float* mass = fw_at<float>(T.base, T.o_mass) + T.node0;
+    const uint16_t* cptr = fw_at<const uint16_t>(T.base, T.o_cptr) + T.ptr_off;
+    if (S.hot.mat_type == FW_MAT_SPRING_DEV) {
+        const float m_min = 4.0f * E * dt_target * dt_target;
+        for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
+            if (fw_ld(cptr + i + 1) == fw_ld(cptr + i)) continue;  // no spring ends at this node
+            const float m = fw_ld(mass + i);
+            if (m < m_min) fw_st(mass + i, m_min);
+        }
+        return;
+    }
+    const float2* pos = fw_at<const float2>(T.base, buf ? T.o_pos[1] : T.o_pos[0]) + T.node0;
+    const float2* hpos = fw_at<const float2>(T.base, buf ? T.o_hpos[1] : T.o_hpos[0]) + T.hnode_off;
+    const float2* rest = fw_at<const float2>(T.base, T.o_aux) + T.node0;
+    const float2* hrest = fw_at<const float2>(T.base, T.o_haux[0]) + T.hnode_off;
+    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
+        lds_x[i] = fw_ld(pos + i);
+        lds_X[i] = fw_ld(rest + i);
+    }
+    for (int h = threadIdx.x; h < T.n_hnodes; h += BLOCK) {
+        lds_x[T.n_nodes + h] = fw_ld(hpos + h);
+        lds_X[T.n_nodes + h] = fw_ld(hrest + h);
+    }
+    for (int i = threadIdx.x; i <= T.n_nodes; i += BLOCK) lds_cp[i] = fw_ld(cptr + i);
+    __syncthreads();
+    const uint2* rec = fw_at<const uint2>(T.base, T.o_rec) + T.rec_off;
+    const float rho = S.rho;
+    const float safety = 0.4f;
+    const float c_e = sqrtf(E / rho);
+    const int nslots = T.n_own + T.n_halo;
+    for (int sl = threadIdx.x; sl < nslots; sl += BLOCK) {
+        uint32_t l[3], q[3];
+        unpack_rec(fw_ld(rec + sl), T.n_nodes, lds_cp, l, q);
+        const float2 p1 = lds_x[l[0]], p2 = lds_x[l[1]], p3 = lds_x[l[2]];
+        const float a0 = p2.x - p1.x, a1 = p2.y - p1.y;
+        const float b0 = p3.x - p2.x, b1 = p3.y - p2.y;
+        const float c0 = p1.x - p3.x, c1 = p1.y - p3.y;
+        const float e01 = sqrtf((0.0f + a0 * a0) + a1 * a1);
+        const float e12 = sqrtf((0.0f + b0 * b0) + b1 * b1);
+        const float e20 = sqrtf((0.0f + c0 * c0) + c1 * c1);
+        const float m12 = e12 < e20 ? e12 : e20;  // Warp's min
+        const float L = e01 < m12 ? e01 : m12;
+        const float dt_elem = safety * L / c_e;
+        float dm = 0.0f;
+        if (dt_elem < dt_target && L > 1e-15f) {
+            float4 ji;
+            float area;
+            rest_frame(lds_X[l[0]], lds_X[l[1]], lds_X[l[2]], ji, area);  // the prepare kernel's area, bit for bit
+            const float r = dt_target / (safety * L);
+            const float rho_new = E * r * r;
+            dm = (rho_new - rho) * area / 3.0f;
+        }
+        if (q[0] != 0xFFFFu) lds_m[q[0]] = dm;
+        if (q[1] != 0xFFFFu) lds_m[q[1]] = dm;
+        if (q[2] != 0xFFFFu) lds_m[q[2]] = dm;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.n_nodes; i += BLOCK) {
+        float m = fw_ld(mass + i);
+        for (int j = lds_cp[i]; j < lds_cp[i + 1]; ++j) m = m + lds_m[j];
+        fw_st(mass + i, m);
     }
 }
 
@@ -578,6 +723,20 @@ extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* t
     }
     FW_SHAPES(FW_PREP)
 #undef FW_PREP
+    return hipErrorInvalidValue;
+}
+
+extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
+                                             float dt_target, int block, int slots, hipStream_t stream) {
+    if (tile_count <= 0) return hipSuccess;
+#define FW_MS(B, S, N, P)                                                                                  \
+    if (block == B && slots == S) {                                                                       \
+        hipLaunchKernelGGL((fw::mass_scale_kernel<256, S, N, B * P>), dim3(tile_count), dim3(256), 0, stream, \
+                           scenes, tiles, tile_count, buf, dt_target);                                    \
+        return hipGetLastError();                                                                         \
+    }
+    FW_SHAPES(FW_MS)
+#undef FW_MS
     return hipErrorInvalidValue;
 }
 

@@ -6,11 +6,14 @@
 //            disp float2 (optional), force float2 (diag), cflag uint8 (diag)
 //   halo     hpos[2] float2 per (tile, halo node): copies of neighbour-owned positions,
 //            written by the owner tile in the same substep as pos (same ping-pong);
-//            hrest float2 per (tile, halo node): their rest positions
+//            hrest float2 per (tile, halo node): their rest positions (triangles);
+//            hvel[2] float2 per (tile, halo node): velocity copies (springs, ping-pong)
 //   elements jinv float4 + area float (written once, for readback only: the substep
 //            recomputes both from the rest positions), j2[2] float4 (eps_xx, eps_yy,
 //            eps_xy, eqps; ping-pong, J2 scenes only), stress / strain float4 (diag),
 //            yflag uint8 (diag)
+//            spring scenes hold no element arrays: a spring's rest length rides in its
+//            slot record
 //   tiles    rec uint2 per element slot, hel int32 per halo slot, csr_ptr uint16 per
 //            owned node (+1), push int2 per halo copy, grouped by the tile owning the
 //            source node
@@ -20,6 +23,7 @@
 #include <stdint.h>
 
 #define FW_MAX_GROUNDS_DEV 4
+#define FW_MAT_SPRING_DEV 4  // FW_MAT_SPRING of the public header
 
 struct GroundDev {
     int32_t kind;
@@ -34,7 +38,8 @@ struct GroundDev {
 };
 
 // The scene fields every tile reads (material, dt, gate state), kept together at
-// the front of the scene entry.
+// the front of the scene entry.  Spring scenes keep the stiffness in E and the
+// spring damping coefficient in nu.
 struct SceneHot {
     int32_t mat_type, n_grounds;
     float E, nu, sigma_y, H;
@@ -57,6 +62,7 @@ struct SceneDev {
     float2* force;  // diag
     uint8_t* cflag; // diag
     float2* hpos[2];       // halo copies (host bookkeeping; the substep uses the tile's pointers)
+    float2* hvel[2];       // halo velocity copies, spring scenes (host bookkeeping)
     // element arrays
     float4* jinv;
     float* area;
@@ -66,7 +72,7 @@ struct SceneDev {
     uint8_t* yflag;  // diag
     // scalars
     int32_t n_nodes, n_elems;
-    float rho;
+    float rho;        // density; spring scenes: the mass of every node
     int32_t s0;       // global step at which the scene runs its warm-up (local step -1 is s0 - 1)
     int32_t horizon;  // substeps after the warm-up; INT32_MAX when unbounded
     int32_t live;     // 0 once removed (host bookkeeping: the tile table holds live scenes only)
@@ -79,7 +85,8 @@ struct SceneDev {
 // for material, ground and diagnostics.
 struct TileDev {
     char* base;              // the scene's block; array = base + (offset << 8) (blocks are 256-byte aligned)
-    uint32_t o_rec;          // uint2 per element slot: .x = w0 | w1 << 16, .y = w2 (bits 16-31 zero);
+    uint32_t o_rec;          // uint2 per element slot: .x = w0 | w1 << 16, .y = w2 (bits 16-31 zero)
+    //   for a triangle, .y = the rest length's float bits for a spring;
     //   corner word w = l | r << FW_REC_ID_BITS: l the local id of the corner's node (owned
     //   nodes first, then the tile's halo nodes), r the rank of this (element, corner)
     //   in the node's incidence list.  An owned node's corner force goes to LDS entry
@@ -88,8 +95,10 @@ struct TileDev {
     uint32_t o_bcv;          // float2 BC value table of the scene (indexed by the node flag's row)
     uint32_t o_pos[2];       // float2 node positions (ping-pong)
     uint32_t o_hpos[2];      // float2 halo copies (ping-pong)
-    uint32_t o_rest;         // float2 rest positions
-    uint32_t o_hrest;        // float2 halo rest positions
+    uint32_t o_aux;          // float2 second per-node stage array: rest positions (triangles) or
+                             // velocities (springs, whose force needs both ends' velocities)
+    uint32_t o_haux[2];      // its halo copies: rest positions (both entries) or velocity copies
+                             // (ping-pong like o_hpos)
     uint32_t o_vel;          // float2
     uint32_t o_mass;         // float
     uint32_t o_flag;         // uint32

@@ -1,10 +1,12 @@
-"""ExplicitLoop drop-in for FEM bodies on static grounds (flatworld/explicitloop.py:71-589).
+"""ExplicitLoop drop-in for FEM bodies and spring networks on static grounds
+(flatworld/explicitloop.py:71-589).
 
-Setup follows flatworld/explicitloop.py:72-398 for the FEM-vs-ground subset:
+Setup follows flatworld/explicitloop.py:72-398 for the body-vs-ground subset:
 
 * loop stable time = min(1.0, 1e-3 when a plane exists (RigidManager default,
-  flatworld/rigidmanager.py:1780), every FEM domain's stable time, every
-  contact pair's stable time);
+  flatworld/rigidmanager.py:1780), every FEM / spring domain's stable time),
+  raised to ``mass_scaling_dt`` when selective mass scaling is asked for
+  (explicitloop.py:164-170), then lowered to every contact pair's stable time;
 * contact pairs are formed over domains with ``considerContact`` in the
   reference's O(D^2) order and split into the batched plane / height-field /
   voxel work lists (flatworld/mixedcontact.py:1756-2269).  As in the reference,
@@ -26,6 +28,7 @@ from .femcontact import contactHelper
 from .femspringmanager import FemSpringManager
 
 _GROUND_TYPES = (DomainType.ANALYTICAL, DomainType.HEIGHTFIELD, DomainType.VOXELMAP)
+_BODY_TYPES = (DomainType.FEM, DomainType.SPRINGMASS)
 
 
 class ExplicitLoop:
@@ -47,13 +50,9 @@ class ExplicitLoop:
     ):
         if len(joints) > 0 or ties:
             raise NotImplementedError("joints and ties act on rigid bodies, which are not part of this package")
-        if mass_scaling_dt > 0.0:
-            raise NotImplementedError(
-                "selective mass scaling (flatworld/femspringmanager.py:141-201) is not part of this package"
-            )
         for d in domains:
-            if d.type not in (DomainType.FEM,) + _GROUND_TYPES:
-                raise NotImplementedError(f"domain type {d.type:#b} (rigid / spring-mass) is not part of this package")
+            if d.type not in _BODY_TYPES + _GROUND_TYPES:
+                raise NotImplementedError(f"domain type {d.type:#b} (rigid) is not part of this package")
         self.domains = domains
         self.d = domains[0].d
         self.numDomains = len(domains)
@@ -82,7 +81,7 @@ class ExplicitLoop:
                     continue
                 if hasattr(dom_i, "env_id") and hasattr(dom_j, "env_id") and dom_i.env_id != dom_j.env_id:
                     continue
-                if dom_i.type != DomainType.FEM and dom_j.type != DomainType.FEM:
+                if dom_i.type not in _BODY_TYPES and dom_j.type not in _BODY_TYPES:
                     continue  # ground-ground: no contact object in the reference either
                 c = contactHelper(dom_i, dom_j, ctype)
                 if c is not None:
@@ -111,6 +110,9 @@ class ExplicitLoop:
                                                  arena_kwargs=arena_kwargs)
         self.stableTime = min(self.femSpringManager.stableTime, self.stableTime)
         self.femSpringManager.setGlobalAABB(None)
+        if mass_scaling_dt > 0.0:
+            self.femSpringManager.applyMassScaling(mass_scaling_dt)
+            self.stableTime = max(self.stableTime, mass_scaling_dt)
         self.numContacts = len(self.contacts)
         for c in self.contacts:
             self.stableTime = min(self.stableTime, c.stableTime)

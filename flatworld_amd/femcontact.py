@@ -1,15 +1,17 @@
-"""FEM-vs-ground contact pair records: penalty stiffness and contact stable time.
+"""Body-vs-ground contact pair records: penalty stiffness and contact stable time.
 
-Host scalars only, following flatworld/femcontact.py:94-197 and 283-303:
+Host scalars only, following flatworld/femcontact.py:94-197 and 256-303:
 
-* bulk modulus ``K = E / (3 (1 - 2 min(nu, 0.4999)))``;
-* 2D segment penalty ``k = 2 K`` (independent of mesh size);
-* contact stable time ``0.9 sqrt(m_node / k)`` with
-  ``m_node = rho * (L_min^2 / 2) / 3``.
+* FEM body: bulk modulus ``K = E / (3 (1 - 2 min(nu, 0.4999)))``, 2D segment
+  penalty ``k = 2 K`` (independent of mesh size), contact stable time
+  ``0.9 sqrt(m_node / k)`` with ``m_node = rho * (L_min^2 / 2) / 3``;
+* spring network: ``k = 100 * spring``, stable time ``0.5 sqrt(mass / k)``.
 
 The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip).
-FEM-FEM, rigid and spring contacts are outside this package: constructing one
-raises ``NotImplementedError`` instead of silently dropping it.
+FEM-FEM, FEM-spring and rigid contacts are outside this package: constructing
+one raises ``NotImplementedError`` instead of silently dropping it.  Pairs the
+reference's factory has no class for (spring-spring, spring-voxel) give no
+contact, as there.
 """
 
 from .definitions import ContactType, DomainType
@@ -84,8 +86,45 @@ class ContactFlexVoxelMap(_FlexGround):
     ground_type = DomainType.VOXELMAP
 
 
+class _SpringGround(ContactBase):
+    """Spring network against a static ground; ``domain1`` is always the springs."""
+
+    ground_type = None
+    _SPRING_PENALTY_SCALE = 100.0
+
+    def __init__(self, domain1, domain2, type, is_tied=False):
+        super().__init__(domain1, domain2, type, is_tied)
+        if self.domain1.type == self.ground_type:
+            self.domain1, self.domain2 = self.domain2, self.domain1
+        self.penalty = self.domain1.spring * self._SPRING_PENALTY_SCALE
+        self.stableTime = 0.5 * (self.domain1.mass / self.penalty) ** 0.5
+
+
+class ContactSpringAnalytical(_SpringGround):
+    """Springs vs plane (flatworld/femcontact.py:256-265)."""
+
+    ground_type = DomainType.ANALYTICAL
+
+
+class ContactSpringHeightField(_SpringGround):
+    """Springs vs height field (flatworld/femcontact.py:269-279)."""
+
+    ground_type = DomainType.HEIGHTFIELD
+
+
 def contactHelper(domain1, domain2, type):
-    """Pair factory (flatworld/femcontact.py:283-303) restricted to FEM-vs-ground."""
+    """Pair factory (flatworld/femcontact.py:283-303) restricted to body-vs-ground."""
+    if type == ContactType.ANALYTICALSPRING:
+        return ContactSpringAnalytical(domain1, domain2, type)
+    if type == ContactType.SPRINGHEIGHTFIELD:
+        return ContactSpringHeightField(domain1, domain2, type)
+    if type in (ContactType.SPRINGSPRING, ContactType.SPRINGVOXELMAP):
+        return None  # no contact class in the reference either
+    if type == ContactType.FLEXSPRING:
+        raise NotImplementedError(
+            "FEM-spring penalty contact (flatworld/femcontact.py:13-80, 200-240) is not part of this package; "
+            "give the domains considerContact=False"
+        )
     if type == ContactType.FLEXANLAYTICAL:
         return ContactFlexAnalytical(domain1, domain2, type)
     if type == ContactType.FEXHEIGHTFIELD:

@@ -1,19 +1,19 @@
 """FemSpringManager drop-in over the HIP arena (flatworld/femspringmanager.py:495-1033).
 
-The reference packs every FEM domain of a loop into global Warp arrays and
-launches force -> integrate -> AABB -> clear kernels per substep.  Here every
-FEM domain is one scene of a ``flatworld_amd._native.Arena`` and a substep is
-one fused kernel launch over all of them.  Readbacks keep the reference's
-attribute names and global layout: ``coords.numpy()`` returns the
-``MAX_NODES x 2`` array with each domain at its node offset, exactly as the
-Warp array would hold it.
+The reference packs every FEM and spring-mass domain of a loop into global Warp
+arrays and launches force -> springs -> integrate -> AABB -> clear kernels per
+substep.  Here every such domain is one scene of a ``flatworld_amd._native.Arena``
+and a substep is one fused kernel launch over all of them.  Readbacks keep the
+reference's attribute names and global layout: ``coords.numpy()`` returns the
+``MAX_NODES x 2`` array with each domain at its node offset, exactly as the Warp
+array would hold it.
 """
 
 import numpy as np
 
 from . import _native
 from .definitions import DomainType
-from .scene import pack_fem_domain
+from .scene import pack_domain
 
 
 class DeviceField:
@@ -46,10 +46,10 @@ class ScalarField:
 
 
 class FemSpringManager:
-    """FEM domains of one loop, resident in one HIP arena.
+    """FEM and spring-mass domains of one loop, resident in one HIP arena.
 
-    ``grounds_of`` maps a FEM domain index (position in ``domains``) to its
-    ordered list of ``(ground_domain, contact)`` pairs; ExplicitLoop fills it.
+    ``grounds_of`` maps a domain index (position in ``domains``) to its ordered
+    list of ``(ground_domain, contact)`` pairs; ExplicitLoop fills it.
     """
 
     def __init__(self, d, domains, spatial_hash=None, grounds_of=None, device=0, track_disp=True, arena_kwargs=None):
@@ -57,14 +57,8 @@ class FemSpringManager:
             raise ValueError(f"FemSpringManager supports d=2 only (got {d})")
         self.d = d
         self.aabb = None
-        fem = []
-        for idx, dom in enumerate(domains):
-            if dom.type == DomainType.FEM:
-                fem.append((idx, dom))
-            elif dom.type == DomainType.SPRINGMASS:
-                raise NotImplementedError(
-                    "spring-mass domains (flatworld/femspringmanager.py:283-312, springmass.py) are not part of this package"
-                )
+        # FEM and spring domains in the order given (flatworld/femspringmanager.py:647-658)
+        fem = [(idx, dom) for idx, dom in enumerate(domains) if dom.type in (DomainType.FEM, DomainType.SPRINGMASS)]
         count_nodes = sum(dom.nnodes for _, dom in fem)
         count_elements = sum(dom.nelements for _, dom in fem)
         self.MAX_NODES = max(count_nodes + 1024, 2048)
@@ -79,27 +73,36 @@ class FemSpringManager:
         self._node_offsets = []
         self._elem_offsets = []
         grounds_of = grounds_of or {}
+        conn_rows = []
         for local_idx, (idx, dom) in enumerate(fem):
-            spec = pack_fem_domain(dom, grounds_of.get(idx, []))
+            spec = pack_domain(dom, grounds_of.get(idx, []))
             sid = self._arena.add(spec, track_disp=track_disp)
             self._scene_ids.append(sid)
             self._node_offsets.append(node_off)
             self._elem_offsets.append(elem_off)
+            # global connectivity rows: node ids offset by the domain, -1 padded to 4
+            # (flatworld/femspringmanager.py:744-749, 855-858)
+            rows = np.full((dom.nelements, 4), -1, dtype=np.int32)
+            rows[:, : spec.conn.shape[1]] = spec.conn + node_off
+            conn_rows.append(rows)
             node_off += dom.nnodes
             elem_off += dom.nelements
             self.domains.append(dom)
-            self.domainMeshes.append(dom.mesh)
+            self.domainMeshes.append(getattr(dom, "mesh", None))
             dom.attach(self, local_idx)
             self.stableTime = min(dom.stableTime, self.stableTime)
         self._n_nodes = node_off
         self._n_elems = elem_off
+        self._spring_rest = {k: np.asarray(dom.restLength, np.float32)
+                             for k, (_, dom) in enumerate(fem) if dom.type == DomainType.SPRINGMASS}
+        n_spring = len(self._spring_rest)
         self.femDomainIds = ScalarField(0)
-        self.numFemDomains = ScalarField(len(fem))
-        self.numSpringDomains = ScalarField(0)
+        self.numFemDomains = ScalarField(len(fem) - n_spring)
+        self.numSpringDomains = ScalarField(n_spring)
         self.totalNodes = ScalarField(node_off)
         self.totalElements = ScalarField(elem_off)
-        self.totalFEMElements = ScalarField(elem_off)
-        self.totalFEMNodes = ScalarField(node_off)
+        self.totalFEMElements = ScalarField(sum(dom.nelements for _, dom in fem if dom.type == DomainType.FEM))
+        self.totalFEMNodes = ScalarField(sum(dom.nnodes for _, dom in fem if dom.type == DomainType.FEM))
         self.domainNodeOffset = _OffsetField(self._node_offsets, self.MAX_DOMAINS)
         self.domainElemOffset = _OffsetField(self._elem_offsets, self.MAX_DOMAINS)
         self._global_ids = [idx for idx, _ in fem]
@@ -119,6 +122,11 @@ class FemSpringManager:
         self.yieldFlag = DeviceField(self, _native.FIELD_YIELD, E, 1, np.uint8, per_node=False)
         self.contactFlag = DeviceField(self, _native.FIELD_CONTACT, N, 1, np.uint8)
         self.A = _ZeroField((N, 2))
+        conn = np.zeros((E, 4), dtype=np.int32)
+        if conn_rows:
+            allrows = np.concatenate(conn_rows)
+            conn[: allrows.shape[0]] = allrows
+        self.connectivity = _HostField(conn)
 
     # ------------------------------------------------------------ stepping
     def setGlobalAABB(self, aabb=None):
@@ -139,6 +147,20 @@ class FemSpringManager:
         flags = _native.FW_STEP_DIAG if diag else 0
         self._arena.step(int(n), float(np.float32(dt)), flags, damping_override=float(np.float32(damping)))
 
+    def applyMassScaling(self, dt_target):
+        """Selective mass scaling of every domain at the current positions
+        (flatworld/femspringmanager.py:141-201, 899-924); prints the added mass."""
+        before = self.mass.numpy().astype(np.float64)
+        for sid in self._scene_ids:
+            self._arena.mass_scaling(sid, dt_target)
+        after = self.mass.numpy().astype(np.float64)
+        added = float(np.sum(after - before))
+        total = float(np.sum(after))
+        pct = added / total * 100.0 if total > 0 else 0.0
+        print(f"[MassScaling] dt_target={dt_target:.3e}  added_mass={added:.4e}  "
+              f"total_mass={total:.4e}  ({pct:.2f}%)")
+        return added
+
     def reset_gate(self):
         for sid in self._scene_ids:
             self._arena.reset_gate(sid)
@@ -150,8 +172,11 @@ class FemSpringManager:
     def _gather(self, field, capacity, width, dtype, per_node):
         out = np.zeros((capacity, width) if width > 1 else (capacity,), dtype=dtype)
         offs = self._node_offsets if per_node else self._elem_offsets
-        for sid, off in zip(self._scene_ids, offs):
-            a = self._arena.read(sid, field)
+        for k, (sid, off) in enumerate(zip(self._scene_ids, offs)):
+            if field == _native.FIELD_AREA and k in self._spring_rest:
+                a = self._spring_rest[k]  # a spring's elemVolume is its rest length (femspringmanager.py:860)
+            else:
+                a = self._arena.read(sid, field)
             out[off : off + a.shape[0]] = a
         return out
 
@@ -170,6 +195,17 @@ class _OffsetField:
     def __init__(self, offsets, cap):
         self._a = np.zeros(cap, dtype=np.int32)
         self._a[: len(offsets)] = offsets
+
+    def numpy(self):
+        return self._a.copy()
+
+
+class _HostField:
+    """Host-held array with the ``.numpy()`` accessor (constant after construction)."""
+
+    def __init__(self, a):
+        self._a = a
+        self.shape = a.shape
 
     def numpy(self):
         return self._a.copy()

@@ -1,11 +1,11 @@
 """Packed scene description: the flat arrays handed across the C-ABI.
 
-A *scene* is one FEM body (one ``FemDomain``) with the ordered list of grounds it
-touches.  The packing mirrors what the reference manager and contact builder
-upload to the device:
+A *scene* is one FEM body (one ``FemDomain``) or one spring network (one
+``SpringMassDomain``) with the ordered list of grounds it touches.  The packing
+mirrors what the reference manager and contact builder upload to the device:
 
 * node/element arrays and per-element material
-  (flatworld/femspringmanager.py:706-813),
+  (flatworld/femspringmanager.py:706-813; springs 814-870),
 * BC bits ORed per node with the last-written value vector, then initial
   velocities (flatworld/femspringmanager.py:660-704),
 * one ground record per (FEM, ground) contact pair with the float32 penalty,
@@ -35,6 +35,11 @@ GROUND_PLANE = 1
 GROUND_HEIGHTFIELD = 2
 GROUND_VOXEL = 3
 
+# material code of a spring-mass scene (FW_MAT_SPRING of include/flatworld_fem.h);
+# the reference has no material type for springs: their elements carry matE and
+# matDamping only (flatworld/femspringmanager.py:859-862)
+MAT_SPRING = 4
+
 
 @dataclass
 class GroundSpec:
@@ -58,11 +63,16 @@ class GroundSpec:
 
 @dataclass
 class SceneSpec:
-    """Flat arrays of one FEM body and its ground contacts."""
+    """Flat arrays of one FEM body (or spring network) and its ground contacts.
+
+    Spring scenes (``mat_type == MAT_SPRING``): ``conn`` is int32[E, 2], ``E`` the
+    stiffness, ``rho`` the mass of every node, ``spring_c`` the damping coefficient
+    and ``rest_len`` the float32[E] rest lengths; every node is a boundary node.
+    """
 
     x0: np.ndarray  # float32[N, 2]
     v0: np.ndarray  # float32[N, 2]
-    conn: np.ndarray  # int32[E, 3]
+    conn: np.ndarray  # int32[E, 3] (triangles) or int32[E, 2] (springs)
     bc: np.ndarray  # int32[N] BC bits
     bcv: np.ndarray  # float32[N, 2] last-written BC value
     bnodes: np.ndarray  # int32[Nb] sorted boundary nodes
@@ -76,6 +86,8 @@ class SceneSpec:
     dt: float = 0.0  # host float64; rounded to float32 on upload
     damping: float = 0.0
     horizon: int = 0  # substeps to run when served; 0 = unbounded
+    rest_len: Optional[np.ndarray] = None  # springs: float32[E]
+    spring_c: np.float32 = np.float32(0.0)  # springs: damping coefficient
 
     @property
     def n_nodes(self):
@@ -205,6 +217,45 @@ def pack_fem_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
     )
 
 
+def pack_spring_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
+    """SceneSpec for one SpringMassDomain (flatworld/femspringmanager.py:814-870).
+
+    Every node takes the domain mass, every spring the domain stiffness and damping
+    coefficient; every node is a ground-contact work item and counts for the AABB
+    (flatworld/mixedcontact.py:1815-1829, 2159-2167; femspringmanager.py:397-408).
+    """
+    x0 = np.ascontiguousarray(np.asarray(domain.coords, dtype=np.float32)[:, :2])
+    conn = np.ascontiguousarray(np.asarray(domain.connectivity, dtype=np.int32)[:, :2])
+    bc, bcv, v0 = pack_bcs(domain)
+    return SceneSpec(
+        x0=x0,
+        v0=v0,
+        conn=conn,
+        bc=bc,
+        bcv=bcv,
+        bnodes=np.arange(domain.nnodes, dtype=np.int32),
+        mat_type=MAT_SPRING,
+        E=np.float32(float(domain.spring)),
+        nu=np.float32(0.0),
+        rho=np.float32(float(domain.mass)),
+        sigma_y=np.float32(0.0),
+        H=np.float32(0.0),
+        grounds=[ground_spec(g, c) for g, c in grounds],
+        dt=float(dt),
+        damping=float(damping),
+        horizon=int(horizon),
+        rest_len=np.ascontiguousarray(domain.restLength, dtype=np.float32),
+        spring_c=np.float32(float(domain.damping)),
+    )
+
+
+def pack_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
+    """SceneSpec of a FEM or spring-mass domain."""
+    if domain.type == DomainType.SPRINGMASS:
+        return pack_spring_domain(domain, grounds, dt, damping, horizon)
+    return pack_fem_domain(domain, grounds, dt, damping, horizon)
+
+
 def scene_stable_time(domain, contacts, has_plane):
     """Loop stable time for one FEM body (flatworld/explicitloop.py:145-170, 312-318)."""
     st = 1.0
@@ -216,4 +267,5 @@ def scene_stable_time(domain, contacts, has_plane):
     return st
 
 
-MATERIAL_NAMES = {MaterialType.ELASTIC: "elastic", MaterialType.NEOHOOKEAN: "neohookean", MaterialType.MISES: "j2"}
+MATERIAL_NAMES = {MaterialType.ELASTIC: "elastic", MaterialType.NEOHOOKEAN: "neohookean", MaterialType.MISES: "j2",
+                  MAT_SPRING: "spring"}

@@ -11,6 +11,8 @@ Configurations (sizes as in the benchmark definition):
   j2voxel   512x512 block, J2(1e5, .3, 1000, sy=500, H=1e3), v0=(0,-2), 256x256 value-noise voxels
   batched   4096 scenes, side log-uniform in [8,128], Elastic/NH/J2 mix, one height field each
   serving   Poisson stream, side in [16,256], plane / height field / voxel grounds, horizon 60..600
+  springs   4096 spring lattices (structural, shear and bending springs), side log-uniform in
+            [8,128], one height field each
 """
 
 import math
@@ -24,7 +26,8 @@ from .femcontact import contactHelper
 from .femesher import structured_grid
 from .materials import Elastic, HyperElastic, MisesPlastic, SolidProp
 from .mesh import Mesh
-from .scene import pack_fem_domain, scene_stable_time
+from .scene import MAT_SPRING, pack_domain, scene_stable_time
+from .springmass import SpringMassDomain
 
 
 def rng_for(seed):
@@ -39,7 +42,7 @@ def one_body_spec(domain, ground=None, dt=None, damping=0.0, horizon=0):
         pairs.append((ground, c))
     if dt is None:
         dt = scene_stable_time(domain, [c for _, c in pairs], has_plane=isinstance(ground, GroundDomain))
-    return pack_fem_domain(domain, pairs, dt=dt, damping=damping, horizon=horizon)
+    return pack_domain(domain, pairs, dt=dt, damping=damping, horizon=horizon)
 
 
 def make_material(kind, E, nu, rho, sigma_y=None, H=None):
@@ -159,7 +162,44 @@ def serving_job(rng, side_lo=16, side_hi=256, horizon_lo=60, horizon_hi=600):
     return one_body_spec(dom, ground, horizon=horizon)
 
 
+def spring_lattice(side, lower, spacing, prop, bcs, initials=(), jitter=0.0, seed=0, bending=True):
+    """SpringMassDomain on a (side+1)^2 node lattice: structural springs along both axes,
+    shear springs along both diagonals and (optionally) bending springs skipping one node.
+    Node (i, j) has id i * (side + 1) + j, at lower + spacing * (i, j) (+ jitter)."""
+    n = side + 1
+    rng = rng_for(seed)
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    xy = np.stack([i.reshape(-1), j.reshape(-1)], 1).astype(np.float64) * spacing + np.asarray(lower, np.float64)
+    if jitter > 0.0:
+        xy += rng.uniform(-jitter * spacing, jitter * spacing, xy.shape)
+    ids = np.arange(n * n).reshape(n, n)
+    pairs = [np.stack([ids[:-1, :], ids[1:, :]], -1).reshape(-1, 2), np.stack([ids[:, :-1], ids[:, 1:]], -1).reshape(-1, 2),
+             np.stack([ids[:-1, :-1], ids[1:, 1:]], -1).reshape(-1, 2), np.stack([ids[1:, :-1], ids[:-1, 1:]], -1).reshape(-1, 2)]
+    if bending:
+        pairs += [np.stack([ids[:-2, :], ids[2:, :]], -1).reshape(-1, 2), np.stack([ids[:, :-2], ids[:, 2:]], -1).reshape(-1, 2)]
+    conn = np.concatenate(pairs).astype(np.int32)
+    return SpringMassDomain(2, xy.astype(np.float32), conn, list(prop), list(bcs), initials=list(initials))
+
+
+def spring_batched_config(n_scenes=4096, seed=0, side_lo=8, side_hi=128, spacing=0.01):
+    """Uneven spring lattices: side log-uniform in [side_lo, side_hi] at a fixed spacing (a
+    spring's force per unit length is k / l0, so the spacing, not the side, sets the stable
+    step), random stiffness, light damping, one height field each."""
+    rng = rng_for(seed)
+    specs = []
+    for _ in range(n_scenes):
+        s = int(round(side_lo * (side_hi / side_lo) ** rng.uniform()))
+        k = float(rng.uniform(50.0, 500.0))
+        m = 0.01
+        c = 0.02 * math.sqrt(k * m)
+        width = s * spacing
+        dom = spring_lattice(s, (0.5 - 0.5 * width, MIX_Y0), spacing, [k, c, m], [Gravity([0.0, -9.8])])
+        specs.append(one_body_spec(dom, sinusoid_heightfield(rng)))
+    return specs
+
+
 def algorithmic_bytes(spec):
-    """E*B_e + N*B_n with B_e = 32 (64 for J2) and B_n = 40 bytes."""
-    be = 64 if spec.mat_type == MaterialType.MISES else 32
+    """E*B_e + N*B_n with B_e = 32 (64 for J2, 12 for a spring: two node ids and the rest
+    length) and B_n = 40 bytes."""
+    be = 64 if spec.mat_type == MaterialType.MISES else (12 if spec.mat_type == MAT_SPRING else 32)
     return spec.n_elems * be + spec.n_nodes * 40

@@ -9,8 +9,8 @@
  * call only; all device memory belongs to the arena.
  *
  * Model: an *arena* is a packed region of HBM on one GPU holding any number of
- * *scenes*.  A scene is one FEM body (linear triangles, one material) with up
- * to FW_MAX_GROUNDS ground contacts.  fw_step advances every resident scene by
+ * *scenes*.  A scene is one FEM body (linear triangles, one material) or one
+ * spring-mass body (two-node springs) with up to FW_MAX_GROUNDS ground contacts.  fw_step advances every resident scene by
  * n substeps, one fused kernel launch per substep; each scene carries its own
  * dt, damping, step counter and horizon.
  *
@@ -26,6 +26,10 @@
  *   fw_scene_read     the .numpy() reads of coords/V/U/Fext/elemStress/...
  *                     (flatworld/femspringmanager.py:566-614; femdomain.py:47-50)
  *   fw_scene_remove   no counterpart: the reference never removes a domain.
+ *   fw_scene_mass_scaling  FemSpringManager.applyMassScaling
+ *                     (flatworld/femspringmanager.py:141-201, 899-924)
+ * Spring-mass scenes replace addSpringDomain and _spring_force_kernel
+ * (flatworld/femspringmanager.py:814-870, 283-312; flatworld/springmass.py).
  */
 #ifndef FLATWORLD_FEM_H
 #define FLATWORLD_FEM_H
@@ -36,7 +40,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 1
+#define FW_ABI_VERSION 2
 #define FW_MAX_GROUNDS 4
 
 /* status codes */
@@ -51,6 +55,7 @@ extern "C" {
 #define FW_MAT_ELASTIC 1
 #define FW_MAT_NEOHOOKEAN 2
 #define FW_MAT_MISES 3
+#define FW_MAT_SPRING 4 /* spring-mass domain: two-node springs (flatworld/springmass.py) */
 
 /* ground kinds */
 #define FW_GROUND_PLANE 1
@@ -84,8 +89,10 @@ extern "C" {
 #define FW_FIELD_CONTACT 13 /* uint8[N] a ground pushed the node in the last DIAG substep */
 #define FW_FIELD_AABB 14    /* float32[4] boundary-node AABB lb.x lb.y ub.x ub.y at the current x */
 
-/* One FEM-vs-ground contact pair (one work-list entry group of
- * flatworld/mixedcontact.py:1794-1856, 2131-2261). */
+/* One body-vs-ground contact pair (one work-list entry group of
+ * flatworld/mixedcontact.py:1794-1856, 2131-2261).  A spring scene's penalty is
+ * 100 x its stiffness (flatworld/femcontact.py:256-279) and every one of its nodes
+ * is a work item: flag them all FW_NODE_BOUNDARY. */
 typedef struct {
     int32_t kind;       /* FW_GROUND_* */
     float penalty;      /* float32 penalty k = 2K (flatworld/femcontact.py:100-119) */
@@ -106,7 +113,11 @@ typedef struct {
     float vox_dz;
 } fw_ground_desc;
 
-/* One FEM body; arrays in the caller's node/element order. */
+/* One FEM or spring-mass body; arrays in the caller's node/element order.
+ * FW_MAT_SPRING: conn holds [2E] node ids, E is the stiffness (matE), rho the mass
+ * of every node, spring_c the spring damping coefficient (matDamping) and rest_len
+ * the [E] rest lengths (flatworld/femspringmanager.py:846-862); nu, sigma_y and H
+ * are unused. */
 typedef struct {
     int32_t n_nodes;
     int32_t n_elems;
@@ -114,7 +125,7 @@ typedef struct {
     int32_t n_bc_values;
     const float* x0;        /* [2N] rest = initial positions */
     const float* v0;        /* [2N] initial velocities */
-    const int32_t* conn;    /* [3E] triangle node ids */
+    const int32_t* conn;    /* [3E] triangle node ids; [2E] for springs */
     const uint32_t* flags;  /* [N] node flag words (FW_NODE_*) */
     const float* bc_values; /* [2 * n_bc_values] BC value table */
     int32_t mat_type;       /* FW_MAT_* */
@@ -124,6 +135,8 @@ typedef struct {
     int32_t horizon;        /* substeps to run after the warm-up; 0 = unbounded */
     int32_t track_disp;     /* 1: accumulate U every substep (the reference always does) */
     const fw_ground_desc* grounds;
+    const float* rest_len;  /* FW_MAT_SPRING: [E] rest lengths; NULL otherwise */
+    float spring_c;         /* FW_MAT_SPRING: damping coefficient */
 } fw_scene_desc;
 
 typedef struct {
@@ -185,6 +198,13 @@ int32_t fw_scene_remove(fw_arena* a, int32_t scene_id);
 /* Restart the scene's BVH-rebuild schedule: the next substep counts as step 0
  * of the activation gate (ExplicitLoop.reset, flatworld/explicitloop.py:635-679). */
 int32_t fw_scene_reset_gate(fw_arena* a, int32_t scene_id);
+/* Selective mass scaling (FemSpringManager.applyMassScaling): enqueued behind the
+ * substeps already launched, so it sees the current positions.  Every triangle
+ * whose critical step 0.4 L / sqrt(E / rho) (L its shortest edge) is below
+ * dt_target adds (E (dt_target / (0.4 L))^2 - rho) * area / 3 to each of its
+ * nodes; every spring end node is raised to at least 4 k dt_target^2.
+ * ExplicitLoop calls it right after the warm-up (flatworld/explicitloop.py:164-170). */
+int32_t fw_scene_mass_scaling(fw_arena* a, int32_t scene_id, float dt_target);
 /* Substeps completed by the scene (warm-up excluded) and whether it reached its horizon. */
 int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, int32_t* finished);
 

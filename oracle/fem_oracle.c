@@ -240,6 +240,74 @@ void ora_element_force(const ora_scene* s, int32_t e, float f[6], float cauchy[3
     }
 }
 
+/* ------------------------------------------------------------------ springs */
+
+/* Warp's dot of two vec2: a zero-initialised sum over the components. */
+static float dot2(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
+
+/* flatworld/femspringmanager.py:283-312: xab = x_a - x_b, lnew = |xab|,
+ * dir = xab / lnew, f = k (lnew / l0 - 1) dir + c (v_ab . dir) dir. */
+void ora_spring_force(const ora_scene* s, int32_t e, float f[2]) {
+    const int32_t ia = s->conn[2 * e], ib = s->conn[2 * e + 1];
+    const float xab0 = s->x[2 * ia] - s->x[2 * ib];
+    const float xab1 = s->x[2 * ia + 1] - s->x[2 * ib + 1];
+    const float lnew = sqrtf(dot2(xab0, xab1, xab0, xab1));
+    const float d0 = xab0 / lnew, d1 = xab1 / lnew;
+    const float a = s->E * (lnew / s->rest_len[e] - 1.0f);
+    float f0 = a * d0, f1 = a * d1;
+    const float vab0 = s->v[2 * ia] - s->v[2 * ib];
+    const float vab1 = s->v[2 * ia + 1] - s->v[2 * ib + 1];
+    const float b = s->spring_c * dot2(vab0, vab1, d0, d1);
+    f0 = f0 + b * d0;
+    f1 = f1 + b * d1;
+    f[0] = f0;
+    f[1] = f1;
+}
+
+/* flatworld/femspringmanager.py:141-201, one thread: FEM elements first, then springs. */
+float ora_mass_scaling(ora_scene* s, float dt_target) {
+    const float safety = 0.4f;
+    float added = 0.0f;
+    const float* x = s->x;
+    if (s->mat_type != ORA_SPRING) {
+        for (int32_t e = 0; e < s->n_elems; ++e) {
+            const int32_t n1 = s->conn[3 * e], n2 = s->conn[3 * e + 1], n3 = s->conn[3 * e + 2];
+            const float a0 = x[2 * n2] - x[2 * n1], a1 = x[2 * n2 + 1] - x[2 * n1 + 1];
+            const float b0 = x[2 * n3] - x[2 * n2], b1 = x[2 * n3 + 1] - x[2 * n2 + 1];
+            const float c0 = x[2 * n1] - x[2 * n3], c1 = x[2 * n1 + 1] - x[2 * n3 + 1];
+            const float e01 = sqrtf(dot2(a0, a1, a0, a1));
+            const float e12 = sqrtf(dot2(b0, b1, b0, b1));
+            const float e20 = sqrtf(dot2(c0, c1, c0, c1));
+            const float m12 = e12 < e20 ? e12 : e20; /* wp.min */
+            const float L = e01 < m12 ? e01 : m12;
+            const float c_e = sqrtf(s->E / s->rho);
+            const float dt_elem = safety * L / c_e;
+            if (dt_elem < dt_target && L > 1e-15f) {
+                const float r = dt_target / (safety * L);
+                const float rho_new = s->E * r * r;
+                const float dm = (rho_new - s->rho) * s->area[e] / 3.0f;
+                s->mass[n1] = s->mass[n1] + dm;
+                s->mass[n2] = s->mass[n2] + dm;
+                s->mass[n3] = s->mass[n3] + dm;
+                added = added + (rho_new - s->rho) * s->area[e];
+            }
+        }
+    } else {
+        for (int32_t e = 0; e < s->n_elems; ++e) {
+            const float m_min = 4.0f * s->E * dt_target * dt_target;
+            for (int c = 0; c < 2; ++c) {
+                const int32_t nid = s->conn[2 * e + c];
+                if (s->mass[nid] < m_min) {
+                    const float dm = m_min - s->mass[nid];
+                    s->mass[nid] = m_min;
+                    added = added + dm;
+                }
+            }
+        }
+    }
+    return added;
+}
+
 /* ------------------------------------------------------------------ grounds */
 
 /* flatworld/mixedcontact.py:166-217 (_nearest_on_curve_2d). */
@@ -436,6 +504,12 @@ void ora_update_bbox(ora_scene* s) {
 
 /* flatworld/femspringmanager.py:113-138 */
 void ora_prepare(ora_scene* s) {
+    if (s->mat_type == ORA_SPRING) {
+        /* np.full(nnodes, float(domain.mass), float32); springs have no area or J^-1 */
+        for (int32_t i = 0; i < s->n_nodes; ++i) s->mass[i] = s->rho;
+        ora_update_bbox(s);
+        return;
+    }
     for (int32_t i = 0; i < s->n_nodes; ++i) s->mass[i] = 0.0f;
     for (int32_t e = 0; e < s->n_elems; ++e) {
         const int32_t n1 = s->conn[3 * e], n2 = s->conn[3 * e + 1], n3 = s->conn[3 * e + 2];
@@ -477,8 +551,21 @@ void ora_substep(ora_scene* s, float dt, float damping, int32_t counter, int32_t
         }
     }
 
+    /* spring forces, ascending element id (flatworld/femspringmanager.py:969-983, 283-312) */
+    if (s->mat_type == ORA_SPRING) {
+        for (int32_t e = 0; e < s->n_elems; ++e) {
+            float f[2];
+            ora_spring_force(s, e, f);
+            const int32_t ia = s->conn[2 * e], ib = s->conn[2 * e + 1];
+            fx[2 * ia] += -f[0];
+            fx[2 * ia + 1] += -f[1];
+            fx[2 * ib] += f[0];
+            fx[2 * ib + 1] += f[1];
+        }
+    }
+
     /* element forces, ascending element id (flatworld/femspringmanager.py:947-967) */
-    for (int32_t e = 0; e < s->n_elems; ++e) {
+    for (int32_t e = 0; e < s->n_elems && s->mat_type != ORA_SPRING; ++e) {
         float f[6], cauchy[3], strain[3], epn[3], eqn = 0.0f;
         int32_t yielded = 0;
         ora_element_force(s, e, f, cauchy, strain, epn, &eqn, &yielded);

@@ -22,7 +22,10 @@
 extern "C" {
 #endif
 
-enum { ORA_ELASTIC = 1, ORA_NEOHOOKEAN = 2, ORA_MISES = 3 };
+/* ORA_SPRING: a spring-mass domain (flatworld/springmass.py); its elements are
+ * two-node springs, conn holds [2E] node ids, E is the stiffness, rho the mass of
+ * every node and spring_c the damping coefficient (femspringmanager.py:814-870). */
+enum { ORA_ELASTIC = 1, ORA_NEOHOOKEAN = 2, ORA_MISES = 3, ORA_SPRING = 4 };
 enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3 };
 
 typedef struct {
@@ -58,7 +61,7 @@ typedef struct {
     float* u;        /* [2N] accumulated displacement */
     float* fext;     /* [2N] nodal force of the last substep (after BCs) */
     float* mass;     /* [N] lumped mass, filled by ora_prepare */
-    const int32_t* conn;   /* [3E] */
+    const int32_t* conn;   /* [3E] triangles, [2E] springs */
     const int32_t* bc;     /* [N] BC bits */
     const float* bcv;      /* [2N] BC value vector */
     const int32_t* bnodes; /* [Nb] boundary nodes */
@@ -74,10 +77,18 @@ typedef struct {
     uint8_t* contact_flag; /* [N] a ground pushed this node in the last substep */
     float aabb[4];   /* boundary-node AABB lb.x, lb.y, ub.x, ub.y at the current x */
     const ora_ground* grounds;
+    const float* rest_len; /* [E] spring rest lengths (ORA_SPRING) */
+    float spring_c;        /* spring damping coefficient (ORA_SPRING) */
 } ora_scene;
 
-/* flatworld/femspringmanager.py:113-138 (area, J^-1, lumped mass by atomics). */
+/* flatworld/femspringmanager.py:113-138 (area, J^-1, lumped mass by atomics);
+ * a spring domain's nodes all take the domain mass (femspringmanager.py:852-853). */
 void ora_prepare(ora_scene* s);
+
+/* FemSpringManager.applyMassScaling (flatworld/femspringmanager.py:141-201, 899-924),
+ * run after the warm-up as ExplicitLoop does (explicitloop.py:164-170).  Returns the
+ * added mass as the reference accumulates it. */
+float ora_mass_scaling(ora_scene* s, float dt_target);
 
 /* One ExplicitLoop.advance() for a one-FEM-body loop
  * (flatworld/explicitloop.py:473-555): gate, ground forces, element forces,
@@ -91,6 +102,9 @@ int32_t ora_mises2d(float E, float nu, float sy, float H, const float F[4], cons
                     float eqps_old, float stress[3], float eps_p_new[3], float* eqps_new, float strain[3]);
 void ora_element_force(const ora_scene* s, int32_t e, float f[6], float cauchy[3], float strain[3],
                        float eps_p_new[3], float* eqps_new, int32_t* yielded);
+/* flatworld/femspringmanager.py:283-312: the force node 0 of spring e receives
+ * with a minus sign and node 1 with a plus sign. */
+void ora_spring_force(const ora_scene* s, int32_t e, float f[2]);
 int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[2], float dt, float force[2]);
 int32_t ora_gate(const float fem_lb[2], const float fem_ub[2], const ora_ground* g, int32_t counter);
 void ora_update_bbox(ora_scene* s);

@@ -81,6 +81,8 @@ class OraScene(ctypes.Structure):
         ("contact_flag", _u8p),
         ("aabb", ctypes.c_float * 4),
         ("grounds", ctypes.POINTER(OraGround)),
+        ("rest_len", _f32p),
+        ("spring_c", ctypes.c_float),
     ]
 
 
@@ -108,6 +110,9 @@ def lib():
         L.ora_gate.argtypes = [_f32p, _f32p, ctypes.POINTER(OraGround), ctypes.c_int32]
         L.ora_gate.restype = ctypes.c_int32
         L.ora_update_bbox.argtypes = [ctypes.POINTER(OraScene)]
+        L.ora_spring_force.argtypes = [ctypes.POINTER(OraScene), ctypes.c_int32, _f32p]
+        L.ora_mass_scaling.argtypes = [ctypes.POINTER(OraScene), ctypes.c_float]
+        L.ora_mass_scaling.restype = ctypes.c_float
         L.ora_log_f32.argtypes = [ctypes.c_float]
         L.ora_log_f32.restype = ctypes.c_float
         _lib = L
@@ -148,7 +153,7 @@ def make_ground(gs, keep):
 
 
 class OracleScene:
-    """Serial CPU run of one packed scene (one FEM body + grounds)."""
+    """Serial CPU run of one packed scene (one FEM or spring-mass body + grounds)."""
 
     def __init__(self, spec):
         L = lib()
@@ -160,6 +165,8 @@ class OracleScene:
         self.fext = np.zeros((N, 2), np.float32)
         self.mass = np.zeros(N, np.float32)
         self.conn = np.ascontiguousarray(spec.conn, dtype=np.int32)
+        rl = getattr(spec, "rest_len", None)
+        self.rest_len = np.ascontiguousarray(rl if rl is not None else np.zeros(E), dtype=np.float32)
         self.bc = np.ascontiguousarray(spec.bc, dtype=np.int32)
         self.bcv = np.ascontiguousarray(spec.bcv, dtype=np.float32)
         self.bnodes = np.ascontiguousarray(spec.bnodes, dtype=np.int32)
@@ -186,6 +193,8 @@ class OracleScene:
         s.stress, s.strain = _p(self.stress, _f32p), _p(self.strain, _f32p)
         s.yield_flag, s.contact_flag = _p(self.yield_flag, _u8p), _p(self.contact_flag, _u8p)
         s.grounds = ctypes.cast(self._grounds, ctypes.POINTER(OraGround))
+        s.rest_len = _p(self.rest_len, _f32p)
+        s.spring_c = float(getattr(spec, "spring_c", 0.0))
         self._s = s
         self._lib = L
         self.counter = 0
@@ -198,6 +207,18 @@ class OracleScene:
     def warmup(self):
         """Zero-dt femStep run by setGlobalAABB (flatworld/femspringmanager.py:872-879)."""
         self._lib.ora_substep(ctypes.byref(self._s), 0.0, 0.0, 0, 0)
+
+    def mass_scaling(self, dt_target):
+        """FemSpringManager.applyMassScaling (flatworld/femspringmanager.py:141-201, 899-924), which
+        ExplicitLoop runs right after the warm-up (flatworld/explicitloop.py:164-170); returns the
+        added mass as the reference accumulates it."""
+        return float(self._lib.ora_mass_scaling(ctypes.byref(self._s), float(np.float32(dt_target))))
+
+    def spring_force(self, e):
+        """Force of spring e on its first node, with a minus sign (flatworld/femspringmanager.py:283-312)."""
+        f = np.zeros(2, np.float32)
+        self._lib.ora_spring_force(ctypes.byref(self._s), int(e), _p(f, _f32p))
+        return f
 
     def substep(self, dt, damping=0.0):
         """One ExplicitLoop.advance(); ``dt`` is rounded to float32 like a Warp float argument."""

@@ -176,7 +176,7 @@ def test_library_exports_declared_symbols():
     for name in sorted(declared):
         assert hasattr(L, name), name
         assert name in _native.SIGNATURES, f"{name} not bound in _native.SIGNATURES"
-    assert L.fw_abi_version() == 1
+    assert L.fw_abi_version() == 2
 
 
 @pytest.mark.parametrize("tile_nodes,tile_slots", [(512, 1024), (64, 1024), (1024, 2048), (7, 1024)])
@@ -228,8 +228,9 @@ def test_explicitloop_rejects_out_of_scope_domains():
     b = fw.FemDomain(fw.Mesh(2, conn, coords + 2.0), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
     with pytest.raises(NotImplementedError, match="FEM-FEM"):
         ExplicitLoop(0.0, [a, b])
-    with pytest.raises(NotImplementedError, match="mass scaling"):
-        ExplicitLoop(0.0, [a], mass_scaling_dt=1e-3)
+    sp = fw.SpringMassDomain(2, coords + 4.0, np.array([[0, 1], [1, 2]], np.int32))
+    with pytest.raises(NotImplementedError, match="FEM-spring"):
+        ExplicitLoop(0.0, [a, sp])
 
 
 def test_prepare_is_host_only_and_thread_safe():
@@ -283,3 +284,93 @@ def test_isolated_node_and_degenerate_element_tile():
     for tile_nodes, tile_slots in ((512, 1024), (8, 512)):
         stats = _native.tiling_check(spec, tile_nodes, tile_slots)
         assert stats["tiles"] >= 1 and stats["csr_entries"] == 3 * spec.n_elems
+
+
+# ------------------------------------------------------------------ spring-mass domains
+
+
+def _spring_net(n, seed=0, jitter=0.2):
+    """A spring network over an n x n node lattice: edges, diagonals and a few bending springs."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    ij = np.array([(i, j) for i in range(n) for j in range(n)])
+    X = (ij * h + rng.uniform(-jitter * h, jitter * h, ij.shape)).astype(np.float32)
+    idx = lambda i, j: i * n + j  # noqa: E731
+    conn = []
+    for i in range(n):
+        for j in range(n):
+            for di, dj in ((1, 0), (0, 1), (1, 1), (1, -1), (2, 0)):
+                if 0 <= i + di < n and 0 <= j + dj < n:
+                    conn.append((idx(i, j), idx(i + di, j + dj)))
+    return X, np.array(conn, np.int32)
+
+
+def test_spring_rest_lengths_equal_reference_loop():
+    """Vectorized rest lengths == the reference's per-spring np.linalg.norm loop (flatworld/springmass.py:46-52)."""
+    from flatworld_amd.springmass import spring_rest_lengths
+
+    for seed in range(3):
+        X, conn = _spring_net(20, seed)
+        X = X * np.float32(3.7) - np.float32(1.3)
+        loop = np.zeros(conn.shape[0], np.float32)
+        for i in range(conn.shape[0]):
+            loop[i] = np.linalg.norm(X[conn[i, 0]] - X[conn[i, 1]])
+        assert np.array_equal(spring_rest_lengths(X, conn), loop)
+    dom = fw.SpringMassDomain(2, X, conn, [10.0, 1.0, 2.0])
+    assert dom.stableTime == 0.5 * np.sqrt(2.0 / 10.0) and dom.massInv == 0.5
+
+
+def test_spring_packing():
+    X, conn = _spring_net(6)
+    dom = fw.SpringMassDomain(2, X, conn, [80.0, 0.5, 0.25], [fw.Gravity([0.0, -9.8]), fw.Fixed([0, 1])],
+                              initials=[fw.InitialVel("ALL", [0.1, 0.0])])
+    spec = fw.pack_spring_domain(dom)
+    assert spec.mat_type == 4 and spec.conn.shape == (conn.shape[0], 2)
+    assert spec.E == np.float32(80.0) and spec.rho == np.float32(0.25) and spec.spring_c == np.float32(0.5)
+    assert np.array_equal(spec.bnodes, np.arange(X.shape[0]))  # every node is a contact work item
+    flags, _ = spec.node_flags()
+    assert np.all(flags & fw.definitions.NODE_BOUNDARY_BIT)
+    assert np.array_equal(spec.rest_len, dom.restLength) and np.all(spec.v0[:, 0] == np.float32(0.1))
+    d, keep = _native.scene_desc(spec)
+    assert d.mat_type == 4 and bool(d.rest_len) and abs(d.spring_c - 0.5) < 1e-7
+
+
+def test_spring_contact_factory():
+    from flatworld_amd.femcontact import ContactSpringAnalytical, ContactSpringHeightField, contactHelper
+
+    X, conn = _spring_net(4)
+    sp = fw.SpringMassDomain(2, X, conn, [30.0, 0.0, 0.2])
+    g = fw.GroundDomain(2, [0, 0], [0, 1])
+    hf = fw.HeightFieldDomain(2, np.zeros(8, np.float32), [0.0, 0.0], [1.0, 1.0])
+    vox = fw.VoxelGridDomain(2, 4, 4, [0.0, 0.0], [1.0, 1.0])
+    c = contactHelper(g, sp, g.type | sp.type)
+    assert isinstance(c, ContactSpringAnalytical) and c.domain1 is sp and c.penalty == 3000.0
+    assert c.stableTime == 0.5 * (0.2 / 3000.0) ** 0.5
+    assert isinstance(contactHelper(sp, hf, sp.type | hf.type), ContactSpringHeightField)
+    # the reference's factory has no spring-voxel or spring-spring class: no contact
+    assert contactHelper(sp, vox, sp.type | vox.type) is None
+    assert contactHelper(sp, sp, sp.type) is None
+
+
+@pytest.mark.parametrize("shape", [(256, 512), (512, 1024), (1024, 2048)])
+def test_spring_tiling_invariants(shape):
+    """The host tiler on spring networks: one tile and many (halo velocity copies pushed once each)."""
+    tile_nodes, tile_slots = shape
+    for n in (5, 40):
+        X, conn = _spring_net(n, seed=n)
+        spec = fw.pack_spring_domain(fw.SpringMassDomain(2, X, conn, [10.0, 1.0, 1.0]))
+        st = _native.tiling_check(spec, tile_nodes, tile_slots)
+        assert st["csr_entries"] == 2 * conn.shape[0]
+        if n == 40:
+            assert st["tiles"] > 1 and st["halo_slots"] > 0
+
+
+def test_spring_scene_validation():
+    X, conn = _spring_net(4)
+    spec = fw.pack_spring_domain(fw.SpringMassDomain(2, X, conn, [10.0, 1.0, 1.0]))
+    with pytest.raises(ValueError, match="rest_len"):
+        _native.tiling_check(dataclasses.replace(spec, rest_len=spec.rest_len[:-1]))
+    bad = dataclasses.replace(spec, conn=spec.conn.copy())
+    bad.conn[3, 1] = X.shape[0] + 2
+    with pytest.raises(ValueError, match="outside"):
+        _native.tiling_check(bad)
