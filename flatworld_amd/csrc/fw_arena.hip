@@ -792,7 +792,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
     D.hot.mat_type = d->mat_type;
     D.hot.n_grounds = d->n_grounds;
     D.hot.E = d->E;
-    D.hot.nu = d->nu;
+    D.hot.nu = d->mat_type == FW_MAT_SPRING ? d->spring_c : d->nu;  // springs: the damping coefficient
     D.rho = d->rho;
     D.hot.sigma_y = d->sigma_y;
     D.hot.H = d->H;
