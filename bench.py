@@ -21,6 +21,9 @@ Printed JSON line:
                    ground, horizon 60-600 substeps) at 80% of the capacity measured
                    on a closed-loop burst: p50 / p99 request latency, throughput,
                    hbm_frac of the served work over the wall clock
+  springs          4096 spring lattices (side log-uniform in [8, 128], one height field
+                   each) batched the same way: spring-substeps/s, the spring substep's
+                   hbm_frac, its own autotuned shape and a CPU baseline on a sample
   cpu_baseline     the oracle (serial C restatement) on a bounded sample
 """
 
@@ -36,7 +39,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "substep_batched4096_traffic.json")
+TRAFFIC_PROFILES = {
+    "batched": os.path.join(ROOT, "profiles", "substep_batched4096_traffic.json"),
+    "springs": os.path.join(ROOT, "profiles", "substep_springs4096_traffic.json"),
+}
 
 
 def log(msg):
@@ -54,6 +60,8 @@ def build_specs(config, seed, n_scenes):
         return [W.j2_voxel_config(seed=seed)]
     if config == "small":
         return [W.small_config()]
+    if config == "springs":
+        return W.spring_batched_config(n_scenes=n_scenes, seed=seed)
     raise SystemExit(f"unknown config {config}")
 
 
@@ -109,7 +117,7 @@ def cpu_baseline(specs, budget_s=12.0):
         "unit": "element-substeps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{len(sample)} of {len(specs)} scenes ({elems} triangles) x {n1} substeps, serial C oracle",
+        "sample": f"{len(sample)} of {len(specs)} scenes ({elems} elements) x {n1} substeps, serial C oracle",
         "all_cores": {"value": multi, "cores": cores,
                       "sample": f"same scenes x {reps} substeps, spread over {cores} threads"},
     }
@@ -164,14 +172,104 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
 
 def traffic_from_profile(shape, config):
     """HBM bytes per substep launch from the committed counter profile of this shape, if any."""
+    path = TRAFFIC_PROFILES.get(config.split("-")[0])
     try:
-        with open(TRAFFIC_PROFILE) as fh:
+        with open(path) as fh:
             prof = json.load(fh)
-    except (OSError, ValueError):
+    except (OSError, ValueError, TypeError):
         return None, None
     if prof.get("workload") != config or any(prof.get("shape", {}).get(k) != v for k, v in shape.items()):
         return None, None
-    return prof.get("bytes_per_launch"), os.path.relpath(TRAFFIC_PROFILE, ROOT)
+    return prof.get("bytes_per_launch"), os.path.relpath(path, ROOT)
+
+
+def tune_for(specs, args):
+    """Kernel shape, workgroups per CU and substeps per graph for these scenes."""
+    from flatworld_amd import autotune
+
+    if args.tile_nodes > 0 or args.no_tune:
+        return {"tile_nodes": args.tile_nodes or 512, "tile_slots": args.tile_slots, "block": args.block,
+                "grid_per_cu": args.grid_per_cu, "frame": args.frame or 60, "source": "explicit"}
+    t = time.perf_counter()
+    res = autotune.tune(specs, log=log)
+    tuning = {k: res[k] for k in ("tile_nodes", "tile_slots", "block", "grid_per_cu", "frame")}
+    tuning["source"] = "autotuner (cached)" if res.get("cached") else "autotuner"
+    tuning["tune_s"] = time.perf_counter() - t
+    if args.frame:
+        tuning["frame"] = args.frame
+    return tuning
+
+
+def timed_arena(specs, tuning, steps, warmup, no_graph=False):
+    """Every scene resident in one arena, warmed up; ``steps`` substeps timed as graph frames,
+    then the average substep-kernel duration from HIP events on the arena stream."""
+    from flatworld_amd import _native
+    from flatworld_amd import autotune
+
+    arena = _native.Arena(**autotune.arena_kwargs(tuning))
+    for s in specs:
+        arena.add(s, track_disp=True)
+    arena.step(1, -1.0, 0)  # zero-dt warm-up of every scene
+    arena.sync()
+    frame = max(1, tuning["frame"])
+
+    def run(n):
+        if no_graph:
+            arena.step(n, -1.0, 0)
+            return
+        full, rest = divmod(n, frame)
+        for _ in range(full):
+            arena.step_graph(frame, 0)
+        if rest:
+            arena.step(rest, -1.0, 0)
+
+    run(warmup)
+    arena.sync()
+    t0 = time.perf_counter()
+    run(steps)
+    arena.sync()
+    total_s = time.perf_counter() - t0
+    _, kernel_ms = arena.time_steps(min(steps, 240), mode=0)
+    return arena, total_s, kernel_ms
+
+
+def spring_bench(args):
+    """4096 batched spring lattices: throughput, HBM fraction of the spring substep, CPU baseline."""
+    from flatworld_amd import workloads as W
+
+    t = time.perf_counter()
+    specs = W.spring_batched_config(n_scenes=args.spring_scenes, seed=args.seed)
+    E = sum(s.n_elems for s in specs)
+    N = sum(s.n_nodes for s in specs)
+    algo = float(sum(W.algorithmic_bytes(s) for s in specs))
+    tuning = tune_for(specs, args)
+    arena, total_s, kernel_ms = timed_arena(specs, tuning, args.spring_steps, args.warmup, args.no_graph)
+    info = arena.info()
+    arena.close()
+    achieved = algo / (kernel_ms * 1e-3) / 1e9
+    log(f"[bench] springs: {len(specs)} lattices, springs={E}, nodes={N}, tiles={info['n_tiles']}, "
+        f"{args.spring_steps} substeps in {total_s * 1e3:.1f} ms, kernel {kernel_ms * 1e3:.1f} us "
+        f"({time.perf_counter() - t:.1f}s with setup)")
+    out = {
+        "value": E * args.spring_steps / total_s,
+        "unit": "spring-substeps/s",
+        "workload": f"springs-{len(specs)}: spring lattices (structural, shear, bending springs), side log-uniform "
+                    "in [8,128], spacing 0.01, one height field each",
+        "scenes": len(specs),
+        "springs": E,
+        "nodes": N,
+        "ms_per_step": total_s * 1e3 / args.spring_steps,
+        "kernel_ms": kernel_ms,
+        "algo_bytes_per_substep": algo,
+        "hbm_frac": achieved / HBM_PEAK_GBS,
+        "achieved_GBps": achieved,
+        "tuning": tuning,
+    }
+    if not args.no_cpu:
+        cpu = cpu_baseline(specs, args.cpu_budget / 3.0)
+        cpu["unit"] = "spring-substeps/s"
+        out["cpu_baseline"] = cpu
+    return out
 
 
 def main():
@@ -180,7 +278,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1200, help="timed substeps (whole frames replay as hipGraphs)")
     ap.add_argument("--warmup", type=int, default=120)
     ap.add_argument("--frame", type=int, default=0, help="substeps per hipGraph frame (0: the autotuner's choice)")
-    ap.add_argument("--config", default="batched", choices=["batched", "large", "j2voxel", "small"])
+    ap.add_argument("--config", default="batched", choices=["batched", "large", "j2voxel", "small", "springs"])
     ap.add_argument("--scenes", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tile-nodes", type=int, default=0, help="explicit kernel shape (skips the autotuner)")
@@ -195,6 +293,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph frames")
+    ap.add_argument("--spring-scenes", type=int, default=4096, help="spring lattices of the spring leg (0: skip)")
+    ap.add_argument("--spring-steps", type=int, default=600)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -210,7 +310,6 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from flatworld_amd import _native
     from flatworld_amd import autotune
     from flatworld_amd import workloads as W
 
@@ -221,56 +320,20 @@ def main():
     algo = float(sum(W.algorithmic_bytes(s) for s in specs))
 
     # kernel shape, workgroups per CU, substeps per graph
-    if args.tile_nodes > 0 or args.no_tune:
-        tuning = {"tile_nodes": args.tile_nodes or 512, "tile_slots": args.tile_slots, "block": args.block,
-                  "grid_per_cu": args.grid_per_cu, "frame": args.frame or 60, "source": "explicit"}
-    else:
-        t = time.perf_counter()
-        res = autotune.tune(specs, log=log)
-        tuning = {k: res[k] for k in ("tile_nodes", "tile_slots", "block", "grid_per_cu", "frame")}
-        tuning["source"] = "autotuner (cached)" if res.get("cached") else "autotuner"
-        tuning["tune_s"] = time.perf_counter() - t
-        if args.frame:
-            tuning["frame"] = args.frame
+    tuning = tune_for(specs, args)
     arena_kw = autotune.arena_kwargs(tuning)
     log(f"[bench] tuning: {tuning}")
-
-    arena = _native.Arena(**arena_kw)
-    for s in specs:
-        arena.add(s, track_disp=True)
-    arena.step(1, -1.0, 0)  # zero-dt warm-up of every scene
-    arena.sync()
-    info = arena.info()
-    t_setup = time.perf_counter() - t_setup
-    log(f"[bench] {args.config}: {len(specs)} scenes, E={E}, N={N}, tiles={info['n_tiles']}, "
-        f"halo slots={info['halo_slots']}, setup {t_setup:.1f}s")
-
     frame = max(1, tuning["frame"])
 
-    def run(n):
-        if args.no_graph:
-            arena.step(n, -1.0, 0)
-            return
-        full, rest = divmod(n, frame)
-        for _ in range(full):
-            arena.step_graph(frame, 0)
-        if rest:
-            arena.step(rest, -1.0, 0)
-
-    run(args.warmup)
-    arena.sync()
-    t0 = time.perf_counter()
-    run(args.steps)
-    arena.sync()
-    t1 = time.perf_counter()
-    total_s = t1 - t0
+    # timed graph frames, then the live kernel duration: HIP events around each substep
+    # kernel on the arena stream
+    arena, total_s, kernel_ms = timed_arena(specs, tuning, args.steps, args.warmup, args.no_graph)
+    info = arena.info()
     value = E * args.steps / total_s
-    log(f"[bench] batched: {args.steps} substeps in {total_s * 1e3:.1f} ms")
-
-    # live kernel duration: HIP events around each substep kernel on the arena stream
-    n_k = min(args.steps, 240)
-    _, kernel_ms = arena.time_steps(n_k, mode=0)
     achieved = algo / (kernel_ms * 1e-3) / 1e9
+    log(f"[bench] {args.config}: {len(specs)} scenes, E={E}, N={N}, tiles={info['n_tiles']}, "
+        f"halo slots={info['halo_slots']}, {time.perf_counter() - t_setup:.1f}s with setup; "
+        f"{args.steps} substeps in {total_s * 1e3:.1f} ms")
 
     per_scene = None
     if len(specs) > 1 and args.per_scene_steps > 0:
@@ -290,6 +353,10 @@ def main():
     serving = None
     if args.serve_jobs > 0 and args.config == "batched":
         serving = serving_bench(args.serve_jobs, args.seed, arena_kw, args.serve_frame, args.serve_burst)
+
+    springs = None
+    if args.spring_scenes > 0 and args.config == "batched":
+        springs = spring_bench(args)
 
     cpu = None
     if not args.no_cpu:
@@ -314,7 +381,7 @@ def main():
         "config": {
             "workload": f"{args.config}-{len(specs)}",
             "scenes": len(specs),
-            "triangles": E,
+            "elements": E,
             "nodes": N,
             "frame_substeps": frame,
             "seed": args.seed,
@@ -333,6 +400,7 @@ def main():
         },
         "per_scene": per_scene,
         "serving": serving,
+        "springs": springs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -33,7 +33,7 @@
 
 extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
                                         int tile_base, int tile_count, float dt_override, float damping_override,
-                                        uint32_t flags, int grid, int block, int slots, hipStream_t stream);
+                                        uint32_t flags, int grid, int block, int slots, int kind, hipStream_t stream);
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
                                         int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
@@ -565,7 +565,8 @@ struct fw_arena {
     // host state
     std::vector<std::unique_ptr<SceneHost>> scenes;
     std::vector<int> free_ids;
-    std::vector<TileDev> tile_table;
+    std::vector<TileDev> tile_table;  // triangle scenes' tiles first, then spring scenes'
+    int n_tri_tiles = 0;
     bool tiles_dirty = false;
     int64_t clock = 0;  // host mirror of ctl->clock
     std::vector<std::unique_ptr<Staging>> staging;
@@ -574,6 +575,7 @@ struct fw_arena {
     int graph_n = -1;
     uint32_t graph_flags = 0;
     int graph_grid = 0;
+    int graph_kinds = 0;  // bit 0: triangle launch captured, bit 1: spring launch captured
     // one-entry tiling cache: fw_scene_footprint followed by fw_scene_add (or a retry of
     // an add that found the arena full) reuses the tiling instead of rebuilding it
     struct {
@@ -695,10 +697,13 @@ int upload(fw_arena* a, void* dst, const void* src, int64_t bytes) {
 int ensure_tiles(fw_arena* a) {
     if (!a->tiles_dirty) return FW_OK;
     a->tile_table.clear();
-    for (auto& sp : a->scenes) {
-        if (!sp || !sp->live) continue;
-        sp->tile_base = (int)a->tile_table.size();
-        a->tile_table.insert(a->tile_table.end(), sp->tiles.begin(), sp->tiles.end());
+    for (int pass = 0; pass < 2; ++pass) {  // triangle scenes, then spring scenes
+        for (auto& sp : a->scenes) {
+            if (!sp || !sp->live || (sp->mat_type == FW_MAT_SPRING) != (pass == 1)) continue;
+            sp->tile_base = (int)a->tile_table.size();
+            a->tile_table.insert(a->tile_table.end(), sp->tiles.begin(), sp->tiles.end());
+        }
+        if (pass == 0) a->n_tri_tiles = (int)a->tile_table.size();
     }
     const int nt = (int)a->tile_table.size();
     if (nt > a->tile_cap) {
@@ -716,8 +721,8 @@ int ensure_tiles(fw_arena* a) {
     }
     int rc = upload(a, a->d_tiles, a->tile_table.data(), (int64_t)sizeof(TileDev) * nt);
     if (rc) return rc;
-    ArenaCtl ctl{(int32_t)a->clock, nt};
-    rc = upload(a, &a->d_ctl->n_tiles, &ctl.n_tiles, sizeof(int32_t));
+    ArenaCtl ctl{(int32_t)a->clock, nt, a->n_tri_tiles};
+    rc = upload(a, &a->d_ctl->n_tiles, &ctl.n_tiles, 2 * sizeof(int32_t));  // n_tiles, n_tri_tiles
     if (rc) return rc;
     a->tiles_dirty = false;
     return FW_OK;
@@ -1066,7 +1071,7 @@ int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning*
     a->max_scenes = 16384;
     HIPCHK(hipMalloc(&a->d_scenes, sizeof(SceneDev) * (size_t)a->max_scenes));
     HIPCHK(hipMalloc(&a->d_ctl, sizeof(ArenaCtl)));
-    ArenaCtl c0{0, 0};
+    ArenaCtl c0{0, 0, 0};
     HIPCHK(hipMemcpy(a->d_ctl, &c0, sizeof(c0), hipMemcpyHostToDevice));
     *out = a.release();
     return FW_OK;
@@ -1203,10 +1208,27 @@ int32_t fw_scene_progress(fw_arena* a, int32_t id, int64_t* steps_done, int32_t*
     return FW_OK;
 }
 
+// The substep launches of one substep over the whole tile table: one per element
+// kind present (triangle tiles, then spring tiles), each covering its kind's tiles
+// as the control block counts them at run time.
+static hipError_t launch_all(fw_arena* a, int k, float dt_override, float damping_override, uint32_t f, int grid_tri,
+                      int grid_spr) {
+    if (grid_tri > 0) {
+        hipError_t e = fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
+                                         grid_tri, a->tune.block, a->tune.tile_slots, 0, a->stream);
+        if (e != hipSuccess) return e;
+    }
+    if (grid_spr > 0)
+        return fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
+                                 grid_spr, a->tune.block, a->tune.tile_slots, 1, a->stream);
+    return hipSuccess;
+}
+
 static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
     int rc = ensure_tiles(a);
     if (rc) return rc;
     const int nt = (int)a->tile_table.size();
+    const int n_tri = a->n_tri_tiles, n_spr = nt - n_tri;
     for (int k = 0; k < n; ++k) {
         uint32_t f = flags & ~FW_STEP_PER_SCENE;
         if (k != n - 1) f &= ~FW_STEP_DIAG;
@@ -1216,11 +1238,12 @@ static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_ov
                     if (!sp || !sp->live) continue;
                     const int cnt = (int)sp->tiles.size();
                     HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, sp->tile_base, cnt, dt_override, damping_override, f,
-                                             launch_grid(a, cnt), a->tune.block, a->tune.tile_slots, a->stream));
+                                             launch_grid(a, cnt), a->tune.block, a->tune.tile_slots,
+                                             sp->mat_type == FW_MAT_SPRING ? 1 : 0, a->stream));
                 }
             } else {
-                HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
-                                         launch_grid(a, nt), a->tune.block, a->tune.tile_slots, a->stream));
+                HIPCHK(launch_all(a, k, dt_override, damping_override, f, n_tri ? launch_grid(a, n_tri) : 0,
+                                  n_spr ? launch_grid(a, n_spr) : 0));
             }
         }
     }
@@ -1354,7 +1377,10 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
     if (rc) return rc;
     const int nt = (int)a->tile_table.size();
     const int grid = launch_grid(a, std::max(nt, 1));
-    if (!a->graph || a->graph_n != n || a->graph_flags != flags || grid > a->graph_grid) {
+    // the element kinds present decide which launches a frame holds; the graph's grids
+    // cover the table's capacity, so scenes can come and go without a new capture
+    const int kinds = (a->n_tri_tiles > 0 || nt == 0 ? 1 : 0) | (nt > a->n_tri_tiles ? 2 : 0);
+    if (!a->graph || a->graph_n != n || a->graph_flags != flags || grid > a->graph_grid || kinds != a->graph_kinds) {
         if (a->graph) {
             (void)hipGraphExecDestroy(a->graph);
             a->graph = nullptr;
@@ -1365,8 +1391,7 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
         for (int k = 0; k < n; ++k) {
             uint32_t f = flags;
             if (k != n - 1) f &= ~FW_STEP_DIAG;
-            hipError_t e = fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, -1.0f, -1.0f, f, cap_grid,
-                                             a->tune.block, a->tune.tile_slots, a->stream);
+            hipError_t e = launch_all(a, k, -1.0f, -1.0f, f, (kinds & 1) ? cap_grid : 0, (kinds & 2) ? cap_grid : 0);
             if (e != hipSuccess) {
                 (void)hipStreamEndCapture(a->stream, &g);
                 return fail(FW_E_HIP, std::string("capture launch: ") + hipGetErrorString(e));
@@ -1380,6 +1405,7 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
         a->graph_n = n;
         a->graph_flags = flags;
         a->graph_grid = cap_grid;
+        a->graph_kinds = kinds;
     }
     HIPCHK(hipGraphLaunch(a->graph, a->stream));
     a->clock += n;
@@ -1537,11 +1563,11 @@ int32_t fw_time_steps(fw_arena* a, int32_t n, int32_t mode, double* total_ms, do
         std::vector<hipEvent_t> ev(2 * (size_t)n);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         const int nt = (int)a->tile_table.size();
+        const int n_tri = a->n_tri_tiles, n_spr = nt - n_tri;
         HIPCHK(hipEventRecord(t0, a->stream));
         for (int k = 0; k < n; ++k) {
             HIPCHK(hipEventRecord(ev[2 * k], a->stream));
-            HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, -1.0f, -1.0f, 0, launch_grid(a, nt),
-                                     a->tune.block, a->tune.tile_slots, a->stream));
+            HIPCHK(launch_all(a, k, -1.0f, -1.0f, 0, n_tri ? launch_grid(a, n_tri) : 0, n_spr ? launch_grid(a, n_spr) : 0));
             HIPCHK(hipEventRecord(ev[2 * k + 1], a->stream));
         }
         HIPCHK(fw_launch_advance_clock(a->d_ctl, n, a->stream));

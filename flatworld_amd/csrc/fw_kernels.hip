@@ -158,11 +158,16 @@ struct StepArgs {
     const ArenaCtl* ctl;
     int k;           // substep index within this launch group (global step = ctl->clock + k)
     int tile_base;   // first tile of the range
-    int tile_count;  // tiles in the range; < 0: up to ctl->n_tiles
+    int tile_count;  // tiles in the range; < 0: every tile of this kernel's element kind (from ctl)
     float dt_override;
     float damping_override;
     uint32_t flags;
 };
+
+// Element kinds of the substep kernel: the arena keeps triangle tiles first and
+// spring tiles after them, and launches one instantiation per kind present, so
+// neither kind's registers pay for the other's code.
+constexpr int KIND_TRI = 0, KIND_SPRING = 1;
 
 // Node data phase B needs, loaded before the tile's barrier.
 struct NodeIn {
@@ -250,7 +255,7 @@ __device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, L
 // A workgroup loops over tiles (grid-stride, XCD-aware).  Each tile starts with
 // one batch of independent loads (records, positions, rest positions) whose
 // addresses come from the tile record alone, read with scalar loads.
-template <int BLOCK, int SLOTS, int NODES, int NPT>
+template <int BLOCK, int SLOTS, int NODES, int NPT, int KIND>
 __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     constexpr int SPT = (SLOTS + BLOCK - 1) / BLOCK;
     constexpr int LPT = (NODES + BLOCK - 1) / BLOCK;
@@ -266,14 +271,19 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     float2* const lds_vn = lds_f + 2 * SLOTS;
 
     const int G = fw_ldc1(&A.ctl->clock) + A.k;
-    const int ntiles = A.tile_count >= 0 ? A.tile_count : (fw_ldc1(&A.ctl->n_tiles) - A.tile_base);
+    int tbase = A.tile_base, ntiles = A.tile_count;
+    if (ntiles < 0) {
+        const int n_tri = fw_ldc1(&A.ctl->n_tri_tiles);
+        tbase = KIND == KIND_TRI ? 0 : n_tri;
+        ntiles = KIND == KIND_TRI ? n_tri : fw_ldc1(&A.ctl->n_tiles) - n_tri;
+    }
     const int tid = threadIdx.x;
     const bool diag = (A.flags & STEP_DIAG) != 0;
 
     for (int round = 0;; ++round) {
         const int ti = tile_of_block(round, ntiles);
         if (ti >= ntiles) break;
-        const TileDev T = fw_ldc(A.tiles + A.tile_base + ti);  // scalar loads; stores below cannot alias it
+        const TileDev T = fw_ldc(A.tiles + tbase + ti);  // scalar loads; stores below cannot alias it
         if (!tile_live(T, G)) continue;  // uniform per block
         Stage<SPT, LPT, CPT> st;
         stage_load<BLOCK>(T, G, st);
@@ -281,7 +291,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         const int wr = local & 1;
         const int nslots = T.n_own + T.n_halo;
         const int n_nodes = T.n_nodes;
-        const bool j2 = T.o_j2[0] != 0;
+        const bool j2 = !(KIND == KIND_SPRING) && T.o_j2[0] != 0;
         const float4* __restrict__ j2_rd = fw_at<const float4>(T.base, (local + 1) & 1 ? T.o_j2[1] : T.o_j2[0]);
 
         // ---------------- stage: registers -> LDS
@@ -315,14 +325,14 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         const int n_grounds = fw_ldc1(&sh->n_grounds);
         const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && n_grounds > 0;
         Mat m;  // spring tiles: lam = stiffness, mu = damping coefficient
-        m.type = fw_ldc1(&sh->mat_type);
-        const bool spring = m.type == FW_MAT_SPRING_DEV;
+        constexpr bool spring = KIND == KIND_SPRING;
         {
             const float E = fw_ldc1(&sh->E), nu = fw_ldc1(&sh->nu);
+            m.type = spring ? FW_MAT_SPRING_DEV : fw_ldc1(&sh->mat_type);
             m.lam = spring ? E : E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
             m.mu = spring ? nu : E / (2.0f * (1.0f + nu));
-            m.sy = fw_ldc1(&sh->sigma_y);
-            m.H = fw_ldc1(&sh->H);
+            m.sy = spring ? 0.0f : fw_ldc1(&sh->sigma_y);
+            m.H = spring ? 0.0f : fw_ldc1(&sh->H);
         }
         float4* const st_stress = diag ? fw_ldc1(&Sp->stress) : nullptr;
         float4* const st_strain = diag ? fw_ldc1(&Sp->strain) : nullptr;
@@ -347,7 +357,10 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                     const int i = tid + kn * BLOCK;
                     if (i < n_nodes) {
                         const int n = T.node0 + i;
-                        nd[kn].v = fw_ld(fw_at<const float2>(T.base, T.o_vel) + n);
+                        if constexpr (spring)
+                            nd[kn].v = lds_X[i];  // the stage brought it: spring tiles stage velocities
+                        else
+                            nd[kn].v = fw_ld(fw_at<const float2>(T.base, T.o_vel) + n);
                         nd[kn].u = make_float2(0.0f, 0.0f);
                         if (T.o_disp) nd[kn].u = fw_ld(fw_at<const float2>(T.base, T.o_disp) + n);
                         nd[kn].m = fw_ld(fw_at<const float>(T.base, T.o_mass) + n);
@@ -356,7 +369,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 }
             }
             if (sl < nslots) {
-                if (spring) {
+                if constexpr (spring) {
                     uint32_t l[2], q[2];
                     float l0;
                     unpack_spring(st.rc[k], n_nodes, lds_cp, l, q, l0);
@@ -687,14 +700,18 @@ __global__ void advance_clock_kernel(ArenaCtl* ctl, int n) {
 // Kernel shapes (FW_SHAPES): (threads, element slots, LDS nodes, owned nodes per
 // thread).  The host keeps tile_nodes <= NPT * threads and the LDS node count of a
 // tile <= the shape's LDS nodes.
+// kind: 0 triangle tiles, 1 spring tiles.
 extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
                                         int tile_base, int tile_count, float dt_override, float damping_override,
-                                        uint32_t flags, int grid, int block, int slots, hipStream_t stream) {
+                                        uint32_t flags, int grid, int block, int slots, int kind, hipStream_t stream) {
     fw::StepArgs A{scenes, tiles, ctl, k, tile_base, tile_count, dt_override, damping_override, flags};
-#define FW_LAUNCH(B, S, N, P)                                                                              \
-    if (block == B && slots == S) {                                                                       \
-        hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P>), dim3(grid), dim3(B), 0, stream, A);         \
-        return hipGetLastError();                                                                         \
+#define FW_LAUNCH(B, S, N, P)                                                                                    \
+    if (block == B && slots == S) {                                                                             \
+        if (kind == fw::KIND_SPRING)                                                                            \
+            hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, fw::KIND_SPRING>), dim3(grid), dim3(B), 0, stream, A); \
+        else                                                                                                    \
+            hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, fw::KIND_TRI>), dim3(grid), dim3(B), 0, stream, A);    \
+        return hipGetLastError();                                                                               \
     }
     FW_SHAPES(FW_LAUNCH)
 #undef FW_LAUNCH
@@ -706,7 +723,8 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
 extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu) {
     *per_cu = 0;
 #define FW_OCC(B, S, N, P) \
-    if (block == B && slots == S) return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<B, S, N, P>, B, 0);
+    if (block == B && slots == S)  \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<B, S, N, P, fw::KIND_TRI>, B, 0);
     FW_SHAPES(FW_OCC)
 #undef FW_OCC
     return hipErrorInvalidValue;

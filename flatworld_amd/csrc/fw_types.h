@@ -196,8 +196,9 @@ __device__ inline P* fw_at(char* base, uint32_t off) {
 
 // Arena control block in device memory.
 struct ArenaCtl {
-    int32_t clock;   // global substep counter; substep kernel uses clock + k
-    int32_t n_tiles; // tiles in the table
+    int32_t clock;        // global substep counter; substep kernel uses clock + k
+    int32_t n_tiles;      // tiles in the table
+    int32_t n_tri_tiles;  // the first n_tri_tiles are triangle tiles, the rest spring tiles
 };
 
 // Ordered-int encoding of floats so that integer min/max follow float order.

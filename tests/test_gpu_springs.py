@@ -243,3 +243,42 @@ def test_mass_scaling_loop():
         for sd in loop.substep_sequence(FRAME):
             o.substep(sd, loop.damping)
     assert np.array_equal(loop.femSpringManager.coords.numpy()[: dom.nnodes], o.x)
+
+
+def test_mixed_arena_graph_and_per_scene_launches_equal_plain():
+    """Triangle tiles and spring tiles run as two launches per substep; graph-replayed frames
+    and one launch per scene give the same bits as plain launches, and adding or removing
+    the last spring scene re-captures the frame graph with the right launches."""
+    specs = [_lattice_spec(30, "hf", seed=11), W.small_config(), _lattice_spec(16, "plane", seed=12)]
+    arenas = {m: _native.Arena(tile_nodes=256, tile_slots=1024) for m in ("plain", "graph", "per_scene")}
+    ids = {m: [a.add(s) for s in specs] for m, a in arenas.items()}
+    log(f"[case] mixed arena launch modes: {[(s.mat_type, s.n_elems) for s in specs]}")
+    for a in arenas.values():
+        a.step(1, -1.0, 0)
+    for _ in range(3):
+        arenas["plain"].step(40, -1.0, 0)
+        arenas["graph"].step_graph(40, 0)
+        arenas["per_scene"].step(40, -1.0, _native.FW_STEP_PER_SCENE)
+    for k in range(len(specs)):
+        ref = arenas["plain"].read(ids["plain"][k], _native.FIELD_X)
+        for m in ("graph", "per_scene"):
+            assert np.array_equal(arenas[m].read(ids[m][k], _native.FIELD_X), ref), (m, k)
+    # drop the spring scenes: frames hold the triangle launch only; add one back: both again
+    for m in ("plain", "graph"):
+        arenas[m].remove(ids[m][0])
+        arenas[m].remove(ids[m][2])
+    arenas["plain"].step(40, -1.0, 0)
+    arenas["graph"].step_graph(40, 0)
+    assert np.array_equal(arenas["graph"].read(ids["graph"][1], _native.FIELD_X),
+                          arenas["plain"].read(ids["plain"][1], _native.FIELD_X))
+    extra = _lattice_spec(20, "plane", seed=13)
+    e_plain, e_graph = arenas["plain"].add(extra), arenas["graph"].add(extra)
+    for _ in range(2):
+        arenas["plain"].step(40, -1.0, 0)
+        arenas["graph"].step_graph(40, 0)
+    for sp, sg in ((ids["plain"][1], ids["graph"][1]), (e_plain, e_graph)):
+        assert np.array_equal(arenas["graph"].read(sg, _native.FIELD_X), arenas["plain"].read(sp, _native.FIELD_X))
+    o = OracleScene(extra)
+    o.warmup()  # the first of the 80 launches after the add is the scene's warm-up
+    o.run(79)
+    assert np.array_equal(arenas["graph"].read(e_graph, _native.FIELD_X), o.x)
