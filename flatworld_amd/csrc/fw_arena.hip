@@ -591,14 +591,17 @@ namespace {
 
 // LDS nodes (owned + halo) a tile of the compiled shapes holds: FW_SHAPES gives the
 // same count for every thread count of a slot count.
-int node_cap_of_slots(int slots) {
-#define FW_CAP(B, S, N, P) \
-    if (slots == S) return N;
+int node_cap_of_slots(int slots, bool spring = false) {
+#define FW_CAP(B, S, N, P, NS) \
+    if (slots == S) return spring ? NS : N;
     FW_SHAPES(FW_CAP)
 #undef FW_CAP
     return 0;
 }
-int node_cap_of(const fw_tuning& t) { return node_cap_of_slots(t.tile_slots); }
+// LDS nodes a tile of this scene may hold: spring tiles have their own, smaller LDS layout
+int node_cap_of(const fw_tuning& t, const fw_scene_desc* d) {
+    return node_cap_of_slots(t.tile_slots, d->mat_type == FW_MAT_SPRING);
+}
 
 // Tiling of a scene description, from the cache when the same arrays are presented again.
 int tiling_for(fw_arena* a, const fw_scene_desc* d, std::unique_ptr<Tiling>* out) {
@@ -618,7 +621,7 @@ int tiling_for(fw_arena* a, const fw_scene_desc* d, std::unique_ptr<Tiling>* out
         return FW_OK;
     }
     auto T = std::make_unique<Tiling>();
-    int rc = build_tiling(d, a->tune.tile_nodes, a->tune.tile_slots, node_cap_of(a->tune), *T);
+    int rc = build_tiling(d, a->tune.tile_nodes, a->tune.tile_slots, node_cap_of(a->tune, d), *T);
     if (rc) return rc;
     c.x0 = d->x0;
     c.conn = d->conn;
@@ -775,7 +778,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             rc = tiling_for(cache_arena, d, &Tp);
         } else {
             Tp = std::make_unique<Tiling>();
-            rc = build_tiling(d, tune.tile_nodes, tune.tile_slots, node_cap_of(tune), *Tp);
+            rc = build_tiling(d, tune.tile_nodes, tune.tile_slots, node_cap_of(tune, d), *Tp);
         }
         if (rc) return rc;
         P->T = std::move(*Tp);
@@ -1018,7 +1021,7 @@ int tuning_of(const fw_tuning* tuning, fw_tuning* out) {
     t.grid_per_cu = 0;
     if (tuning) {
         int npt = 0;
-#define FW_NPT(B, S, N, P) \
+#define FW_NPT(B, S, N, P, NS) \
     if (tuning->block == B && tuning->tile_slots == S) npt = P;
         FW_SHAPES(FW_NPT)
 #undef FW_NPT
@@ -1269,7 +1272,7 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
     int rc = validate(d);
     if (rc) return rc;
     Tiling T;
-    const int node_cap = node_cap_of_slots(tile_slots);
+    const int node_cap = node_cap_of_slots(tile_slots, d->mat_type == FW_MAT_SPRING);
     if (node_cap == 0) return fail(FW_E_INVALID, "tile_slots must be one of the compiled shapes (512, 768, 1024, 2048)");
     rc = build_tiling(d, tile_nodes, tile_slots, node_cap, T);
     if (rc) return rc;

@@ -245,30 +245,36 @@ __device__ __forceinline__ void stage_load(const TileDev& T, int G, Stage<SPT, L
     }
 }
 
-// BLOCK threads; at most SLOTS element slots, NODES LDS nodes (owned + halo) and
-// NPT * BLOCK owned nodes per tile.  LDS: 16 B per node + 24 B per slot, nothing
-// else (the AABB reduction reuses the rest-position array once phase A is done),
-// so 1024 slots take exactly 40 KiB and four tiles fit one CU's 160 KiB.  A
-// spring slot has two corners, so the last third of the force entries is free in
-// spring tiles: it holds the owned nodes' new velocities for the halo push.
+// BLOCK threads; at most SLOTS element slots, NODES_TRI (triangle tiles) or
+// NODES_SPR (spring tiles) LDS nodes (owned + halo) and NPT * BLOCK owned nodes per
+// tile.  Triangle tiles: 16 B per node + 24 B per slot, nothing else (the AABB
+// reduction reuses the rest-position array once phase A is done), so 1024 slots
+// take exactly 40 KiB and four tiles fit one CU's 160 KiB.  Spring tiles: 16 B per
+// node + 16 B per slot and a small reduction array; their staged velocities are
+// dead after phase A, so phase B overwrites them with the owned nodes' new
+// velocities, the source of the halo velocity copies.
 //
 // A workgroup loops over tiles (grid-stride, XCD-aware).  Each tile starts with
 // one batch of independent loads (records, positions, rest positions) whose
 // addresses come from the tile record alone, read with scalar loads.
-template <int BLOCK, int SLOTS, int NODES, int NPT, int KIND>
+template <int BLOCK, int SLOTS, int NODES_TRI, int NPT, int NODES_SPR, int KIND>
 __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
+    constexpr bool spring = KIND == KIND_SPRING;
+    constexpr int NODES = spring ? NODES_SPR : NODES_TRI;
+    constexpr int CORNERS = spring ? 2 : 3;
     constexpr int SPT = (SLOTS + BLOCK - 1) / BLOCK;
     constexpr int LPT = (NODES + BLOCK - 1) / BLOCK;
     constexpr int CPT = NPT + 1;  // CSR offsets per thread: owned nodes + 1
+    constexpr int NRED = (BLOCK / 64) * 4;
     __shared__ float2 lds_x[NODES];
     __shared__ float2 lds_X[NODES];
-    __shared__ float2 lds_f[SLOTS * 3];
+    __shared__ float2 lds_f[SLOTS * CORNERS];
     __shared__ uint16_t lds_cp[NPT * BLOCK + 1];
-    static_assert((BLOCK / 64) * 4 * sizeof(int32_t) <= NODES * sizeof(float2), "reduction scratch");
+    __shared__ int32_t lds_red_spr[spring ? NRED : 1];
+    static_assert(NRED * sizeof(int32_t) <= NODES_TRI * sizeof(float2), "reduction scratch");
     static_assert(NODES <= (int)FW_REC_ID_MASK + 1, "local node ids fit the record");
-    static_assert(NPT * BLOCK <= SLOTS, "spring tiles keep new velocities in the free force entries");
-    int32_t* lds_red = reinterpret_cast<int32_t*>(lds_X);
-    float2* const lds_vn = lds_f + 2 * SLOTS;
+    int32_t* lds_red = spring ? lds_red_spr : reinterpret_cast<int32_t*>(lds_X);
+    float2* const lds_vn = lds_X;  // spring tiles, phase B on: new velocities of the owned nodes
 
     const int G = fw_ldc1(&A.ctl->clock) + A.k;
     int tbase = A.tile_base, ntiles = A.tile_count;
@@ -325,7 +331,6 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         const int n_grounds = fw_ldc1(&sh->n_grounds);
         const bool contact = !warm && !(A.flags & STEP_NO_CONTACT) && n_grounds > 0;
         Mat m;  // spring tiles: lam = stiffness, mu = damping coefficient
-        constexpr bool spring = KIND == KIND_SPRING;
         {
             const float E = fw_ldc1(&sh->E), nu = fw_ldc1(&sh->nu);
             m.type = spring ? FW_MAT_SPRING_DEV : fw_ldc1(&sh->mat_type);
@@ -705,12 +710,12 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
                                         int tile_base, int tile_count, float dt_override, float damping_override,
                                         uint32_t flags, int grid, int block, int slots, int kind, hipStream_t stream) {
     fw::StepArgs A{scenes, tiles, ctl, k, tile_base, tile_count, dt_override, damping_override, flags};
-#define FW_LAUNCH(B, S, N, P)                                                                                    \
+#define FW_LAUNCH(B, S, N, P, NS)                                                                                    \
     if (block == B && slots == S) {                                                                             \
         if (kind == fw::KIND_SPRING)                                                                            \
-            hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, fw::KIND_SPRING>), dim3(grid), dim3(B), 0, stream, A); \
+            hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, NS, fw::KIND_SPRING>), dim3(grid), dim3(B), 0, stream, A); \
         else                                                                                                    \
-            hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, fw::KIND_TRI>), dim3(grid), dim3(B), 0, stream, A);    \
+            hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, NS, fw::KIND_TRI>), dim3(grid), dim3(B), 0, stream, A);    \
         return hipGetLastError();                                                                               \
     }
     FW_SHAPES(FW_LAUNCH)
@@ -722,9 +727,9 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
 // calculator: LDS, registers and wave slots).
 extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu) {
     *per_cu = 0;
-#define FW_OCC(B, S, N, P) \
+#define FW_OCC(B, S, N, P, NS) \
     if (block == B && slots == S)  \
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<B, S, N, P, fw::KIND_TRI>, B, 0);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<B, S, N, P, NS, fw::KIND_TRI>, B, 0);
     FW_SHAPES(FW_OCC)
 #undef FW_OCC
     return hipErrorInvalidValue;
@@ -733,7 +738,7 @@ extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu) {
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
                                         int block, int slots, hipStream_t stream) {
     if (tile_count <= 0) return hipSuccess;
-#define FW_PREP(B, S, N, P)                                                                                \
+#define FW_PREP(B, S, N, P, NS)                                                                                \
     if (block == B && slots == S) {                                                                       \
         hipLaunchKernelGGL((fw::prepare_kernel<256, S, N, B * P>), dim3(tile_count), dim3(256), 0, stream, \
                            scenes, tiles, tile_base, tile_count);                                         \
@@ -747,7 +752,7 @@ extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* t
 extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
                                              float dt_target, int block, int slots, hipStream_t stream) {
     if (tile_count <= 0) return hipSuccess;
-#define FW_MS(B, S, N, P)                                                                                  \
+#define FW_MS(B, S, N, P, NS)                                                                                  \
     if (block == B && slots == S) {                                                                       \
         hipLaunchKernelGGL((fw::mass_scale_kernel<256, S, N, B * P>), dim3(tile_count), dim3(256), 0, stream, \
                            scenes, tiles, tile_count, buf, dt_target);                                    \

@@ -123,16 +123,20 @@ struct TileDev {
 #define FW_REC_MAX_RANK 31
 
 // Compiled substep kernel shapes: (threads, element slots, LDS nodes, owned nodes per
-// thread).  LDS per workgroup = 16 B per LDS node (positions, rest positions) + 24 B
-// per slot (three corner forces) + 2 B per CSR offset (owned nodes + 1), sized so
-// that 4 (1024 slots), 5 (768), 8 (512) or 2 (2048) workgroups share a CU's 160 KiB.
-#define FW_SHAPES(X)         \
-    X(256, 1024, 952, 2)     \
-    X(256, 768, 704, 2)      \
-    X(512, 1024, 952, 1)     \
-    X(256, 512, 472, 1)      \
-    X(512, 2048, 1912, 2)    \
-    X(1024, 2048, 1912, 1)
+// thread, LDS nodes of a spring tile).  Triangle tiles: 16 B per LDS node (positions,
+// rest positions) + 24 B per slot (three corner forces) + 2 B per CSR offset (owned
+// nodes + 1), sized so that 4 (1024 slots), 5 (768), 8 (512) or 2 (2048) workgroups
+// share a CU's 160 KiB.  Spring tiles: 16 B per LDS node (positions, velocities) +
+// 16 B per slot (two corner forces); a spring tile fills its slots at about 6 springs
+// per owned node, so its node capacity is smaller and 6 (1024 slots), 8 (768, 512)
+// or 3 (2048) of its workgroups share a CU.
+#define FW_SHAPES(X)              \
+    X(256, 1024, 952, 2, 448)     \
+    X(256, 768, 704, 2, 384)      \
+    X(512, 1024, 952, 1, 448)     \
+    X(256, 512, 472, 1, 256)      \
+    X(512, 2048, 1912, 2, 768)    \
+    X(1024, 2048, 1912, 1, 768)
 
 // HBM accesses of the kernels go through fw_ld / fw_st: they address global
 // memory explicitly.  A plain access through a pointer the compiler cannot prove
