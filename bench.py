@@ -265,6 +265,8 @@ def spring_bench(args):
         "achieved_GBps": achieved,
         "tuning": tuning,
     }
+    shape = {k: tuning[k] for k in ("tile_nodes", "tile_slots", "block")}
+    out["traffic"], out["traffic_source"] = traffic_from_profile(shape, f"springs-{len(specs)}")
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget / 3.0)
         cpu["unit"] = "spring-substeps/s"
