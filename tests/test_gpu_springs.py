@@ -282,3 +282,32 @@ def test_mixed_arena_graph_and_per_scene_launches_equal_plain():
     o.warmup()  # the first of the 80 launches after the add is the scene's warm-up
     o.run(79)
     assert np.array_equal(arenas["graph"].read(e_graph, _native.FIELD_X), o.x)
+
+
+def test_served_spring_and_triangle_jobs_equal_solo_runs():
+    """Continuous batching with spring lattices among triangle jobs: every served result equals its solo run."""
+    import dataclasses
+
+    from flatworld_amd.batch import FemBatch
+    from flatworld_amd.server import Job, SimServer
+
+    rng = W.rng_for(41)
+    specs = []
+    for k in range(10):
+        if k % 2:
+            specs.append(W.serving_job(rng, 6, 20, 20, 120))
+        else:
+            s = _lattice_spec(int(rng.integers(6, 30)), "plane" if k % 4 else "hf", seed=50 + k)
+            specs.append(dataclasses.replace(s, horizon=int(rng.integers(20, 120))))
+    jobs = [Job(i, s, 0.002 * i) for i, s in enumerate(specs)]
+    log(f"[case] serve {len(jobs)} mixed spring / triangle jobs")
+    srv = SimServer(frame=20, tile_nodes=256)
+    res = {r.job_id: r for r in srv.run(jobs)}
+    assert len(res) == len(jobs) and all(r.ok for r in res.values())
+    for j in jobs:
+        b = FemBatch(tile_nodes=256)
+        sid = b.add(j.spec)
+        b.step(1 + j.spec.horizon, graph=False)
+        x = b.positions(sid)
+        b.close()
+        assert np.array_equal(res[j.job_id].positions, x), f"job {j.job_id} (material {j.spec.mat_type})"
