@@ -89,9 +89,11 @@ uint32_t spread16(uint32_t v) {
 // Corners per element: three for a triangle, two for a spring.
 inline int corners_of(const fw_scene_desc* d) { return d->mat_type == FW_MAT_SPRING ? 2 : 3; }
 
-int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int node_cap, Tiling& T) {
+// K corners per element, a compile-time constant: the incidence codes K * e + c are
+// divided by K in every loop below.
+template <int K>
+int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int node_cap, Tiling& T) {
     const int N = d->n_nodes, E = d->n_elems;
-    const int K = corners_of(d);
     // incidence (caller ids): node -> (element, corner), element-major so lists are ascending
     std::vector<int32_t> inc_ptr(N + 1, 0);
     for (int64_t i = 0; i < K * (int64_t)E; ++i) {
@@ -137,15 +139,15 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
             }
             key[i] = spread16(q[0]) | (spread16(q[1]) << 1);
         }
-        // stable LSD radix sort of the node ids by key, two 16-bit passes
+        // stable LSD radix sort of the node ids by key, three 11-bit passes
         std::vector<int32_t> tmp(N);
-        std::vector<uint32_t> cnt(65537);
-        for (int pass = 0; pass < 2; ++pass) {
-            const int sh = 16 * pass;
+        std::vector<uint32_t> cnt(2049);
+        for (int pass = 0; pass < 3; ++pass) {
+            const int sh = 11 * pass;
             std::fill(cnt.begin(), cnt.end(), 0u);
-            for (int i = 0; i < N; ++i) cnt[((key[T.node_perm[i]] >> sh) & 0xFFFFu) + 1]++;
-            for (int b = 0; b < 65536; ++b) cnt[b + 1] += cnt[b];
-            for (int i = 0; i < N; ++i) tmp[cnt[(key[T.node_perm[i]] >> sh) & 0xFFFFu]++] = T.node_perm[i];
+            for (int i = 0; i < N; ++i) cnt[((key[T.node_perm[i]] >> sh) & 0x7FFu) + 1]++;
+            for (int b = 0; b < 2048; ++b) cnt[b + 1] += cnt[b];
+            for (int i = 0; i < N; ++i) tmp[cnt[(key[T.node_perm[i]] >> sh) & 0x7FFu]++] = T.node_perm[i];
             T.node_perm.swap(tmp);
         }
     }
@@ -235,6 +237,13 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
     T.tiles.resize(ntiles);
     std::fill(stamp.begin(), stamp.end(), -1);
     std::vector<int32_t> slot_of(E, -1), lid(N, -1), nst(N, -1);
+    // scratch reused by every tile, and the outputs sized for a typical halo up front
+    std::vector<int32_t> slots, hl, hn;
+    std::vector<uint8_t> rank;
+    T.rec.reserve(2 * ((size_t)E + E / 4 + 64));
+    T.hel.reserve((size_t)E / 4 + 64);
+    T.csr_ptr.reserve((size_t)N + ntiles);
+    T.hnode.reserve((size_t)N / 4 + 64);
     for (int t = 0; t < ntiles; ++t) {
         TileDev& td = T.tiles[t];
         std::memset(&td, 0, sizeof(td));
@@ -245,9 +254,9 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         td.n_own = own_count[t + 1] - own_count[t];
         td.multi = ntiles > 1;
         // slots: owned elements in storage order, then halo elements (ascending storage index)
-        std::vector<int32_t> slots;  // internal element index per slot
+        slots.clear();  // internal element index per slot
         for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) slots.push_back(i);
-        std::vector<int32_t> hl;
+        hl.clear();
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
@@ -267,7 +276,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         for (int s = 0; s < nslots; ++s) slot_of[T.elem_perm[slots[s]]] = s;
         // local node ids: owned nodes first (p - p0), then halo nodes in ascending storage order
         for (int p = p0; p < p1; ++p) lid[T.node_perm[p]] = p - p0;
-        std::vector<int32_t> hn;
+        hn.clear();
         for (int s = 0; s < nslots; ++s) {
             const int e = T.elem_perm[slots[s]];
             for (int c = 0; c < K; ++c) {
@@ -288,7 +297,7 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         // node-major incidence positions: node i of the tile owns LDS entries
         // [ptr[i], ptr[i+1]) in ascending (caller element id, corner); a corner's
         // rank is its position in its node's list
-        std::vector<uint8_t> rank(K * (size_t)nslots, 0);
+        rank.assign(K * (size_t)nslots, 0);
         td.ptr_off = (int32_t)T.csr_ptr.size();
         int cnt = 0;
         T.csr_ptr.push_back(0);
@@ -346,6 +355,11 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
         T.push.insert(T.push.end(), by_src[t].begin(), by_src[t].end());
     }
     return FW_OK;
+}
+
+int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int node_cap, Tiling& T) {
+    return corners_of(d) == 2 ? build_tiling_k<2>(d, tile_nodes, tile_slots, node_cap, T)
+                              : build_tiling_k<3>(d, tile_nodes, tile_slots, node_cap, T);
 }
 
 // Height-field block: nx float heights, then (8-byte aligned) nx float2 unit normals.
