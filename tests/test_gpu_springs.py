@@ -311,3 +311,24 @@ def test_served_spring_and_triangle_jobs_equal_solo_runs():
         x = b.positions(sid)
         b.close()
         assert np.array_equal(res[j.job_id].positions, x), f"job {j.job_id} (material {j.spec.mat_type})"
+
+
+def test_spring_4096_arena_sampled_scenes_bitexact():
+    """The full 4096-lattice spring arena of the benchmark, in its tuned shape with graph frames:
+    32 sampled lattices against the oracle after 60 substeps."""
+    specs = W.spring_batched_config(n_scenes=4096, seed=0)
+    log(f"[case] spring arena, 4096 lattices, {sum(s.n_elems for s in specs)} springs")
+    arena = _native.Arena(tile_nodes=1024, tile_slots=2048, block=512)
+    ids = [arena.add(s) for s in specs]
+    arena.step(1, -1.0, 0)
+    arena.step_graph(59, 0)
+    arena.step(1, -1.0, _native.FW_STEP_DIAG)
+    pick = W.rng_for(98).choice(len(specs), size=32, replace=False)
+    for k in pick:
+        o = OracleScene(specs[k])
+        o.warmup()
+        o.run(60)
+        for fld, ref in ((_native.FIELD_X, o.x), (_native.FIELD_V, o.v), (_native.FIELD_FORCE, o.fext)):
+            assert np.isfinite(ref).all()
+            assert np.array_equal(arena.read(ids[k], fld), ref), f"lattice {k} field {fld}"
+    arena.close()
