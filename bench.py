@@ -125,8 +125,15 @@ def cpu_baseline(specs, budget_s=12.0):
     }
 
 
-def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
-    """Capacity on a closed-loop burst, then a Poisson stream at 80% of it."""
+def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs, tune_resident=24):
+    """Capacity on a closed-loop burst, then a Poisson stream at 80% of it.
+
+    With ``tune_resident`` > 0 the served arena's kernel shape and workgroups per CU
+    come from the autotuner run on that many of the jobs: a served arena holds a few
+    dozen jobs at a time, a different regime from the 4096-scene arena ``arena_kw``
+    was tuned for (there, one workgroup per tile wins; here a persistent grid does).
+    """
+    from flatworld_amd import autotune
     from flatworld_amd import workloads as W
     from flatworld_amd.server import Job, SimServer, poisson_arrivals
 
@@ -134,6 +141,14 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
     t = time.perf_counter()
     specs = [W.serving_job(rng) for _ in range(n_jobs)]  # the requests' payloads, built before the clock starts
     log(f"[bench] serving: {n_jobs} job payloads built in {time.perf_counter() - t:.1f}s")
+    tuning = {k: arena_kw[k] for k in ("tile_nodes", "tile_slots", "block", "grid_per_cu")}
+    tuning["source"] = "the batched arena's tuning"
+    if tune_resident > 0:
+        res = autotune.tune(specs[:tune_resident], frames=[frame], log=log)
+        tuning = {k: res[k] for k in ("tile_nodes", "tile_slots", "block", "grid_per_cu")}
+        tuning["source"] = f"autotuner on {min(tune_resident, n_jobs)} served jobs" + (" (cached)" if res.get("cached") else "")
+        arena_kw = autotune.arena_kwargs(dict(res, frame=frame))
+    log(f"[bench] serving: tuning {tuning}")
 
     nb = min(burst_jobs, n_jobs)
     srv = SimServer(frame=frame, keep_results=False, **arena_kw)
@@ -167,6 +182,7 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs):
         "burst_elem_substeps_per_s": burst["elem_substeps_per_s"],
         "gpu_busy_frac": st["gpu_busy_frac"],
         "frame_substeps": frame,
+        "tuning": tuning,
         "workload": "Poisson stream: grid side U{16..256}, Elastic/Neo-Hookean/J2, plane/height field/voxel ground, "
                     "horizon U{60..600}; latency = arrival -> final positions on the host",
     }
