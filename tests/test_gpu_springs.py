@@ -332,3 +332,23 @@ def test_spring_4096_arena_sampled_scenes_bitexact():
             assert np.isfinite(ref).all()
             assert np.array_equal(arena.read(ids[k], fld), ref), f"lattice {k} field {fld}"
     arena.close()
+
+
+def test_irregular_spring_network_bitexact():
+    """A random network: edges of a jittered triangle mesh, a random subset dropped, plus
+    duplicate springs and isolated nodes, so node degrees run from 0 to about 8 and the
+    incidence ranks and halos are irregular; several tiles, plane contact."""
+    rng = np.random.default_rng(77)
+    conn_t, X = fw.structured_grid(40, lower=(0.2, 0.03), extent=0.6, jitter=0.3, seed=77)
+    e = np.concatenate([conn_t[:, [0, 1]], conn_t[:, [1, 2]], conn_t[:, [2, 0]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    e = e[rng.random(e.shape[0]) < 0.8]
+    e = np.concatenate([e, e[rng.choice(e.shape[0], 50, replace=False)]])  # duplicate springs
+    e = e[rng.permutation(e.shape[0])].astype(np.int32)
+    flip = rng.random(e.shape[0]) < 0.5  # either orientation
+    e[flip] = e[flip][:, ::-1]
+    X = np.concatenate([X, rng.uniform(0.2, 0.8, (5, 2)).astype(np.float32)])  # five isolated nodes
+    dom = fw.SpringMassDomain(2, X, np.ascontiguousarray(e), [150.0, 0.2, 0.01], [fw.Gravity([0.0, -9.8])])
+    spec = W.one_body_spec(dom, fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0]))
+    assert np.bincount(e.reshape(-1), minlength=X.shape[0]).min() == 0
+    _run(spec, [1, 100, 200], "irregular spring network", tile_nodes=128, tile_slots=1024)
