@@ -26,6 +26,8 @@ FW_STEP_PER_SCENE = 4
 FIELD_X, FIELD_V, FIELD_U, FIELD_FORCE, FIELD_MASS = 1, 2, 3, 4, 5
 FIELD_STRESS, FIELD_STRAIN, FIELD_EPS_P, FIELD_EQPS = 6, 7, 8, 9
 FIELD_AREA, FIELD_JINV, FIELD_YIELD, FIELD_CONTACT, FIELD_AABB = 10, 11, 12, 13, 14
+FLEX_CACHE, FLEX_GRID, FLEX_CELL_START, FLEX_CELL_END, FLEX_SORTED = 1, 2, 3, 4, 5
+GROUND_FLEX = 4
 
 # (per_node, width, dtype)
 FIELD_SHAPES = {
@@ -68,6 +70,7 @@ class GroundDesc(ctypes.Structure):
         ("vox_lb", ctypes.c_float * 2),
         ("vox_dx", ctypes.c_float),
         ("vox_dz", ctypes.c_float),
+        ("flex_cap", ctypes.c_int32),
     ]
 
 
@@ -95,6 +98,31 @@ class SceneDesc(ctypes.Structure):
         ("grounds", ctypes.POINTER(GroundDesc)),
         ("rest_len", _f32p),
         ("spring_c", ctypes.c_float),
+    ]
+
+
+class FlexDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_bodies", ctypes.c_int32),
+        ("scene_ids", _i32p),
+        ("body_seg_off", _i32p),
+        ("seg_node", _i32p),
+        ("n_segs", ctypes.c_int32),
+        ("n_items", ctypes.c_int32),
+        ("item_body", _i32p),
+        ("item_node", _i32p),
+        ("item_target", _i32p),
+        ("item_pair", _i32p),
+        ("item_penalty", _f32p),
+        ("item_friction", _f32p),
+        ("n_pairs", ctypes.c_int32),
+        ("pair_body", _i32p),
+        ("body_bnodes_off", _i32p),
+        ("body_bnodes", _i32p),
+        ("gated", ctypes.c_int32),
+        ("max_cells", ctypes.c_int32),
+        ("max_query", ctypes.c_int32),
+        ("max_pairs", ctypes.c_int32),
     ]
 
 
@@ -149,6 +177,10 @@ SIGNATURES = {
     "fw_scene_add_prepared": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "fw_prepared_free": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_scene_mass_scaling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float]),
+    "fw_flex_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(FlexDesc), ctypes.POINTER(ctypes.c_int32)]),
+    "fw_flex_rebuild": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_flex_remove": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_flex_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
 }
 
 _lib = None
@@ -233,6 +265,8 @@ def scene_desc(spec, track_disp=True):
             d.vox_lb[:] = [float(v) for v in g.vox_lb]
             d.vox_dx = float(g.vox_dx)
             d.vox_dz = float(g.vox_dz)
+        elif g.kind == GROUND_FLEX:
+            d.flex_cap = int(g.flex_cap)
         gs.append(d)
     garr = (GroundDesc * max(len(gs), 1))(*gs)
     keep.append(garr)
@@ -325,6 +359,7 @@ class Arena:
         self.h = h
         self.tuning = dict(tile_nodes=tile_nodes, tile_slots=tile_slots, block=block, grid_per_cu=grid_per_cu)
         self._shapes = {}
+        self._flex = {}
 
     def close(self):
         if self.h:
@@ -408,6 +443,57 @@ class Arena:
     def write(self, sid, field, arr):
         a = np.ascontiguousarray(arr, dtype=np.float32)
         check(self._L.fw_scene_write(self.h, int(sid), int(field), a.ctypes.data, a.nbytes), "fw_scene_write")
+
+    def flex_add(self, group, scene_ids):
+        """Register a FEM-FEM contact group (a ``flexcontact.FlexGroupSpec``) over the given scenes."""
+        g = group
+        arrs = dict(
+            scene_ids=np.ascontiguousarray(scene_ids, dtype=np.int32),
+            body_seg_off=np.ascontiguousarray(g.body_seg_off, dtype=np.int32),
+            seg_node=np.ascontiguousarray(g.seg_node, dtype=np.int32).reshape(-1),
+            item_body=np.ascontiguousarray(g.item_body, dtype=np.int32),
+            item_node=np.ascontiguousarray(g.item_node, dtype=np.int32),
+            item_target=np.ascontiguousarray(g.item_target, dtype=np.int32),
+            item_pair=np.ascontiguousarray(g.item_pair, dtype=np.int32),
+            item_penalty=np.ascontiguousarray(g.item_penalty, dtype=np.float32),
+            item_friction=np.ascontiguousarray(g.item_friction, dtype=np.float32),
+            pair_body=np.ascontiguousarray(g.pair_body, dtype=np.int32).reshape(-1),
+            body_bnodes_off=np.ascontiguousarray(g.body_bnodes_off, dtype=np.int32),
+            body_bnodes=np.ascontiguousarray(g.body_bnodes, dtype=np.int32),
+        )
+        d = FlexDesc()
+        d.n_bodies = int(len(scene_ids))
+        d.n_segs = int(g.n_segs)
+        d.n_items = int(g.n_items)
+        d.n_pairs = int(g.n_pairs)
+        for k, a in arrs.items():
+            setattr(d, k, _ptr(a, _f32p if a.dtype == np.float32 else _i32p))
+        d.gated = 1 if g.gated else 0
+        d.max_cells, d.max_query, d.max_pairs = int(g.max_cells), int(g.max_query), int(g.max_pairs)
+        gid = ctypes.c_int32()
+        check(self._L.fw_flex_add(self.h, ctypes.byref(d), ctypes.byref(gid)), "fw_flex_add")
+        self._flex[gid.value] = g
+        return gid.value
+
+    def flex_rebuild(self, gid):
+        check(self._L.fw_flex_rebuild(self.h, int(gid)), "fw_flex_rebuild")
+
+    def flex_remove(self, gid):
+        check(self._L.fw_flex_remove(self.h, int(gid)), "fw_flex_remove")
+        self._flex.pop(gid, None)
+
+    def flex_read(self, gid, field):
+        g = self._flex[gid]
+        if field == FLEX_GRID:
+            raw = np.zeros(8, np.int32)
+            check(self._L.fw_flex_read(self.h, int(gid), field, raw.ctypes.data, raw.nbytes), "fw_flex_read")
+            return dict(glb=raw[:2].view(np.float32).copy(), gs=raw[2:4].view(np.float32).copy(),
+                        cn=raw[4:6].copy(), tc=int(raw[6]), exact=bool(raw[7]))
+        count = {FLEX_CACHE: g.n_items, FLEX_CELL_START: g.max_cells, FLEX_CELL_END: g.max_cells,
+                 FLEX_SORTED: g.max_pairs}[field]
+        out = np.zeros(count, np.int32)
+        check(self._L.fw_flex_read(self.h, int(gid), int(field), out.ctypes.data, out.nbytes), "fw_flex_read")
+        return out
 
     def time_steps(self, n, mode=0):
         tot = ctypes.c_double()

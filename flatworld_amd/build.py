@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libflatworld_fem.so")
-SOURCES = ["fw_kernels.hip", "fw_arena.hip"]
+SOURCES = ["fw_kernels.hip", "fw_flex.hip", "fw_arena.hip"]
 ARCH = os.environ.get("FW_OFFLOAD_ARCH", "gfx950")
 
 

@@ -41,6 +41,11 @@ extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
 extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
                                              float dt_target, int block, int slots, hipStream_t stream);
+extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, const SceneDev* scenes, const ArenaCtl* ctl, int gid,
+                                          hipStream_t stream);
+extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const SceneDev* scenes,
+                                             const ArenaCtl* ctl, int k, uint32_t flags, hipStream_t stream);
+extern "C" int fw_flex_max_pairs_per_group(void);
 
 namespace {
 
@@ -442,6 +447,10 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
         if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
     }
     L.host_bytes = o;
+    // FEM-FEM contributions: int2 (first, count) per node, then the float2 entries;
+    // cleared on the device, filled by the contact group before each substep
+    for (int g = 0; g < d->n_grounds; ++g)
+        if (d->grounds[g].kind == FW_GROUND_FLEX) L.gdata[g] = take(8 * N + 8 * (int64_t)d->grounds[g].flex_cap);
     // arrays that start at zero (cleared on the device) or that the device writes
     // before reading: the second position and halo buffers (the warm-up substep
     // writes them), mass / J^-1 / area (prepare kernel), U, J2 state, diagnostics
@@ -490,8 +499,14 @@ int validate(const fw_scene_desc* d) {
             return fail(FW_E_INVALID, "voxel grid needs nx, ny >= 1 and an occupancy array");
         if (gd.kind == FW_GROUND_VOXEL && (gd.vox_nx > 65534 || gd.vox_ny > 65534))
             return fail(FW_E_INVALID, "voxel grid is limited to 65534 cells per axis");
-        if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_VOXEL)
+        if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_FLEX)
             return fail(FW_E_INVALID, "unknown ground kind " + std::to_string(gd.kind));
+        if (gd.kind == FW_GROUND_FLEX) {
+            if (d->mat_type == FW_MAT_SPRING) return fail(FW_E_UNSUPPORTED, "FEM-FEM contact records need a FEM scene");
+            if (gd.flex_cap < 0) return fail(FW_E_INVALID, "FEM-FEM record needs flex_cap >= 0");
+            for (int h = 0; h < g; ++h)
+                if (d->grounds[h].kind == FW_GROUND_FLEX) return fail(FW_E_INVALID, "at most one FEM-FEM record per scene");
+        }
     }
     return FW_OK;
 }
@@ -538,8 +553,17 @@ struct Staging {
 
 }  // namespace
 
+// A FEM-FEM contact group: its device block (arena memory) and table entry.
+struct FlexHost {
+    int64_t block_off = 0, block_bytes = 0;
+    std::vector<int32_t> scenes;
+    int32_t n_items = 0, max_cells = 0, max_pairs = 0;
+    FlexDev dev{};
+};
+
 struct SceneHost {
     bool live = false;
+    int flex_group = -1;  // contact group holding this scene, -1 for none
     int64_t block_off = 0, block_bytes = 0;
     const TileDev* tiles_dev = nullptr;  // the scene's own copy of its tile records (in its block)
     int32_t n_nodes = 0, n_elems = 0, mat_type = 0;
@@ -590,6 +614,13 @@ struct fw_arena {
     uint32_t graph_flags = 0;
     int graph_grid = 0;
     int graph_kinds = 0;  // bit 0: triangle launch captured, bit 1: spring launch captured
+    // FEM-FEM contact groups (id -> group, nullptr once removed) and the device
+    // table of the live ones, in id order; flex_row[id] is a group's table row
+    std::vector<std::unique_ptr<FlexHost>> flex;
+    std::vector<int> flex_row;
+    FlexDev* d_flex = nullptr;
+    int flex_cap = 0, n_flex_live = 0;
+    bool flex_dirty = false;
     // one-entry tiling cache: fw_scene_footprint followed by fw_scene_add (or a retry of
     // an add that found the arena full) reuses the tiling instead of rebuilding it
     struct {
@@ -742,6 +773,33 @@ int ensure_tiles(fw_arena* a) {
     rc = upload(a, &a->d_ctl->n_tiles, &ctl.n_tiles, 2 * sizeof(int32_t));  // n_tiles, n_tri_tiles
     if (rc) return rc;
     a->tiles_dirty = false;
+    return FW_OK;
+}
+
+// Device table of the live FEM-FEM groups, in group-id order.
+int ensure_flex(fw_arena* a) {
+    if (!a->flex_dirty) return FW_OK;
+    std::vector<FlexDev> rows;
+    a->flex_row.assign(a->flex.size(), -1);
+    for (size_t g = 0; g < a->flex.size(); ++g) {
+        if (!a->flex[g]) continue;
+        a->flex_row[g] = (int)rows.size();
+        rows.push_back(a->flex[g]->dev);
+    }
+    const int n = (int)rows.size();
+    if (n > a->flex_cap) {
+        const int cap = std::max(n, std::max(8, 2 * a->flex_cap));
+        FlexDev* nd = nullptr;
+        HIPCHK(hipStreamSynchronize(a->stream));
+        HIPCHK(hipMalloc(&nd, sizeof(FlexDev) * (size_t)cap));
+        if (a->d_flex) HIPCHK(hipFree(a->d_flex));
+        a->d_flex = nd;
+        a->flex_cap = cap;
+    }
+    int rc = upload(a, a->d_flex, rows.data(), (int64_t)sizeof(FlexDev) * n);
+    if (rc) return rc;
+    a->n_flex_live = n;
+    a->flex_dirty = false;
     return FW_OK;
 }
 
@@ -913,6 +971,9 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             G.reverse = gd.reverse;
             std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
             build_hf_normals(G, gd.heights, (float2*)(img + L.gdata[g] + hf_normals_offset(gd.hf_nx)));
+        } else if (gd.kind == FW_GROUND_FLEX) {
+            G.nx = N;             // the entries follow the per-node ranges
+            G.ny = gd.flex_cap;
         } else {
             G.a[0] = gd.vox_lb[0];
             G.a[1] = gd.vox_lb[1];
@@ -1107,6 +1168,7 @@ int32_t fw_arena_destroy(fw_arena* a) {
     (void)hipFree(a->d_scenes);
     (void)hipFree(a->d_tiles);
     (void)hipFree(a->d_ctl);
+    (void)hipFree(a->d_flex);
     (void)hipStreamDestroy(a->stream);
     delete a;
     return FW_OK;
@@ -1198,6 +1260,9 @@ int32_t fw_scene_remove(fw_arena* a, int32_t id) {
     if (!a) return fail(FW_E_INVALID, "null arena");
     SceneHost* s = get_scene(a, id);
     if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    if (s->flex_group >= 0)
+        return fail(FW_E_INVALID, "scene " + std::to_string(id) + " belongs to FEM-FEM group " +
+                                      std::to_string(s->flex_group) + "; remove the group first");
     // No synchronisation: launches already in flight may still read this scene's
     // table entry and arrays, but every later write to them -- the live=0 flag
     // below, a reused block's upload, a reused id's entry -- is a copy on the
@@ -1230,6 +1295,10 @@ int32_t fw_scene_progress(fw_arena* a, int32_t id, int64_t* steps_done, int32_t*
 // as the control block counts them at run time.
 static hipError_t launch_all(fw_arena* a, int k, float dt_override, float damping_override, uint32_t f, int grid_tri,
                       int grid_spr) {
+    if (a->n_flex_live > 0) {  // FEM-FEM contact lists before the substep that adds them
+        hipError_t e = fw_launch_flex_contact(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, k, f, a->stream);
+        if (e != hipSuccess) return e;
+    }
     if (grid_tri > 0) {
         hipError_t e = fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
                                          grid_tri, a->tune.block, a->tune.tile_slots, 0, a->stream);
@@ -1244,6 +1313,10 @@ static hipError_t launch_all(fw_arena* a, int k, float dt_override, float dampin
 static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
     int rc = ensure_tiles(a);
     if (rc) return rc;
+    rc = ensure_flex(a);
+    if (rc) return rc;
+    if ((flags & FW_STEP_PER_SCENE) && a->n_flex_live > 0)
+        return fail(FW_E_UNSUPPORTED, "FW_STEP_PER_SCENE with FEM-FEM contact groups resident");
     const int nt = (int)a->tile_table.size();
     const int n_tri = a->n_tri_tiles, n_spr = nt - n_tri;
     for (int k = 0; k < n; ++k) {
@@ -1392,6 +1465,11 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
     HIPCHK(hipSetDevice(a->device));
     int rc = ensure_tiles(a);
     if (rc) return rc;
+    rc = ensure_flex(a);
+    if (rc) return rc;
+    if (a->n_flex_live > 0)
+        return fail(FW_E_UNSUPPORTED, "FEM-FEM contact groups step through fw_step (their hash rebuilds follow the "
+                                      "caller's schedule)");
     const int nt = (int)a->tile_table.size();
     const int grid = launch_grid(a, std::max(nt, 1));
     // the element kinds present decide which launches a frame holds; the graph's grids
@@ -1574,6 +1652,8 @@ int32_t fw_time_steps(fw_arena* a, int32_t n, int32_t mode, double* total_ms, do
     hipEvent_t t0, t1;
     HIPCHK(hipEventCreate(&t0));
     HIPCHK(hipEventCreate(&t1));
+    rc = ensure_flex(a);
+    if (rc) return rc;
     double ksum = 0.0;
     if (mode == 0) {
         // per-launch events bracket each substep kernel on the launch stream
@@ -1614,6 +1694,280 @@ int32_t fw_time_steps(fw_arena* a, int32_t n, int32_t mode, double* total_ms, do
     if (mode != 0) *kernel_ms = tot / n;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
+    return FW_OK;
+}
+
+int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
+    if (!a || !d || !group_id) return fail(FW_E_INVALID, "bad arguments");
+    const int B = d->n_bodies, S = d->n_segs, W = d->n_items, P = d->n_pairs;
+    if (B < 2 || S < 1 || W < 0 || P < 1) return fail(FW_E_INVALID, "a group needs two bodies, segments and pairs");
+    if (!d->scene_ids || !d->body_seg_off || !d->seg_node || !d->pair_body || !d->body_bnodes_off || !d->body_bnodes ||
+        (W > 0 && (!d->item_body || !d->item_node || !d->item_target || !d->item_pair || !d->item_penalty ||
+                   !d->item_friction)))
+        return fail(FW_E_INVALID, "group arrays are required");
+    if (P > fw_flex_max_pairs_per_group())
+        return fail(FW_E_UNSUPPORTED, "a group holds at most " + std::to_string(fw_flex_max_pairs_per_group()) + " pairs");
+    if (d->max_cells < 1 || d->max_query < 1 || d->max_pairs < 1) return fail(FW_E_INVALID, "hash limits must be >= 1");
+    HIPCHK(hipSetDevice(a->device));
+
+    std::vector<SceneHost*> sc(B);
+    for (int b = 0; b < B; ++b) {
+        sc[b] = get_scene(a, d->scene_ids[b]);
+        if (!sc[b]) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(d->scene_ids[b]));
+        if (sc[b]->mat_type == FW_MAT_SPRING) return fail(FW_E_INVALID, "FEM-FEM groups hold FEM scenes only");
+        if (sc[b]->flex_group >= 0) return fail(FW_E_INVALID, "scene already belongs to a FEM-FEM group");
+        if (sc[b]->s0 != sc[0]->s0) return fail(FW_E_INVALID, "a group's scenes must be added before the same substep");
+        for (int c = 0; c < b; ++c)
+            if (d->scene_ids[c] == d->scene_ids[b]) return fail(FW_E_INVALID, "a scene appears twice in the group");
+    }
+    if (d->body_seg_off[0] != 0 || d->body_seg_off[B] != S || d->body_bnodes_off[0] != 0)
+        return fail(FW_E_INVALID, "segment / boundary-node offsets must start at 0 and end at the counts");
+    std::vector<int32_t> seg_body(S);
+    for (int b = 0; b < B; ++b) {
+        if (d->body_seg_off[b + 1] < d->body_seg_off[b] || d->body_bnodes_off[b + 1] < d->body_bnodes_off[b])
+            return fail(FW_E_INVALID, "offsets must be non-decreasing");
+        for (int k = d->body_seg_off[b]; k < d->body_seg_off[b + 1]; ++k) seg_body[k] = b;
+    }
+    std::vector<char> in_pair(B, 0);
+    for (int q = 0; q < P; ++q) {
+        const int pa = d->pair_body[2 * q], pb = d->pair_body[2 * q + 1];
+        if (pa < 0 || pa >= B || pb < 0 || pb >= B || pa == pb) return fail(FW_E_INVALID, "bad pair bodies");
+        in_pair[pa] = in_pair[pb] = 1;
+    }
+    // slots: the boundary nodes of the bodies in some pair
+    std::vector<std::vector<int32_t>> slot_of(B);
+    std::vector<int32_t> body_slot_off(B + 1, 0), slot_node, slot_body;
+    for (int b = 0; b < B; ++b) {
+        body_slot_off[b] = (int32_t)slot_node.size();
+        if (!in_pair[b]) continue;
+        const int N = sc[b]->n_nodes;
+        slot_of[b].assign(N, -1);
+        for (int k = d->body_bnodes_off[b]; k < d->body_bnodes_off[b + 1]; ++k) {
+            const int o = d->body_bnodes[k];
+            if (o < 0 || o >= N) return fail(FW_E_INVALID, "boundary node outside its body");
+            if (slot_of[b][o] >= 0) continue;
+            slot_of[b][o] = (int32_t)slot_node.size();
+            slot_node.push_back(sc[b]->tiling.node_inv[o]);
+            slot_body.push_back(b);
+        }
+    }
+    body_slot_off[B] = (int32_t)slot_node.size();
+    const int NS = (int)slot_node.size();
+    std::vector<int2> seg_node(S), seg_slot(S);
+    for (int k = 0; k < S; ++k) {
+        const int b = seg_body[k], N = sc[b]->n_nodes;
+        const int o0 = d->seg_node[2 * k], o1 = d->seg_node[2 * k + 1];
+        if (o0 < 0 || o0 >= N || o1 < 0 || o1 >= N || o0 == o1) return fail(FW_E_INVALID, "bad segment nodes");
+        seg_node[k] = make_int2(sc[b]->tiling.node_inv[o0], sc[b]->tiling.node_inv[o1]);
+        seg_slot[k] = make_int2(-1, -1);
+        if (in_pair[b]) {
+            seg_slot[k] = make_int2(slot_of[b][o0], slot_of[b][o1]);
+            if (seg_slot[k].x < 0 || seg_slot[k].y < 0) return fail(FW_E_INVALID, "segment node is not a boundary node");
+        }
+    }
+    std::vector<int32_t> item_node(W), item_slot(W), need(B, 0);
+    for (int w = 0; w < W; ++w) {
+        const int b = d->item_body[w], t = d->item_target[w], q = d->item_pair[w];
+        if (b < 0 || b >= B || t < 0 || t >= B || b == t || q < 0 || q >= P || !in_pair[b] || !in_pair[t])
+            return fail(FW_E_INVALID, "bad work item " + std::to_string(w));
+        const int o = d->item_node[w];
+        if (o < 0 || o >= sc[b]->n_nodes || slot_of[b][o] < 0)
+            return fail(FW_E_INVALID, "work item node is not a boundary node");
+        item_node[w] = sc[b]->tiling.node_inv[o];
+        item_slot[w] = slot_of[b][o];
+        need[b] += 1;
+        need[t] += 2;
+    }
+    std::vector<char*> body_flex(B, nullptr);
+    std::vector<int32_t> body_nn(B);
+    for (int b = 0; b < B; ++b) {
+        body_nn[b] = sc[b]->n_nodes;
+        if (!in_pair[b]) continue;
+        for (int g = 0; g < sc[b]->dev.hot.n_grounds; ++g) {
+            const GroundDev& G = sc[b]->dev.ground[g];
+            if (G.kind != FW_GROUND_FLEX_DEV) continue;
+            if (G.ny < need[b])
+                return fail(FW_E_INVALID, "scene " + std::to_string(d->scene_ids[b]) + " FEM-FEM record holds " +
+                                              std::to_string(G.ny) + " contributions, the group needs " +
+                                              std::to_string(need[b]));
+            body_flex[b] = (char*)G.data;
+        }
+        if (!body_flex[b])
+            return fail(FW_E_INVALID, "scene " + std::to_string(d->scene_ids[b]) + " has no FEM-FEM ground record");
+    }
+
+    // one block: tables (uploaded), then hash and scratch (cleared)
+    int64_t o = 0;
+    auto take = [&](int64_t bytes) {
+        int64_t r = o;
+        o += align_up(std::max<int64_t>(bytes, 4));
+        return r;
+    };
+    const int64_t o_scene = take(4 * B), o_soff = take(4 * (B + 1)), o_sloff = take(4 * (B + 1)), o_flex = take(8 * B),
+                  o_nn = take(4 * B), o_snode = take(4 * NS), o_sbody = take(4 * NS), o_segb = take(4 * S),
+                  o_segn = take(8 * S), o_segs = take(8 * S), o_ib = take(4 * W), o_in = take(4 * W),
+                  o_is = take(4 * W), o_it = take(4 * W), o_ip = take(4 * W), o_ipen = take(4 * W),
+                  o_ifr = take(4 * W), o_cache = take(4 * W), o_pair = take(8 * P);
+    const int64_t host_bytes = o;
+    const int64_t o_grid = take(sizeof(FlexGrid)), o_box = take(16 * S), o_segp = take(4 * S),
+                  o_cs = take(4 * (int64_t)d->max_cells), o_ce = take(4 * (int64_t)d->max_cells),
+                  o_sorted = take(4 * (int64_t)d->max_pairs), o_ri = take(8 * W), o_rf = take(16 * W),
+                  o_slst = take(4 * NS), o_slcur = take(4 * NS), o_ek = take(12 * (int64_t)W), o_ef = take(24 * (int64_t)W);
+    const int64_t total = o;
+    const int64_t off = arena_alloc(a, total);
+    if (off < 0) return fail(FW_E_CAPACITY, "arena full: FEM-FEM group needs " + std::to_string(total) + " bytes");
+    char* blk = a->base + off;
+    std::vector<char> img((size_t)host_bytes, 0);
+    auto put = [&](int64_t at, const void* src, int64_t bytes) {
+        if (bytes > 0) std::memcpy(img.data() + at, src, (size_t)bytes);
+    };
+    std::vector<char*> flex_dev(B);
+    for (int b = 0; b < B; ++b) flex_dev[b] = body_flex[b];
+    put(o_scene, d->scene_ids, 4 * B);
+    put(o_soff, d->body_seg_off, 4 * (B + 1));
+    put(o_sloff, body_slot_off.data(), 4 * (B + 1));
+    put(o_flex, flex_dev.data(), 8 * B);
+    put(o_nn, body_nn.data(), 4 * B);
+    put(o_snode, slot_node.data(), 4 * NS);
+    put(o_sbody, slot_body.data(), 4 * NS);
+    put(o_segb, seg_body.data(), 4 * S);
+    put(o_segn, seg_node.data(), 8 * S);
+    put(o_segs, seg_slot.data(), 8 * S);
+    put(o_ib, d->item_body, 4 * W);
+    put(o_in, item_node.data(), 4 * W);
+    put(o_is, item_slot.data(), 4 * W);
+    put(o_it, d->item_target, 4 * W);
+    put(o_ip, d->item_pair, 4 * W);
+    put(o_ipen, d->item_penalty, 4 * W);
+    put(o_ifr, d->item_friction, 4 * W);
+    {
+        std::vector<int32_t> minus(W, -1);
+        put(o_cache, minus.data(), 4 * W);
+    }
+    put(o_pair, d->pair_body, 8 * P);
+    int rc = upload(a, blk, img.data(), host_bytes);
+    if (rc) {
+        arena_free(a, off, total);
+        return rc;
+    }
+    HIPCHK(hipMemsetAsync(blk + host_bytes, 0, (size_t)(total - host_bytes), a->stream));
+
+    auto F = std::make_unique<FlexHost>();
+    F->block_off = off;
+    F->block_bytes = total;
+    F->scenes.assign(d->scene_ids, d->scene_ids + B);
+    F->n_items = W;
+    F->max_cells = d->max_cells;
+    F->max_pairs = d->max_pairs;
+    FlexDev& D = F->dev;
+    D.n_bodies = B;
+    D.n_segs = S;
+    D.n_items = W;
+    D.n_pairs = P;
+    D.n_slots = NS;
+    D.gated = d->gated ? 1 : 0;
+    D.max_cells = d->max_cells;
+    D.max_query = d->max_query;
+    D.max_pairs = d->max_pairs;
+    D.body_scene = (const int32_t*)(blk + o_scene);
+    D.body_seg_off = (const int32_t*)(blk + o_soff);
+    D.body_slot_off = (const int32_t*)(blk + o_sloff);
+    D.body_flex = (char* const*)(blk + o_flex);
+    D.body_nn = (const int32_t*)(blk + o_nn);
+    D.slot_node = (const int32_t*)(blk + o_snode);
+    D.slot_body = (const int32_t*)(blk + o_sbody);
+    D.seg_body = (const int32_t*)(blk + o_segb);
+    D.seg_node = (const int2*)(blk + o_segn);
+    D.seg_slot = (const int2*)(blk + o_segs);
+    D.item_body = (const int32_t*)(blk + o_ib);
+    D.item_node = (const int32_t*)(blk + o_in);
+    D.item_slot = (const int32_t*)(blk + o_is);
+    D.item_target = (const int32_t*)(blk + o_it);
+    D.item_pair = (const int32_t*)(blk + o_ip);
+    D.item_penalty = (const float*)(blk + o_ipen);
+    D.item_friction = (const float*)(blk + o_ifr);
+    D.cache = (int32_t*)(blk + o_cache);
+    D.pair_body = (const int2*)(blk + o_pair);
+    D.grid = (FlexGrid*)(blk + o_grid);
+    D.seg_box = (float4*)(blk + o_box);
+    D.seg_pairs = (int32_t*)(blk + o_segp);
+    D.cell_start = (int32_t*)(blk + o_cs);
+    D.cell_end = (int32_t*)(blk + o_ce);
+    D.sorted = (int32_t*)(blk + o_sorted);
+    D.res_i = (int2*)(blk + o_ri);
+    D.res_f = (float4*)(blk + o_rf);
+    D.slot_start = (int32_t*)(blk + o_slst);
+    D.slot_cur = (int32_t*)(blk + o_slcur);
+    D.ent_key = (int32_t*)(blk + o_ek);
+    D.ent_f = (float2*)(blk + o_ef);
+
+    int id = (int)a->flex.size();
+    for (int g = 0; g < (int)a->flex.size(); ++g)
+        if (!a->flex[g]) {
+            id = g;
+            break;
+        }
+    for (int b = 0; b < B; ++b) sc[b]->flex_group = id;
+    if (id == (int)a->flex.size())
+        a->flex.push_back(std::move(F));
+    else
+        a->flex[id] = std::move(F);
+    a->flex_dirty = true;
+    *group_id = id;
+    return FW_OK;
+}
+
+int32_t fw_flex_rebuild(fw_arena* a, int32_t gid) {
+    if (!a || gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]) return fail(FW_E_NOT_FOUND, "no FEM-FEM group");
+    HIPCHK(hipSetDevice(a->device));
+    int rc = ensure_tiles(a);
+    if (rc) return rc;
+    rc = ensure_flex(a);
+    if (rc) return rc;
+    const SceneHost* s0 = get_scene(a, a->flex[gid]->scenes[0]);
+    if (!s0 || a->clock < s0->s0)
+        return fail(FW_E_INVALID, "the group's scenes have not run their warm-up substep yet");
+    HIPCHK(fw_launch_flex_hash(a->d_flex, a->d_scenes, a->d_ctl, a->flex_row[gid], a->stream));
+    return FW_OK;
+}
+
+int32_t fw_flex_remove(fw_arena* a, int32_t gid) {
+    if (!a || gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]) return fail(FW_E_NOT_FOUND, "no FEM-FEM group");
+    HIPCHK(hipSetDevice(a->device));
+    FlexHost& F = *a->flex[gid];
+    // later substeps must not add the last contributions again: clear the ranges
+    for (int32_t sid : F.scenes) {
+        SceneHost* s = get_scene(a, sid);
+        if (!s) continue;
+        s->flex_group = -1;
+        for (int g = 0; g < s->dev.hot.n_grounds; ++g)
+            if (s->dev.ground[g].kind == FW_GROUND_FLEX_DEV)
+                HIPCHK(hipMemsetAsync(const_cast<void*>(s->dev.ground[g].data), 0, 8 * (size_t)s->n_nodes, a->stream));
+    }
+    arena_free(a, F.block_off, F.block_bytes);
+    a->flex[gid].reset();
+    a->flex_dirty = true;
+    return FW_OK;
+}
+
+int32_t fw_flex_read(fw_arena* a, int32_t gid, int32_t field, void* out, int64_t out_bytes) {
+    if (!a || !out || gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid])
+        return fail(FW_E_NOT_FOUND, "no FEM-FEM group");
+    const FlexHost& F = *a->flex[gid];
+    const void* src = nullptr;
+    int64_t bytes = 0;
+    switch (field) {
+        case FW_FLEX_CACHE: src = F.dev.cache; bytes = 4 * (int64_t)F.n_items; break;
+        case FW_FLEX_GRID: src = F.dev.grid; bytes = sizeof(FlexGrid); break;
+        case FW_FLEX_CELL_START: src = F.dev.cell_start; bytes = 4 * (int64_t)F.max_cells; break;
+        case FW_FLEX_CELL_END: src = F.dev.cell_end; bytes = 4 * (int64_t)F.max_cells; break;
+        case FW_FLEX_SORTED: src = F.dev.sorted; bytes = 4 * (int64_t)F.max_pairs; break;
+        default: return fail(FW_E_INVALID, "unknown FEM-FEM field " + std::to_string(field));
+    }
+    if (out_bytes < bytes) return fail(FW_E_INVALID, "output buffer too small: need " + std::to_string(bytes));
+    HIPCHK(hipSetDevice(a->device));
+    HIPCHK(hipStreamSynchronize(a->stream));
+    HIPCHK(hipMemcpy(out, src, (size_t)bytes, hipMemcpyDeviceToHost));
     return FW_OK;
 }
 

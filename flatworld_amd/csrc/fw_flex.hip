@@ -1,0 +1,536 @@
+// fw_flex.hip -- FEM-FEM penalty contact between the FEM bodies of one loop:
+// the spatial hash over their boundary segments (flatworld/spatialmanager.py,
+// flatworld/femspringmanager.py:421-487, 1035-1073) and the batched contact pass
+// (flatworld/mixedcontact.py:886-1025).  One workgroup per contact group.
+//
+// The reference runs both with atomics (the hash scatter, the Fext updates) and
+// with one query buffer shared by every thread.  Here every result that order
+// decides equals the reference's kernels run serially in thread order: a hash
+// cell lists its segments in ascending order, every work item walks its own
+// query, and a node's contributions are listed in ascending work item.  The
+// substep kernel adds each node's list through the scene's FEM-FEM ground record,
+// which sits after the plane records and before the height-field and voxel
+// records, as the reference launches its contact kernels
+// (flatworld/mixedcontact.py:1654-1754).
+//
+// Phases hand data between threads of the workgroup through global memory, each
+// behind an agent-scope fence and a workgroup barrier.  Integer atomics only
+// count and reserve places; every order-sensitive result is then sorted.
+#include <hip/hip_runtime.h>
+
+#include "fw_types.h"
+
+namespace fw {
+namespace {
+
+constexpr int FLEX_BLOCK = 1024;
+constexpr int FLEX_MAX_PAIRS = 4096;  // pairs of one group (LDS activation flags)
+constexpr uint32_t FLEX_NO_CONTACT = 2u;  // FW_STEP_NO_CONTACT
+
+__device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
+__device__ __forceinline__ float len2(float x, float y) { return sqrtf(dot2(x, y, x, y)); }
+__device__ __forceinline__ float fmin_(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmax_(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ int imin_(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax_(int a, int b) { return a > b ? a : b; }
+__device__ __forceinline__ int mod3_(int k) {
+    const int r = k % 3;
+    return r < 0 ? r + 3 : r;
+}
+
+__device__ __forceinline__ void phase_barrier() {
+    __threadfence();
+    __syncthreads();
+}
+
+// pointToEdgeContact (flatworld/contact_detection.py:15-61); returns is_inside.
+__device__ __forceinline__ bool point_edge(float px, float py, float ax, float ay, float bx, float by, float& pen,
+                                           float& nx, float& ny, float& w0, float& w1) {
+    const float ex = bx - ax, ey = by - ay;
+    const float len = len2(ex, ey);
+    bool inside = false;
+    pen = 1e9f;
+    nx = ny = 0.0f;
+    w0 = w1 = 0.0f;
+    if (len > 1e-9f) {
+        const float dx = ex / len, dy = ey / len;
+        nx = dy;
+        ny = -dx;
+        const float tx = px - ax, ty = py - ay;
+        float t = dot2(tx, ty, dx, dy);
+        t = t / len;
+        if (0.0f <= t && t <= 1.0f) {
+            inside = true;
+            pen = dot2(tx, ty, nx, ny);
+            w0 = 1.0f - t;
+            w1 = t;
+        }
+    }
+    return inside;
+}
+
+// detectPointToMeshBoundary (flatworld/contact_detection.py:64-90): the
+// penetration when the point projects inside within `limit`, else 1e9.
+__device__ __forceinline__ float detect(float px, float py, float2 a, float2 b, float limit, float& nx, float& ny,
+                                        float& w0, float& w1) {
+    float pen, n0, n1;
+    const bool inside = point_edge(px, py, a.x, a.y, b.x, b.y, pen, n0, n1, w0, w1);
+    if (fabsf(pen) < limit && inside) {
+        nx = n0;
+        ny = n1;
+        return pen;
+    }
+    nx = ny = 0.0f;
+    return 1e9f;
+}
+
+struct CellRange {
+    int lx, ly, ux, uy;
+};
+
+// Cells of a box, clamped to the grid (flatworld/spatialmanager.py:160-175, 276-291).
+__device__ __forceinline__ CellRange cell_range(const FlexGrid& g, float lbx, float lby, float ubx, float uby) {
+    const float dlx = lbx - g.glb[0], dly = lby - g.glb[1];
+    const float dux = ubx - g.glb[0], duy = uby - g.glb[1];
+    CellRange r;
+    r.lx = imin_(imax_((int)floorf(dlx / g.gs[0]), 0), g.cn[0] - 1);
+    r.ly = imin_(imax_((int)floorf(dly / g.gs[1]), 0), g.cn[1] - 1);
+    r.ux = imin_(imax_((int)floorf(dux / g.gs[0]), 0), g.cn[0] - 1);
+    r.uy = imin_(imax_((int)floorf(duy / g.gs[1]), 0), g.cn[1] - 1);
+    return r;
+}
+
+// The boundary AABB a scene's gate reads at `local` (ordered-int encoded).
+__device__ __forceinline__ void scene_aabb(const SceneDev* S, int ar3, float bb[4]) {
+    const int32_t* e = S->hot.aabb[ar3];
+    for (int k = 0; k < 4; ++k) bb[k] = fw_o2f(e[k]);
+}
+
+// Workgroup exclusive scan of one value per thread; *total gets the sum.
+__device__ int block_exclusive_scan(int v, int* lds, int* total) {
+    const int tid = threadIdx.x;
+    lds[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < FLEX_BLOCK; off <<= 1) {
+        const int t = tid >= off ? lds[tid - off] : 0;
+        __syncthreads();
+        lds[tid] += t;
+        __syncthreads();
+    }
+    const int incl = lds[tid];
+    *total = lds[FLEX_BLOCK - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// In-place exclusive scan of a[0, n) (copy, when given, receives the same); returns the sum.
+__device__ int scan_global(int32_t* a, int32_t* copy, int n, int* lds) {
+    int carry = 0;
+    for (int base = 0; base < n; base += FLEX_BLOCK) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < n ? a[i] : 0;
+        int tot;
+        const int ex = block_exclusive_scan(v, lds, &tot);
+        if (i < n) {
+            a[i] = carry + ex;
+            if (copy) copy[i] = carry + ex;
+        }
+        carry += tot;
+    }
+    return carry;
+}
+
+// populate_fem_spatial_hash + SpatialHashManager.build at the state the next
+// substep starts from (velocity buffer 0).
+__global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* groups, const SceneDev* scenes,
+                                                               const ArenaCtl* ctl, int gid) {
+    const FlexDev& F = groups[gid];
+    __shared__ int lds[FLEX_BLOCK];
+    __shared__ float sh_bb[4];
+    __shared__ int sh_est;
+    __shared__ FlexGrid sh_g;
+    const int tid = threadIdx.x;
+    const SceneDev* S0 = scenes + F.body_scene[0];
+    const int local = ctl->clock - S0->s0;
+    const int rd = (local + 1) & 1;
+    const int ar3 = mod3_(local);
+    const int ns = F.n_segs;
+
+    // global bounds: union of the bodies' AABBs, widened by 1% of the larger extent
+    if (tid == 0) {
+        float l0 = 1e9f, l1 = 1e9f, u0 = -1e9f, u1 = -1e9f;
+        for (int b = 0; b < F.n_bodies; ++b) {
+            float bb[4];
+            scene_aabb(scenes + F.body_scene[b], ar3, bb);
+            l0 = fmin_(l0, bb[0]);
+            l1 = fmin_(l1, bb[1]);
+            u0 = fmax_(u0, bb[2]);
+            u1 = fmax_(u1, bb[3]);
+        }
+        const float extent = fmax_(u0 - l0, u1 - l1);
+        const float expand = extent * 0.01f + 0.0f;
+        sh_bb[0] = l0 - expand;
+        sh_bb[1] = l1 - expand;
+        sh_bb[2] = u0 + expand;
+        sh_bb[3] = u1 + expand;
+        sh_est = fw_f2o(1e30f);
+    }
+    __syncthreads();
+
+    // segment boxes (two nodes) and the smallest box diagonal
+    for (int s = tid; s < ns; s += FLEX_BLOCK) {
+        const SceneDev* Sb = scenes + F.body_scene[F.seg_body[s]];
+        const float2* pos = Sb->pos[rd];
+        const int2 sn = F.seg_node[s];
+        const float2 a = pos[sn.x], c = pos[sn.y];
+        float lb0 = 1e30f, lb1 = 1e30f, ub0 = -1e30f, ub1 = -1e30f;
+        lb0 = fmin_(lb0, a.x);
+        lb1 = fmin_(lb1, a.y);
+        ub0 = fmax_(ub0, a.x);
+        ub1 = fmax_(ub1, a.y);
+        lb0 = fmin_(lb0, c.x);
+        lb1 = fmin_(lb1, c.y);
+        ub0 = fmax_(ub0, c.x);
+        ub1 = fmax_(ub1, c.y);
+        const float4 box = make_float4(lb0 - 0.0f, lb1 - 0.0f, ub0 + 0.0f, ub1 + 0.0f);
+        F.seg_box[s] = box;
+        atomicMin(&sh_est, fw_f2o(len2(box.z - box.x, box.w - box.y)));
+    }
+    __syncthreads();
+
+    // _build_grid_kernel (flatworld/spatialmanager.py:217-245); pow(max_cells, 0.5)
+    // is the correctly rounded square root
+    if (tid == 0) {
+        FlexGrid g;
+        float cell = fw_o2f(sh_est) * 1.5f;
+        const float ex = sh_bb[2] - sh_bb[0], ey = sh_bb[3] - sh_bb[1];
+        const float mpd = fmax_(sqrtf((float)F.max_cells), 1.0f);
+        cell = fmax_(cell, ex / mpd);
+        cell = fmax_(cell, ey / mpd);
+        const float cs = cell + 1e-6f;
+        const int nx = (int)fmax_(ceilf(ex / cs), 1.0f);
+        const int ny = (int)fmax_(ceilf(ey / cs), 1.0f);
+        g.glb[0] = sh_bb[0];
+        g.glb[1] = sh_bb[1];
+        g.gs[0] = ex / (float)nx;
+        g.gs[1] = ey / (float)ny;
+        g.cn[0] = nx;
+        g.cn[1] = ny;
+        g.tc = imin_(nx * ny, F.max_cells);
+        g.exact = nx * ny <= F.max_cells;
+        sh_g = g;
+    }
+    __syncthreads();
+    const FlexGrid g = sh_g;
+    if (ns == 0) {
+        if (tid == 0) *F.grid = g;
+        return;
+    }
+
+    // pairs per segment, then their serial index base (pairs past max_pairs are
+    // dropped in that order)
+    for (int s = tid; s < ns; s += FLEX_BLOCK) {
+        const float4 box = F.seg_box[s];
+        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        int cnt = 0;
+        for (int ix = r.lx; ix <= r.ux; ++ix)
+            for (int iy = r.ly; iy <= r.uy; ++iy) cnt += ix + iy * g.cn[0] < g.tc;
+        F.seg_pairs[s] = cnt;
+    }
+    for (int c = tid; c < g.tc; c += FLEX_BLOCK) F.cell_end[c] = 0;
+    phase_barrier();
+    const int total = scan_global(F.seg_pairs, nullptr, ns, lds);
+    phase_barrier();
+    for (int s = tid; s < ns; s += FLEX_BLOCK) {
+        const float4 box = F.seg_box[s];
+        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        int pi = F.seg_pairs[s];
+        for (int ix = r.lx; ix <= r.ux; ++ix)
+            for (int iy = r.ly; iy <= r.uy; ++iy) {
+                const int cid = ix + iy * g.cn[0];
+                if (cid < g.tc) {
+                    if (pi < F.max_pairs) atomicAdd(&F.cell_end[cid], 1);
+                    ++pi;
+                }
+            }
+    }
+    phase_barrier();
+    scan_global(F.cell_end, F.cell_start, g.tc, lds);
+    phase_barrier();
+    for (int s = tid; s < ns; s += FLEX_BLOCK) {
+        const float4 box = F.seg_box[s];
+        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        int pi = F.seg_pairs[s];
+        for (int ix = r.lx; ix <= r.ux; ++ix)
+            for (int iy = r.ly; iy <= r.uy; ++iy) {
+                const int cid = ix + iy * g.cn[0];
+                if (cid < g.tc) {
+                    if (pi < F.max_pairs) F.sorted[atomicAdd(&F.cell_end[cid], 1)] = s;
+                    ++pi;
+                }
+            }
+    }
+    phase_barrier();
+    // a cell's entries in ascending segment: the serial scatter order
+    for (int c = tid; c < g.tc; c += FLEX_BLOCK) {
+        const int b = F.cell_start[c], e = F.cell_end[c];
+        for (int i = b + 1; i < e; ++i) {
+            const int v = F.sorted[i];
+            int j = i - 1;
+            while (j >= b && F.sorted[j] > v) {
+                F.sorted[j + 1] = F.sorted[j];
+                --j;
+            }
+            F.sorted[j + 1] = v;
+        }
+    }
+    if (tid == 0) {
+        FlexGrid out = g;
+        out.exact = g.exact && total <= F.max_pairs;
+        *F.grid = out;
+    }
+}
+
+// Whether the query walk met segment s in an earlier cell or earlier in this
+// cell.  With every segment in every cell of its box (grid.exact), that is
+// whether (ix, iy) is not the first cell of the walk inside the segment's box;
+// otherwise the walk is repeated up to this entry.
+__device__ __forceinline__ bool seen_before(const FlexDev& F, const FlexGrid& g, const CellRange& q, int s, int ix,
+                                            int iy, int pos) {
+    if (g.exact) {
+        const float4 box = F.seg_box[s];
+        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        return !(ix == imax_(r.lx, q.lx) && iy == imax_(r.ly, q.ly));
+    }
+    for (int jx = q.lx; jx <= q.ux; ++jx)
+        for (int jy = q.ly; jy <= q.uy; ++jy) {
+            const int cid = jx + jy * g.cn[0];
+            if (cid < 0 || cid >= g.tc) continue;
+            const int e = F.cell_end[cid];
+            for (int p = F.cell_start[cid]; p < e; ++p) {
+                if (jx == ix && jy == iy && p == pos) return false;
+                if (F.sorted[p] == s) return true;
+            }
+        }
+    return false;
+}
+
+// _batched_flexflex_contact_kernel_sh (flatworld/mixedcontact.py:886-1025) for
+// every work item of the group, then the node-major contribution lists the
+// substep kernel adds.
+__global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev* groups, const SceneDev* scenes,
+                                                                  const ArenaCtl* ctl, int k, uint32_t flags) {
+    const FlexDev& F = groups[blockIdx.x];
+    __shared__ int lds[FLEX_BLOCK];
+    __shared__ uint8_t act[FLEX_MAX_PAIRS];
+    const int tid = threadIdx.x;
+    const SceneDev* S0 = scenes + F.body_scene[0];
+    const int local = ctl->clock + k - S0->s0;
+    if (local < 0 || local >= S0->horizon || (flags & FLEX_NO_CONTACT)) return;
+    const int rd = (local + 1) & 1;
+    const int ar3 = mod3_(local);
+    const bool pad = ((local - S0->hot.gate_base) % 100) == 0;
+    const FlexGrid g = *F.grid;
+
+    // pair activation: the BVH overlap test (flatworld/explicitloop.py:477-487), or
+    // every pair when the loop skips its BVH
+    for (int q = tid; q < F.n_pairs; q += FLEX_BLOCK) {
+        bool on = true;
+        if (F.gated) {
+            const int2 pb = F.pair_body[q];
+            float a[4], b[4];
+            scene_aabb(scenes + F.body_scene[pb.x], ar3, a);
+            scene_aabb(scenes + F.body_scene[pb.y], ar3, b);
+            if (pad) {
+                for (int c = 0; c < 2; ++c) {
+                    const float ba = 0.1f * (a[c + 2] - a[c]);
+                    const float la = a[c] - ba, ua = a[c + 2] + ba;
+                    const float bb = 0.1f * (b[c + 2] - b[c]);
+                    const float lb = b[c] - bb, ub = b[c + 2] + bb;
+                    a[c] = la;
+                    a[c + 2] = ua;
+                    b[c] = lb;
+                    b[c + 2] = ub;
+                }
+            }
+            on = a[0] <= b[2] && a[2] >= b[0] && a[1] <= b[3] && a[3] >= b[1];
+        }
+        act[q] = on ? 1 : 0;
+    }
+    for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) F.slot_start[sl] = 0;
+    phase_barrier();
+
+    const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.016f);
+    for (int w = tid; w < F.n_items; w += FLEX_BLOCK) {
+        int2 ri = make_int2(0, -1);
+        if (act[F.item_pair[w]]) {
+            const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
+            const int nid = F.item_node[w];
+            const float2 p = SB->pos[rd][nid];
+            const float2 vel = SB->vel[nid];
+            const int tb = F.item_target[w];
+            const SceneDev* ST = scenes + F.body_scene[tb];
+            const float2* tpos = ST->pos[rd];
+            const int be_off = F.body_seg_off[tb], be_cnt = F.body_seg_off[tb + 1] - be_off;
+            const float penalty = F.item_penalty[w], friction = F.item_friction[w];
+
+            float best_pen = 1e9f, bnx = 0.0f, bny = 0.0f, bw0 = 0.0f, bw1 = 0.0f;
+            bool found = false;
+            int bs = -1;
+            const int cached = F.cache[w];
+            if (cached >= be_off && cached < be_off + be_cnt) {
+                const int2 sn = F.seg_node[cached];
+                const float2 a = tpos[sn.x], c = tpos[sn.y];
+                float lb0 = 1e30f, lb1 = 1e30f, ub0 = -1e30f, ub1 = -1e30f;
+                lb0 = fmin_(lb0, a.x);
+                lb1 = fmin_(lb1, a.y);
+                ub0 = fmax_(ub0, a.x);
+                ub1 = fmax_(ub1, a.y);
+                lb0 = fmin_(lb0, c.x);
+                lb1 = fmin_(lb1, c.y);
+                ub0 = fmax_(ub0, c.x);
+                ub1 = fmax_(ub1, c.y);
+                const float margin = fmax_(1e-4f, 1e-2f * len2(ub0 - lb0, ub1 - lb1));
+                lb0 = lb0 - margin;
+                lb1 = lb1 - margin;
+                ub0 = ub0 + margin;
+                ub1 = ub1 + margin;
+                // _point_outside_aabb (flatworld/mixedcontact.py:113-119)
+                const bool outside = fmin_(p.x - lb0, p.y - lb1) < 0.0f || fmin_(ub0 - p.x, ub1 - p.y) < 0.0f;
+                if (!outside) {
+                    float nx, ny, w0, w1;
+                    const float pen = detect(p.x, p.y, a, c, margin, nx, ny, w0, w1);
+                    if (pen < 1e-4f && fabsf(pen) < margin) {
+                        best_pen = pen;
+                        bnx = nx;
+                        bny = ny;
+                        bw0 = w0;
+                        bw1 = w1;
+                        bs = cached;
+                        found = true;
+                    }
+                }
+            }
+            if (!found && g.tc > 0) {
+                // query_point_with_buffer (flatworld/spatialmanager.py:131-199), walked in place
+                const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
+                int n = 0;
+                int cache_new = F.cache[w];
+                for (int ix = q.lx; ix <= q.ux; ++ix)
+                    for (int iy = q.ly; iy <= q.uy; ++iy) {
+                        const int cid = ix + iy * g.cn[0];
+                        if (cid < 0 || cid >= g.tc) continue;
+                        const int e = F.cell_end[cid];
+                        for (int pp = F.cell_start[cid]; pp < e; ++pp) {
+                            const int s = F.sorted[pp];
+                            if (s < 0 || F.seg_body[s] != tb) continue;
+                            if (n >= F.max_query || seen_before(F, g, q, s, ix, iy, pp)) continue;
+                            ++n;
+                            const int2 sn = F.seg_node[s];
+                            float nx, ny, w0, w1;
+                            const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
+                            if (pen < best_pen && pen < 1e-4f && fabsf(pen) < qbuf) {
+                                best_pen = pen;
+                                bnx = nx;
+                                bny = ny;
+                                bw0 = w0;
+                                bw1 = w1;
+                                bs = s;
+                                found = true;
+                                cache_new = s;
+                            }
+                        }
+                    }
+                F.cache[w] = cache_new;
+            }
+            if (!found) F.cache[w] = -1;
+            if (found && best_pen < 0.0f) {
+                const float nfx = ((-bnx) * penalty) * best_pen;
+                const float nfy = ((-bny) * penalty) * best_pen;
+                float tx = nfx, ty = nfy;
+                if (friction > 1e-9f) {
+                    const int2 sn = F.seg_node[bs];
+                    const float2 v0 = ST->vel[sn.x], v1 = ST->vel[sn.y];
+                    const float svx = v0.x * bw0 + v1.x * bw1;
+                    const float svy = v0.y * bw0 + v1.y * bw1;
+                    const float rvx = vel.x - svx, rvy = vel.y - svy;
+                    const float d = dot2(rvx, rvy, bnx, bny);
+                    const float tgx = rvx - d * bnx, tgy = rvy - d * bny;
+                    const float tlen = len2(tgx, tgy);
+                    if (tlen > 1e-9f) {
+                        const float fdx = (-tgx) / tlen, fdy = (-tgy) / tlen;
+                        const float nl = len2(nfx, nfy);
+                        tx = tx + (fdx * friction) * nl;
+                        ty = ty + (fdy * friction) * nl;
+                    }
+                }
+                ri = make_int2(1, bs);
+                F.res_f[w] = make_float4(tx, ty, bw0, bw1);
+                const int2 ss = F.seg_slot[bs];
+                atomicAdd(&F.slot_start[F.item_slot[w]], 1);
+                atomicAdd(&F.slot_start[ss.x], 1);
+                atomicAdd(&F.slot_start[ss.y], 1);
+            }
+        }
+        F.res_i[w] = ri;
+    }
+    phase_barrier();
+    scan_global(F.slot_start, F.slot_cur, F.n_slots, lds);
+    phase_barrier();
+    for (int w = tid; w < F.n_items; w += FLEX_BLOCK) {
+        const int2 ri = F.res_i[w];
+        if (!ri.x) continue;
+        const float4 rf = F.res_f[w];
+        const int2 ss = F.seg_slot[ri.y];
+        const int sl[3] = {F.item_slot[w], ss.x, ss.y};
+        const float2 val[3] = {make_float2(rf.x, rf.y), make_float2(-(rf.x * rf.z), -(rf.y * rf.z)),
+                               make_float2(-(rf.x * rf.w), -(rf.y * rf.w))};
+        for (int c = 0; c < 3; ++c) {
+            const int at = atomicAdd(&F.slot_cur[sl[c]], 1);
+            F.ent_key[at] = w;
+            F.ent_f[at] = val[c];
+        }
+    }
+    phase_barrier();
+    // each slot's entries in ascending work item, into its body's ranges
+    for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) {
+        const int b0 = F.slot_start[sl], e = F.slot_cur[sl];
+        for (int i = b0 + 1; i < e; ++i) {
+            const int kv = F.ent_key[i];
+            const float2 fv = F.ent_f[i];
+            int j = i - 1;
+            while (j >= b0 && F.ent_key[j] > kv) {
+                F.ent_key[j + 1] = F.ent_key[j];
+                F.ent_f[j + 1] = F.ent_f[j];
+                --j;
+            }
+            F.ent_key[j + 1] = kv;
+            F.ent_f[j + 1] = fv;
+        }
+        const int b = F.slot_body[sl];
+        const int base = F.slot_start[F.body_slot_off[b]];
+        int2* rng = reinterpret_cast<int2*>(F.body_flex[b]);
+        float2* ent = reinterpret_cast<float2*>(rng + F.body_nn[b]);
+        rng[F.slot_node[sl]] = make_int2(b0 - base, e - b0);
+        for (int i = b0; i < e; ++i) ent[i - base] = F.ent_f[i];
+    }
+}
+
+}  // namespace
+}  // namespace fw
+
+extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, const SceneDev* scenes, const ArenaCtl* ctl, int gid,
+                                          hipStream_t stream) {
+    hipLaunchKernelGGL(fw::flex_hash_kernel, dim3(1), dim3(fw::FLEX_BLOCK), 0, stream, groups, scenes, ctl, gid);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const SceneDev* scenes,
+                                             const ArenaCtl* ctl, int k, uint32_t flags, hipStream_t stream) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fw::flex_contact_kernel, dim3(n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups, scenes, ctl,
+                       k, flags);
+    return hipGetLastError();
+}
+
+extern "C" int fw_flex_max_pairs_per_group(void) { return fw::FLEX_MAX_PAIRS; }

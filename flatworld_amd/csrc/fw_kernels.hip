@@ -408,7 +408,9 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 const GroundDev gd = fw_ldc(&Sp->ground[gi]);
                 const float* lo = pad ? gd.plo : gd.lo;
                 const float* hi = pad ? gd.phi : gd.hi;
-                if (lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1]) active |= 1u << gi;
+                // the FEM-FEM record is gated per pair by the contact pass itself
+                if (gd.kind == FW_GROUND_FLEX_DEV || (lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1]))
+                    active |= 1u << gi;
             }
         }
         lds_barrier();  // lds_f complete; lds_X free from here on
@@ -442,8 +444,23 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             float fx = 0.0f, fy = 0.0f;
             bool hit = false;
             if (active && (fl & NODE_BOUNDARY)) {
-                for (int gi = 0; gi < n_grounds; ++gi)
-                    if (active & (1u << gi)) hit |= ground_force(fw_ldc(&Sp->ground[gi]), p.x, p.y, v.x, v.y, dt, fx, fy);
+                for (int gi = 0; gi < n_grounds; ++gi) {
+                    if (!(active & (1u << gi))) continue;
+                    const GroundDev gd = fw_ldc(&Sp->ground[gi]);
+                    if (gd.kind == FW_GROUND_FLEX_DEV) {
+                        // FEM-FEM contributions of this substep, in ascending work item
+                        const int2* rng = static_cast<const int2*>(gd.data);
+                        const float2* ent = reinterpret_cast<const float2*>(rng + gd.nx);
+                        const int2 r = fw_ld(rng + n);
+                        for (int j = 0; j < r.y; ++j) {
+                            const float2 c = fw_ld(ent + r.x + j);
+                            fx = fx + c.x;
+                            fy = fy + c.y;
+                        }
+                    } else {
+                        hit |= ground_force(gd, p.x, p.y, v.x, v.y, dt, fx, fy);
+                    }
+                }
             }
             // element forces in ascending element id: a contiguous run of LDS entries
             // (two entries per iteration, so that their LDS reads overlap; same order)

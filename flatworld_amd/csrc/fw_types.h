@@ -24,6 +24,7 @@
 
 #define FW_MAX_GROUNDS_DEV 4
 #define FW_MAT_SPRING_DEV 4  // FW_MAT_SPRING of the public header
+#define FW_GROUND_FLEX_DEV 4 // FW_GROUND_FLEX: the scene's FEM-FEM contact contributions
 
 struct GroundDev {
     int32_t kind;
@@ -34,7 +35,57 @@ struct GroundDev {
     int32_t nx, ny, reverse;
     float hf_dxw;          // HF: sample spacing (ub_x - lb_x) / (nx - 1), rounded once in float32 on the host
     const void* data;      // HF heights (float) | voxel cell extents (uint16 x4 per cell: scan end points
-                           // along -x, +x, -z, +z; first = 0xFFFF when the cell is not occupied)
+                           // along -x, +x, -z, +z; first = 0xFFFF when the cell is not occupied) |
+                           // FEM-FEM: int2 (first entry, count) per node, then the float2 entries
+                           // (nx = the scene's node count); never gated
+};
+
+// FEM-FEM contact group: the FEM bodies of one loop with the work list of their
+// FEM-FEM pairs and the spatial hash over their boundary segments.  Node ids are
+// scene-internal; a "slot" is a boundary node of a body in some pair, the unit
+// of the per-substep contribution lists.
+struct FlexGrid {
+    float glb[2], gs[2];
+    int32_t cn[2], tc;
+    int32_t exact;  // every segment sits in every cell of its box (no cell or pair cap hit)
+};
+
+struct FlexDev {
+    int32_t n_bodies, n_segs, n_items, n_pairs, n_slots;
+    int32_t gated, max_cells, max_query, max_pairs;
+    const int32_t* body_scene;     // [B] arena scene id
+    const int32_t* body_seg_off;   // [B + 1]
+    const int32_t* body_slot_off;  // [B + 1]
+    char* const* body_flex;        // [B] the scene's FEM-FEM block (nullptr: body in no pair)
+    const int32_t* body_nn;        // [B] node count
+    const int32_t* slot_node;      // [n_slots] internal node id
+    const int32_t* slot_body;      // [n_slots]
+    const int32_t* seg_body;       // [S]
+    const int2* seg_node;          // [S] internal node ids
+    const int2* seg_slot;          // [S] slots of its nodes (-1: body in no pair)
+    const int32_t* item_body;      // [W]
+    const int32_t* item_node;      // [W] internal node id
+    const int32_t* item_slot;      // [W]
+    const int32_t* item_target;    // [W]
+    const int32_t* item_pair;      // [W]
+    const float* item_penalty;     // [W]
+    const float* item_friction;    // [W]
+    int32_t* cache;                // [W] cached segment or -1
+    const int2* pair_body;         // [P]
+    // spatial hash, rebuilt by flex_hash_kernel
+    FlexGrid* grid;
+    float4* seg_box;               // [S] segment boxes of the last build
+    int32_t* seg_pairs;            // [S] scratch: pairs per segment, then the serial pair base
+    int32_t* cell_start;           // [max_cells]
+    int32_t* cell_end;             // [max_cells]
+    int32_t* sorted;               // [max_pairs] segment ids, ascending within a cell
+    // per-substep scratch of flex_contact_kernel
+    int2* res_i;                   // [W] (hit, segment)
+    float4* res_f;                 // [W] (force x, y, w0, w1)
+    int32_t* slot_start;           // [n_slots]
+    int32_t* slot_cur;             // [n_slots]
+    int32_t* ent_key;              // [3W] work item of each entry
+    float2* ent_f;                 // [3W]
 };
 
 // The scene fields every tile reads (material, dt, gate state), kept together at

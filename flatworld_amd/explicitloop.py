@@ -11,7 +11,13 @@ Setup follows flatworld/explicitloop.py:72-398 for the body-vs-ground subset:
   reference's O(D^2) order and split into the batched plane / height-field /
   voxel work lists (flatworld/mixedcontact.py:1756-2269).  As in the reference,
   a loop with more than one distinct height field (or voxel grid) leaves those
-  contacts unbatched, which means they never run; a warning says so.
+  contacts unbatched, which means they never run; a warning says so;
+* FEM-FEM pairs become one contact group over the loop's FEM bodies
+  (flatworld/mixedcontact.py:1857-1957): a spatial hash over their boundary
+  edges, built after the warm-up (explicitloop.py:383-385) and rebuilt every
+  1/500 s of simulated time, and a contact pass before every substep.  Its
+  pairs are gated by the bodies' AABB overlap when the loop runs its BVH (any
+  ground contact present) and always active otherwise (explicitloop.py:367-381).
 
 ``advance`` is one fused substep launch: activation gate, ground forces,
 element forces, symplectic Euler and AABB, all on the device.
@@ -24,8 +30,9 @@ import warnings
 import numpy as np
 
 from .definitions import ContactType, DomainType
-from .femcontact import contactHelper
+from .femcontact import ContactFlexFlex, contactHelper
 from .femspringmanager import FemSpringManager
+from .flexcontact import build_flex_group
 
 _GROUND_TYPES = (DomainType.ANALYTICAL, DomainType.HEIGHTFIELD, DomainType.VOXELMAP)
 _BODY_TYPES = (DomainType.FEM, DomainType.SPRINGMASS)
@@ -106,13 +113,40 @@ class ExplicitLoop:
             fem_idx = self.domains.index(c.domain1)
             grounds_of.setdefault(fem_idx, []).append((c.domain2, c))
 
+        # FEM-FEM: one group over the loop's FEM bodies (flatworld/mixedcontact.py:1857-1957)
+        flex_contacts = [c for c in self.contacts if isinstance(c, ContactFlexFlex)]
+        flex = None
+        if flex_contacts:
+            managed = [d for d in domains if d.type in _BODY_TYPES]
+            fem_bodies = [d for d in managed if d.type == DomainType.FEM]
+            if managed[: len(fem_bodies)] != fem_bodies:
+                raise NotImplementedError(
+                    "FEM-FEM contact with a spring-mass domain listed before a FEM domain: the reference's "
+                    "spatial hash indexes the FEM bodies by manager position (flatworld/femspringmanager.py:455-466)"
+                )
+            # the BVH runs when any ground contact exists (flatworld/explicitloop.py:367-381)
+            gated = len(anal) + len(hf) + len(vox) + len(self._unbatched_contacts) > 0
+            if gated:
+                for b, dom in enumerate(fem_bodies):
+                    if self.domains.index(dom) != b:
+                        raise NotImplementedError(
+                            "FEM-FEM contact in a loop with grounds needs the FEM domains listed first: the "
+                            "reference's per-domain BVH mask is read at the body's FEM index "
+                            "(flatworld/mixedcontact.py:925, explicitloop.py:501-509)"
+                        )
+            count_nodes = sum(d.nnodes for d in managed)
+            flex = build_flex_group(fem_bodies, flex_contacts, gated, count_nodes)
+
         self.femSpringManager = FemSpringManager(self.d, domains, grounds_of=grounds_of, device=device,
-                                                 arena_kwargs=arena_kwargs)
+                                                 arena_kwargs=arena_kwargs, flex=flex)
         self.stableTime = min(self.femSpringManager.stableTime, self.stableTime)
         self.femSpringManager.setGlobalAABB(None)
         if mass_scaling_dt > 0.0:
             self.femSpringManager.applyMassScaling(mass_scaling_dt)
             self.stableTime = max(self.stableTime, mass_scaling_dt)
+        if flex is not None:
+            # initial FEM spatial hash, so that the first substep has one (flatworld/explicitloop.py:383-385)
+            self.femSpringManager.populate_fem_spatial_hash(velocity_buffer=0.0)
         self.numContacts = len(self.contacts)
         for c in self.contacts:
             self.stableTime = min(self.stableTime, c.stableTime)
@@ -126,7 +160,7 @@ class ExplicitLoop:
     # ------------------------------------------------------------ stepping
     def advance(self):
         """One substep: gate, ground forces, element forces, integrate (flatworld/explicitloop.py:473-555)."""
-        self.femSpringManager.advance_substeps(1, self.dt, self.damping, diag=True)
+        self.femSpringManager.advance_substeps(1, self.dt, self.damping, diag=True, dt64=[self.dt])
         self._counter_py += 1
 
     def substep_sequence(self, frame_dt):
@@ -152,7 +186,8 @@ class ExplicitLoop:
         ends = starts[1:] + [len(dts)]
         for r, (s, e) in enumerate(zip(starts, ends)):
             self.dt = seq[e - 1]
-            self.femSpringManager.advance_substeps(e - s, float(dts[s]), self.damping, diag=(r == len(starts) - 1))
+            self.femSpringManager.advance_substeps(e - s, float(dts[s]), self.damping, diag=(r == len(starts) - 1),
+                                                   dt64=seq[s:e])
         self._counter_py += len(seq)
         if verbose:
             print(f"Completed {len(seq)} substeps for {frame_dt} seconds of simulation time.")

@@ -1,15 +1,17 @@
-"""Body-vs-ground contact pair records: penalty stiffness and contact stable time.
+"""Contact pair records: penalty stiffness and contact stable time.
 
 Host scalars only, following flatworld/femcontact.py:94-197 and 256-303:
 
 * FEM body: bulk modulus ``K = E / (3 (1 - 2 min(nu, 0.4999)))``, 2D segment
   penalty ``k = 2 K`` (independent of mesh size), contact stable time
   ``0.9 sqrt(m_node / k)`` with ``m_node = rho * (L_min^2 / 2) / 3``;
+* FEM-FEM: the smaller of the two bodies' segment penalties, and the smaller of
+  the two stable times;
 * spring network: ``k = 100 * spring``, stable time ``0.5 sqrt(mass / k)``.
 
-The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip).
-FEM-FEM, FEM-spring and rigid contacts are outside this package: constructing
-one raises ``NotImplementedError`` instead of silently dropping it.  Pairs the
+The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip,
+fw_flex.hip).  FEM-spring and rigid contacts are outside this package:
+constructing one raises ``NotImplementedError`` instead of silently dropping it.  Pairs the
 reference's factory has no class for (spring-spring, spring-voxel) give no
 contact, as there.
 """
@@ -53,6 +55,16 @@ class ContactBase:
 
     def calculate(self, dt):
         pass
+
+
+class ContactFlexFlex(ContactBase):
+    """FEM vs FEM (flatworld/femcontact.py:164-172): symmetric, the more conservative side's penalty."""
+
+    def __init__(self, domain1, domain2, type, is_tied=False):
+        super().__init__(domain1, domain2, type, is_tied)
+        self.penalty = min(self._segment_penalty(self.domain1), self._segment_penalty(self.domain2)) * self.penalty
+        self.stableTime = self.calStableTime(self.penalty, self.domain1)
+        self.stableTime = min(self.stableTime, self.calStableTime(self.penalty, self.domain2))
 
 
 class _FlexGround(ContactBase):
@@ -113,7 +125,7 @@ class ContactSpringHeightField(_SpringGround):
 
 
 def contactHelper(domain1, domain2, type):
-    """Pair factory (flatworld/femcontact.py:283-303) restricted to body-vs-ground."""
+    """Pair factory (flatworld/femcontact.py:283-303) without the rigid and FEM-spring pairs."""
     if type == ContactType.ANALYTICALSPRING:
         return ContactSpringAnalytical(domain1, domain2, type)
     if type == ContactType.SPRINGHEIGHTFIELD:
@@ -132,10 +144,7 @@ def contactHelper(domain1, domain2, type):
     if type == ContactType.FLEXVOXELMAP:
         return ContactFlexVoxelMap(domain1, domain2, type)
     if type == ContactType.FLEXFLEX:
-        raise NotImplementedError(
-            "FEM-FEM penalty contact (flatworld/mixedcontact.py:687-1025) is not part of this "
-            "package; give the FEM domains considerContact=False or keep one FEM body per loop"
-        )
+        return ContactFlexFlex(domain1, domain2, type)
     if type in (ContactType.RIGIDANLAYTICAL, ContactType.RIGIDRIGID, ContactType.ANALYTICALANALYTICAL):
         return None
     raise NotImplementedError(f"contact type {type:#b} is not part of the explicit-FEM path")

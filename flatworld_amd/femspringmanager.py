@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _native
 from .definitions import DomainType
-from .scene import pack_domain
+from .flexcontact import REBUILD_INTERVAL
+from .scene import add_flex_record, pack_domain
 
 
 class DeviceField:
@@ -49,10 +50,13 @@ class FemSpringManager:
     """FEM and spring-mass domains of one loop, resident in one HIP arena.
 
     ``grounds_of`` maps a domain index (position in ``domains``) to its ordered
-    list of ``(ground_domain, contact)`` pairs; ExplicitLoop fills it.
+    list of ``(ground_domain, contact)`` pairs; ExplicitLoop fills it.  ``flex``,
+    when the loop has FEM-FEM contacts, is the ``flexcontact.FlexGroupSpec`` over
+    the loop's FEM bodies in this manager's order.
     """
 
-    def __init__(self, d, domains, spatial_hash=None, grounds_of=None, device=0, track_disp=True, arena_kwargs=None):
+    def __init__(self, d, domains, spatial_hash=None, grounds_of=None, device=0, track_disp=True, arena_kwargs=None,
+                 flex=None):
         if d != 2:
             raise ValueError(f"FemSpringManager supports d=2 only (got {d})")
         self.d = d
@@ -74,9 +78,16 @@ class FemSpringManager:
         self._elem_offsets = []
         grounds_of = grounds_of or {}
         conn_rows = []
+        fem_body = 0
+        self._specs = []  # the packed scenes, in manager order
         for local_idx, (idx, dom) in enumerate(fem):
             spec = pack_domain(dom, grounds_of.get(idx, []))
+            if flex is not None and dom.type == DomainType.FEM:
+                if flex.in_pair(fem_body):
+                    add_flex_record(spec, flex.flex_cap(fem_body))
+                fem_body += 1
             sid = self._arena.add(spec, track_disp=track_disp)
+            self._specs.append(spec)
             self._scene_ids.append(sid)
             self._node_offsets.append(node_off)
             self._elem_offsets.append(elem_off)
@@ -127,6 +138,14 @@ class FemSpringManager:
             allrows = np.concatenate(conn_rows)
             conn[: allrows.shape[0]] = allrows
         self.connectivity = _HostField(conn)
+        # FEM-FEM contact group and its hash schedule (flatworld/femspringmanager.py:520-524)
+        self._flex = flex
+        self._flex_gid = None
+        self._sh_rebuild_interval = REBUILD_INTERVAL
+        self._sh_fem_elapsed = 0.0
+        if flex is not None:
+            bodies = [sid for sid, (_, dom) in zip(self._scene_ids, fem) if dom.type == DomainType.FEM]
+            self._flex_gid = self._arena.flex_add(flex, bodies)
 
     # ------------------------------------------------------------ stepping
     def setGlobalAABB(self, aabb=None):
@@ -142,10 +161,49 @@ class FemSpringManager:
         self._arena.step(1, float(np.float32(dt)), _native.FW_STEP_NO_CONTACT | _native.FW_STEP_DIAG,
                          damping_override=float(np.float32(damping)))
 
-    def advance_substeps(self, n, dt, damping, diag=True):
-        """n full substeps (ground contact + femStep) with one fused launch each."""
+    def advance_substeps(self, n, dt, damping, diag=True, dt64=None):
+        """n full substeps (ground contact + femStep) with one fused launch each.
+
+        With FEM-FEM contact, ``dt64`` (one float64 per substep, default ``dt``)
+        advances the hash schedule: a rebuild due at a substep is enqueued before
+        it, as run_batched_contacts does (flatworld/mixedcontact.py:1672-1678)."""
         flags = _native.FW_STEP_DIAG if diag else 0
-        self._arena.step(int(n), float(np.float32(dt)), flags, damping_override=float(np.float32(damping)))
+        dt32, damp32 = float(np.float32(dt)), float(np.float32(damping))
+        n = int(n)
+        if self._flex_gid is None:
+            self._arena.step(n, dt32, flags, damping_override=damp32)
+            return
+        seq = [float(dt)] * n if dt64 is None else [float(v) for v in dt64]
+        pending = 0
+        for k in range(n):
+            self._sh_fem_elapsed += seq[k]
+            if self._sh_fem_elapsed >= self._sh_rebuild_interval:
+                if pending:
+                    self._arena.step(pending, dt32, flags & ~_native.FW_STEP_DIAG, damping_override=damp32)
+                    pending = 0
+                self._arena.flex_rebuild(self._flex_gid)
+                self._sh_fem_elapsed = 0.0
+            pending += 1
+        if pending:
+            self._arena.step(pending, dt32, flags, damping_override=damp32)
+
+    def populate_fem_spatial_hash(self, velocity_buffer=0.0):
+        """Build the FEM spatial hash at the current state (flatworld/femspringmanager.py:1041-1065)."""
+        if velocity_buffer != 0.0:
+            raise NotImplementedError("the FEM spatial hash is built with velocity_buffer=0 only, as the reference calls it")
+        if self._flex_gid is not None:
+            self._arena.flex_rebuild(self._flex_gid)
+
+    def maybe_rebuild_fem_spatial_hash(self, dt):
+        """flatworld/femspringmanager.py:1035-1039."""
+        self._sh_fem_elapsed += dt
+        if self._sh_fem_elapsed >= self._sh_rebuild_interval:
+            self.populate_fem_spatial_hash(velocity_buffer=0.0)
+            self._sh_fem_elapsed = 0.0
+
+    def flex_cache(self):
+        """Cached target segment of every FEM-FEM work item (-1: none)."""
+        return self._arena.flex_read(self._flex_gid, _native.FLEX_CACHE)
 
     def applyMassScaling(self, dt_target):
         """Selective mass scaling of every domain at the current positions

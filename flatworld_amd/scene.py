@@ -34,6 +34,9 @@ from .definitions import (
 GROUND_PLANE = 1
 GROUND_HEIGHTFIELD = 2
 GROUND_VOXEL = 3
+# the scene's FEM-FEM contact contributions (FW_GROUND_FLEX): filled by its
+# contact group before each substep, placed after the plane records
+GROUND_FLEX = 4
 
 # material code of a spring-mass scene (FW_MAT_SPRING of include/flatworld_fem.h);
 # the reference has no material type for springs: their elements carry matE and
@@ -59,6 +62,7 @@ class GroundSpec:
     vox_lb: Optional[np.ndarray] = None
     vox_dx: np.float32 = np.float32(0.0)
     vox_dz: np.float32 = np.float32(0.0)
+    flex_cap: int = 0  # FEM-FEM record: contributions per substep
 
 
 @dataclass
@@ -254,6 +258,16 @@ def pack_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
     if domain.type == DomainType.SPRINGMASS:
         return pack_spring_domain(domain, grounds, dt, damping, horizon)
     return pack_fem_domain(domain, grounds, dt, damping, horizon)
+
+
+def add_flex_record(spec, cap):
+    """Insert the FEM-FEM contact record where the reference launches that kernel:
+    after the plane kernel, before the height-field and voxel kernels
+    (flatworld/mixedcontact.py:1654-1754)."""
+    at = sum(1 for g in spec.grounds if g.kind == GROUND_PLANE)
+    rec = GroundSpec(GROUND_FLEX, np.float32(0.0), np.zeros(2, np.float32), np.zeros(2, np.float32), flex_cap=int(cap))
+    spec.grounds.insert(at, rec)
+    return spec
 
 
 def scene_stable_time(domain, contacts, has_plane):

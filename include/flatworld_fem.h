@@ -28,6 +28,15 @@
  *   fw_scene_remove   no counterpart: the reference never removes a domain.
  *   fw_scene_mass_scaling  FemSpringManager.applyMassScaling
  *                     (flatworld/femspringmanager.py:141-201, 899-924)
+ *   fw_flex_add       the FlexFlex part of MixedContact._build_batched_contacts
+ *                     and the FEM spatial hash's allocation
+ *                     (flatworld/mixedcontact.py:1857-1957;
+ *                      flatworld/femspringmanager.py:556; spatialmanager.py:508-565)
+ *   fw_flex_rebuild   FemSpringManager.populate_fem_spatial_hash
+ *                     (flatworld/femspringmanager.py:421-487, 1041-1065;
+ *                      flatworld/spatialmanager.py:567-712)
+ *   fw_step (groups)  _batched_flexflex_contact_kernel_sh, launched by
+ *                     run_batched_contacts (flatworld/mixedcontact.py:886-1025, 1663-1695)
  * Spring-mass scenes replace addSpringDomain and _spring_force_kernel
  * (flatworld/femspringmanager.py:814-870, 283-312; flatworld/springmass.py).
  */
@@ -40,7 +49,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 2
+#define FW_ABI_VERSION 3
 #define FW_MAX_GROUNDS 4
 
 /* status codes */
@@ -61,6 +70,12 @@ extern "C" {
 #define FW_GROUND_PLANE 1
 #define FW_GROUND_HEIGHTFIELD 2
 #define FW_GROUND_VOXEL 3
+/* FW_GROUND_FLEX: the scene's FEM-FEM contact contributions, filled each substep by
+ * its contact group (fw_flex_add).  It goes where the reference launches the
+ * FEM-FEM kernel: after the plane records, before the height-field and voxel
+ * records.  Of its descriptor only kind and flex_cap are read: flex_cap bounds the
+ * contributions per substep (own work items + 2 x work items that target this body). */
+#define FW_GROUND_FLEX 4
 
 /* node flag word: bits 0..8 BC bits (flatworld/definitions.py:95-104),
  * bit 15 boundary node, bits 16..31 row of the scene's BC value table */
@@ -111,6 +126,7 @@ typedef struct {
     float vox_lb[2];
     float vox_dx;
     float vox_dz;
+    int32_t flex_cap;   /* FW_GROUND_FLEX: contribution capacity */
 } fw_ground_desc;
 
 /* One FEM or spring-mass body; arrays in the caller's node/element order.
@@ -161,6 +177,42 @@ typedef struct {
     int32_t tiles_per_cu;      /* substep workgroups resident per CU for the tuning (HIP occupancy) */
 } fw_arena_info;
 
+/* FEM-FEM contact group: the FEM bodies of one loop (scenes added to the same
+ * arena before the same substep) and the batched work list of their FEM-FEM pairs
+ * (flatworld/mixedcontact.py:1857-1911).  Node ids are body-local, in the
+ * caller's order.  Segments are every body's boundary edges, body by body, each
+ * body's in boundary-edge order (flatworld/femspringmanager.py:780-799). */
+typedef struct {
+    int32_t n_bodies;
+    const int32_t* scene_ids;     /* [n_bodies] */
+    const int32_t* body_seg_off;  /* [n_bodies + 1] */
+    const int32_t* seg_node;      /* [2 * n_segs] */
+    int32_t n_segs;
+    int32_t n_items;
+    const int32_t* item_body;     /* body of the contact node */
+    const int32_t* item_node;     /* its body-local id (a boundary node) */
+    const int32_t* item_target;   /* body whose segments it is checked against */
+    const int32_t* item_pair;
+    const float* item_penalty;
+    const float* item_friction;
+    int32_t n_pairs;
+    const int32_t* pair_body;     /* [2 * n_pairs] */
+    const int32_t* body_bnodes_off; /* [n_bodies + 1] */
+    const int32_t* body_bnodes;   /* boundary nodes of each body (body-local ids) */
+    int32_t gated;                /* 0: every pair is active (the loop skips its BVH) */
+    int32_t max_cells;            /* spatial hash limits (SpatialHashManager defaults: 100000, 512) */
+    int32_t max_query;
+    int32_t max_pairs;
+} fw_flex_desc;
+
+/* fields for fw_flex_read */
+#define FW_FLEX_CACHE 1      /* int32[n_items] cached segment per work item, -1 for none */
+#define FW_FLEX_GRID 2       /* 8 x 4 bytes: lower corner x, y and cell size x, y (float32); cells x, y,
+                                cells used, exact flag (int32) */
+#define FW_FLEX_CELL_START 3 /* int32[max_cells] */
+#define FW_FLEX_CELL_END 4   /* int32[max_cells] */
+#define FW_FLEX_SORTED 5     /* int32[max_pairs] segment ids, ascending within each cell */
+
 typedef struct fw_arena fw_arena;
 typedef struct fw_prepared fw_prepared;
 
@@ -207,6 +259,19 @@ int32_t fw_scene_reset_gate(fw_arena* a, int32_t scene_id);
 int32_t fw_scene_mass_scaling(fw_arena* a, int32_t scene_id, float dt_target);
 /* Substeps completed by the scene (warm-up excluded) and whether it reached its horizon. */
 int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, int32_t* finished);
+
+/* FEM-FEM contact groups.  fw_flex_add needs every body in a pair to carry one
+ * FW_GROUND_FLEX record of sufficient capacity; its scenes cannot be removed
+ * before the group.  fw_flex_rebuild enqueues the spatial-hash build at the state
+ * the next substep starts from (after the scenes' warm-up); the caller keeps the
+ * reference's rebuild schedule (every 1/500 s of simulated time,
+ * flatworld/femspringmanager.py:1035-1039).  fw_step then runs the group's
+ * contact pass before every substep.  fw_step_graph and FW_STEP_PER_SCENE are
+ * refused while a group exists. */
+int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* desc, int32_t* group_id);
+int32_t fw_flex_rebuild(fw_arena* a, int32_t group_id);
+int32_t fw_flex_remove(fw_arena* a, int32_t group_id);
+int32_t fw_flex_read(fw_arena* a, int32_t group_id, int32_t field, void* out, int64_t out_bytes);
 
 /* Enqueue n substeps of every resident scene on the arena stream (asynchronous).
  * dt_override >= 0 replaces every scene's dt and damping_override >= 0 every

@@ -538,6 +538,17 @@ void ora_substep(ora_scene* s, float dt, float damping, int32_t counter, int32_t
     if (with_contact) {
         for (int32_t gi = 0; gi < s->n_grounds; ++gi) {
             const ora_ground* g = s->grounds + gi;
+            if (g->kind == ORA_FLEX) {
+                /* contributions in ascending work item, after the plane kernel */
+                for (int32_t k = 0; k < s->n_bnodes; ++k) {
+                    int32_t n = s->bnodes[k];
+                    for (int32_t j = g->flex_off[n]; j < g->flex_off[n + 1]; ++j) {
+                        fx[2 * n] += g->flex_f[2 * j];
+                        fx[2 * n + 1] += g->flex_f[2 * j + 1];
+                    }
+                }
+                continue;
+            }
             if (!ora_gate(s->aabb, s->aabb + 2, g, counter)) continue;
             for (int32_t k = 0; k < s->n_bnodes; ++k) {
                 int32_t n = s->bnodes[k];

@@ -26,7 +26,11 @@ extern "C" {
  * two-node springs, conn holds [2E] node ids, E is the stiffness, rho the mass of
  * every node and spring_c the damping coefficient (femspringmanager.py:814-870). */
 enum { ORA_ELASTIC = 1, ORA_NEOHOOKEAN = 2, ORA_MISES = 3, ORA_SPRING = 4 };
-enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3 };
+/* ORA_FLEX: not a ground but the FEM-FEM contact contributions of the substep
+ * (ora_flex_contacts), placed in the ground list where the reference's batched
+ * FEM-FEM kernel runs: after the plane kernel, before the height-field and voxel
+ * kernels (flatworld/mixedcontact.py:1654-1754).  It is never gated. */
+enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3, ORA_FLEX = 4 };
 
 typedef struct {
     int32_t kind;
@@ -49,6 +53,9 @@ typedef struct {
     float vox_lb[2];
     float vox_dx;
     float vox_dz;
+    /* ORA_FLEX: contributions to node n are flex_f[2j .. 2j+1], j in [flex_off[n], flex_off[n+1]) */
+    const int32_t* flex_off;
+    const float* flex_f;
 } ora_ground;
 
 typedef struct {
@@ -109,6 +116,62 @@ int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[
 int32_t ora_gate(const float fem_lb[2], const float fem_ub[2], const ora_ground* g, int32_t counter);
 void ora_update_bbox(ora_scene* s);
 float ora_log_f32(float x);
+
+/* ------------------------------------------------------------------ FEM-FEM contact
+ * The batched FEM-FEM penalty contact with the FEM spatial hash
+ * (flatworld/mixedcontact.py:886-1025, spatialmanager.py, femspringmanager.py:421-487,
+ * 1035-1073), restated with the reference's kernels run serially in thread order.
+ * The reference runs them in parallel: its spatial-hash scatter and its Fext
+ * updates are atomics, and every thread of the contact kernel shares one query
+ * buffer.  Serial thread order is the one execution of those kernels that is
+ * race-free; it fixes the order of a hash cell's entries (ascending segment), of
+ * each node's contributions (ascending work item) and gives every work item its
+ * own query list. */
+typedef struct {
+    /* boundary segments of the loop's FEM bodies: body by body in loop order, each
+       body's boundary edges in insertion order (femspringmanager.py:780-799) */
+    int32_t n_bodies;
+    int32_t n_segs;
+    const int32_t* body_seg_off; /* [n_bodies + 1] */
+    const int32_t* seg_body;     /* [n_segs] */
+    const int32_t* seg_node;     /* [2 n_segs] body-local node ids */
+    /* work items in the reference's order (mixedcontact.py:1862-1911) */
+    int32_t n_items;
+    const int32_t* item_body;    /* body of the contact node */
+    const int32_t* item_node;    /* body-local node id */
+    const int32_t* item_target;  /* body whose segments the node is checked against */
+    const int32_t* item_pair;
+    const float* item_penalty;
+    const float* item_friction;
+    int32_t* cache;              /* [n_items] cached segment (global index) or -1 */
+    int32_t n_pairs;
+    const int32_t* pair_body;    /* [2 n_pairs] */
+    int32_t gated;               /* 0: every pair active (the loop skips its BVH) */
+    /* spatial hash */
+    int32_t max_cells, max_query, max_pairs;
+    float glb[2], gub[2], gs[2];
+    int32_t cn[2], total_cells, n_hash_pairs;
+    int32_t* cell_start;         /* [max_cells] */
+    int32_t* cell_end;           /* [max_cells] */
+    int32_t* sorted_seg;         /* [max_pairs] */
+    /* output of ora_flex_contacts, per body: node CSR [N_b + 1] and float2 entries */
+    int32_t** c_off;
+    float** c_f;
+} ora_flex;
+
+/* pointToEdgeContact (flatworld/contact_detection.py:15-61): returns is_inside. */
+int32_t ora_point_edge(const float p[2], const float a[2], const float b[2], float* pen, float normal[2], float w[2]);
+/* FemSpringManager.populate_fem_spatial_hash + SpatialHashManager.build at the
+ * bodies' current positions and AABBs (velocity buffer 0). */
+void ora_flex_hash_build(ora_flex* f, ora_scene* const* bodies);
+/* query_point_with_buffer (spatialmanager.py:131-199): segments of body `target`
+ * in the cells the buffered point overlaps, deduplicated, at most max_query. */
+int32_t ora_flex_query(const ora_flex* f, const float p[2], float buffer, int32_t target, int32_t* out);
+/* One launch of the batched FEM-FEM kernel at the bodies' current state: updates
+ * the cache and writes each body's contribution lists (c_off / c_f). */
+void ora_flex_contacts(ora_flex* f, ora_scene* const* bodies, int32_t counter);
+int32_t ora_gate_pair(const float alb[2], const float aub[2], const float blb[2], const float bub[2],
+                      int32_t counter);
 
 #ifdef __cplusplus
 }

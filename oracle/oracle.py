@@ -8,6 +8,9 @@ Contents
 * ``OracleScene``: ctypes driver of oracle/liboracle.so (fem_oracle.c), the
   serial float32 restatement of the substep, fed the same ``SceneSpec`` arrays
   the device receives.
+* ``OracleFlexGroup`` / ``OracleLoop``: the FEM-FEM contact of a loop
+  (flex_oracle.c) and a multi-body loop driver with the reference's hash
+  schedule.
 * ``boundary_edges_dict``: a straightforward restatement of
   flatworld/mesh.py:53-71 (a dict over element edges), used to pin the
   vectorized topology of flatworld_amd.mesh bit for bit.
@@ -47,6 +50,8 @@ class OraGround(ctypes.Structure):
         ("vox_lb", ctypes.c_float * 2),
         ("vox_dx", ctypes.c_float),
         ("vox_dz", ctypes.c_float),
+        ("flex_off", _i32p),
+        ("flex_f", _f32p),
     ]
 
 
@@ -86,6 +91,41 @@ class OraScene(ctypes.Structure):
     ]
 
 
+class OraFlex(ctypes.Structure):
+    _fields_ = [
+        ("n_bodies", ctypes.c_int32),
+        ("n_segs", ctypes.c_int32),
+        ("body_seg_off", _i32p),
+        ("seg_body", _i32p),
+        ("seg_node", _i32p),
+        ("n_items", ctypes.c_int32),
+        ("item_body", _i32p),
+        ("item_node", _i32p),
+        ("item_target", _i32p),
+        ("item_pair", _i32p),
+        ("item_penalty", _f32p),
+        ("item_friction", _f32p),
+        ("cache", _i32p),
+        ("n_pairs", ctypes.c_int32),
+        ("pair_body", _i32p),
+        ("gated", ctypes.c_int32),
+        ("max_cells", ctypes.c_int32),
+        ("max_query", ctypes.c_int32),
+        ("max_pairs", ctypes.c_int32),
+        ("glb", ctypes.c_float * 2),
+        ("gub", ctypes.c_float * 2),
+        ("gs", ctypes.c_float * 2),
+        ("cn", ctypes.c_int32 * 2),
+        ("total_cells", ctypes.c_int32),
+        ("n_hash_pairs", ctypes.c_int32),
+        ("cell_start", _i32p),
+        ("cell_end", _i32p),
+        ("sorted_seg", _i32p),
+        ("c_off", ctypes.POINTER(_i32p)),
+        ("c_f", ctypes.POINTER(_f32p)),
+    ]
+
+
 _lib = None
 
 
@@ -115,6 +155,15 @@ def lib():
         L.ora_mass_scaling.restype = ctypes.c_float
         L.ora_log_f32.argtypes = [ctypes.c_float]
         L.ora_log_f32.restype = ctypes.c_float
+        _scenes = ctypes.POINTER(ctypes.POINTER(OraScene))
+        L.ora_point_edge.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.ora_point_edge.restype = ctypes.c_int32
+        L.ora_flex_hash_build.argtypes = [ctypes.POINTER(OraFlex), _scenes]
+        L.ora_flex_query.argtypes = [ctypes.POINTER(OraFlex), _f32p, ctypes.c_float, ctypes.c_int32, _i32p]
+        L.ora_flex_query.restype = ctypes.c_int32
+        L.ora_flex_contacts.argtypes = [ctypes.POINTER(OraFlex), _scenes, ctypes.c_int32]
+        L.ora_gate_pair.argtypes = [_f32p, _f32p, _f32p, _f32p, ctypes.c_int32]
+        L.ora_gate_pair.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -149,6 +198,7 @@ def make_ground(gs, keep):
         g.vox_lb[:] = [float(v) for v in gs.vox_lb]
         g.vox_dx = float(gs.vox_dx)
         g.vox_dz = float(gs.vox_dz)
+    # kind 4 (FEM-FEM contributions): linked to its group's lists by OracleFlexGroup
     return g
 
 
@@ -228,6 +278,132 @@ class OracleScene:
     def run(self, n, dt=None, damping=None):
         dt = self.spec.dt if dt is None else dt
         damping = self.spec.damping if damping is None else damping
+        for _ in range(int(n)):
+            self.substep(dt, damping)
+
+
+def point_edge(p, a, b):
+    """pointToEdgeContact: (is_inside, penetration, normal, weights)."""
+    arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in (p, a, b)]
+    pen = np.zeros(1, np.float32)
+    n = np.zeros(2, np.float32)
+    w = np.zeros(2, np.float32)
+    ins = lib().ora_point_edge(*(_p(v, _f32p) for v in arrs), _p(pen, _f32p), _p(n, _f32p), _p(w, _f32p))
+    return bool(ins), pen[0], n, w
+
+
+class OracleFlexGroup:
+    """Serial CPU run of one FEM-FEM contact group over OracleScene bodies.
+
+    ``bodies`` are the loop's FEM bodies in manager order (OracleScene, whose
+    specs already carry the FEM-FEM ground record where the body is in a pair)
+    and ``group`` the ``flatworld_amd.flexcontact.FlexGroupSpec``."""
+
+    def __init__(self, bodies, group):
+        L = lib()
+        self._lib = L
+        self.bodies = bodies
+        self.group = g = group
+        B = g.n_bodies
+        self._arr = keep = {}
+        keep["seg_off"] = np.ascontiguousarray(g.body_seg_off, np.int32)
+        keep["seg_body"] = np.repeat(np.arange(B, dtype=np.int32), np.diff(g.body_seg_off)).astype(np.int32)
+        keep["seg_node"] = np.ascontiguousarray(g.seg_node, np.int32).reshape(-1)
+        for k in ("item_body", "item_node", "item_target", "item_pair"):
+            keep[k] = np.ascontiguousarray(getattr(g, k), np.int32)
+        keep["item_penalty"] = np.ascontiguousarray(g.item_penalty, np.float32)
+        keep["item_friction"] = np.ascontiguousarray(g.item_friction, np.float32)
+        self.cache = np.full(g.n_items, -1, np.int32)
+        keep["pair_body"] = np.ascontiguousarray(g.pair_body, np.int32).reshape(-1)
+        self.cell_start = np.zeros(g.max_cells, np.int32)
+        self.cell_end = np.zeros(g.max_cells, np.int32)
+        self.sorted_seg = np.zeros(g.max_pairs, np.int32)
+        self.c_off = [np.zeros(b.spec.n_nodes + 1, np.int32) for b in bodies]
+        self.c_f = [np.zeros(2 * max(g.flex_cap(k), 1), np.float32) for k in range(B)]
+        F = OraFlex()
+        F.n_bodies, F.n_segs = B, g.n_segs
+        F.body_seg_off, F.seg_body, F.seg_node = (_p(keep[k], _i32p) for k in ("seg_off", "seg_body", "seg_node"))
+        F.n_items = g.n_items
+        F.item_body, F.item_node, F.item_target, F.item_pair = (
+            _p(keep[k], _i32p) for k in ("item_body", "item_node", "item_target", "item_pair"))
+        F.item_penalty, F.item_friction = _p(keep["item_penalty"], _f32p), _p(keep["item_friction"], _f32p)
+        F.cache = _p(self.cache, _i32p)
+        F.n_pairs = g.n_pairs
+        F.pair_body = _p(keep["pair_body"], _i32p)
+        F.gated = 1 if g.gated else 0
+        F.max_cells, F.max_query, F.max_pairs = g.max_cells, g.max_query, g.max_pairs
+        F.cell_start, F.cell_end, F.sorted_seg = (_p(a, _i32p) for a in (self.cell_start, self.cell_end, self.sorted_seg))
+        self._offp = (_i32p * B)(*[_p(a, _i32p) for a in self.c_off])
+        self._fp = (_f32p * B)(*[_p(a, _f32p) for a in self.c_f])
+        F.c_off = ctypes.cast(self._offp, ctypes.POINTER(_i32p))
+        F.c_f = ctypes.cast(self._fp, ctypes.POINTER(_f32p))
+        self._F = F
+        self._scenes = (ctypes.POINTER(OraScene) * B)(*[ctypes.pointer(b._s) for b in bodies])
+        # link each body's FEM-FEM ground record to its contribution lists
+        for k, body in enumerate(bodies):
+            for gi in range(body._s.n_grounds):
+                if body._grounds[gi].kind == 4:
+                    body._grounds[gi].flex_off = _p(self.c_off[k], _i32p)
+                    body._grounds[gi].flex_f = _p(self.c_f[k], _f32p)
+
+    def rebuild(self):
+        self._lib.ora_flex_hash_build(ctypes.byref(self._F), self._scenes)
+
+    def contacts(self, counter):
+        self._lib.ora_flex_contacts(ctypes.byref(self._F), self._scenes, int(counter))
+
+    def query(self, p, buffer, target):
+        out = np.zeros(self.group.max_query, np.int32)
+        pp = np.ascontiguousarray(p, dtype=np.float32)
+        n = self._lib.ora_flex_query(ctypes.byref(self._F), _p(pp, _f32p), float(np.float32(buffer)), int(target),
+                                     _p(out, _i32p))
+        return out[:n].copy()
+
+    @property
+    def grid(self):
+        F = self._F
+        return dict(glb=np.array(F.glb[:], np.float32), gs=np.array(F.gs[:], np.float32),
+                    cn=np.array(F.cn[:], np.int32), tc=int(F.total_cells), n_pairs=int(F.n_hash_pairs))
+
+    def contributions(self, b):
+        """Node CSR and float2 entries of body b from the last contact pass."""
+        off = self.c_off[b]
+        return off.copy(), self.c_f[b][: 2 * off[-1]].reshape(-1, 2).copy()
+
+
+class OracleLoop:
+    """Serial CPU run of an ExplicitLoop over several bodies: the zero-dt warm-up,
+    the initial hash build, then per substep the hash schedule, the FEM-FEM pass
+    and every body's substep, with the loop's counter driving the gates
+    (flatworld/explicitloop.py:383-385, 473-555; femspringmanager.py:1035-1039)."""
+
+    def __init__(self, specs, group=None):
+        self.bodies = [OracleScene(sp) for sp in specs]
+        self.group = group
+        self.flex = None
+        if group is not None:
+            fem = [b for b in self.bodies if b.spec.mat_type != 4]
+            self.flex = OracleFlexGroup(fem, group)
+        self.counter = 0
+        self.elapsed = 0.0
+        for b in self.bodies:
+            b.warmup()
+        if self.flex is not None:
+            self.flex.rebuild()
+
+    def substep(self, dt, damping=0.0):
+        if self.flex is not None:
+            self.elapsed += float(dt)
+            if self.elapsed >= self.group.rebuild_interval:
+                self.flex.rebuild()
+                self.elapsed = 0.0
+            self.flex.contacts(self.counter)
+        d32, c32 = float(np.float32(dt)), float(np.float32(damping))
+        for b in self.bodies:
+            b._lib.ora_substep(ctypes.byref(b._s), d32, c32, self.counter, 1)
+        self.counter += 1
+
+    def run(self, n, dt, damping=0.0):
         for _ in range(int(n)):
             self.substep(dt, damping)
 
