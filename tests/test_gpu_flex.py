@@ -1,0 +1,162 @@
+"""FEM-FEM contact on the GPU against the oracle, bit for bit.
+
+Loops run through ``ExplicitLoop.advanceWithTime``; the oracle (OracleLoop) gets
+the same packed scenes, the same contact group and the same float64 dt sequence,
+so positions, velocities and every work item's cached segment must agree
+exactly.  Cases: the reference's own two-circle test
+(test2D/test_2Dfem_contact.py), stacked boxes on a plane (the gated path, ties
+between collinear edges, friction), three bodies of three materials on a height
+field, and bodies spread over many tiles.
+
+FEM-FEM contact at the reference's adaptive step diverges as soon as both
+bodies press on each other through two work items (the contact stable time
+counts one penalty spring), in the oracle exactly as on the device, so the
+multi-body cases below step at a fixed fraction of it.
+"""
+
+import numpy as np
+import pytest
+
+import flatworld_amd as fw
+from flatworld_amd import _native
+from oracle.oracle import OracleLoop
+from tests.parity import log
+
+pytestmark = pytest.mark.gpu
+
+FRAME = 1.0 / 60.0
+
+
+def ccw_grid(side, lower, extent, mat, bcs=(), initials=(), jitter=0.0, seed=0, friction=0.0):
+    conn, coords = fw.structured_grid(side, lower=lower, extent=extent, jitter=jitter, seed=seed)
+    conn = np.ascontiguousarray(conn[:, [0, 2, 1]])
+    return fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(mat), list(bcs), initials=list(initials),
+                        friction=friction)
+
+
+def oracle_of(loop):
+    m = loop.femSpringManager
+    return OracleLoop(m._specs, m._flex)
+
+
+def assert_same(loop, o, what):
+    m = loop.femSpringManager
+    for k, body in enumerate(o.bodies):
+        x = m.domain_coords(k)
+        v = m._arena.read(m._scene_ids[k], _native.FIELD_V)
+        assert np.array_equal(x, body.x), f"{what}: positions of body {k} differ (max {np.abs(x - body.x).max():.3e})"
+        assert np.array_equal(v, body.v), f"{what}: velocities of body {k} differ"
+    if o.flex is not None:
+        assert np.array_equal(m.flex_cache(), o.flex.cache), f"{what}: cached segments differ"
+
+
+def run_frames(loop, o, frames, check_every=1, what=""):
+    contact = 0
+    for f in range(frames):
+        loop.advanceWithTime(FRAME)
+        for sd in loop.substep_sequence(FRAME):
+            o.substep(sd, loop.damping)
+        contact += int((o.flex.cache >= 0).sum())
+        if (f + 1) % check_every == 0 or f == frames - 1:
+            assert_same(loop, o, f"{what} frame {f + 1}")
+    return contact
+
+
+def test_reference_two_circles_bitexact():
+    """test2D/test_2Dfem_contact.py: two 16-segment circles under opposite gravity."""
+    msh = fw.FEMesher(2)
+    prop = fw.SolidProp(fw.Elastic(E=2e4, nu=0.2, rho=40.0))
+    d1 = fw.FemDomain(msh.createCircle([0.5, 0.5], 0.1), prop, [fw.Gravity([0, 10.0])])
+    d2 = fw.FemDomain(msh.createCircle([0.5, 0.8], 0.1), prop, [fw.Gravity([0, -10.0])])
+    loop = fw.ExplicitLoop(0.0, [d1, d2], useAdapativeDT=True)
+    log(f"[case] two circles, dt={loop.dt:.3e}, items={loop.femSpringManager._flex.n_items}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 60, check_every=5, what="two circles")
+    assert contact > 0
+
+
+def test_hash_build_matches_oracle():
+    mat = fw.Elastic(1e4, 0.3, 10.0)
+    a = ccw_grid(8, (0.0, 0.0), 0.3, mat, [fw.Gravity([0, -5.0])], jitter=0.2, seed=1)
+    b = ccw_grid(6, (0.05, 0.305), 0.2, mat, [fw.Gravity([0, -5.0])], jitter=0.2, seed=2)
+    loop = fw.ExplicitLoop(0.0, [a, b], useAdapativeDT=True)
+    o = oracle_of(loop)
+    m = loop.femSpringManager
+    for rnd in range(3):
+        gd = m._arena.flex_read(m._flex_gid, _native.FLEX_GRID)
+        go = o.flex.grid
+        log(f"[case] hash round {rnd}: cells {gd['cn']}, used {gd['tc']}")
+        assert np.array_equal(gd["glb"], go["glb"]) and np.array_equal(gd["gs"], go["gs"])
+        assert np.array_equal(gd["cn"], go["cn"]) and gd["tc"] == go["tc"] and gd["exact"]
+        cs = m._arena.flex_read(m._flex_gid, _native.FLEX_CELL_START)[: gd["tc"]]
+        ce = m._arena.flex_read(m._flex_gid, _native.FLEX_CELL_END)[: gd["tc"]]
+        srt = m._arena.flex_read(m._flex_gid, _native.FLEX_SORTED)
+        assert np.array_equal(cs, o.flex.cell_start[: go["tc"]]) and np.array_equal(ce, o.flex.cell_end[: go["tc"]])
+        assert np.array_equal(srt[: ce.max()], o.flex.sorted_seg[: ce.max()])
+        run_frames(loop, o, 2, what=f"hash round {rnd}")  # several rebuilds per frame
+
+
+def test_stacked_boxes_on_plane_bitexact():
+    """A box dropped on a box resting on a plane: the BVH-gated path, aligned grids
+    whose nodes sit exactly over edge ends (penetration ties), and friction."""
+    mat = fw.Elastic(2e4, 0.3, 50.0)
+    lower = ccw_grid(6, (0.2, 0.0), 0.3, mat, [fw.Gravity([0, -9.8])], friction=0.3)
+    upper = ccw_grid(6, (0.2, 0.31), 0.3, mat, [fw.Gravity([0, -9.8])], friction=0.1)
+    plane = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    loop = fw.ExplicitLoop(3.5e-4, [lower, upper, plane], useAdapativeDT=False, damping=0.5)  # half the stable step
+    g = loop.femSpringManager._flex
+    assert g.gated
+    log(f"[case] stacked boxes on a plane, dt={loop.dt:.3e}, items={g.n_items}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 20, check_every=4, what="stacked boxes")
+    assert contact > 0
+
+
+def test_three_materials_on_height_field_bitexact():
+    hf_h = (0.05 + 0.03 * np.sin(np.linspace(0.0, 6.0, 65))).astype(np.float32)
+    hf = fw.HeightFieldDomain(2, hf_h, lb=[0.0, 0.0], ub=[1.0, 1.0])
+    a = ccw_grid(8, (0.1, 0.1), 0.25, fw.Elastic(3e4, 0.3, 60.0), [fw.Gravity([0, -9.8])], jitter=0.15, seed=3)
+    b = ccw_grid(7, (0.3, 0.37), 0.22, fw.HyperElastic(2e4, 0.35, 40.0), [fw.Gravity([0, -9.8])], jitter=0.15,
+                 seed=4, initials=[fw.InitialVel("ALL", [-0.2, -0.5])])
+    c = ccw_grid(6, (0.12, 0.62), 0.2, fw.MisesPlastic(2e4, 0.3, 50.0, 300.0, 1e3), [fw.Gravity([0, -9.8])],
+                 jitter=0.1, seed=5, friction=0.2)
+    loop = fw.ExplicitLoop(1e-4, [a, b, c, hf], useAdapativeDT=False)  # 0.38 x the stable step
+    g = loop.femSpringManager._flex
+    assert g.gated and g.n_pairs == 3
+    log(f"[case] three materials on a height field, dt={loop.dt:.3e}, items={g.n_items}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 10, check_every=5, what="three materials")
+    assert contact > 0
+
+
+def test_bodies_over_many_tiles_bitexact():
+    """40 x 40 and 30 x 30 bodies cut into 128-node tiles: internal node order
+    differs from the caller's, halos cross the contact zone."""
+    mat = fw.Elastic(1e4, 0.3, 20.0)
+    a = ccw_grid(40, (0.0, 0.0), 0.4, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=7)
+    b = ccw_grid(30, (0.1, 0.401), 0.3, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=8,
+                 initials=[fw.InitialVel("ALL", [0.0, -0.8])])
+    loop = fw.ExplicitLoop(3e-5, [a, b], useAdapativeDT=False,  # 0.3 x the stable step
+                           arena_kwargs=dict(tile_nodes=128, tile_slots=1024))
+    assert loop.femSpringManager.arena_info()["n_tiles"] > 10
+    log(f"[case] many tiles, dt={loop.dt:.3e}, items={loop.femSpringManager._flex.n_items}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 3, check_every=1, what="many tiles")
+    assert contact > 0
+
+
+def test_group_refuses_graph_and_scene_removal():
+    mat = fw.Elastic(1e4, 0.3, 10.0)
+    a = ccw_grid(4, (0.0, 0.0), 0.2, mat)
+    b = ccw_grid(4, (0.0, 0.21), 0.2, mat)
+    loop = fw.ExplicitLoop(0.0, [a, b], useAdapativeDT=True)
+    m = loop.femSpringManager
+    with pytest.raises(NotImplementedError, match="fw_step"):
+        m._arena.step_graph(4)
+    with pytest.raises(ValueError, match="remove the group first"):
+        m._arena.remove(m._scene_ids[0])
+    m._arena.flex_remove(m._flex_gid)
+    m._flex_gid = None
+    m._arena.step_graph(4)
+    m._arena.remove(m._scene_ids[0])
+    m._arena.sync()
