@@ -26,6 +26,9 @@ Printed JSON line:
   springs          4096 spring lattices (side log-uniform in [8, 128], one height field
                    each) batched the same way: spring-substeps/s, the spring substep's
                    hbm_frac, its own autotuned shape and a CPU baseline on a sample
+  contact          2048 two-block loops with FEM-FEM contact on a plane: element-
+                   substeps/s with the contact pass and the hash rebuilds included,
+                   the contact + substep kernel time and a CPU baseline on a sample
   cpu_baseline     the oracle (serial C restatement) on a bounded sample
 """
 
@@ -293,6 +296,113 @@ def spring_bench(args):
     return out
 
 
+def contact_bench(args):
+    """Batched two-block loops with FEM-FEM contact: per substep one contact-pass launch
+    (one workgroup per loop) and the substep launch; the hash of every loop is rebuilt
+    in one launch every 1/500 s of simulated time, as each loop's schedule has it."""
+    from flatworld_amd import _native
+    from flatworld_amd import workloads as W
+    from flatworld_amd.flexcontact import REBUILD_INTERVAL
+
+    t = time.perf_counter()
+    groups = W.contact_batched_config(n_groups=args.contact_groups, seed=args.seed)
+    specs = [s for sp, _ in groups for s in sp]
+    E = sum(s.n_elems for s in specs)
+    N = sum(s.n_nodes for s in specs)
+    items = sum(g.n_items for _, g in groups)
+    slots = sum(int(g.body_bnodes.shape[0]) for _, g in groups)
+    dt = specs[0].dt
+    every = max(1, int(np.ceil(REBUILD_INTERVAL / dt - 1e-9)))
+    algo_sub = float(sum(W.algorithmic_bytes(s) for s in specs))
+    algo_contact = float(items * W.CONTACT_ITEM_BYTES + slots * W.CONTACT_SLOT_BYTES)
+    arena = _native.Arena(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
+    for sp, g in groups:
+        arena.flex_add(g, [arena.add(s, track_disp=True) for s in sp])
+    arena.step(1, -1.0, 0)  # zero-dt warm-up of every scene
+    arena.flex_rebuild(-1)
+    arena.sync()
+
+    def run(n):
+        done = 0
+        while done < n:
+            k = min(every, n - done)
+            arena.step(k, -1.0, 0)
+            arena.flex_rebuild(-1)
+            done += k
+
+    # timed by the wall clock: the first contact_steps substeps after release
+    # (approach and impact), then two rebuild intervals with the blocks in contact;
+    # then the contact pass + substep kernels with HIP events over one interval
+    t0 = time.perf_counter()
+    run(args.contact_steps)
+    arena.sync()
+    total_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    run(2 * every)
+    arena.sync()
+    contact_s = time.perf_counter() - t0
+    _, kernel_ms = arena.time_steps(every, mode=0)
+    t1 = time.perf_counter()
+    arena.flex_rebuild(-1)
+    arena.sync()
+    rebuild_ms = (time.perf_counter() - t1) * 1e3
+    info = arena.info()
+    arena.close()
+    achieved = (algo_sub + algo_contact) / (kernel_ms * 1e-3) / 1e9  # in contact
+    log(f"[bench] contact: {len(groups)} loops, elements={E}, nodes={N}, work items={items}, tiles={info['n_tiles']}, "
+        f"{args.contact_steps} substeps in {total_s * 1e3:.1f} ms, in contact {contact_s * 1e6 / (2 * every):.1f} us/substep, "
+        f"contact+substep kernels {kernel_ms * 1e3:.1f} us, "
+        f"rebuild {rebuild_ms:.2f} ms every {every} substeps ({time.perf_counter() - t:.1f}s with setup)")
+    out = {
+        "value": E * args.contact_steps / total_s,
+        "unit": "element-substeps/s",
+        "workload": f"contact-{len(groups)}: two-block loops (Elastic / J2, side log-uniform in [8,32]) on a plane, "
+                    "the upper block falling onto the lower; FEM-FEM contact with the spatial hash",
+        "loops": len(groups),
+        "elements": E,
+        "nodes": N,
+        "work_items": items,
+        "dt": dt,
+        "rebuild_every": every,
+        "ms_per_step": total_s * 1e3 / args.contact_steps,
+        "in_contact_ms_per_step": contact_s * 1e3 / (2 * every),
+        "window": f"value, ms_per_step: substeps 1-{args.contact_steps} after release; in_contact_ms_per_step: the "
+                  f"next {2 * every} (rebuilds included); kernel_ms: the {every} after those (HIP events)",
+        "kernel_ms": kernel_ms,
+        "rebuild_ms": rebuild_ms,
+        "algo_bytes_per_substep": algo_sub + algo_contact,
+        "algo_bytes_contact": algo_contact,
+        "hbm_frac": achieved / HBM_PEAK_GBS,
+        "achieved_GBps": achieved,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = contact_cpu_baseline(groups, args.cpu_budget / 3.0)
+    return out
+
+
+def contact_cpu_baseline(groups, budget_s):
+    """The oracle (serial C, contact pass included) on a bounded sample of the loops."""
+    from oracle.oracle import OracleLoop
+
+    order = np.random.Generator(np.random.PCG64(1)).permutation(len(groups))
+    sample, elems = [], 0
+    for i in order:
+        sample.append(groups[i])
+        elems += sum(s.n_elems for s in groups[i][0])
+        if elems >= 100_000:
+            break
+    loops = [OracleLoop(sp, g) for sp, g in sample]
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < budget_s:
+        for o, (sp, _) in zip(loops, sample):
+            o.substep(sp[0].dt)
+        n += 1
+    rate = elems * n / (time.perf_counter() - t0)
+    return {"value": rate, "unit": "element-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{len(sample)} of {len(groups)} loops ({elems} elements) x {n} substeps, serial C oracle"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -317,6 +427,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph frames")
     ap.add_argument("--spring-scenes", type=int, default=4096, help="spring lattices of the spring leg (0: skip)")
     ap.add_argument("--spring-steps", type=int, default=600)
+    ap.add_argument("--contact-groups", type=int, default=2048,
+                    help="two-block loops of the FEM-FEM contact leg (0: skip)")
+    ap.add_argument("--contact-steps", type=int, default=600)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -380,6 +493,10 @@ def main():
     if args.spring_scenes > 0 and args.config == "batched":
         springs = spring_bench(args)
 
+    contact = None
+    if args.contact_groups > 0 and args.config == "batched":
+        contact = contact_bench(args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget)
@@ -423,6 +540,7 @@ def main():
         "per_scene": per_scene,
         "serving": serving,
         "springs": springs,
+        "contact": contact,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -41,8 +41,8 @@ extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
 extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
                                              float dt_target, int block, int slots, hipStream_t stream);
-extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, const SceneDev* scenes, const ArenaCtl* ctl, int gid,
-                                          hipStream_t stream);
+extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
+                                          const ArenaCtl* ctl, int row, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const SceneDev* scenes,
                                              const ArenaCtl* ctl, int k, uint32_t flags, hipStream_t stream);
 extern "C" int fw_flex_max_pairs_per_group(void);
@@ -1705,6 +1705,8 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
         (W > 0 && (!d->item_body || !d->item_node || !d->item_target || !d->item_pair || !d->item_penalty ||
                    !d->item_friction)))
         return fail(FW_E_INVALID, "group arrays are required");
+    if (B > 256 || S >= (1 << 21))
+        return fail(FW_E_UNSUPPORTED, "a group holds at most 256 bodies and 2^21 - 1 boundary segments");
     if (P > fw_flex_max_pairs_per_group())
         return fail(FW_E_UNSUPPORTED, "a group holds at most " + std::to_string(fw_flex_max_pairs_per_group()) + " pairs");
     if (d->max_cells < 1 || d->max_query < 1 || d->max_pairs < 1) return fail(FW_E_INVALID, "hash limits must be >= 1");
@@ -1809,9 +1811,11 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
                   o_is = take(4 * W), o_it = take(4 * W), o_ip = take(4 * W), o_ipen = take(4 * W),
                   o_ifr = take(4 * W), o_cache = take(4 * W), o_pair = take(8 * P);
     const int64_t host_bytes = o;
-    const int64_t o_grid = take(sizeof(FlexGrid)), o_box = take(16 * S), o_segp = take(4 * S),
+    const int64_t o_grid = take(sizeof(FlexGrid)), o_box = take(16 * S), o_scell = take(16 * S), o_segp = take(4 * S),
+                  o_crng = take(8 * (int64_t)d->max_cells),
                   o_cs = take(4 * (int64_t)d->max_cells), o_ce = take(4 * (int64_t)d->max_cells),
-                  o_sorted = take(4 * (int64_t)d->max_pairs), o_ri = take(8 * W), o_rf = take(16 * W),
+                  o_sorted = take(4 * (int64_t)d->max_pairs), o_cent = take(4 * (int64_t)d->max_pairs),
+                  o_ri = take(8 * W), o_rf = take(16 * W),
                   o_slst = take(4 * NS), o_slcur = take(4 * NS), o_ek = take(12 * (int64_t)W), o_ef = take(24 * (int64_t)W);
     const int64_t total = o;
     const int64_t off = arena_alloc(a, total);
@@ -1890,10 +1894,13 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     D.pair_body = (const int2*)(blk + o_pair);
     D.grid = (FlexGrid*)(blk + o_grid);
     D.seg_box = (float4*)(blk + o_box);
+    D.seg_cells = (int4*)(blk + o_scell);
+    D.cell_rng = (int2*)(blk + o_crng);
     D.seg_pairs = (int32_t*)(blk + o_segp);
     D.cell_start = (int32_t*)(blk + o_cs);
     D.cell_end = (int32_t*)(blk + o_ce);
     D.sorted = (int32_t*)(blk + o_sorted);
+    D.cell_ent = (int32_t*)(blk + o_cent);
     D.res_i = (int2*)(blk + o_ri);
     D.res_f = (float4*)(blk + o_rf);
     D.slot_start = (int32_t*)(blk + o_slst);
@@ -1918,16 +1925,22 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
 }
 
 int32_t fw_flex_rebuild(fw_arena* a, int32_t gid) {
-    if (!a || gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]) return fail(FW_E_NOT_FOUND, "no FEM-FEM group");
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    if (gid != -1 && (gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]))
+        return fail(FW_E_NOT_FOUND, "no FEM-FEM group " + std::to_string(gid));
     HIPCHK(hipSetDevice(a->device));
     int rc = ensure_tiles(a);
     if (rc) return rc;
     rc = ensure_flex(a);
     if (rc) return rc;
-    const SceneHost* s0 = get_scene(a, a->flex[gid]->scenes[0]);
-    if (!s0 || a->clock < s0->s0)
-        return fail(FW_E_INVALID, "the group's scenes have not run their warm-up substep yet");
-    HIPCHK(fw_launch_flex_hash(a->d_flex, a->d_scenes, a->d_ctl, a->flex_row[gid], a->stream));
+    for (size_t g = 0; g < a->flex.size(); ++g) {
+        if (!a->flex[g] || (gid >= 0 && (int)g != gid)) continue;
+        const SceneHost* s0 = get_scene(a, a->flex[g]->scenes[0]);
+        if (!s0 || a->clock < s0->s0)
+            return fail(FW_E_INVALID, "group " + std::to_string(g) + " has not run its warm-up substep yet");
+    }
+    HIPCHK(fw_launch_flex_hash(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, gid >= 0 ? a->flex_row[gid] : -1,
+                               a->stream));
     return FW_OK;
 }
 

@@ -13,9 +13,10 @@
 // records, as the reference launches its contact kernels
 // (flatworld/mixedcontact.py:1654-1754).
 //
-// Phases hand data between threads of the workgroup through global memory, each
-// behind an agent-scope fence and a workgroup barrier.  Integer atomics only
-// count and reserve places; every order-sensitive result is then sorted.
+// Phases hand data between threads of the workgroup through global memory behind
+// __syncthreads(), whose workgroup-scope release and acquire fences cover global
+// memory: every wave of a workgroup runs on one CU.  Integer atomics only count
+// and reserve places; every order-sensitive result is then sorted.
 #include <hip/hip_runtime.h>
 
 #include "fw_types.h"
@@ -23,7 +24,7 @@
 namespace fw {
 namespace {
 
-constexpr int FLEX_BLOCK = 1024;
+constexpr int FLEX_BLOCK = 256;  // a group's work items and slots are looped over in strides
 constexpr int FLEX_MAX_PAIRS = 4096;  // pairs of one group (LDS activation flags)
 constexpr uint32_t FLEX_NO_CONTACT = 2u;  // FW_STEP_NO_CONTACT
 
@@ -38,10 +39,7 @@ __device__ __forceinline__ int mod3_(int k) {
     return r < 0 ? r + 3 : r;
 }
 
-__device__ __forceinline__ void phase_barrier() {
-    __threadfence();
-    __syncthreads();
-}
+__device__ __forceinline__ void phase_barrier() { __syncthreads(); }
 
 // pointToEdgeContact (flatworld/contact_detection.py:15-61); returns is_inside.
 __device__ __forceinline__ bool point_edge(float px, float py, float ax, float ay, float bx, float by, float& pen,
@@ -142,9 +140,10 @@ __device__ int scan_global(int32_t* a, int32_t* copy, int n, int* lds) {
 
 // populate_fem_spatial_hash + SpatialHashManager.build at the state the next
 // substep starts from (velocity buffer 0).
+// One workgroup per group: the group at table row `row`, or row blockIdx.x when row < 0.
 __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* groups, const SceneDev* scenes,
-                                                               const ArenaCtl* ctl, int gid) {
-    const FlexDev& F = groups[gid];
+                                                               const ArenaCtl* ctl, int row) {
+    const FlexDev& F = groups[row >= 0 ? row : (int)blockIdx.x];
     __shared__ int lds[FLEX_BLOCK];
     __shared__ float sh_bb[4];
     __shared__ int sh_est;
@@ -232,6 +231,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
     for (int s = tid; s < ns; s += FLEX_BLOCK) {
         const float4 box = F.seg_box[s];
         const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        F.seg_cells[s] = make_int4(r.lx, r.ly, r.ux, r.uy);
         int cnt = 0;
         for (int ix = r.lx; ix <= r.ux; ++ix)
             for (int iy = r.ly; iy <= r.uy; ++iy) cnt += ix + iy * g.cn[0] < g.tc;
@@ -242,11 +242,10 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
     const int total = scan_global(F.seg_pairs, nullptr, ns, lds);
     phase_barrier();
     for (int s = tid; s < ns; s += FLEX_BLOCK) {
-        const float4 box = F.seg_box[s];
-        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        const int4 r = F.seg_cells[s];
         int pi = F.seg_pairs[s];
-        for (int ix = r.lx; ix <= r.ux; ++ix)
-            for (int iy = r.ly; iy <= r.uy; ++iy) {
+        for (int ix = r.x; ix <= r.z; ++ix)
+            for (int iy = r.y; iy <= r.w; ++iy) {
                 const int cid = ix + iy * g.cn[0];
                 if (cid < g.tc) {
                     if (pi < F.max_pairs) atomicAdd(&F.cell_end[cid], 1);
@@ -258,11 +257,10 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
     scan_global(F.cell_end, F.cell_start, g.tc, lds);
     phase_barrier();
     for (int s = tid; s < ns; s += FLEX_BLOCK) {
-        const float4 box = F.seg_box[s];
-        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
+        const int4 r = F.seg_cells[s];
         int pi = F.seg_pairs[s];
-        for (int ix = r.lx; ix <= r.ux; ++ix)
-            for (int iy = r.ly; iy <= r.uy; ++iy) {
+        for (int ix = r.x; ix <= r.z; ++ix)
+            for (int iy = r.y; iy <= r.w; ++iy) {
                 const int cid = ix + iy * g.cn[0];
                 if (cid < g.tc) {
                     if (pi < F.max_pairs) F.sorted[atomicAdd(&F.cell_end[cid], 1)] = s;
@@ -283,6 +281,13 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
             }
             F.sorted[j + 1] = v;
         }
+        F.cell_rng[c] = make_int2(b, e);
+        const int cx = c % g.cn[0], cy = c / g.cn[0];
+        for (int i = b; i < e; ++i) {
+            const int sg = F.sorted[i];
+            const int4 r = F.seg_cells[sg];
+            F.cell_ent[i] = sg << 10 | F.seg_body[sg] << 2 | (cx == r.x) << 1 | (cy == r.y);
+        }
     }
     if (tid == 0) {
         FlexGrid out = g;
@@ -298,22 +303,23 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
 __device__ __forceinline__ bool seen_before(const FlexDev& F, const FlexGrid& g, const CellRange& q, int s, int ix,
                                             int iy, int pos) {
     if (g.exact) {
-        const float4 box = F.seg_box[s];
-        const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
-        return !(ix == imax_(r.lx, q.lx) && iy == imax_(r.ly, q.ly));
+        const int4 r = F.seg_cells[s];
+        return !(ix == imax_(r.x, q.lx) && iy == imax_(r.y, q.ly));
     }
     for (int jx = q.lx; jx <= q.ux; ++jx)
         for (int jy = q.ly; jy <= q.uy; ++jy) {
             const int cid = jx + jy * g.cn[0];
             if (cid < 0 || cid >= g.tc) continue;
-            const int e = F.cell_end[cid];
-            for (int p = F.cell_start[cid]; p < e; ++p) {
+            const int2 cr = F.cell_rng[cid];
+            for (int p = cr.x; p < cr.y; ++p) {
                 if (jx == ix && jy == iy && p == pos) return false;
                 if (F.sorted[p] == s) return true;
             }
         }
     return false;
 }
+
+constexpr int WALK_BATCH = 8;  // cells whose ranges are loaded together in a query walk
 
 // _batched_flexflex_contact_kernel_sh (flatworld/mixedcontact.py:886-1025) for
 // every work item of the group, then the node-major contribution lists the
@@ -416,15 +422,29 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
                 const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
                 int n = 0;
                 int cache_new = F.cache[w];
-                for (int ix = q.lx; ix <= q.ux; ++ix)
-                    for (int iy = q.ly; iy <= q.uy; ++iy) {
-                        const int cid = ix + iy * g.cn[0];
-                        if (cid < 0 || cid >= g.tc) continue;
-                        const int e = F.cell_end[cid];
-                        for (int pp = F.cell_start[cid]; pp < e; ++pp) {
-                            const int s = F.sorted[pp];
-                            if (s < 0 || F.seg_body[s] != tb) continue;
-                            if (n >= F.max_query || seen_before(F, g, q, s, ix, iy, pp)) continue;
+                // cells in walk order (x outer, y inner), their ranges loaded a batch at a time
+                const int nyq = q.uy - q.ly + 1, ncell = (q.ux - q.lx + 1) * nyq;
+                for (int t0 = 0; t0 < ncell; t0 += WALK_BATCH) {
+                    int2 cr[WALK_BATCH];
+#pragma unroll
+                    for (int u = 0; u < WALK_BATCH; ++u) {
+                        const int t = t0 + u;
+                        const int cid = (q.lx + t / nyq) + (q.ly + t % nyq) * g.cn[0];
+                        cr[u] = (t < ncell && cid >= 0 && cid < g.tc) ? F.cell_rng[cid] : make_int2(0, 0);
+                    }
+                    for (int u = 0; u < WALK_BATCH && t0 + u < ncell; ++u) {
+                        const int ix = q.lx + (t0 + u) / nyq, iy = q.ly + (t0 + u) % nyq;
+                        const int e = cr[u].y;
+                        for (int pp = cr[u].x; pp < e; ++pp) {
+                            const int ce = F.cell_ent[pp];
+                            if (((ce >> 2) & 0xFF) != tb) continue;
+                            const int s = ce >> 10;
+                            // first meeting of s in the walk: the first cell of the walk inside
+                            // the segment's box, i.e. its first column or the walk's, and its
+                            // first row or the walk's
+                            const bool seen = g.exact ? !((ix == q.lx || (ce & 2)) && (iy == q.ly || (ce & 1)))
+                                                      : seen_before(F, g, q, s, ix, iy, pp);
+                            if (n >= F.max_query || seen) continue;
                             ++n;
                             const int2 sn = F.seg_node[s];
                             float nx, ny, w0, w1;
@@ -441,6 +461,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
                             }
                         }
                     }
+                }
                 F.cache[w] = cache_new;
             }
             if (!found) F.cache[w] = -1;
@@ -519,9 +540,12 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
 }  // namespace
 }  // namespace fw
 
-extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, const SceneDev* scenes, const ArenaCtl* ctl, int gid,
-                                          hipStream_t stream) {
-    hipLaunchKernelGGL(fw::flex_hash_kernel, dim3(1), dim3(fw::FLEX_BLOCK), 0, stream, groups, scenes, ctl, gid);
+// row >= 0: that table row; row < 0: every one of the n_groups rows, one workgroup each.
+extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
+                                          const ArenaCtl* ctl, int row, hipStream_t stream) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fw::flex_hash_kernel, dim3(row >= 0 ? 1 : n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups,
+                       scenes, ctl, row);
     return hipGetLastError();
 }
 

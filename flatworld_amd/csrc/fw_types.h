@@ -75,10 +75,15 @@ struct FlexDev {
     // spatial hash, rebuilt by flex_hash_kernel
     FlexGrid* grid;
     float4* seg_box;               // [S] segment boxes of the last build
+    int4* seg_cells;               // [S] their clamped cell ranges (lx, ly, ux, uy)
     int32_t* seg_pairs;            // [S] scratch: pairs per segment, then the serial pair base
     int32_t* cell_start;           // [max_cells]
     int32_t* cell_end;             // [max_cells]
+    int2* cell_rng;                // [max_cells] (start, end) together: one load per cell in a query
     int32_t* sorted;               // [max_pairs] segment ids, ascending within a cell
+    int32_t* cell_ent;             // [max_pairs] the same entries packed for the query walk:
+                                   // segment << 10 | body << 2 | (cell is the segment's first
+                                   // column) << 1 | (first row)
     // per-substep scratch of flex_contact_kernel
     int2* res_i;                   // [W] (hit, segment)
     float4* res_f;                 // [W] (force x, y, w0, w1)

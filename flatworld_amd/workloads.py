@@ -126,8 +126,8 @@ MIX_RHO = 100.0
 MIX_Y0 = 0.1
 
 
-def random_material(rng, rho=MIX_RHO):
-    kind = [MaterialType.ELASTIC, MaterialType.NEOHOOKEAN, MaterialType.MISES][int(rng.integers(0, 3))]
+def random_material(rng, rho=MIX_RHO, kinds=(MaterialType.ELASTIC, MaterialType.NEOHOOKEAN, MaterialType.MISES)):
+    kind = kinds[int(rng.integers(0, len(kinds)))]
     E = float(rng.uniform(1e4, 1e5))
     nu = float(rng.uniform(0.2, 0.4))
     return make_material(kind, E, nu, rho, 5e-3 * E, 1e-2 * E)
@@ -196,6 +196,66 @@ def spring_batched_config(n_scenes=4096, seed=0, side_lo=8, side_hi=128, spacing
         dom = spring_lattice(s, (0.5 - 0.5 * width, MIX_Y0), spacing, [k, c, m], [Gravity([0.0, -9.8])])
         specs.append(one_body_spec(dom, sinusoid_heightfield(rng)))
     return specs
+
+
+def contact_pair_group(rng, side_lo=8, side_hi=32, dt_fraction=0.35):
+    """Two stacked blocks on a plane in one loop: the lower one resting on the plane,
+    the upper one falling onto it with a random offset; both with counter-clockwise
+    elements (outward boundary normals), a random material each and friction.
+    Returns (packed scenes, FEM-FEM group) as ExplicitLoop would build them, with
+    dt at ``dt_fraction`` of the loop's stable time (the reference's adaptive step
+    lets two bodies pressing on each other through two work items diverge)."""
+    from .definitions import ContactType
+    from .femcontact import ContactFlexFlex
+    from .flexcontact import build_flex_group
+    from .scene import add_flex_record
+
+    def block(side, lower, extent, mat, initials=(), friction=0.0):
+        conn, coords = structured_grid(side, lower=lower, extent=extent)
+        conn = np.ascontiguousarray(conn[:, [0, 2, 1]])
+        return FemDomain(Mesh(2, conn, coords), SolidProp(mat), [Gravity([0.0, -9.8])], initials=list(initials),
+                         friction=friction)
+
+    s1 = int(round(side_lo * (side_hi / side_lo) ** rng.uniform()))
+    s2 = int(round(side_lo * (side_hi / side_lo) ** rng.uniform()))
+    w1, w2 = 0.3, float(rng.uniform(0.15, 0.3))
+    # Elastic and J2: an impact can invert elements, where Neo-Hookean stress is NaN (log J < 0)
+    kinds = (MaterialType.ELASTIC, MaterialType.MISES)
+    lower = block(s1, (0.2, 0.0), w1, random_material(rng, kinds=kinds), friction=float(rng.uniform(0.0, 0.4)))
+    upper = block(s2, (0.2 + float(rng.uniform(-0.1, 0.2)), w1 + 0.002), w2, random_material(rng, kinds=kinds),
+                  initials=[InitialVel("ALL", [0.0, -float(rng.uniform(0.1, 0.5))])])
+    plane = GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    pair = ContactFlexFlex(upper, lower, ContactType.FLEXFLEX)
+    grounds = {k: [(plane, contactHelper(plane, b, plane.type | b.type))] for k, b in enumerate((lower, upper))}
+    group = build_flex_group([lower, upper], [pair], True, lower.nnodes + upper.nnodes)
+    st = min([1.0, 1e-3, lower.stableTime, upper.stableTime, pair.stableTime] + [c.stableTime for (_, c), in grounds.values()])
+    dt = dt_fraction * st
+    specs = []
+    for k, b in enumerate((lower, upper)):
+        spec = add_flex_record(pack_domain(b, grounds[k], dt=dt), group.flex_cap(k))
+        specs.append(spec)
+    return specs, group
+
+
+def contact_batched_config(n_groups=1024, seed=0, side_lo=8, side_hi=32):
+    """Many independent two-block loops with FEM-FEM contact: a list of (scenes, group).
+    Every loop takes the smallest loop dt, so that all share one hash-rebuild schedule."""
+    rng = rng_for(seed)
+    groups = [contact_pair_group(rng, side_lo, side_hi) for _ in range(n_groups)]
+    dt = min(s.dt for specs, _ in groups for s in specs)
+    for specs, _ in groups:
+        for s in specs:
+            s.dt = dt
+    return groups
+
+
+# Algorithmic bytes of the FEM-FEM contact pass per substep: per work item its record
+# (body, node, slot, target, pair, penalty, friction: 28 B), its cache read and
+# written (8 B) and the node's position and velocity (16 B); per slot the (first,
+# count) range written (8 B).  Candidate segments and contributions depend on the
+# contact state and are not counted.
+CONTACT_ITEM_BYTES = 52
+CONTACT_SLOT_BYTES = 8
 
 
 def algorithmic_bytes(spec):

@@ -263,7 +263,8 @@ int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, in
 /* FEM-FEM contact groups.  fw_flex_add needs every body in a pair to carry one
  * FW_GROUND_FLEX record of sufficient capacity; its scenes cannot be removed
  * before the group.  fw_flex_rebuild enqueues the spatial-hash build at the state
- * the next substep starts from (after the scenes' warm-up); the caller keeps the
+ * the next substep starts from (after the scenes' warm-up), for one group or, with
+ * group_id -1, for every group in one launch; the caller keeps the
  * reference's rebuild schedule (every 1/500 s of simulated time,
  * flatworld/femspringmanager.py:1035-1039).  fw_step then runs the group's
  * contact pass before every substep.  fw_step_graph and FW_STEP_PER_SCENE are
