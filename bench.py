@@ -322,13 +322,23 @@ def contact_bench(args):
     arena.flex_rebuild(-1)
     arena.sync()
 
+    elapsed = [0.0]
+
     def run(n):
-        done = 0
-        while done < n:
-            k = min(every, n - done)
-            arena.step(k, -1.0, 0)
-            arena.flex_rebuild(-1)
-            done += k
+        # the reference's schedule: a rebuild precedes the substep at which 1/500 s
+        # of simulated time has passed (flatworld/femspringmanager.py:1035-1039)
+        pending = 0
+        for _ in range(n):
+            elapsed[0] += dt
+            if elapsed[0] >= REBUILD_INTERVAL:
+                if pending:
+                    arena.step(pending, -1.0, 0)
+                    pending = 0
+                arena.flex_rebuild(-1)
+                elapsed[0] = 0.0
+            pending += 1
+        if pending:
+            arena.step(pending, -1.0, 0)
 
     # timed by the wall clock: the first contact_steps substeps after release
     # (approach and impact), then two rebuild intervals with the blocks in contact;

@@ -160,3 +160,53 @@ def test_group_refuses_graph_and_scene_removal():
     m._arena.step_graph(4)
     m._arena.remove(m._scene_ids[0])
     m._arena.sync()
+
+
+def test_contact_workload_sampled_loops_bitexact():
+    """The bench's 2048-loop contact workload in one arena, 600 substeps on the
+    reference's rebuild schedule (one launch rebuilds every loop's hash); 24
+    sampled loops against the oracle: positions, velocities and caches."""
+    from flatworld_amd import workloads as W
+    from flatworld_amd.flexcontact import REBUILD_INTERVAL
+
+    groups = W.contact_batched_config(n_groups=2048, seed=0)
+    dt = groups[0][0][0].dt
+    arena = _native.Arena(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
+    sids, gids = [], []
+    for sp, g in groups:
+        s = [arena.add(x) for x in sp]
+        sids.append(s)
+        gids.append(arena.flex_add(g, s))
+    arena.step(1, -1.0, 0)
+    arena.flex_rebuild(-1)
+    elapsed, pending = 0.0, 0
+    for _ in range(600):
+        elapsed += dt
+        if elapsed >= REBUILD_INTERVAL:
+            if pending:
+                arena.step(pending, -1.0, 0)
+                pending = 0
+            arena.flex_rebuild(-1)
+            elapsed = 0.0
+        pending += 1
+    arena.step(pending, -1.0, 0)
+    arena.sync()
+    sample = np.random.default_rng(5).choice(len(groups), 24, replace=False)
+    hits = 0
+    for gi in sample:
+        sp, g = groups[gi]
+        o = OracleLoop(sp, g)
+        for _ in range(600):
+            o.substep(dt)
+        for k, body in enumerate(o.bodies):
+            assert np.all(np.isfinite(body.x)), f"loop {gi}: oracle not finite"
+            x = arena.read(sids[gi][k], _native.FIELD_X)
+            v = arena.read(sids[gi][k], _native.FIELD_V)
+            assert np.array_equal(x, body.x), f"loop {gi} body {k}: positions differ"
+            assert np.array_equal(v, body.v), f"loop {gi} body {k}: velocities differ"
+        cache = arena.flex_read(gids[gi], _native.FLEX_CACHE)
+        assert np.array_equal(cache, o.flex.cache), f"loop {gi}: caches differ"
+        hits += int((cache >= 0).sum())
+    log(f"[case] contact workload: 24 of 2048 loops bit-exact after 600 substeps, {hits} cached segments")
+    assert hits > 0
+    arena.close()
