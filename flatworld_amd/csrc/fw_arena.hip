@@ -1718,7 +1718,8 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
         if (!sc[b]) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(d->scene_ids[b]));
         if (sc[b]->mat_type == FW_MAT_SPRING) return fail(FW_E_INVALID, "FEM-FEM groups hold FEM scenes only");
         if (sc[b]->flex_group >= 0) return fail(FW_E_INVALID, "scene already belongs to a FEM-FEM group");
-        if (sc[b]->s0 != sc[0]->s0) return fail(FW_E_INVALID, "a group's scenes must be added before the same substep");
+        if (sc[b]->s0 != sc[0]->s0 || sc[b]->horizon != sc[0]->horizon)
+            return fail(FW_E_INVALID, "a group's scenes must be added before the same substep, with the same horizon");
         for (int c = 0; c < b; ++c)
             if (d->scene_ids[c] == d->scene_ids[b]) return fail(FW_E_INVALID, "a scene appears twice in the group");
     }

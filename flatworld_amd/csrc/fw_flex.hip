@@ -320,6 +320,69 @@ __device__ __forceinline__ bool seen_before(const FlexDev& F, const FlexGrid& g,
 }
 
 constexpr int WALK_BATCH = 8;  // cells whose ranges are loaded together in a query walk
+constexpr int FLEX_LPI = 4;    // lanes per work item in the contact pass
+
+// The best candidate so far: the most negative penetration, the earliest walk
+// position on ties (key = walk cell << 32 | entry).
+struct Best {
+    float pen = 1e9f, nx = 0.0f, ny = 0.0f, w0 = 0.0f, w1 = 0.0f;
+    int s = -1;
+    long long key = 0x7FFFFFFFFFFFFFFFll;
+    bool found = false;
+    __device__ void take(float p, float x, float y, float a, float c, int sg, long long k) {
+        pen = p;
+        nx = x;
+        ny = y;
+        w0 = a;
+        w1 = c;
+        s = sg;
+        key = k;
+        found = true;
+    }
+};
+
+// Walk the query cells lane, lane + stride, ... (x outer, y inner), ranges loaded
+// WALK_BATCH at a time, and keep the best qualifying candidate among the segments
+// of body tb met for the first time; `met` counts those first meetings.  A single
+// lane (stride 1) stops evaluating once max_query segments were met, as the
+// reference's query list stops growing.
+__device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, float2 p, int tb, const float2* tpos,
+                     float qbuf, int lane, int stride, bool exact, int max_query, int& met, Best& best) {
+    const int nyq = q.uy - q.ly + 1, ncell = (q.ux - q.lx + 1) * nyq;
+    for (int i0 = 0; i0 * stride + lane < ncell; i0 += WALK_BATCH) {
+        int2 cr[WALK_BATCH];
+#pragma unroll
+        for (int u = 0; u < WALK_BATCH; ++u) {
+            const int t = (i0 + u) * stride + lane;
+            const int cid = (q.lx + t / nyq) + (q.ly + t % nyq) * g.cn[0];
+            cr[u] = (t < ncell && cid >= 0 && cid < g.tc) ? F.cell_rng[cid] : make_int2(0, 0);
+        }
+        for (int u = 0; u < WALK_BATCH; ++u) {
+            const int t = (i0 + u) * stride + lane;
+            if (t >= ncell) break;
+            const int ix = q.lx + t / nyq, iy = q.ly + t % nyq;
+            const int e = cr[u].y;
+            for (int pp = cr[u].x; pp < e; ++pp) {
+                const int ce = F.cell_ent[pp];
+                if (((ce >> 2) & 0xFF) != tb) continue;
+                const int s = ce >> 10;
+                // first meeting of s in the walk: the first cell of the walk inside the
+                // segment's box, i.e. its first column or the walk's, and its first row
+                // or the walk's
+                const bool seen = exact ? !((ix == q.lx || (ce & 2)) && (iy == q.ly || (ce & 1)))
+                                        : seen_before(F, g, q, s, ix, iy, pp);
+                if (seen) continue;
+                if (stride == 1 && met >= max_query) continue;
+                ++met;
+                const int2 sn = F.seg_node[s];
+                float nx, ny, w0, w1;
+                const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
+                if (pen < best.pen && pen < 1e-4f && fabsf(pen) < qbuf)
+                    best.take(pen, nx, ny, w0, w1, s, ((long long)t << 32) | (long long)pp);
+            }
+        }
+    }
+}
 
 // _batched_flexflex_contact_kernel_sh (flatworld/mixedcontact.py:886-1025) for
 // every work item of the group, then the node-major contribution lists the
@@ -367,7 +430,11 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
     phase_barrier();
 
     const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.016f);
-    for (int w = tid; w < F.n_items; w += FLEX_BLOCK) {
+    // FLEX_LPI lanes per work item: the lanes of an item run in step through its
+    // loop (one trip count), split its query walk and reduce; lane 0 of the item
+    // does its stores and atomics
+    const int sub = tid % FLEX_LPI;
+    for (int w = tid / FLEX_LPI; w < F.n_items; w += FLEX_BLOCK / FLEX_LPI) {
         int2 ri = make_int2(0, -1);
         if (act[F.item_pair[w]]) {
             const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
@@ -380,9 +447,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
             const int be_off = F.body_seg_off[tb], be_cnt = F.body_seg_off[tb + 1] - be_off;
             const float penalty = F.item_penalty[w], friction = F.item_friction[w];
 
-            float best_pen = 1e9f, bnx = 0.0f, bny = 0.0f, bw0 = 0.0f, bw1 = 0.0f;
-            bool found = false;
-            int bs = -1;
+            Best b;
             const int cached = F.cache[w];
             if (cached >= be_off && cached < be_off + be_cnt) {
                 const int2 sn = F.seg_node[cached];
@@ -406,77 +471,74 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
                 if (!outside) {
                     float nx, ny, w0, w1;
                     const float pen = detect(p.x, p.y, a, c, margin, nx, ny, w0, w1);
-                    if (pen < 1e-4f && fabsf(pen) < margin) {
-                        best_pen = pen;
-                        bnx = nx;
-                        bny = ny;
-                        bw0 = w0;
-                        bw1 = w1;
-                        bs = cached;
-                        found = true;
-                    }
+                    if (pen < 1e-4f && fabsf(pen) < margin) b.take(pen, nx, ny, w0, w1, cached, 0);
                 }
             }
-            if (!found && g.tc > 0) {
+            int cache_new = cached;
+            if (!b.found && g.tc > 0) {
                 // query_point_with_buffer (flatworld/spatialmanager.py:131-199), walked in place
                 const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
-                int n = 0;
-                int cache_new = F.cache[w];
-                // cells in walk order (x outer, y inner), their ranges loaded a batch at a time
-                const int nyq = q.uy - q.ly + 1, ncell = (q.ux - q.lx + 1) * nyq;
-                for (int t0 = 0; t0 < ncell; t0 += WALK_BATCH) {
-                    int2 cr[WALK_BATCH];
+                bool serial = !g.exact;
+                if (g.exact) {
+                    // the item's lanes take every FLEX_LPI-th cell of the walk; the best
+                    // candidate is the minimum of (penetration, walk position), the serial
+                    // walk's choice, as long as no more than max_query distinct segments
+                    // are met (else the cap decides and the serial walk is repeated)
+                    int met = 0;
+                    Best lb;
+                    walk(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
 #pragma unroll
-                    for (int u = 0; u < WALK_BATCH; ++u) {
-                        const int t = t0 + u;
-                        const int cid = (q.lx + t / nyq) + (q.ly + t % nyq) * g.cn[0];
-                        cr[u] = (t < ncell && cid >= 0 && cid < g.tc) ? F.cell_rng[cid] : make_int2(0, 0);
-                    }
-                    for (int u = 0; u < WALK_BATCH && t0 + u < ncell; ++u) {
-                        const int ix = q.lx + (t0 + u) / nyq, iy = q.ly + (t0 + u) % nyq;
-                        const int e = cr[u].y;
-                        for (int pp = cr[u].x; pp < e; ++pp) {
-                            const int ce = F.cell_ent[pp];
-                            if (((ce >> 2) & 0xFF) != tb) continue;
-                            const int s = ce >> 10;
-                            // first meeting of s in the walk: the first cell of the walk inside
-                            // the segment's box, i.e. its first column or the walk's, and its
-                            // first row or the walk's
-                            const bool seen = g.exact ? !((ix == q.lx || (ce & 2)) && (iy == q.ly || (ce & 1)))
-                                                      : seen_before(F, g, q, s, ix, iy, pp);
-                            if (n >= F.max_query || seen) continue;
-                            ++n;
-                            const int2 sn = F.seg_node[s];
-                            float nx, ny, w0, w1;
-                            const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
-                            if (pen < best_pen && pen < 1e-4f && fabsf(pen) < qbuf) {
-                                best_pen = pen;
-                                bnx = nx;
-                                bny = ny;
-                                bw0 = w0;
-                                bw1 = w1;
-                                bs = s;
-                                found = true;
-                                cache_new = s;
+                    for (int off = 1; off < FLEX_LPI; off <<= 1) met += __shfl_xor(met, off, FLEX_LPI);
+                    if (met > F.max_query) {
+                        serial = true;
+                    } else {
+                        int bl = sub;
+                        float bp = lb.pen;
+                        long long bk = lb.key;
+#pragma unroll
+                        for (int off = 1; off < FLEX_LPI; off <<= 1) {
+                            const float op = __shfl_xor(bp, off, FLEX_LPI);
+                            const long long ok = __shfl_xor(bk, off, FLEX_LPI);
+                            const int ol = __shfl_xor(bl, off, FLEX_LPI);
+                            if (op < bp || (op == bp && ok < bk)) {
+                                bp = op;
+                                bk = ok;
+                                bl = ol;
                             }
+                        }
+                        const int ws = __shfl(lb.s, bl, FLEX_LPI);
+                        const float wnx = __shfl(lb.nx, bl, FLEX_LPI), wny = __shfl(lb.ny, bl, FLEX_LPI);
+                        const float ww0 = __shfl(lb.w0, bl, FLEX_LPI), ww1 = __shfl(lb.w1, bl, FLEX_LPI);
+                        if (ws >= 0) {
+                            b.take(bp, wnx, wny, ww0, ww1, ws, bk);
+                            cache_new = ws;
                         }
                     }
                 }
-                F.cache[w] = cache_new;
+                if (serial) {
+                    int met = 0;
+                    Best sb;
+                    walk(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
+                    if (sb.found) {
+                        b = sb;
+                        cache_new = sb.s;
+                    }
+                }
             }
-            if (!found) F.cache[w] = -1;
-            if (found && best_pen < 0.0f) {
-                const float nfx = ((-bnx) * penalty) * best_pen;
-                const float nfy = ((-bny) * penalty) * best_pen;
+            if (!b.found) cache_new = -1;
+            if (sub == 0) F.cache[w] = cache_new;
+            if (b.found && b.pen < 0.0f) {
+                const float nfx = ((-b.nx) * penalty) * b.pen;
+                const float nfy = ((-b.ny) * penalty) * b.pen;
                 float tx = nfx, ty = nfy;
                 if (friction > 1e-9f) {
-                    const int2 sn = F.seg_node[bs];
+                    const int2 sn = F.seg_node[b.s];
                     const float2 v0 = ST->vel[sn.x], v1 = ST->vel[sn.y];
-                    const float svx = v0.x * bw0 + v1.x * bw1;
-                    const float svy = v0.y * bw0 + v1.y * bw1;
+                    const float svx = v0.x * b.w0 + v1.x * b.w1;
+                    const float svy = v0.y * b.w0 + v1.y * b.w1;
                     const float rvx = vel.x - svx, rvy = vel.y - svy;
-                    const float d = dot2(rvx, rvy, bnx, bny);
-                    const float tgx = rvx - d * bnx, tgy = rvy - d * bny;
+                    const float d = dot2(rvx, rvy, b.nx, b.ny);
+                    const float tgx = rvx - d * b.nx, tgy = rvy - d * b.ny;
                     const float tlen = len2(tgx, tgy);
                     if (tlen > 1e-9f) {
                         const float fdx = (-tgx) / tlen, fdy = (-tgy) / tlen;
@@ -485,15 +547,17 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev*
                         ty = ty + (fdy * friction) * nl;
                     }
                 }
-                ri = make_int2(1, bs);
-                F.res_f[w] = make_float4(tx, ty, bw0, bw1);
-                const int2 ss = F.seg_slot[bs];
-                atomicAdd(&F.slot_start[F.item_slot[w]], 1);
-                atomicAdd(&F.slot_start[ss.x], 1);
-                atomicAdd(&F.slot_start[ss.y], 1);
+                ri = make_int2(1, b.s);
+                if (sub == 0) {
+                    F.res_f[w] = make_float4(tx, ty, b.w0, b.w1);
+                    const int2 ss = F.seg_slot[b.s];
+                    atomicAdd(&F.slot_start[F.item_slot[w]], 1);
+                    atomicAdd(&F.slot_start[ss.x], 1);
+                    atomicAdd(&F.slot_start[ss.y], 1);
+                }
             }
         }
-        F.res_i[w] = ri;
+        if (sub == 0) F.res_i[w] = ri;
     }
     phase_barrier();
     scan_global(F.slot_start, F.slot_cur, F.n_slots, lds);

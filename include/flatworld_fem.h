@@ -261,7 +261,8 @@ int32_t fw_scene_mass_scaling(fw_arena* a, int32_t scene_id, float dt_target);
 int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, int32_t* finished);
 
 /* FEM-FEM contact groups.  fw_flex_add needs every body in a pair to carry one
- * FW_GROUND_FLEX record of sufficient capacity; its scenes cannot be removed
+ * FW_GROUND_FLEX record of sufficient capacity, and all of the group's scenes
+ * added before the same substep with the same horizon; they cannot be removed
  * before the group.  fw_flex_rebuild enqueues the spatial-hash build at the state
  * the next substep starts from (after the scenes' warm-up), for one group or, with
  * group_id -1, for every group in one launch; the caller keeps the
