@@ -210,3 +210,50 @@ def test_contact_workload_sampled_loops_bitexact():
     log(f"[case] contact workload: 24 of 2048 loops bit-exact after 600 substeps, {hits} cached segments")
     assert hits > 0
     arena.close()
+
+
+@pytest.mark.parametrize("caps", [dict(max_cells=40), dict(max_query=2), dict(max_pairs=60),
+                                  dict(max_cells=25, max_query=3, max_pairs=40)])
+def test_capped_hash_and_query_bitexact(caps):
+    """Hash limits small enough to bind: a grid clipped to max_cells, pairs dropped
+    past max_pairs (both make the walk repeat itself to deduplicate) and query
+    lists cut at max_query (the serial walk decides)."""
+    import dataclasses
+
+    from flatworld_amd import workloads as W
+
+    rng = np.random.default_rng(11)
+    loops = [W.contact_pair_group(rng, 10, 16) for _ in range(6)]
+    dt = min(s.dt for sp, _ in loops for s in sp)
+    arena = _native.Arena(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
+    ids = []
+    for k, (sp, g) in enumerate(loops):
+        g = dataclasses.replace(g, **caps)
+        loops[k] = (sp, g)
+        s = [arena.add(x) for x in sp]
+        ids.append((s, arena.flex_add(g, s)))
+    arena.step(1, -1.0, 0)
+    arena.flex_rebuild(-1)
+    oracles = [OracleLoop(sp, g) for sp, g in loops]
+    elapsed, hits = 0.0, 0
+    for step in range(700):
+        elapsed += dt
+        if elapsed >= loops[0][1].rebuild_interval:
+            arena.flex_rebuild(-1)
+            elapsed = 0.0
+        arena.step(1, float(np.float32(dt)), 0)
+        for o in oracles:
+            o.substep(dt)
+    arena.sync()
+    for (sids, gid), o in zip(ids, oracles):
+        grid = arena.flex_read(gid, _native.FLEX_GRID)
+        assert grid["tc"] == o.flex.grid["tc"]
+        for k, body in enumerate(o.bodies):
+            assert np.all(np.isfinite(body.x))
+            assert np.array_equal(arena.read(sids[k], _native.FIELD_X), body.x), f"{caps}: positions differ"
+            assert np.array_equal(arena.read(sids[k], _native.FIELD_V), body.v), f"{caps}: velocities differ"
+        cache = arena.flex_read(gid, _native.FLEX_CACHE)
+        assert np.array_equal(cache, o.flex.cache), f"{caps}: caches differ"
+        hits += int((cache >= 0).sum())
+    log(f"[case] capped hash {caps}: 6 loops bit-exact after 700 substeps, {hits} cached segments")
+    arena.close()
