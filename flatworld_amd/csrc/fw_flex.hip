@@ -387,7 +387,7 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
 // _batched_flexflex_contact_kernel_sh (flatworld/mixedcontact.py:886-1025) for
 // every work item of the group, then the node-major contribution lists the
 // substep kernel adds.
-__global__ __launch_bounds__(FLEX_BLOCK) void flex_contact_kernel(const FlexDev* groups, const SceneDev* scenes,
+__global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_contact_kernel(const FlexDev* groups, const SceneDev* scenes,
                                                                   const ArenaCtl* ctl, int k, uint32_t flags) {
     const FlexDev& F = groups[blockIdx.x];
     __shared__ int lds[FLEX_BLOCK];
