@@ -43,8 +43,9 @@ extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileD
                                              float dt_target, int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
                                           const ArenaCtl* ctl, int row, hipStream_t stream);
-extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const SceneDev* scenes,
-                                             const ArenaCtl* ctl, int k, uint32_t flags, hipStream_t stream);
+extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
+                                             int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
+                                             uint32_t flags, hipStream_t stream);
 extern "C" int fw_flex_max_pairs_per_group(void);
 
 namespace {
@@ -619,7 +620,8 @@ struct fw_arena {
     std::vector<std::unique_ptr<FlexHost>> flex;
     std::vector<int> flex_row;
     FlexDev* d_flex = nullptr;
-    int flex_cap = 0, n_flex_live = 0;
+    int32_t* d_flex_items = nullptr;  // first global work item of each table row (+ the total)
+    int flex_cap = 0, n_flex_live = 0, n_flex_items = 0;
     bool flex_dirty = false;
     // one-entry tiling cache: fw_scene_footprint followed by fw_scene_add (or a retry of
     // an add that found the arena full) reuses the tiling instead of rebuilding it
@@ -790,15 +792,24 @@ int ensure_flex(fw_arena* a) {
     if (n > a->flex_cap) {
         const int cap = std::max(n, std::max(8, 2 * a->flex_cap));
         FlexDev* nd = nullptr;
+        int32_t* ni = nullptr;
         HIPCHK(hipStreamSynchronize(a->stream));
         HIPCHK(hipMalloc(&nd, sizeof(FlexDev) * (size_t)cap));
+        HIPCHK(hipMalloc(&ni, sizeof(int32_t) * (size_t)(cap + 1)));
         if (a->d_flex) HIPCHK(hipFree(a->d_flex));
+        if (a->d_flex_items) HIPCHK(hipFree(a->d_flex_items));
         a->d_flex = nd;
+        a->d_flex_items = ni;
         a->flex_cap = cap;
     }
+    std::vector<int32_t> item_off(n + 1, 0);
+    for (int r = 0; r < n; ++r) item_off[r + 1] = item_off[r] + rows[r].n_items;
     int rc = upload(a, a->d_flex, rows.data(), (int64_t)sizeof(FlexDev) * n);
     if (rc) return rc;
+    rc = upload(a, a->d_flex_items, item_off.data(), (int64_t)sizeof(int32_t) * (n + 1));
+    if (rc) return rc;
     a->n_flex_live = n;
+    a->n_flex_items = item_off[n];
     a->flex_dirty = false;
     return FW_OK;
 }
@@ -1169,6 +1180,7 @@ int32_t fw_arena_destroy(fw_arena* a) {
     (void)hipFree(a->d_tiles);
     (void)hipFree(a->d_ctl);
     (void)hipFree(a->d_flex);
+    (void)hipFree(a->d_flex_items);
     (void)hipStreamDestroy(a->stream);
     delete a;
     return FW_OK;
@@ -1296,7 +1308,8 @@ int32_t fw_scene_progress(fw_arena* a, int32_t id, int64_t* steps_done, int32_t*
 static hipError_t launch_all(fw_arena* a, int k, float dt_override, float damping_override, uint32_t f, int grid_tri,
                       int grid_spr) {
     if (a->n_flex_live > 0) {  // FEM-FEM contact lists before the substep that adds them
-        hipError_t e = fw_launch_flex_contact(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, k, f, a->stream);
+        hipError_t e = fw_launch_flex_contact(a->d_flex, a->n_flex_live, a->d_flex_items, a->n_flex_items, a->d_scenes,
+                                              a->d_ctl, k, f, a->stream);
         if (e != hipSuccess) return e;
     }
     if (grid_tri > 0) {

@@ -25,7 +25,7 @@ namespace fw {
 namespace {
 
 constexpr int FLEX_BLOCK = 256;  // a group's work items and slots are looped over in strides
-constexpr int FLEX_MAX_PAIRS = 4096;  // pairs of one group (LDS activation flags)
+constexpr int FLEX_MAX_PAIRS = 4096;  // pairs of one group
 constexpr uint32_t FLEX_NO_CONTACT = 2u;  // FW_STEP_NO_CONTACT
 
 __device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
@@ -384,182 +384,199 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
     }
 }
 
-// _batched_flexflex_contact_kernel_sh (flatworld/mixedcontact.py:886-1025) for
-// every work item of the group, then the node-major contribution lists the
-// substep kernel adds.
-__global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_contact_kernel(const FlexDev* groups, const SceneDev* scenes,
-                                                                  const ArenaCtl* ctl, int k, uint32_t flags) {
-    const FlexDev& F = groups[blockIdx.x];
-    __shared__ int lds[FLEX_BLOCK];
-    __shared__ uint8_t act[FLEX_MAX_PAIRS];
-    const int tid = threadIdx.x;
+// Pair activation: the BVH overlap test (flatworld/explicitloop.py:477-487), or
+// every pair when the loop skips its BVH.
+__device__ __forceinline__ bool pair_active(const FlexDev& F, const SceneDev* scenes, int q, int ar3, bool pad) {
+    if (!F.gated) return true;
+    const int2 pb = F.pair_body[q];
+    float a[4], b[4];
+    scene_aabb(scenes + F.body_scene[pb.x], ar3, a);
+    scene_aabb(scenes + F.body_scene[pb.y], ar3, b);
+    if (pad) {
+        for (int c = 0; c < 2; ++c) {
+            const float ba = 0.1f * (a[c + 2] - a[c]);
+            const float la = a[c] - ba, ua = a[c + 2] + ba;
+            const float bb = 0.1f * (b[c + 2] - b[c]);
+            const float lb = b[c] - bb, ub = b[c + 2] + bb;
+            a[c] = la;
+            a[c + 2] = ua;
+            b[c] = lb;
+            b[c + 2] = ub;
+        }
+    }
+    return a[0] <= b[2] && a[2] >= b[0] && a[1] <= b[3] && a[3] >= b[1];
+}
+
+// The substep a group's contact pass serves, or -1 when it has none this launch.
+__device__ __forceinline__ int group_local(const FlexDev& F, const SceneDev* scenes, const ArenaCtl* ctl, int k,
+                                           uint32_t flags) {
     const SceneDev* S0 = scenes + F.body_scene[0];
     const int local = ctl->clock + k - S0->s0;
-    if (local < 0 || local >= S0->horizon || (flags & FLEX_NO_CONTACT)) return;
+    if (local < 0 || local >= S0->horizon || (flags & FLEX_NO_CONTACT)) return -1;
+    return local;
+}
+
+// _batched_flexflex_contact_kernel_sh (flatworld/mixedcontact.py:886-1025), one
+// work item per FLEX_LPI lanes over the work items of every group (item_off:
+// each group's first global item, n_groups + 1 entries), so that the items of a
+// group with long queries spread over many workgroups.  The lanes of an item
+// split its query walk and reduce; lane 0 does its stores and counts its
+// contributions per slot.
+__global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev* groups, const int32_t* item_off,
+                                                                   int n_groups, const SceneDev* scenes,
+                                                                   const ArenaCtl* ctl, int k, uint32_t flags) {
+    const int gw = (int)((blockIdx.x * FLEX_BLOCK + threadIdx.x) / FLEX_LPI);
+    const int sub = threadIdx.x % FLEX_LPI;
+    if (gw >= item_off[n_groups]) return;
+    int lo = 0, hi = n_groups;  // item_off[lo] <= gw < item_off[lo + 1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (item_off[mid] <= gw) lo = mid;
+        else hi = mid;
+    }
+    const FlexDev& F = groups[lo];
+    const int w = gw - item_off[lo];
+    const int local = group_local(F, scenes, ctl, k, flags);
+    if (local < 0) return;
     const int rd = (local + 1) & 1;
     const int ar3 = mod3_(local);
-    const bool pad = ((local - S0->hot.gate_base) % 100) == 0;
+    const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
     const FlexGrid g = *F.grid;
-
-    // pair activation: the BVH overlap test (flatworld/explicitloop.py:477-487), or
-    // every pair when the loop skips its BVH
-    for (int q = tid; q < F.n_pairs; q += FLEX_BLOCK) {
-        bool on = true;
-        if (F.gated) {
-            const int2 pb = F.pair_body[q];
-            float a[4], b[4];
-            scene_aabb(scenes + F.body_scene[pb.x], ar3, a);
-            scene_aabb(scenes + F.body_scene[pb.y], ar3, b);
-            if (pad) {
-                for (int c = 0; c < 2; ++c) {
-                    const float ba = 0.1f * (a[c + 2] - a[c]);
-                    const float la = a[c] - ba, ua = a[c + 2] + ba;
-                    const float bb = 0.1f * (b[c + 2] - b[c]);
-                    const float lb = b[c] - bb, ub = b[c + 2] + bb;
-                    a[c] = la;
-                    a[c + 2] = ua;
-                    b[c] = lb;
-                    b[c + 2] = ub;
-                }
-            }
-            on = a[0] <= b[2] && a[2] >= b[0] && a[1] <= b[3] && a[3] >= b[1];
-        }
-        act[q] = on ? 1 : 0;
-    }
-    for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) F.slot_start[sl] = 0;
-    phase_barrier();
-
     const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.016f);
-    // FLEX_LPI lanes per work item: the lanes of an item run in step through its
-    // loop (one trip count), split its query walk and reduce; lane 0 of the item
-    // does its stores and atomics
-    const int sub = tid % FLEX_LPI;
-    for (int w = tid / FLEX_LPI; w < F.n_items; w += FLEX_BLOCK / FLEX_LPI) {
-        int2 ri = make_int2(0, -1);
-        if (act[F.item_pair[w]]) {
-            const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
-            const int nid = F.item_node[w];
-            const float2 p = SB->pos[rd][nid];
-            const float2 vel = SB->vel[nid];
-            const int tb = F.item_target[w];
-            const SceneDev* ST = scenes + F.body_scene[tb];
-            const float2* tpos = ST->pos[rd];
-            const int be_off = F.body_seg_off[tb], be_cnt = F.body_seg_off[tb + 1] - be_off;
-            const float penalty = F.item_penalty[w], friction = F.item_friction[w];
+    int2 ri = make_int2(0, -1);
+    if (pair_active(F, scenes, F.item_pair[w], ar3, pad)) {
+        const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
+        const int nid = F.item_node[w];
+        const float2 p = SB->pos[rd][nid];
+        const float2 vel = SB->vel[nid];
+        const int tb = F.item_target[w];
+        const SceneDev* ST = scenes + F.body_scene[tb];
+        const float2* tpos = ST->pos[rd];
+        const int be_off = F.body_seg_off[tb], be_cnt = F.body_seg_off[tb + 1] - be_off;
+        const float penalty = F.item_penalty[w], friction = F.item_friction[w];
 
-            Best b;
-            const int cached = F.cache[w];
-            if (cached >= be_off && cached < be_off + be_cnt) {
-                const int2 sn = F.seg_node[cached];
-                const float2 a = tpos[sn.x], c = tpos[sn.y];
-                float lb0 = 1e30f, lb1 = 1e30f, ub0 = -1e30f, ub1 = -1e30f;
-                lb0 = fmin_(lb0, a.x);
-                lb1 = fmin_(lb1, a.y);
-                ub0 = fmax_(ub0, a.x);
-                ub1 = fmax_(ub1, a.y);
-                lb0 = fmin_(lb0, c.x);
-                lb1 = fmin_(lb1, c.y);
-                ub0 = fmax_(ub0, c.x);
-                ub1 = fmax_(ub1, c.y);
-                const float margin = fmax_(1e-4f, 1e-2f * len2(ub0 - lb0, ub1 - lb1));
-                lb0 = lb0 - margin;
-                lb1 = lb1 - margin;
-                ub0 = ub0 + margin;
-                ub1 = ub1 + margin;
-                // _point_outside_aabb (flatworld/mixedcontact.py:113-119)
-                const bool outside = fmin_(p.x - lb0, p.y - lb1) < 0.0f || fmin_(ub0 - p.x, ub1 - p.y) < 0.0f;
-                if (!outside) {
-                    float nx, ny, w0, w1;
-                    const float pen = detect(p.x, p.y, a, c, margin, nx, ny, w0, w1);
-                    if (pen < 1e-4f && fabsf(pen) < margin) b.take(pen, nx, ny, w0, w1, cached, 0);
-                }
+        Best b;
+        const int cached = F.cache[w];
+        if (cached >= be_off && cached < be_off + be_cnt) {
+            const int2 sn = F.seg_node[cached];
+            const float2 a = tpos[sn.x], c = tpos[sn.y];
+            float lb0 = 1e30f, lb1 = 1e30f, ub0 = -1e30f, ub1 = -1e30f;
+            lb0 = fmin_(lb0, a.x);
+            lb1 = fmin_(lb1, a.y);
+            ub0 = fmax_(ub0, a.x);
+            ub1 = fmax_(ub1, a.y);
+            lb0 = fmin_(lb0, c.x);
+            lb1 = fmin_(lb1, c.y);
+            ub0 = fmax_(ub0, c.x);
+            ub1 = fmax_(ub1, c.y);
+            const float margin = fmax_(1e-4f, 1e-2f * len2(ub0 - lb0, ub1 - lb1));
+            lb0 = lb0 - margin;
+            lb1 = lb1 - margin;
+            ub0 = ub0 + margin;
+            ub1 = ub1 + margin;
+            // _point_outside_aabb (flatworld/mixedcontact.py:113-119)
+            const bool outside = fmin_(p.x - lb0, p.y - lb1) < 0.0f || fmin_(ub0 - p.x, ub1 - p.y) < 0.0f;
+            if (!outside) {
+                float nx, ny, w0, w1;
+                const float pen = detect(p.x, p.y, a, c, margin, nx, ny, w0, w1);
+                if (pen < 1e-4f && fabsf(pen) < margin) b.take(pen, nx, ny, w0, w1, cached, 0);
             }
-            int cache_new = cached;
-            if (!b.found && g.tc > 0) {
-                // query_point_with_buffer (flatworld/spatialmanager.py:131-199), walked in place
-                const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
-                bool serial = !g.exact;
-                if (g.exact) {
-                    // the item's lanes take every FLEX_LPI-th cell of the walk; the best
-                    // candidate is the minimum of (penetration, walk position), the serial
-                    // walk's choice, as long as no more than max_query distinct segments
-                    // are met (else the cap decides and the serial walk is repeated)
-                    int met = 0;
-                    Best lb;
-                    walk(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
+        }
+        int cache_new = cached;
+        if (!b.found && g.tc > 0) {
+            // query_point_with_buffer (flatworld/spatialmanager.py:131-199), walked in place
+            const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
+            bool serial = !g.exact;
+            if (g.exact) {
+                // the item's lanes take every FLEX_LPI-th cell of the walk; the best
+                // candidate is the minimum of (penetration, walk position), the serial
+                // walk's choice, as long as no more than max_query distinct segments
+                // are met (else the cap decides and the serial walk is repeated)
+                int met = 0;
+                Best lb;
+                walk(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
 #pragma unroll
-                    for (int off = 1; off < FLEX_LPI; off <<= 1) met += __shfl_xor(met, off, FLEX_LPI);
-                    if (met > F.max_query) {
-                        serial = true;
-                    } else {
-                        int bl = sub;
-                        float bp = lb.pen;
-                        long long bk = lb.key;
+                for (int off = 1; off < FLEX_LPI; off <<= 1) met += __shfl_xor(met, off, FLEX_LPI);
+                if (met > F.max_query) {
+                    serial = true;
+                } else {
+                    int bl = sub;
+                    float bp = lb.pen;
+                    long long bk = lb.key;
 #pragma unroll
-                        for (int off = 1; off < FLEX_LPI; off <<= 1) {
-                            const float op = __shfl_xor(bp, off, FLEX_LPI);
-                            const long long ok = __shfl_xor(bk, off, FLEX_LPI);
-                            const int ol = __shfl_xor(bl, off, FLEX_LPI);
-                            if (op < bp || (op == bp && ok < bk)) {
-                                bp = op;
-                                bk = ok;
-                                bl = ol;
-                            }
-                        }
-                        const int ws = __shfl(lb.s, bl, FLEX_LPI);
-                        const float wnx = __shfl(lb.nx, bl, FLEX_LPI), wny = __shfl(lb.ny, bl, FLEX_LPI);
-                        const float ww0 = __shfl(lb.w0, bl, FLEX_LPI), ww1 = __shfl(lb.w1, bl, FLEX_LPI);
-                        if (ws >= 0) {
-                            b.take(bp, wnx, wny, ww0, ww1, ws, bk);
-                            cache_new = ws;
+                    for (int off = 1; off < FLEX_LPI; off <<= 1) {
+                        const float op = __shfl_xor(bp, off, FLEX_LPI);
+                        const long long ok = __shfl_xor(bk, off, FLEX_LPI);
+                        const int ol = __shfl_xor(bl, off, FLEX_LPI);
+                        if (op < bp || (op == bp && ok < bk)) {
+                            bp = op;
+                            bk = ok;
+                            bl = ol;
                         }
                     }
-                }
-                if (serial) {
-                    int met = 0;
-                    Best sb;
-                    walk(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
-                    if (sb.found) {
-                        b = sb;
-                        cache_new = sb.s;
+                    const int ws = __shfl(lb.s, bl, FLEX_LPI);
+                    const float wnx = __shfl(lb.nx, bl, FLEX_LPI), wny = __shfl(lb.ny, bl, FLEX_LPI);
+                    const float ww0 = __shfl(lb.w0, bl, FLEX_LPI), ww1 = __shfl(lb.w1, bl, FLEX_LPI);
+                    if (ws >= 0) {
+                        b.take(bp, wnx, wny, ww0, ww1, ws, bk);
+                        cache_new = ws;
                     }
                 }
             }
-            if (!b.found) cache_new = -1;
-            if (sub == 0) F.cache[w] = cache_new;
-            if (b.found && b.pen < 0.0f) {
-                const float nfx = ((-b.nx) * penalty) * b.pen;
-                const float nfy = ((-b.ny) * penalty) * b.pen;
-                float tx = nfx, ty = nfy;
-                if (friction > 1e-9f) {
-                    const int2 sn = F.seg_node[b.s];
-                    const float2 v0 = ST->vel[sn.x], v1 = ST->vel[sn.y];
-                    const float svx = v0.x * b.w0 + v1.x * b.w1;
-                    const float svy = v0.y * b.w0 + v1.y * b.w1;
-                    const float rvx = vel.x - svx, rvy = vel.y - svy;
-                    const float d = dot2(rvx, rvy, b.nx, b.ny);
-                    const float tgx = rvx - d * b.nx, tgy = rvy - d * b.ny;
-                    const float tlen = len2(tgx, tgy);
-                    if (tlen > 1e-9f) {
-                        const float fdx = (-tgx) / tlen, fdy = (-tgy) / tlen;
-                        const float nl = len2(nfx, nfy);
-                        tx = tx + (fdx * friction) * nl;
-                        ty = ty + (fdy * friction) * nl;
-                    }
-                }
-                ri = make_int2(1, b.s);
-                if (sub == 0) {
-                    F.res_f[w] = make_float4(tx, ty, b.w0, b.w1);
-                    const int2 ss = F.seg_slot[b.s];
-                    atomicAdd(&F.slot_start[F.item_slot[w]], 1);
-                    atomicAdd(&F.slot_start[ss.x], 1);
-                    atomicAdd(&F.slot_start[ss.y], 1);
+            if (serial) {
+                int met = 0;
+                Best sb;
+                walk(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
+                if (sb.found) {
+                    b = sb;
+                    cache_new = sb.s;
                 }
             }
         }
-        if (sub == 0) F.res_i[w] = ri;
+        if (!b.found) cache_new = -1;
+        if (sub == 0) F.cache[w] = cache_new;
+        if (b.found && b.pen < 0.0f) {
+            const float nfx = ((-b.nx) * penalty) * b.pen;
+            const float nfy = ((-b.ny) * penalty) * b.pen;
+            float tx = nfx, ty = nfy;
+            if (friction > 1e-9f) {
+                const int2 sn = F.seg_node[b.s];
+                const float2 v0 = ST->vel[sn.x], v1 = ST->vel[sn.y];
+                const float svx = v0.x * b.w0 + v1.x * b.w1;
+                const float svy = v0.y * b.w0 + v1.y * b.w1;
+                const float rvx = vel.x - svx, rvy = vel.y - svy;
+                const float d = dot2(rvx, rvy, b.nx, b.ny);
+                const float tgx = rvx - d * b.nx, tgy = rvy - d * b.ny;
+                const float tlen = len2(tgx, tgy);
+                if (tlen > 1e-9f) {
+                    const float fdx = (-tgx) / tlen, fdy = (-tgy) / tlen;
+                    const float nl = len2(nfx, nfy);
+                    tx = tx + (fdx * friction) * nl;
+                    ty = ty + (fdy * friction) * nl;
+                }
+            }
+            ri = make_int2(1, b.s);
+            if (sub == 0) {
+                F.res_f[w] = make_float4(tx, ty, b.w0, b.w1);
+                const int2 ss = F.seg_slot[b.s];
+                atomicAdd(&F.slot_start[F.item_slot[w]], 1);
+                atomicAdd(&F.slot_start[ss.x], 1);
+                atomicAdd(&F.slot_start[ss.y], 1);
+            }
+        }
     }
-    phase_barrier();
+    if (sub == 0) F.res_i[w] = ri;
+}
+
+// The node-major contribution lists of one group (one workgroup): slot ranges by
+// a scan of the counts, entries scattered and sorted by work item, written to
+// each body's FEM-FEM block; the counts are then cleared for the next substep.
+__global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* groups, const SceneDev* scenes,
+                                                                const ArenaCtl* ctl, int k, uint32_t flags) {
+    const FlexDev& F = groups[blockIdx.x];
+    __shared__ int lds[FLEX_BLOCK];
+    const int tid = threadIdx.x;
+    if (group_local(F, scenes, ctl, k, flags) < 0) return;
     scan_global(F.slot_start, F.slot_cur, F.n_slots, lds);
     phase_barrier();
     for (int w = tid; w < F.n_items; w += FLEX_BLOCK) {
@@ -599,12 +616,13 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_contact_kernel(const FlexD
         rng[F.slot_node[sl]] = make_int2(b0 - base, e - b0);
         for (int i = b0; i < e; ++i) ent[i - base] = F.ent_f[i];
     }
+    phase_barrier();
+    for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) F.slot_start[sl] = 0;
 }
 
 }  // namespace
 }  // namespace fw
 
-// row >= 0: that table row; row < 0: every one of the n_groups rows, one workgroup each.
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
                                           const ArenaCtl* ctl, int row, hipStream_t stream) {
     if (n_groups <= 0) return hipSuccess;
@@ -613,11 +631,19 @@ extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, c
     return hipGetLastError();
 }
 
-extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const SceneDev* scenes,
-                                             const ArenaCtl* ctl, int k, uint32_t flags, hipStream_t stream) {
+extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
+                                             int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
+                                             uint32_t flags, hipStream_t stream) {
     if (n_groups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fw::flex_contact_kernel, dim3(n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups, scenes, ctl,
-                       k, flags);
+    if (total_items > 0) {
+        const int blocks = (int)(((int64_t)total_items * fw::FLEX_LPI + fw::FLEX_BLOCK - 1) / fw::FLEX_BLOCK);
+        hipLaunchKernelGGL(fw::flex_items_kernel, dim3(blocks), dim3(fw::FLEX_BLOCK), 0, stream, groups, item_off,
+                           n_groups, scenes, ctl, k, flags);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(fw::flex_lists_kernel, dim3(n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups, scenes, ctl, k,
+                       flags);
     return hipGetLastError();
 }
 
