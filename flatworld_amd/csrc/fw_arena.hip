@@ -46,7 +46,6 @@ extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, c
 extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
                                              uint32_t flags, hipStream_t stream);
-extern "C" int fw_flex_max_pairs_per_group(void);
 
 namespace {
 
@@ -1720,8 +1719,6 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
         return fail(FW_E_INVALID, "group arrays are required");
     if (B > 256 || S >= (1 << 21))
         return fail(FW_E_UNSUPPORTED, "a group holds at most 256 bodies and 2^21 - 1 boundary segments");
-    if (P > fw_flex_max_pairs_per_group())
-        return fail(FW_E_UNSUPPORTED, "a group holds at most " + std::to_string(fw_flex_max_pairs_per_group()) + " pairs");
     if (d->max_cells < 1 || d->max_query < 1 || d->max_pairs < 1) return fail(FW_E_INVALID, "hash limits must be >= 1");
     HIPCHK(hipSetDevice(a->device));
 

@@ -25,7 +25,6 @@ namespace fw {
 namespace {
 
 constexpr int FLEX_BLOCK = 256;  // a group's work items and slots are looped over in strides
-constexpr int FLEX_MAX_PAIRS = 4096;  // pairs of one group
 constexpr uint32_t FLEX_NO_CONTACT = 2u;  // FW_STEP_NO_CONTACT
 
 __device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
@@ -647,4 +646,3 @@ extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups
     return hipGetLastError();
 }
 
-extern "C" int fw_flex_max_pairs_per_group(void) { return fw::FLEX_MAX_PAIRS; }
