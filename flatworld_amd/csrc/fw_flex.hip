@@ -1,7 +1,9 @@
 // fw_flex.hip -- FEM-FEM penalty contact between the FEM bodies of one loop:
 // the spatial hash over their boundary segments (flatworld/spatialmanager.py,
 // flatworld/femspringmanager.py:421-487, 1035-1073) and the batched contact pass
-// (flatworld/mixedcontact.py:886-1025).  One workgroup per contact group.
+// (flatworld/mixedcontact.py:886-1025).  The hash build and the contribution
+// lists take one workgroup per contact group; the work-item pass spreads the work
+// items of every group over the GPU, four lanes per item.
 //
 // The reference runs both with atomics (the hash scatter, the Fext updates) and
 // with one query buffer shared by every thread.  Here every result that order
@@ -13,10 +15,11 @@
 // records, as the reference launches its contact kernels
 // (flatworld/mixedcontact.py:1654-1754).
 //
-// Phases hand data between threads of the workgroup through global memory behind
-// __syncthreads(), whose workgroup-scope release and acquire fences cover global
-// memory: every wave of a workgroup runs on one CU.  Integer atomics only count
-// and reserve places; every order-sensitive result is then sorted.
+// Within a per-group kernel, phases hand data between threads of the workgroup
+// through global memory behind __syncthreads(), whose workgroup-scope release and
+// acquire fences cover global memory: every wave of a workgroup runs on one CU.
+// Integer atomics only count and reserve places; every order-sensitive result is
+// then sorted.
 #include <hip/hip_runtime.h>
 
 #include "fw_types.h"
@@ -24,7 +27,7 @@
 namespace fw {
 namespace {
 
-constexpr int FLEX_BLOCK = 256;  // a group's work items and slots are looped over in strides
+constexpr int FLEX_BLOCK = 256;  // threads per workgroup; per-group loops stride over segments, cells and slots
 constexpr uint32_t FLEX_NO_CONTACT = 2u;  // FW_STEP_NO_CONTACT
 
 __device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
