@@ -257,3 +257,28 @@ def test_capped_hash_and_query_bitexact(caps):
         hits += int((cache >= 0).sum())
     log(f"[case] capped hash {caps}: 6 loops bit-exact after 700 substeps, {hits} cached segments")
     arena.close()
+
+
+def test_contact_beside_a_quiet_body_and_springs_bitexact():
+    """A loop holding two FEM bodies in contact, a FEM body outside every pair (its
+    edges are in the hash, but no work item targets it) and a spring lattice, on a
+    plane: the contact passes run beside the triangle and spring launches."""
+    from flatworld_amd import workloads as W
+
+    mat = fw.Elastic(2e4, 0.3, 50.0)
+    lower = ccw_grid(8, (0.1, 0.0), 0.3, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=21)
+    upper = ccw_grid(6, (0.15, 0.302), 0.2, fw.MisesPlastic(3e4, 0.3, 60.0, 200.0, 1e3), [fw.Gravity([0, -9.8])],
+                     jitter=0.1, seed=22, initials=[fw.InitialVel("ALL", [0.05, -0.3])], friction=0.2)
+    conn, coords = fw.structured_grid(5, lower=(0.32, 0.2), extent=0.15)
+    quiet = fw.FemDomain(fw.Mesh(2, np.ascontiguousarray(conn[:, [0, 2, 1]]), coords), fw.SolidProp(mat),
+                         [fw.Gravity([0, -9.8])], considerContact=False)
+    lattice = W.spring_lattice(6, (0.6, 0.3), 0.02, [200.0, 0.3, 0.01], [fw.Gravity([0.0, -9.8])])
+    lattice.considerContact = False
+    plane = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    loop = fw.ExplicitLoop(5e-5, [lower, upper, quiet, lattice, plane], useAdapativeDT=False)
+    g = loop.femSpringManager._flex
+    assert g.n_bodies == 3 and g.n_pairs == 1 and g.gated
+    log(f"[case] contact beside a quiet body and springs, items={g.n_items}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 6, check_every=2, what="quiet body and springs")
+    assert contact > 0
