@@ -27,6 +27,7 @@ FIELD_X, FIELD_V, FIELD_U, FIELD_FORCE, FIELD_MASS = 1, 2, 3, 4, 5
 FIELD_STRESS, FIELD_STRAIN, FIELD_EPS_P, FIELD_EQPS = 6, 7, 8, 9
 FIELD_AREA, FIELD_JINV, FIELD_YIELD, FIELD_CONTACT, FIELD_AABB = 10, 11, 12, 13, 14
 FLEX_CACHE, FLEX_GRID, FLEX_CELL_START, FLEX_CELL_END, FLEX_SORTED = 1, 2, 3, 4, 5
+FLEX_TIE_SEG, FLEX_TIE_STATE = 6, 7
 GROUND_FLEX = 4
 
 # (per_node, width, dtype)
@@ -123,6 +124,7 @@ class FlexDesc(ctypes.Structure):
         ("max_cells", ctypes.c_int32),
         ("max_query", ctypes.c_int32),
         ("max_pairs", ctypes.c_int32),
+        ("item_tied", _i32p),
     ]
 
 
@@ -179,6 +181,7 @@ SIGNATURES = {
     "fw_scene_mass_scaling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float]),
     "fw_flex_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(FlexDesc), ctypes.POINTER(ctypes.c_int32)]),
     "fw_flex_rebuild": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_flex_init_ties": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_flex_remove": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_flex_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
 }
@@ -460,6 +463,7 @@ class Arena:
             pair_body=np.ascontiguousarray(g.pair_body, dtype=np.int32).reshape(-1),
             body_bnodes_off=np.ascontiguousarray(g.body_bnodes_off, dtype=np.int32),
             body_bnodes=np.ascontiguousarray(g.body_bnodes, dtype=np.int32),
+            item_tied=np.ascontiguousarray(g.tied_flags(), dtype=np.int32),
         )
         d = FlexDesc()
         d.n_bodies = int(len(scene_ids))
@@ -478,6 +482,9 @@ class Arena:
     def flex_rebuild(self, gid):
         check(self._L.fw_flex_rebuild(self.h, int(gid)), "fw_flex_rebuild")
 
+    def flex_init_ties(self, gid):
+        check(self._L.fw_flex_init_ties(self.h, int(gid)), "fw_flex_init_ties")
+
     def flex_remove(self, gid):
         check(self._L.fw_flex_remove(self.h, int(gid)), "fw_flex_remove")
         self._flex.pop(gid, None)
@@ -489,8 +496,12 @@ class Arena:
             check(self._L.fw_flex_read(self.h, int(gid), field, raw.ctypes.data, raw.nbytes), "fw_flex_read")
             return dict(glb=raw[:2].view(np.float32).copy(), gs=raw[2:4].view(np.float32).copy(),
                         cn=raw[4:6].copy(), tc=int(raw[6]), exact=bool(raw[7]))
+        if field == FLEX_TIE_STATE:
+            out = np.zeros((g.n_items, 4), np.float32)
+            check(self._L.fw_flex_read(self.h, int(gid), field, out.ctypes.data, out.nbytes), "fw_flex_read")
+            return out
         count = {FLEX_CACHE: g.n_items, FLEX_CELL_START: g.max_cells, FLEX_CELL_END: g.max_cells,
-                 FLEX_SORTED: g.max_pairs}[field]
+                 FLEX_SORTED: g.max_pairs, FLEX_TIE_SEG: g.n_items}[field]
         out = np.zeros(count, np.int32)
         check(self._L.fw_flex_read(self.h, int(gid), int(field), out.ctypes.data, out.nbytes), "fw_flex_read")
         return out

@@ -43,6 +43,8 @@ extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileD
                                              float dt_target, int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
                                           const ArenaCtl* ctl, int row, hipStream_t stream);
+extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_items, const SceneDev* scenes,
+                                          const ArenaCtl* ctl, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
                                              uint32_t flags, hipStream_t stream);
@@ -557,7 +559,7 @@ struct Staging {
 struct FlexHost {
     int64_t block_off = 0, block_bytes = 0;
     std::vector<int32_t> scenes;
-    int32_t n_items = 0, max_cells = 0, max_pairs = 0;
+    int32_t n_items = 0, max_cells = 0, max_pairs = 0, n_tied = 0;
     FlexDev dev{};
 };
 
@@ -1778,8 +1780,14 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
             if (seg_slot[k].x < 0 || seg_slot[k].y < 0) return fail(FW_E_INVALID, "segment node is not a boundary node");
         }
     }
-    std::vector<int32_t> item_node(W), item_slot(W), need(B, 0);
+    std::vector<int32_t> item_node(W), item_slot(W), need(B, 0), item_tied(W, 0);
+    int n_tied = 0;
     for (int w = 0; w < W; ++w) {
+        if (d->item_tied) {
+            if (d->item_tied[w] != 0 && d->item_tied[w] != 1) return fail(FW_E_INVALID, "item_tied entries are 0 or 1");
+            item_tied[w] = d->item_tied[w];
+            n_tied += item_tied[w];
+        }
         const int b = d->item_body[w], t = d->item_target[w], q = d->item_pair[w];
         if (b < 0 || b >= B || t < 0 || t >= B || b == t || q < 0 || q >= P || !in_pair[b] || !in_pair[t])
             return fail(FW_E_INVALID, "bad work item " + std::to_string(w));
@@ -1820,14 +1828,16 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
                   o_nn = take(4 * B), o_snode = take(4 * NS), o_sbody = take(4 * NS), o_segb = take(4 * S),
                   o_segn = take(8 * S), o_segs = take(8 * S), o_ib = take(4 * W), o_in = take(4 * W),
                   o_is = take(4 * W), o_it = take(4 * W), o_ip = take(4 * W), o_ipen = take(4 * W),
-                  o_ifr = take(4 * W), o_cache = take(4 * W), o_pair = take(8 * P);
+                  o_ifr = take(4 * W), o_cache = take(4 * W), o_pair = take(8 * P), o_tied = take(4 * W),
+                  o_tseg = take(4 * W);
     const int64_t host_bytes = o;
     const int64_t o_grid = take(sizeof(FlexGrid)), o_box = take(16 * S), o_scell = take(16 * S), o_segp = take(4 * S),
                   o_crng = take(8 * (int64_t)d->max_cells),
                   o_cs = take(4 * (int64_t)d->max_cells), o_ce = take(4 * (int64_t)d->max_cells),
                   o_sorted = take(4 * (int64_t)d->max_pairs), o_cent = take(4 * (int64_t)d->max_pairs),
                   o_ri = take(8 * W), o_rf = take(16 * W),
-                  o_slst = take(4 * NS), o_slcur = take(4 * NS), o_ek = take(12 * (int64_t)W), o_ef = take(24 * (int64_t)W);
+                  o_slst = take(4 * NS), o_slcur = take(4 * NS), o_ek = take(12 * (int64_t)W), o_ef = take(24 * (int64_t)W),
+                  o_twg = take(16 * (int64_t)W);
     const int64_t total = o;
     const int64_t off = arena_alloc(a, total);
     if (off < 0) return fail(FW_E_CAPACITY, "arena full: FEM-FEM group needs " + std::to_string(total) + " bytes");
@@ -1858,7 +1868,9 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     {
         std::vector<int32_t> minus(W, -1);
         put(o_cache, minus.data(), 4 * W);
+        put(o_tseg, minus.data(), 4 * W);
     }
+    put(o_tied, item_tied.data(), 4 * W);
     put(o_pair, d->pair_body, 8 * P);
     int rc = upload(a, blk, img.data(), host_bytes);
     if (rc) {
@@ -1874,6 +1886,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     F->n_items = W;
     F->max_cells = d->max_cells;
     F->max_pairs = d->max_pairs;
+    F->n_tied = n_tied;
     FlexDev& D = F->dev;
     D.n_bodies = B;
     D.n_segs = S;
@@ -1903,6 +1916,9 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     D.item_friction = (const float*)(blk + o_ifr);
     D.cache = (int32_t*)(blk + o_cache);
     D.pair_body = (const int2*)(blk + o_pair);
+    D.item_tied = (const int32_t*)(blk + o_tied);
+    D.tie_seg = (int32_t*)(blk + o_tseg);
+    D.tie_wg = (float4*)(blk + o_twg);
     D.grid = (FlexGrid*)(blk + o_grid);
     D.seg_box = (float4*)(blk + o_box);
     D.seg_cells = (int4*)(blk + o_scell);
@@ -1955,6 +1971,22 @@ int32_t fw_flex_rebuild(fw_arena* a, int32_t gid) {
     return FW_OK;
 }
 
+int32_t fw_flex_init_ties(fw_arena* a, int32_t gid) {
+    if (!a || gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]) return fail(FW_E_NOT_FOUND, "no FEM-FEM group");
+    HIPCHK(hipSetDevice(a->device));
+    int rc = ensure_tiles(a);
+    if (rc) return rc;
+    rc = ensure_flex(a);
+    if (rc) return rc;
+    const FlexHost& F = *a->flex[gid];
+    const SceneHost* s0 = get_scene(a, F.scenes[0]);
+    if (!s0 || a->clock < s0->s0)
+        return fail(FW_E_INVALID, "group " + std::to_string(gid) + " has not run its warm-up substep yet");
+    if (F.n_tied == 0) return FW_OK;
+    HIPCHK(fw_launch_flex_ties(a->d_flex, a->flex_row[gid], F.n_items, a->d_scenes, a->d_ctl, a->stream));
+    return FW_OK;
+}
+
 int32_t fw_flex_remove(fw_arena* a, int32_t gid) {
     if (!a || gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]) return fail(FW_E_NOT_FOUND, "no FEM-FEM group");
     HIPCHK(hipSetDevice(a->device));
@@ -1986,6 +2018,8 @@ int32_t fw_flex_read(fw_arena* a, int32_t gid, int32_t field, void* out, int64_t
         case FW_FLEX_CELL_START: src = F.dev.cell_start; bytes = 4 * (int64_t)F.max_cells; break;
         case FW_FLEX_CELL_END: src = F.dev.cell_end; bytes = 4 * (int64_t)F.max_cells; break;
         case FW_FLEX_SORTED: src = F.dev.sorted; bytes = 4 * (int64_t)F.max_pairs; break;
+        case FW_FLEX_TIE_SEG: src = F.dev.tie_seg; bytes = 4 * (int64_t)F.n_items; break;
+        case FW_FLEX_TIE_STATE: src = F.dev.tie_wg; bytes = 16 * (int64_t)F.n_items; break;
         default: return fail(FW_E_INVALID, "unknown FEM-FEM field " + std::to_string(field));
     }
     if (out_bytes < bytes) return fail(FW_E_INVALID, "output buffer too small: need " + std::to_string(bytes));

@@ -347,7 +347,10 @@ struct Best {
 // WALK_BATCH at a time, and keep the best qualifying candidate among the segments
 // of body tb met for the first time; `met` counts those first meetings.  A single
 // lane (stride 1) stops evaluating once max_query segments were met, as the
-// reference's query list stops growing.
+// reference's query list stops growing.  The candidate rule is the contact
+// kernel's (deepest penetration below 1e-4) or, with NEAREST, the tie
+// initialisation's (smallest |penetration|, flatworld/mixedcontact.py:757-762).
+template <bool NEAREST>
 __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, float2 p, int tb, const float2* tpos,
                      float qbuf, int lane, int stride, bool exact, int max_query, int& met, Best& best) {
     const int nyq = q.uy - q.ly + 1, ncell = (q.ux - q.lx + 1) * nyq;
@@ -379,7 +382,9 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
                 const int2 sn = F.seg_node[s];
                 float nx, ny, w0, w1;
                 const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
-                if (pen < best.pen && pen < 1e-4f && fabsf(pen) < qbuf)
+                const bool better = NEAREST ? (fabsf(pen) < fabsf(best.pen) && fabsf(pen) < qbuf)
+                                            : (pen < best.pen && pen < 1e-4f && fabsf(pen) < qbuf);
+                if (better)
                     best.take(pen, nx, ny, w0, w1, s, ((long long)t << 32) | (long long)pp);
             }
         }
@@ -407,6 +412,54 @@ __device__ __forceinline__ bool pair_active(const FlexDev& F, const SceneDev* sc
         }
     }
     return a[0] <= b[2] && a[2] >= b[0] && a[1] <= b[3] && a[3] >= b[1];
+}
+
+// _eval_fixed_tie_point and _fem_fem_tie_elem_normal (flatworld/mixedcontact.py:136-163):
+// the weighted point of the edge and its left unit normal, (0, 1) for an edge
+// shorter than 1e-10.
+__device__ __forceinline__ float2 tie_point(float2 a, float2 c, float w0, float w1) {
+    return make_float2((0.0f + a.x * w0) + c.x * w1, (0.0f + a.y * w0) + c.y * w1);
+}
+__device__ __forceinline__ float2 tie_normal(float2 a, float2 c) {
+    const float ex = c.x - a.x, ey = c.y - a.y;
+    const float elen = len2(ex, ey);
+    float tx = 1.0f, ty = 0.0f;
+    if (elen > 1e-10f) {
+        tx = ex / elen;
+        ty = ey / elen;
+    }
+    return make_float2(-ty, tx);
+}
+
+// _initialize_fem_fem_ties_once_kernel (flatworld/mixedcontact.py:687-770) at the
+// state the next substep starts from: every tied, unresolved work item of group
+// row `row` takes the edge of its target body nearest to it (smallest
+// |penetration| within the query buffer) and keeps the edge, the weights and the
+// normal gap.  One lane per work item, each with its own query walk.
+__global__ __launch_bounds__(FLEX_BLOCK) void flex_ties_kernel(const FlexDev* groups, int row, const SceneDev* scenes,
+                                                               const ArenaCtl* ctl) {
+    const FlexDev& F = groups[row];
+    const int w = (int)(blockIdx.x * FLEX_BLOCK + threadIdx.x);
+    if (w >= F.n_items || !F.item_tied[w] || F.tie_seg[w] >= 0) return;
+    const SceneDev* S0 = scenes + F.body_scene[0];
+    const int rd = (ctl->clock - S0->s0 + 1) & 1;
+    const FlexGrid g = *F.grid;
+    const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.001f);
+    const float2 p = scenes[F.body_scene[F.item_body[w]]].pos[rd][F.item_node[w]];
+    const int tb = F.item_target[w];
+    const float2* tpos = scenes[F.body_scene[tb]].pos[rd];
+    if (g.tc <= 0) return;
+    const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
+    int met = 0;
+    Best b;
+    walk<true>(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, b);
+    if (!b.found) return;
+    const int2 sn = F.seg_node[b.s];
+    const float2 a = tpos[sn.x], c = tpos[sn.y];
+    const float2 cp = tie_point(a, c, b.w0, b.w1);
+    const float2 n = tie_normal(a, c);
+    F.tie_seg[w] = b.s;
+    F.tie_wg[w] = make_float4(b.w0, b.w1, dot2(p.x - cp.x, p.y - cp.y, n.x, n.y), 0.0f);
 }
 
 // The substep a group's contact pass serves, or -1 when it has none this launch.
@@ -443,9 +496,36 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
     const int rd = (local + 1) & 1;
     const int ar3 = mod3_(local);
     const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
+    int2 ri = make_int2(0, -1);
+    if (F.item_tied[w]) {
+        // _apply_fem_fem_ties_fixed_kernel (flatworld/mixedcontact.py:773-811): a
+        // spring toward the resolved edge point at the initial normal gap, every
+        // substep, whatever the pair gate; tied items never slide
+        const int ts = F.tie_seg[w];
+        if (sub != 0) return;
+        if (ts >= 0) {
+            const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
+            const float2 x = SB->pos[rd][F.item_node[w]];
+            const float2* tpos = scenes[F.body_scene[F.item_target[w]]].pos[rd];
+            const int2 sn = F.seg_node[ts];
+            const float2 a = tpos[sn.x], c = tpos[sn.y];
+            const float4 wg = F.tie_wg[w];
+            const float2 cp = tie_point(a, c, wg.x, wg.y);
+            const float2 n = tie_normal(a, c);
+            const float k = F.item_penalty[w];
+            const float gx = cp.x + wg.z * n.x, gy = cp.y + wg.z * n.y;
+            F.res_f[w] = make_float4(k * (gx - x.x), k * (gy - x.y), wg.x, wg.y);
+            const int2 ss = F.seg_slot[ts];
+            atomicAdd(&F.slot_start[F.item_slot[w]], 1);
+            atomicAdd(&F.slot_start[ss.x], 1);
+            atomicAdd(&F.slot_start[ss.y], 1);
+            ri = make_int2(1, ts);
+        }
+        F.res_i[w] = ri;
+        return;
+    }
     const FlexGrid g = *F.grid;
     const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.016f);
-    int2 ri = make_int2(0, -1);
     if (pair_active(F, scenes, F.item_pair[w], ar3, pad)) {
         const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
         const int nid = F.item_node[w];
@@ -496,7 +576,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
                 // are met (else the cap decides and the serial walk is repeated)
                 int met = 0;
                 Best lb;
-                walk(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
+                walk<false>(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
 #pragma unroll
                 for (int off = 1; off < FLEX_LPI; off <<= 1) met += __shfl_xor(met, off, FLEX_LPI);
                 if (met > F.max_query) {
@@ -528,7 +608,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             if (serial) {
                 int met = 0;
                 Best sb;
-                walk(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
+                walk<false>(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
                 if (sb.found) {
                     b = sb;
                     cache_new = sb.s;
@@ -591,7 +671,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* g
                                make_float2(-(rf.x * rf.w), -(rf.y * rf.w))};
         for (int c = 0; c < 3; ++c) {
             const int at = atomicAdd(&F.slot_cur[sl[c]], 1);
-            F.ent_key[at] = w;
+            F.ent_key[at] = F.item_tied[w] ? w : F.n_items + w;  // a node's tie springs before its contacts
             F.ent_f[at] = val[c];
         }
     }
@@ -630,6 +710,14 @@ extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, c
     if (n_groups <= 0) return hipSuccess;
     hipLaunchKernelGGL(fw::flex_hash_kernel, dim3(row >= 0 ? 1 : n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups,
                        scenes, ctl, row);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_items, const SceneDev* scenes,
+                                          const ArenaCtl* ctl, hipStream_t stream) {
+    if (n_items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fw::flex_ties_kernel, dim3((n_items + fw::FLEX_BLOCK - 1) / fw::FLEX_BLOCK),
+                       dim3(fw::FLEX_BLOCK), 0, stream, groups, row, scenes, ctl);
     return hipGetLastError();
 }
 

@@ -72,6 +72,11 @@ struct FlexDev {
     const float* item_friction;    // [W]
     int32_t* cache;                // [W] cached segment or -1
     const int2* pair_body;         // [P]
+    // tied pairs: per work item the tie flag, the edge flex_ties_kernel resolved
+    // (-1: none yet) and (weight 0, weight 1, normal gap, 0)
+    const int32_t* item_tied;      // [W]
+    int32_t* tie_seg;              // [W]
+    float4* tie_wg;                // [W]
     // spatial hash, rebuilt by flex_hash_kernel
     FlexGrid* grid;
     float4* seg_box;               // [S] segment boxes of the last build

@@ -55,14 +55,19 @@ class ExplicitLoop:
         device=0,
         arena_kwargs=None,
     ):
-        if len(joints) > 0 or ties:
-            raise NotImplementedError("joints and ties act on rigid bodies, which are not part of this package")
+        if len(joints) > 0:
+            raise NotImplementedError("joints act on rigid bodies, which are not part of this package")
         for d in domains:
             if d.type not in _BODY_TYPES + _GROUND_TYPES:
                 raise NotImplementedError(f"domain type {d.type:#b} (rigid) is not part of this package")
         self.domains = domains
         self.d = domains[0].d
         self.numDomains = len(domains)
+        # ties: index pairs of domains whose contact is tied (flatworld/explicitloop.py:111-113, 400-439);
+        # only FEM-FEM pairs act on it, as in the reference's batched kernels
+        self.ties = list(ties) if ties is not None else []
+        self._tie_pairs = set()
+        self._refresh_tie_pairs()
         self._max_domains = self.numDomains + 10
         self.damping = damping
         self.useAdapative = useAdapativeDT
@@ -92,6 +97,8 @@ class ExplicitLoop:
                     continue  # ground-ground: no contact object in the reference either
                 c = contactHelper(dom_i, dom_j, ctype)
                 if c is not None:
+                    if self._is_tie_pair(i, j):
+                        c.tied = True
                     self.contacts.append(c)
                     self.contactPairIds.append((i, j))
 
@@ -156,6 +163,39 @@ class ExplicitLoop:
         else:
             self.dt = dt
         self.default_dt = dt
+
+    # ------------------------------------------------------------ ties
+    def _parse_tie_entry(self, tie):
+        """A pair of domain indices (negative ones count from the end) as (low, high), or
+        None for anything else, an index out of range or a domain tied to itself
+        (flatworld/explicitloop.py:400-420)."""
+        if not isinstance(tie, (tuple, list)) or len(tie) != 2:
+            return None
+        a, b = tie
+        if not isinstance(a, (int, np.integer)) or not isinstance(b, (int, np.integer)):
+            return None
+        n = len(self.domains)
+        ia, ib = int(a), int(b)
+        ia += n if ia < 0 else 0
+        ib += n if ib < 0 else 0
+        if not (0 <= ia < n and 0 <= ib < n) or ia == ib:
+            return None
+        return (min(ia, ib), max(ia, ib))
+
+    def _refresh_tie_pairs(self):
+        self._tie_pairs.clear()
+        bad = 0
+        for tie in self.ties:
+            pair = self._parse_tie_entry(tie)
+            if pair is None:
+                bad += 1
+            else:
+                self._tie_pairs.add(pair)
+        if bad:
+            print(f"Warning: {bad} tie entries could not be resolved and were ignored")
+
+    def _is_tie_pair(self, idx_a, idx_b):
+        return (min(idx_a, idx_b), max(idx_a, idx_b)) in self._tie_pairs
 
     # ------------------------------------------------------------ stepping
     def advance(self):

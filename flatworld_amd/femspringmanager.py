@@ -143,6 +143,7 @@ class FemSpringManager:
         self._flex_gid = None
         self._sh_rebuild_interval = REBUILD_INTERVAL
         self._sh_fem_elapsed = 0.0
+        self._ties_done = False
         if flex is not None:
             bodies = [sid for sid, (_, dom) in zip(self._scene_ids, fem) if dom.type == DomainType.FEM]
             self._flex_gid = self._arena.flex_add(flex, bodies)
@@ -174,15 +175,25 @@ class FemSpringManager:
             self._arena.step(n, dt32, flags, damping_override=damp32)
             return
         seq = [float(dt)] * n if dt64 is None else [float(v) for v in dt64]
+        n_tied = self._flex.n_tied
         pending = 0
         for k in range(n):
-            self._sh_fem_elapsed += seq[k]
-            if self._sh_fem_elapsed >= self._sh_rebuild_interval:
+            # the hash is refreshed while sliding items exist, or until the ties are
+            # resolved (flatworld/mixedcontact.py:1668-1689)
+            if n_tied < self._flex.n_items or not self._ties_done:
+                self._sh_fem_elapsed += seq[k]
+                if self._sh_fem_elapsed >= self._sh_rebuild_interval:
+                    if pending:
+                        self._arena.step(pending, dt32, flags & ~_native.FW_STEP_DIAG, damping_override=damp32)
+                        pending = 0
+                    self._arena.flex_rebuild(self._flex_gid)
+                    self._sh_fem_elapsed = 0.0
+            if n_tied > 0 and not self._ties_done:
                 if pending:
                     self._arena.step(pending, dt32, flags & ~_native.FW_STEP_DIAG, damping_override=damp32)
                     pending = 0
-                self._arena.flex_rebuild(self._flex_gid)
-                self._sh_fem_elapsed = 0.0
+                self._arena.flex_init_ties(self._flex_gid)
+                self._ties_done = True
             pending += 1
         if pending:
             self._arena.step(pending, dt32, flags, damping_override=damp32)
@@ -200,6 +211,12 @@ class FemSpringManager:
         if self._sh_fem_elapsed >= self._sh_rebuild_interval:
             self.populate_fem_spatial_hash(velocity_buffer=0.0)
             self._sh_fem_elapsed = 0.0
+
+    def flex_ties(self):
+        """Resolved tie edge (-1: none) and (weight 0, weight 1, normal gap) of every work item."""
+        seg = self._arena.flex_read(self._flex_gid, _native.FLEX_TIE_SEG)
+        state = self._arena.flex_read(self._flex_gid, _native.FLEX_TIE_STATE)
+        return seg, state[:, :3].copy()
 
     def flex_cache(self):
         """Cached target segment of every FEM-FEM work item (-1: none)."""

@@ -10,7 +10,11 @@ Follows the FlexFlex part of MixedContact._build_batched_contacts
   checked against ``domain1``'s edges, then ``domain1``'s against ``domain2``'s,
   with the pair's float32 penalty and the larger of the two frictions;
 * hash limits: 100000 cells, 512 query results, and 9 pairs per element slot of
-  ``MAX_BOUNDARY_ELEMENTS = max(boundary elements + 1024, MAX_NODES)``.
+  ``MAX_BOUNDARY_ELEMENTS = max(boundary elements + 1024, MAX_NODES)``;
+* a tied contact (``contact.tied``, set by ExplicitLoop's ``ties``) flags every
+  work item of its pair; tied items are resolved once to their nearest edge and
+  then held there by a penalty spring instead of sliding
+  (flatworld/mixedcontact.py:687-811, 1871-1932).
 
 The contact itself runs on the device (flatworld_amd/csrc/fw_flex.hip).
 """
@@ -41,6 +45,7 @@ class FlexGroupSpec:
     item_friction: np.ndarray
     pair_body: np.ndarray  # int32[P, 2]
     gated: bool
+    item_tied: np.ndarray = None  # int32[W], 1 where the item's pair is tied (None: none)
     max_cells: int = MAX_CELLS
     max_query: int = MAX_QUERY
     max_pairs: int = 9 * 4096
@@ -61,6 +66,13 @@ class FlexGroupSpec:
     @property
     def n_pairs(self):
         return int(self.pair_body.shape[0])
+
+    def tied_flags(self):
+        return np.zeros(self.n_items, np.int32) if self.item_tied is None else np.asarray(self.item_tied, np.int32)
+
+    @property
+    def n_tied(self):
+        return int(self.tied_flags().sum())
 
     def in_pair(self, b):
         return bool(np.any(self.pair_body == b))
@@ -87,11 +99,12 @@ def build_flex_group(fem_domains, flex_contacts, gated, count_nodes):
         bn = np.asarray(m.boundaryNodes, dtype=np.int32) if m.numBoundNodes else np.zeros(0, np.int32)
         bns.append(bn)
         bn_off.append(bn_off[-1] + bn.shape[0])
-    ib, inode, it, ip, ipen, ifr, pairs = [], [], [], [], [], [], []
+    ib, inode, it, ip, ipen, ifr, itied, pairs = [], [], [], [], [], [], [], []
     for q, c in enumerate(flex_contacts):
         d1, d2 = idx[id(c.domain1)], idx[id(c.domain2)]
         pen = np.float32(float(c.penalty))
         fric = np.float32(max(float(c.domain1.friction), float(c.domain2.friction)))
+        tied = 1 if getattr(c, "tied", False) else 0
         pairs.append((d1, d2))
         for src, dst in ((d2, d1), (d1, d2)):
             n = bns[src].shape[0]
@@ -101,6 +114,7 @@ def build_flex_group(fem_domains, flex_contacts, gated, count_nodes):
             ip.append(np.full(n, q, np.int32))
             ipen.append(np.full(n, pen, np.float32))
             ifr.append(np.full(n, fric, np.float32))
+            itied.append(np.full(n, tied, np.int32))
     cat = lambda a, t: np.ascontiguousarray(np.concatenate(a).astype(t)) if a else np.zeros(0, t)  # noqa: E731
     max_nodes = max(count_nodes + 1024, 2048)
     max_elements = max(n_be + 1024, max_nodes)
@@ -117,5 +131,6 @@ def build_flex_group(fem_domains, flex_contacts, gated, count_nodes):
         item_friction=cat(ifr, np.float32),
         pair_body=np.asarray(pairs, np.int32).reshape(-1, 2),
         gated=bool(gated),
+        item_tied=cat(itied, np.int32),
         max_pairs=9 * max_elements,
     )

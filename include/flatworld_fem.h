@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 3
+#define FW_ABI_VERSION 4
 #define FW_MAX_GROUNDS 4
 
 /* status codes */
@@ -203,6 +203,9 @@ typedef struct {
     int32_t max_cells;            /* spatial hash limits (SpatialHashManager defaults: 100000, 512) */
     int32_t max_query;
     int32_t max_pairs;
+    /* [n_items] 1 where the item's pair is tied (ExplicitLoop ties=, contact.tied),
+       0 elsewhere; NULL: no tied pair (flatworld/mixedcontact.py:1871-1932) */
+    const int32_t* item_tied;
 } fw_flex_desc;
 
 /* fields for fw_flex_read */
@@ -212,6 +215,8 @@ typedef struct {
 #define FW_FLEX_CELL_START 3 /* int32[max_cells] */
 #define FW_FLEX_CELL_END 4   /* int32[max_cells] */
 #define FW_FLEX_SORTED 5     /* int32[max_pairs] segment ids, ascending within each cell */
+#define FW_FLEX_TIE_SEG 6    /* int32[n_items] resolved tie edge per work item, -1 for none */
+#define FW_FLEX_TIE_STATE 7  /* float32[4 * n_items] weight 0, weight 1, normal gap, 0 */
 
 typedef struct fw_arena fw_arena;
 typedef struct fw_prepared fw_prepared;
@@ -269,9 +274,19 @@ int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, in
  * reference's rebuild schedule (every 1/500 s of simulated time,
  * flatworld/femspringmanager.py:1035-1039).  fw_step then runs the group's
  * contact pass before every substep.  fw_step_graph and FW_STEP_PER_SCENE are
- * refused while a group exists. */
+ * refused while a group exists.
+ * Tied work items never slide: fw_flex_init_ties enqueues, at the state the next
+ * substep starts from and with the group's current hash, the one-time resolution
+ * of every tied, unresolved item to its nearest edge
+ * (_initialize_fem_fem_ties_once_kernel, flatworld/mixedcontact.py:687-770, which
+ * run_batched_contacts launches once, after the hash refresh of the first
+ * substep, mixedcontact.py:1668-1689); from then on every substep's contact
+ * pass applies a penalty spring toward the resolved edge point at the initial
+ * normal gap (_apply_fem_fem_ties_fixed_kernel, mixedcontact.py:773-811), each
+ * node's tie contributions before its sliding-contact ones. */
 int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* desc, int32_t* group_id);
 int32_t fw_flex_rebuild(fw_arena* a, int32_t group_id);
+int32_t fw_flex_init_ties(fw_arena* a, int32_t group_id);
 int32_t fw_flex_remove(fw_arena* a, int32_t group_id);
 int32_t fw_flex_read(fw_arena* a, int32_t group_id, int32_t field, void* out, int64_t out_bytes);
 

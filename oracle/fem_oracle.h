@@ -147,6 +147,12 @@ typedef struct {
     int32_t n_pairs;
     const int32_t* pair_body;    /* [2 n_pairs] */
     int32_t gated;               /* 0: every pair active (the loop skips its BVH) */
+    /* tied pairs (mixedcontact.py:687-811): per work item the tie flag and the
+       state ora_flex_ties_init resolves once (edge, weights, normal gap) */
+    const int32_t* item_tied;    /* [n_items] 1: the item's pair is tied */
+    int32_t* tie_elem;           /* [n_items] resolved edge or -1 */
+    float* tie_w;                /* [2 n_items] */
+    float* tie_gap;              /* [n_items] */
     /* spatial hash */
     int32_t max_cells, max_query, max_pairs;
     float glb[2], gub[2], gs[2];
@@ -167,8 +173,14 @@ void ora_flex_hash_build(ora_flex* f, ora_scene* const* bodies);
 /* query_point_with_buffer (spatialmanager.py:131-199): segments of body `target`
  * in the cells the buffered point overlaps, deduplicated, at most max_query. */
 int32_t ora_flex_query(const ora_flex* f, const float p[2], float buffer, int32_t target, int32_t* out);
-/* One launch of the batched FEM-FEM kernel at the bodies' current state: updates
- * the cache and writes each body's contribution lists (c_off / c_f). */
+/* _initialize_fem_fem_ties_once_kernel (mixedcontact.py:687-770) at the bodies'
+ * current state: every tied, unresolved work item takes its nearest edge. */
+void ora_flex_ties_init(ora_flex* f, ora_scene* const* bodies);
+/* One run of the FEM-FEM kernels at the bodies' current state: the tie springs of
+ * resolved tied items (mixedcontact.py:773-811), then the sliding contact of the
+ * others (886-1025); updates the cache and writes each body's contribution lists
+ * (c_off / c_f): per node the tie contributions in ascending work item, then the
+ * contact contributions in ascending work item, the order of the two launches. */
 void ora_flex_contacts(ora_flex* f, ora_scene* const* bodies, int32_t counter);
 int32_t ora_gate_pair(const float alb[2], const float aub[2], const float blb[2], const float bub[2],
                       int32_t counter);

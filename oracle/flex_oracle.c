@@ -231,6 +231,63 @@ typedef struct {
     float tf[2], w[2];
 } item_result;
 
+/* _fem_fem_tie_elem_normal (mixedcontact.py:150-163): the edge's left normal
+ * (-t.y, t.x), the opposite of the contact normal, (1, 0) for a degenerate edge. */
+static void tie_normal(const float a[2], const float c[2], float n[2]) {
+    const float ex = c[0] - a[0], ey = c[1] - a[1];
+    const float elen = len2f(ex, ey);
+    float tx = 1.0f, ty = 0.0f;
+    if (elen > 1e-10f) {
+        tx = ex / elen;
+        ty = ey / elen;
+    }
+    n[0] = -ty;
+    n[1] = tx;
+}
+
+/* _eval_fixed_tie_point (mixedcontact.py:136-147) */
+static void tie_point(const float a[2], const float c[2], const float w[2], float cp[2]) {
+    cp[0] = (0.0f + a[0] * w[0]) + c[0] * w[1];
+    cp[1] = (0.0f + a[1] * w[0]) + c[1] * w[1];
+}
+
+void ora_flex_ties_init(ora_flex* f, ora_scene* const* bodies) {
+    const float qbuf = maxf_(maxf_(1e-4f, maxf_(f->gs[0], f->gs[1])), 0.001f);
+    int32_t* list = (int32_t*)malloc(sizeof(int32_t) * (size_t)(f->max_query > 0 ? f->max_query : 1));
+    for (int32_t w = 0; w < f->n_items; ++w) {
+        if (!f->item_tied[w] || f->tie_elem[w] >= 0) continue;
+        const ora_scene* B = bodies[f->item_body[w]];
+        const int32_t nid = f->item_node[w];
+        const float p[2] = {B->x[2 * nid], B->x[2 * nid + 1]};
+        float best = 1e9f, bw[2] = {0.0f, 0.0f};
+        int32_t be = -1;
+        const int32_t cnt = ora_flex_query(f, p, qbuf, f->item_target[w], list);
+        for (int32_t j = 0; j < cnt; ++j) {
+            const int32_t s = list[j];
+            float n[2], wt[2];
+            const float pen = detect(p, seg_xy(f, bodies, s, 0), seg_xy(f, bodies, s, 1), qbuf, n, wt);
+            if (fabsf(pen) < fabsf(best) && fabsf(pen) < qbuf) {
+                best = pen;
+                bw[0] = wt[0];
+                bw[1] = wt[1];
+                be = s;
+            }
+        }
+        if (be >= 0) {
+            const float* a = seg_xy(f, bodies, be, 0);
+            const float* c = seg_xy(f, bodies, be, 1);
+            float cp[2], n[2];
+            tie_point(a, c, bw, cp);
+            tie_normal(a, c, n);
+            f->tie_elem[w] = be;
+            f->tie_w[2 * w] = bw[0];
+            f->tie_w[2 * w + 1] = bw[1];
+            f->tie_gap[w] = dot2f(p[0] - cp[0], p[1] - cp[1], n[0], n[1]);
+        }
+    }
+    free(list);
+}
+
 /* _batched_flexflex_contact_kernel_sh (mixedcontact.py:886-1025) for every work
  * item, then the node-major contribution lists. */
 void ora_flex_contacts(ora_flex* f, ora_scene* const* bodies, int32_t counter) {
@@ -250,8 +307,29 @@ void ora_flex_contacts(ora_flex* f, ora_scene* const* bodies, int32_t counter) {
     const float min_buf = maxf_(f->gs[0], f->gs[1]);
     const float qbuf = maxf_(maxf_(1e-4f, min_buf), 0.016f);
 
+    /* tie springs: every resolved tied item, no pair gate (mixedcontact.py:773-811) */
+    item_result* tie = (item_result*)calloc((size_t)(W > 0 ? W : 1), sizeof(item_result));
     for (int32_t w = 0; w < W; ++w) {
-        if (!active[f->item_pair[w]]) continue;
+        if (!f->item_tied[w] || f->tie_elem[w] < 0) continue;
+        const ora_scene* B = bodies[f->item_body[w]];
+        const int32_t nid = f->item_node[w], s = f->tie_elem[w];
+        const float* a = seg_xy(f, bodies, s, 0);
+        const float* c = seg_xy(f, bodies, s, 1);
+        float cp[2], n[2];
+        tie_point(a, c, f->tie_w + 2 * w, cp);
+        tie_normal(a, c, n);
+        const float gap = f->tie_gap[w], k = f->item_penalty[w];
+        const float tx = cp[0] + gap * n[0], ty = cp[1] + gap * n[1];
+        tie[w].hit = 1;
+        tie[w].seg = s;
+        tie[w].tf[0] = k * (tx - B->x[2 * nid]);
+        tie[w].tf[1] = k * (ty - B->x[2 * nid + 1]);
+        tie[w].w[0] = f->tie_w[2 * w];
+        tie[w].w[1] = f->tie_w[2 * w + 1];
+    }
+
+    for (int32_t w = 0; w < W; ++w) {
+        if (f->item_tied[w] || !active[f->item_pair[w]]) continue;
         const ora_scene* B = bodies[f->item_body[w]];
         const int32_t nid = f->item_node[w];
         const float p[2] = {B->x[2 * nid], B->x[2 * nid + 1]};
@@ -359,16 +437,18 @@ void ora_flex_contacts(ora_flex* f, ora_scene* const* bodies, int32_t counter) {
                 }
             }
         }
-        for (int32_t w = 0; w < W; ++w) {
-            if (!res[w].hit) continue;
-            const int32_t s = res[w].seg;
+        for (int32_t r = 0; r < 2 * W; ++r) {
+            const item_result* R = r < W ? tie + r : res + (r - W);  /* ties first, then contacts */
+            const int32_t w = r < W ? r : r - W;
+            if (!R->hit) continue;
+            const int32_t s = R->seg;
             const int32_t tgt[3][2] = {{f->item_body[w], f->item_node[w]},
                                        {f->seg_body[s], f->seg_node[2 * s]},
                                        {f->seg_body[s], f->seg_node[2 * s + 1]}};
-            const float tx = res[w].tf[0], ty = res[w].tf[1];
+            const float tx = R->tf[0], ty = R->tf[1];
             const float val[3][2] = {{tx, ty},
-                                     {-(tx * res[w].w[0]), -(ty * res[w].w[0])},
-                                     {-(tx * res[w].w[1]), -(ty * res[w].w[1])}};
+                                     {-(tx * R->w[0]), -(ty * R->w[0])},
+                                     {-(tx * R->w[1]), -(ty * R->w[1])}};
             for (int k = 0; k < 3; ++k) {
                 int32_t* off = f->c_off[tgt[k][0]];
                 const int32_t n = tgt[k][1];
@@ -385,4 +465,5 @@ void ora_flex_contacts(ora_flex* f, ora_scene* const* bodies, int32_t counter) {
     free(list);
     free(active);
     free(res);
+    free(tie);
 }

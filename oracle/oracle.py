@@ -109,6 +109,10 @@ class OraFlex(ctypes.Structure):
         ("n_pairs", ctypes.c_int32),
         ("pair_body", _i32p),
         ("gated", ctypes.c_int32),
+        ("item_tied", _i32p),
+        ("tie_elem", _i32p),
+        ("tie_w", _f32p),
+        ("tie_gap", _f32p),
         ("max_cells", ctypes.c_int32),
         ("max_query", ctypes.c_int32),
         ("max_pairs", ctypes.c_int32),
@@ -162,6 +166,7 @@ def lib():
         L.ora_flex_query.argtypes = [ctypes.POINTER(OraFlex), _f32p, ctypes.c_float, ctypes.c_int32, _i32p]
         L.ora_flex_query.restype = ctypes.c_int32
         L.ora_flex_contacts.argtypes = [ctypes.POINTER(OraFlex), _scenes, ctypes.c_int32]
+        L.ora_flex_ties_init.argtypes = [ctypes.POINTER(OraFlex), _scenes]
         L.ora_gate_pair.argtypes = [_f32p, _f32p, _f32p, _f32p, ctypes.c_int32]
         L.ora_gate_pair.restype = ctypes.c_int32
         _lib = L
@@ -314,6 +319,10 @@ class OracleFlexGroup:
         keep["item_penalty"] = np.ascontiguousarray(g.item_penalty, np.float32)
         keep["item_friction"] = np.ascontiguousarray(g.item_friction, np.float32)
         self.cache = np.full(g.n_items, -1, np.int32)
+        keep["item_tied"] = np.ascontiguousarray(g.tied_flags(), np.int32)
+        self.tie_elem = np.full(g.n_items, -1, np.int32)
+        self.tie_w = np.zeros(2 * max(g.n_items, 1), np.float32)
+        self.tie_gap = np.zeros(max(g.n_items, 1), np.float32)
         keep["pair_body"] = np.ascontiguousarray(g.pair_body, np.int32).reshape(-1)
         self.cell_start = np.zeros(g.max_cells, np.int32)
         self.cell_end = np.zeros(g.max_cells, np.int32)
@@ -331,6 +340,8 @@ class OracleFlexGroup:
         F.n_pairs = g.n_pairs
         F.pair_body = _p(keep["pair_body"], _i32p)
         F.gated = 1 if g.gated else 0
+        F.item_tied = _p(keep["item_tied"], _i32p)
+        F.tie_elem, F.tie_w, F.tie_gap = _p(self.tie_elem, _i32p), _p(self.tie_w, _f32p), _p(self.tie_gap, _f32p)
         F.max_cells, F.max_query, F.max_pairs = g.max_cells, g.max_query, g.max_pairs
         F.cell_start, F.cell_end, F.sorted_seg = (_p(a, _i32p) for a in (self.cell_start, self.cell_end, self.sorted_seg))
         self._offp = (_i32p * B)(*[_p(a, _i32p) for a in self.c_off])
@@ -348,6 +359,16 @@ class OracleFlexGroup:
 
     def rebuild(self):
         self._lib.ora_flex_hash_build(ctypes.byref(self._F), self._scenes)
+
+    def ties_init(self):
+        self._lib.ora_flex_ties_init(ctypes.byref(self._F), self._scenes)
+
+    @property
+    def ties(self):
+        """Resolved tie edge (-1: none) and (weight 0, weight 1, normal gap) per work item."""
+        W = self.group.n_items
+        state = np.stack([self.tie_w[0:2 * W:2], self.tie_w[1:2 * W:2], self.tie_gap[:W]], axis=1)
+        return self.tie_elem.copy(), state.astype(np.float32)
 
     def contacts(self, counter):
         self._lib.ora_flex_contacts(ctypes.byref(self._F), self._scenes, int(counter))
@@ -373,9 +394,10 @@ class OracleFlexGroup:
 
 class OracleLoop:
     """Serial CPU run of an ExplicitLoop over several bodies: the zero-dt warm-up,
-    the initial hash build, then per substep the hash schedule, the FEM-FEM pass
-    and every body's substep, with the loop's counter driving the gates
-    (flatworld/explicitloop.py:383-385, 473-555; femspringmanager.py:1035-1039)."""
+    the initial hash build, then per substep the hash schedule, the one-time tie
+    resolution at the first substep, the FEM-FEM pass and every body's substep,
+    with the loop's counter driving the gates (flatworld/explicitloop.py:383-385,
+    473-555; femspringmanager.py:1035-1039; mixedcontact.py:1668-1695)."""
 
     def __init__(self, specs, group=None):
         self.bodies = [OracleScene(sp) for sp in specs]
@@ -386,6 +408,7 @@ class OracleLoop:
             self.flex = OracleFlexGroup(fem, group)
         self.counter = 0
         self.elapsed = 0.0
+        self.ties_done = False
         for b in self.bodies:
             b.warmup()
         if self.flex is not None:
@@ -393,10 +416,15 @@ class OracleLoop:
 
     def substep(self, dt, damping=0.0):
         if self.flex is not None:
-            self.elapsed += float(dt)
-            if self.elapsed >= self.group.rebuild_interval:
-                self.flex.rebuild()
-                self.elapsed = 0.0
+            g = self.group
+            if g.n_tied < g.n_items or not self.ties_done:
+                self.elapsed += float(dt)
+                if self.elapsed >= g.rebuild_interval:
+                    self.flex.rebuild()
+                    self.elapsed = 0.0
+            if g.n_tied > 0 and not self.ties_done:
+                self.flex.ties_init()
+                self.ties_done = True
             self.flex.contacts(self.counter)
         d32, c32 = float(np.float32(dt)), float(np.float32(damping))
         for b in self.bodies:

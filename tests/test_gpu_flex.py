@@ -6,7 +6,9 @@ so positions, velocities and every work item's cached segment must agree
 exactly.  Cases: the reference's own two-circle test
 (test2D/test_2Dfem_contact.py), stacked boxes on a plane (the gated path, ties
 between collinear edges, friction), three bodies of three materials on a height
-field, and bodies spread over many tiles.
+field, bodies spread over many tiles, and tied pairs (ExplicitLoop ``ties``):
+resolved once at the first substep, then held by penalty springs, alone and
+beside sliding pairs.
 
 FEM-FEM contact at the reference's adaptive step diverges as soon as both
 bodies press on each other through two work items (the contact stable time
@@ -48,6 +50,10 @@ def assert_same(loop, o, what):
         assert np.array_equal(v, body.v), f"{what}: velocities of body {k} differ"
     if o.flex is not None:
         assert np.array_equal(m.flex_cache(), o.flex.cache), f"{what}: cached segments differ"
+        if o.flex.group.n_tied:
+            seg, state = m.flex_ties()
+            so, sto = o.flex.ties
+            assert np.array_equal(seg, so) and np.array_equal(state, sto), f"{what}: tie state differs"
 
 
 def run_frames(loop, o, frames, check_every=1, what=""):
@@ -282,3 +288,46 @@ def test_contact_beside_a_quiet_body_and_springs_bitexact():
     o = oracle_of(loop)
     contact = run_frames(loop, o, 6, check_every=2, what="quiet body and springs")
     assert contact > 0
+
+
+def test_tied_blocks_bitexact():
+    """Two jittered blocks 2 mm apart, tied, the second launched away: the ties
+    resolve once at the first substep, every substep applies the springs, and the
+    hash stops refreshing once no sliding item is left."""
+    mat = fw.Elastic(1e4, 0.3, 10.0)
+    a = ccw_grid(6, (0.0, 0.0), 0.2, mat, jitter=0.2, seed=0)
+    b = ccw_grid(5, (0.202, 0.03), 0.18, mat, jitter=0.2, seed=7, initials=[fw.InitialVel("ALL", [0.3, 0.1])])
+    loop = fw.ExplicitLoop(1.5e-4, [a, b], ties=[(0, 1)], useAdapativeDT=False)  # 0.35 x the stable step
+    m = loop.femSpringManager
+    g = m._flex
+    assert g.n_tied == g.n_items and not g.gated
+    log(f"[case] tied blocks, dt={loop.dt:.3e}, items={g.n_items}")
+    o = oracle_of(loop)
+    run_frames(loop, o, 2, check_every=1, what="tied blocks")
+    seg, _ = m.flex_ties()
+    assert (seg >= 0).sum() > 10 and (seg < 0).sum() > 0
+    gd, go = m._arena.flex_read(m._flex_gid, _native.FLEX_GRID), o.flex.grid
+    assert np.array_equal(gd["glb"], go["glb"]) and np.array_equal(gd["gs"], go["gs"])
+    for body in o.bodies:
+        assert np.all(np.isfinite(body.x))
+
+
+def test_tied_and_sliding_pairs_on_plane_bitexact():
+    """Two blocks tied side by side on a plane and a third dropped across the
+    joint: one group of tied and sliding pairs on the plane-gated path; the joint's
+    nodes take their tie springs before their contacts."""
+    mat = fw.Elastic(2e4, 0.3, 50.0)
+    left = ccw_grid(6, (0.1, 0.0), 0.3, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=31)
+    right = ccw_grid(6, (0.4, 0.0), 0.3, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=32)
+    top = ccw_grid(5, (0.3, 0.301), 0.2, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=33, friction=0.2,
+                   initials=[fw.InitialVel("ALL", [0.0, -0.5])])
+    plane = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    loop = fw.ExplicitLoop(1e-4, [left, right, top, plane], ties=[(1, 0)], useAdapativeDT=False)
+    g = loop.femSpringManager._flex
+    assert g.gated and g.n_pairs == 3 and 0 < g.n_tied < g.n_items
+    log(f"[case] tied and sliding pairs on a plane, dt={loop.dt:.3e}, items={g.n_items}, tied={g.n_tied}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 2, check_every=1, what="tied and sliding")
+    assert contact > 0
+    for body in o.bodies:
+        assert np.all(np.isfinite(body.x))

@@ -176,7 +176,7 @@ def test_library_exports_declared_symbols():
     for name in sorted(declared):
         assert hasattr(L, name), name
         assert name in _native.SIGNATURES, f"{name} not bound in _native.SIGNATURES"
-    assert L.fw_abi_version() == 3
+    assert L.fw_abi_version() == 4
 
 
 @pytest.mark.parametrize("tile_nodes,tile_slots", [(512, 1024), (64, 1024), (1024, 2048), (7, 1024)])
@@ -379,3 +379,38 @@ def test_spring_scene_validation():
     bad.conn[3, 1] = X.shape[0] + 2
     with pytest.raises(ValueError, match="outside"):
         _native.tiling_check(bad)
+
+
+def test_tie_entries_parse_as_the_reference(capsys):
+    """flatworld/explicitloop.py:400-439: pairs of domain indices, negative ones from
+    the end, stored as (low, high); anything else is counted and ignored."""
+    import numpy as np
+
+    from flatworld_amd.explicitloop import ExplicitLoop
+
+    loop = ExplicitLoop.__new__(ExplicitLoop)
+    loop.domains = [object()] * 4
+    loop.ties = [(0, 1), (-1, 1), ("a", 1), (2, 2), (0, 7), [3, np.int64(0)], (1, 2, 3), {"domain1": 0}]
+    loop._tie_pairs = set()
+    loop._refresh_tie_pairs()
+    assert loop._tie_pairs == {(0, 1), (1, 3), (0, 3)}
+    assert loop._is_tie_pair(3, 0) and loop._is_tie_pair(1, 0) and not loop._is_tie_pair(1, 2)
+    assert "5 tie entries could not be resolved" in capsys.readouterr().out
+
+
+def test_tied_contact_flags_its_work_items():
+    import numpy as np
+
+    import flatworld_amd as fw
+    from flatworld_amd.definitions import ContactType
+    from flatworld_amd.femcontact import ContactFlexFlex
+    from flatworld_amd.flexcontact import build_flex_group
+
+    mat = fw.Elastic(1e4, 0.3, 1.0)
+    b = [fw.FemDomain(fw.Mesh(2, *fw.structured_grid(2, lower=(2.0 * k, 0.0), extent=1.0)), fw.SolidProp(mat), [])
+         for k in range(3)]
+    cs = [ContactFlexFlex(b[1], b[0], ContactType.FLEXFLEX), ContactFlexFlex(b[2], b[1], ContactType.FLEXFLEX)]
+    cs[1].tied = True
+    g = build_flex_group(b, cs, False, sum(x.nnodes for x in b))
+    assert g.n_tied == int(np.sum(g.item_pair == 1)) > 0
+    assert np.array_equal(g.tied_flags(), (g.item_pair == 1).astype(np.int32))

@@ -15,6 +15,12 @@ the mathematics fixes:
   flat penetration gives force n k d on the node and -force * weights on the
   edge's nodes; every pass's contributions sum to zero (action and reaction);
   the cache keeps a segment while the node stays in its margin box;
+* tied pairs (flatworld/mixedcontact.py:687-811): each tied item resolves to the
+  brute-force nearest edge with the weights of its projection and a gap equal to
+  minus its penetration; at the state it was resolved in the tie pulls with no
+  force, and a rigid shift d of the node's body against the edge's body gives
+  -k d on the node (+k d when the edge's body moves); each node lists its tie
+  contributions before its sliding-contact ones; tied items never slide;
 * whole loops: two free bodies colliding conserve momentum; the reference's
   two-circle scenario runs 1.5 s with contact and without blowing up.
 """
@@ -73,12 +79,16 @@ def _grid_body(side, lower, extent, mat, bcs=(), initials=(), jitter=0.0, seed=0
                         friction=friction)
 
 
-def _loop(bodies, gated=False):
-    """OracleLoop of FEM bodies with every pair in contact, in the reference's pair order."""
+def _loop(bodies, gated=False, tied=()):
+    """OracleLoop of FEM bodies with every pair in contact, in the reference's pair
+    order; ``tied`` lists the (i, j) body pairs whose contact is tied."""
     contacts = []
+    tied = {frozenset(t) for t in tied}
     for i in range(len(bodies)):
         for j in range(i):
-            contacts.append(ContactFlexFlex(bodies[i], bodies[j], ContactType.FLEXFLEX))
+            c = ContactFlexFlex(bodies[i], bodies[j], ContactType.FLEXFLEX)
+            c.tied = frozenset((i, j)) in tied
+            contacts.append(c)
     g = build_flex_group(bodies, contacts, gated, sum(b.nnodes for b in bodies))
     specs = [add_flex_record(pack_domain(b, []), g.flex_cap(k)) for k, b in enumerate(bodies)]
     dt = min([1.0] + [b.stableTime for b in bodies] + [c.stableTime for c in contacts])
@@ -330,3 +340,144 @@ def test_work_list_order_and_scalars():
     spec = add_flex_record(pack_domain(b[0], []), g.flex_cap(0))
     assert [gr.kind for gr in spec.grounds] == [GROUND_FLEX]
     assert g.flex_cap(0) == int(np.sum(g.item_body == 0) + 2 * np.sum(g.item_target == 0))
+
+
+# ------------------------------------------------------------------ tied pairs
+
+
+def _tie_blocks(seed, gap=0.002, tied=((1, 0),)):
+    mat = fw.Elastic(1e4, 0.3, 10.0)
+    a = _grid_body(6, (0.0, 0.0), 0.2, mat, jitter=0.2, seed=seed)
+    b = _grid_body(5, (0.2 + gap, 0.03), 0.18, mat, jitter=0.2, seed=seed + 7)
+    return _loop([a, b], tied=tied)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_tie_init_resolves_the_nearest_edge(seed):
+    o, g, _ = _tie_blocks(seed)
+    assert g.n_tied == g.n_items
+    o.flex.ties_init()
+    seg, state = o.flex.ties
+    qbuf = np.float32(max(max(1e-4, float(max(o.flex.grid["gs"]))), 0.001))
+    resolved = 0
+    for w in range(g.n_items):
+        b, nid, t = int(g.item_body[w]), int(g.item_node[w]), int(g.item_target[w])
+        p = o.flex.bodies[b].x[nid]
+        cands = []
+        for s in range(g.body_seg_off[t], g.body_seg_off[t + 1]):
+            _, a, c = _seg_xy(o, g, s)
+            ins, pen, n, wts = point_edge(p, a, c)
+            if ins and abs(pen) < qbuf:
+                cands.append((abs(float(pen)), s, pen, wts))
+        if not cands:
+            assert seg[w] == -1 and np.all(state[w] == 0.0), w
+            continue
+        resolved += 1
+        best = min(x[0] for x in cands)
+        hit = [x for x in cands if x[1] == seg[w]]
+        assert hit and hit[0][0] == best, w
+        _, _, pen, wts = hit[0]
+        assert np.array_equal(state[w, :2], wts)
+        # the tie normal is the contact normal turned by pi: gap = -penetration
+        assert abs(float(state[w, 2]) + float(pen)) <= 2e-7 * (1.0 + abs(float(pen))) + 1e-7, w
+    assert resolved > 10
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_tie_force_is_a_spring_to_the_resolved_point(seed):
+    o, g, _ = _tie_blocks(seed)
+    o.flex.ties_init()
+    seg, _ = o.flex.ties
+    k = float(g.item_penalty[0])
+    L = 0.4  # coordinates are below 0.4: rounding of a point is about L * 2^-24
+    o.flex.contacts(1)
+    for b in range(2):
+        off, f = o.flex.contributions(b)
+        assert off[-1] > 0
+        assert np.abs(f).max() <= k * 8 * L * 2.0 ** -24
+    # shift body 0 rigidly: its tied nodes feel -k d, body 1's tied nodes +k d
+    d = np.array([1.5e-3, -1e-3], np.float32)
+    o.flex.bodies[0].x[:] += d
+    o.flex.contacts(2)
+    for w in range(g.n_items):
+        if seg[w] < 0:
+            continue
+        b = int(g.item_body[w])
+        off, f = o.flex.contributions(b)
+        nid = int(g.item_node[w])
+        # the node's own entry: its first one that is not a reaction; items of this
+        # pair are the node's only contributions, so the entry is unique per item
+        own = f[off[nid]:off[nid + 1]]
+        exp = -k * d if b == 0 else k * d
+        assert np.any(np.all(np.abs(own - exp) <= 1e-3 * k * float(np.abs(d).max()), axis=1)), w
+    tot = sum(o.flex.contributions(b)[1].astype(np.float64).sum(axis=0) for b in range(2))
+    assert np.all(np.abs(tot) <= 1e-5 * k * float(np.abs(d).max()) * g.n_items)
+
+
+def test_tied_items_never_slide():
+    """A tied pair overlapping by more than the sliding contact would accept still
+    gives only tie springs, and the cache stays empty."""
+    o, g, _ = _tie_blocks(0, gap=-0.01)
+    o.flex.ties_init()
+    o.flex.contacts(1)
+    assert np.all(o.flex.cache == -1)
+    tot = max(float(np.abs(o.flex.contributions(b)[1]).max(initial=0.0)) for b in range(2))
+    assert tot <= float(g.item_penalty[0]) * 8 * 0.4 * 2.0 ** -24
+    # the same overlap untied slides: contacts push the bodies apart
+    o2, g2, _ = _tie_blocks(0, gap=-0.01, tied=())
+    o2.flex.contacts(1)
+    assert np.any(o2.flex.cache >= 0)
+    assert float(np.abs(o2.flex.contributions(0)[1]).max()) > 1e-3 * float(g2.item_penalty[0]) * 0.01
+
+
+def test_ties_come_before_contacts_in_each_node():
+    """Body 1 is tied to body 0 on its left side and pressed by body 2 on its top:
+    its top-left nodes receive both; every node lists the tie entries (shifted
+    body 0: forces along x) before the contact entries (flat top: forces along y)."""
+    mat = fw.Elastic(1e4, 0.3, 10.0)
+    b0 = _grid_body(4, (0.0, 0.0), 0.2, mat)
+    b1 = _grid_body(4, (0.2, 0.0), 0.2, mat)
+    b2 = _grid_body(4, (0.19, 0.195), 0.2, mat)  # 0.005 into body 1's top
+    o, g, _ = _loop([b0, b1, b2], tied=[(1, 0)])
+    assert 0 < g.n_tied < g.n_items
+    o.flex.ties_init()
+    o.flex.bodies[0].x[:] += np.array([1e-3, 0.0], np.float32)
+    o.flex.contacts(1)
+    off, f = o.flex.contributions(1)
+    both = 0
+    for n in range(len(off) - 1):
+        e = f[off[n]:off[n + 1]]
+        kinds = [0 if abs(x[0]) > abs(x[1]) else 1 for x in e if np.any(x != 0.0)]
+        assert kinds == sorted(kinds), (n, e)
+        both += (0 in kinds and 1 in kinds)
+    assert both >= 1
+
+
+def test_tied_free_bodies_stay_together():
+    """Two free bodies side by side, one launched away from the other: tied, the
+    interface stays closed and the total momentum is kept; untied, it opens.
+
+    The reference's tie is not conservative: its query buffer (at least the cell
+    size) also ties nodes a cell away from the interface with a nonzero normal
+    gap, and the force on such a node and its reactions on the edge form a couple
+    (gap n x F) under shear, so the tied pair's oscillation grows.  The check stays
+    within the first 80 half-steps, before that growth dominates."""
+    mat = fw.Elastic(5e3, 0.3, 20.0)
+    out = {}
+    for tied in ([(1, 0)], []):
+        a = _grid_body(5, (0.0, 0.0), 0.2, mat)
+        b = _grid_body(5, (0.2, 0.0), 0.2, mat, initials=[fw.InitialVel("ALL", [0.5, 0.0])])
+        o, g, dt = _loop([a, b], tied=tied)
+        def momentum():
+            return sum((bd.mass[:, None].astype(np.float64) * bd.v.astype(np.float64)).sum(axis=0) for bd in o.bodies)
+        p0 = momentum()
+        # coincident sides: each node is tied both ways, which doubles the stiffness
+        # the contact's stable step assumes, so the loop runs at half of it
+        for _ in range(80):
+            o.substep(0.5 * dt)
+        assert o.ties_done == bool(tied)
+        p1 = momentum()
+        assert np.all(np.isfinite(p1))
+        assert np.allclose(p1, p0, rtol=0, atol=1e-6 * np.abs(p0).max())
+        out[bool(tied)] = o.bodies[1].x[:, 0].min() - o.bodies[0].x[:, 0].max()
+    assert abs(out[True]) < 1e-3 and out[False] > 5e-3, out
