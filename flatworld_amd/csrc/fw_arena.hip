@@ -1897,6 +1897,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     D.max_cells = d->max_cells;
     D.max_query = d->max_query;
     D.max_pairs = d->max_pairs;
+    D.n_tied = n_tied;
     D.body_scene = (const int32_t*)(blk + o_scene);
     D.body_seg_off = (const int32_t*)(blk + o_soff);
     D.body_slot_off = (const int32_t*)(blk + o_sloff);

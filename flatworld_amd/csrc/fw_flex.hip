@@ -497,7 +497,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
     const int ar3 = mod3_(local);
     const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
     int2 ri = make_int2(0, -1);
-    if (F.item_tied[w]) {
+    if (F.n_tied > 0 && F.item_tied[w]) {
         // _apply_fem_fem_ties_fixed_kernel (flatworld/mixedcontact.py:773-811): a
         // spring toward the resolved edge point at the initial normal gap, every
         // substep, whatever the pair gate; tied items never slide
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* g
                                make_float2(-(rf.x * rf.w), -(rf.y * rf.w))};
         for (int c = 0; c < 3; ++c) {
             const int at = atomicAdd(&F.slot_cur[sl[c]], 1);
-            F.ent_key[at] = F.item_tied[w] ? w : F.n_items + w;  // a node's tie springs before its contacts
+            F.ent_key[at] = (F.n_tied > 0 && F.item_tied[w]) ? w : F.n_items + w;  // tie springs before contacts
             F.ent_f[at] = val[c];
         }
     }

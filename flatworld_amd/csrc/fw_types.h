@@ -53,6 +53,7 @@ struct FlexGrid {
 struct FlexDev {
     int32_t n_bodies, n_segs, n_items, n_pairs, n_slots;
     int32_t gated, max_cells, max_query, max_pairs;
+    int32_t n_tied;                // tied work items (0: item_tied is never read)
     const int32_t* body_scene;     // [B] arena scene id
     const int32_t* body_seg_off;   // [B + 1]
     const int32_t* body_slot_off;  // [B + 1]
