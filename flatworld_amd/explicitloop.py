@@ -30,7 +30,7 @@ import warnings
 import numpy as np
 
 from .definitions import ContactType, DomainType
-from .femcontact import ContactFlexFlex, contactHelper
+from .femcontact import ContactFlexFlex, ContactSpringFlex, contactHelper
 from .femspringmanager import FemSpringManager
 from .flexcontact import build_flex_group
 
@@ -116,6 +116,15 @@ class ExplicitLoop:
                 )
                 self._unbatched_contacts += [c for c, _ in group]
                 group.clear()
+        # FEM-spring pairs: unbatched in the reference and never applied
+        # (flatworld/mixedcontact.py:1789-1792, 2263-2268)
+        spring_fem = [c for c in self.contacts if isinstance(c, ContactSpringFlex)]
+        if spring_fem:
+            warnings.warn(
+                f"{len(spring_fem)} FEM-spring contact pair(s): as in flatworld/mixedcontact.py:1789-1792 they stay "
+                "unbatched and apply no force; they lower the stable time and turn the BVH gate on"
+            )
+            self._unbatched_contacts += spring_fem
         for c, _ in anal + hf + vox:
             fem_idx = self.domains.index(c.domain1)
             grounds_of.setdefault(fem_idx, []).append((c.domain2, c))

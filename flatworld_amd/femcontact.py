@@ -7,13 +7,16 @@ Host scalars only, following flatworld/femcontact.py:94-197 and 256-303:
   ``0.9 sqrt(m_node / k)`` with ``m_node = rho * (L_min^2 / 2) / 3``;
 * FEM-FEM: the smaller of the two bodies' segment penalties, and the smaller of
   the two stable times;
-* spring network: ``k = 100 * spring``, stable time ``0.5 sqrt(mass / k)``.
+* spring network: ``k = 100 * spring``, stable time ``0.5 sqrt(mass / k)``;
+* spring network against a FEM body: ``100 min(spring, 2 K)`` and the FEM body's
+  contact stable time.
 
 The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip,
-fw_flex.hip).  FEM-spring and rigid contacts are outside this package:
-constructing one raises ``NotImplementedError`` instead of silently dropping it.  Pairs the
-reference's factory has no class for (spring-spring, spring-voxel) give no
-contact, as there.
+fw_flex.hip).  A FEM-spring pair exists as in the reference but applies no force
+there: ExplicitLoop files it among the unbatched contacts, which no substep runs.
+Rigid contacts are outside this package: constructing one raises
+``NotImplementedError`` instead of silently dropping it.  Pairs the reference's
+factory has no class for (spring-spring, spring-voxel) give no contact, as there.
 """
 
 from .definitions import ContactType, DomainType
@@ -124,8 +127,25 @@ class ContactSpringHeightField(_SpringGround):
     ground_type = DomainType.HEIGHTFIELD
 
 
+class ContactSpringFlex(ContactBase):
+    """Springs vs FEM body (flatworld/femcontact.py:200-213); ``domain1`` is always the
+    springs.  The reference files this pair among its unbatched contacts
+    (flatworld/mixedcontact.py:1789-1792), which no substep applies, and its zero-dt
+    warm-up ``calculate`` returns at once (femcontact.py:214-217): the pair lowers the
+    loop's stable time and turns the BVH gate on, and moves no node."""
+
+    _SPRING_PENALTY_SCALE = 100.0
+
+    def __init__(self, domain1, domain2, type, is_tied=False):
+        super().__init__(domain1, domain2, type, is_tied)
+        if self.domain1.type == DomainType.FEM:
+            self.domain1, self.domain2 = self.domain2, self.domain1
+        self.penalty = min(self.domain1.spring, self._segment_penalty(self.domain2)) * self._SPRING_PENALTY_SCALE
+        self.stableTime = self.calStableTime(self.penalty, self.domain2)
+
+
 def contactHelper(domain1, domain2, type):
-    """Pair factory (flatworld/femcontact.py:283-303) without the rigid and FEM-spring pairs."""
+    """Pair factory (flatworld/femcontact.py:283-303) without the rigid pairs."""
     if type == ContactType.ANALYTICALSPRING:
         return ContactSpringAnalytical(domain1, domain2, type)
     if type == ContactType.SPRINGHEIGHTFIELD:
@@ -133,10 +153,7 @@ def contactHelper(domain1, domain2, type):
     if type in (ContactType.SPRINGSPRING, ContactType.SPRINGVOXELMAP):
         return None  # no contact class in the reference either
     if type == ContactType.FLEXSPRING:
-        raise NotImplementedError(
-            "FEM-spring penalty contact (flatworld/femcontact.py:13-80, 200-240) is not part of this package; "
-            "give the domains considerContact=False"
-        )
+        return ContactSpringFlex(domain1, domain2, type)
     if type == ContactType.FLEXANLAYTICAL:
         return ContactFlexAnalytical(domain1, domain2, type)
     if type == ContactType.FEXHEIGHTFIELD:

@@ -331,3 +331,28 @@ def test_tied_and_sliding_pairs_on_plane_bitexact():
     assert contact > 0
     for body in o.bodies:
         assert np.all(np.isfinite(body.x))
+
+
+def test_spring_fem_pairs_gate_fem_fem_contact_bitexact():
+    """A spring lattice in contact range of two FEM blocks: the FEM-spring pairs
+    apply no force (the reference never runs its unbatched contacts) but lower the
+    stable time and turn the BVH gate on, so the blocks' FEM-FEM pair is gated
+    with no ground in the loop."""
+    from flatworld_amd import workloads as W
+    from flatworld_amd.femcontact import ContactSpringFlex
+
+    mat = fw.Elastic(2e4, 0.3, 50.0)
+    lower = ccw_grid(6, (0.1, 0.0), 0.3, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=41)
+    upper = ccw_grid(5, (0.15, 0.31), 0.2, mat, [fw.Gravity([0, -9.8])], jitter=0.1, seed=42,
+                     initials=[fw.InitialVel("ALL", [0.0, -0.8])])
+    lattice = W.spring_lattice(6, (0.2, 0.1), 0.02, [200.0, 0.3, 0.01], [fw.Gravity([0.0, -9.8])])
+    with pytest.warns(UserWarning, match="FEM-spring"):
+        loop = fw.ExplicitLoop(1e-4, [lower, upper, lattice], useAdapativeDT=False)
+    fs = [c for c in loop.contacts if isinstance(c, ContactSpringFlex)]
+    assert len(fs) == 2 and loop.stableTime <= min(c.stableTime for c in fs)
+    g = loop.femSpringManager._flex
+    assert g.gated and g.n_pairs == 1
+    log(f"[case] FEM-spring pairs gate a FEM-FEM pair, items={g.n_items}")
+    o = oracle_of(loop)
+    contact = run_frames(loop, o, 3, check_every=1, what="FEM-spring gate")
+    assert contact > 0

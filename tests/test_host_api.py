@@ -226,9 +226,6 @@ def test_explicitloop_rejects_out_of_scope_domains():
     conn, coords = fw.structured_grid(2)
     a = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
     b = fw.FemDomain(fw.Mesh(2, conn, coords + 2.0), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [])
-    sp = fw.SpringMassDomain(2, coords + 4.0, np.array([[0, 1], [1, 2]], np.int32))
-    with pytest.raises(NotImplementedError, match="FEM-spring"):
-        ExplicitLoop(0.0, [a, sp])
     # FEM-FEM orderings whose reference behaviour reads another domain's entry
     ground = fw.GroundDomain(2, [0.0, -1.0], [0.0, 1.0])
     with pytest.raises(NotImplementedError, match="FEM domains listed first"):
@@ -355,6 +352,25 @@ def test_spring_contact_factory():
     # the reference's factory has no spring-voxel or spring-spring class: no contact
     assert contactHelper(sp, vox, sp.type | vox.type) is None
     assert contactHelper(sp, sp, sp.type) is None
+
+
+def test_spring_fem_contact_scalars():
+    """flatworld/femcontact.py:200-213: springs first whatever the argument order,
+    penalty 100 min(spring, 2 K), the FEM body's contact stable time."""
+    from flatworld_amd.femcontact import ContactSpringFlex, contactHelper
+
+    X, conn = _spring_net(4)
+    sp = fw.SpringMassDomain(2, X, conn, [30.0, 0.0, 0.2])
+    c2, x2 = fw.structured_grid(3, extent=0.3)
+    fem = fw.FemDomain(fw.Mesh(2, c2, x2), fw.SolidProp(fw.Elastic(1e4, 0.3, 50.0)), [])
+    for a, b in ((fem, sp), (sp, fem)):
+        c = contactHelper(a, b, a.type | b.type)
+        assert isinstance(c, ContactSpringFlex) and c.domain1 is sp and c.domain2 is fem
+        assert c.penalty == min(30.0, 2.0 * 1e4 / (3.0 * (1.0 - 0.6))) * 100.0 == 3000.0
+        L = fem.mesh.charLength
+        assert c.stableTime == 0.9 * ((50.0 * (L * L * 0.5) / 3.0) / 3000.0) ** 0.5
+    stiff = fw.SpringMassDomain(2, X, conn, [1e5, 0.0, 0.2])
+    assert contactHelper(stiff, fem, stiff.type | fem.type).penalty == 2.0 * 1e4 / (3.0 * (1.0 - 0.6)) * 100.0
 
 
 @pytest.mark.parametrize("shape", [(256, 512), (512, 1024), (1024, 2048)])
