@@ -1837,7 +1837,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
                   o_sorted = take(4 * (int64_t)d->max_pairs), o_cent = take(4 * (int64_t)d->max_pairs),
                   o_ri = take(8 * W), o_rf = take(16 * W),
                   o_slst = take(4 * NS), o_slcur = take(4 * NS), o_ek = take(12 * (int64_t)W), o_ef = take(24 * (int64_t)W),
-                  o_twg = take(16 * (int64_t)W);
+                  o_twg = take(16 * (int64_t)W), o_ltot = take(4);
     const int64_t total = o;
     const int64_t off = arena_alloc(a, total);
     if (off < 0) return fail(FW_E_CAPACITY, "arena full: FEM-FEM group needs " + std::to_string(total) + " bytes");
@@ -1935,6 +1935,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     D.slot_cur = (int32_t*)(blk + o_slcur);
     D.ent_key = (int32_t*)(blk + o_ek);
     D.ent_f = (float2*)(blk + o_ef);
+    D.list_total = (int32_t*)(blk + o_ltot);
 
     int id = (int)a->flex.size();
     for (int g = 0; g < (int)a->flex.size(); ++g)

@@ -659,8 +659,12 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* g
     __shared__ int lds[FLEX_BLOCK];
     const int tid = threadIdx.x;
     if (group_local(F, scenes, ctl, k, flags) < 0) return;
-    scan_global(F.slot_start, F.slot_cur, F.n_slots, lds);
+    const int last = *F.list_total;
+    const int total = scan_global(F.slot_start, F.slot_cur, F.n_slots, lds);
     phase_barrier();
+    // no entries now or in the previous substep: every node's range already reads
+    // zero entries and the counts are zero
+    if (total == 0 && last == 0) return;
     for (int w = tid; w < F.n_items; w += FLEX_BLOCK) {
         const int2 ri = F.res_i[w];
         if (!ri.x) continue;
@@ -700,6 +704,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* g
     }
     phase_barrier();
     for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) F.slot_start[sl] = 0;
+    if (tid == 0) *F.list_total = total;
 }
 
 }  // namespace

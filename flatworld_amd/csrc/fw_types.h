@@ -97,6 +97,7 @@ struct FlexDev {
     int32_t* slot_cur;             // [n_slots]
     int32_t* ent_key;              // [3W] work item of each entry
     float2* ent_f;                 // [3W]
+    int32_t* list_total;           // [1] entries placed by the previous substep's lists pass
 };
 
 // The scene fields every tile reads (material, dt, gate state), kept together at
