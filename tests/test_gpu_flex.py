@@ -218,6 +218,72 @@ def test_contact_workload_sampled_loops_bitexact():
     arena.close()
 
 
+def test_contact_workload_with_tied_loops_bitexact():
+    """The 2048-loop contact workload with every third loop tied: tied and sliding
+    loops share each contact launch, and each tied loop's ties resolve once, in a
+    launch of its own before the first substep.  All hashes are rebuilt in one
+    launch on the schedule (a tied loop's hash is not read after its ties
+    resolve); 12 tied and 12 sliding loops against the oracle after 400 substeps:
+    positions, velocities, caches and tie state."""
+    from flatworld_amd import workloads as W
+    from flatworld_amd.flexcontact import REBUILD_INTERVAL
+
+    groups = W.contact_batched_config(n_groups=2048, seed=1)
+    tied = list(range(0, len(groups), 3))
+    for gi in tied:
+        g = groups[gi][1]
+        g.item_tied = np.ones(g.n_items, np.int32)
+    dt = groups[0][0][0].dt
+    arena = _native.Arena(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
+    sids, gids = [], []
+    for sp, g in groups:
+        s = [arena.add(x) for x in sp]
+        sids.append(s)
+        gids.append(arena.flex_add(g, s))
+    arena.step(1, -1.0, 0)
+    arena.flex_rebuild(-1)
+    elapsed, pending = 0.0, 0
+    for n in range(400):
+        elapsed += dt
+        if elapsed >= REBUILD_INTERVAL:
+            if pending:
+                arena.step(pending, -1.0, 0)
+                pending = 0
+            arena.flex_rebuild(-1)
+            elapsed = 0.0
+        if n == 0:
+            for gi in tied:
+                arena.flex_init_ties(gids[gi])
+        pending += 1
+    arena.step(pending, -1.0, 0)
+    arena.sync()
+    rng = np.random.default_rng(9)
+    untied = [gi for gi in range(len(groups)) if gi % 3]
+    resolved = hits = 0
+    for gi in list(rng.choice(tied, 12, replace=False)) + list(rng.choice(untied, 12, replace=False)):
+        sp, g = groups[gi]
+        o = OracleLoop(sp, g)
+        for _ in range(400):
+            o.substep(dt)
+        for k, body in enumerate(o.bodies):
+            assert np.all(np.isfinite(body.x)), f"loop {gi}: oracle not finite"
+            assert np.array_equal(arena.read(sids[gi][k], _native.FIELD_X), body.x), f"loop {gi} body {k}: positions"
+            assert np.array_equal(arena.read(sids[gi][k], _native.FIELD_V), body.v), f"loop {gi} body {k}: velocities"
+        cache = arena.flex_read(gids[gi], _native.FLEX_CACHE)
+        assert np.array_equal(cache, o.flex.cache), f"loop {gi}: caches differ"
+        hits += int((cache >= 0).sum())
+        if g.n_tied:
+            seg = arena.flex_read(gids[gi], _native.FLEX_TIE_SEG)
+            state = arena.flex_read(gids[gi], _native.FLEX_TIE_STATE)[:, :3]
+            so, sto = o.flex.ties
+            assert np.array_equal(seg, so) and np.array_equal(state, sto), f"loop {gi}: tie state differs"
+            resolved += int((seg >= 0).sum())
+    log(f"[case] contact workload with tied loops: 24 loops bit-exact after 400 substeps, {resolved} ties, "
+        f"{hits} cached segments")
+    assert resolved > 0 and hits > 0
+    arena.close()
+
+
 @pytest.mark.parametrize("caps", [dict(max_cells=40), dict(max_query=2), dict(max_pairs=60),
                                   dict(max_cells=25, max_query=3, max_pairs=40)])
 def test_capped_hash_and_query_bitexact(caps):
