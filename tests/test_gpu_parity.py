@@ -3,7 +3,10 @@
 Every material must agree bit for bit: both sides evaluate the same float32
 operations in the same order without contraction, and the Neo-Hookean log is
 one shared double-operation routine (correctly rounded in practice, pinned
-against numpy in the oracle tests).  The Neo-Hookean case also reports the
+against numpy in the oracle tests).  Every comparison covers positions,
+velocities, nodal forces, displacement U, contact-active flags, the boundary
+AABB, Cauchy stress, Green strain, the plastic-yield flag and the J2 state
+(tests/parity.py ``check_fields``).  The Neo-Hookean case also reports the
 tolerance figures (nodal forces within 1e-5 of the largest nodal force,
 positions within 1e-4 of the minimum edge length) for a reader comparing a
 build whose log differs.
@@ -17,7 +20,7 @@ from flatworld_amd import _native
 from flatworld_amd import autotune
 from flatworld_amd import workloads as W
 from oracle.oracle import OracleScene
-from tests.parity import FORCE_RTOL, POS_TOL_EDGE, GpuScene, compare, log
+from tests.parity import FORCE_RTOL, POS_TOL_EDGE, GpuScene, check_fields, compare, log
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +36,7 @@ def run_both(spec, chunks, exact, label, **arena_kw):
     for n in chunks:
         g.run(n)
         o.run(n)
-        out.append(compare(g, o, exact, label))
+        out.append(compare(g, o, exact, label + f" after {o.counter} substeps"))
     return g, o, out
 
 
@@ -144,39 +147,53 @@ def test_graph_replay_equals_plain():
 # ---------------------------------------------------------------- full sizes of the benchmark configurations
 
 
+@pytest.mark.timeout(400)
 def test_large_sheet_full_size_bitexact():
-    """1000x1000 jittered Neo-Hookean sheet (2 M triangles) on a plane, CFL dt, 100 substeps."""
+    """1000x1000 jittered Neo-Hookean sheet (2 M triangles) on a plane, CFL dt, all 1000 substeps
+    of the configuration, every field compared after 1, 100 and 1000 substeps."""
     spec = W.large_config()
-    g, o, _ = run_both(spec, [1, 99], True, "large 1000x1000 nh, plane")
+    g, o, out = run_both(spec, [1, 99, 900], True, "large 1000x1000 nh, plane")
     res = compare(g, o, False, "large tolerance figures")
-    log(f"  {res}")
+    log(f"  {res} {out}")
     assert res["force_rel"] < FORCE_RTOL and res["pos_err_edges"] < POS_TOL_EDGE
-    assert np.count_nonzero(o.contact_flag) > 0, "jittered bottom row should touch the plane"
+    assert all(c["contact"] > 0 for c in out), "jittered bottom row should touch the plane throughout"
 
 
+@pytest.mark.timeout(400)
 def test_j2_voxel_full_size_bitexact():
-    """512x512 J2 block (524 k triangles) dropped at 2 m/s onto 256x256 value-noise voxels, 200 substeps."""
+    """512x512 J2 block (524 k triangles) dropped at 2 m/s onto 256x256 value-noise voxels, all 2000
+    substeps of the configuration; the block must be in voxel contact and yielding."""
     spec = W.j2_voxel_config()
-    run_both(spec, [1, 199], True, "j2 512x512 on 256x256 voxels")
+    g, o, out = run_both(spec, [1, 199, 800, 1000], True, "j2 512x512 on 256x256 voxels")
+    log(f"  contact / yield per chunk: {out}")
+    assert all(c["contact"] > 0 and c["yielded"] > 0 for c in out[1:]), "block should touch the voxels and yield"
+    assert o.eqps.max() > 0.0
 
 
+@pytest.mark.timeout(400)
 def test_batched_4096_sampled_scenes_bitexact():
-    """The full 4096-scene arena of the benchmark: 48 sampled scenes against the oracle after 30 substeps."""
+    """The full 4096-scene arena of the benchmark, stepped as the benchmark steps it (hipGraph
+    frames of 60 substeps) for 1200 substeps, long enough for most blocks to land on their
+    height fields: 48 sampled scenes against the oracle, every field."""
     specs = W.batched_config(n_scenes=4096, seed=0)
-    log(f"[case] batched 4096 scenes, {sum(s.n_elems for s in specs)} triangles")
+    log(f"[case] batched 4096 scenes, {sum(s.n_elems for s in specs)} triangles, 20 graph frames")
     arena = _native.Arena()
     ids = [arena.add(s) for s in specs]
     arena.step(1, -1.0, 0)
-    arena.step(30, -1.0, _native.FW_STEP_DIAG)
+    for _ in range(20):
+        arena.step_graph(60, _native.FW_STEP_DIAG)
+    arena.sync()
     pick = W.rng_for(99).choice(len(specs), size=48, replace=False)
+    touched = yielded = 0
     for k in pick:
         o = OracleScene(specs[k])
         o.warmup()
-        o.run(30)
-        for fld, ref in ((_native.FIELD_X, o.x), (_native.FIELD_V, o.v), (_native.FIELD_FORCE, o.fext)):
-            got = arena.read(ids[k], fld)
-            assert np.isfinite(ref).all()
-            assert np.array_equal(got, ref), f"scene {k} field {fld}"
+        o.run(1200)
+        c = check_fields(lambda f: arena.read(ids[k], f), o, f"scene {k}")
+        touched += int(c["contact"] > 0)
+        yielded += int(c["yielded"] > 0)
+    log(f"  sampled scenes in contact at the end: {touched}, yielding: {yielded}")
+    assert touched >= 10 and yielded >= 5, "the window should cover height-field contact and yielding"
     arena.close()
 
 
@@ -244,14 +261,14 @@ def test_ragged_arena_bitexact():
         o = OracleScene(s)
         o.warmup()
         o.run(120)
-        assert np.isfinite(o.x).all()
-        for fld, ref in ((_native.FIELD_X, o.x), (_native.FIELD_V, o.v), (_native.FIELD_FORCE, o.fext)):
-            assert np.array_equal(arena.read(ids[k], fld), ref), f"scene {k} field {fld}"
+        check_fields(lambda f: arena.read(ids[k], f), o, f"ragged scene {k}")
     arena.close()
 
 
 def test_degenerate_element_and_isolated_node_match_oracle():
-    """A zero-area element (repeated node) and a node no element references: NaN appears where the oracle's does."""
+    """A zero-area element (repeated node) and a node no element references: NaN appears where the
+    oracle's does.  The reference's wp.inverse of a singular matrix cannot be reproduced without Warp,
+    so this pins the choice both sides make (adjugate times 1/det, no guard), not the reference."""
     conn, coords = fw.structured_grid(5, lower=(0.3, 0.2), extent=0.3)
     coords = np.concatenate([coords, [[0.9, 0.9]]]).astype(np.float32)
     conn = np.concatenate([conn, [[0, 1, 1]]]).astype(np.int32)

@@ -109,3 +109,46 @@ def test_prepared_scenes_equal_direct_adds():
         b.add_prepared(other)
     a.close()
     b.close()
+
+
+@pytest.mark.timeout(400)
+def test_served_c5_stream_matches_oracle():
+    """A served stream shaped like the serving configuration (grid side 16..256, horizon
+    60..600 substeps, plane / height-field / voxel grounds chosen uniformly, Poisson arrivals
+    while earlier jobs are on the GPU), with a stratified sample of the served jobs -- one per
+    (ground kind, size third) and the largest -- checked bit for bit against the oracle."""
+    from flatworld_amd.server import poisson_arrivals
+    from oracle.oracle import OracleScene
+
+    rng = W.rng_for(55)
+    n = 36
+    jobs = [Job(i, W.serving_job(rng), 0.0) for i in range(n)]
+    for j, t in zip(jobs, poisson_arrivals(n, 300.0, rng)):
+        j.arrival = float(t)
+    sides = {j.job_id: int(round(np.sqrt(j.spec.n_elems / 2))) for j in jobs}
+    log(f"[case] served C5-shaped stream: {n} jobs, sides {min(sides.values())}..{max(sides.values())}, "
+        f"horizons {min(j.spec.horizon for j in jobs)}..{max(j.spec.horizon for j in jobs)}")
+    srv = SimServer(frame=60)
+    res = {r.job_id: r for r in srv.run(jobs)}
+    assert len(res) == n and all(r.ok for r in res.values())
+    # overlap: some job was admitted while another was still resident
+    spans = sorted((r.admitted, r.finished) for r in res.values())
+    assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1)), "jobs should overlap on the device"
+    strata = {}
+    for j in jobs:
+        key = (j.spec.grounds[0].kind, min(2, (sides[j.job_id] - 16) * 3 // 241))
+        strata.setdefault(key, j)
+    sample = list(strata.values())
+    largest = max(jobs, key=lambda j: j.spec.n_elems)
+    if largest not in sample:
+        sample.append(largest)
+    assert {j.spec.grounds[0].kind for j in sample} == {1, 2, 3}
+    for j in sample:
+        o = OracleScene(j.spec)
+        o.warmup()
+        o.run(j.spec.horizon)
+        log(f"  job {j.job_id}: side {sides[j.job_id]}, ground {j.spec.grounds[0].kind}, horizon {j.spec.horizon}, "
+            f"contact nodes at the end {int(o.contact_flag.sum())}")
+        assert np.isfinite(o.x).all()
+        assert np.array_equal(res[j.job_id].positions, o.x), f"job {j.job_id} positions"
+        assert np.array_equal(res[j.job_id].velocities, o.v), f"job {j.job_id} velocities"
