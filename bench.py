@@ -4,8 +4,12 @@
 Default workload: the batched mix -- 4096 independent scenes, grid side
 log-uniform in [8, 128], Elastic / Neo-Hookean / J2 mix, one height field per
 scene (24.08 M triangles, 12.40 M nodes).  A "step" is one substep of every
-scene; the timed region replays hipGraph-captured frames.  Inputs are resident
-in HBM before timing starts.
+scene.  Inputs are resident in HBM before timing starts.  Before the timed
+region the arena advances (untimed) until every block has reached its height
+field, so the timed substeps run with ground contact active and J2 elements
+yielding; the timed region is replays of one captured hipGraph whose length
+divides --steps (at most the 60-substep frame).  config.timed_window states the
+launch mode and the contact / yield fractions at the end of the window.
 
 Printed JSON line:
   value            element-substeps/s of the batched, graph-replayed form
@@ -29,7 +33,12 @@ Printed JSON line:
   contact          2048 two-block loops with FEM-FEM contact on a plane: element-
                    substeps/s with the contact pass and the hash rebuilds included,
                    the contact + substep kernel time and a CPU baseline on a sample
+  single_scene     the 1000x1000 Neo-Hookean sheet on a plane and the 512x512 J2 block on
+                   256x256 voxels, each tuned, graph-replayed and timed the same way, with
+                   its roofline fraction and the oracle on the same scene as CPU baseline
   cpu_baseline     the oracle (serial C restatement) on a bounded sample
+roofline.frac uses the live kernel time (HIP events); roofline.frac_wall uses the
+wall-clock time per substep of the timed region.
 """
 
 import argparse
@@ -70,7 +79,7 @@ def build_specs(config, seed, n_scenes):
     raise SystemExit(f"unknown config {config}")
 
 
-def cpu_baseline(specs, budget_s=12.0):
+def cpu_baseline(specs, budget_s=12.0, all_cores=True):
     """The oracle (serial C restatement, the checker) on a bounded sample of the same scenes.
 
     Timed twice: serially on one core, as Warp-CPU launches run, and with the
@@ -103,6 +112,9 @@ def cpu_baseline(specs, budget_s=12.0):
         return elems * n / (time.perf_counter() - t0), n
 
     one, n1 = serial(budget_s / 2)
+    if not all_cores:
+        return {"value": one, "unit": "element-substeps/s", "cores": 1, "kind": "port",
+                "sample": f"{len(sample)} of {len(specs)} scenes ({elems} elements) x {n1} substeps, serial C oracle"}
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     cores = max(1, min(cores, 16))
     groups = [scenes[k::cores] for k in range(cores)]
@@ -222,37 +234,169 @@ def tune_for(specs, args):
     return tuning
 
 
-def timed_arena(specs, tuning, steps, warmup, no_graph=False):
-    """Every scene resident in one arena, warmed up; ``steps`` substeps timed as graph frames,
-    then the average substep-kernel duration from HIP events on the arena stream."""
+def graph_length(steps, frame):
+    """Substeps per replayed graph in the timed region: the longest length no larger than
+    ``frame`` that divides ``steps``, so that the whole timed region is graph replays."""
+    for g in range(min(frame, steps), 0, -1):
+        if steps % g == 0:
+            return g
+    return 1
+
+
+def timed_arena(specs, tuning, steps, warmup, no_graph=False, settle=0, window_stats=False):
+    """Every scene resident in one arena and warmed up; ``settle`` substeps (untimed, in
+    graph frames) carry the scenes past their approach; ``warmup`` more substeps and one
+    replay capture the timed graph; then ``steps`` substeps are timed as replays of one
+    graph of ``graph_length(steps, frame)`` substeps (plain launches with ``no_graph``).
+    Returns the arena, the wall time of the timed region, the average substep-kernel
+    duration from HIP events on the arena stream, and the timed region's description."""
     from flatworld_amd import _native
     from flatworld_amd import autotune
 
     arena = _native.Arena(**autotune.arena_kwargs(tuning))
-    for s in specs:
-        arena.add(s, track_disp=True)
+    ids = [arena.add(s, track_disp=True) for s in specs]
     arena.step(1, -1.0, 0)  # zero-dt warm-up of every scene
     arena.sync()
     frame = max(1, tuning["frame"])
+    g = graph_length(steps, frame)
 
-    def run(n):
+    def run(n, length):
         if no_graph:
             arena.step(n, -1.0, 0)
             return
-        full, rest = divmod(n, frame)
+        full, rest = divmod(n, length)
         for _ in range(full):
-            arena.step_graph(frame, 0)
+            arena.step_graph(length, 0)
         if rest:
             arena.step(rest, -1.0, 0)
 
-    run(warmup)
+    t = time.perf_counter()
+    run(settle, frame)
+    run(warmup, g)
+    if not no_graph:
+        arena.step_graph(g, 0)  # captures (if new) and warms the timed graph
     arena.sync()
+    settle_s = time.perf_counter() - t
     t0 = time.perf_counter()
-    run(steps)
+    run(steps, g)
     arena.sync()
     total_s = time.perf_counter() - t0
+    window = {
+        "settle_substeps": settle,
+        "warmup_substeps": warmup + (0 if no_graph else g),
+        "launch": "plain launches, one per substep and element kind" if no_graph
+        else f"hipGraph replay: {steps // g} graphs of {g} substeps",
+        "settle_s": settle_s,
+    }
+    if window_stats:
+        window.update(window_fractions(arena, ids, specs))
     _, kernel_ms = arena.time_steps(min(steps, 240), mode=0)
-    return arena, total_s, kernel_ms
+    return arena, total_s, kernel_ms, window
+
+
+def window_fractions(arena, ids, specs):
+    """Contact and yield state at the end of the timed region, from one more substep with
+    diagnostics: scenes with a ground-contact node, contact-active boundary nodes, scenes
+    and J2 elements yielding, scenes whose state is not finite."""
+    from flatworld_amd import _native
+
+    arena.step(1, -1.0, _native.FW_STEP_DIAG)
+    arena.sync()
+    sc = bn = nb = sy = ey = ej2 = bad = 0
+    for sid, s in zip(ids, specs):
+        c = arena.read(sid, _native.FIELD_CONTACT)
+        hit = int(np.count_nonzero(c))
+        sc += int(hit > 0)
+        bn += hit
+        nb += len(s.bnodes)
+        if s.mat_type == 3:
+            y = arena.read(sid, _native.FIELD_YIELD)
+            ny = int(np.count_nonzero(y))
+            sy += int(ny > 0)
+            ey += ny
+            ej2 += s.n_elems
+        if not np.isfinite(arena.read(sid, _native.FIELD_X)).all():
+            bad += 1
+    return {
+        "contact_scenes_frac": sc / len(specs),
+        "contact_boundary_nodes_frac": bn / max(nb, 1),
+        "j2_scenes_yielding_frac": sy / max(sum(1 for s in specs if s.mat_type == 3), 1),
+        "j2_elements_yielding_frac": ey / max(ej2, 1),
+        "nonfinite_scenes": bad,
+    }
+
+
+def landing_substeps(specs, g=9.8, margin=60):
+    """Substeps until every scene standing over a height field has reached it: the
+    largest free-fall time from the body's lowest node to the highest height sample
+    under it, over the scene's dt, plus a margin.  Scenes on other grounds count 0."""
+    worst = 0
+    for s in specs:
+        for gr in s.grounds:
+            if gr.kind != 2:
+                continue
+            h = np.asarray(gr.heights, dtype=np.float64)
+            xs = np.linspace(float(gr.hf_lb_x), float(gr.hf_ub_x), h.size)
+            x0 = np.asarray(s.x0, dtype=np.float64)
+            under = (xs >= x0[:, 0].min()) & (xs <= x0[:, 0].max())
+            gap = x0[:, 1].min() - (h[under].max() if under.any() else h.max())
+            if gap > 0:
+                worst = max(worst, int(np.ceil(np.sqrt(2.0 * gap / g) / float(s.dt))))
+    return worst + margin if worst else 0
+
+
+def single_scene_leg(config, args):
+    """One of the single-scene configurations (the 1000x1000 Neo-Hookean sheet on a plane,
+    or the 512x512 J2 block on 256x256 voxels), timed the way the batched mix is: tuned,
+    graph-replayed, kernel time from HIP events, contact and yield state of the window,
+    and the oracle on the same scene as its CPU baseline."""
+    from flatworld_amd import workloads as W
+
+    t = time.perf_counter()
+    specs = build_specs(config, args.seed, 1)
+    E = specs[0].n_elems
+    N = specs[0].n_nodes
+    algo = float(W.algorithmic_bytes(specs[0]))
+    tuning = tune_for(specs, args)
+    # the J2 block falls 2 mm into the top voxel row in about 100 substeps; the sheet
+    # presses on its plane from the first substep
+    settle = 200 if config == "j2voxel" else 0
+    arena, total_s, kernel_ms, window = timed_arena(specs, tuning, args.single_steps, args.warmup, args.no_graph,
+                                                    settle=settle, window_stats=True)
+    info = arena.info()
+    arena.close()
+    achieved = algo / (kernel_ms * 1e-3) / 1e9
+    ms = total_s * 1e3 / args.single_steps
+    log(f"[bench] {config}: E={E}, tiles={info['n_tiles']}, {args.single_steps} substeps in {total_s * 1e3:.1f} ms, "
+        f"kernel {kernel_ms * 1e3:.1f} us, frac {achieved / HBM_PEAK_GBS:.3f} ({time.perf_counter() - t:.1f}s with setup)")
+    out = {
+        "value": E * args.single_steps / total_s,
+        "unit": "element-substeps/s",
+        "workload": {"large": "1000x1000 jittered grid, 2 M triangles, Neo-Hookean, plane y=0, CFL dt",
+                     "j2voxel": "512x512 block, 524 k triangles, J2, falling at 2 m/s onto 256x256 value-noise "
+                                "voxels"}[config],
+        "elements": E,
+        "nodes": N,
+        "tiles": info["n_tiles"],
+        "tiles_per_cu": info["tiles_per_cu"],
+        "steps": args.single_steps,
+        "ms_per_step": ms,
+        "window": window,
+        "tuning": tuning,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "frac_wall": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernel_ms": kernel_ms,
+            "algo_bytes_per_substep": algo,
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(specs, args.cpu_budget / 3.0, all_cores=False)
+    return out
 
 
 def spring_bench(args):
@@ -265,7 +409,8 @@ def spring_bench(args):
     N = sum(s.n_nodes for s in specs)
     algo = float(sum(W.algorithmic_bytes(s) for s in specs))
     tuning = tune_for(specs, args)
-    arena, total_s, kernel_ms = timed_arena(specs, tuning, args.spring_steps, args.warmup, args.no_graph)
+    arena, total_s, kernel_ms, window = timed_arena(specs, tuning, args.spring_steps, args.warmup, args.no_graph,
+                                                    settle=landing_substeps(specs))
     info = arena.info()
     arena.close()
     achieved = algo / (kernel_ms * 1e-3) / 1e9
@@ -286,6 +431,7 @@ def spring_bench(args):
         "hbm_frac": achieved / HBM_PEAK_GBS,
         "achieved_GBps": achieved,
         "tuning": tuning,
+        "window": window,
     }
     shape = {k: tuning[k] for k in ("tile_nodes", "tile_slots", "block")}
     out["traffic"], out["traffic_source"] = traffic_from_profile(shape, f"springs-{len(specs)}")
@@ -435,6 +581,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-graph", action="store_true", help="plain launches instead of hipGraph frames")
+    ap.add_argument("--settle", type=int, default=-1,
+                    help="untimed substeps before the timed region (-1: until every block has reached its height field)")
+    ap.add_argument("--single-steps", type=int, default=240,
+                    help="timed substeps of the single-scene legs (the 1000x1000 sheet, the J2 block on voxels)")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-scene legs")
     ap.add_argument("--spring-scenes", type=int, default=4096, help="spring lattices of the spring leg (0: skip)")
     ap.add_argument("--spring-steps", type=int, default=600)
     ap.add_argument("--contact-groups", type=int, default=2048,
@@ -472,9 +623,12 @@ def main():
 
     # timed graph frames, then the live kernel duration: HIP events around each substep
     # kernel on the arena stream
-    arena, total_s, kernel_ms = timed_arena(specs, tuning, args.steps, args.warmup, args.no_graph)
+    settle = landing_substeps(specs) if args.settle < 0 else args.settle
+    arena, total_s, kernel_ms, window = timed_arena(specs, tuning, args.steps, args.warmup, args.no_graph,
+                                                    settle=settle, window_stats=True)
     info = arena.info()
     value = E * args.steps / total_s
+    log(f"[bench] window: {window}")
     achieved = algo / (kernel_ms * 1e-3) / 1e9
     log(f"[bench] {args.config}: {len(specs)} scenes, E={E}, N={N}, tiles={info['n_tiles']}, "
         f"halo slots={info['halo_slots']}, {time.perf_counter() - t_setup:.1f}s with setup; "
@@ -507,6 +661,10 @@ def main():
     if args.contact_groups > 0 and args.config == "batched":
         contact = contact_bench(args)
 
+    single = None
+    if args.config == "batched" and not args.no_single:
+        single = {c: single_scene_leg(c, args) for c in ("large", "j2voxel")}
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, args.cpu_budget)
@@ -534,6 +692,7 @@ def main():
             "nodes": N,
             "frame_substeps": frame,
             "seed": args.seed,
+            "timed_window": window,
         },
         "tuning": tuning,
         "roofline": {
@@ -542,6 +701,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "frac_wall": algo / (total_s / args.steps) / 1e9 / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
             "kernel_ms": kernel_ms,
@@ -551,6 +711,7 @@ def main():
         "serving": serving,
         "springs": springs,
         "contact": contact,
+        "single_scene": single,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -169,6 +169,8 @@ SIGNATURES = {
     "fw_scene_reset_gate": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_step_graph": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]),
     "fw_sync": (ctypes.c_int32, [ctypes.c_void_p]),
+    "fw_arena_track_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_arena_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "fw_scene_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_scene_write": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_tiling_check": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
@@ -430,6 +432,17 @@ class Arena:
 
     def sync(self):
         check(self._L.fw_sync(self.h), "fw_sync")
+
+    def track_busy(self, enable=True):
+        """Bracket every stepping call with HIP events (GPU-busy accounting)."""
+        check(self._L.fw_arena_track_busy(self.h, 1 if enable else 0), "fw_arena_track_busy")
+
+    def busy(self):
+        """(GPU milliseconds of the bracketed stepping calls, number of calls) since tracking began."""
+        ms = ctypes.c_double()
+        calls = ctypes.c_int64()
+        check(self._L.fw_arena_busy(self.h, ctypes.byref(ms), ctypes.byref(calls)), "fw_arena_busy")
+        return ms.value, int(calls.value)
 
     def read(self, sid, field):
         if field == FIELD_AABB:

@@ -57,13 +57,16 @@ class FemBatch:
         return self.arena.progress(sid)
 
     # ------------------------------------------------------------ stepping
-    def step(self, n, graph=True, diag=False):
-        """Advance every resident scene by n substeps (asynchronous)."""
+    def step(self, n, graph=True, diag=False, frame=None):
+        """Advance every resident scene by n substeps (asynchronous): whole frames of
+        ``frame`` substeps (default: the batch's) as graph replays, the rest as plain
+        launches."""
         flags = _native.FW_STEP_DIAG if diag else 0
-        if graph and self.frame > 0:
-            full, rest = divmod(int(n), self.frame)
+        frame = self.frame if frame is None else int(frame)
+        if graph and frame > 0:
+            full, rest = divmod(int(n), frame)
             for _ in range(full):
-                self.arena.step_graph(self.frame, flags)
+                self.arena.step_graph(frame, flags)
             if rest:
                 self.arena.step(rest, -1.0, flags)
         else:

@@ -610,10 +610,21 @@ struct fw_arena {
     bool tiles_dirty = false;
     int64_t clock = 0;  // host mirror of ctl->clock
     std::vector<std::unique_ptr<Staging>> staging;
-    // graph cache
-    hipGraphExec_t graph = nullptr;
-    int graph_n = -1;
-    uint32_t graph_flags = 0;
+    // graph cache: one executable graph per (substeps, flags), all captured for the same
+    // launch grid and element kinds; dropped together when either changes
+    struct GraphEntry {
+        hipGraphExec_t exec;
+        int n;
+        uint32_t flags;
+    };
+    std::vector<GraphEntry> graphs;
+    // GPU-busy accounting (fw_arena_busy): events bracketing each fw_step / fw_step_graph
+    // call on the arena stream, resolved when queried
+    bool track_busy = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> busy_pending;
+    std::vector<hipEvent_t> busy_pool;
+    double busy_ms = 0.0;
+    int64_t busy_calls = 0;
     int graph_grid = 0;
     int graph_kinds = 0;  // bit 0: triangle launch captured, bit 1: spring launch captured
     // FEM-FEM contact groups (id -> group, nullptr once removed) and the device
@@ -745,6 +756,11 @@ int upload(fw_arena* a, void* dst, const void* src, int64_t bytes) {
     return FW_OK;
 }
 
+void drop_graphs(fw_arena* a) {
+    for (auto& g : a->graphs) (void)hipGraphExecDestroy(g.exec);
+    a->graphs.clear();
+}
+
 int ensure_tiles(fw_arena* a) {
     if (!a->tiles_dirty) return FW_OK;
     a->tile_table.clear();
@@ -765,10 +781,7 @@ int ensure_tiles(fw_arena* a) {
         if (a->d_tiles) HIPCHK(hipFree(a->d_tiles));
         a->d_tiles = nd;
         a->tile_cap = cap;
-        if (a->graph) {  // graph holds the old table pointer
-            (void)hipGraphExecDestroy(a->graph);
-            a->graph = nullptr;
-        }
+        drop_graphs(a);  // the graphs hold the old table pointer
     }
     int rc = upload(a, a->d_tiles, a->tile_table.data(), (int64_t)sizeof(TileDev) * nt);
     if (rc) return rc;
@@ -1171,7 +1184,12 @@ int32_t fw_arena_destroy(fw_arena* a) {
     if (!a) return FW_OK;
     (void)hipSetDevice(a->device);
     (void)hipStreamSynchronize(a->stream);
-    if (a->graph) (void)hipGraphExecDestroy(a->graph);
+    drop_graphs(a);
+    for (auto& p : a->busy_pending) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    for (auto e : a->busy_pool) (void)hipEventDestroy(e);
     for (auto& s : a->staging) {
         (void)hipHostFree(s->ptr);
         (void)hipEventDestroy(s->ev);
@@ -1303,6 +1321,74 @@ int32_t fw_scene_progress(fw_arena* a, int32_t id, int64_t* steps_done, int32_t*
     return FW_OK;
 }
 
+// GPU-busy accounting: an event pair around each stepping call, when enabled.
+static int busy_event(fw_arena* a, hipEvent_t* e) {
+    if (!a->busy_pool.empty()) {
+        *e = a->busy_pool.back();
+        a->busy_pool.pop_back();
+        return FW_OK;
+    }
+    HIPCHK(hipEventCreate(e));
+    return FW_OK;
+}
+
+static int busy_begin(fw_arena* a, hipEvent_t* e0) {
+    if (!a->track_busy) return FW_OK;
+    int rc = busy_event(a, e0);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(*e0, a->stream));
+    return FW_OK;
+}
+
+static int busy_end(fw_arena* a, hipEvent_t e0) {
+    if (!a->track_busy) return FW_OK;
+    hipEvent_t e1;
+    int rc = busy_event(a, &e1);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e1, a->stream));
+    a->busy_pending.push_back({e0, e1});
+    // resolve completed pairs now and then, so the pending list stays short
+    if (a->busy_pending.size() >= 64) {
+        size_t k = 0;
+        for (; k < a->busy_pending.size(); ++k) {
+            auto& p = a->busy_pending[k];
+            if (hipEventQuery(p.second) != hipSuccess) break;
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
+            a->busy_ms += ms;
+            a->busy_calls += 1;
+            a->busy_pool.push_back(p.first);
+            a->busy_pool.push_back(p.second);
+        }
+        a->busy_pending.erase(a->busy_pending.begin(), a->busy_pending.begin() + (long)k);
+    }
+    return FW_OK;
+}
+
+int32_t fw_arena_track_busy(fw_arena* a, int32_t enable) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    a->track_busy = enable != 0;
+    return FW_OK;
+}
+
+int32_t fw_arena_busy(fw_arena* a, double* busy_ms, int64_t* calls) {
+    if (!a || !busy_ms) return fail(FW_E_INVALID, "bad arguments");
+    HIPCHK(hipSetDevice(a->device));
+    for (auto& p : a->busy_pending) {
+        HIPCHK(hipEventSynchronize(p.second));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, p.first, p.second));
+        a->busy_ms += ms;
+        a->busy_calls += 1;
+        a->busy_pool.push_back(p.first);
+        a->busy_pool.push_back(p.second);
+    }
+    a->busy_pending.clear();
+    *busy_ms = a->busy_ms;
+    if (calls) *calls = a->busy_calls;
+    return FW_OK;
+}
+
 // The substep launches of one substep over the whole tile table: one per element
 // kind present (triangle tiles, then spring tiles), each covering its kind's tiles
 // as the control block counts them at run time.
@@ -1360,7 +1446,14 @@ int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_overrid
     if (!a || n < 0) return fail(FW_E_INVALID, "bad arguments");
     if (n == 0) return FW_OK;
     HIPCHK(hipSetDevice(a->device));
-    return step_impl(a, n, dt_override, damping_override, flags);
+    hipEvent_t e0 = nullptr;
+    int rc = ensure_tiles(a);  // table uploads go ahead of the busy bracket
+    if (rc) return rc;
+    rc = busy_begin(a, &e0);
+    if (rc) return rc;
+    rc = step_impl(a, n, dt_override, damping_override, flags);
+    if (rc) return rc;
+    return busy_end(a, e0);
 }
 
 int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile_slots, int64_t* stats) {
@@ -1489,10 +1582,14 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
     // the element kinds present decide which launches a frame holds; the graph's grids
     // cover the table's capacity, so scenes can come and go without a new capture
     const int kinds = (a->n_tri_tiles > 0 || nt == 0 ? 1 : 0) | (nt > a->n_tri_tiles ? 2 : 0);
-    if (!a->graph || a->graph_n != n || a->graph_flags != flags || grid > a->graph_grid || kinds != a->graph_kinds) {
-        if (a->graph) {
-            (void)hipGraphExecDestroy(a->graph);
-            a->graph = nullptr;
+    if (grid > a->graph_grid || kinds != a->graph_kinds) drop_graphs(a);
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : a->graphs)
+        if (g.n == n && g.flags == flags) exec = g.exec;
+    if (!exec) {
+        if (a->graphs.size() >= 8) {  // keep the cache small: the oldest entry goes
+            (void)hipGraphExecDestroy(a->graphs.front().exec);
+            a->graphs.erase(a->graphs.begin());
         }
         const int cap_grid = std::max(grid, launch_grid(a, a->tile_cap));
         hipGraph_t g;
@@ -1509,14 +1606,21 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
         hipError_t e = fw_launch_advance_clock(a->d_ctl, n, a->stream);
         hipError_t e2 = hipStreamEndCapture(a->stream, &g);
         if (e != hipSuccess || e2 != hipSuccess) return fail(FW_E_HIP, "graph capture failed");
-        HIPCHK(hipGraphInstantiate(&a->graph, g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
-        a->graph_n = n;
-        a->graph_flags = flags;
-        a->graph_grid = cap_grid;
-        a->graph_kinds = kinds;
+        hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(FW_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        if (a->graphs.empty()) {
+            a->graph_grid = cap_grid;
+            a->graph_kinds = kinds;
+        }
+        a->graphs.push_back({exec, n, flags});
     }
-    HIPCHK(hipGraphLaunch(a->graph, a->stream));
+    hipEvent_t e0 = nullptr;
+    rc = busy_begin(a, &e0);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(exec, a->stream));
+    rc = busy_end(a, e0);
+    if (rc) return rc;
     a->clock += n;
     return FW_OK;
 }

@@ -14,6 +14,14 @@ arrives at any time and is:
    first substep in the arena);
 4. retired when it reaches its horizon: its final state is copied out and its
    memory is released (stream-ordered, without stalling the running frame).
+   A frame is shortened (``frame``, then halves down to ``min_frame``
+   substeps) when the resident job closest to its horizon needs fewer
+   substeps, so a job waits at most ``min_frame - 1`` substeps past its horizon
+   for retirement; each frame length is a cached graph of its own.
+
+GPU-busy time is measured on the device: HIP events bracket every frame on the
+arena stream (``Arena.track_busy``), so host admission and retirement between
+frames do not count as busy.
 
 Per-job latency is arrival -> result copied to the host.  A job's result is
 bit-identical whether it runs alone or among others, because no device sum
@@ -67,9 +75,13 @@ class SimServer:
     """
 
     def __init__(self, batch=None, frame=60, keep_results=True, clock=time.perf_counter, prep_workers=None,
-                 **batch_kw):
+                 min_frame=8, **batch_kw):
         self.batch = batch or FemBatch(frame=frame, **batch_kw)
         self.frame = int(frame)
+        min_frame = max(1, min(int(min_frame), self.frame))
+        self.frame_lengths = [self.frame]
+        while self.frame_lengths[-1] > min_frame:
+            self.frame_lengths.append(max(min_frame, self.frame_lengths[-1] // 2))
         self.keep_results = keep_results
         self.clock = clock
         if prep_workers is None:
@@ -80,8 +92,22 @@ class SimServer:
     def reset_stats(self):
         self.results = []
         self.frames = 0
+        self.substeps = 0
         self.busy_time = 0.0
         self.elem_substeps = 0
+
+    def _next_frame(self, resident):
+        """The longest frame length that does not carry the job closest to its horizon past it."""
+        b = self.batch
+        rem = None
+        for sid, (job, _) in resident.items():
+            done = b.progress(sid)[0]
+            r = job.spec.horizon - done + (1 if done == 0 else 0)  # + its warm-up substep
+            rem = r if rem is None else min(rem, r)
+        for n in self.frame_lengths:
+            if n <= rem:
+                return n
+        return self.frame_lengths[-1]
 
     def run(self, jobs, max_wall=None):
         """Serve ``jobs`` (arrival times relative to the start of this call); returns the results.
@@ -133,7 +159,14 @@ class SimServer:
                 try:
                     sid = b.add_prepared(prep) if prep is not None else b.add(job.spec)
                 except RuntimeError:
-                    break  # fragmented: wait for retirements
+                    if resident:
+                        break  # fragmented: wait for retirements
+                    # the empty arena cannot take it: the job fails alone instead of blocking the line
+                    waiting.popleft()
+                    prepared.pop(job.job_id, None)
+                    t = self.clock() - t0
+                    self.results.append(JobResult(job.job_id, job.arrival, t, t, job.spec.horizon, job.spec.n_elems, False))
+                    continue
                 waiting.popleft()
                 prepared.pop(job.job_id, None)
                 resident[sid] = (job, self.clock() - t0)
@@ -156,39 +189,47 @@ class SimServer:
                 )
 
         self._t0 = t0
+        arena = getattr(b, "arena", None)
+        busy0 = None
+        if arena is not None:
+            arena.track_busy(True)
+            busy0 = arena.busy()[0]
         try:
             self._loop(pending, waiting, resident, admit, retire, max_wall)
         finally:
             if pool is not None:
                 pool.shutdown(wait=True)
         self.wall = self.clock() - t0
+        if busy0 is not None:
+            self.busy_time = (arena.busy()[0] - busy0) * 1e-3  # GPU time of the frames, from HIP events
         return self.results
 
     def _loop(self, pending, waiting, resident, admit, retire, max_wall=None):
         b = self.batch
         t0 = self._t0
         in_flight = False
-        launched = 0.0
         while pending or waiting or resident:
             if max_wall is not None and self.clock() - t0 > max_wall:
                 break
             if in_flight:
                 admit()  # overlaps the running frame
                 b.sync()
-                self.busy_time += self.clock() - launched
                 in_flight = False
                 retire()
             admit()
             if resident:
-                launched = self.clock()
-                b.step(self.frame, graph=True)
+                n = self._next_frame(resident)
+                b.step(n, graph=True, frame=n)
                 self.frames += 1
+                self.substeps += n
                 in_flight = True
+            elif waiting:
+                # nothing resident and the head of the line is still being tiled
+                time.sleep(0.0002)
             elif pending:
                 time.sleep(max(0.0, min(pending[0].arrival - (self.clock() - t0), 0.01)))
         if in_flight:
             b.sync()
-            self.busy_time += self.clock() - launched
 
     def stats(self):
         lat = np.array([r.latency for r in self.results]) if self.results else np.zeros(1)
@@ -197,6 +238,7 @@ class SimServer:
             "failed": sum(1 for r in self.results if not r.ok),
             "wall_s": self.wall,
             "frames": self.frames,
+            "substeps": self.substeps,
             "gpu_busy_frac": self.busy_time / max(self.wall, 1e-9),
             "elem_substeps_per_s": self.elem_substeps / max(self.wall, 1e-9),
             "latency_p50_s": float(np.percentile(lat, 50)),

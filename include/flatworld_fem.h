@@ -301,6 +301,15 @@ int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_overrid
 int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags);
 int32_t fw_sync(fw_arena* a);
 
+/* GPU-busy accounting for a server loop.  With tracking on, every fw_step and
+ * fw_step_graph call is bracketed by a pair of HIP events on the arena stream;
+ * fw_arena_busy waits for the bracketed work and returns the summed GPU time of
+ * those calls in milliseconds and their number, since tracking was first turned
+ * on.  Host work between calls (admission, tiling) is outside every bracket.  No
+ * reference counterpart: the reference has no served form. */
+int32_t fw_arena_track_busy(fw_arena* a, int32_t enable);
+int32_t fw_arena_busy(fw_arena* a, double* busy_ms, int64_t* calls);
+
 /* Host-only self-check of the tiling fw_scene_add would build (no GPU needed):
  * stats[0] tiles, [1] halo slots, [2] largest tile slot count, [3] CSR entries.
  * Returns FW_E_INVALID with a message if any invariant fails. */

@@ -22,6 +22,7 @@ class FakeBatch:
         self.next_id = 0
         self.max_resident = 0
         self.admit_order = []
+        self.frames = []
 
     def footprint(self, spec):
         return spec.bytes
@@ -43,7 +44,8 @@ class FakeBatch:
     def remove(self, sid):
         self.used -= self.scenes.pop(sid)["spec"].bytes
 
-    def step(self, n, graph=True):
+    def step(self, n, graph=True, frame=None):
+        self.frames.append(n)
         for s in self.scenes.values():
             s["ran"] += n
 
@@ -142,3 +144,16 @@ def test_prepared_admission_keeps_fifo_and_isolates_untileable_jobs():
     assert b.admit_order == [0, 1, 2, 3, 4]
     assert b.max_resident == 2
     assert len(b.prepared_on) >= 1
+
+
+def test_frames_shorten_near_a_horizon():
+    """A job is retired at most min_frame - 1 substeps past its horizon: the frame
+    halves (60, 30, 15, 8) when the closest job needs fewer substeps."""
+    b = FakeBatch(capacity=100)
+    srv = SimServer(batch=b, frame=60, min_frame=8)
+    assert srv.frame_lengths == [60, 30, 15, 8]
+    res = srv.run([Job(0, spec(0, 100, 40), 0.0)])
+    assert res[0].ok
+    # warm-up + 100 substeps = 101: 60 + 30 + 8 + 8 (overshoot 5 < 8)
+    assert b.frames == [60, 30, 8, 8]
+    assert sum(b.frames) - 101 < 8
