@@ -249,7 +249,8 @@ def timed_arena(specs, tuning, steps, warmup, no_graph=False, settle=0, window_s
     replay capture the timed graph; then ``steps`` substeps are timed as replays of one
     graph of ``graph_length(steps, frame)`` substeps (plain launches with ``no_graph``).
     Returns the arena, the wall time of the timed region, the average substep-kernel
-    duration from HIP events on the arena stream, and the timed region's description."""
+    duration from HIP events over the timed region on the arena stream, and the timed
+    region's description."""
     from flatworld_amd import _native
     from flatworld_amd import autotune
 
@@ -277,10 +278,18 @@ def timed_arena(specs, tuning, steps, warmup, no_graph=False, settle=0, window_s
         arena.step_graph(g, 0)  # captures (if new) and warms the timed graph
     arena.sync()
     settle_s = time.perf_counter() - t
+    # HIP events bracket every stepping call of the timed region on the arena stream:
+    # their summed GPU time over the substeps is the average substep-kernel duration
+    # (within a graph the launches run back to back; the bracket also holds the
+    # one-thread clock kernel that ends each graph)
+    arena.track_busy(True)
+    busy0 = arena.busy()[0]
     t0 = time.perf_counter()
     run(steps, g)
     arena.sync()
     total_s = time.perf_counter() - t0
+    kernel_ms = (arena.busy()[0] - busy0) / steps
+    arena.track_busy(False)
     window = {
         "settle_substeps": settle,
         "warmup_substeps": warmup + (0 if no_graph else g),
@@ -290,7 +299,8 @@ def timed_arena(specs, tuning, steps, warmup, no_graph=False, settle=0, window_s
     }
     if window_stats:
         window.update(window_fractions(arena, ids, specs))
-    _, kernel_ms = arena.time_steps(min(steps, 240), mode=0)
+    # for comparison: events around each plain launch of the same kernel, after the window
+    _, window["kernel_ms_per_launch_events"] = arena.time_steps(min(steps, 240), mode=0)
     return arena, total_s, kernel_ms, window
 
 

@@ -259,7 +259,12 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         td.n_nodes = p1 - p0;
         td.elem0 = own_count[t];
         td.n_own = own_count[t + 1] - own_count[t];
-        td.multi = ntiles > 1;
+        {
+            // AABB sub-slot: tiles share one only when the scene has more tiles than sub-slots
+            const int kk = std::min(ntiles, FW_AABB_MAX_SUB);
+            td.aabb_slot = t % kk;
+            td.aabb_mode = ntiles <= kk ? 0 : (t < kk ? 2 : 1);
+        }
         // slots: owned elements in storage order, then halo elements (ascending storage index)
         slots.clear();  // internal element index per slot
         for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) slots.push_back(i);
@@ -408,7 +413,8 @@ void build_hf_normals(const GroundDev& G, const float* h, float2* nrm) {
 struct Layout {
     int64_t pos0, pos1, rest, vel, mass, flag, bcv, disp, force, cflag;
     int64_t jinv, area, j2a, j2b, stress, strain, yflag;
-    int64_t rec, hel, hpos0, hpos1, hrest, hvel0, hvel1, push, cptr, tiles;
+    int64_t rec, hel, hpos0, hpos1, hrest, hvel0, hvel1, push, cptr, tiles, aabb;
+    int32_t aabb_k;
     int64_t gdata[FW_MAX_GROUNDS];
     int64_t host_bytes;  // [0, host_bytes) is built on the host; the rest is cleared on the device
     int64_t total;
@@ -441,6 +447,8 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
     L.push = take(8 * n_hnode);
     L.cptr = take(2 * n_ptr);
     L.tiles = take((int64_t)sizeof(TileDev) * n_tiles);
+    L.aabb_k = (int32_t)std::max<int64_t>(1, std::min<int64_t>(n_tiles, FW_AABB_MAX_SUB));
+    L.aabb = take(3 * (int64_t)L.aabb_k * FW_AABB_SUB_INTS * 4);
     for (int g = 0; g < FW_MAX_GROUNDS; ++g) {
         L.gdata[g] = -1;
         if (g >= d->n_grounds) continue;
@@ -905,12 +913,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
     D.hot.damping = d->damping;
     D.horizon = P->horizon;
     D.live = 1;
-    for (int b = 0; b < 3; ++b) {
-        D.hot.aabb[b][0] = fw_f2o(1e30f);
-        D.hot.aabb[b][1] = fw_f2o(1e30f);
-        D.hot.aabb[b][2] = fw_f2o(-1e30f);
-        D.hot.aabb[b][3] = fw_f2o(-1e30f);
-    }
+    D.hot.aabb_k = L.aabb_k;
     auto o8 = [](int64_t off) { return (uint32_t)(off >> 8); };  // layout offsets are 256-byte aligned
     for (auto& td : T.tiles) {
         td.base = nullptr;
@@ -953,6 +956,16 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
         flag[i] = d->flags[o];
     }
     if (L.rest >= 0) std::memcpy(img + L.rest, img + L.pos0, 8 * (size_t)N);
+    {
+        // every AABB sub-slot of the three buffers starts empty
+        int32_t* ab = (int32_t*)(img + L.aabb);
+        for (int j = 0; j < 3 * L.aabb_k; ++j) {
+            ab[j * FW_AABB_SUB_INTS + 0] = fw_f2o(1e30f);
+            ab[j * FW_AABB_SUB_INTS + 1] = fw_f2o(1e30f);
+            ab[j * FW_AABB_SUB_INTS + 2] = fw_f2o(-1e30f);
+            ab[j * FW_AABB_SUB_INTS + 3] = fw_f2o(-1e30f);
+        }
+    }
     {
         float2* hp = (float2*)(img + L.hpos0);
         for (size_t j = 0; j < T.hnode.size(); ++j) hp[j] = pos0[T.hnode[j]];
@@ -1063,6 +1076,7 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
     D.stress = (float4*)at(L.stress);
     D.strain = (float4*)at(L.strain);
     D.yflag = (uint8_t*)at(L.yflag);
+    D.hot.aabb = (int32_t*)(blk + L.aabb);
     for (int g = 0; g < D.hot.n_grounds; ++g)
         if (L.gdata[g] >= 0) D.ground[g].data = blk + L.gdata[g];
     // warm-up runs at the next substep launched (local step -1)
@@ -1668,12 +1682,22 @@ int32_t fw_scene_read(fw_arena* a, int32_t id, int32_t field, void* out, int64_t
     HIPCHK(hipStreamSynchronize(a->stream));
     if (field == FW_FIELD_AABB) {
         if (out_bytes < 16) return fail(FW_E_INVALID, "output buffer too small");
-        // the AABB at the current x was written by the last completed local step
-        int32_t enc[3][4];
-        HIPCHK(hipMemcpy(enc, &a->d_scenes[id].hot.aabb[0][0], sizeof(enc), hipMemcpyDeviceToHost));
-        int slot = warmed ? (int)(((done % 3) + 3) % 3) : 0;
+        // the AABB at the current x was written by the last completed local step, in the
+        // sub-slots of one buffer
+        const int kk = s->dev.hot.aabb_k;
+        std::vector<int32_t> enc((size_t)3 * kk * FW_AABB_SUB_INTS);
+        HIPCHK(hipMemcpy(enc.data(), s->dev.hot.aabb, enc.size() * 4, hipMemcpyDeviceToHost));
+        const int slot = warmed ? (int)(((done % 3) + 3) % 3) : 0;
+        int32_t r[4] = {fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)};
+        for (int j = 0; j < kk; ++j) {
+            const int32_t* e = enc.data() + ((size_t)slot * kk + j) * FW_AABB_SUB_INTS;
+            r[0] = std::min(r[0], e[0]);
+            r[1] = std::min(r[1], e[1]);
+            r[2] = std::max(r[2], e[2]);
+            r[3] = std::max(r[3], e[3]);
+        }
         float* o = (float*)out;
-        for (int k = 0; k < 4; ++k) o[k] = fw_o2f(enc[slot][k]);
+        for (int k = 0; k < 4; ++k) o[k] = fw_o2f(r[k]);
         return FW_OK;
     }
     const int64_t count = per_node ? s->n_nodes : s->n_elems;

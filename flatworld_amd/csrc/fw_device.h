@@ -268,6 +268,22 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Min of v[0..1] and max of v[2..3] over the 64 lanes of a wave (ordered ints);
+// every lane ends with the result.
+__device__ __forceinline__ void wave_minmax(int32_t v[4]) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int32_t a0 = __shfl_xor(v[0], off, 64);
+        const int32_t a1 = __shfl_xor(v[1], off, 64);
+        const int32_t a2 = __shfl_xor(v[2], off, 64);
+        const int32_t a3 = __shfl_xor(v[3], off, 64);
+        v[0] = min(v[0], a0);
+        v[1] = min(v[1], a1);
+        v[2] = max(v[2], a2);
+        v[3] = max(v[3], a3);
+    }
+}
+
 template <int BLOCK>
 __device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
     // v: lb.x, lb.y (min), ub.x, ub.y (max) as ordered ints

@@ -100,10 +100,20 @@ __device__ __forceinline__ CellRange cell_range(const FlexGrid& g, float lbx, fl
     return r;
 }
 
-// The boundary AABB a scene's gate reads at `local` (ordered-int encoded).
+// The boundary AABB a scene's gate reads at `local`: min / max over the sub-slots of
+// buffer ar3 (ordered-int encoded).
 __device__ __forceinline__ void scene_aabb(const SceneDev* S, int ar3, float bb[4]) {
-    const int32_t* e = S->hot.aabb[ar3];
-    for (int k = 0; k < 4; ++k) bb[k] = fw_o2f(e[k]);
+    const int kk = S->hot.aabb_k;
+    const int32_t* e = S->hot.aabb + ar3 * kk * FW_AABB_SUB_INTS;
+    int32_t r[4] = {e[0], e[1], e[2], e[3]};
+    for (int j = 1; j < kk; ++j) {
+        const int32_t* f = e + j * FW_AABB_SUB_INTS;
+        r[0] = imin_(r[0], f[0]);
+        r[1] = imin_(r[1], f[1]);
+        r[2] = imax_(r[2], f[2]);
+        r[3] = imax_(r[3], f[3]);
+    }
+    for (int k = 0; k < 4; ++k) bb[k] = fw_o2f(r[k]);
 }
 
 // Workgroup exclusive scan of one value per thread; *total gets the sum.

@@ -339,6 +339,14 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             m.sy = spring ? 0.0f : fw_ldc1(&sh->sigma_y);
             m.H = spring ? 0.0f : fw_ldc1(&sh->H);
         }
+        // the gate's AABB: every lane loads one sub-slot of this step's buffer (lanes past
+        // the sub-slot count repeat the last), reduced over the wave after phase A
+        int32_t* const aabb_sub = fw_ldc1(&sh->aabb);
+        const int aabb_k = fw_ldc1(&sh->aabb_k);
+        int4 gate_bb = make_int4(0, 0, 0, 0);
+        if (contact)
+            gate_bb = fw_ld(reinterpret_cast<const int4*>(
+                aabb_sub + (mod3(local) * aabb_k + min(tid & 63, aabb_k - 1)) * FW_AABB_SUB_INTS));
         float4* const st_stress = diag ? fw_ldc1(&Sp->stress) : nullptr;
         float4* const st_strain = diag ? fw_ldc1(&Sp->strain) : nullptr;
         uint8_t* const st_yflag = diag ? fw_ldc1(&Sp->yflag) : nullptr;
@@ -393,9 +401,10 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // ---------------- activation gate (flatworld/explicitloop.py:477-487, bvh.py:39-46, 321-342, 393-416)
         uint32_t active = 0;
         if (contact) {
-            const int ar3 = mod3(local);
-            float lbx = fw_o2f(fw_ldc1(&sh->aabb[ar3][0])), lby = fw_o2f(fw_ldc1(&sh->aabb[ar3][1]));
-            float ubx = fw_o2f(fw_ldc1(&sh->aabb[ar3][2])), uby = fw_o2f(fw_ldc1(&sh->aabb[ar3][3]));
+            int32_t g4[4] = {gate_bb.x, gate_bb.y, gate_bb.z, gate_bb.w};
+            wave_minmax(g4);
+            float lbx = fw_o2f(g4[0]), lby = fw_o2f(g4[1]);
+            float ubx = fw_o2f(g4[2]), uby = fw_o2f(g4[3]);
             const bool pad = ((local - fw_ldc1(&sh->gate_base)) % 100) == 0;
             if (pad) {
                 const float bx = 0.1f * (ubx - lbx), by = 0.1f * (uby - lby);
@@ -527,22 +536,22 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // ---------------- boundary AABB at x_{n+1} (flatworld/femspringmanager.py:361-408)
         block_minmax<BLOCK>(bb, lds_red);  // its barrier also publishes the lds_x updates
         if (tid == 0) {
-            int32_t* aw = const_cast<SceneDev*>(Sp)->hot.aabb[mod3(local + 1)];
-            int32_t* az = const_cast<SceneDev*>(Sp)->hot.aabb[mod3(local + 2)];
-            if (T.multi) {
+            // this tile's AABB sub-slot of the next step's buffer; a shared sub-slot takes
+            // atomics, and its resetting tile empties it in the buffer of two steps on (read
+            // at step local + 2, written at local + 1, so never touched during this step)
+            int32_t* aw = aabb_sub + (mod3(local + 1) * aabb_k + T.aabb_slot) * FW_AABB_SUB_INTS;
+            if (T.aabb_mode == 0) {
+                fw_st(reinterpret_cast<int4*>(aw), make_int4(bb[0], bb[1], bb[2], bb[3]));
+            } else {
                 atomicMin(aw + 0, bb[0]);
                 atomicMin(aw + 1, bb[1]);
                 atomicMax(aw + 2, bb[2]);
                 atomicMax(aw + 3, bb[3]);
-                fw_st(az + 0, fw_f2o(1e30f));
-                fw_st(az + 1, fw_f2o(1e30f));
-                fw_st(az + 2, fw_f2o(-1e30f));
-                fw_st(az + 3, fw_f2o(-1e30f));
-            } else {
-                fw_st(aw + 0, bb[0]);
-                fw_st(aw + 1, bb[1]);
-                fw_st(aw + 2, bb[2]);
-                fw_st(aw + 3, bb[3]);
+                if (T.aabb_mode == 2) {
+                    int32_t* az = aabb_sub + (mod3(local + 2) * aabb_k + T.aabb_slot) * FW_AABB_SUB_INTS;
+                    fw_st(reinterpret_cast<int4*>(az),
+                          make_int4(fw_f2o(1e30f), fw_f2o(1e30f), fw_f2o(-1e30f), fw_f2o(-1e30f)));
+                }
             }
         }
 

@@ -108,9 +108,19 @@ struct SceneHot {
     float E, nu, sigma_y, H;
     float dt, damping;
     int32_t gate_base;  // local step that counts as 0 for the gate's rebuild schedule
-    int32_t pad_;
-    int32_t aabb[3][4];  // boundary AABB (ordered-int encoded), triple buffered by local step
+    // Boundary AABB (ordered-int encoded), triple buffered by local step and split in
+    // aabb_k sub-slots (aabb_k = min(tiles, FW_AABB_MAX_SUB)): tile t of the scene
+    // writes sub-slot t % aabb_k, so no more than ceil(tiles / FW_AABB_MAX_SUB) tiles
+    // update one word with atomics, and a reader takes the min / max over the
+    // sub-slots.  Buffer b, sub-slot j: aabb + (b * aabb_k + j) * FW_AABB_SUB_INTS.
+    int32_t aabb_k;
+    int32_t* aabb;
 };
+
+// Sub-slots of a scene's boundary AABB: each 64 bytes apart, so that sub-slots do
+// not share a cache line.
+#define FW_AABB_MAX_SUB 64
+#define FW_AABB_SUB_INTS 16
 
 struct SceneDev {
     SceneHot hot;
@@ -169,7 +179,10 @@ struct TileDev {
     uint32_t o_cptr;         // uint16 node CSR: n_nodes + 1 offsets per tile into its node-major LDS entries
     uint32_t o_push;         // int2 push list: (local owned node, halo copy index in the scene)
     uint32_t o_j2[2];        // float4 J2 state (ping-pong); 0 unless J2
-    int32_t scene, s0, horizon, multi;  // s0 / horizon as in SceneDev; multi: scene has > 1 tile
+    int32_t scene, s0, horizon;         // s0 / horizon as in SceneDev
+    int32_t aabb_slot;                  // AABB sub-slot this tile writes (its index in the scene % aabb_k)
+    int32_t aabb_mode;                  // 0: the sub-slot is this tile's alone (plain stores); 1: shared
+                                        // (atomics); 2: shared, and this tile resets it for two steps on
     int32_t node0, n_nodes;             // owned nodes [node0, node0 + n_nodes)
     int32_t elem0, n_own;               // owned elements [elem0, elem0 + n_own)
     int32_t n_halo;                     // halo element slots after the owned ones
@@ -212,6 +225,7 @@ typedef float fw_v4f __attribute__((ext_vector_type(4)));
 typedef uint32_t fw_v2u __attribute__((ext_vector_type(2)));
 typedef uint32_t fw_v4u __attribute__((ext_vector_type(4)));
 typedef int32_t fw_v2i __attribute__((ext_vector_type(2)));
+typedef int32_t fw_v4i __attribute__((ext_vector_type(4)));
 
 __device__ inline float2 fw_ld(const float2* p) { const fw_v2f v = *(const FW_GLOBAL fw_v2f*)p; return make_float2(v.x, v.y); }
 __device__ inline float4 fw_ld(const float4* p) {
@@ -221,6 +235,10 @@ __device__ inline float4 fw_ld(const float4* p) {
 __device__ inline uint4 fw_ld(const uint4* p) {
     const fw_v4u v = *(const FW_GLOBAL fw_v4u*)p;
     return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ inline int4 fw_ld(const int4* p) {
+    const fw_v4i v = *(const FW_GLOBAL fw_v4i*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
 }
 __device__ inline uint2 fw_ld(const uint2* p) { const fw_v2u v = *(const FW_GLOBAL fw_v2u*)p; return make_uint2(v.x, v.y); }
 __device__ inline int2 fw_ld(const int2* p) { const fw_v2i v = *(const FW_GLOBAL fw_v2i*)p; return make_int2(v.x, v.y); }
@@ -233,6 +251,7 @@ __device__ inline void fw_st(float2* p, float2 a) { *(FW_GLOBAL fw_v2f*)p = fw_v
 __device__ inline void fw_st(float4* p, float4 a) { *(FW_GLOBAL fw_v4f*)p = fw_v4f{a.x, a.y, a.z, a.w}; }
 __device__ inline void fw_st(float* p, float a) { *(FW_GLOBAL float*)p = a; }
 __device__ inline void fw_st(int32_t* p, int32_t a) { *(FW_GLOBAL int32_t*)p = a; }
+__device__ inline void fw_st(int4* p, int4 a) { *(FW_GLOBAL fw_v4i*)p = fw_v4i{a.x, a.y, a.z, a.w}; }
 __device__ inline void fw_st(uint8_t* p, uint8_t a) { *(FW_GLOBAL uint8_t*)p = a; }
 
 // Tile and scene tables are never written while a kernel runs (the host edits

@@ -117,13 +117,20 @@ def j2_voxel_config(side=512, vox=256, seed=0):
     return one_body_spec(dom, ground)
 
 
-# Density and drop height of the mixed-scene workloads.  With rho = 1000 and a
-# 0.3 m drop the softest blocks (E = 1e4: wave speed 3 m/s) hit the ground at
-# 2.2 m/s and invert elements, which turns Neo-Hookean stress into NaN (log of
-# a negative J) within a few hundred substeps; rho = 100 and a block resting
-# 0.05 m above the highest ground point keep every scene finite.
-MIX_RHO = 100.0
-MIX_Y0 = 0.1
+# Density and placement of the mixed-scene workloads (both unspecified by the
+# configurations, so chosen here).  Heavier blocks dropped from higher up invert
+# elements on the rough height fields: with rho = 1000 and a 0.3 m drop 23 of 96
+# sampled scenes went NaN (Neo-Hookean stress is log J, NaN once J < 0); with
+# rho = 100 and a drop of 0.05 to 0.15 m, 119 of the 4096 scenes were NaN after
+# 7200 substeps.  rho = 10 with each block's bottom 5 mm above the highest ground
+# point under it kept all 4096 finite over 12000 substeps, with about 62% of the
+# scenes in ground contact and half of the J2 scenes yielding once they landed.
+MIX_RHO = 10.0
+MIX_GAP = 0.005
+# the spring lattices start with their lowest row at y = 0.1, above every height field
+SPRING_Y0 = 0.1
+# density of the two-block FEM-FEM contact loops
+CONTACT_RHO = 100.0
 
 
 def random_material(rng, rho=MIX_RHO, kinds=(MaterialType.ELASTIC, MaterialType.NEOHOOKEAN, MaterialType.MISES)):
@@ -133,24 +140,44 @@ def random_material(rng, rho=MIX_RHO, kinds=(MaterialType.ELASTIC, MaterialType.
     return make_material(kind, E, nu, rho, 5e-3 * E, 1e-2 * E)
 
 
-def batched_config(n_scenes=4096, seed=0, side_lo=8, side_hi=128):
-    """Uneven scenes: side log-uniform in [side_lo, side_hi], one height field each."""
+def ground_top(ground, x_lo=0.3, x_hi=0.7):
+    """Highest point of a ground under the x range [x_lo, x_hi]: the plane's height at
+    the range (a horizontal plane's y), the highest height sample, or the top of the
+    highest occupied voxel."""
+    if isinstance(ground, GroundDomain):
+        return float(ground.point[1])
+    if isinstance(ground, HeightFieldDomain):
+        xs = np.linspace(ground.lb[0], ground.ub[0], ground.height.shape[0])
+        under = (xs >= x_lo) & (xs <= x_hi)
+        return float(ground.height[under].max() if under.any() else ground.height.max())
+    occ = np.asarray(ground.occ)
+    dx = float(ground.dx)
+    cx = float(ground.lb[0]) + (np.arange(occ.shape[0]) + 0.5) * dx  # cell centres
+    ci = (cx >= x_lo - dx) & (cx <= x_hi + dx)
+    rows = np.flatnonzero(occ[ci].any(axis=0))
+    return float(ground.lb[1] + (rows.max() + 1) * float(ground.dz)) if rows.size else float(ground.lb[1])
+
+
+def batched_config(n_scenes=4096, seed=0, side_lo=8, side_hi=128, rho=MIX_RHO, gap=MIX_GAP, damping=0.0):
+    """Uneven scenes: side log-uniform in [side_lo, side_hi], one height field each; each
+    block's bottom ``gap`` above the highest height sample under it."""
     rng = rng_for(seed)
     specs = []
     for _ in range(n_scenes):
         s = int(round(side_lo * (side_hi / side_lo) ** rng.uniform()))
-        mat = random_material(rng)
-        dom = grid_domain(s, (0.3, MIX_Y0), 0.4, mat, [Gravity([0.0, -9.8])])
+        mat = random_material(rng, rho=rho)
         hf = sinusoid_heightfield(rng)
-        specs.append(one_body_spec(dom, hf))
+        dom = grid_domain(s, (0.3, ground_top(hf) + gap), 0.4, mat, [Gravity([0.0, -9.8])])
+        specs.append(one_body_spec(dom, hf, damping=damping))
     return specs
 
 
-def serving_job(rng, side_lo=16, side_hi=256, horizon_lo=60, horizon_hi=600):
-    """One request: random size, material, ground kind and horizon."""
+def serving_job(rng, side_lo=16, side_hi=256, horizon_lo=60, horizon_hi=600, gap=0.0):
+    """One request: random size, material, ground kind and horizon; the block's bottom
+    ``gap`` above the highest point of its ground under it (0: touching it, so that a
+    horizon of 60 to 600 substeps covers ground contact)."""
     s = int(rng.integers(side_lo, side_hi + 1))
     mat = random_material(rng)
-    dom = grid_domain(s, (0.3, MIX_Y0), 0.4, mat, [Gravity([0.0, -9.8])])
     kind = int(rng.integers(0, 3))
     if kind == 0:
         ground = GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
@@ -158,6 +185,7 @@ def serving_job(rng, side_lo=16, side_hi=256, horizon_lo=60, horizon_hi=600):
         ground = sinusoid_heightfield(rng)
     else:
         ground = value_noise_voxels(rng, 128, 8, [-0.5, 0.0], [1.5, 0.05], spacing=4, floor_rows=2)
+    dom = grid_domain(s, (0.3, ground_top(ground) + gap), 0.4, mat, [Gravity([0.0, -9.8])])
     horizon = int(rng.integers(horizon_lo, horizon_hi + 1))
     return one_body_spec(dom, ground, horizon=horizon)
 
@@ -193,7 +221,7 @@ def spring_batched_config(n_scenes=4096, seed=0, side_lo=8, side_hi=128, spacing
         m = 0.01
         c = 0.02 * math.sqrt(k * m)
         width = s * spacing
-        dom = spring_lattice(s, (0.5 - 0.5 * width, MIX_Y0), spacing, [k, c, m], [Gravity([0.0, -9.8])])
+        dom = spring_lattice(s, (0.5 - 0.5 * width, SPRING_Y0), spacing, [k, c, m], [Gravity([0.0, -9.8])])
         specs.append(one_body_spec(dom, sinusoid_heightfield(rng)))
     return specs
 
@@ -221,8 +249,9 @@ def contact_pair_group(rng, side_lo=8, side_hi=32, dt_fraction=0.35):
     w1, w2 = 0.3, float(rng.uniform(0.15, 0.3))
     # Elastic and J2: an impact can invert elements, where Neo-Hookean stress is NaN (log J < 0)
     kinds = (MaterialType.ELASTIC, MaterialType.MISES)
-    lower = block(s1, (0.2, 0.0), w1, random_material(rng, kinds=kinds), friction=float(rng.uniform(0.0, 0.4)))
-    upper = block(s2, (0.2 + float(rng.uniform(-0.1, 0.2)), w1 + 0.002), w2, random_material(rng, kinds=kinds),
+    lower = block(s1, (0.2, 0.0), w1, random_material(rng, rho=CONTACT_RHO, kinds=kinds),
+                  friction=float(rng.uniform(0.0, 0.4)))
+    upper = block(s2, (0.2 + float(rng.uniform(-0.1, 0.2)), w1 + 0.002), w2, random_material(rng, rho=CONTACT_RHO, kinds=kinds),
                   initials=[InitialVel("ALL", [0.0, -float(rng.uniform(0.1, 0.5))])])
     plane = GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
     pair = ContactFlexFlex(upper, lower, ContactType.FLEXFLEX)

@@ -184,16 +184,16 @@ def test_batched_4096_sampled_scenes_bitexact():
         arena.step_graph(60, _native.FW_STEP_DIAG)
     arena.sync()
     pick = W.rng_for(99).choice(len(specs), size=48, replace=False)
-    touched = yielded = 0
+    touched = plastic = 0
     for k in pick:
         o = OracleScene(specs[k])
         o.warmup()
         o.run(1200)
         c = check_fields(lambda f: arena.read(ids[k], f), o, f"scene {k}")
         touched += int(c["contact"] > 0)
-        yielded += int(c["yielded"] > 0)
-    log(f"  sampled scenes in contact at the end: {touched}, yielding: {yielded}")
-    assert touched >= 10 and yielded >= 5, "the window should cover height-field contact and yielding"
+        plastic += int(specs[k].mat_type == 3 and o.eqps.max() > 0.0)
+    log(f"  sampled scenes in contact at the end: {touched}, J2 scenes with plastic strain: {plastic}")
+    assert touched >= 10 and plastic >= 5, "the window should cover height-field contact and yielding"
     arena.close()
 
 

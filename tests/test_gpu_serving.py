@@ -143,6 +143,7 @@ def test_served_c5_stream_matches_oracle():
     if largest not in sample:
         sample.append(largest)
     assert {j.spec.grounds[0].kind for j in sample} == {1, 2, 3}
+    touching = 0
     for j in sample:
         o = OracleScene(j.spec)
         o.warmup()
@@ -152,3 +153,5 @@ def test_served_c5_stream_matches_oracle():
         assert np.isfinite(o.x).all()
         assert np.array_equal(res[j.job_id].positions, o.x), f"job {j.job_id} positions"
         assert np.array_equal(res[j.job_id].velocities, o.v), f"job {j.job_id} velocities"
+        touching += int(o.contact_flag.any())
+    assert touching >= len(sample) // 2, "served jobs start on their grounds: most should be in contact"
