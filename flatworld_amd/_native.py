@@ -168,6 +168,7 @@ SIGNATURES = {
     "fw_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
     "fw_scene_reset_gate": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_step_graph": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]),
+    "fw_step_noalloc": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
     "fw_sync": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_arena_track_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_arena_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
@@ -419,6 +420,12 @@ class Arena:
 
     def step(self, n, dt_override=-1.0, flags=0, damping_override=-1.0):
         check(self._L.fw_step(self.h, int(n), float(dt_override), float(damping_override), int(flags)), "fw_step")
+
+    def step_noalloc(self, n, dt_override=-1.0, flags=0, damping_override=-1.0):
+        """step() through fw_step_noalloc: no allocation; the FEM-FEM contact scratch lives in
+        a fixed per-GPU array.  After adding or removing scenes or groups, call step(0) first."""
+        check(self._L.fw_step_noalloc(self.h, int(n), float(dt_override), float(damping_override), int(flags)),
+              "fw_step_noalloc")
 
     def reset_gate(self, sid):
         check(self._L.fw_scene_reset_gate(self.h, int(sid)), "fw_scene_reset_gate")

@@ -47,14 +47,24 @@ extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_
                                           const ArenaCtl* ctl, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
-                                             uint32_t flags, hipStream_t stream);
+                                             uint32_t flags, char* scr, hipStream_t stream);
+extern "C" hipError_t fw_flex_noalloc_scratch(int device, char** out);
+extern "C" hipError_t fw_launch_flex_scratch_init(const FlexDev* groups, int n_groups, char* scr, hipStream_t stream);
 
 namespace {
 
 thread_local std::string g_err;
+thread_local const char* g_err_static = nullptr;  // a message with static storage (no allocation)
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
+    g_err_static = nullptr;
+    return code;
+}
+
+// fail() for the paths that must not allocate: the message has static storage.
+int fail_static(int code, const char* msg) {
+    g_err_static = msg;
     return code;
 }
 
@@ -642,6 +652,7 @@ struct fw_arena {
     FlexDev* d_flex = nullptr;
     int32_t* d_flex_items = nullptr;  // first global work item of each table row (+ the total)
     int flex_cap = 0, n_flex_live = 0, n_flex_items = 0;
+    int64_t flex_scratch_bytes = 0;  // compact per-substep scratch of the live groups (fw_step_noalloc)
     bool flex_dirty = false;
     // one-entry tiling cache: fw_scene_footprint followed by fw_scene_add (or a retry of
     // an add that found the arena full) reuses the tiling instead of rebuilding it
@@ -826,6 +837,12 @@ int ensure_flex(fw_arena* a) {
     }
     std::vector<int32_t> item_off(n + 1, 0);
     for (int r = 0; r < n; ++r) item_off[r + 1] = item_off[r] + rows[r].n_items;
+    int64_t scr = 0;  // the compact per-substep scratch layout of fw_step_noalloc
+    for (int r = 0; r < n; ++r) {
+        rows[r].scr_off = scr;
+        scr += flex_scratch_layout(rows[r].n_items, rows[r].n_slots).bytes;
+    }
+    a->flex_scratch_bytes = scr;
     int rc = upload(a, a->d_flex, rows.data(), (int64_t)sizeof(FlexDev) * n);
     if (rc) return rc;
     rc = upload(a, a->d_flex_items, item_off.data(), (int64_t)sizeof(int32_t) * (n + 1));
@@ -1157,7 +1174,7 @@ int tuning_of(const fw_tuning* tuning, fw_tuning* out) {
 
 extern "C" {
 
-const char* fw_last_error(void) { return g_err.c_str(); }
+const char* fw_last_error(void) { return g_err_static ? g_err_static : g_err.c_str(); }
 
 int32_t fw_abi_version(void) { return FW_ABI_VERSION; }
 
@@ -1407,10 +1424,10 @@ int32_t fw_arena_busy(fw_arena* a, double* busy_ms, int64_t* calls) {
 // kind present (triangle tiles, then spring tiles), each covering its kind's tiles
 // as the control block counts them at run time.
 static hipError_t launch_all(fw_arena* a, int k, float dt_override, float damping_override, uint32_t f, int grid_tri,
-                      int grid_spr) {
+                      int grid_spr, char* flex_scr = nullptr) {
     if (a->n_flex_live > 0) {  // FEM-FEM contact lists before the substep that adds them
         hipError_t e = fw_launch_flex_contact(a->d_flex, a->n_flex_live, a->d_flex_items, a->n_flex_items, a->d_scenes,
-                                              a->d_ctl, k, f, a->stream);
+                                              a->d_ctl, k, f, flex_scr, a->stream);
         if (e != hipSuccess) return e;
     }
     if (grid_tri > 0) {
@@ -1458,8 +1475,11 @@ static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_ov
 
 int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
     if (!a || n < 0) return fail(FW_E_INVALID, "bad arguments");
-    if (n == 0) return FW_OK;
     HIPCHK(hipSetDevice(a->device));
+    if (n == 0) {  // no substep: only upload the tables that scene / group changes left pending
+        int rc = ensure_tiles(a);
+        return rc ? rc : ensure_flex(a);
+    }
     hipEvent_t e0 = nullptr;
     int rc = ensure_tiles(a);  // table uploads go ahead of the busy bracket
     if (rc) return rc;
@@ -1468,6 +1488,53 @@ int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_overrid
     rc = step_impl(a, n, dt_override, damping_override, flags);
     if (rc) return rc;
     return busy_end(a, e0);
+}
+
+#define HIPCHK_NOALLOC(expr)                                                \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) return fail_static(FW_E_HIP, hipGetErrorString(e_)); \
+    } while (0)
+
+int32_t fw_step_noalloc(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
+    if (!a || n < 0) return fail_static(FW_E_INVALID, "fw_step_noalloc: bad arguments");
+    if (n == 0) return FW_OK;
+    if (a->tiles_dirty || a->flex_dirty)
+        return fail_static(FW_E_INVALID, "fw_step_noalloc: scenes or groups changed since the last fw_step; call "
+                                         "fw_step(a, 0, ...) once to upload the tables");
+    if ((flags & FW_STEP_PER_SCENE) && a->n_flex_live > 0)
+        return fail_static(FW_E_UNSUPPORTED, "FW_STEP_PER_SCENE with FEM-FEM contact groups resident");
+    HIPCHK_NOALLOC(hipSetDevice(a->device));
+    char* scr = nullptr;
+    if (a->n_flex_live > 0) {
+        if (a->flex_scratch_bytes > (int64_t)FW_NOALLOC_SCRATCH_BYTES)
+            return fail_static(FW_E_CAPACITY, "fw_step_noalloc: the contact groups need more per-substep scratch "
+                                              "than FW_NOALLOC_SCRATCH_BYTES");
+        HIPCHK_NOALLOC(fw_flex_noalloc_scratch(a->device, &scr));
+        HIPCHK_NOALLOC(fw_launch_flex_scratch_init(a->d_flex, a->n_flex_live, scr, a->stream));
+    }
+    const int nt = (int)a->tile_table.size();
+    const int n_tri = a->n_tri_tiles, n_spr = nt - n_tri;
+    for (int k = 0; k < n; ++k) {
+        uint32_t f = flags & ~FW_STEP_PER_SCENE;
+        if (k != n - 1) f &= ~FW_STEP_DIAG;
+        if (nt <= 0) continue;
+        if (flags & FW_STEP_PER_SCENE) {
+            for (auto& sp : a->scenes) {
+                if (!sp || !sp->live) continue;
+                const int cnt = (int)sp->tiles.size();
+                HIPCHK_NOALLOC(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, sp->tile_base, cnt, dt_override,
+                                                 damping_override, f, launch_grid(a, cnt), a->tune.block,
+                                                 a->tune.tile_slots, sp->mat_type == FW_MAT_SPRING ? 1 : 0, a->stream));
+            }
+        } else {
+            HIPCHK_NOALLOC(launch_all(a, k, dt_override, damping_override, f, n_tri ? launch_grid(a, n_tri) : 0,
+                                      n_spr ? launch_grid(a, n_spr) : 0, scr));
+        }
+    }
+    HIPCHK_NOALLOC(fw_launch_advance_clock(a->d_ctl, n, a->stream));
+    a->clock += n;
+    return FW_OK;
 }
 
 int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile_slots, int64_t* stats) {

@@ -22,6 +22,9 @@
 // then sorted.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
+#include "flatworld_fem.h"
 #include "fw_types.h"
 
 namespace fw {
@@ -42,6 +45,28 @@ __device__ __forceinline__ int mod3_(int k) {
 }
 
 __device__ __forceinline__ void phase_barrier() { __syncthreads(); }
+
+// The per-substep scratch of a group's contact pass: the group's own arrays (scr
+// null), or its part of the compact layout at scr (fw_step_noalloc).
+struct FlexScratch {
+    int2* res_i;
+    float4* res_f;
+    int32_t* slot_start;
+    int32_t* slot_cur;
+    int32_t* ent_key;
+    float2* ent_f;
+    int32_t* list_total;
+};
+
+__device__ __forceinline__ FlexScratch flex_scratch(const FlexDev& F, char* scr) {
+    if (!scr) return {F.res_i, F.res_f, F.slot_start, F.slot_cur, F.ent_key, F.ent_f, F.list_total};
+    const FlexScratchLayout L = flex_scratch_layout(F.n_items, F.n_slots);
+    char* p = scr + F.scr_off;
+    return {reinterpret_cast<int2*>(p + L.res_i),        reinterpret_cast<float4*>(p + L.res_f),
+            reinterpret_cast<int32_t*>(p + L.slot_start), reinterpret_cast<int32_t*>(p + L.slot_cur),
+            reinterpret_cast<int32_t*>(p + L.ent_key),    reinterpret_cast<float2*>(p + L.ent_f),
+            reinterpret_cast<int32_t*>(p + L.list_total)};
+}
 
 // pointToEdgeContact (flatworld/contact_detection.py:15-61); returns is_inside.
 __device__ __forceinline__ bool point_edge(float px, float py, float ax, float ay, float bx, float by, float& pen,
@@ -489,7 +514,8 @@ __device__ __forceinline__ int group_local(const FlexDev& F, const SceneDev* sce
 // contributions per slot.
 __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev* groups, const int32_t* item_off,
                                                                    int n_groups, const SceneDev* scenes,
-                                                                   const ArenaCtl* ctl, int k, uint32_t flags) {
+                                                                   const ArenaCtl* ctl, int k, uint32_t flags,
+                                                                   char* scr) {
     const int gw = (int)((blockIdx.x * FLEX_BLOCK + threadIdx.x) / FLEX_LPI);
     const int sub = threadIdx.x % FLEX_LPI;
     if (gw >= item_off[n_groups]) return;
@@ -503,6 +529,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
     const int w = gw - item_off[lo];
     const int local = group_local(F, scenes, ctl, k, flags);
     if (local < 0) return;
+    const FlexScratch X = flex_scratch(F, scr);
     const int rd = (local + 1) & 1;
     const int ar3 = mod3_(local);
     const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
@@ -524,14 +551,14 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             const float2 n = tie_normal(a, c);
             const float k = F.item_penalty[w];
             const float gx = cp.x + wg.z * n.x, gy = cp.y + wg.z * n.y;
-            F.res_f[w] = make_float4(k * (gx - x.x), k * (gy - x.y), wg.x, wg.y);
+            X.res_f[w] = make_float4(k * (gx - x.x), k * (gy - x.y), wg.x, wg.y);
             const int2 ss = F.seg_slot[ts];
-            atomicAdd(&F.slot_start[F.item_slot[w]], 1);
-            atomicAdd(&F.slot_start[ss.x], 1);
-            atomicAdd(&F.slot_start[ss.y], 1);
+            atomicAdd(&X.slot_start[F.item_slot[w]], 1);
+            atomicAdd(&X.slot_start[ss.x], 1);
+            atomicAdd(&X.slot_start[ss.y], 1);
             ri = make_int2(1, ts);
         }
-        F.res_i[w] = ri;
+        X.res_i[w] = ri;
         return;
     }
     const FlexGrid g = *F.grid;
@@ -649,76 +676,121 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             }
             ri = make_int2(1, b.s);
             if (sub == 0) {
-                F.res_f[w] = make_float4(tx, ty, b.w0, b.w1);
+                X.res_f[w] = make_float4(tx, ty, b.w0, b.w1);
                 const int2 ss = F.seg_slot[b.s];
-                atomicAdd(&F.slot_start[F.item_slot[w]], 1);
-                atomicAdd(&F.slot_start[ss.x], 1);
-                atomicAdd(&F.slot_start[ss.y], 1);
+                atomicAdd(&X.slot_start[F.item_slot[w]], 1);
+                atomicAdd(&X.slot_start[ss.x], 1);
+                atomicAdd(&X.slot_start[ss.y], 1);
             }
         }
     }
-    if (sub == 0) F.res_i[w] = ri;
+    if (sub == 0) X.res_i[w] = ri;
 }
 
 // The node-major contribution lists of one group (one workgroup): slot ranges by
 // a scan of the counts, entries scattered and sorted by work item, written to
 // each body's FEM-FEM block; the counts are then cleared for the next substep.
 __global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* groups, const SceneDev* scenes,
-                                                                const ArenaCtl* ctl, int k, uint32_t flags) {
+                                                                const ArenaCtl* ctl, int k, uint32_t flags,
+                                                                char* scr) {
     const FlexDev& F = groups[blockIdx.x];
     __shared__ int lds[FLEX_BLOCK];
     const int tid = threadIdx.x;
     if (group_local(F, scenes, ctl, k, flags) < 0) return;
-    const int last = *F.list_total;
-    const int total = scan_global(F.slot_start, F.slot_cur, F.n_slots, lds);
+    const FlexScratch X = flex_scratch(F, scr);
+    const int last = *X.list_total;
+    const int total = scan_global(X.slot_start, X.slot_cur, F.n_slots, lds);
     phase_barrier();
     // no entries now or in the previous substep: every node's range already reads
     // zero entries and the counts are zero
     if (total == 0 && last == 0) return;
     for (int w = tid; w < F.n_items; w += FLEX_BLOCK) {
-        const int2 ri = F.res_i[w];
+        const int2 ri = X.res_i[w];
         if (!ri.x) continue;
-        const float4 rf = F.res_f[w];
+        const float4 rf = X.res_f[w];
         const int2 ss = F.seg_slot[ri.y];
         const int sl[3] = {F.item_slot[w], ss.x, ss.y};
         const float2 val[3] = {make_float2(rf.x, rf.y), make_float2(-(rf.x * rf.z), -(rf.y * rf.z)),
                                make_float2(-(rf.x * rf.w), -(rf.y * rf.w))};
         for (int c = 0; c < 3; ++c) {
-            const int at = atomicAdd(&F.slot_cur[sl[c]], 1);
-            F.ent_key[at] = (F.n_tied > 0 && F.item_tied[w]) ? w : F.n_items + w;  // tie springs before contacts
-            F.ent_f[at] = val[c];
+            const int at = atomicAdd(&X.slot_cur[sl[c]], 1);
+            X.ent_key[at] = (F.n_tied > 0 && F.item_tied[w]) ? w : F.n_items + w;  // tie springs before contacts
+            X.ent_f[at] = val[c];
         }
     }
     phase_barrier();
     // each slot's entries in ascending work item, into its body's ranges
     for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) {
-        const int b0 = F.slot_start[sl], e = F.slot_cur[sl];
+        const int b0 = X.slot_start[sl], e = X.slot_cur[sl];
         for (int i = b0 + 1; i < e; ++i) {
-            const int kv = F.ent_key[i];
-            const float2 fv = F.ent_f[i];
+            const int kv = X.ent_key[i];
+            const float2 fv = X.ent_f[i];
             int j = i - 1;
-            while (j >= b0 && F.ent_key[j] > kv) {
-                F.ent_key[j + 1] = F.ent_key[j];
-                F.ent_f[j + 1] = F.ent_f[j];
+            while (j >= b0 && X.ent_key[j] > kv) {
+                X.ent_key[j + 1] = X.ent_key[j];
+                X.ent_f[j + 1] = X.ent_f[j];
                 --j;
             }
-            F.ent_key[j + 1] = kv;
-            F.ent_f[j + 1] = fv;
+            X.ent_key[j + 1] = kv;
+            X.ent_f[j + 1] = fv;
         }
         const int b = F.slot_body[sl];
-        const int base = F.slot_start[F.body_slot_off[b]];
+        const int base = X.slot_start[F.body_slot_off[b]];
         int2* rng = reinterpret_cast<int2*>(F.body_flex[b]);
         float2* ent = reinterpret_cast<float2*>(rng + F.body_nn[b]);
         rng[F.slot_node[sl]] = make_int2(b0 - base, e - b0);
-        for (int i = b0; i < e; ++i) ent[i - base] = F.ent_f[i];
+        for (int i = b0; i < e; ++i) ent[i - base] = X.ent_f[i];
     }
     phase_barrier();
-    for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) F.slot_start[sl] = 0;
-    if (tid == 0) *F.list_total = total;
+    for (int sl = tid; sl < F.n_slots; sl += FLEX_BLOCK) X.slot_start[sl] = 0;
+    if (tid == 0) *X.list_total = total;
+}
+
+// Start of a fw_step_noalloc call: the compact scratch holds whatever the last call
+// on this GPU left, so every group's slot counts are cleared and its entry total set
+// to 1, which makes the first lists pass rewrite every node's range; the group's own
+// entry total is set to 1 as well, so that a later fw_step rewrites the ranges this
+// call leaves behind.
+__global__ __launch_bounds__(FLEX_BLOCK) void flex_scratch_init_kernel(const FlexDev* groups, char* scr) {
+    const FlexDev& F = groups[blockIdx.x];
+    const FlexScratch X = flex_scratch(F, scr);
+    for (int sl = threadIdx.x; sl < F.n_slots; sl += FLEX_BLOCK) X.slot_start[sl] = 0;
+    if (threadIdx.x == 0) {
+        *X.list_total = 1;
+        *F.list_total = 1;
+    }
 }
 
 }  // namespace
+
+// Scratch of fw_step_noalloc: one fixed region per GPU, shared by every call on it.
+static __device__ __align__(256) char scratch[FW_NOALLOC_SCRATCH_BYTES];
+
 }  // namespace fw
+
+// Device address of the no-allocation scratch on `device` (the current device):
+// looked up once per device with hipGetSymbolAddress and kept; the array's host-side
+// name is only a stand-in and is never passed to a kernel.
+extern "C" hipError_t fw_flex_noalloc_scratch(int device, char** out) {
+    static std::mutex mu;
+    static char* ptrs[64] = {};
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(fw::scratch));
+        if (e != hipSuccess) return e;
+        ptrs[device] = static_cast<char*>(p);
+    }
+    *out = ptrs[device];
+    return hipSuccess;
+}
+
+extern "C" hipError_t fw_launch_flex_scratch_init(const FlexDev* groups, int n_groups, char* scr, hipStream_t stream) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fw::flex_scratch_init_kernel, dim3(n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups, scr);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
                                           const ArenaCtl* ctl, int row, hipStream_t stream) {
@@ -738,17 +810,17 @@ extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_
 
 extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
-                                             uint32_t flags, hipStream_t stream) {
+                                             uint32_t flags, char* scr, hipStream_t stream) {
     if (n_groups <= 0) return hipSuccess;
     if (total_items > 0) {
         const int blocks = (int)(((int64_t)total_items * fw::FLEX_LPI + fw::FLEX_BLOCK - 1) / fw::FLEX_BLOCK);
         hipLaunchKernelGGL(fw::flex_items_kernel, dim3(blocks), dim3(fw::FLEX_BLOCK), 0, stream, groups, item_off,
-                           n_groups, scenes, ctl, k, flags);
+                           n_groups, scenes, ctl, k, flags, scr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(fw::flex_lists_kernel, dim3(n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups, scenes, ctl, k,
-                       flags);
+                       flags, scr);
     return hipGetLastError();
 }
 

@@ -98,7 +98,39 @@ struct FlexDev {
     int32_t* ent_key;              // [3W] work item of each entry
     float2* ent_f;                 // [3W]
     int32_t* list_total;           // [1] entries placed by the previous substep's lists pass
+    // offset of this group's per-substep scratch in the compact layout fw_step_noalloc
+    // uses (the live groups' scratch one after another, flex_scratch_bytes each)
+    int64_t scr_off;
 };
+
+// The per-substep scratch of one group's contact pass, laid out compactly for a
+// caller-independent region: (force, weights) and (hit, segment) per work item,
+// up to three (work item, force) entries per work item, a count / start and a
+// cursor per slot, and the previous substep's entry total.  Offsets in bytes.
+struct FlexScratchLayout {
+    int64_t res_f, res_i, ent_f, ent_key, slot_start, slot_cur, list_total, bytes;
+};
+__host__ __device__ inline int64_t fw_align16(int64_t b) { return (b + 15) & ~(int64_t)15; }
+__host__ __device__ inline FlexScratchLayout flex_scratch_layout(int64_t n_items, int64_t n_slots) {
+    FlexScratchLayout L;
+    int64_t o = 0;
+    L.res_f = o;
+    o += fw_align16(16 * n_items);
+    L.res_i = o;
+    o += fw_align16(8 * n_items);
+    L.ent_f = o;
+    o += fw_align16(24 * n_items);
+    L.ent_key = o;
+    o += fw_align16(12 * n_items);
+    L.slot_start = o;
+    o += fw_align16(4 * n_slots);
+    L.slot_cur = o;
+    o += fw_align16(4 * n_slots);
+    L.list_total = o;
+    o += 16;
+    L.bytes = fw_align16(o);
+    return L;
+}
 
 // The scene fields every tile reads (material, dt, gate state), kept together at
 // the front of the scene entry.  Spring scenes keep the stiffness in E and the

@@ -290,13 +290,31 @@ int32_t fw_flex_init_ties(fw_arena* a, int32_t group_id);
 int32_t fw_flex_remove(fw_arena* a, int32_t group_id);
 int32_t fw_flex_read(fw_arena* a, int32_t group_id, int32_t field, void* out, int64_t out_bytes);
 
-/* Enqueue n substeps of every resident scene on the arena stream (asynchronous).
+/* Enqueue n substeps of every resident scene on the arena stream (asynchronous);
+ * n == 0 only uploads the device tables that adding or removing scenes and groups
+ * left pending.
  * dt_override >= 0 replaces every scene's dt and damping_override >= 0 every
  * scene's damping for these substeps (FemSpringManager.substep(dt, damping),
  * flatworld/femspringmanager.py:938-939).  With
  * FW_STEP_PER_SCENE each substep is issued as one launch per scene instead of
  * one launch for the whole arena; results are identical. */
 int32_t fw_step(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags);
+/* fw_step for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation).  No reference counterpart: it
+ * exists for those callers only.  It calls no hipMalloc / hipFree and no allocating
+ * library routine, and keeps the FEM-FEM contact pass's per-substep scratch (per
+ * work item results, per slot counts and cursors, the entry lists) in one fixed
+ * device array of FW_NOALLOC_SCRATCH_BYTES per GPU instead of the groups' arena
+ * blocks.  Results are identical to fw_step's.  Limits: the live groups' scratch
+ * must fit that array (FW_E_CAPACITY otherwise), and after scenes or groups are
+ * added or removed, fw_step(a, 0, ...) must run once first to upload the tables
+ * (FW_E_INVALID otherwise).  Every call on one GPU shares that GPU's array: calls on
+ * different arenas of one GPU that overlap in time (other host threads, other
+ * streams) race, and must be serialised by the caller.  Its calls are not counted
+ * by fw_arena_busy.  Error messages have static storage. */
+#define FW_NOALLOC_SCRATCH_BYTES (32 << 20)
+int32_t fw_step_noalloc(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags);
+
 /* n substeps captured once as a HIP graph (per n and flags) and replayed. */
 int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags);
 int32_t fw_sync(fw_arena* a);
