@@ -29,6 +29,7 @@ FIELD_AREA, FIELD_JINV, FIELD_YIELD, FIELD_CONTACT, FIELD_AABB = 10, 11, 12, 13,
 FLEX_CACHE, FLEX_GRID, FLEX_CELL_START, FLEX_CELL_END, FLEX_SORTED = 1, 2, 3, 4, 5
 FLEX_TIE_SEG, FLEX_TIE_STATE = 6, 7
 GROUND_FLEX = 4
+GROUND_RIGID = 5
 
 # (per_node, width, dtype)
 FIELD_SHAPES = {
@@ -72,6 +73,15 @@ class GroundDesc(ctypes.Structure):
         ("vox_dx", ctypes.c_float),
         ("vox_dz", ctypes.c_float),
         ("flex_cap", ctypes.c_int32),
+        ("rigid_type", ctypes.c_int32),
+        ("rigid_center", ctypes.c_float * 2),
+        ("rigid_rot", ctypes.c_float * 4),
+        ("rigid_prim", ctypes.c_float * 2),
+        ("rigid_radius", ctypes.c_float),
+        ("rigid_vel", ctypes.c_float * 2),
+        ("rigid_omega", ctypes.c_float),
+        ("rigid_friction", ctypes.c_float),
+        ("rigid_tied", ctypes.c_int32),
     ]
 
 
@@ -182,6 +192,7 @@ SIGNATURES = {
     "fw_scene_add_prepared": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "fw_prepared_free": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_scene_mass_scaling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float]),
+    "fw_scene_set_rigid": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GroundDesc)]),
     "fw_flex_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(FlexDesc), ctypes.POINTER(ctypes.c_int32)]),
     "fw_flex_rebuild": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_flex_init_ties": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
@@ -237,6 +248,19 @@ def _ptr(a, t):
     return a.ctypes.data_as(t)
 
 
+def fill_rigid(d, g):
+    """The rigid_* fields of a GroundDesc from a rigid GroundSpec (kind 5)."""
+    d.rigid_type = int(g.rigid_type)
+    d.rigid_center[:] = [float(v) for v in np.asarray(g.rigid_center, np.float32)]
+    d.rigid_rot[:] = [float(v) for v in np.asarray(g.rigid_rot, np.float32).reshape(4)]
+    d.rigid_prim[:] = [float(v) for v in np.asarray(g.rigid_prim, np.float32)]
+    d.rigid_radius = float(g.rigid_radius)
+    d.rigid_vel[:] = [float(v) for v in np.asarray(g.rigid_vel, np.float32)]
+    d.rigid_omega = float(g.rigid_omega)
+    d.rigid_friction = float(g.rigid_friction)
+    d.rigid_tied = int(g.rigid_tied)
+
+
 def scene_desc(spec, track_disp=True):
     """Build the C description of a SceneSpec; returns (desc, keepalive list)."""
     keep = []
@@ -273,6 +297,8 @@ def scene_desc(spec, track_disp=True):
             d.vox_dz = float(g.vox_dz)
         elif g.kind == GROUND_FLEX:
             d.flex_cap = int(g.flex_cap)
+        elif g.kind == GROUND_RIGID:
+            fill_rigid(d, g)
         gs.append(d)
     garr = (GroundDesc * max(len(gs), 1))(*gs)
     keep.append(garr)
@@ -433,6 +459,17 @@ class Arena:
     def mass_scaling(self, sid, dt_target):
         """Selective mass scaling of one scene at its current positions (asynchronous)."""
         check(self._L.fw_scene_mass_scaling(self.h, int(sid), float(np.float32(dt_target))), "fw_scene_mass_scaling")
+
+    def set_rigid(self, sid, ground_index, g):
+        """Move rigid record ``ground_index`` of scene ``sid`` to the pose of GroundSpec ``g``
+        (stream-ordered: later substeps see it)."""
+        d = GroundDesc()
+        d.kind = int(g.kind)
+        d.penalty = float(g.penalty)
+        d.aabb_lo[:] = [float(v) for v in g.aabb_lo]
+        d.aabb_hi[:] = [float(v) for v in g.aabb_hi]
+        fill_rigid(d, g)
+        check(self._L.fw_scene_set_rigid(self.h, int(sid), int(ground_index), ctypes.byref(d)), "fw_scene_set_rigid")
 
     def step_graph(self, n, flags=0):
         check(self._L.fw_step_graph(self.h, int(n), int(flags)), "fw_step_graph")

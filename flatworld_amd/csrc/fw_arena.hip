@@ -496,6 +496,42 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
     return L;
 }
 
+int validate_rigid(const fw_ground_desc& gd) {
+    if (gd.rigid_type != FW_RIGID_BALL && gd.rigid_type != FW_RIGID_BOX && gd.rigid_type != FW_RIGID_CAPSULE)
+        return fail(FW_E_INVALID, "rigid record: type must be FW_RIGID_BALL, FW_RIGID_BOX or FW_RIGID_CAPSULE");
+    if (gd.rigid_type == FW_RIGID_CAPSULE && gd.rigid_prim[0] == 0.0f && gd.rigid_prim[1] == 0.0f)
+        return fail(FW_E_INVALID, "rigid record: a capsule needs a nonzero axis");
+    if (gd.rigid_tied != 0 && gd.rigid_tied != 1) return fail(FW_E_INVALID, "rigid record: rigid_tied must be 0 or 1");
+    return FW_OK;
+}
+
+// The device fields of a rigid record (GroundDev layout in fw_types.h).
+void pack_rigid(GroundDev& G, const fw_ground_desc& gd) {
+    G.a[0] = gd.rigid_center[0];
+    G.a[1] = gd.rigid_center[1];
+    G.a[2] = gd.rigid_prim[0];
+    G.a[3] = gd.rigid_prim[1];
+    G.nx = gd.rigid_type;
+    G.ny = gd.rigid_tied;
+    G.hf_dxw = gd.rigid_radius;
+    for (int k = 0; k < 4; ++k) G.b[k] = gd.rigid_rot[k];
+    G.b[4] = gd.rigid_vel[0];
+    G.b[5] = gd.rigid_vel[1];
+    G.b[6] = gd.rigid_omega;
+    G.b[7] = gd.rigid_friction;
+}
+
+// The ground AABB the gate reads, raw and padded by 0.1 x extent.
+void pack_aabb(GroundDev& G, const fw_ground_desc& gd) {
+    for (int k = 0; k < 2; ++k) {
+        G.lo[k] = gd.aabb_lo[k];
+        G.hi[k] = gd.aabb_hi[k];
+        float b = 0.1f * (gd.aabb_hi[k] - gd.aabb_lo[k]);
+        G.plo[k] = gd.aabb_lo[k] - b;
+        G.phi[k] = gd.aabb_hi[k] + b;
+    }
+}
+
 int validate(const fw_scene_desc* d) {
     if (!d) return fail(FW_E_INVALID, "null scene description");
     if (d->n_nodes <= 0 || d->n_elems <= 0) return fail(FW_E_INVALID, "scene needs at least one node and one element");
@@ -519,8 +555,14 @@ int validate(const fw_scene_desc* d) {
             return fail(FW_E_INVALID, "voxel grid needs nx, ny >= 1 and an occupancy array");
         if (gd.kind == FW_GROUND_VOXEL && (gd.vox_nx > 65534 || gd.vox_ny > 65534))
             return fail(FW_E_INVALID, "voxel grid is limited to 65534 cells per axis");
-        if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_FLEX)
+        if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_RIGID)
             return fail(FW_E_INVALID, "unknown ground kind " + std::to_string(gd.kind));
+        if (gd.kind == FW_GROUND_RIGID) {
+            if (d->mat_type == FW_MAT_SPRING)
+                return fail(FW_E_UNSUPPORTED, "rigid contact records need a FEM scene (spring-rigid contact is not built)");
+            int rc = validate_rigid(gd);
+            if (rc) return rc;
+        }
         if (gd.kind == FW_GROUND_FLEX) {
             if (d->mat_type == FW_MAT_SPRING) return fail(FW_E_UNSUPPORTED, "FEM-FEM contact records need a FEM scene");
             if (gd.flex_cap < 0) return fail(FW_E_INVALID, "FEM-FEM record needs flex_cap >= 0");
@@ -1002,13 +1044,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
         GroundDev& G = D.ground[g];
         G.kind = gd.kind;
         G.penalty = gd.penalty;
-        for (int k = 0; k < 2; ++k) {
-            G.lo[k] = gd.aabb_lo[k];
-            G.hi[k] = gd.aabb_hi[k];
-            float b = 0.1f * (gd.aabb_hi[k] - gd.aabb_lo[k]);
-            G.plo[k] = gd.aabb_lo[k] - b;
-            G.phi[k] = gd.aabb_hi[k] + b;
-        }
+        pack_aabb(G, gd);
         if (gd.kind == FW_GROUND_PLANE) {
             G.a[0] = gd.point[0];
             G.a[1] = gd.point[1];
@@ -1029,6 +1065,8 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
         } else if (gd.kind == FW_GROUND_FLEX) {
             G.nx = N;             // the entries follow the per-node ranges
             G.ny = gd.flex_cap;
+        } else if (gd.kind == FW_GROUND_RIGID) {
+            pack_rigid(G, gd);
         } else {
             G.a[0] = gd.vox_lb[0];
             G.a[1] = gd.vox_lb[1];
@@ -1337,6 +1375,28 @@ int32_t fw_scene_remove(fw_arena* a, int32_t id) {
     a->scenes[id].reset();
     a->free_ids.push_back(id);
     a->tiles_dirty = true;
+    return FW_OK;
+}
+
+int32_t fw_scene_set_rigid(fw_arena* a, int32_t id, int32_t gi, const fw_ground_desc* g) {
+    if (!a || !g) return fail(FW_E_INVALID, "null argument");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    if (gi < 0 || gi >= s->dev.hot.n_grounds || s->dev.ground[gi].kind != FW_GROUND_RIGID)
+        return fail(FW_E_INVALID, "ground " + std::to_string(gi) + " of scene " + std::to_string(id) +
+                                      " is not a rigid record");
+    if (g->kind != FW_GROUND_RIGID) return fail(FW_E_INVALID, "descriptor kind must be FW_GROUND_RIGID");
+    int rc = validate_rigid(*g);
+    if (rc) return rc;
+    GroundDev G = s->dev.ground[gi];
+    if (g->rigid_type != G.nx || g->rigid_tied != G.ny)
+        return fail(FW_E_INVALID, "a rigid record keeps its type and tied flag");
+    pack_aabb(G, *g);
+    pack_rigid(G, *g);
+    HIPCHK(hipSetDevice(a->device));
+    rc = upload(a, &a->d_scenes[id].ground[gi], &G, (int64_t)sizeof(GroundDev));
+    if (rc) return rc;
+    s->dev.ground[gi] = G;
     return FW_OK;
 }
 

@@ -203,9 +203,130 @@ __device__ __forceinline__ float voxel_signed(const GroundDev& g, float px, floa
 
 // One ground on one boundary node; adds into (fx, fy) only when it pushes.
 // Plane flatworld/mixedcontact.py:660-684, height field 1566-1602, voxel 1605-1643.
+// Warp's vec2 helpers as the rigid kernels use them: dot and length as a
+// zero-initialised sum of products.
+__device__ __forceinline__ float wdot(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
+__device__ __forceinline__ float wlen(float x, float y) { return sqrtf(wdot(x, y, x, y)); }
+__device__ __forceinline__ float wmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float wmin(float a, float b) { return a < b ? a : b; }
+
+// detectPointToPrimitive (flatworld/contact_detection.py:147-191) with sphereSDFQuery,
+// obbSDFQuery and capsuleSDFQuery / calMinDisNode2Segment (flatworld/utils.py:34-145):
+// penetration, normal and (when penetrating) the contact point.
+__device__ __forceinline__ float rigid_detect(const GroundDev& g, float px, float py, float& n0, float& n1,
+                                              float& cx, float& cy) {
+    const float* r = g.b;  // rotation, row-major
+    float pen = 1e9f;
+    n0 = 0.0f;
+    n1 = 0.0f;
+    cx = px;
+    cy = py;
+    if (g.nx == 1) {  // ball
+        const float dx = px - g.a[0], dy = py - g.a[1];
+        const float nl = wlen(dx, dy);
+        pen = nl - g.hf_dxw;
+        const float d = wmax(nl, 1e-9f);
+        n0 = dx / d;
+        n1 = dy / d;
+    } else if (g.nx == 2) {  // box, full extents a[2..3]
+        float best = -1e9f, m0 = 0.0f, m1 = 0.0f;
+        const float h0 = 0.5f * g.a[2], h1 = 0.5f * g.a[3];
+        const float q0 = px - g.a[0], q1 = py - g.a[1];
+        const float l0 = r[0] * q0 + r[2] * q1;  // transpose(rot) @ q
+        const float l1 = r[1] * q0 + r[3] * q1;
+        const float sd0 = l0 - h0, sd1 = -l0 - h0, sd2 = l1 - h1, sd3 = -l1 - h1;
+        if (sd0 > best) { best = sd0; m0 = 1.0f; m1 = 0.0f; }
+        if (sd1 > best) { best = sd1; m0 = -1.0f; m1 = 0.0f; }
+        if (sd2 > best) { best = sd2; m0 = 0.0f; m1 = 1.0f; }
+        if (sd3 > best) { best = sd3; m0 = 0.0f; m1 = -1.0f; }
+        if (sd0 > 0.0f || sd1 > 0.0f || sd2 > 0.0f || sd3 > 0.0f) {
+            float kx = l0, ky = l1;
+            if (l0 < -h0) kx = -h0;
+            else if (l0 > h0) kx = h0;
+            if (l1 < -h1) ky = -h1;
+            else if (l1 > h1) ky = h1;
+            const float dx = l0 - kx, dy = l1 - ky;
+            const float dist = wlen(dx, dy);
+            if (dist > 1e-9f) {
+                m0 = dx / dist;
+                m1 = dy / dist;
+            }
+            best = dist;
+        }
+        n0 = r[0] * m0 + r[1] * m1;
+        n1 = r[2] * m0 + r[3] * m1;
+        pen = best;
+    } else if (g.nx == 8) {  // capsule: axis from lc = centre + rot @ a[2..3] to 2 centre - lc
+        const float lcx = g.a[0] + (r[0] * g.a[2] + r[1] * g.a[3]);
+        const float lcy = g.a[1] + (r[2] * g.a[2] + r[3] * g.a[3]);
+        const float qx = 2.0f * g.a[0] - lcx, qy = 2.0f * g.a[1] - lcy;
+        const float apx = px - lcx, apy = py - lcy;
+        const float abx = qx - lcx, aby = qy - lcy;
+        const float ab_sq = wdot(abx, aby, abx, aby);
+        float t = wdot(apx, apy, abx, aby) / ab_sq;
+        t = wmax(0.0f, wmin(1.0f, t));
+        const float clx = lcx + t * abx, cly = lcy + t * aby;
+        pen = wlen(px - clx, py - cly) - g.hf_dxw;
+        const float vx = px - clx, vy = py - cly;
+        const float d = wmax(wlen(vx, vy), 1e-9f);
+        n0 = vx / d;
+        n1 = vy / d;
+    } else {
+        return pen;
+    }
+    if (pen < 0.0f) {
+        cx = px - n0 * pen;
+        cy = py - n1 * pen;
+    }
+    return pen;
+}
+
+// _batched_rigid_prim_kernel (flatworld/mixedcontact.py:1025-1117), the node's part:
+// nothing outside the rigid's AABB; a tied node pulled to its contact point within the
+// margin (half the AABB diagonal); otherwise the penalty force k (1e-4 - pen) n where
+// pen < 1e-4, with Coulomb friction against the rigid's surface velocity.
+__device__ __forceinline__ bool rigid_force(const GroundDev& g, float px, float py, float vx, float vy, float& fx,
+                                            float& fy) {
+    if (wmin(px - g.lo[0], py - g.lo[1]) < 0.0f || wmin(g.hi[0] - px, g.hi[1] - py) < 0.0f) return false;
+    float n0, n1, cx, cy;
+    const float pen = rigid_detect(g, px, py, n0, n1, cx, cy);
+    const float margin = wlen(g.hi[0] - g.lo[0], g.hi[1] - g.lo[1]) * 0.5f;
+    const float k = g.penalty;
+    if (g.ny == 1 && pen < margin) {
+        fx = fx + k * (cx - px);
+        fy = fy + k * (cy - py);
+        return true;
+    }
+    if (!(pen < margin)) return false;
+    const float penetration = pen - 1e-4f;
+    if (!(penetration < 0.0f)) return false;
+    const float nf0 = ((-n0) * k) * penetration, nf1 = ((-n1) * k) * penetration;
+    float t0 = nf0, t1 = nf1;
+    const float mu = g.b[7];
+    if (mu > 1e-9f) {
+        const float rx = cx - g.a[0], ry = cy - g.a[1];
+        const float w = g.b[6];
+        const float sv0 = g.b[4] + (-w * ry), sv1 = g.b[5] + w * rx;
+        const float rv0 = vx - sv0, rv1 = vy - sv1;
+        const float vn = wdot(rv0, rv1, n0, n1);
+        const float tv0 = rv0 - vn * n0, tv1 = rv1 - vn * n1;
+        const float tlen = wlen(tv0, tv1);
+        if (tlen > 1e-9f) {
+            const float fd0 = -tv0 / tlen, fd1 = -tv1 / tlen;
+            const float nfl = wlen(nf0, nf1);
+            t0 = t0 + (fd0 * mu) * nfl;
+            t1 = t1 + (fd1 * mu) * nfl;
+        }
+    }
+    fx = fx + t0;
+    fy = fy + t1;
+    return true;
+}
+
 __device__ __forceinline__ bool ground_force(const GroundDev& g, float px, float py, float vx, float vy, float dt,
                                              float& fx, float& fy) {
     const float k = g.penalty;
+    if (g.kind == FW_GROUND_RIGID_DEV) return rigid_force(g, px, py, vx, vy, fx, fy);
     if (g.kind == 1) {
         float pen = (px - g.a[0]) * g.a[2] + (py - g.a[1]) * g.a[3];
         if (pen < 0.0f) {

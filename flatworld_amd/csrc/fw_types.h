@@ -22,9 +22,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define FW_MAX_GROUNDS_DEV 4
+#define FW_MAX_GROUNDS_DEV 8
 #define FW_MAT_SPRING_DEV 4  // FW_MAT_SPRING of the public header
 #define FW_GROUND_FLEX_DEV 4 // FW_GROUND_FLEX: the scene's FEM-FEM contact contributions
+#define FW_GROUND_RIGID_DEV 5 // FW_GROUND_RIGID: a rigid primitive at the pose the host set
 
 struct GroundDev {
     int32_t kind;
@@ -34,6 +35,9 @@ struct GroundDev {
     float a[4];            // plane: p.x p.y n.x n.y | HF: lb_x ub_x | voxel: lb.x lb.y dx dz
     int32_t nx, ny, reverse;
     float hf_dxw;          // HF: sample spacing (ub_x - lb_x) / (nx - 1), rounded once in float32 on the host
+    float b[8];            // rigid: rotation (row-major, 4), velocity (2), angular velocity, friction;
+                           // with a[0..1] the centre, a[2..3] the primary parameters, nx the
+                           // type, ny the tied flag, hf_dxw the radius, lo / hi the rigid's AABB
     const void* data;      // HF heights (float) | voxel cell extents (uint16 x4 per cell: scan end points
                            // along -x, +x, -z, +z; first = 0xFFFF when the cell is not occupied) |
                            // FEM-FEM: int2 (first entry, count) per node, then the float2 entries

@@ -60,6 +60,15 @@ class BoundaryConditionType:
     FIXED_ALL = 14
 
 
+class RigidType:
+    """Rigid primitive codes (flatworld/definitions.py:50-54)."""
+
+    BALL = 0b00001
+    BOX = 0b00010
+    CAPSULE = 0b01000
+    MESH = 0b10000
+
+
 class MaterialType:
     """Constitutive law codes (flatworld/definitions.py:80-83)."""
 
@@ -100,6 +109,7 @@ __all__ = [
     "DomainType",
     "ContactType",
     "BoundaryConditionType",
+    "RigidType",
     "MaterialType",
     "COLLISION_CATEGORY_ROBOT",
     "COLLISION_CATEGORY_GROUND",

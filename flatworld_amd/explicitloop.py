@@ -19,6 +19,14 @@ Setup follows flatworld/explicitloop.py:72-398 for the body-vs-ground subset:
   pairs are gated by the bodies' AABB overlap when the loop runs its BVH (any
   ground contact present) and always active otherwise (explicitloop.py:367-381).
 
+* FEM-rigid pairs (a FEM body against a ``flatworld_amd.rigid.RigidBodyDomain``
+  ball, box or capsule) become contact records after the FEM-FEM record and
+  before the height-field and voxel records, where the reference launches its
+  rigid kernel (flatworld/mixedcontact.py:1697-1714).  The rigid is kinematic:
+  its pose is pushed to the device before every substep and advanced at its
+  constant velocity after it; the reference's RigidManager dynamics are not part
+  of this package.
+
 ``advance`` is one fused substep launch: activation gate, ground forces,
 element forces, symplectic Euler and AABB, all on the device.
 ``advanceWithTime`` computes the reference's float64 substep sequence on the
@@ -34,7 +42,7 @@ from .femcontact import ContactFlexFlex, ContactSpringFlex, contactHelper
 from .femspringmanager import FemSpringManager
 from .flexcontact import build_flex_group
 
-_GROUND_TYPES = (DomainType.ANALYTICAL, DomainType.HEIGHTFIELD, DomainType.VOXELMAP)
+_GROUND_TYPES = (DomainType.ANALYTICAL, DomainType.HEIGHTFIELD, DomainType.VOXELMAP, DomainType.RIGID)
 _BODY_TYPES = (DomainType.FEM, DomainType.SPRINGMASS)
 
 
@@ -56,10 +64,12 @@ class ExplicitLoop:
         arena_kwargs=None,
     ):
         if len(joints) > 0:
-            raise NotImplementedError("joints act on rigid bodies, which are not part of this package")
+            raise NotImplementedError("joints act on rigid-body dynamics, which are not part of this package")
         for d in domains:
             if d.type not in _BODY_TYPES + _GROUND_TYPES:
-                raise NotImplementedError(f"domain type {d.type:#b} (rigid) is not part of this package")
+                raise NotImplementedError(f"domain type {d.type:#b} is not part of this package")
+            if d.type == DomainType.RIGID and not hasattr(d, "record"):
+                raise NotImplementedError("rigid domains are kinematic flatworld_amd.rigid.RigidBodyDomain primitives here")
         self.domains = domains
         self.d = domains[0].d
         self.numDomains = len(domains)
@@ -74,7 +84,9 @@ class ExplicitLoop:
         self._counter_py = 0
 
         self.stableTime = 1.0
-        if any(d.type == DomainType.ANALYTICAL for d in domains):
+        # a RigidManager exists with any rigid or analytical domain; its default stable time
+        # is 1e-3 (flatworld/explicitloop.py:145-160, rigidmanager.py:1780)
+        if any(d.type in (DomainType.ANALYTICAL, DomainType.RIGID) for d in domains):
             self.stableTime = min(1.0 / 1000.0, self.stableTime)
 
         # contact pairs in the reference's order (flatworld/explicitloop.py:237-279)
@@ -107,6 +119,7 @@ class ExplicitLoop:
         anal = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.ANALYTICAL]
         hf = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.HEIGHTFIELD]
         vox = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.VOXELMAP]
+        rig = [(c, p) for c, p in zip(self.contacts, self.contactPairIds) if c.domain2.type == DomainType.RIGID]
         self._unbatched_contacts = []
         for name, group in (("HeightField", hf), ("VoxelMap", vox)):
             if len({id(c.domain2) for c, _ in group}) > 1:
@@ -125,7 +138,7 @@ class ExplicitLoop:
                 "unbatched and apply no force; they lower the stable time and turn the BVH gate on"
             )
             self._unbatched_contacts += spring_fem
-        for c, _ in anal + hf + vox:
+        for c, _ in anal + rig + hf + vox:
             fem_idx = self.domains.index(c.domain1)
             grounds_of.setdefault(fem_idx, []).append((c.domain2, c))
 
@@ -141,7 +154,7 @@ class ExplicitLoop:
                     "spatial hash indexes the FEM bodies by manager position (flatworld/femspringmanager.py:455-466)"
                 )
             # the BVH runs when any ground contact exists (flatworld/explicitloop.py:367-381)
-            gated = len(anal) + len(hf) + len(vox) + len(self._unbatched_contacts) > 0
+            gated = len(anal) + len(rig) + len(hf) + len(vox) + len(self._unbatched_contacts) > 0
             if gated:
                 for b, dom in enumerate(fem_bodies):
                     if self.domains.index(dom) != b:

@@ -14,9 +14,12 @@ Host scalars only, following flatworld/femcontact.py:94-197 and 256-303:
 The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip,
 fw_flex.hip).  A FEM-spring pair exists as in the reference but applies no force
 there: ExplicitLoop files it among the unbatched contacts, which no substep runs.
-Rigid contacts are outside this package: constructing one raises
-``NotImplementedError`` instead of silently dropping it.  Pairs the reference's
-factory has no class for (spring-spring, spring-voxel) give no contact, as there.
+A FEM body against a rigid primitive (flatworld/femcontact.py:173-180) takes the
+FEM body's segment penalty ``2 K`` and its contact stable time; the rigid is a
+kinematic primitive (flatworld_amd.rigid).  Spring-rigid pairs are outside this
+package: constructing one raises ``NotImplementedError`` instead of silently
+dropping it.  Pairs the reference's factory has no class for (spring-spring,
+spring-voxel) give no contact, as there.
 """
 
 from .definitions import ContactType, DomainType
@@ -144,6 +147,18 @@ class ContactSpringFlex(ContactBase):
         self.stableTime = self.calStableTime(self.penalty, self.domain2)
 
 
+class ContactFlexRigid(ContactBase):
+    """FEM body against a rigid primitive (flatworld/femcontact.py:173-180): domain1 is the
+    FEM body, domain2 the rigid; penalty 2 K of the FEM body, its contact stable time."""
+
+    def __init__(self, domain1, domain2, type, is_tied=False):
+        super().__init__(domain1, domain2, type, is_tied)
+        if self.domain1.type == DomainType.RIGID:
+            self.domain1, self.domain2 = self.domain2, self.domain1
+        self.penalty = self._segment_penalty(self.domain1) * self.penalty
+        self.stableTime = self.calStableTime(self.penalty, self.domain1)
+
+
 def contactHelper(domain1, domain2, type):
     """Pair factory (flatworld/femcontact.py:283-303) without the rigid pairs."""
     if type == ContactType.ANALYTICALSPRING:
@@ -162,6 +177,8 @@ def contactHelper(domain1, domain2, type):
         return ContactFlexVoxelMap(domain1, domain2, type)
     if type == ContactType.FLEXFLEX:
         return ContactFlexFlex(domain1, domain2, type)
+    if type == ContactType.FLEXRIGID:
+        return ContactFlexRigid(domain1, domain2, type)
     if type in (ContactType.RIGIDANLAYTICAL, ContactType.RIGIDRIGID, ContactType.ANALYTICALANALYTICAL):
         return None
     raise NotImplementedError(f"contact type {type:#b} is not part of the explicit-FEM path")

@@ -14,7 +14,7 @@ import numpy as np
 from . import _native
 from .definitions import DomainType
 from .flexcontact import REBUILD_INTERVAL
-from .scene import add_flex_record, pack_domain
+from .scene import GROUND_RIGID, add_flex_record, pack_domain, update_rigid_spec
 
 
 class DeviceField:
@@ -147,6 +147,22 @@ class FemSpringManager:
         if flex is not None:
             bodies = [sid for sid, (_, dom) in zip(self._scene_ids, fem) if dom.type == DomainType.FEM]
             self._flex_gid = self._arena.flex_add(flex, bodies)
+        # kinematic rigid records: (scene, record index, record) per FEM-rigid pair
+        self._rigid_records = [(sid, gi, g) for sid, spec in zip(self._scene_ids, self._specs)
+                               for gi, g in enumerate(spec.grounds) if g.kind == GROUND_RIGID]
+        self._rigids = []
+        for _, _, g in self._rigid_records:
+            if all(g.rigid_domain is not r for r in self._rigids):
+                self._rigids.append(g.rigid_domain)
+
+    def sync_rigids(self):
+        """Push every rigid domain's current pose to its contact records (stream-ordered)."""
+        for sid, gi, g in self._rigid_records:
+            update_rigid_spec(g)
+            self._arena.set_rigid(sid, gi, g)
+
+    def _moving_rigids(self):
+        return [r for r in self._rigids if r.omega != 0.0 or np.any(r.velocity != 0.0)]
 
     # ------------------------------------------------------------ stepping
     def setGlobalAABB(self, aabb=None):
@@ -167,7 +183,21 @@ class FemSpringManager:
 
         With FEM-FEM contact, ``dt64`` (one float64 per substep, default ``dt``)
         advances the hash schedule: a rebuild due at a substep is enqueued before
-        it, as run_batched_contacts does (flatworld/mixedcontact.py:1672-1678)."""
+        it, as run_batched_contacts does (flatworld/mixedcontact.py:1672-1678).
+        Moving kinematic rigids are stepped one substep at a time: their pose is
+        pushed before the substep and advanced after it."""
+        moving = self._moving_rigids()
+        if not moving:
+            self._advance_block(n, dt, damping, diag, dt64)
+            return
+        seq = [float(dt)] * int(n) if dt64 is None else [float(v) for v in dt64]
+        for k in range(int(n)):
+            self.sync_rigids()
+            self._advance_block(1, dt, damping, diag and k == int(n) - 1, [seq[k]])
+            for r in moving:
+                r.advance(seq[k])
+
+    def _advance_block(self, n, dt, damping, diag=True, dt64=None):
         flags = _native.FW_STEP_DIAG if diag else 0
         dt32, damp32 = float(np.float32(dt)), float(np.float32(damping))
         n = int(n)

@@ -37,6 +37,10 @@ GROUND_VOXEL = 3
 # the scene's FEM-FEM contact contributions (FW_GROUND_FLEX): filled by its
 # contact group before each substep, placed after the plane records
 GROUND_FLEX = 4
+# a kinematic rigid primitive (FW_GROUND_RIGID): placed after the FEM-FEM record
+# and before the height-field and voxel records, where the reference launches its
+# FEM-rigid kernel (flatworld/mixedcontact.py:1697-1714)
+GROUND_RIGID = 5
 
 # material code of a spring-mass scene (FW_MAT_SPRING of include/flatworld_fem.h);
 # the reference has no material type for springs: their elements carry matE and
@@ -63,6 +67,17 @@ class GroundSpec:
     vox_dx: np.float32 = np.float32(0.0)
     vox_dz: np.float32 = np.float32(0.0)
     flex_cap: int = 0  # FEM-FEM record: contributions per substep
+    # rigid record (flatworld_amd.rigid.RigidBodyDomain.record): the pose fields
+    rigid_type: int = 0
+    rigid_center: Optional[np.ndarray] = None
+    rigid_rot: Optional[np.ndarray] = None  # float32[2, 2]
+    rigid_prim: Optional[np.ndarray] = None
+    rigid_radius: np.float32 = np.float32(0.0)
+    rigid_vel: Optional[np.ndarray] = None
+    rigid_omega: np.float32 = np.float32(0.0)
+    rigid_friction: np.float32 = np.float32(0.0)
+    rigid_tied: int = 0
+    rigid_domain: object = None  # the RigidBodyDomain the record follows
 
 
 @dataclass
@@ -188,7 +203,26 @@ def ground_spec(ground, contact):
             vox_dx=np.float32(float(ground.dx)),
             vox_dz=np.float32(float(ground.dz)),
         )
+    if ground.type == DomainType.RIGID:
+        return rigid_spec(ground, contact)
     raise ValueError(f"not a ground domain: type {ground.type}")
+
+
+def rigid_spec(rigid, contact):
+    """Contact record of one FEM-vs-rigid pair at the rigid's current pose
+    (flatworld/mixedcontact.py:1960-2043: penalty, the rigid's friction, tied flag)."""
+    g = GroundSpec(GROUND_RIGID, np.float32(float(contact.penalty)), np.zeros(2, np.float32), np.zeros(2, np.float32),
+                   rigid_friction=np.float32(float(rigid.friction)), rigid_tied=1 if getattr(contact, "tied", False) else 0,
+                   rigid_domain=rigid)
+    update_rigid_spec(g)
+    return g
+
+
+def update_rigid_spec(g):
+    """Refresh a rigid record from its domain's current pose and velocity."""
+    for k, v in g.rigid_domain.record().items():
+        setattr(g, k, v)
+    return g
 
 
 def pack_fem_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
@@ -262,7 +296,7 @@ def pack_domain(domain, grounds=(), dt=0.0, damping=0.0, horizon=0):
 
 def add_flex_record(spec, cap):
     """Insert the FEM-FEM contact record where the reference launches that kernel:
-    after the plane kernel, before the height-field and voxel kernels
+    after the plane kernel, before the rigid, height-field and voxel kernels
     (flatworld/mixedcontact.py:1654-1754)."""
     at = sum(1 for g in spec.grounds if g.kind == GROUND_PLANE)
     rec = GroundSpec(GROUND_FLEX, np.float32(0.0), np.zeros(2, np.float32), np.zeros(2, np.float32), flex_cap=int(cap))

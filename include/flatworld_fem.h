@@ -49,8 +49,8 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 4
-#define FW_MAX_GROUNDS 4
+#define FW_ABI_VERSION 5
+#define FW_MAX_GROUNDS 8
 
 /* status codes */
 #define FW_OK 0
@@ -76,6 +76,21 @@ extern "C" {
  * records.  Of its descriptor only kind and flex_cap are read: flex_cap bounds the
  * contributions per substep (own work items + 2 x work items that target this body). */
 #define FW_GROUND_FLEX 4
+/* FW_GROUND_RIGID: a rigid primitive (ball, box, capsule) at a pose the caller sets
+ * (fw_scene_set_rigid), pushing the scene's boundary nodes with the reference's
+ * FEM-rigid penalty contact (_batched_rigid_prim_kernel,
+ * flatworld/mixedcontact.py:1025-1117).  It goes where the reference launches that
+ * kernel: after the FEM-FEM record, before the height-field and voxel records.
+ * aabb_lo / aabb_hi hold the primitive's AABB at the pose (getBallBBox /
+ * getBoxBBox / getCapsuleBBox, flatworld/utils.py:235-302); the gate pads it on BVH
+ * rebuild substeps like any ground AABB.  The rigid is kinematic: its reaction
+ * impulses are not accumulated. */
+#define FW_GROUND_RIGID 5
+
+/* rigid primitive types (flatworld/definitions.py:50-54) */
+#define FW_RIGID_BALL 1
+#define FW_RIGID_BOX 2
+#define FW_RIGID_CAPSULE 8
 
 /* node flag word: bits 0..8 BC bits (flatworld/definitions.py:95-104),
  * bit 15 boundary node, bits 16..31 row of the scene's BC value table */
@@ -127,6 +142,20 @@ typedef struct {
     float vox_dx;
     float vox_dz;
     int32_t flex_cap;   /* FW_GROUND_FLEX: contribution capacity */
+    /* FW_GROUND_RIGID: FW_RIGID_* type; centre (rigidParams[i, 0]); rotation matrix,
+       row-major [[cos a, -sin a], [sin a, cos a]]; primary parameters (rigidParams[i, 1]:
+       box full extents, capsule axis end lc relative to the centre); radius (ball,
+       capsule); linear and angular velocity (friction only); the rigid's friction
+       coefficient; 1 for a tied pair */
+    int32_t rigid_type;
+    float rigid_center[2];
+    float rigid_rot[4];
+    float rigid_prim[2];
+    float rigid_radius;
+    float rigid_vel[2];
+    float rigid_omega;
+    float rigid_friction;
+    int32_t rigid_tied;
 } fw_ground_desc;
 
 /* One FEM or spring-mass body; arrays in the caller's node/element order.
@@ -262,6 +291,13 @@ int32_t fw_scene_reset_gate(fw_arena* a, int32_t scene_id);
  * nodes; every spring end node is raised to at least 4 k dt_target^2.
  * ExplicitLoop calls it right after the warm-up (flatworld/explicitloop.py:164-170). */
 int32_t fw_scene_mass_scaling(fw_arena* a, int32_t scene_id, float dt_target);
+/* Move a FW_GROUND_RIGID record of a scene to a new pose: of `g` the rigid_* fields
+ * and aabb_lo / aabb_hi are read (kind must be FW_GROUND_RIGID, and rigid_type and
+ * rigid_tied must equal the record's).  Stream-ordered: substeps launched after the
+ * call see the new pose, those launched before the old one.  Replaces the pose
+ * update the reference's RigidManager does between substeps
+ * (flatworld/rigidmanager.py:765-793, 2113-2200), for a rigid the caller moves. */
+int32_t fw_scene_set_rigid(fw_arena* a, int32_t scene_id, int32_t ground_index, const fw_ground_desc* g);
 /* Substeps completed by the scene (warm-up excluded) and whether it reached its horizon. */
 int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, int32_t* finished);
 

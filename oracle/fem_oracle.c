@@ -409,6 +409,8 @@ static float voxel_signed(const ora_ground* g, const float p[2], float n[2]) {
  * Plane: flatworld/mixedcontact.py:660-684 (no damping).
  * Height field: flatworld/mixedcontact.py:1566-1602.
  * Voxel: flatworld/mixedcontact.py:1605-1643. */
+static int32_t rigid_force(const ora_ground* g, const float p[2], const float vel[2], float force[2]);
+
 int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[2], float dt, float force[2]) {
     const float k = g->penalty;
     const float damping_coeff = 0.5f;
@@ -440,6 +442,7 @@ int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[
         }
         return 0;
     }
+    if (g->kind == ORA_RIGID) return rigid_force(g, p, vel, force);
     if (g->kind == ORA_VOXEL) {
         float n[2];
         float sd = voxel_signed(g, p, n);
@@ -459,6 +462,184 @@ int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[
         return 0;
     }
     return 0;
+}
+
+/* ---------------------------------------------------------------- rigid primitives
+ * Warp's vector helpers as the reference kernels use them: dot and length are a
+ * zero-initialised sum of products, a mat22-vec2 product is (m_i0 v_0) + (m_i1 v_1),
+ * vec / scalar and vec * scalar act per component. */
+static float rdot(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
+static float rlen(float x, float y) { return sqrtf(rdot(x, y, x, y)); }
+static float rmaxf(float a, float b) { return a > b ? a : b; }
+static float rminf(float a, float b) { return a < b ? a : b; }
+
+/* sphereSDFQuery (flatworld/utils.py:34-40) */
+static float sphere_sdf(const float p[2], const float c[2], float radius, float n[2]) {
+    const float dx = p[0] - c[0], dy = p[1] - c[1];
+    const float nlen = rlen(dx, dy);
+    const float pen = nlen - radius;
+    const float d = rmaxf(nlen, 1e-9f);
+    n[0] = dx / d;
+    n[1] = dy / d;
+    return pen;
+}
+
+/* obbSDFQuery (flatworld/utils.py:57-131): rot row-major [[r0, r1], [r2, r3]] */
+static float obb_sdf(const float p[2], const float c[2], const float ext[2], const float rot[4], float n[2]) {
+    float best = -1e9f;
+    float nl0 = 0.0f, nl1 = 0.0f;
+    const float h0 = 0.5f * ext[0], h1 = 0.5f * ext[1];
+    const float q0 = p[0] - c[0], q1 = p[1] - c[1];
+    /* transpose(rot) @ q */
+    const float l0 = rot[0] * q0 + rot[2] * q1;
+    const float l1 = rot[1] * q0 + rot[3] * q1;
+    const float sd0 = l0 - h0, sd1 = -l0 - h0, sd2 = l1 - h1, sd3 = -l1 - h1;
+    if (sd0 > best) { best = sd0; nl0 = 1.0f; nl1 = 0.0f; }
+    if (sd1 > best) { best = sd1; nl0 = -1.0f; nl1 = 0.0f; }
+    if (sd2 > best) { best = sd2; nl0 = 0.0f; nl1 = 1.0f; }
+    if (sd3 > best) { best = sd3; nl0 = 0.0f; nl1 = -1.0f; }
+    if (sd0 > 0.0f || sd1 > 0.0f || sd2 > 0.0f || sd3 > 0.0f) {
+        float cx = l0, cy = l1;
+        if (l0 < -h0) cx = -h0;
+        else if (l0 > h0) cx = h0;
+        if (l1 < -h1) cy = -h1;
+        else if (l1 > h1) cy = h1;
+        const float dx = l0 - cx, dy = l1 - cy;
+        const float dist = rlen(dx, dy);
+        if (dist > 1e-9f) {
+            nl0 = dx / dist;
+            nl1 = dy / dist;
+        }
+        best = dist;
+    }
+    n[0] = rot[0] * nl0 + rot[1] * nl1;
+    n[1] = rot[2] * nl0 + rot[3] * nl1;
+    return best;
+}
+
+/* capsuleSDFQuery with calMinDisNode2Segment (flatworld/utils.py:43-55, 133-145):
+ * segment from lc to 2 origin - lc */
+static float capsule_sdf(const float p[2], const float origin[2], const float lc[2], float radius, float n[2]) {
+    const float qx = 2.0f * origin[0] - lc[0], qy = 2.0f * origin[1] - lc[1];
+    const float apx = p[0] - lc[0], apy = p[1] - lc[1];
+    const float abx = qx - lc[0], aby = qy - lc[1];
+    const float ab_sq = rdot(abx, aby, abx, aby);
+    float t = rdot(apx, apy, abx, aby) / ab_sq;
+    t = rmaxf(0.0f, rminf(1.0f, t));
+    const float clx = lc[0] + t * abx, cly = lc[1] + t * aby;
+    const float dis = rlen(p[0] - clx, p[1] - cly);
+    const float pen = dis - radius;
+    const float nvx = p[0] - clx, nvy = p[1] - cly;
+    const float nlen = rlen(nvx, nvy);
+    const float d = rmaxf(nlen, 1e-9f);
+    n[0] = nvx / d;
+    n[1] = nvy / d;
+    return pen;
+}
+
+float ora_rigid_detect(const ora_ground* g, const float p[2], float normal[2], float cpoint[2]) {
+    float pen = 1e9f;
+    normal[0] = 0.0f;
+    normal[1] = 0.0f;
+    cpoint[0] = p[0];
+    cpoint[1] = p[1];
+    if (g->rigid_type == ORA_RIGID_BALL) {
+        pen = sphere_sdf(p, g->rigid_center, g->rigid_radius, normal);
+    } else if (g->rigid_type == ORA_RIGID_BOX) {
+        pen = obb_sdf(p, g->rigid_center, g->rigid_prim, g->rigid_rot, normal);
+    } else if (g->rigid_type == ORA_RIGID_CAPSULE) {
+        const float* r = g->rigid_rot;
+        const float lc[2] = {g->rigid_center[0] + (r[0] * g->rigid_prim[0] + r[1] * g->rigid_prim[1]),
+                             g->rigid_center[1] + (r[2] * g->rigid_prim[0] + r[3] * g->rigid_prim[1])};
+        pen = capsule_sdf(p, g->rigid_center, lc, g->rigid_radius, normal);
+    } else {
+        return pen;
+    }
+    if (pen < 0.0f) {
+        cpoint[0] = p[0] - normal[0] * pen;
+        cpoint[1] = p[1] - normal[1] * pen;
+    }
+    return pen;
+}
+
+void ora_rigid_aabb(int32_t type, const float c[2], const float r[4], const float e[2], float radius, float lb[2],
+                    float ub[2]) {
+    if (type == ORA_RIGID_BALL) {
+        lb[0] = c[0] - radius;
+        lb[1] = c[1] - radius;
+        ub[0] = c[0] + radius;
+        ub[1] = c[1] + radius;
+    } else if (type == ORA_RIGID_BOX) {
+        /* corners centre + 0.5 rot (+-e0, +-e1); the first one as -0.5 * (rot @ e) */
+        float cx[4], cy[4];
+        const float s0[4] = {1.0f, 1.0f, 1.0f, -1.0f}, s1[4] = {1.0f, -1.0f, 1.0f, 1.0f};
+        for (int k = 0; k < 4; ++k) {
+            const float e0 = s0[k] * e[0], e1 = s1[k] * e[1];
+            const float f = k == 0 ? -0.5f : 0.5f;
+            cx[k] = c[0] + f * (r[0] * e0 + r[1] * e1);
+            cy[k] = c[1] + f * (r[2] * e0 + r[3] * e1);
+        }
+        lb[0] = rminf(rminf(cx[0], cx[1]), rminf(cx[2], cx[3]));
+        lb[1] = rminf(rminf(cy[0], cy[1]), rminf(cy[2], cy[3]));
+        ub[0] = rmaxf(rmaxf(cx[0], cx[1]), rmaxf(cx[2], cx[3]));
+        ub[1] = rmaxf(rmaxf(cy[0], cy[1]), rmaxf(cy[2], cy[3]));
+    } else if (type == ORA_RIGID_CAPSULE) {
+        const float alen = rlen(e[0], e[1]);
+        const float a0 = e[0] / alen, a1 = e[1] / alen;
+        const float d0 = r[0] * a0 + r[1] * a1, d1 = r[2] * a0 + r[3] * a1;
+        const float p00 = c[0] - d0 * alen, p01 = c[1] - d1 * alen;
+        const float p10 = c[0] + d0 * alen, p11 = c[1] + d1 * alen;
+        const float perp0 = radius * sqrtf(rmaxf(0.0f, 1.0f - d0 * d0)), axial0 = radius * fabsf(d0);
+        const float perp1 = radius * sqrtf(rmaxf(0.0f, 1.0f - d1 * d1)), axial1 = radius * fabsf(d1);
+        lb[0] = (rminf(p00, p10) - perp0) - axial0;
+        lb[1] = (rminf(p01, p11) - perp1) - axial1;
+        ub[0] = (rmaxf(p00, p10) + perp0) + axial0;
+        ub[1] = (rmaxf(p01, p11) + perp1) + axial1;
+    }
+}
+
+/* _batched_rigid_prim_kernel (flatworld/mixedcontact.py:1025-1117), the FEM node's
+ * part: nothing outside the rigid's AABB; a tied node is pulled to its contact point
+ * while within the margin (half the AABB diagonal); otherwise a penalty force on
+ * penetration deeper than 1e-4, with Coulomb friction against the rigid's surface
+ * velocity.  Returns 1 when a force is applied. */
+static int32_t rigid_force(const ora_ground* g, const float p[2], const float vel[2], float force[2]) {
+    const float* lb = g->aabb_lo;
+    const float* ub = g->aabb_hi;
+    if (rminf(p[0] - lb[0], p[1] - lb[1]) < 0.0f || rminf(ub[0] - p[0], ub[1] - p[1]) < 0.0f) return 0;
+    float n[2], cp[2];
+    const float pen = ora_rigid_detect(g, p, n, cp);
+    const float margin = rlen(ub[0] - lb[0], ub[1] - lb[1]) * 0.5f;
+    const float k = g->penalty;
+    if (g->rigid_tied == 1 && pen < margin) {
+        force[0] = k * (cp[0] - p[0]);
+        force[1] = k * (cp[1] - p[1]);
+        return 1;
+    }
+    if (!(pen < margin)) return 0;
+    const float penetration = pen - 1e-4f;
+    if (!(penetration < 0.0f)) return 0;
+    const float nf0 = ((-n[0]) * k) * penetration, nf1 = ((-n[1]) * k) * penetration;
+    float t0 = nf0, t1 = nf1;
+    const float mu = g->rigid_friction;
+    if (mu > 1e-9f) {
+        const float rx = cp[0] - g->rigid_center[0], ry = cp[1] - g->rigid_center[1];
+        const float w = g->rigid_omega;
+        const float sv0 = g->rigid_vel[0] + (-w * ry), sv1 = g->rigid_vel[1] + w * rx;
+        const float rv0 = vel[0] - sv0, rv1 = vel[1] - sv1;
+        const float vn = rdot(rv0, rv1, n[0], n[1]);
+        const float tv0 = rv0 - vn * n[0], tv1 = rv1 - vn * n[1];
+        const float tlen = rlen(tv0, tv1);
+        if (tlen > 1e-9f) {
+            const float fd0 = -tv0 / tlen, fd1 = -tv1 / tlen;
+            const float nflen = rlen(nf0, nf1);
+            t0 = t0 + (fd0 * mu) * nflen;
+            t1 = t1 + (fd1 * mu) * nflen;
+        }
+    }
+    force[0] = t0;
+    force[1] = t1;
+    return 1;
 }
 
 /* Pair activation: inclusive AABB overlap of the FEM boundary AABB and the

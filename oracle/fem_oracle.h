@@ -30,7 +30,9 @@ enum { ORA_ELASTIC = 1, ORA_NEOHOOKEAN = 2, ORA_MISES = 3, ORA_SPRING = 4 };
  * (ora_flex_contacts), placed in the ground list where the reference's batched
  * FEM-FEM kernel runs: after the plane kernel, before the height-field and voxel
  * kernels (flatworld/mixedcontact.py:1654-1754).  It is never gated. */
-enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3, ORA_FLEX = 4 };
+enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3, ORA_FLEX = 4, ORA_RIGID = 5 };
+/* rigid primitive types (flatworld/definitions.py:50-55) */
+enum { ORA_RIGID_BALL = 1, ORA_RIGID_BOX = 2, ORA_RIGID_CAPSULE = 8 };
 
 typedef struct {
     int32_t kind;
@@ -56,6 +58,19 @@ typedef struct {
     /* ORA_FLEX: contributions to node n are flex_f[2j .. 2j+1], j in [flex_off[n], flex_off[n+1]) */
     const int32_t* flex_off;
     const float* flex_f;
+    /* ORA_RIGID: a rigid primitive at a given pose (aabb_lo / aabb_hi hold its AABB at
+       that pose): type, centre, rotation matrix (row-major), primary parameters (box:
+       full extents; capsule: one axis end relative to the centre), radius, linear and
+       angular velocity (friction), friction coefficient and tied flag */
+    int32_t rigid_type;
+    float rigid_center[2];
+    float rigid_rot[4];
+    float rigid_prim[2];
+    float rigid_radius;
+    float rigid_vel[2];
+    float rigid_omega;
+    float rigid_friction;
+    int32_t rigid_tied;
 } ora_ground;
 
 typedef struct {
@@ -113,6 +128,13 @@ void ora_element_force(const ora_scene* s, int32_t e, float f[6], float cauchy[3
  * with a minus sign and node 1 with a plus sign. */
 void ora_spring_force(const ora_scene* s, int32_t e, float f[2]);
 int32_t ora_ground_force(const ora_ground* g, const float p[2], const float vel[2], float dt, float force[2]);
+/* detectPointToPrimitive (flatworld/contact_detection.py:147-191): penetration,
+ * normal and contact point of point p against the rigid primitive of record g. */
+float ora_rigid_detect(const ora_ground* g, const float p[2], float normal[2], float cpoint[2]);
+/* AABB of a rigid primitive at its pose: getBallBBox / getBoxBBox / getCapsuleBBox
+ * (flatworld/utils.py:235-302). */
+void ora_rigid_aabb(int32_t type, const float center[2], const float rot[4], const float prim[2], float radius,
+                    float lb[2], float ub[2]);
 int32_t ora_gate(const float fem_lb[2], const float fem_ub[2], const ora_ground* g, int32_t counter);
 void ora_update_bbox(ora_scene* s);
 float ora_log_f32(float x);

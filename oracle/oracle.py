@@ -52,6 +52,15 @@ class OraGround(ctypes.Structure):
         ("vox_dz", ctypes.c_float),
         ("flex_off", _i32p),
         ("flex_f", _f32p),
+        ("rigid_type", ctypes.c_int32),
+        ("rigid_center", ctypes.c_float * 2),
+        ("rigid_rot", ctypes.c_float * 4),
+        ("rigid_prim", ctypes.c_float * 2),
+        ("rigid_radius", ctypes.c_float),
+        ("rigid_vel", ctypes.c_float * 2),
+        ("rigid_omega", ctypes.c_float),
+        ("rigid_friction", ctypes.c_float),
+        ("rigid_tied", ctypes.c_int32),
     ]
 
 
@@ -157,6 +166,9 @@ def lib():
         L.ora_spring_force.argtypes = [ctypes.POINTER(OraScene), ctypes.c_int32, _f32p]
         L.ora_mass_scaling.argtypes = [ctypes.POINTER(OraScene), ctypes.c_float]
         L.ora_mass_scaling.restype = ctypes.c_float
+        L.ora_rigid_detect.argtypes = [ctypes.POINTER(OraGround), _f32p, _f32p, _f32p]
+        L.ora_rigid_detect.restype = ctypes.c_float
+        L.ora_rigid_aabb.argtypes = [ctypes.c_int32, _f32p, _f32p, _f32p, ctypes.c_float, _f32p, _f32p]
         L.ora_log_f32.argtypes = [ctypes.c_float]
         L.ora_log_f32.restype = ctypes.c_float
         _scenes = ctypes.POINTER(ctypes.POINTER(OraScene))
@@ -203,8 +215,25 @@ def make_ground(gs, keep):
         g.vox_lb[:] = [float(v) for v in gs.vox_lb]
         g.vox_dx = float(gs.vox_dx)
         g.vox_dz = float(gs.vox_dz)
+    elif gs.kind == 5:
+        set_rigid(g, gs)
     # kind 4 (FEM-FEM contributions): linked to its group's lists by OracleFlexGroup
     return g
+
+
+def set_rigid(g, gs):
+    """Copy a rigid record's pose (flatworld_amd.scene.GroundSpec of kind 5) into an OraGround."""
+    g.aabb_lo[:] = [float(v) for v in gs.aabb_lo]
+    g.aabb_hi[:] = [float(v) for v in gs.aabb_hi]
+    g.rigid_type = int(gs.rigid_type)
+    g.rigid_center[:] = [float(v) for v in gs.rigid_center]
+    g.rigid_rot[:] = [float(v) for v in np.asarray(gs.rigid_rot, np.float32).reshape(4)]
+    g.rigid_prim[:] = [float(v) for v in gs.rigid_prim]
+    g.rigid_radius = float(gs.rigid_radius)
+    g.rigid_vel[:] = [float(v) for v in gs.rigid_vel]
+    g.rigid_omega = float(gs.rigid_omega)
+    g.rigid_friction = float(gs.rigid_friction)
+    g.rigid_tied = int(gs.rigid_tied)
 
 
 class OracleScene:
@@ -258,6 +287,11 @@ class OracleScene:
     @property
     def aabb(self):
         return np.array(self._s.aabb[:], dtype=np.float32)
+
+    def set_rigid(self, ground_index, gs):
+        """Move rigid record ``ground_index`` to the pose of GroundSpec ``gs``."""
+        assert self._grounds[ground_index].kind == 5
+        set_rigid(self._grounds[ground_index], gs)
 
     def warmup(self):
         """Zero-dt femStep run by setGlobalAABB (flatworld/femspringmanager.py:872-879)."""
@@ -472,6 +506,27 @@ def ground_force(gs, p, vel, dt):
     f = np.zeros(2, np.float32)
     hit = lib().ora_ground_force(ctypes.byref(g), _p(p, _f32p), _p(v, _f32p), float(np.float32(dt)), _p(f, _f32p))
     return f, bool(hit)
+
+
+def rigid_detect(gs, p):
+    """detectPointToPrimitive of the oracle: (penetration, normal, contact point)."""
+    keep = []
+    g = make_ground(gs, keep)
+    pp = np.ascontiguousarray(np.asarray(p, dtype=np.float32).reshape(2))
+    n = np.zeros(2, np.float32)
+    cp = np.zeros(2, np.float32)
+    pen = lib().ora_rigid_detect(ctypes.byref(g), _p(pp, _f32p), _p(n, _f32p), _p(cp, _f32p))
+    return np.float32(pen), n, cp
+
+
+def rigid_aabb(rtype, center, rot, prim, radius):
+    """The oracle's getBallBBox / getBoxBBox / getCapsuleBBox."""
+    arrs = [np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(-1)) for v in (center, rot, prim)]
+    lb = np.zeros(2, np.float32)
+    ub = np.zeros(2, np.float32)
+    lib().ora_rigid_aabb(int(rtype), *(_p(a, _f32p) for a in arrs), float(np.float32(radius)), _p(lb, _f32p),
+                         _p(ub, _f32p))
+    return lb, ub
 
 
 def gate(fem_lb, fem_ub, gs, counter):
