@@ -256,7 +256,7 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
     std::vector<int32_t> slot_of(E, -1), lid(N, -1), nst(N, -1);
     // scratch reused by every tile, and the outputs sized for a typical halo up front
     std::vector<int32_t> slots, hl, hn;
-    std::vector<uint8_t> rank;
+    std::vector<uint16_t> rank;
     T.rec.reserve(2 * ((size_t)E + E / 4 + 64));
     T.hel.reserve((size_t)E / 4 + 64);
     T.csr_ptr.reserve((size_t)N + ntiles);
@@ -325,13 +325,14 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         T.csr_ptr.push_back(0);
         for (int p = p0; p < p1; ++p) {
             const int n = T.node_perm[p];
-            if (inc_ptr[n + 1] - inc_ptr[n] > FW_REC_MAX_RANK + 1)
+            const int max_rank = K == 3 ? FW_REC_MAX_RANK_TRI : FW_REC_MAX_RANK;
+            if (inc_ptr[n + 1] - inc_ptr[n] > max_rank + 1)
                 return fail(FW_E_UNSUPPORTED, "node " + std::to_string(n) + " has " +
                                                   std::to_string(inc_ptr[n + 1] - inc_ptr[n]) + " element corners; the substep kernel "
-                                                  "supports at most " + std::to_string(FW_REC_MAX_RANK + 1));
+                                                  "supports at most " + std::to_string(max_rank + 1));
             for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
                 int e = inc[j] / K, c = inc[j] % K;
-                rank[K * (size_t)slot_of[e] + c] = (uint8_t)(j - inc_ptr[n]);
+                rank[K * (size_t)slot_of[e] + c] = (uint16_t)(j - inc_ptr[n]);
                 ++cnt;
             }
             if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
@@ -346,14 +347,17 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         for (int s = 0; s < nslots; ++s) {
             const int e = T.elem_perm[slots[s]];
             uint32_t w[3] = {0u, 0u, 0u};
+            uint32_t hi = 0u;  // triangles: rank bits 5..9 of each corner
             for (int c = 0; c < K; ++c) {
                 const uint32_t l = (uint32_t)lid[d->conn[K * e + c]];
                 if (l > FW_REC_ID_MASK) return fail(FW_E_INVALID, "internal tiling error: local id overflow");
-                w[c] = l | ((uint32_t)rank[K * (size_t)s + c] << FW_REC_ID_BITS);
+                const uint32_t rk = rank[K * (size_t)s + c];
+                w[c] = l | ((rk & 31u) << FW_REC_ID_BITS);
+                hi |= ((rk >> 5) & 31u) << (16 + 5 * c);
             }
             if (K == 2) std::memcpy(&w[2], d->rest_len + e, 4);
             rec[2 * s + 0] = w[0] | (w[1] << 16);
-            rec[2 * s + 1] = w[2];
+            rec[2 * s + 1] = K == 3 ? (w[2] | hi) : w[2];
         }
         td.hel_off = (int32_t)T.hel.size();
         for (int s = td.n_own; s < nslots; ++s) T.hel.push_back(slots[s]);
@@ -1643,7 +1647,7 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
                 prev_halo = eint;
             }
             const int ecaller = T.elem_perm[eint];
-            if (K == 3 && (r[1] >> 16)) return fail(FW_E_INVALID, "slot record has stray bits");
+            if (K == 3 && (r[1] >> 31)) return fail(FW_E_INVALID, "slot record has stray bits");
             if (K == 2 && std::memcmp(&r[1], d->rest_len + ecaller, 4) != 0)
                 return fail(FW_E_INVALID, "spring slot record holds the wrong rest length");
             const uint32_t words[3] = {r[0] & 0xFFFFu, r[0] >> 16, r[1] & 0xFFFFu};
@@ -1652,7 +1656,8 @@ int32_t fw_tiling_check(const fw_scene_desc* d, int32_t tile_nodes, int32_t tile
                 if (lidc >= local.size() || T.node_perm[local[lidc]] != d->conn[K * ecaller + c])
                     return fail(FW_E_INVALID, "slot record names the wrong node");
                 if ((int)lidc >= td.n_nodes) continue;  // halo node: its owner tile assembles it
-                const uint32_t pos = (uint32_t)cp[lidc] + (words[c] >> FW_REC_ID_BITS);
+                const uint32_t rk = (words[c] >> FW_REC_ID_BITS) | (K == 3 ? ((r[1] >> (16 + 5 * c)) & 31u) << 5 : 0u);
+                const uint32_t pos = (uint32_t)cp[lidc] + rk;
                 if (pos >= (uint32_t)cp[lidc + 1]) return fail(FW_E_INVALID, "corner rank beyond its node's list");
                 if (pos >= at.size() || at[pos] != -1) return fail(FW_E_INVALID, "slot destinations collide");
                 at[pos] = K * (int64_t)ecaller + c;

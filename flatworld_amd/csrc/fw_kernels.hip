@@ -57,9 +57,11 @@ __device__ __forceinline__ void unpack_rec(uint2 r, int n_own_nodes, const uint1
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         l[c] = w[c] & FW_REC_ID_MASK;
+        // rank: five bits in the corner word, five more in the upper half of .y
+        const uint32_t rk = (w[c] >> FW_REC_ID_BITS) | (((r.y >> (16 + 5 * c)) & 31u) << 5);
         // branch-free: halo corners read a valid entry and discard it
         const uint32_t base = lds_cp[min((int)l[c], n_own_nodes)];
-        q[c] = (int)l[c] < n_own_nodes ? base + (w[c] >> FW_REC_ID_BITS) : 0xFFFFu;
+        q[c] = (int)l[c] < n_own_nodes ? base + rk : 0xFFFFu;
     }
 }
 

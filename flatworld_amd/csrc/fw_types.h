@@ -229,10 +229,14 @@ struct TileDev {
     int32_t hel_off;                    // hel[hel_off + slot - n_own] for halo slots
 };
 
-// Slot record corner word: local node id in the low bits, incidence rank above.
+// Slot record corner word: local node id in the low bits, incidence rank above.  A
+// triangle's record keeps five more rank bits per corner in the free upper half of
+// .y (corner c at bit 16 + 5c), so a triangle corner's rank reaches 1023; a spring's
+// .y holds its rest length, so a spring corner's rank stops at 31.
 #define FW_REC_ID_BITS 11
 #define FW_REC_ID_MASK 0x7FFu
 #define FW_REC_MAX_RANK 31
+#define FW_REC_MAX_RANK_TRI 1023
 
 // Compiled substep kernel shapes: (threads, element slots, LDS nodes, owned nodes per
 // thread, LDS nodes of a spring tile).  Triangle tiles: 16 B per LDS node (positions,

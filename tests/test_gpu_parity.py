@@ -286,3 +286,24 @@ def test_degenerate_element_and_isolated_node_match_oracle():
             assert _nan_equal(a, b), f"{name} (tile_nodes {tile_nodes})"
         assert np.isnan(o.x[-1]).any(), "the massless isolated node should be NaN, as in the reference"
         g.arena.close()
+
+
+def _fan(n_tri, r=0.1, c=(0.5, 0.102)):
+    ang = np.linspace(0.0, 2.0 * np.pi, n_tri, endpoint=False)
+    coords = np.concatenate([[c], np.asarray(c) + r * np.stack([np.cos(ang), np.sin(ang)], 1)]).astype(np.float32)
+    conn = np.array([[0, 1 + k, 1 + (k + 1) % n_tri] for k in range(n_tri)], dtype=np.int32)
+    return conn, coords
+
+
+@pytest.mark.parametrize("n_tri,shape", [(64, (256, 1024, 64)), (300, (256, 1024, 8)), (1000, (512, 2048, 64))])
+def test_high_valence_fan_bitexact(n_tri, shape):
+    """A disc of n_tri triangles around one node (n_tri element corners at the centre, beyond
+    the 32 a corner word holds) falling onto a plane, cut into several tiles."""
+    block, slots, nodes = shape
+    conn, coords = _fan(n_tri)
+    dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(2e4, 0.3, 100.0)), [fw.Gravity([0.0, -9.8])],
+                       initials=[fw.InitialVel("ALL", [0.0, -1.0])])
+    spec = W.one_body_spec(dom, fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0]))
+    _, o, out = run_both(spec, [1, 200, 200], True, f"fan of {n_tri} triangles", tile_nodes=nodes, tile_slots=slots,
+                         block=block)
+    assert _native.tiling_check(spec, nodes, slots)["tiles"] > 1

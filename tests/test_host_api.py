@@ -265,15 +265,24 @@ def _fan(n_tri, r=0.1):
 
 
 def test_node_degree_limit_is_reported():
-    """A node with more than 32 element corners cannot be encoded in a slot record: a clear error."""
-    tune = dict(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
-    ok_conn, ok_coords = _fan(32)
-    dom = fw.FemDomain(fw.Mesh(2, ok_conn, ok_coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
-    assert _native.Prepared(fw.pack_fem_domain(dom), tune).footprint > 0
-    conn, coords = _fan(33)
+    """A triangle node takes up to 1024 element corners (its rank spread over the slot
+    record's corner word and the upper half of .y); fans of 33, 64 and 1024 triangles
+    tile with every invariant checked; 1025 is refused with a clear error.  A spring
+    node stays limited to 32 (the record's .y holds the rest length)."""
+    for n_tri, slots in ((33, 1024), (64, 1024), (1024, 2048)):
+        conn, coords = _fan(n_tri)
+        dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
+        st = _native.tiling_check(fw.pack_fem_domain(dom), 64, slots)
+        assert st["csr_entries"] == 3 * n_tri
+    conn, coords = _fan(1025)
     dom = fw.FemDomain(fw.Mesh(2, conn, coords), fw.SolidProp(fw.Elastic(1e4, 0.3, 1.0)), [fw.Gravity([0.0, -9.8])])
+    with pytest.raises(NotImplementedError, match="at most 1024"):
+        _native.Prepared(fw.pack_fem_domain(dom), dict(tile_nodes=1024, tile_slots=2048, block=512, grid_per_cu=0))
+    xy = np.concatenate([[[0.5, 0.5]], 0.5 + 0.1 * np.stack([np.cos(np.arange(33)), np.sin(np.arange(33))], 1)])
+    star = np.stack([np.zeros(33, np.int64), np.arange(1, 34)], 1).astype(np.int32)
+    sdom = fw.SpringMassDomain(2, xy.astype(np.float32), star, [100.0, 0.1, 0.01], [fw.Gravity([0.0, -9.8])])
     with pytest.raises(NotImplementedError, match="at most 32"):
-        _native.Prepared(fw.pack_fem_domain(dom), tune)
+        _native.Prepared(fw.pack_domain(sdom), dict(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0))
 
 
 def test_isolated_node_and_degenerate_element_tile():
