@@ -49,6 +49,8 @@ extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
                                              uint32_t flags, char* scr, hipStream_t stream);
 extern "C" hipError_t fw_flex_noalloc_scratch(int device, char** out);
+extern "C" hipError_t fw_launch_rigid_pass(const SceneDev* scenes, const int2* rows, int n_rows, const ArenaCtl* ctl,
+                                           int k, uint32_t flags, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_scratch_init(const FlexDev* groups, int n_groups, char* scr, hipStream_t stream);
 
 namespace {
@@ -469,6 +471,8 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
         const fw_ground_desc& gd = d->grounds[g];
         if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(hf_bytes(gd.hf_nx));
         if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
+        // rigid: the rigid pass's float4 per node, then the boundary nodes' internal ids
+        if (gd.kind == FW_GROUND_RIGID) L.gdata[g] = take(16 * N + 4 * (int64_t)N);
     }
     L.host_bytes = o;
     // FEM-FEM contributions: int2 (first, count) per node, then the float2 entries;
@@ -699,6 +703,9 @@ struct fw_arena {
     int32_t* d_flex_items = nullptr;  // first global work item of each table row (+ the total)
     int flex_cap = 0, n_flex_live = 0, n_flex_items = 0;
     int64_t flex_scratch_bytes = 0;  // compact per-substep scratch of the live groups (fw_step_noalloc)
+    // (scene, record) of every rigid contact record of the live scenes, for the rigid pass
+    int2* d_rigid = nullptr;
+    int rigid_cap = 0, n_rigid_live = 0;
     bool flex_dirty = false;
     // one-entry tiling cache: fw_scene_footprint followed by fw_scene_add (or a retry of
     // an add that found the arena full) reuses the tiling instead of rebuilding it
@@ -837,6 +844,29 @@ int ensure_tiles(fw_arena* a) {
         }
         if (pass == 0) a->n_tri_tiles = (int)a->tile_table.size();
     }
+    std::vector<int2> rigid_rows;
+    for (size_t id = 0; id < a->scenes.size(); ++id) {
+        const auto& sp = a->scenes[id];
+        if (!sp || !sp->live) continue;
+        for (int g = 0; g < sp->dev.hot.n_grounds; ++g)
+            if (sp->dev.ground[g].kind == FW_GROUND_RIGID) rigid_rows.push_back(make_int2((int)id, g));
+    }
+    if ((int)rigid_rows.size() > a->rigid_cap) {
+        const int cap = std::max((int)rigid_rows.size(), std::max(64, 2 * a->rigid_cap));
+        int2* nd = nullptr;
+        HIPCHK(hipStreamSynchronize(a->stream));
+        HIPCHK(hipMalloc(&nd, sizeof(int2) * (size_t)cap));
+        if (a->d_rigid) HIPCHK(hipFree(a->d_rigid));
+        a->d_rigid = nd;
+        a->rigid_cap = cap;
+        drop_graphs(a);  // the graphs hold the old table pointer
+    }
+    if (!rigid_rows.empty()) {
+        int rc0 = upload(a, a->d_rigid, rigid_rows.data(), (int64_t)sizeof(int2) * (int64_t)rigid_rows.size());
+        if (rc0) return rc0;
+    }
+    if (a->n_rigid_live != (int)rigid_rows.size()) drop_graphs(a);  // the graphs hold the rigid launch's grid
+    a->n_rigid_live = (int)rigid_rows.size();
     const int nt = (int)a->tile_table.size();
     if (nt > a->tile_cap) {
         int cap = std::max(nt, std::max(1024, a->tile_cap * 2));
@@ -1071,6 +1101,11 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             G.ny = gd.flex_cap;
         } else if (gd.kind == FW_GROUND_RIGID) {
             pack_rigid(G, gd);
+            int32_t* bn = (int32_t*)(img + L.gdata[g] + 16 * (int64_t)N);
+            int nb = 0;
+            for (int i = 0; i < N; ++i)
+                if (flag[i] & FW_NODE_BOUNDARY) bn[nb++] = i;
+            G.reverse = nb;
         } else {
             G.a[0] = gd.vox_lb[0];
             G.a[1] = gd.vox_lb[1];
@@ -1273,6 +1308,7 @@ int32_t fw_arena_destroy(fw_arena* a) {
     (void)hipFree(a->d_ctl);
     (void)hipFree(a->d_flex);
     (void)hipFree(a->d_flex_items);
+    (void)hipFree(a->d_rigid);
     (void)hipStreamDestroy(a->stream);
     delete a;
     return FW_OK;
@@ -1489,6 +1525,10 @@ int32_t fw_arena_busy(fw_arena* a, double* busy_ms, int64_t* calls) {
 // as the control block counts them at run time.
 static hipError_t launch_all(fw_arena* a, int k, float dt_override, float damping_override, uint32_t f, int grid_tri,
                       int grid_spr, char* flex_scr = nullptr) {
+    if (a->n_rigid_live > 0) {  // FEM-rigid forces before the substep that adds them
+        hipError_t e = fw_launch_rigid_pass(a->d_scenes, a->d_rigid, a->n_rigid_live, a->d_ctl, k, f, a->stream);
+        if (e != hipSuccess) return e;
+    }
     if (a->n_flex_live > 0) {  // FEM-FEM contact lists before the substep that adds them
         hipError_t e = fw_launch_flex_contact(a->d_flex, a->n_flex_live, a->d_flex_items, a->n_flex_items, a->d_scenes,
                                               a->d_ctl, k, f, flex_scr, a->stream);
@@ -1519,6 +1559,7 @@ static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_ov
         if (k != n - 1) f &= ~FW_STEP_DIAG;
         if (nt > 0) {
             if (flags & FW_STEP_PER_SCENE) {
+                HIPCHK(fw_launch_rigid_pass(a->d_scenes, a->d_rigid, a->n_rigid_live, a->d_ctl, k, f, a->stream));
                 for (auto& sp : a->scenes) {
                     if (!sp || !sp->live) continue;
                     const int cnt = (int)sp->tiles.size();
@@ -1584,6 +1625,7 @@ int32_t fw_step_noalloc(fw_arena* a, int32_t n, float dt_override, float damping
         if (k != n - 1) f &= ~FW_STEP_DIAG;
         if (nt <= 0) continue;
         if (flags & FW_STEP_PER_SCENE) {
+            HIPCHK_NOALLOC(fw_launch_rigid_pass(a->d_scenes, a->d_rigid, a->n_rigid_live, a->d_ctl, k, f, a->stream));
             for (auto& sp : a->scenes) {
                 if (!sp || !sp->live) continue;
                 const int cnt = (int)sp->tiles.size();

@@ -285,6 +285,8 @@ __device__ __forceinline__ float rigid_detect(const GroundDev& g, float px, floa
 // nothing outside the rigid's AABB; a tied node pulled to its contact point within the
 // margin (half the AABB diagonal); otherwise the penalty force k (1e-4 - pen) n where
 // pen < 1e-4, with Coulomb friction against the rigid's surface velocity.
+// Evaluated by the rigid pass ahead of the substep (fw_kernels.hip rigid_pass_kernel),
+// whose force the substep adds at the record's place among the node's ground forces.
 __device__ __forceinline__ bool rigid_force(const GroundDev& g, float px, float py, float vx, float vy, float& fx,
                                             float& fy) {
     if (wmin(px - g.lo[0], py - g.lo[1]) < 0.0f || wmin(g.hi[0] - px, g.hi[1] - py) < 0.0f) return false;
@@ -326,7 +328,6 @@ __device__ __forceinline__ bool rigid_force(const GroundDev& g, float px, float 
 __device__ __forceinline__ bool ground_force(const GroundDev& g, float px, float py, float vx, float vy, float dt,
                                              float& fx, float& fy) {
     const float k = g.penalty;
-    if (g.kind == FW_GROUND_RIGID_DEV) return rigid_force(g, px, py, vx, vy, fx, fy);
     if (g.kind == 1) {
         float pen = (px - g.a[0]) * g.a[2] + (py - g.a[1]) * g.a[3];
         if (pen < 0.0f) {

@@ -57,11 +57,11 @@ __device__ __forceinline__ void unpack_rec(uint2 r, int n_own_nodes, const uint1
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         l[c] = w[c] & FW_REC_ID_MASK;
-        // rank: five bits in the corner word, five more in the upper half of .y
-        const uint32_t rk = (w[c] >> FW_REC_ID_BITS) | (((r.y >> (16 + 5 * c)) & 31u) << 5);
-        // branch-free: halo corners read a valid entry and discard it
+        // branch-free: halo corners read a valid entry and discard it; the rank is five
+        // bits in the corner word and five more in the upper half of .y
         const uint32_t base = lds_cp[min((int)l[c], n_own_nodes)];
-        q[c] = (int)l[c] < n_own_nodes ? base + rk : 0xFFFFu;
+        q[c] = (int)l[c] < n_own_nodes ? base + ((w[c] >> FW_REC_ID_BITS) | (((r.y >> (16 + 5 * c)) & 31u) << 5))
+                                       : 0xFFFFu;
     }
 }
 
@@ -360,6 +360,12 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // slots before the end, so that they land during the last slots and the gate.
         constexpr int ND_AT = SPT >= 2 ? SPT - 2 : 0;
         NodeIn nd[NPT];
+        // BC value rows, fetched in the last slot iteration once the flags (which hold the
+        // row) have landed, so that their round trip overlaps the last slot and the gate
+        const float2* __restrict__ bcv = fw_at<const float2>(T.base, T.o_bcv);
+        float2 bv[NPT];
+#pragma unroll
+        for (int kn = 0; kn < NPT; ++kn) bv[kn] = make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int k = 0; k < SPT; ++k) {
             const int sl = tid + k * BLOCK;
@@ -382,6 +388,11 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                         nd[kn].fl = fw_ld(fw_at<const uint32_t>(T.base, T.o_flag) + n);
                     }
                 }
+            }
+            if (SPT >= 2 && k == SPT - 1) {
+#pragma unroll
+                for (int kn = 0; kn < NPT; ++kn)
+                    if (tid + kn * BLOCK < n_nodes) bv[kn] = fw_ld(bcv + (nd[kn].fl >> NODE_BCV_SHIFT));
             }
             if (sl < nslots) {
                 if constexpr (spring) {
@@ -430,12 +441,10 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
         // first push entry per thread, loaded now so that it lands during phase B
         int2 pe = make_int2(0, 0);
         if (tid < T.n_push) pe = fw_ld(fw_at<const int2>(T.base, T.o_push) + T.push_off + tid);
-        const float2* __restrict__ bcv = fw_at<const float2>(T.base, T.o_bcv);
-        float2 bv[NPT];
+        if (SPT < 2) {
 #pragma unroll
-        for (int k = 0; k < NPT; ++k) {
-            bv[k] = make_float2(0.0f, 0.0f);
-            if (tid + k * BLOCK < n_nodes) bv[k] = fw_ld(bcv + (nd[k].fl >> NODE_BCV_SHIFT));
+            for (int k = 0; k < NPT; ++k)
+                if (tid + k * BLOCK < n_nodes) bv[k] = fw_ld(bcv + (nd[k].fl >> NODE_BCV_SHIFT));
         }
         float2* const st_force = diag ? fw_ldc1(&Sp->force) : nullptr;
         uint8_t* const st_cflag = diag ? fw_ldc1(&Sp->cflag) : nullptr;
@@ -469,7 +478,17 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                             fy = fy + c.y;
                         }
                     } else {
-                        hit |= ground_force(gd, p.x, p.y, v.x, v.y, dt, fx, fy);
+                        if (gd.kind == FW_GROUND_RIGID_DEV) {
+                            // the rigid pass evaluated this node's force against the primitive
+                            const float4 rf = fw_ld(static_cast<const float4*>(gd.data) + n);
+                            if (rf.z != 0.0f) {
+                                fx = fx + rf.x;
+                                fy = fy + rf.y;
+                                hit = true;
+                            }
+                        } else {
+                            hit |= ground_force(gd, p.x, p.y, v.x, v.y, dt, fx, fy);
+                        }
                     }
                 }
             }
@@ -572,6 +591,34 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             }
         }
         lds_barrier();  // LDS reuse by the next tile of this block
+    }
+}
+
+// ------------------------------------------------------------------ rigid pass
+
+// FEM-rigid contact forces of one substep (_batched_rigid_prim_kernel,
+// flatworld/mixedcontact.py:1025-1117), one workgroup per (scene, rigid record), ahead
+// of the substep launch: every boundary node's force against the primitive at x_n,
+// v_n, into the record's per-node slots, which the substep adds at the record's place
+// among the node's ground forces when the pair gate is active.  Kept out of the
+// substep kernel so that its registers do not pay for the primitive queries.
+__global__ __launch_bounds__(256) void rigid_pass_kernel(const SceneDev* scenes, const int2* rows, const ArenaCtl* ctl,
+                                                         int k, uint32_t flags) {
+    const int2 row = rows[blockIdx.x];
+    const SceneDev* S = scenes + row.x;
+    const int local = fw_ldc1(&ctl->clock) + k - S->s0;
+    if (local < 0 || local >= S->horizon || (flags & STEP_NO_CONTACT)) return;  // warm-up: no contact
+    const GroundDev g = S->ground[row.y];
+    const float2* __restrict__ pos = S->pos[(local + 1) & 1];
+    const float2* __restrict__ vel = S->vel;
+    float4* __restrict__ out = static_cast<float4*>(const_cast<void*>(g.data));
+    const int32_t* __restrict__ bn = reinterpret_cast<const int32_t*>(out + S->n_nodes);
+    for (int j = threadIdx.x; j < g.reverse; j += blockDim.x) {
+        const int n = fw_ld(bn + j);
+        const float2 p = fw_ld(pos + n), v = fw_ld(vel + n);
+        float fx = 0.0f, fy = 0.0f;
+        const bool hit = rigid_force(g, p.x, p.y, v.x, v.y, fx, fy);
+        fw_st(out + n, make_float4(fx, fy, hit ? 1.0f : 0.0f, 0.0f));
     }
 }
 
@@ -794,6 +841,13 @@ extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileD
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream) {
     hipLaunchKernelGGL(fw::copy_probe_kernel, dim3(grid), dim3(256), 0, stream, (const float4*)src, (float4*)dst,
                        n_float4);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t fw_launch_rigid_pass(const SceneDev* scenes, const int2* rows, int n_rows, const ArenaCtl* ctl,
+                                           int k, uint32_t flags, hipStream_t stream) {
+    if (n_rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fw::rigid_pass_kernel, dim3(n_rows), dim3(256), 0, stream, scenes, rows, ctl, k, flags);
     return hipGetLastError();
 }
 

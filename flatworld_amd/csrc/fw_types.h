@@ -37,7 +37,10 @@ struct GroundDev {
     float hf_dxw;          // HF: sample spacing (ub_x - lb_x) / (nx - 1), rounded once in float32 on the host
     float b[8];            // rigid: rotation (row-major, 4), velocity (2), angular velocity, friction;
                            // with a[0..1] the centre, a[2..3] the primary parameters, nx the
-                           // type, ny the tied flag, hf_dxw the radius, lo / hi the rigid's AABB
+                           // type, ny the tied flag, hf_dxw the radius, reverse the scene's
+                           // boundary-node count, lo / hi the rigid's AABB; data: float4 per node
+                           // (force x, y, 1 when a force was applied, 0) written by the rigid
+                           // pass before each substep, then the boundary nodes' internal ids
     const void* data;      // HF heights (float) | voxel cell extents (uint16 x4 per cell: scan end points
                            // along -x, +x, -z, +z; first = 0xFFFF when the cell is not occupied) |
                            // FEM-FEM: int2 (first entry, count) per node, then the float2 entries

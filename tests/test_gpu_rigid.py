@@ -155,3 +155,40 @@ def test_explicitloop_with_kinematic_ball():
         assert np.array_equal(pos, o.x), f"frame {frame}"
         touched += int(o.contact_flag.any())
     assert touched > 0
+
+
+def test_rigid_records_under_graph_per_scene_and_noalloc_launches():
+    """Rigid-bearing scenes beside plain ones: hipGraph replay, one launch per scene and
+    fw_step_noalloc give the bits of plain launches (the rigid pass runs in each)."""
+    specs = W.batched_config(n_scenes=4, seed=21, side_lo=6, side_hi=20)
+    for k, (rig, fr) in enumerate(((fw.BallRigid(2, [0.5, 0.0], 0.08, 1.0), 0.0),
+                                  (fw.BoxRigid(2, [0.5, -0.02], [0.3, 0.1], 0.2, 1.0), 0.3))):
+        dom = W.grid_domain(12, (0.4, 0.085), 0.2, fw.Elastic(2e4, 0.3, 100.0), [fw.Gravity([0.0, -9.8])],
+                            initials=[fw.InitialVel("ALL", [0.1, -0.5])])
+        specs.append(_scene(dom, [fw.RigidBodyDomain(rig, friction=fr)]))
+    log("[case] rigid records: plain vs graph vs per-scene vs noalloc")
+    arenas = [_native.Arena(tile_nodes=128) for _ in range(4)]
+    ids = [[a.add(s) for s in specs] for a in arenas]
+    for a in arenas:
+        a.step(1, -1.0, 0)
+    a0, a1, a2, a3 = arenas
+    a3.step(0)
+    for rep in range(5):
+        a0.step(60, -1.0, _native.FW_STEP_DIAG)
+        a1.step_graph(60, _native.FW_STEP_DIAG)
+        a2.step(60, -1.0, _native.FW_STEP_DIAG | _native.FW_STEP_PER_SCENE)
+        a3.step_noalloc(60, -1.0, _native.FW_STEP_DIAG)
+        for a in arenas:
+            a.sync()
+        for j in range(len(specs)):
+            for fld in (_native.FIELD_X, _native.FIELD_V, _native.FIELD_FORCE, _native.FIELD_CONTACT):
+                ref = a0.read(ids[0][j], fld)
+                for m in (1, 2, 3):
+                    assert np.array_equal(arenas[m].read(ids[m][j], fld), ref), f"rep {rep} scene {j} field {fld} mode {m}"
+    o = OracleScene(specs[-1])
+    o.warmup()
+    o.run(300)
+    assert o.contact_flag.any()
+    check_fields(lambda f: a0.read(ids[0][-1], f), o, "rigid scene in the mixed arena")
+    for a in arenas:
+        a.close()
