@@ -566,8 +566,6 @@ int validate(const fw_scene_desc* d) {
         if (gd.kind < FW_GROUND_PLANE || gd.kind > FW_GROUND_RIGID)
             return fail(FW_E_INVALID, "unknown ground kind " + std::to_string(gd.kind));
         if (gd.kind == FW_GROUND_RIGID) {
-            if (d->mat_type == FW_MAT_SPRING)
-                return fail(FW_E_UNSUPPORTED, "rigid contact records need a FEM scene (spring-rigid contact is not built)");
             int rc = validate_rigid(gd);
             if (rc) return rc;
         }
@@ -628,6 +626,7 @@ struct FlexHost {
     int64_t block_off = 0, block_bytes = 0;
     std::vector<int32_t> scenes;
     int32_t n_items = 0, max_cells = 0, max_pairs = 0, n_tied = 0;
+    bool hash_built = false;  // set by the first fw_flex_rebuild; tie resolution walks that hash
     FlexDev dev{};
 };
 
@@ -2272,6 +2271,8 @@ int32_t fw_flex_rebuild(fw_arena* a, int32_t gid) {
     }
     HIPCHK(fw_launch_flex_hash(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, gid >= 0 ? a->flex_row[gid] : -1,
                                a->stream));
+    for (size_t g = 0; g < a->flex.size(); ++g)
+        if (a->flex[g] && (gid < 0 || (int)g == gid)) a->flex[g]->hash_built = true;
     return FW_OK;
 }
 
@@ -2287,6 +2288,8 @@ int32_t fw_flex_init_ties(fw_arena* a, int32_t gid) {
     if (!s0 || a->clock < s0->s0)
         return fail(FW_E_INVALID, "group " + std::to_string(gid) + " has not run its warm-up substep yet");
     if (F.n_tied == 0) return FW_OK;
+    if (!F.hash_built)
+        return fail(FW_E_INVALID, "group " + std::to_string(gid) + " has no spatial hash yet: call fw_flex_rebuild first");
     HIPCHK(fw_launch_flex_ties(a->d_flex, a->flex_row[gid], F.n_items, a->d_scenes, a->d_ctl, a->stream));
     return FW_OK;
 }

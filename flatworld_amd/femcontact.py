@@ -15,11 +15,11 @@ The force laws themselves run on the device (flatworld_amd/csrc/fw_kernels.hip,
 fw_flex.hip).  A FEM-spring pair exists as in the reference but applies no force
 there: ExplicitLoop files it among the unbatched contacts, which no substep runs.
 A FEM body against a rigid primitive (flatworld/femcontact.py:173-180) takes the
-FEM body's segment penalty ``2 K`` and its contact stable time; the rigid is a
-kinematic primitive (flatworld_amd.rigid).  Spring-rigid pairs are outside this
-package: constructing one raises ``NotImplementedError`` instead of silently
-dropping it.  Pairs the reference's factory has no class for (spring-spring,
-spring-voxel) give no contact, as there.
+FEM body's segment penalty ``2 K`` and its contact stable time; a spring network
+against one (femcontact.py:243-253) takes ``100 spring`` and ``0.5 sqrt(mass / k)``,
+with every node a work item of the same primitive kernel.  The rigid is a
+kinematic primitive (flatworld_amd.rigid).  Pairs the reference's factory has no
+class for (spring-spring, spring-voxel) give no contact, as there.
 """
 
 from .definitions import ContactType, DomainType
@@ -130,6 +130,13 @@ class ContactSpringHeightField(_SpringGround):
     ground_type = DomainType.HEIGHTFIELD
 
 
+class ContactSpringRigid(_SpringGround):
+    """Springs vs rigid primitive (flatworld/femcontact.py:243-253); every spring node is a
+    work item of the rigid-primitive kernel (flatworld/mixedcontact.py:2001-2011)."""
+
+    ground_type = DomainType.RIGID
+
+
 class ContactSpringFlex(ContactBase):
     """Springs vs FEM body (flatworld/femcontact.py:200-213); ``domain1`` is always the
     springs.  The reference files this pair among its unbatched contacts
@@ -160,7 +167,8 @@ class ContactFlexRigid(ContactBase):
 
 
 def contactHelper(domain1, domain2, type):
-    """Pair factory (flatworld/femcontact.py:283-303) without the rigid pairs."""
+    """Pair factory (flatworld/femcontact.py:283-303); rigid-rigid and rigid-plane pairs
+    belong to the rigid-body dynamics and give no contact object here."""
     if type == ContactType.ANALYTICALSPRING:
         return ContactSpringAnalytical(domain1, domain2, type)
     if type == ContactType.SPRINGHEIGHTFIELD:
@@ -179,6 +187,8 @@ def contactHelper(domain1, domain2, type):
         return ContactFlexFlex(domain1, domain2, type)
     if type == ContactType.FLEXRIGID:
         return ContactFlexRigid(domain1, domain2, type)
+    if type == ContactType.RIGIDSPRING:
+        return ContactSpringRigid(domain1, domain2, type)
     if type in (ContactType.RIGIDANLAYTICAL, ContactType.RIGIDRIGID, ContactType.ANALYTICALANALYTICAL):
         return None
     raise NotImplementedError(f"contact type {type:#b} is not part of the explicit-FEM path")

@@ -71,23 +71,36 @@ class FemSpringManager:
         self.domains = []
         self.domainMeshes = []
         self.stableTime = 1.0
-        self._arena = _native.Arena(device=device, **(arena_kwargs or {}))
-        self._scene_ids = []
-        node_off, elem_off = 0, 0
-        self._node_offsets = []
-        self._elem_offsets = []
+        # the scenes are tiled and packed on the host first, so that the arena takes the
+        # HBM they need (plus the FEM-FEM group's block), not a share of the free memory:
+        # several loops then live side by side on one GPU
+        arena_kwargs = dict(arena_kwargs or {})
+        shape = {k: arena_kwargs[k] for k in ("tile_nodes", "tile_slots", "block", "grid_per_cu") if k in arena_kwargs}
+        shape = dict(dict(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0), **shape)
         grounds_of = grounds_of or {}
-        conn_rows = []
-        fem_body = 0
         self._specs = []  # the packed scenes, in manager order
-        for local_idx, (idx, dom) in enumerate(fem):
+        fem_body = 0
+        for idx, dom in fem:
             spec = pack_domain(dom, grounds_of.get(idx, []))
             if flex is not None and dom.type == DomainType.FEM:
                 if flex.in_pair(fem_body):
                     add_flex_record(spec, flex.flex_cap(fem_body))
                 fem_body += 1
-            sid = self._arena.add(spec, track_disp=track_disp)
             self._specs.append(spec)
+        prepared = [_native.Prepared(spec, shape, track_disp) for spec in self._specs]
+        if "capacity_bytes" not in arena_kwargs:
+            need = sum(p.footprint for p in prepared)
+            if flex is not None:
+                need += flex.block_bytes_bound()
+            arena_kwargs["capacity_bytes"] = need + (16 << 20)
+        self._arena = _native.Arena(device=device, **arena_kwargs)
+        self._scene_ids = []
+        node_off, elem_off = 0, 0
+        self._node_offsets = []
+        self._elem_offsets = []
+        conn_rows = []
+        for local_idx, ((idx, dom), spec, prep) in enumerate(zip(fem, self._specs, prepared)):
+            sid = self._arena.add_prepared(prep)
             self._scene_ids.append(sid)
             self._node_offsets.append(node_off)
             self._elem_offsets.append(elem_off)

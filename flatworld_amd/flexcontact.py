@@ -67,6 +67,14 @@ class FlexGroupSpec:
     def n_pairs(self):
         return int(self.pair_body.shape[0])
 
+    def block_bytes_bound(self):
+        """An upper bound of the arena bytes fw_flex_add takes for this group: uploaded
+        tables, hash arrays and per-substep scratch, each padded to 256 bytes."""
+        W, S, B = self.n_items, self.n_segs, self.n_bodies
+        NS = int(self.body_bnodes.shape[0])
+        return (168 * W + 60 * S + 12 * NS + 8 * self.n_pairs + 24 * B + 16 * self.max_cells + 8 * self.max_pairs
+                + 48 * 256)
+
     def tied_flags(self):
         return np.zeros(self.n_items, np.int32) if self.item_tied is None else np.asarray(self.item_tied, np.int32)
 

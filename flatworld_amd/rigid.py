@@ -5,9 +5,10 @@ integrated by its RigidManager (flatworld/rigidmanager.py: gravity, PGS contact
 between rigids, joints).  That solver is outside this package.  Here a rigid
 primitive -- ball, box or capsule -- is *kinematic*: its pose (centre, angle) and
 velocity are set by the caller (``set_pose`` / ``set_motion``, or advanced by
-``advance`` at constant velocity), and FEM bodies touch it through the
-reference's FEM-rigid penalty contact (flatworld/mixedcontact.py:1025-1117), which
-runs on the device as a contact record of the FEM scene.  The reaction on the
+``advance`` at constant velocity), and FEM bodies and spring networks touch it
+through the reference's FEM-rigid / spring-rigid penalty contact
+(flatworld/mixedcontact.py:1025-1117), which runs on the device as a contact
+record of the FEM or spring scene.  The reaction on the
 rigid (the impulses the reference accumulates for its solver) is not computed.
 
 The device reads, per rigid record, the quantities the reference's kernel reads

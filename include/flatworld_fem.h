@@ -77,9 +77,10 @@ extern "C" {
  * contributions per substep (own work items + 2 x work items that target this body). */
 #define FW_GROUND_FLEX 4
 /* FW_GROUND_RIGID: a rigid primitive (ball, box, capsule) at a pose the caller sets
- * (fw_scene_set_rigid), pushing the scene's boundary nodes with the reference's
- * FEM-rigid penalty contact (_batched_rigid_prim_kernel,
- * flatworld/mixedcontact.py:1025-1117).  It goes where the reference launches that
+ * (fw_scene_set_rigid), pushing the scene's boundary nodes (a spring scene: every
+ * node) with the reference's FEM-rigid and spring-rigid penalty contact
+ * (_batched_rigid_prim_kernel, flatworld/mixedcontact.py:1025-1117; work lists at
+ * mixedcontact.py:1987-2030).  It goes where the reference launches that
  * kernel: after the FEM-FEM record, before the height-field and voxel records.
  * aabb_lo / aabb_hi hold the primitive's AABB at the pose (getBallBBox /
  * getBoxBBox / getCapsuleBBox, flatworld/utils.py:235-302); the gate pads it on BVH
@@ -319,7 +320,8 @@ int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, in
  * substep, mixedcontact.py:1668-1689); from then on every substep's contact
  * pass applies a penalty spring toward the resolved edge point at the initial
  * normal gap (_apply_fem_fem_ties_fixed_kernel, mixedcontact.py:773-811), each
- * node's tie contributions before its sliding-contact ones. */
+ * node's tie contributions before its sliding-contact ones.  It returns
+ * FW_E_INVALID for a group with tied items whose hash was never built. */
 int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* desc, int32_t* group_id);
 int32_t fw_flex_rebuild(fw_arena* a, int32_t group_id);
 int32_t fw_flex_init_ties(fw_arena* a, int32_t group_id);

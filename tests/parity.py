@@ -38,11 +38,19 @@ def min_edge(spec):
 
 
 class GpuScene:
-    """One scene alone in its own arena."""
+    """One scene alone in its own arena, sized to the scene (so that arenas a test
+    has not released yet do not hold the GPU's memory)."""
 
     def __init__(self, spec, arena=None, **arena_kw):
-        self.arena = arena or _native.Arena(**arena_kw)
-        self.sid = self.arena.add(spec)
+        if arena is None:
+            shape = {k: v for k, v in arena_kw.items() if k != "capacity_bytes"}
+            prep = _native.Prepared(spec, dict(dict(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0), **shape))
+            arena_kw.setdefault("capacity_bytes", prep.footprint + (16 << 20))
+            arena = _native.Arena(**arena_kw)
+            self.sid = arena.add_prepared(prep)
+        else:
+            self.sid = arena.add(spec)
+        self.arena = arena
         self.spec = spec
 
     def warmup(self):

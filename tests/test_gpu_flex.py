@@ -168,6 +168,24 @@ def test_group_refuses_graph_and_scene_removal():
     m._arena.sync()
 
 
+def test_tie_resolution_needs_a_built_hash():
+    """Tie resolution walks the group's spatial hash: before the first rebuild
+    the call is refused instead of resolving nothing."""
+    from flatworld_amd import workloads as W
+
+    (sp, g), = W.contact_batched_config(n_groups=1, seed=4)
+    g.item_tied = np.ones(g.n_items, np.int32)
+    arena = _native.Arena(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0)
+    gid = arena.flex_add(g, [arena.add(x) for x in sp])
+    arena.step(1, -1.0, 0)
+    with pytest.raises(ValueError, match="no spatial hash yet"):
+        arena.flex_init_ties(gid)
+    arena.flex_rebuild(gid)
+    arena.flex_init_ties(gid)
+    arena.step(4, -1.0, 0)
+    arena.sync()
+
+
 def test_contact_workload_sampled_loops_bitexact():
     """The bench's 2048-loop contact workload in one arena, 600 substeps on the
     reference's rebuild schedule (one launch rebuilds every loop's hash); 24

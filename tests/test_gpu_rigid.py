@@ -91,6 +91,25 @@ def test_moving_spinning_capsule_pushes_block_on_plane():
     assert max(contacts) > 0
 
 
+def test_spring_lattice_falls_on_ball_and_rotated_box():
+    """Spring-rigid pairs (flatworld/femcontact.py:243-253): every lattice node is a work
+    item of the rigid-primitive contact; a jittered lattice with bending springs drops
+    onto a ball and a rotated box with friction, on one tile and on several."""
+    dom = W.spring_lattice(24, (0.38, 0.30), 0.01, [300.0, 0.4, 0.01], [fw.Gravity([0.0, -9.8])],
+                           initials=[fw.InitialVel("ALL", [0.1, -0.4])], jitter=0.1, seed=3)
+    ball = fw.RigidBodyDomain(fw.BallRigid(2, [0.45, 0.2], 0.08, 1.0))
+    box = fw.RigidBodyDomain(fw.BoxRigid(2, [0.62, 0.18], [0.3, 0.06], -0.3, 1.0), friction=0.3)
+    pairs = [(g, contactHelper(g, dom, g.type | dom.type)) for g in (ball, box)]
+    assert all(type(c).__name__ == "ContactSpringRigid" for _, c in pairs)
+    dt = min([1e-3, dom.stableTime] + [c.stableTime for _, c in pairs])
+    spec = fw.pack_domain(dom, pairs, dt=dt)
+    assert [g.kind for g in spec.grounds] == [5, 5]
+    for kw in ({}, dict(tile_nodes=128)):
+        # the lattice bounces: compare every 40 substeps so that the checks see its contacts
+        _, o, contacts = _run(spec, [1] + [40] * 30, f"spring lattice onto a ball and a box {kw}", **kw)
+        assert sum(c > 0 for c in contacts) >= 5, "the lattice should reach the rigids"
+
+
 def test_tied_block_hangs_from_box():
     dom = W.grid_domain(12, (0.4, 0.1995), 0.2, fw.Elastic(2e4, 0.3, 100.0), [fw.Gravity([0.0, -9.8])])
     box = fw.RigidBodyDomain(fw.BoxRigid(2, [0.5, 0.1], [0.6, 0.2], 0.0, 1.0))
@@ -157,6 +176,38 @@ def test_explicitloop_with_kinematic_ball():
     assert touched > 0
 
 
+def test_explicitloop_spring_network_with_kinematic_box():
+    """The drop-in API with a spring network: a lattice on a plane, a box sliding along
+    the plane into it with friction; the oracle follows the same float64 dt sequence
+    and the same box poses."""
+    def build():
+        dom = W.spring_lattice(12, (0.45, 0.001), 0.01, [200.0, 0.3, 0.01], [fw.Gravity([0.0, -9.8])])
+        plane = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+        box = fw.RigidBodyDomain(fw.BoxRigid(2, [0.398, 0.05], [0.1, 0.1], 0.0, 1.0), friction=0.3)
+        box.set_motion([0.6, 0.0], 0.0)
+        return dom, plane, box
+
+    dom, plane, box = build()
+    loop = fw.ExplicitLoop(0.0, [plane, dom, box])
+    odom, oplane, obox = build()
+    c_pl = contactHelper(oplane, odom, oplane.type | odom.type)
+    c_b = contactHelper(obox, odom, obox.type | odom.type)
+    spec = fw.pack_domain(odom, [(oplane, c_pl), (obox, c_b)])
+    o = OracleScene(spec)
+    o.warmup()
+    log(f"[case] ExplicitLoop: spring network and a kinematic box, dt={loop.dt:.3e}")
+    for frame in range(10):
+        loop.advanceWithTime(1.0 / 240.0)
+        for sd in loop.substep_sequence(1.0 / 240.0):
+            update_rigid_spec(spec.grounds[1])
+            o.set_rigid(1, spec.grounds[1])
+            o.substep(sd, loop.damping)
+            obox.advance(sd)
+        pos = loop.femSpringManager.coords.numpy()[: dom.nnodes]
+        assert np.array_equal(pos, o.x), f"frame {frame}"
+    assert o.x[:, 0].min() > 0.46, "the box should have pushed the lattice"
+
+
 def test_rigid_records_under_graph_per_scene_and_noalloc_launches():
     """Rigid-bearing scenes beside plain ones: hipGraph replay, one launch per scene and
     fw_step_noalloc give the bits of plain launches (the rigid pass runs in each)."""
@@ -167,7 +218,7 @@ def test_rigid_records_under_graph_per_scene_and_noalloc_launches():
                             initials=[fw.InitialVel("ALL", [0.1, -0.5])])
         specs.append(_scene(dom, [fw.RigidBodyDomain(rig, friction=fr)]))
     log("[case] rigid records: plain vs graph vs per-scene vs noalloc")
-    arenas = [_native.Arena(tile_nodes=128) for _ in range(4)]
+    arenas = [_native.Arena(tile_nodes=128, capacity_bytes=256 << 20) for _ in range(4)]
     ids = [[a.add(s) for s in specs] for a in arenas]
     for a in arenas:
         a.step(1, -1.0, 0)

@@ -171,3 +171,29 @@ def test_block_on_wide_box_matches_plane():
     assert a.x[spec_box.bnodes, 1].max() > 0.1 and a.x[spec_box.bnodes][:, 1][a.contact_flag[spec_box.bnodes] > 0].max() <= 0.0
     h = 0.2 / 6
     assert np.abs(a.x - b.x).max() < 1e-4 * h
+
+
+def test_spring_network_on_wide_box_matches_plane():
+    """Spring-rigid pairs (flatworld/femcontact.py:243-253): penalty 100 x spring and
+    stable time 0.5 sqrt(mass / k), as against a plane; every spring node is a work
+    item.  A lattice pushed down into a wide static box top at y = 0 moves as on a
+    plane at y = 1e-4 while its touching nodes stay below the box top."""
+    dom = W.spring_lattice(5, (0.3, 0.0), 0.02, [200.0, 0.3, 0.01], [fw.Gravity([0.0, -9.8])],
+                           initials=[fw.InitialVel("ALL", [0.0, -0.2])])
+    box = fw.RigidBodyDomain(fw.BoxRigid(2, [0.4, -0.5], [4.0, 1.0], 0.0, 1.0))
+    plane = fw.GroundDomain(2, [0.0, 1e-4], [0.0, 1.0])
+    c_box = contactHelper(box, dom, box.type | dom.type)
+    c_pl = contactHelper(plane, dom, plane.type | dom.type)
+    assert type(c_box).__name__ == "ContactSpringRigid" and c_box.domain1 is dom
+    assert c_box.penalty == 100.0 * 200.0 == c_pl.penalty
+    assert c_box.stableTime == c_pl.stableTime == 0.5 * math.sqrt(0.01 / 2e4)
+    spec_box = fw.pack_domain(dom, [(box, c_box)], dt=2e-5)
+    spec_pl = fw.pack_domain(dom, [(plane, c_pl)], dt=2e-5)
+    assert spec_box.bnodes.shape[0] == dom.nnodes
+    a, b = OracleScene(spec_box), OracleScene(spec_pl)
+    for o in (a, b):
+        o.warmup()
+        o.run(200)
+    assert a.contact_flag.sum() >= 6 and b.contact_flag.any()
+    assert a.x[a.contact_flag > 0, 1].max() <= 0.0
+    assert np.abs(a.x - b.x).max() < 1e-4 * 0.02
