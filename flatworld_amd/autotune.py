@@ -34,6 +34,7 @@ SHAPES = [
     (512, 1024, 512),
     (256, 512, 256),
     (512, 2048, 1024),
+    (512, 2048, 512),
     (1024, 2048, 1024),
 ]
 GRID_PER_CU = [0, 4]  # 0: one workgroup per tile; 4: persistent grid

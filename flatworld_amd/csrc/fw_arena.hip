@@ -33,11 +33,12 @@
 
 extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
                                         int tile_base, int tile_count, float dt_override, float damping_override,
-                                        uint32_t flags, int grid, int block, int slots, int kind, hipStream_t stream);
+                                        uint32_t flags, int grid, int block, int slots, int npt, int kind,
+                                        hipStream_t stream);
 extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* tiles, int tile_base, int tile_count,
                                         int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
-extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu);
+extern "C" hipError_t fw_substep_occupancy(int block, int slots, int npt, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
 extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
                                              float dt_target, int block, int slots, hipStream_t stream);
@@ -422,6 +423,23 @@ void build_hf_normals(const GroundDev& G, const float* h, float2* nrm) {
         }
         nrm[ir] = make_float2(n0, n1);
     }
+}
+
+// Height beyond which a node cannot touch a height field: the interpolated height
+// h0 + (h1 - h0) t (t in [0, 1]) stays within (|highest| + |lowest|) 2^-22 of the
+// sample range in float32, and penetration needs z < h (z > h for a reversed field,
+// whose normals point down), so a node above the returned level (below it, reversed)
+// has a signed distance >= 0 or NaN: no contact.  The device skips the query there.
+float hf_skip_level(const float* h, int nx, bool reversed) {
+    double hi = -INFINITY, lo = INFINITY;
+    for (int i = 0; i < nx; ++i) {
+        if (h[i] > hi) hi = h[i];
+        if (h[i] < lo) lo = h[i];
+    }
+    if (!(hi >= lo) || std::isinf(hi) || std::isinf(lo)) return reversed ? -INFINITY : INFINITY;
+    const double m = (std::fabs(hi) + std::fabs(lo)) * std::ldexp(1.0, -22) + 1e-37;
+    if (reversed) return std::nextafter((float)(lo - m), -INFINITY);
+    return std::nextafter((float)(hi + m), INFINITY);
 }
 
 // ------------------------------------------------------------------ scene layout
@@ -1095,6 +1113,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             G.reverse = gd.reverse;
             std::memcpy(img + L.gdata[g], gd.heights, 4 * (size_t)gd.hf_nx);
             build_hf_normals(G, gd.heights, (float2*)(img + L.gdata[g] + hf_normals_offset(gd.hf_nx)));
+            G.b[0] = hf_skip_level(gd.heights, gd.hf_nx, gd.reverse == 1);
         } else if (gd.kind == FW_GROUND_FLEX) {
             G.nx = N;             // the entries follow the per-node ranges
             G.ny = gd.flex_cap;
@@ -1220,6 +1239,18 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
 
 // The arena's tuning for a caller's fw_tuning (NULL: the defaults), checked
 // against the compiled kernel shapes.
+// Owned nodes per thread of the compiled shape that runs a tuning: the smallest whose
+// threads x NPT holds tile_nodes (so (512, 2048) with tile_nodes <= 512 takes the
+// one-node-per-lane kernel, with fewer registers).
+int shape_npt(const fw_tuning& t) {
+    int npt = 0;
+#define FW_NPT1(B, S, N, P, NS) \
+    if (t.block == B && t.tile_slots == S && t.tile_nodes <= P * B && (npt == 0 || P < npt)) npt = P;
+    FW_SHAPES(FW_NPT1)
+#undef FW_NPT1
+    return npt;
+}
+
 int tuning_of(const fw_tuning* tuning, fw_tuning* out) {
     fw_tuning t{};
     t.tile_nodes = 512;
@@ -1227,16 +1258,16 @@ int tuning_of(const fw_tuning* tuning, fw_tuning* out) {
     t.block = 256;
     t.grid_per_cu = 0;
     if (tuning) {
-        int npt = 0;
+        int npt_max = 0;
 #define FW_NPT(B, S, N, P, NS) \
-    if (tuning->block == B && tuning->tile_slots == S) npt = P;
+    if (tuning->block == B && tuning->tile_slots == S) npt_max = std::max(npt_max, P);
         FW_SHAPES(FW_NPT)
 #undef FW_NPT
-        if (!npt)
+        if (!npt_max)
             return fail(FW_E_INVALID, "unsupported (block, tile_slots); compiled shapes: (256,1024) (256,768) "
                                       "(512,1024) (256,512) (512,2048) (1024,2048)");
-        if (tuning->tile_nodes < 1 || tuning->tile_nodes > npt * tuning->block)
-            return fail(FW_E_INVALID, "tile_nodes must be in [1, " + std::to_string(npt * tuning->block) + "]");
+        if (tuning->tile_nodes < 1 || tuning->tile_nodes > npt_max * tuning->block)
+            return fail(FW_E_INVALID, "tile_nodes must be in [1, " + std::to_string(npt_max * tuning->block) + "]");
         if (tuning->grid_per_cu < 0) return fail(FW_E_INVALID, "grid_per_cu must be >= 0");
         t = *tuning;
     }
@@ -1334,7 +1365,7 @@ int32_t fw_arena_get_info(fw_arena* a, fw_arena_info* out) {
     out->algo_bytes_per_substep = algo;
     out->n_cu = a->n_cu;
     HIPCHK(hipSetDevice(a->device));
-    HIPCHK(fw_substep_occupancy(a->tune.block, a->tune.tile_slots, &out->tiles_per_cu));
+    HIPCHK(fw_substep_occupancy(a->tune.block, a->tune.tile_slots, shape_npt(a->tune), &out->tiles_per_cu));
     return FW_OK;
 }
 
@@ -1535,12 +1566,12 @@ static hipError_t launch_all(fw_arena* a, int k, float dt_override, float dampin
     }
     if (grid_tri > 0) {
         hipError_t e = fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
-                                         grid_tri, a->tune.block, a->tune.tile_slots, 0, a->stream);
+                                         grid_tri, a->tune.block, a->tune.tile_slots, shape_npt(a->tune), 0, a->stream);
         if (e != hipSuccess) return e;
     }
     if (grid_spr > 0)
         return fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
-                                 grid_spr, a->tune.block, a->tune.tile_slots, 1, a->stream);
+                                 grid_spr, a->tune.block, a->tune.tile_slots, shape_npt(a->tune), 1, a->stream);
     return hipSuccess;
 }
 
@@ -1563,7 +1594,7 @@ static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_ov
                     if (!sp || !sp->live) continue;
                     const int cnt = (int)sp->tiles.size();
                     HIPCHK(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, sp->tile_base, cnt, dt_override, damping_override, f,
-                                             launch_grid(a, cnt), a->tune.block, a->tune.tile_slots,
+                                             launch_grid(a, cnt), a->tune.block, a->tune.tile_slots, shape_npt(a->tune),
                                              sp->mat_type == FW_MAT_SPRING ? 1 : 0, a->stream));
                 }
             } else {
@@ -1630,7 +1661,8 @@ int32_t fw_step_noalloc(fw_arena* a, int32_t n, float dt_override, float damping
                 const int cnt = (int)sp->tiles.size();
                 HIPCHK_NOALLOC(fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, sp->tile_base, cnt, dt_override,
                                                  damping_override, f, launch_grid(a, cnt), a->tune.block,
-                                                 a->tune.tile_slots, sp->mat_type == FW_MAT_SPRING ? 1 : 0, a->stream));
+                                                 a->tune.tile_slots, shape_npt(a->tune),
+                                                 sp->mat_type == FW_MAT_SPRING ? 1 : 0, a->stream));
             }
         } else {
             HIPCHK_NOALLOC(launch_all(a, k, dt_override, damping_override, f, n_tri ? launch_grid(a, n_tri) : 0,

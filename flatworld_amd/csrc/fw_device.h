@@ -339,6 +339,9 @@ __device__ __forceinline__ bool ground_force(const GroundDev& g, float px, float
     }
     float n0, n1, sd;
     if (g.kind == 2) {
+        // above the highest sample plus the rounding margin (below the lowest, reversed),
+        // the signed distance is >= 0 or NaN: no contact, without the query's loads
+        if (g.reverse == 1 ? py < g.b[0] : py > g.b[0]) return false;
         sd = hf_signed(g, px, py, n0, n1);
         if (!(sd < 0.0f)) return false;
         float c0 = ((-n0) * k) * sd;
@@ -390,20 +393,35 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// One DPP step of a wave reduction: v combined with the value DPP control CTRL moves
+// into each lane of the rows ROW_MASK enables (lanes without a source keep v).
+template <int CTRL, int ROW_MASK, bool IS_MAX>
+__device__ __forceinline__ int32_t dpp_step(int32_t v) {
+    const int32_t o = __builtin_amdgcn_update_dpp(v, v, CTRL, ROW_MASK, 0xF, false);
+    return IS_MAX ? max(v, o) : min(v, o);
+}
+
+template <bool IS_MAX>
+__device__ __forceinline__ int32_t wave_reduce(int32_t v) {
+    // inclusive scan inside each row of 16 lanes (row_shr 1, 2, 4, 8), then lane 15 of
+    // rows 0 and 2 into rows 1 and 3 (row_bcast 15), then lane 31 into rows 2 and 3
+    // (row_bcast 31): lane 63 holds the result, read back as a uniform value
+    v = dpp_step<0x111, 0xF, IS_MAX>(v);
+    v = dpp_step<0x112, 0xF, IS_MAX>(v);
+    v = dpp_step<0x114, 0xF, IS_MAX>(v);
+    v = dpp_step<0x118, 0xF, IS_MAX>(v);
+    v = dpp_step<0x142, 0xA, IS_MAX>(v);
+    v = dpp_step<0x143, 0xC, IS_MAX>(v);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Min of v[0..1] and max of v[2..3] over the 64 lanes of a wave (ordered ints);
 // every lane ends with the result.
 __device__ __forceinline__ void wave_minmax(int32_t v[4]) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const int32_t a0 = __shfl_xor(v[0], off, 64);
-        const int32_t a1 = __shfl_xor(v[1], off, 64);
-        const int32_t a2 = __shfl_xor(v[2], off, 64);
-        const int32_t a3 = __shfl_xor(v[3], off, 64);
-        v[0] = min(v[0], a0);
-        v[1] = min(v[1], a1);
-        v[2] = max(v[2], a2);
-        v[3] = max(v[3], a3);
-    }
+    v[0] = wave_reduce<false>(v[0]);
+    v[1] = wave_reduce<false>(v[1]);
+    v[2] = wave_reduce<true>(v[2]);
+    v[3] = wave_reduce<true>(v[3]);
 }
 
 template <int BLOCK>
@@ -411,17 +429,7 @@ __device__ __forceinline__ void block_minmax(int32_t v[4], int32_t* red) {
     // v: lb.x, lb.y (min), ub.x, ub.y (max) as ordered ints
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        int32_t a0 = __shfl_xor(v[0], off, 64);
-        int32_t a1 = __shfl_xor(v[1], off, 64);
-        int32_t a2 = __shfl_xor(v[2], off, 64);
-        int32_t a3 = __shfl_xor(v[3], off, 64);
-        v[0] = min(v[0], a0);
-        v[1] = min(v[1], a1);
-        v[2] = max(v[2], a2);
-        v[3] = max(v[3], a3);
-    }
+    wave_minmax(v);
     if (lane == 0) {
         red[wid * 4 + 0] = v[0];
         red[wid * 4 + 1] = v[1];

@@ -121,6 +121,9 @@ __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, f
 }
 
 // Corner words of a spring's slot record (.x: both ends, .y: the rest length's bits).
+// sign bit of a spring incidence entry: the entry's node takes +f (node b of the spring)
+constexpr uint32_t SPRING_NEG = 0x8000u;
+
 __device__ __forceinline__ void unpack_spring(uint2 r, int n_own_nodes, const uint16_t* lds_cp, uint32_t l[2],
                                               uint32_t q[2], float& rest) {
     const uint32_t w[2] = {r.x & 0xFFFFu, r.x >> 16};
@@ -136,9 +139,11 @@ __device__ __forceinline__ void unpack_spring(uint2 r, int n_own_nodes, const ui
 // One spring slot (flatworld/femspringmanager.py:283-312): xab = x_a - x_b,
 // lnew = |xab| (Warp's dot: a zero-initialised sum), dir = xab / lnew,
 // f = k (lnew / l0 - 1) dir + c (v_ab . dir) dir.  Node a receives -f and node b
-// +f; the node sums subtract their entries, so a's entry holds f and b's -f.
+// +f.  The force is stored once, at the slot; each corner's incidence entry holds the
+// slot with a sign bit (a: +, b: -), and the node sums subtract the signed force, so
+// a's term is -f and b's is +f, in ascending spring id.
 __device__ __forceinline__ void spring_slot(float k, float c, float2 pa, float2 pb, float2 va, float2 vb, float l0,
-                                            const uint32_t q[2], float2* lds_f) {
+                                            const uint32_t q[2], int sl, float2* lds_f, uint16_t* lds_inc) {
     const float xab0 = pa.x - pb.x, xab1 = pa.y - pb.y;
     const float lnew = sqrtf((0.0f + xab0 * xab0) + xab1 * xab1);
     const float d0 = xab0 / lnew, d1 = xab1 / lnew;
@@ -148,8 +153,9 @@ __device__ __forceinline__ void spring_slot(float k, float c, float2 pa, float2 
     const float b = c * ((0.0f + vab0 * d0) + vab1 * d1);
     f0 = f0 + b * d0;
     f1 = f1 + b * d1;
-    if (q[0] != 0xFFFFu) lds_f[q[0]] = make_float2(f0, f1);
-    if (q[1] != 0xFFFFu) lds_f[q[1]] = make_float2(-f0, -f1);
+    lds_f[sl] = make_float2(f0, f1);
+    if (q[0] != 0xFFFFu) lds_inc[q[0]] = (uint16_t)sl;
+    if (q[1] != 0xFFFFu) lds_inc[q[1]] = (uint16_t)(sl | SPRING_NEG);
 }
 
 // ------------------------------------------------------------------ substep kernel
@@ -270,11 +276,15 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     constexpr int NRED = (BLOCK / 64) * 4;
     __shared__ float2 lds_x[NODES];
     __shared__ float2 lds_X[NODES];
-    __shared__ float2 lds_f[SLOTS * CORNERS];
+    // triangles: one force per (slot, corner) at the corner's incidence entry; springs: one
+    // force per slot and a 16-bit (slot, sign) per incidence entry (half the LDS)
+    __shared__ float2 lds_f[SLOTS * (spring ? 1 : CORNERS)];
+    __shared__ uint16_t lds_inc[spring ? SLOTS * 2 : 1];
     __shared__ uint16_t lds_cp[NPT * BLOCK + 1];
     __shared__ int32_t lds_red_spr[spring ? NRED : 1];
     static_assert(NRED * sizeof(int32_t) <= NODES_TRI * sizeof(float2), "reduction scratch");
     static_assert(NODES <= (int)FW_REC_ID_MASK + 1, "local node ids fit the record");
+    static_assert(!spring || SLOTS <= (int)SPRING_NEG, "spring slot ids fit an incidence entry");
     int32_t* lds_red = spring ? lds_red_spr : reinterpret_cast<int32_t*>(lds_X);
     float2* const lds_vn = lds_X;  // spring tiles, phase B on: new velocities of the owned nodes
 
@@ -399,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                     uint32_t l[2], q[2];
                     float l0;
                     unpack_spring(st.rc[k], n_nodes, lds_cp, l, q, l0);
-                    spring_slot(m.lam, m.mu, lds_x[l[0]], lds_x[l[1]], lds_X[l[0]], lds_X[l[1]], l0, q, lds_f);
+                    spring_slot(m.lam, m.mu, lds_x[l[0]], lds_x[l[1]], lds_X[l[0]], lds_X[l[1]], l0, q, sl, lds_f, lds_inc);
                 } else {
                     uint32_t l[3], q[3];
                     unpack_rec(st.rc[k], n_nodes, lds_cp, l, q);
@@ -496,17 +506,29 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             // (two entries per iteration, so that their LDS reads overlap; same order)
             int j = lds_cp[i];
             const int j_end = lds_cp[i + 1];
-            for (; j + 1 < j_end; j += 2) {
-                const float2 f0 = lds_f[j], f1 = lds_f[j + 1];
-                fx = fx + (-f0.x);
-                fy = fy + (-f0.y);
-                fx = fx + (-f1.x);
-                fy = fy + (-f1.y);
-            }
-            if (j < j_end) {
-                const float2 f = lds_f[j];
-                fx = fx + (-f.x);
-                fy = fy + (-f.y);
+            if constexpr (spring) {
+                // the entry's sign bit moved to bit 31 flips the force's sign bits: the
+                // float negation, exactly
+                for (; j < j_end; ++j) {
+                    const uint32_t e = lds_inc[j];
+                    const float2 f = lds_f[e & ~SPRING_NEG];
+                    const uint32_t sg = (e & SPRING_NEG) << 16;
+                    fx = fx + (-__uint_as_float(__float_as_uint(f.x) ^ sg));
+                    fy = fy + (-__uint_as_float(__float_as_uint(f.y) ^ sg));
+                }
+            } else {
+                for (; j + 1 < j_end; j += 2) {
+                    const float2 f0 = lds_f[j], f1 = lds_f[j + 1];
+                    fx = fx + (-f0.x);
+                    fy = fy + (-f0.y);
+                    fx = fx + (-f1.x);
+                    fy = fy + (-f1.y);
+                }
+                if (j < j_end) {
+                    const float2 f = lds_f[j];
+                    fx = fx + (-f.x);
+                    fy = fy + (-f.y);
+                }
             }
             // symplectic Euler with BC precedence (flatworld/femspringmanager.py:315-358)
             if (fl & BC_U) {
@@ -779,14 +801,15 @@ __global__ void advance_clock_kernel(ArenaCtl* ctl, int n) {
 
 // Kernel shapes (FW_SHAPES): (threads, element slots, LDS nodes, owned nodes per
 // thread).  The host keeps tile_nodes <= NPT * threads and the LDS node count of a
-// tile <= the shape's LDS nodes.
+// tile <= the shape's LDS nodes; a shape is named by (threads, slots, NPT).
 // kind: 0 triangle tiles, 1 spring tiles.
 extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* tiles, const ArenaCtl* ctl, int k,
                                         int tile_base, int tile_count, float dt_override, float damping_override,
-                                        uint32_t flags, int grid, int block, int slots, int kind, hipStream_t stream) {
+                                        uint32_t flags, int grid, int block, int slots, int npt, int kind,
+                                        hipStream_t stream) {
     fw::StepArgs A{scenes, tiles, ctl, k, tile_base, tile_count, dt_override, damping_override, flags};
 #define FW_LAUNCH(B, S, N, P, NS)                                                                                    \
-    if (block == B && slots == S) {                                                                             \
+    if (block == B && slots == S && npt == P) {                                                                 \
         if (kind == fw::KIND_SPRING)                                                                            \
             hipLaunchKernelGGL((fw::substep_kernel<B, S, N, P, NS, fw::KIND_SPRING>), dim3(grid), dim3(B), 0, stream, A); \
         else                                                                                                    \
@@ -800,10 +823,10 @@ extern "C" hipError_t fw_launch_substep(const SceneDev* scenes, const TileDev* t
 
 // Workgroups of the substep kernel resident per CU for a shape (HIP's occupancy
 // calculator: LDS, registers and wave slots).
-extern "C" hipError_t fw_substep_occupancy(int block, int slots, int* per_cu) {
+extern "C" hipError_t fw_substep_occupancy(int block, int slots, int npt, int* per_cu) {
     *per_cu = 0;
 #define FW_OCC(B, S, N, P, NS) \
-    if (block == B && slots == S)  \
+    if (block == B && slots == S && npt == P)  \
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, fw::substep_kernel<B, S, N, P, NS, fw::KIND_TRI>, B, 0);
     FW_SHAPES(FW_OCC)
 #undef FW_OCC

@@ -35,7 +35,9 @@ struct GroundDev {
     float a[4];            // plane: p.x p.y n.x n.y | HF: lb_x ub_x | voxel: lb.x lb.y dx dz
     int32_t nx, ny, reverse;
     float hf_dxw;          // HF: sample spacing (ub_x - lb_x) / (nx - 1), rounded once in float32 on the host
-    float b[8];            // rigid: rotation (row-major, 4), velocity (2), angular velocity, friction;
+    float b[8];            // HF: b[0] the skip level (no node above it, below it when reversed, can
+                           // touch the field);
+                           // rigid: rotation (row-major, 4), velocity (2), angular velocity, friction;
                            // with a[0..1] the centre, a[2..3] the primary parameters, nx the
                            // type, ny the tied flag, hf_dxw the radius, reverse the scene's
                            // boundary-node count, lo / hi the rigid's AABB; data: float4 per node
@@ -255,6 +257,7 @@ struct TileDev {
     X(512, 1024, 952, 1, 448)     \
     X(256, 512, 472, 1, 256)      \
     X(512, 2048, 1912, 2, 768)    \
+    X(512, 2048, 1912, 1, 768)    \
     X(1024, 2048, 1912, 1, 768)
 
 // HBM accesses of the kernels go through fw_ld / fw_st: they address global

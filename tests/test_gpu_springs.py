@@ -69,7 +69,7 @@ def test_spring_lattice_multitile_heightfield():
 
 
 @pytest.mark.parametrize("shape", [(256, 512, 256), (256, 768, 512), (256, 1024, 512), (512, 1024, 512),
-                                   (512, 2048, 1024), (1024, 2048, 1024)])
+                                   (512, 2048, 1024), (512, 2048, 512), (1024, 2048, 1024)])
 def test_spring_every_kernel_shape(shape):
     block, slots, nodes = shape
     spec = _lattice_spec(36, "plane", jitter=0.2, seed=2)
