@@ -120,42 +120,38 @@ __device__ __forceinline__ void slot_force(const Mat& m, float2 p1, float2 p2, f
     }
 }
 
-// Corner words of a spring's slot record (.x: both ends, .y: the rest length's bits).
 // sign bit of a spring incidence entry: the entry's node takes +f (node b of the spring)
 constexpr uint32_t SPRING_NEG = 0x8000u;
 
-__device__ __forceinline__ void unpack_spring(uint2 r, int n_own_nodes, const uint16_t* lds_cp, uint32_t l[2],
-                                              uint32_t q[2], float& rest) {
-    const uint32_t w[2] = {r.x & 0xFFFFu, r.x >> 16};
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        l[c] = w[c] & FW_REC_ID_MASK;
-        const uint32_t base = lds_cp[min((int)l[c], n_own_nodes)];
-        q[c] = (int)l[c] < n_own_nodes ? base + (w[c] >> FW_REC_ID_BITS) : 0xFFFFu;
-    }
-    rest = __uint_as_float(r.y);
-}
-
-// One spring slot (flatworld/femspringmanager.py:283-312): xab = x_a - x_b,
-// lnew = |xab| (Warp's dot: a zero-initialised sum), dir = xab / lnew,
+// One spring slot (flatworld/femspringmanager.py:283-312).  Its record: .x the two
+// corner words (local node id, incidence rank), .y the rest length's bits.
+// xab = x_a - x_b, lnew = |xab| (Warp's dot: a zero-initialised sum), dir = xab / lnew,
 // f = k (lnew / l0 - 1) dir + c (v_ab . dir) dir.  Node a receives -f and node b
 // +f.  The force is stored once, at the slot; each corner's incidence entry holds the
 // slot with a sign bit (a: +, b: -), and the node sums subtract the signed force, so
-// a's term is -f and b's is +f, in ascending spring id.
-__device__ __forceinline__ void spring_slot(float k, float c, float2 pa, float2 pb, float2 va, float2 vb, float l0,
-                                            const uint32_t q[2], int sl, float2* lds_f, uint16_t* lds_inc) {
+// a's term is -f and b's is +f, in ascending spring id.  The incidence offsets are
+// read unconditionally (clamped id) and a halo corner writes the spare entry
+// `spare`, so that the LDS reads of a slot issue together.
+__device__ __forceinline__ void spring_slot(float k, float c, uint2 r, int n_own, const float2* lds_x,
+                                            const float2* lds_v, const uint16_t* lds_cp, int sl, int spare,
+                                            float2* lds_f, uint16_t* lds_inc) {
+    const uint32_t w0 = r.x & 0xFFFFu, w1 = r.x >> 16;
+    const int l0 = (int)(w0 & FW_REC_ID_MASK), l1 = (int)(w1 & FW_REC_ID_MASK);
+    const float2 pa = lds_x[l0], pb = lds_x[l1], va = lds_v[l0], vb = lds_v[l1];
+    const uint32_t c0 = lds_cp[min(l0, n_own)], c1 = lds_cp[min(l1, n_own)];
+    const float l0r = __uint_as_float(r.y);
     const float xab0 = pa.x - pb.x, xab1 = pa.y - pb.y;
     const float lnew = sqrtf((0.0f + xab0 * xab0) + xab1 * xab1);
     const float d0 = xab0 / lnew, d1 = xab1 / lnew;
-    const float a = k * (lnew / l0 - 1.0f);
+    const float a = k * (lnew / l0r - 1.0f);
     float f0 = a * d0, f1 = a * d1;
     const float vab0 = va.x - vb.x, vab1 = va.y - vb.y;
     const float b = c * ((0.0f + vab0 * d0) + vab1 * d1);
     f0 = f0 + b * d0;
     f1 = f1 + b * d1;
     lds_f[sl] = make_float2(f0, f1);
-    if (q[0] != 0xFFFFu) lds_inc[q[0]] = (uint16_t)sl;
-    if (q[1] != 0xFFFFu) lds_inc[q[1]] = (uint16_t)(sl | SPRING_NEG);
+    lds_inc[l0 < n_own ? (int)(c0 + (w0 >> FW_REC_ID_BITS)) : spare] = (uint16_t)sl;
+    lds_inc[l1 < n_own ? (int)(c1 + (w1 >> FW_REC_ID_BITS)) : spare] = (uint16_t)(sl | SPRING_NEG);
 }
 
 // ------------------------------------------------------------------ substep kernel
@@ -279,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
     // triangles: one force per (slot, corner) at the corner's incidence entry; springs: one
     // force per slot and a 16-bit (slot, sign) per incidence entry (half the LDS)
     __shared__ float2 lds_f[SLOTS * (spring ? 1 : CORNERS)];
-    __shared__ uint16_t lds_inc[spring ? SLOTS * 2 : 1];
+    __shared__ uint16_t lds_inc[spring ? SLOTS * 2 + 2 : 1];  // + a spare entry for halo corners
     __shared__ uint16_t lds_cp[NPT * BLOCK + 1];
     __shared__ int32_t lds_red_spr[spring ? NRED : 1];
     static_assert(NRED * sizeof(int32_t) <= NODES_TRI * sizeof(float2), "reduction scratch");
@@ -406,10 +402,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             }
             if (sl < nslots) {
                 if constexpr (spring) {
-                    uint32_t l[2], q[2];
-                    float l0;
-                    unpack_spring(st.rc[k], n_nodes, lds_cp, l, q, l0);
-                    spring_slot(m.lam, m.mu, lds_x[l[0]], lds_x[l[1]], lds_X[l[0]], lds_X[l[1]], l0, q, sl, lds_f, lds_inc);
+                    spring_slot(m.lam, m.mu, st.rc[k], n_nodes, lds_x, lds_X, lds_cp, sl, SLOTS * 2, lds_f, lds_inc);
                 } else {
                     uint32_t l[3], q[3];
                     unpack_rec(st.rc[k], n_nodes, lds_cp, l, q);
@@ -509,7 +502,16 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
             if constexpr (spring) {
                 // the entry's sign bit moved to bit 31 flips the force's sign bits: the
                 // float negation, exactly
-                for (; j < j_end; ++j) {
+                for (; j + 1 < j_end; j += 2) {
+                    const uint32_t e0 = lds_inc[j], e1 = lds_inc[j + 1];
+                    const float2 f0 = lds_f[e0 & ~SPRING_NEG], f1 = lds_f[e1 & ~SPRING_NEG];
+                    const uint32_t s0 = (e0 & SPRING_NEG) << 16, s1 = (e1 & SPRING_NEG) << 16;
+                    fx = fx + (-__uint_as_float(__float_as_uint(f0.x) ^ s0));
+                    fy = fy + (-__uint_as_float(__float_as_uint(f0.y) ^ s0));
+                    fx = fx + (-__uint_as_float(__float_as_uint(f1.x) ^ s1));
+                    fy = fy + (-__uint_as_float(__float_as_uint(f1.y) ^ s1));
+                }
+                if (j < j_end) {
                     const uint32_t e = lds_inc[j];
                     const float2 f = lds_f[e & ~SPRING_NEG];
                     const uint32_t sg = (e & SPRING_NEG) << 16;
