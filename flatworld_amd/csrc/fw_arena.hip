@@ -46,7 +46,7 @@ extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, c
                                           const ArenaCtl* ctl, int row, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_items, const SceneDev* scenes,
                                           const ArenaCtl* ctl, hipStream_t stream);
-extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
+extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int2* item_map,
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
                                              uint32_t flags, char* scr, hipStream_t stream);
 extern "C" hipError_t fw_flex_noalloc_scratch(int device, char** out);
@@ -717,8 +717,8 @@ struct fw_arena {
     std::vector<std::unique_ptr<FlexHost>> flex;
     std::vector<int> flex_row;
     FlexDev* d_flex = nullptr;
-    int32_t* d_flex_items = nullptr;  // first global work item of each table row (+ the total)
-    int flex_cap = 0, n_flex_live = 0, n_flex_items = 0;
+    int2* d_flex_items = nullptr;  // (table row, work item) of every global work item
+    int flex_cap = 0, n_flex_live = 0, n_flex_items = 0, flex_items_cap = 0;
     int64_t flex_scratch_bytes = 0;  // compact per-substep scratch of the live groups (fw_step_noalloc)
     // (scene, record) of every rigid contact record of the live scenes, for the rigid pass
     int2* d_rigid = nullptr;
@@ -918,18 +918,26 @@ int ensure_flex(fw_arena* a) {
     if (n > a->flex_cap) {
         const int cap = std::max(n, std::max(8, 2 * a->flex_cap));
         FlexDev* nd = nullptr;
-        int32_t* ni = nullptr;
         HIPCHK(hipStreamSynchronize(a->stream));
         HIPCHK(hipMalloc(&nd, sizeof(FlexDev) * (size_t)cap));
-        HIPCHK(hipMalloc(&ni, sizeof(int32_t) * (size_t)(cap + 1)));
         if (a->d_flex) HIPCHK(hipFree(a->d_flex));
-        if (a->d_flex_items) HIPCHK(hipFree(a->d_flex_items));
         a->d_flex = nd;
-        a->d_flex_items = ni;
         a->flex_cap = cap;
     }
     std::vector<int32_t> item_off(n + 1, 0);
     for (int r = 0; r < n; ++r) item_off[r + 1] = item_off[r] + rows[r].n_items;
+    if (item_off[n] > a->flex_items_cap) {
+        const int cap = std::max(item_off[n], 2 * a->flex_items_cap);
+        int2* ni = nullptr;
+        HIPCHK(hipStreamSynchronize(a->stream));
+        HIPCHK(hipMalloc(&ni, sizeof(int2) * (size_t)cap));
+        if (a->d_flex_items) HIPCHK(hipFree(a->d_flex_items));
+        a->d_flex_items = ni;
+        a->flex_items_cap = cap;
+    }
+    std::vector<int2> item_map((size_t)item_off[n]);
+    for (int r = 0; r < n; ++r)
+        for (int w = 0; w < rows[r].n_items; ++w) item_map[(size_t)item_off[r] + w] = make_int2(r, w);
     int64_t scr = 0;  // the compact per-substep scratch layout of fw_step_noalloc
     for (int r = 0; r < n; ++r) {
         rows[r].scr_off = scr;
@@ -938,8 +946,10 @@ int ensure_flex(fw_arena* a) {
     a->flex_scratch_bytes = scr;
     int rc = upload(a, a->d_flex, rows.data(), (int64_t)sizeof(FlexDev) * n);
     if (rc) return rc;
-    rc = upload(a, a->d_flex_items, item_off.data(), (int64_t)sizeof(int32_t) * (n + 1));
-    if (rc) return rc;
+    if (item_off[n] > 0) {
+        rc = upload(a, a->d_flex_items, item_map.data(), (int64_t)sizeof(int2) * item_off[n]);
+        if (rc) return rc;
+    }
     a->n_flex_live = n;
     a->n_flex_items = item_off[n];
     a->flex_dirty = false;
@@ -2157,6 +2167,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
         o += align_up(std::max<int64_t>(bytes, 4));
         return r;
     };
+    const int64_t o_bodies = take((int64_t)sizeof(FlexBody) * B);
     const int64_t o_scene = take(4 * B), o_soff = take(4 * (B + 1)), o_sloff = take(4 * (B + 1)), o_flex = take(8 * B),
                   o_nn = take(4 * B), o_snode = take(4 * NS), o_sbody = take(4 * NS), o_segb = take(4 * S),
                   o_segn = take(8 * S), o_segs = take(8 * S), o_ib = take(4 * W), o_in = take(4 * W),
@@ -2164,7 +2175,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
                   o_ifr = take(4 * W), o_cache = take(4 * W), o_pair = take(8 * P), o_tied = take(4 * W),
                   o_tseg = take(4 * W);
     const int64_t host_bytes = o;
-    const int64_t o_grid = take(sizeof(FlexGrid)), o_box = take(16 * S), o_scell = take(16 * S), o_segp = take(4 * S),
+    const int64_t o_grid = take(sizeof(FlexGrid)), o_box = take(16 * S), o_scell = take(16 * S), o_bcell = take(16 * B), o_segp = take(4 * S),
                   o_crng = take(8 * (int64_t)d->max_cells),
                   o_cs = take(4 * (int64_t)d->max_cells), o_ce = take(4 * (int64_t)d->max_cells),
                   o_sorted = take(4 * (int64_t)d->max_pairs), o_cent = take(4 * (int64_t)d->max_pairs),
@@ -2181,6 +2192,12 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     };
     std::vector<char*> flex_dev(B);
     for (int b = 0; b < B; ++b) flex_dev[b] = body_flex[b];
+    std::vector<FlexBody> bodies(B);
+    for (int b = 0; b < B; ++b) {
+        const SceneDev& SD = sc[b]->dev;
+        bodies[b] = FlexBody{{SD.pos[0], SD.pos[1]}, SD.vel, SD.hot.aabb, SD.hot.aabb_k, 0};
+    }
+    put(o_bodies, bodies.data(), (int64_t)sizeof(FlexBody) * B);
     put(o_scene, d->scene_ids, 4 * B);
     put(o_soff, d->body_seg_off, 4 * (B + 1));
     put(o_sloff, body_slot_off.data(), 4 * (B + 1));
@@ -2231,6 +2248,9 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     D.max_query = d->max_query;
     D.max_pairs = d->max_pairs;
     D.n_tied = n_tied;
+    D.bodies = (const FlexBody*)(blk + o_bodies);
+    D.s0 = sc[0]->s0;
+    D.horizon = sc[0]->horizon;
     D.body_scene = (const int32_t*)(blk + o_scene);
     D.body_seg_off = (const int32_t*)(blk + o_soff);
     D.body_slot_off = (const int32_t*)(blk + o_sloff);
@@ -2256,6 +2276,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     D.grid = (FlexGrid*)(blk + o_grid);
     D.seg_box = (float4*)(blk + o_box);
     D.seg_cells = (int4*)(blk + o_scell);
+    D.body_cells = (int4*)(blk + o_bcell);
     D.cell_rng = (int2*)(blk + o_crng);
     D.seg_pairs = (int32_t*)(blk + o_segp);
     D.cell_start = (int32_t*)(blk + o_cs);

@@ -22,6 +22,7 @@
 // then sorted.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <mutex>
 
 #include "flatworld_fem.h"
@@ -125,11 +126,11 @@ __device__ __forceinline__ CellRange cell_range(const FlexGrid& g, float lbx, fl
     return r;
 }
 
-// The boundary AABB a scene's gate reads at `local`: min / max over the sub-slots of
+// The boundary AABB a body's gate reads at `local`: min / max over the sub-slots of
 // buffer ar3 (ordered-int encoded).
-__device__ __forceinline__ void scene_aabb(const SceneDev* S, int ar3, float bb[4]) {
-    const int kk = S->hot.aabb_k;
-    const int32_t* e = S->hot.aabb + ar3 * kk * FW_AABB_SUB_INTS;
+__device__ __forceinline__ void scene_aabb(const FlexBody& B, int ar3, float bb[4]) {
+    const int kk = B.aabb_k;
+    const int32_t* e = B.aabb + ar3 * kk * FW_AABB_SUB_INTS;
     int32_t r[4] = {e[0], e[1], e[2], e[3]};
     for (int j = 1; j < kk; ++j) {
         const int32_t* f = e + j * FW_AABB_SUB_INTS;
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
         float l0 = 1e9f, l1 = 1e9f, u0 = -1e9f, u1 = -1e9f;
         for (int b = 0; b < F.n_bodies; ++b) {
             float bb[4];
-            scene_aabb(scenes + F.body_scene[b], ar3, bb);
+            scene_aabb(F.bodies[b], ar3, bb);
             l0 = fmin_(l0, bb[0]);
             l1 = fmin_(l1, bb[1]);
             u0 = fmax_(u0, bb[2]);
@@ -264,11 +265,18 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
     }
 
     // pairs per segment, then their serial index base (pairs past max_pairs are
-    // dropped in that order)
+    // dropped in that order); each body's union of segment cell ranges
+    for (int b = tid; b < F.n_bodies; b += FLEX_BLOCK) F.body_cells[b] = make_int4(INT_MAX, INT_MAX, INT_MIN, INT_MIN);
+    phase_barrier();
     for (int s = tid; s < ns; s += FLEX_BLOCK) {
         const float4 box = F.seg_box[s];
         const CellRange r = cell_range(g, box.x, box.y, box.z, box.w);
         F.seg_cells[s] = make_int4(r.lx, r.ly, r.ux, r.uy);
+        int* bc = reinterpret_cast<int*>(F.body_cells + F.seg_body[s]);
+        atomicMin(bc + 0, r.lx);
+        atomicMin(bc + 1, r.ly);
+        atomicMax(bc + 2, r.ux);
+        atomicMax(bc + 3, r.uy);
         int cnt = 0;
         for (int ix = r.lx; ix <= r.ux; ++ix)
             for (int iy = r.ly; iy <= r.uy; ++iy) cnt += ix + iy * g.cn[0] < g.tc;
@@ -426,14 +434,41 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
     }
 }
 
+// The same candidates as walk() on an exact grid, found from the target body's
+// segments instead of the query's cells: with every segment listed in every cell of
+// its (clamped) cell box, the walk meets segment s exactly when that box meets the
+// query's cells, first at the cell (max(box.lx, q.lx), max(box.ly, q.ly)), walk
+// position t, and inside a cell the entries come in ascending segment, so the key
+// (t, s) orders candidates as the walk's (t, entry) does.  Lanes take every
+// stride-th segment of the body; `met` counts the segments met.
+__device__ void scan_segments(const FlexDev& F, const CellRange& q, float2 p, int be_off, int be_cnt,
+                              const float2* tpos, float qbuf, int lane, int stride, int& met, Best& best) {
+    const int nyq = q.uy - q.ly + 1;
+    for (int j = lane; j < be_cnt; j += stride) {
+        const int s = be_off + j;
+        const int4 r = F.seg_cells[s];
+        if (r.x > q.ux || r.z < q.lx || r.y > q.uy || r.w < q.ly) continue;
+        ++met;
+        const int t = (imax_(r.x, q.lx) - q.lx) * nyq + (imax_(r.y, q.ly) - q.ly);
+        const long long key = ((long long)t << 32) | (long long)s;
+        const int2 sn = F.seg_node[s];
+        float nx, ny, w0, w1;
+        const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
+        if (pen < 1e-4f && fabsf(pen) < qbuf && (pen < best.pen || (pen == best.pen && key < best.key)))
+            best.take(pen, nx, ny, w0, w1, s, key);
+    }
+}
+
 // Pair activation: the BVH overlap test (flatworld/explicitloop.py:477-487), or
 // every pair when the loop skips its BVH.
-__device__ __forceinline__ bool pair_active(const FlexDev& F, const SceneDev* scenes, int q, int ar3, bool pad) {
+__device__ __forceinline__ bool pair_active(const FlexDev& F, const SceneDev* scenes, int q, int local) {
     if (!F.gated) return true;
+    const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
+    const int ar3 = mod3_(local);
     const int2 pb = F.pair_body[q];
     float a[4], b[4];
-    scene_aabb(scenes + F.body_scene[pb.x], ar3, a);
-    scene_aabb(scenes + F.body_scene[pb.y], ar3, b);
+    scene_aabb(F.bodies[pb.x], ar3, a);
+    scene_aabb(F.bodies[pb.y], ar3, b);
     if (pad) {
         for (int c = 0; c < 2; ++c) {
             const float ba = 0.1f * (a[c + 2] - a[c]);
@@ -480,9 +515,9 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_ties_kernel(const FlexDev* gr
     const int rd = (ctl->clock - S0->s0 + 1) & 1;
     const FlexGrid g = *F.grid;
     const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.001f);
-    const float2 p = scenes[F.body_scene[F.item_body[w]]].pos[rd][F.item_node[w]];
+    const float2 p = F.bodies[F.item_body[w]].pos[rd][F.item_node[w]];
     const int tb = F.item_target[w];
-    const float2* tpos = scenes[F.body_scene[tb]].pos[rd];
+    const float2* tpos = F.bodies[tb].pos[rd];
     if (g.tc <= 0) return;
     const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
     int met = 0;
@@ -498,11 +533,9 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_ties_kernel(const FlexDev* gr
 }
 
 // The substep a group's contact pass serves, or -1 when it has none this launch.
-__device__ __forceinline__ int group_local(const FlexDev& F, const SceneDev* scenes, const ArenaCtl* ctl, int k,
-                                           uint32_t flags) {
-    const SceneDev* S0 = scenes + F.body_scene[0];
-    const int local = ctl->clock + k - S0->s0;
-    if (local < 0 || local >= S0->horizon || (flags & FLEX_NO_CONTACT)) return -1;
+__device__ __forceinline__ int group_local(const FlexDev& F, const ArenaCtl* ctl, int k, uint32_t flags) {
+    const int local = ctl->clock + k - F.s0;
+    if (local < 0 || local >= F.horizon || (flags & FLEX_NO_CONTACT)) return -1;
     return local;
 }
 
@@ -512,27 +545,20 @@ __device__ __forceinline__ int group_local(const FlexDev& F, const SceneDev* sce
 // group with long queries spread over many workgroups.  The lanes of an item
 // split its query walk and reduce; lane 0 does its stores and counts its
 // contributions per slot.
-__global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev* groups, const int32_t* item_off,
-                                                                   int n_groups, const SceneDev* scenes,
+__global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev* groups, const int2* item_map,
+                                                                   int total_items, const SceneDev* scenes,
                                                                    const ArenaCtl* ctl, int k, uint32_t flags,
                                                                    char* scr) {
     const int gw = (int)((blockIdx.x * FLEX_BLOCK + threadIdx.x) / FLEX_LPI);
     const int sub = threadIdx.x % FLEX_LPI;
-    if (gw >= item_off[n_groups]) return;
-    int lo = 0, hi = n_groups;  // item_off[lo] <= gw < item_off[lo + 1]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (item_off[mid] <= gw) lo = mid;
-        else hi = mid;
-    }
-    const FlexDev& F = groups[lo];
-    const int w = gw - item_off[lo];
-    const int local = group_local(F, scenes, ctl, k, flags);
+    if (gw >= total_items) return;
+    const int2 gi = item_map[gw];  // (group row, work item in the group)
+    const FlexDev& F = groups[gi.x];
+    const int w = gi.y;
+    const int local = group_local(F, ctl, k, flags);
     if (local < 0) return;
     const FlexScratch X = flex_scratch(F, scr);
     const int rd = (local + 1) & 1;
-    const int ar3 = mod3_(local);
-    const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
     int2 ri = make_int2(0, -1);
     if (F.n_tied > 0 && F.item_tied[w]) {
         // _apply_fem_fem_ties_fixed_kernel (flatworld/mixedcontact.py:773-811): a
@@ -541,9 +567,8 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
         const int ts = F.tie_seg[w];
         if (sub != 0) return;
         if (ts >= 0) {
-            const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
-            const float2 x = SB->pos[rd][F.item_node[w]];
-            const float2* tpos = scenes[F.body_scene[F.item_target[w]]].pos[rd];
+            const float2 x = F.bodies[F.item_body[w]].pos[rd][F.item_node[w]];
+            const float2* tpos = F.bodies[F.item_target[w]].pos[rd];
             const int2 sn = F.seg_node[ts];
             const float2 a = tpos[sn.x], c = tpos[sn.y];
             const float4 wg = F.tie_wg[w];
@@ -563,14 +588,14 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
     }
     const FlexGrid g = *F.grid;
     const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.016f);
-    if (pair_active(F, scenes, F.item_pair[w], ar3, pad)) {
-        const SceneDev* SB = scenes + F.body_scene[F.item_body[w]];
+    if (pair_active(F, scenes, F.item_pair[w], local)) {
+        const FlexBody& SB = F.bodies[F.item_body[w]];
         const int nid = F.item_node[w];
-        const float2 p = SB->pos[rd][nid];
-        const float2 vel = SB->vel[nid];
+        const float2 p = SB.pos[rd][nid];
+        const float2 vel = SB.vel[nid];
         const int tb = F.item_target[w];
-        const SceneDev* ST = scenes + F.body_scene[tb];
-        const float2* tpos = ST->pos[rd];
+        const FlexBody& ST = F.bodies[tb];
+        const float2* tpos = ST.pos[rd];
         const int be_off = F.body_seg_off[tb], be_cnt = F.body_seg_off[tb + 1] - be_off;
         const float penalty = F.item_penalty[w], friction = F.item_friction[w];
 
@@ -602,9 +627,16 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             }
         }
         int cache_new = cached;
-        if (!b.found && g.tc > 0) {
-            // query_point_with_buffer (flatworld/spatialmanager.py:131-199), walked in place
-            const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
+        // query_point_with_buffer (flatworld/spatialmanager.py:131-199), walked in place; a
+        // query whose cells miss every cell of the target body's segments meets none of them
+        CellRange q{0, 0, -1, -1};
+        bool reach = false;
+        if (g.tc > 0) {
+            q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
+            const int4 tbc = F.body_cells[tb];
+            reach = !(tbc.x > q.ux || tbc.z < q.lx || tbc.y > q.uy || tbc.w < q.ly);
+        }
+        if (!b.found && reach) {
             bool serial = !g.exact;
             if (g.exact) {
                 // the item's lanes take every FLEX_LPI-th cell of the walk; the best
@@ -613,7 +645,12 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
                 // are met (else the cap decides and the serial walk is repeated)
                 int met = 0;
                 Best lb;
-                walk<false>(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
+                // the shorter of the two: the query's cells or the target body's segments
+                const int ncell = (q.ux - q.lx + 1) * (q.uy - q.ly + 1);
+                if (be_cnt <= 2 * ncell)
+                    scan_segments(F, q, p, be_off, be_cnt, tpos, qbuf, sub, FLEX_LPI, met, lb);
+                else
+                    walk<false>(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
 #pragma unroll
                 for (int off = 1; off < FLEX_LPI; off <<= 1) met += __shfl_xor(met, off, FLEX_LPI);
                 if (met > F.max_query) {
@@ -660,7 +697,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             float tx = nfx, ty = nfy;
             if (friction > 1e-9f) {
                 const int2 sn = F.seg_node[b.s];
-                const float2 v0 = ST->vel[sn.x], v1 = ST->vel[sn.y];
+                const float2 v0 = ST.vel[sn.x], v1 = ST.vel[sn.y];
                 const float svx = v0.x * b.w0 + v1.x * b.w1;
                 const float svy = v0.y * b.w0 + v1.y * b.w1;
                 const float rvx = vel.x - svx, rvy = vel.y - svy;
@@ -696,7 +733,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_lists_kernel(const FlexDev* g
     const FlexDev& F = groups[blockIdx.x];
     __shared__ int lds[FLEX_BLOCK];
     const int tid = threadIdx.x;
-    if (group_local(F, scenes, ctl, k, flags) < 0) return;
+    if (group_local(F, ctl, k, flags) < 0) return;
     const FlexScratch X = flex_scratch(F, scr);
     const int last = *X.list_total;
     const int total = scan_global(X.slot_start, X.slot_cur, F.n_slots, lds);
@@ -808,14 +845,14 @@ extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_
     return hipGetLastError();
 }
 
-extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int32_t* item_off,
+extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int2* item_map,
                                              int total_items, const SceneDev* scenes, const ArenaCtl* ctl, int k,
                                              uint32_t flags, char* scr, hipStream_t stream) {
     if (n_groups <= 0) return hipSuccess;
     if (total_items > 0) {
         const int blocks = (int)(((int64_t)total_items * fw::FLEX_LPI + fw::FLEX_BLOCK - 1) / fw::FLEX_BLOCK);
-        hipLaunchKernelGGL(fw::flex_items_kernel, dim3(blocks), dim3(fw::FLEX_BLOCK), 0, stream, groups, item_off,
-                           n_groups, scenes, ctl, k, flags, scr);
+        hipLaunchKernelGGL(fw::flex_items_kernel, dim3(blocks), dim3(fw::FLEX_BLOCK), 0, stream, groups, item_map,
+                           total_items, scenes, ctl, k, flags, scr);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

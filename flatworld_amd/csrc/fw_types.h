@@ -59,10 +59,23 @@ struct FlexGrid {
     int32_t exact;  // every segment sits in every cell of its box (no cell or pair cap hit)
 };
 
+// A body's arrays the contact pass reads, copied from its scene entry when the
+// group is added (a scene's block does not move while it is resident), so that a
+// work item reaches positions and AABBs without going through the scene table.
+struct FlexBody {
+    const float2* pos[2];
+    const float2* vel;
+    const int32_t* aabb;  // the scene's boundary-AABB sub-slots
+    int32_t aabb_k;
+    int32_t pad_;
+};
+
 struct FlexDev {
     int32_t n_bodies, n_segs, n_items, n_pairs, n_slots;
     int32_t gated, max_cells, max_query, max_pairs;
     int32_t n_tied;                // tied work items (0: item_tied is never read)
+    int32_t s0, horizon;           // the group's scenes' warm-up step and horizon (all equal)
+    const FlexBody* bodies;        // [B]
     const int32_t* body_scene;     // [B] arena scene id
     const int32_t* body_seg_off;   // [B + 1]
     const int32_t* body_slot_off;  // [B + 1]
@@ -91,6 +104,7 @@ struct FlexDev {
     FlexGrid* grid;
     float4* seg_box;               // [S] segment boxes of the last build
     int4* seg_cells;               // [S] their clamped cell ranges (lx, ly, ux, uy)
+    int4* body_cells;              // [B] union of the body's segment cell ranges (empty: lx > ux)
     int32_t* seg_pairs;            // [S] scratch: pairs per segment, then the serial pair base
     int32_t* cell_start;           // [max_cells]
     int32_t* cell_end;             // [max_cells]
