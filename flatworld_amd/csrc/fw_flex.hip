@@ -444,18 +444,28 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
 __device__ void scan_segments(const FlexDev& F, const CellRange& q, float2 p, int be_off, int be_cnt,
                               const float2* tpos, float qbuf, int lane, int stride, int& met, Best& best) {
     const int nyq = q.uy - q.ly + 1;
-    for (int j = lane; j < be_cnt; j += stride) {
-        const int s = be_off + j;
-        const int4 r = F.seg_cells[s];
-        if (r.x > q.ux || r.z < q.lx || r.y > q.uy || r.w < q.ly) continue;
-        ++met;
-        const int t = (imax_(r.x, q.lx) - q.lx) * nyq + (imax_(r.y, q.ly) - q.ly);
-        const long long key = ((long long)t << 32) | (long long)s;
-        const int2 sn = F.seg_node[s];
-        float nx, ny, w0, w1;
-        const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
-        if (pen < 1e-4f && fabsf(pen) < qbuf && (pen < best.pen || (pen == best.pen && key < best.key)))
-            best.take(pen, nx, ny, w0, w1, s, key);
+    constexpr int SB = 4;  // cell boxes loaded together
+    for (int j0 = lane; j0 < be_cnt; j0 += SB * stride) {
+        int4 rb[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int j = j0 + u * stride;
+            rb[u] = j < be_cnt ? F.seg_cells[be_off + j] : make_int4(1, 1, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int4 r = rb[u];
+            if (r.x > q.ux || r.z < q.lx || r.y > q.uy || r.w < q.ly) continue;
+            const int s = be_off + j0 + u * stride;
+            ++met;
+            const int t = (imax_(r.x, q.lx) - q.lx) * nyq + (imax_(r.y, q.ly) - q.ly);
+            const long long key = ((long long)t << 32) | (long long)s;
+            const int2 sn = F.seg_node[s];
+            float nx, ny, w0, w1;
+            const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
+            if (pen < 1e-4f && fabsf(pen) < qbuf && (pen < best.pen || (pen == best.pen && key < best.key)))
+                best.take(pen, nx, ny, w0, w1, s, key);
+        }
     }
 }
 
