@@ -142,21 +142,42 @@ __device__ __forceinline__ void scene_aabb(const FlexBody& B, int ar3, float bb[
     for (int k = 0; k < 4; ++k) bb[k] = fw_o2f(r[k]);
 }
 
-// Workgroup exclusive scan of one value per thread; *total gets the sum.
+// Inclusive prefix sum over the 64 lanes of a wave: inside each row of 16 lanes
+// (row_shr 1, 2, 4, 8), then lane 15 of rows 0 and 2 added to rows 1 and 3 (row_bcast
+// 15), then lane 31 added to rows 2 and 3 (row_bcast 31); a lane without a source
+// adds 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_add(int v) {
+    return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, false);
+}
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+    v = dpp_add<0x111, 0xF>(v);
+    v = dpp_add<0x112, 0xF>(v);
+    v = dpp_add<0x114, 0xF>(v);
+    v = dpp_add<0x118, 0xF>(v);
+    v = dpp_add<0x142, 0xA>(v);
+    v = dpp_add<0x143, 0xC>(v);
+    return v;
+}
+
+// Workgroup exclusive scan of one value per thread; *total gets the sum.  lds holds
+// at least FLEX_BLOCK / 64 ints.
 __device__ int block_exclusive_scan(int v, int* lds, int* total) {
     const int tid = threadIdx.x;
-    lds[tid] = v;
+    const int lane = tid & 63, wid = tid >> 6;
+    const int incl = wave_inclusive_scan(v);
+    if (lane == 63) lds[wid] = incl;
     __syncthreads();
-    for (int off = 1; off < FLEX_BLOCK; off <<= 1) {
-        const int t = tid >= off ? lds[tid - off] : 0;
-        __syncthreads();
-        lds[tid] += t;
-        __syncthreads();
+    int before = 0, all = 0;
+#pragma unroll
+    for (int j = 0; j < FLEX_BLOCK / 64; ++j) {
+        const int t = lds[j];
+        before += j < wid ? t : 0;
+        all += t;
     }
-    const int incl = lds[tid];
-    *total = lds[FLEX_BLOCK - 1];
+    *total = all;
     __syncthreads();
-    return incl - v;
+    return before + incl - v;
 }
 
 // In-place exclusive scan of a[0, n) (copy, when given, receives the same); returns the sum.
