@@ -453,9 +453,11 @@ def spring_bench(args):
 
 
 def contact_bench(args):
-    """Batched two-block loops with FEM-FEM contact: per substep one contact-pass launch
-    (one workgroup per loop) and the substep launch; the hash of every loop is rebuilt
-    in one launch every 1/500 s of simulated time, as each loop's schedule has it."""
+    """Batched two-block loops with FEM-FEM contact: per substep the contact pass (work
+    items of every loop, then one workgroup per loop for the lists) and the substep
+    launch, replayed as 60-substep hipGraph frames; the hash of every loop is rebuilt in
+    one launch every 1/500 s of simulated time, as each loop's schedule has it,
+    scheduled on the device inside the frames."""
     from flatworld_amd import _native
     from flatworld_amd import workloads as W
     from flatworld_amd.flexcontact import REBUILD_INTERVAL
@@ -479,26 +481,36 @@ def contact_bench(args):
     arena.sync()
 
     elapsed = [0.0]
+    frame = 60
 
     def run(n):
         # the reference's schedule: a rebuild precedes the substep at which 1/500 s
-        # of simulated time has passed (flatworld/femspringmanager.py:1035-1039)
-        pending = 0
-        for _ in range(n):
+        # of simulated time has passed (flatworld/femspringmanager.py:1035-1039);
+        # frames of `frame` substeps replay as hipGraphs, each with its rebuilds
+        # scheduled on the device (plain launches with --no-graph)
+        rebuild_at = []
+        for k in range(n):
             elapsed[0] += dt
             if elapsed[0] >= REBUILD_INTERVAL:
-                if pending:
-                    arena.step(pending, -1.0, 0)
-                    pending = 0
-                arena.flex_rebuild(-1)
+                rebuild_at.append(k)
                 elapsed[0] = 0.0
-            pending += 1
-        if pending:
-            arena.step(pending, -1.0, 0)
+        for f0 in range(0, n, frame):
+            m = min(frame, n - f0)
+            for k in rebuild_at:
+                if f0 <= k < f0 + m:
+                    arena.flex_schedule_rebuild(-1, k - f0)
+            if args.no_graph:
+                arena.step(m, -1.0, 0)
+            else:
+                arena.step_graph(m, 0)
 
     # timed by the wall clock: the first contact_steps substeps after release
     # (approach and impact), then two rebuild intervals with the blocks in contact;
-    # then the contact pass + substep kernels with HIP events over one interval
+    # then the contact pass + substep kernels with HIP events over one interval.
+    # The frame graphs are captured before the timed region.
+    if not args.no_graph:
+        for m in {frame, args.contact_steps % frame, (2 * every) % frame} - {0}:
+            arena.prepare_graph(m, 0)
     t0 = time.perf_counter()
     run(args.contact_steps)
     arena.sync()

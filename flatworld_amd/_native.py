@@ -178,6 +178,7 @@ SIGNATURES = {
     "fw_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
     "fw_scene_reset_gate": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_step_graph": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]),
+    "fw_step_graph_prepare": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32]),
     "fw_step_noalloc": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
     "fw_sync": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_arena_track_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
@@ -196,6 +197,7 @@ SIGNATURES = {
     "fw_flex_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(FlexDesc), ctypes.POINTER(ctypes.c_int32)]),
     "fw_flex_rebuild": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_flex_init_ties": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_flex_schedule_rebuild": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
     "fw_flex_remove": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_flex_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
 }
@@ -474,6 +476,10 @@ class Arena:
     def step_graph(self, n, flags=0):
         check(self._L.fw_step_graph(self.h, int(n), int(flags)), "fw_step_graph")
 
+    def prepare_graph(self, n, flags=0):
+        """Capture the graph step_graph(n, flags) replays, without running it."""
+        check(self._L.fw_step_graph_prepare(self.h, int(n), int(flags)), "fw_step_graph_prepare")
+
     def sync(self):
         check(self._L.fw_sync(self.h), "fw_sync")
 
@@ -538,6 +544,11 @@ class Arena:
 
     def flex_rebuild(self, gid):
         check(self._L.fw_flex_rebuild(self.h, int(gid)), "fw_flex_rebuild")
+
+    def flex_schedule_rebuild(self, gid, at):
+        """Rebuild group ``gid``'s hash (-1: every group) on the device before substep ``at``
+        (0: the next one) of the following steps, plain or graph-replayed."""
+        check(self._L.fw_flex_schedule_rebuild(self.h, int(gid), int(at)), "fw_flex_schedule_rebuild")
 
     def flex_init_ties(self, gid):
         check(self._L.fw_flex_init_ties(self.h, int(gid)), "fw_flex_init_ties")

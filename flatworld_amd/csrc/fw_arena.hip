@@ -43,7 +43,7 @@ extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n
 extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
                                              float dt_target, int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
-                                          const ArenaCtl* ctl, int row, hipStream_t stream);
+                                          const ArenaCtl* ctl, int row, int k, int sched, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_ties(const FlexDev* groups, int row, int n_items, const SceneDev* scenes,
                                           const ArenaCtl* ctl, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_contact(const FlexDev* groups, int n_groups, const int2* item_map,
@@ -644,7 +644,8 @@ struct FlexHost {
     int64_t block_off = 0, block_bytes = 0;
     std::vector<int32_t> scenes;
     int32_t n_items = 0, max_cells = 0, max_pairs = 0, n_tied = 0;
-    bool hash_built = false;  // set by the first fw_flex_rebuild; tie resolution walks that hash
+    bool hash_built = false;  // set by the first rebuild; tie resolution walks that hash
+    int32_t sched[FW_FLEX_SCHED];  // host copy of the device schedule (INT32_MIN: free)
     FlexDev dev{};
 };
 
@@ -718,6 +719,7 @@ struct fw_arena {
     std::vector<int> flex_row;
     FlexDev* d_flex = nullptr;
     int2* d_flex_items = nullptr;  // (table row, work item) of every global work item
+    int32_t* d_flex_sched = nullptr;  // FW_FLEX_SCHED scheduled rebuild steps per table row
     int flex_cap = 0, n_flex_live = 0, n_flex_items = 0, flex_items_cap = 0;
     int64_t flex_scratch_bytes = 0;  // compact per-substep scratch of the live groups (fw_step_noalloc)
     // (scene, record) of every rigid contact record of the live scenes, for the rigid pass
@@ -904,9 +906,22 @@ int ensure_tiles(fw_arena* a) {
     return FW_OK;
 }
 
+// Every live group's scheduled rebuild steps, in table-row order, in one copy.
+int upload_sched(fw_arena* a, int n_rows) {
+    if (n_rows <= 0) return FW_OK;
+    std::vector<int32_t> all((size_t)n_rows * FW_FLEX_SCHED, INT32_MIN);
+    for (size_t g = 0; g < a->flex.size(); ++g) {
+        const int r = g < a->flex_row.size() ? a->flex_row[g] : -1;
+        if (!a->flex[g] || r < 0 || r >= n_rows) continue;
+        for (int j = 0; j < FW_FLEX_SCHED; ++j) all[(size_t)r * FW_FLEX_SCHED + j] = a->flex[g]->sched[j];
+    }
+    return upload(a, a->d_flex_sched, all.data(), (int64_t)sizeof(int32_t) * (int64_t)all.size());
+}
+
 // Device table of the live FEM-FEM groups, in group-id order.
 int ensure_flex(fw_arena* a) {
     if (!a->flex_dirty) return FW_OK;
+    drop_graphs(a);  // the graphs hold the group count, the work-item grid and the table pointers
     std::vector<FlexDev> rows;
     a->flex_row.assign(a->flex.size(), -1);
     for (size_t g = 0; g < a->flex.size(); ++g) {
@@ -918,12 +933,17 @@ int ensure_flex(fw_arena* a) {
     if (n > a->flex_cap) {
         const int cap = std::max(n, std::max(8, 2 * a->flex_cap));
         FlexDev* nd = nullptr;
+        int32_t* ns = nullptr;
         HIPCHK(hipStreamSynchronize(a->stream));
         HIPCHK(hipMalloc(&nd, sizeof(FlexDev) * (size_t)cap));
+        HIPCHK(hipMalloc(&ns, sizeof(int32_t) * FW_FLEX_SCHED * (size_t)cap));
         if (a->d_flex) HIPCHK(hipFree(a->d_flex));
+        if (a->d_flex_sched) HIPCHK(hipFree(a->d_flex_sched));
         a->d_flex = nd;
+        a->d_flex_sched = ns;
         a->flex_cap = cap;
     }
+    for (int r = 0; r < n; ++r) rows[r].sched = a->d_flex_sched + (size_t)r * FW_FLEX_SCHED;
     std::vector<int32_t> item_off(n + 1, 0);
     for (int r = 0; r < n; ++r) item_off[r + 1] = item_off[r] + rows[r].n_items;
     if (item_off[n] > a->flex_items_cap) {
@@ -945,6 +965,8 @@ int ensure_flex(fw_arena* a) {
     }
     a->flex_scratch_bytes = scr;
     int rc = upload(a, a->d_flex, rows.data(), (int64_t)sizeof(FlexDev) * n);
+    if (rc) return rc;
+    rc = upload_sched(a, n);
     if (rc) return rc;
     if (item_off[n] > 0) {
         rc = upload(a, a->d_flex_items, item_map.data(), (int64_t)sizeof(int2) * item_off[n]);
@@ -1348,6 +1370,7 @@ int32_t fw_arena_destroy(fw_arena* a) {
     (void)hipFree(a->d_ctl);
     (void)hipFree(a->d_flex);
     (void)hipFree(a->d_flex_items);
+    (void)hipFree(a->d_flex_sched);
     (void)hipFree(a->d_rigid);
     (void)hipStreamDestroy(a->stream);
     delete a;
@@ -1564,7 +1587,11 @@ int32_t fw_arena_busy(fw_arena* a, double* busy_ms, int64_t* calls) {
 // kind present (triangle tiles, then spring tiles), each covering its kind's tiles
 // as the control block counts them at run time.
 static hipError_t launch_all(fw_arena* a, int k, float dt_override, float damping_override, uint32_t f, int grid_tri,
-                      int grid_spr, char* flex_scr = nullptr) {
+                      int grid_spr, char* flex_scr = nullptr, bool sched_hash = false) {
+    if (sched_hash && a->n_flex_live > 0) {  // scheduled hash rebuilds before this substep's contact pass
+        hipError_t e = fw_launch_flex_hash(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, -1, k, 1, a->stream);
+        if (e != hipSuccess) return e;
+    }
     if (a->n_rigid_live > 0) {  // FEM-rigid forces before the substep that adds them
         hipError_t e = fw_launch_rigid_pass(a->d_scenes, a->d_rigid, a->n_rigid_live, a->d_ctl, k, f, a->stream);
         if (e != hipSuccess) return e;
@@ -1583,6 +1610,26 @@ static hipError_t launch_all(fw_arena* a, int k, float dt_override, float dampin
         return fw_launch_substep(a->d_scenes, a->d_tiles, a->d_ctl, k, 0, -1, dt_override, damping_override, f,
                                  grid_spr, a->tune.block, a->tune.tile_slots, shape_npt(a->tune), 1, a->stream);
     return hipSuccess;
+}
+
+// Whether some group has a hash rebuild scheduled before global step `step`.
+static bool sched_at(const fw_arena* a, int64_t step) {
+    for (const auto& F : a->flex)
+        if (F)
+            for (int r = 0; r < FW_FLEX_SCHED; ++r)
+                if (F->sched[r] == step) return true;
+    return false;
+}
+
+// After n substeps from the current clock: the scheduled rebuilds they ran are done.
+static void sched_consume(fw_arena* a, int32_t n) {
+    for (auto& F : a->flex)
+        if (F)
+            for (int r = 0; r < FW_FLEX_SCHED; ++r)
+                if (F->sched[r] != INT32_MIN && F->sched[r] < a->clock + n) {
+                    F->hash_built = true;
+                    F->sched[r] = INT32_MIN;
+                }
 }
 
 static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_override, uint32_t flags) {
@@ -1609,11 +1656,12 @@ static int step_impl(fw_arena* a, int32_t n, float dt_override, float damping_ov
                 }
             } else {
                 HIPCHK(launch_all(a, k, dt_override, damping_override, f, n_tri ? launch_grid(a, n_tri) : 0,
-                                  n_spr ? launch_grid(a, n_spr) : 0));
+                                  n_spr ? launch_grid(a, n_spr) : 0, nullptr, sched_at(a, a->clock + k)));
             }
         }
     }
     HIPCHK(fw_launch_advance_clock(a->d_ctl, n, a->stream));
+    sched_consume(a, n);
     a->clock += n;
     return FW_OK;
 }
@@ -1676,10 +1724,11 @@ int32_t fw_step_noalloc(fw_arena* a, int32_t n, float dt_override, float damping
             }
         } else {
             HIPCHK_NOALLOC(launch_all(a, k, dt_override, damping_override, f, n_tri ? launch_grid(a, n_tri) : 0,
-                                      n_spr ? launch_grid(a, n_spr) : 0, scr));
+                                      n_spr ? launch_grid(a, n_spr) : 0, scr, sched_at(a, a->clock + k)));
         }
     }
     HIPCHK_NOALLOC(fw_launch_advance_clock(a->d_ctl, n, a->stream));
+    sched_consume(a, n);
     a->clock += n;
     return FW_OK;
 }
@@ -1795,7 +1844,9 @@ int32_t fw_scene_reset_gate(fw_arena* a, int32_t id) {
     return upload(a, &a->d_scenes[id].hot.gate_base, &s->dev.hot.gate_base, sizeof(int32_t));
 }
 
-int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
+// The instantiated graph of n substeps with `flags` for the current tables, captured
+// on first use.
+static int graph_for(fw_arena* a, int32_t n, uint32_t flags, hipGraphExec_t* out) {
     if (!a || n <= 0) return fail(FW_E_INVALID, "bad arguments");
     if (flags & FW_STEP_PER_SCENE) return fail(FW_E_INVALID, "graph replay covers the batched launch only");
     HIPCHK(hipSetDevice(a->device));
@@ -1803,9 +1854,6 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
     if (rc) return rc;
     rc = ensure_flex(a);
     if (rc) return rc;
-    if (a->n_flex_live > 0)
-        return fail(FW_E_UNSUPPORTED, "FEM-FEM contact groups step through fw_step (their hash rebuilds follow the "
-                                      "caller's schedule)");
     const int nt = (int)a->tile_table.size();
     const int grid = launch_grid(a, std::max(nt, 1));
     // the element kinds present decide which launches a frame holds; the graph's grids
@@ -1826,7 +1874,8 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
         for (int k = 0; k < n; ++k) {
             uint32_t f = flags;
             if (k != n - 1) f &= ~FW_STEP_DIAG;
-            hipError_t e = launch_all(a, k, -1.0f, -1.0f, f, (kinds & 1) ? cap_grid : 0, (kinds & 2) ? cap_grid : 0);
+            hipError_t e = launch_all(a, k, -1.0f, -1.0f, f, (kinds & 1) ? cap_grid : 0, (kinds & 2) ? cap_grid : 0,
+                                      nullptr, a->n_flex_live > 0);
             if (e != hipSuccess) {
                 (void)hipStreamEndCapture(a->stream, &g);
                 return fail(FW_E_HIP, std::string("capture launch: ") + hipGetErrorString(e));
@@ -1844,12 +1893,26 @@ int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
         }
         a->graphs.push_back({exec, n, flags});
     }
+    *out = exec;
+    return FW_OK;
+}
+
+int32_t fw_step_graph_prepare(fw_arena* a, int32_t n, uint32_t flags) {
+    hipGraphExec_t exec = nullptr;
+    return graph_for(a, n, flags, &exec);
+}
+
+int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags) {
+    hipGraphExec_t exec = nullptr;
+    int rc = graph_for(a, n, flags, &exec);
+    if (rc) return rc;
     hipEvent_t e0 = nullptr;
     rc = busy_begin(a, &e0);
     if (rc) return rc;
     HIPCHK(hipGraphLaunch(exec, a->stream));
     rc = busy_end(a, e0);
     if (rc) return rc;
+    sched_consume(a, n);
     a->clock += n;
     return FW_OK;
 }
@@ -2234,6 +2297,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
     F->block_bytes = total;
     F->scenes.assign(d->scene_ids, d->scene_ids + B);
     F->n_items = W;
+    for (int r = 0; r < FW_FLEX_SCHED; ++r) F->sched[r] = INT32_MIN;
     F->max_cells = d->max_cells;
     F->max_pairs = d->max_pairs;
     F->n_tied = n_tied;
@@ -2322,11 +2386,51 @@ int32_t fw_flex_rebuild(fw_arena* a, int32_t gid) {
         if (!s0 || a->clock < s0->s0)
             return fail(FW_E_INVALID, "group " + std::to_string(g) + " has not run its warm-up substep yet");
     }
-    HIPCHK(fw_launch_flex_hash(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, gid >= 0 ? a->flex_row[gid] : -1,
+    HIPCHK(fw_launch_flex_hash(a->d_flex, a->n_flex_live, a->d_scenes, a->d_ctl, gid >= 0 ? a->flex_row[gid] : -1, 0, 0,
                                a->stream));
     for (size_t g = 0; g < a->flex.size(); ++g)
         if (a->flex[g] && (gid < 0 || (int)g == gid)) a->flex[g]->hash_built = true;
     return FW_OK;
+}
+
+int32_t fw_flex_schedule_rebuild(fw_arena* a, int32_t gid, int32_t at) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    if (at < 0) return fail(FW_E_INVALID, "at must be >= 0 (substeps from the next one)");
+    if (gid != -1 && (gid < 0 || gid >= (int)a->flex.size() || !a->flex[gid]))
+        return fail(FW_E_NOT_FOUND, "no FEM-FEM group " + std::to_string(gid));
+    HIPCHK(hipSetDevice(a->device));
+    const int64_t step = a->clock + (int64_t)at;
+    if (step >= INT32_MAX) return fail(FW_E_INVALID, "scheduled step out of range");
+    // check every group first, so that a refusal changes nothing
+    for (size_t g = 0; g < a->flex.size(); ++g) {
+        if (!a->flex[g] || (gid >= 0 && (int)g != gid)) continue;
+        const FlexHost& F = *a->flex[g];
+        const SceneHost* s0 = get_scene(a, F.scenes[0]);
+        if (!s0 || step < s0->s0)
+            return fail(FW_E_INVALID, "group " + std::to_string(g) + ": a rebuild needs the warm-up substep before it");
+        bool dup = false, room = false;
+        for (int r = 0; r < FW_FLEX_SCHED; ++r) {
+            dup |= F.sched[r] == step;
+            room |= F.sched[r] == INT32_MIN;
+        }
+        if (!dup && !room)
+            return fail(FW_E_CAPACITY, "group " + std::to_string(g) + " already holds " +
+                                           std::to_string(FW_FLEX_SCHED) + " pending rebuilds");
+    }
+    for (size_t g = 0; g < a->flex.size(); ++g) {
+        if (!a->flex[g] || (gid >= 0 && (int)g != gid)) continue;
+        FlexHost& F = *a->flex[g];
+        int slot = -1;
+        bool dup = false;
+        for (int r = 0; r < FW_FLEX_SCHED; ++r) {
+            if (F.sched[r] == step) dup = true;
+            if (F.sched[r] == INT32_MIN && slot < 0) slot = r;
+        }
+        if (!dup) F.sched[slot] = (int32_t)step;
+    }
+    int rc = ensure_flex(a);  // table rows of the live groups
+    if (rc) return rc;
+    return upload_sched(a, a->n_flex_live);
 }
 
 int32_t fw_flex_init_ties(fw_arena* a, int32_t gid) {

@@ -197,19 +197,28 @@ __device__ int scan_global(int32_t* a, int32_t* copy, int n, int* lds) {
     return carry;
 }
 
-// populate_fem_spatial_hash + SpatialHashManager.build at the state the next
-// substep starts from (velocity buffer 0).
+// populate_fem_spatial_hash + SpatialHashManager.build at the state substep k of
+// the launch starts from (velocity buffer 0).
 // One workgroup per group: the group at table row `row`, or row blockIdx.x when row < 0.
+// With `sched`, a group rebuilds only when one of its scheduled steps is this substep's
+// global step (fw_flex_schedule_rebuild), so that a captured frame holds the launch
+// whatever the schedule.
 __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* groups, const SceneDev* scenes,
-                                                               const ArenaCtl* ctl, int row) {
+                                                               const ArenaCtl* ctl, int row, int k, int sched) {
     const FlexDev& F = groups[row >= 0 ? row : (int)blockIdx.x];
+    const int step = ctl->clock + k;
+    if (sched) {
+        bool go = false;
+#pragma unroll
+        for (int r = 0; r < FW_FLEX_SCHED; ++r) go |= F.sched[r] == step;
+        if (!go) return;
+    }
     __shared__ int lds[FLEX_BLOCK];
     __shared__ float sh_bb[4];
     __shared__ int sh_est;
     __shared__ FlexGrid sh_g;
     const int tid = threadIdx.x;
-    const SceneDev* S0 = scenes + F.body_scene[0];
-    const int local = ctl->clock - S0->s0;
+    const int local = step - F.s0;
     const int rd = (local + 1) & 1;
     const int ar3 = mod3_(local);
     const int ns = F.n_segs;
@@ -860,11 +869,13 @@ extern "C" hipError_t fw_launch_flex_scratch_init(const FlexDev* groups, int n_g
     return hipGetLastError();
 }
 
+// row >= 0: that group now; row -1: every group now; sched != 0 (row -1): every group
+// whose schedule names substep k of the launch.
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
-                                          const ArenaCtl* ctl, int row, hipStream_t stream) {
+                                          const ArenaCtl* ctl, int row, int k, int sched, hipStream_t stream) {
     if (n_groups <= 0) return hipSuccess;
     hipLaunchKernelGGL(fw::flex_hash_kernel, dim3(row >= 0 ? 1 : n_groups), dim3(fw::FLEX_BLOCK), 0, stream, groups,
-                       scenes, ctl, row);
+                       scenes, ctl, row, k, sched);
     return hipGetLastError();
 }
 

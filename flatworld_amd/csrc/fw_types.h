@@ -59,6 +59,11 @@ struct FlexGrid {
     int32_t exact;  // every segment sits in every cell of its box (no cell or pair cap hit)
 };
 
+// Pending scheduled hash rebuilds a group holds (the public header's figure).
+#ifndef FW_FLEX_SCHED
+#define FW_FLEX_SCHED 4
+#endif
+
 // A body's arrays the contact pass reads, copied from its scene entry when the
 // group is added (a scene's block does not move while it is resident), so that a
 // work item reaches positions and AABBs without going through the scene table.
@@ -75,6 +80,8 @@ struct FlexDev {
     int32_t gated, max_cells, max_query, max_pairs;
     int32_t n_tied;                // tied work items (0: item_tied is never read)
     int32_t s0, horizon;           // the group's scenes' warm-up step and horizon (all equal)
+    const int32_t* sched;          // [FW_FLEX_SCHED] global steps before which the hash is rebuilt
+                                   // (INT32_MIN: free), written by fw_flex_schedule_rebuild
     const FlexBody* bodies;        // [B]
     const int32_t* body_scene;     // [B] arena scene id
     const int32_t* body_seg_off;   // [B + 1]

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 5
+#define FW_ABI_VERSION 6
 #define FW_MAX_GROUNDS 8
 
 /* status codes */
@@ -310,8 +310,13 @@ int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, in
  * group_id -1, for every group in one launch; the caller keeps the
  * reference's rebuild schedule (every 1/500 s of simulated time,
  * flatworld/femspringmanager.py:1035-1039).  fw_step then runs the group's
- * contact pass before every substep.  fw_step_graph and FW_STEP_PER_SCENE are
- * refused while a group exists.
+ * contact pass before every substep.  fw_flex_schedule_rebuild puts the same
+ * rebuild on the device instead, before substep `at` (0: the next one) of the
+ * following fw_step / fw_step_graph / fw_step_noalloc calls, for one group or,
+ * with group_id -1, every group; a group holds up to FW_FLEX_SCHED pending
+ * rebuilds (FW_E_CAPACITY beyond).  A captured frame checks every group's schedule
+ * before each substep, so fw_step_graph replays groups whatever their rebuild
+ * schedule.  FW_STEP_PER_SCENE is refused while a group exists.
  * Tied work items never slide: fw_flex_init_ties enqueues, at the state the next
  * substep starts from and with the group's current hash, the one-time resolution
  * of every tied, unresolved item to its nearest edge
@@ -323,7 +328,9 @@ int32_t fw_scene_progress(fw_arena* a, int32_t scene_id, int64_t* steps_done, in
  * node's tie contributions before its sliding-contact ones.  It returns
  * FW_E_INVALID for a group with tied items whose hash was never built. */
 int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* desc, int32_t* group_id);
+#define FW_FLEX_SCHED 4
 int32_t fw_flex_rebuild(fw_arena* a, int32_t group_id);
+int32_t fw_flex_schedule_rebuild(fw_arena* a, int32_t group_id, int32_t at);
 int32_t fw_flex_init_ties(fw_arena* a, int32_t group_id);
 int32_t fw_flex_remove(fw_arena* a, int32_t group_id);
 int32_t fw_flex_read(fw_arena* a, int32_t group_id, int32_t field, void* out, int64_t out_bytes);
@@ -355,6 +362,10 @@ int32_t fw_step_noalloc(fw_arena* a, int32_t n, float dt_override, float damping
 
 /* n substeps captured once as a HIP graph (per n and flags) and replayed. */
 int32_t fw_step_graph(fw_arena* a, int32_t n, uint32_t flags);
+/* Capture and instantiate the graph fw_step_graph(a, n, flags) replays, without
+ * launching it (a caller that times frames, or must not capture inside its loop,
+ * prepares its frame lengths first).  No reference counterpart. */
+int32_t fw_step_graph_prepare(fw_arena* a, int32_t n, uint32_t flags);
 int32_t fw_sync(fw_arena* a);
 
 /* GPU-busy accounting for a server loop.  With tracking on, every fw_step and

@@ -151,21 +151,101 @@ def test_bodies_over_many_tiles_bitexact():
     assert contact > 0
 
 
-def test_group_refuses_graph_and_scene_removal():
+def test_group_refuses_scene_removal_and_per_scene_launches():
     mat = fw.Elastic(1e4, 0.3, 10.0)
     a = ccw_grid(4, (0.0, 0.0), 0.2, mat)
     b = ccw_grid(4, (0.0, 0.21), 0.2, mat)
     loop = fw.ExplicitLoop(0.0, [a, b], useAdapativeDT=True)
     m = loop.femSpringManager
-    with pytest.raises(NotImplementedError, match="fw_step"):
-        m._arena.step_graph(4)
+    with pytest.raises(NotImplementedError, match="FW_STEP_PER_SCENE"):
+        m._arena.step(2, -1.0, _native.FW_STEP_PER_SCENE)
     with pytest.raises(ValueError, match="remove the group first"):
         m._arena.remove(m._scene_ids[0])
+    with pytest.raises(ValueError, match=">= 0"):
+        m._arena.flex_schedule_rebuild(m._flex_gid, -1)
+    for at in range(4):
+        m._arena.flex_schedule_rebuild(m._flex_gid, 10 + at)
+    m._arena.flex_schedule_rebuild(m._flex_gid, 11)  # already pending: nothing to add
+    with pytest.raises(RuntimeError, match="pending rebuilds"):
+        m._arena.flex_schedule_rebuild(m._flex_gid, 20)
+    m._arena.step_graph(12)  # runs the first two; the other two stay pending
+    m._arena.flex_schedule_rebuild(m._flex_gid, 5)
+    m._arena.flex_schedule_rebuild(m._flex_gid, 6)
+    with pytest.raises(RuntimeError, match="pending rebuilds"):
+        m._arena.flex_schedule_rebuild(m._flex_gid, 7)
+    m._arena.step_graph(12)
     m._arena.flex_remove(m._flex_gid)
     m._flex_gid = None
     m._arena.step_graph(4)
     m._arena.remove(m._scene_ids[0])
     m._arena.sync()
+
+
+def test_graph_frames_with_scheduled_rebuilds_equal_plain_launches():
+    """256 loops of the contact workload: 60-substep hipGraph frames with every
+    hash rebuild scheduled on the device, against plain launches with the rebuilds
+    enqueued between them (the reference's schedule, one rebuild every 147
+    substeps), and against fw_step_noalloc with scheduled rebuilds: positions,
+    velocities, contact flags and every work item's cache, bit for bit."""
+    from flatworld_amd import workloads as W
+    from flatworld_amd.flexcontact import REBUILD_INTERVAL
+
+    groups = W.contact_batched_config(n_groups=256, seed=2)
+    dt = groups[0][0][0].dt
+    arenas, ids = [], []
+    for _ in range(3):
+        ar = _native.Arena(tile_nodes=512, tile_slots=1024, block=256, grid_per_cu=0, capacity_bytes=1 << 30)
+        sids, gids = [], []
+        for sp, g in groups:
+            s = [ar.add(x) for x in sp]
+            sids.append(s)
+            gids.append(ar.flex_add(g, s))
+        ar.step(1, -1.0, 0)
+        ar.flex_rebuild(-1)
+        arenas.append(ar)
+        ids.append((sids, gids))
+    plain, graph, noalloc = arenas
+    noalloc.step(0)
+    n_total, frame = 660, 60
+    rebuild_at, elapsed = [], 0.0
+    for k in range(n_total):
+        elapsed += dt
+        if elapsed >= REBUILD_INTERVAL:
+            rebuild_at.append(k)
+            elapsed = 0.0
+    assert len(rebuild_at) >= 4
+    log(f"[case] 256 contact loops, rebuilds before substeps {rebuild_at}")
+    pending = 0
+    for k in range(n_total):
+        if k in rebuild_at:
+            if pending:
+                plain.step(pending, -1.0, 0)
+                pending = 0
+            plain.flex_rebuild(-1)
+        pending += 1
+    plain.step(pending, -1.0, 0)
+    for f0 in range(0, n_total, frame):
+        for k in rebuild_at:
+            if f0 <= k < f0 + frame:
+                graph.flex_schedule_rebuild(-1, k - f0)
+                noalloc.flex_schedule_rebuild(-1, k - f0)
+        graph.step_graph(frame, _native.FW_STEP_DIAG)
+        noalloc.step_noalloc(frame, -1.0, _native.FW_STEP_DIAG)
+    for ar in arenas:
+        ar.sync()
+    # the plain run's last substep carried no DIAG: compare positions, velocities, caches
+    cached = 0
+    for gi in range(len(groups)):
+        for m, ar in ((1, graph), (2, noalloc)):
+            for j, sid in enumerate(ids[0][0][gi]):
+                for fld in (_native.FIELD_X, _native.FIELD_V):
+                    assert np.array_equal(ar.read(ids[m][0][gi][j], fld), plain.read(sid, fld)), (gi, m, j, fld)
+            c = plain.flex_read(ids[0][1][gi], _native.FLEX_CACHE)
+            assert np.array_equal(ar.flex_read(ids[m][1][gi], _native.FLEX_CACHE), c), (gi, m)
+        cached += int((c >= 0).sum())
+    assert cached > 100, "the loops should be in contact"
+    for ar in arenas:
+        ar.close()
 
 
 def test_tie_resolution_needs_a_built_hash():
