@@ -174,6 +174,8 @@ SIGNATURES = {
     "fw_scene_footprint": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_int64)]),
     "fw_scene_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(SceneDesc), ctypes.POINTER(ctypes.c_int32)]),
     "fw_scene_remove": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
+    "fw_scene_retire": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
+    "fw_retire_wait": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_scene_progress": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
     "fw_step": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
     "fw_scene_reset_gate": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
@@ -183,6 +185,7 @@ SIGNATURES = {
     "fw_sync": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_arena_track_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_arena_busy": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+    "fw_wait_launches": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     "fw_scene_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_scene_write": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_tiling_check": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
@@ -394,6 +397,7 @@ class Arena:
         self.tuning = dict(tile_nodes=tile_nodes, tile_slots=tile_slots, block=block, grid_per_cu=grid_per_cu)
         self._shapes = {}
         self._flex = {}
+        self._retiring = []
 
     def close(self):
         if self.h:
@@ -439,6 +443,27 @@ class Arena:
     def remove(self, sid):
         check(self._L.fw_scene_remove(self.h, int(sid)), "fw_scene_remove")
         self._shapes.pop(sid, None)
+
+    def retire(self, sid, velocities=True):
+        """Remove a scene, its final positions (and velocities) queued for the host behind
+        the substeps launched so far; the returned arrays are filled by retire_wait()."""
+        n = self._shapes[sid][0]
+        x = np.empty((n, 2), np.float32)
+        v = np.empty((n, 2), np.float32) if velocities else None
+        check(self._L.fw_scene_retire(self.h, int(sid), x.ctypes.data,
+                                      v.ctypes.data if v is not None else None), "fw_scene_retire")
+        self._shapes.pop(sid, None)
+        self._retiring.append((x, v))  # kept alive until the copies land
+        return x, v
+
+    def wait_launches(self, keep=0):
+        """Block until at most ``keep`` tracked stepping calls are unfinished (busy tracking on)."""
+        check(self._L.fw_wait_launches(self.h, int(keep)), "fw_wait_launches")
+
+    def retire_wait(self):
+        """Wait for the retirements queued since the last call and fill their arrays."""
+        check(self._L.fw_retire_wait(self.h), "fw_retire_wait")
+        self._retiring.clear()
 
     def progress(self, sid):
         d = ctypes.c_int64()

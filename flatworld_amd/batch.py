@@ -52,6 +52,19 @@ class FemBatch:
         self.arena.remove(sid)
         self.specs.pop(sid, None)
 
+    def retire(self, sid, velocities=True):
+        """Remove a finished scene; its final (positions, velocities) arrays are filled
+        by the next retire_wait(), which waits only for the launches before this call."""
+        out = self.arena.retire(sid, velocities)
+        self.specs.pop(sid, None)
+        return out
+
+    def retire_wait(self):
+        self.arena.retire_wait()
+
+    def wait_launches(self, keep=0):
+        self.arena.wait_launches(keep)
+
     def progress(self, sid):
         """(substeps completed after the warm-up, reached horizon)."""
         return self.arena.progress(sid)

@@ -635,6 +635,16 @@ struct Staging {
     int64_t size = 0;
     hipEvent_t ev = nullptr;
     bool busy = false;
+    bool held = false;  // holds a retired scene's readback until fw_retire_wait copies it out
+};
+
+// A retired scene whose final state is on its way to the host (fw_scene_retire).
+struct PendingRetire {
+    Staging* st = nullptr;
+    int64_t n_nodes = 0;
+    std::vector<int32_t> perm;  // internal -> caller node order
+    float* x_out = nullptr;
+    float* v_out = nullptr;
 };
 
 }  // namespace
@@ -696,6 +706,7 @@ struct fw_arena {
     bool tiles_dirty = false;
     int64_t clock = 0;  // host mirror of ctl->clock
     std::vector<std::unique_ptr<Staging>> staging;
+    std::vector<PendingRetire> retiring;
     // graph cache: one executable graph per (substeps, flags), all captured for the same
     // launch grid and element kinds; dropped together when either changes
     struct GraphEntry {
@@ -821,7 +832,7 @@ int get_staging(fw_arena* a, int64_t bytes, Staging** out) {
         if (s->busy && hipEventQuery(s->ev) == hipSuccess) s->busy = false;
     }
     for (auto& s : a->staging) {
-        if (!s->busy && s->size >= bytes) {
+        if (!s->busy && !s->held && s->size >= bytes) {
             *out = s.get();
             return FW_OK;
         }
@@ -1459,6 +1470,61 @@ int32_t fw_scene_add(fw_arena* a, const fw_scene_desc* d, int32_t* scene_id) {
     return commit_scene(a, P.get(), scene_id);
 }
 
+int32_t fw_scene_retire(fw_arena* a, int32_t id, float* x_out, float* v_out) {
+    if (!a || !x_out) return fail(FW_E_INVALID, "null argument");
+    SceneHost* s = get_scene(a, id);
+    if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
+    if (s->flex_group >= 0)
+        return fail(FW_E_INVALID, "scene " + std::to_string(id) + " belongs to FEM-FEM group " +
+                                      std::to_string(s->flex_group) + "; remove the group first");
+    HIPCHK(hipSetDevice(a->device));
+    int64_t done;
+    int buf;
+    bool warmed;
+    scene_state(a, s, &done, &buf, &warmed);
+    const int64_t n = s->n_nodes, bytes = 8 * n;
+    Staging* st = nullptr;
+    int rc = get_staging(a, 2 * bytes, &st);
+    if (rc) return rc;
+    // the copies follow every substep launched so far on the stream, and precede the
+    // removal below and any later upload into the freed block
+    HIPCHK(hipMemcpyAsync(st->ptr, s->dev.pos[buf], bytes, hipMemcpyDeviceToHost, a->stream));
+    if (v_out) HIPCHK(hipMemcpyAsync((char*)st->ptr + bytes, s->dev.vel, bytes, hipMemcpyDeviceToHost, a->stream));
+    HIPCHK(hipEventRecord(st->ev, a->stream));
+    st->busy = true;
+    st->held = true;
+    PendingRetire pr;
+    pr.st = st;
+    pr.n_nodes = n;
+    pr.perm = s->tiling.node_perm;
+    pr.x_out = x_out;
+    pr.v_out = v_out;
+    a->retiring.push_back(std::move(pr));
+    return fw_scene_remove(a, id);
+}
+
+int32_t fw_retire_wait(fw_arena* a) {
+    if (!a) return fail(FW_E_INVALID, "null arena");
+    HIPCHK(hipSetDevice(a->device));
+    for (auto& pr : a->retiring) {
+        HIPCHK(hipEventSynchronize(pr.st->ev));
+        const float2* x = (const float2*)pr.st->ptr;
+        const float2* v = x + pr.n_nodes;
+        for (int64_t i = 0; i < pr.n_nodes; ++i) {
+            const int64_t o = pr.perm[i];
+            pr.x_out[2 * o] = x[i].x;
+            pr.x_out[2 * o + 1] = x[i].y;
+            if (pr.v_out) {
+                pr.v_out[2 * o] = v[i].x;
+                pr.v_out[2 * o + 1] = v[i].y;
+            }
+        }
+        pr.st->held = false;
+    }
+    a->retiring.clear();
+    return FW_OK;
+}
+
 int32_t fw_scene_remove(fw_arena* a, int32_t id) {
     if (!a) return fail(FW_E_INVALID, "null arena");
     SceneHost* s = get_scene(a, id);
@@ -1562,6 +1628,15 @@ static int busy_end(fw_arena* a, hipEvent_t e0) {
 int32_t fw_arena_track_busy(fw_arena* a, int32_t enable) {
     if (!a) return fail(FW_E_INVALID, "null arena");
     a->track_busy = enable != 0;
+    return FW_OK;
+}
+
+int32_t fw_wait_launches(fw_arena* a, int32_t keep) {
+    if (!a || keep < 0) return fail(FW_E_INVALID, "bad arguments");
+    if (!a->track_busy) return fail(FW_E_INVALID, "fw_wait_launches needs busy tracking (fw_arena_track_busy)");
+    HIPCHK(hipSetDevice(a->device));
+    const size_t n = a->busy_pending.size();
+    if (n > (size_t)keep) HIPCHK(hipEventSynchronize(a->busy_pending[n - 1 - (size_t)keep].second));
     return FW_OK;
 }
 

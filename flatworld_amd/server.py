@@ -14,6 +14,10 @@ arrives at any time and is:
    first substep in the arena);
 4. retired when it reaches its horizon: its final state is copied out and its
    memory is released (stream-ordered, without stalling the running frame).
+   With a batch that can retire asynchronously, the copies of the jobs a frame
+   finished are queued behind that frame, the next frame is queued behind them,
+   and only then does the host wait -- for the copies, not for the next frame --
+   so the device does not idle while results come back.
    A frame is shortened (``frame``, then halves down to ``min_frame``
    substeps) when the resident job closest to its horizon needs fewer
    substeps, so a job waits at most ``min_frame - 1`` substeps past its horizon
@@ -60,6 +64,7 @@ class JobResult:
     ok: bool
     positions: np.ndarray = field(repr=False, default=None)
     velocities: np.ndarray = field(repr=False, default=None)
+    released: float = None  # when its arena memory was freed (at or before `finished`)
 
     @property
     def latency(self):
@@ -149,7 +154,7 @@ class SimServer:
                     waiting.popleft()
                     prepared.pop(job.job_id, None)
                     t = self.clock() - t0
-                    self.results.append(JobResult(job.job_id, job.arrival, t, t, job.spec.horizon, job.spec.n_elems, False))
+                    self.results.append(JobResult(job.job_id, job.arrival, t, t, job.spec.horizon, job.spec.n_elems, False, released=t))
                     continue
                 need = prep.footprint if prep is not None else b.footprint(job.spec)
                 if need > free:
@@ -165,12 +170,34 @@ class SimServer:
                     waiting.popleft()
                     prepared.pop(job.job_id, None)
                     t = self.clock() - t0
-                    self.results.append(JobResult(job.job_id, job.arrival, t, t, job.spec.horizon, job.spec.n_elems, False))
+                    self.results.append(JobResult(job.job_id, job.arrival, t, t, job.spec.horizon, job.spec.n_elems, False, released=t))
                     continue
                 waiting.popleft()
                 prepared.pop(job.job_id, None)
                 resident[sid] = (job, self.clock() - t0)
                 free = b.free_bytes()
+
+        def retire_queue():
+            # jobs the launched frames finish: their readbacks queued behind those frames
+            out = []
+            for sid in list(resident.keys()):
+                if not b.progress(sid)[1]:
+                    continue
+                job, adm = resident.pop(sid)
+                x, v = b.retire(sid, self.keep_results)
+                out.append((job, adm, x, v, self.clock() - t0))
+            return out
+
+        def retire_finish(done):
+            for job, adm, x, v, rel in done:
+                ok = bool(np.isfinite(x).all())
+                self.elem_substeps += job.spec.n_elems * job.spec.horizon
+                self.results.append(
+                    JobResult(job.job_id, job.arrival, adm, self.clock() - t0, job.spec.horizon, job.spec.n_elems, ok,
+                              x if self.keep_results else None, v, rel)
+                )
+
+        self._retire_queue, self._retire_finish = retire_queue, retire_finish
 
         def retire():
             for sid in list(resident.keys()):
@@ -183,9 +210,10 @@ class SimServer:
                 ok = bool(np.isfinite(x).all())
                 b.remove(sid)
                 self.elem_substeps += job.spec.n_elems * job.spec.horizon
+                t = self.clock() - t0
                 self.results.append(
-                    JobResult(job.job_id, job.arrival, adm, self.clock() - t0, job.spec.horizon, job.spec.n_elems, ok,
-                              x if self.keep_results else None, v)
+                    JobResult(job.job_id, job.arrival, adm, t, job.spec.horizon, job.spec.n_elems, ok,
+                              x if self.keep_results else None, v, t)
                 )
 
         self._t0 = t0
@@ -204,13 +232,32 @@ class SimServer:
             self.busy_time = (arena.busy()[0] - busy0) * 1e-3  # GPU time of the frames, from HIP events
         return self.results
 
+    def _launch(self, resident):
+        n = self._next_frame(resident)
+        self.batch.step(n, graph=True, frame=n)
+        self.frames += 1
+        self.substeps += n
+
     def _loop(self, pending, waiting, resident, admit, retire, max_wall=None):
         b = self.batch
         t0 = self._t0
         in_flight = False
+        overlap = hasattr(b, "retire") and hasattr(b, "wait_launches") and getattr(b, "arena", None) is not None
         while pending or waiting or resident:
             if max_wall is not None and self.clock() - t0 > max_wall:
                 break
+            if in_flight and overlap:
+                # a frame is queued: queue the readbacks of the jobs it finishes, admit,
+                # queue the next frame, then wait for the readbacks only
+                done = self._retire_queue()
+                admit()
+                in_flight = bool(resident)
+                if in_flight:
+                    self._launch(resident)
+                b.wait_launches(1 if in_flight else 0)  # the previous frame (two frames in flight at most)
+                b.retire_wait()
+                self._retire_finish(done)
+                continue
             if in_flight:
                 admit()  # overlaps the running frame
                 b.sync()
@@ -218,10 +265,7 @@ class SimServer:
                 retire()
             admit()
             if resident:
-                n = self._next_frame(resident)
-                b.step(n, graph=True, frame=n)
-                self.frames += 1
-                self.substeps += n
+                self._launch(resident)
                 in_flight = True
             elif waiting:
                 # nothing resident and the head of the line is still being tiled

@@ -282,6 +282,16 @@ int32_t fw_prepared_footprint(const fw_prepared* p, int64_t* bytes);
 int32_t fw_scene_add_prepared(fw_arena* a, fw_prepared* p, int32_t* scene_id);
 int32_t fw_prepared_free(fw_prepared* p);
 int32_t fw_scene_remove(fw_arena* a, int32_t scene_id);
+/* fw_scene_remove with the scene's final state kept: the copies of its positions
+ * (and velocities, when v_out is not NULL) are queued on the arena stream behind
+ * every substep launched so far, then the scene is removed; fw_retire_wait waits
+ * for the copies of every retirement queued since the last wait (not for later
+ * work on the stream) and writes them, in the caller's node order, to x_out /
+ * v_out (N x 2 floats each), which must stay valid until then.  A server retires
+ * finished jobs and launches its next frame before waiting, so the device does not
+ * idle while results come back.  No reference counterpart. */
+int32_t fw_scene_retire(fw_arena* a, int32_t scene_id, float* x_out, float* v_out);
+int32_t fw_retire_wait(fw_arena* a);
 /* Restart the scene's BVH-rebuild schedule: the next substep counts as step 0
  * of the activation gate (ExplicitLoop.reset, flatworld/explicitloop.py:635-679). */
 int32_t fw_scene_reset_gate(fw_arena* a, int32_t scene_id);
@@ -376,6 +386,10 @@ int32_t fw_sync(fw_arena* a);
  * reference counterpart: the reference has no served form. */
 int32_t fw_arena_track_busy(fw_arena* a, int32_t enable);
 int32_t fw_arena_busy(fw_arena* a, double* busy_ms, int64_t* calls);
+/* With busy tracking on: block until at most `keep` of the tracked stepping calls
+ * issued so far are unfinished on the device (keep 1: a server that has queued its
+ * next frame waits for the previous one).  No reference counterpart. */
+int32_t fw_wait_launches(fw_arena* a, int32_t keep);
 
 /* Host-only self-check of the tiling fw_scene_add would build (no GPU needed):
  * stats[0] tiles, [1] halo slots, [2] largest tile slot count, [3] CSR entries.

@@ -60,7 +60,7 @@ def test_admission_waits_for_memory():
     assert len(res) == 8 and all(r.ok for r in res)
     # at every admission the resident footprints fit the arena, and the arena
     # was small enough that some admissions had to wait for a retirement
-    spans = [(r.admitted, r.finished, fp[r.job_id]) for r in res]
+    spans = [(r.admitted, r.released, fp[r.job_id]) for r in res]
     most = 0
     for t, _, _ in spans:
         live = [b for a, f, b in spans if a <= t < f]

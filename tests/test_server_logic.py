@@ -157,3 +157,55 @@ def test_frames_shorten_near_a_horizon():
     # warm-up + 100 substeps = 101: 60 + 30 + 8 + 8 (overshoot 5 < 8)
     assert b.frames == [60, 30, 8, 8]
     assert sum(b.frames) - 101 < 8
+
+
+class FakeAsyncBatch(FakePrepBatch):
+    """FakePrepBatch with asynchronous retirement: a retired job's result exists only
+    once the frames queued before its retirement have run (wait_launches), and the
+    arrays handed out are filled by retire_wait -- as FemBatch.retire does."""
+
+    def __init__(self, capacity):
+        super().__init__(capacity)
+        self.arena = SimpleNamespace(track_busy=lambda on=True: None, busy=lambda: (0.0, 0))
+        self.queued = 0  # frames launched and not yet waited for
+        self.max_queued = 0
+        self.pending = []
+
+    def step(self, n, graph=True, frame=None):
+        super().step(n, graph, frame)
+        self.queued += 1
+        self.max_queued = max(self.max_queued, self.queued)
+
+    def wait_launches(self, keep=0):
+        self.queued = min(self.queued, keep)
+
+    def retire(self, sid, velocities=True):
+        s = self.scenes[sid]
+        x = np.empty((4, 2), np.float32)
+        v = np.empty((4, 2), np.float32) if velocities else None
+        self.pending.append((x, v, s["spec"], self.queued))
+        self.remove(sid)
+        return x, v
+
+    def retire_wait(self):
+        for x, v, sp, q in self.pending:
+            assert self.queued <= 1, "the frame a retirement follows must be done first"
+            x[:] = np.nan if sp.poison else float(sp.name)
+            if v is not None:
+                v[:] = 0.0
+        self.pending.clear()
+
+
+def test_overlapped_retirement_keeps_results_and_two_frames_in_flight():
+    b = FakeAsyncBatch(capacity=100)
+    srv = SimServer(batch=b, frame=60, prep_workers=2)
+    jobs = [Job(i, spec(i, h, 30), 0.0) for i, h in enumerate([30, 200, 59, 60, 500, 90])]
+    bad = spec(7, 60, 10, poison=True)
+    jobs.append(Job(7, bad, 0.0))
+    res = {r.job_id: r for r in srv.run(jobs)}
+    assert sorted(res) == [0, 1, 2, 3, 4, 5, 7]
+    assert all(res[i].ok for i in range(6)) and not res[7].ok
+    for i in range(6):
+        assert np.all(res[i].positions == float(i))
+    assert b.max_queued == 2  # the next frame is queued before the previous one is waited for
+    assert b.admit_order[:3] == [0, 1, 2]
