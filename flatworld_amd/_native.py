@@ -195,6 +195,7 @@ SIGNATURES = {
     "fw_prepared_footprint": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "fw_scene_add_prepared": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     "fw_prepared_free": (ctypes.c_int32, [ctypes.c_void_p]),
+    "fw_prepared_stage": (ctypes.c_int32, [ctypes.c_void_p]),
     "fw_scene_mass_scaling": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float]),
     "fw_scene_set_rigid": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(GroundDesc)]),
     "fw_flex_add": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(FlexDesc), ctypes.POINTER(ctypes.c_int32)]),
@@ -372,6 +373,12 @@ class Prepared:
         check(L.fw_prepared_footprint(self.h, ctypes.byref(b)), "fw_prepared_footprint")
         self.footprint = int(b.value)
 
+    def stage(self):
+        """Copy the packed image into page-locked memory now (worker thread), so that
+        admission only queues its upload."""
+        check(self._L.fw_prepared_stage(self.h), "fw_prepared_stage")
+        return self
+
     def close(self):
         if self.h:
             self._L.fw_prepared_free(self.h)
@@ -409,6 +416,12 @@ class Arena:
             self.close()
         except Exception:
             pass
+
+    def free_bytes(self):
+        """Arena bytes not held by scenes or groups."""
+        i = ArenaInfo()
+        check(self._L.fw_arena_get_info(self.h, ctypes.byref(i)))
+        return int(i.capacity_bytes - i.used_bytes)
 
     def info(self):
         i = ArenaInfo()

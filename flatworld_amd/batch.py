@@ -29,8 +29,7 @@ class FemBatch:
         return self.arena.footprint(spec, self.track_disp)
 
     def free_bytes(self):
-        i = self.arena.info()
-        return i["capacity_bytes"] - i["used_bytes"]
+        return self.arena.free_bytes()
 
     def add(self, spec):
         """Admit a scene; its zero-dt warm-up runs in the next substep."""
@@ -39,8 +38,10 @@ class FemBatch:
         return sid
 
     def prepare(self, spec):
-        """Host half of add(): tiling and packing, thread-safe, no GPU work."""
-        return self.arena.prepare(spec, track_disp=self.track_disp)
+        """Host half of add(): tiling and packing, thread-safe, no GPU work; the packed
+        image is then moved into page-locked memory on the same thread, so that
+        admission only queues its upload."""
+        return self.arena.prepare(spec, track_disp=self.track_disp).stage()
 
     def add_prepared(self, prep):
         """Device half of add(); raises RuntimeError (arena full) without consuming ``prep``."""

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -673,11 +674,61 @@ struct SceneHost {
 };
 
 // A scene prepared on the host for an arena tuning (fw_scene_prepare).
+// Page-locked image buffers of prepared scenes, shared by every thread and arena:
+// a worker thread copies a prepared image into one (fw_prepared_stage), the commit
+// DMAs from it directly, and the buffer is reused once that copy has completed.
+struct PinnedBuf {
+    void* ptr = nullptr;
+    int64_t size = 0;
+    hipEvent_t ev = nullptr;
+    bool in_flight = false;  // a commit's copy may still read it (ev recorded behind it)
+};
+
+static std::mutex g_pin_mu;
+static std::vector<PinnedBuf*> g_pin_free;  // idle or in flight; never freed before the process ends
+// Held by a graph capture and by the pool's HIP calls from worker threads: a HIP call
+// from another thread (an event query, a page-locked allocation) during a capture can
+// invalidate the capture, so the two never overlap.
+static std::mutex g_capture_mu;
+
+static PinnedBuf* pinned_get(int64_t bytes) {
+    std::lock_guard<std::mutex> cap(g_capture_mu);
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (size_t i = 0; i < g_pin_free.size(); ++i) {
+            PinnedBuf* b = g_pin_free[i];
+            if (b->size < bytes || b->size > 4 * bytes + (1 << 20)) continue;
+            if (b->in_flight) {
+                if (hipEventQuery(b->ev) != hipSuccess) continue;
+                b->in_flight = false;
+            }
+            g_pin_free.erase(g_pin_free.begin() + (long)i);
+            return b;
+        }
+    }
+    auto* b = new PinnedBuf;
+    b->size = std::max<int64_t>(bytes, 1 << 16);
+    if (hipHostMalloc(&b->ptr, (size_t)b->size, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (b->ptr) (void)hipHostFree(b->ptr);
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+static void pinned_put(PinnedBuf* b) {
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.push_back(b);
+}
+
 struct fw_prepared {
     fw_tuning tune{};
     Tiling T;               // tiles carry offsets; base pointer, scene id and s0 are set at the commit
     Layout L{};
     std::vector<char> img;  // the block's bytes, tile table excluded (written at the commit)
+    PinnedBuf* pin = nullptr;  // the same bytes in page-locked memory, after fw_prepared_stage
     SceneDev D{};           // scene entry without device addresses
     int32_t n_nodes = 0, n_elems = 0, mat_type = 0, horizon = 0;
     bool consumed = false;
@@ -1193,8 +1244,11 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
         return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
                                        std::to_string(a->capacity - a->used) + " free");
     Staging* st = nullptr;
-    int rc = get_staging(a, L.host_bytes, &st);
-    if (rc) return rc;
+    int rc = FW_OK;
+    if (!p->pin) {
+        rc = get_staging(a, L.host_bytes, &st);
+        if (rc) return rc;
+    }
     int64_t off = arena_alloc(a, L.total);
     if (off < 0)
         return fail(FW_E_CAPACITY, "arena full: scene needs " + std::to_string(L.total) + " bytes, " +
@@ -1243,11 +1297,21 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
         td.scene = id;
         td.s0 = D.s0;
     }
-    std::memcpy(st->ptr, p->img.data(), (size_t)L.host_bytes);
-    std::memcpy((char*)st->ptr + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
-    HIPCHK(hipMemcpyAsync(blk, st->ptr, L.host_bytes, hipMemcpyHostToDevice, a->stream));
-    HIPCHK(hipEventRecord(st->ev, a->stream));
-    st->busy = true;
+    if (p->pin) {  // staged on a worker thread: DMA straight from it
+        PinnedBuf* pb = p->pin;
+        p->pin = nullptr;
+        std::memcpy((char*)pb->ptr + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
+        HIPCHK(hipMemcpyAsync(blk, pb->ptr, L.host_bytes, hipMemcpyHostToDevice, a->stream));
+        HIPCHK(hipEventRecord(pb->ev, a->stream));
+        pb->in_flight = true;
+        pinned_put(pb);
+    } else {
+        std::memcpy(st->ptr, p->img.data(), (size_t)L.host_bytes);
+        std::memcpy((char*)st->ptr + L.tiles, T.tiles.data(), sizeof(TileDev) * T.tiles.size());
+        HIPCHK(hipMemcpyAsync(blk, st->ptr, L.host_bytes, hipMemcpyHostToDevice, a->stream));
+        HIPCHK(hipEventRecord(st->ev, a->stream));
+        st->busy = true;
+    }
     HIPCHK(hipMemsetAsync(blk + L.host_bytes, 0, (size_t)(L.total - L.host_bytes), a->stream));
 
     rc = upload(a, a->d_scenes + id, &D, sizeof(SceneDev));
@@ -1446,7 +1510,20 @@ int32_t fw_prepared_footprint(const fw_prepared* p, int64_t* bytes) {
     return FW_OK;
 }
 
+int32_t fw_prepared_stage(fw_prepared* p) {
+    if (!p) return fail(FW_E_INVALID, "null argument");
+    if (p->consumed) return fail(FW_E_INVALID, "prepared scene was already added");
+    if (p->pin) return FW_OK;
+    PinnedBuf* b = pinned_get(p->L.host_bytes);
+    if (!b) return FW_OK;  // no page-locked memory: the commit copies through the arena's staging
+    std::memcpy(b->ptr, p->img.data(), (size_t)p->L.host_bytes);
+    p->pin = b;
+    std::vector<char>().swap(p->img);
+    return FW_OK;
+}
+
 int32_t fw_prepared_free(fw_prepared* p) {
+    if (p && p->pin) pinned_put(p->pin);
     delete p;
     return FW_OK;
 }
@@ -1945,6 +2022,7 @@ static int graph_for(fw_arena* a, int32_t n, uint32_t flags, hipGraphExec_t* out
         }
         const int cap_grid = std::max(grid, launch_grid(a, a->tile_cap));
         hipGraph_t g;
+        std::lock_guard<std::mutex> cap(g_capture_mu);
         HIPCHK(hipStreamBeginCapture(a->stream, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < n; ++k) {
             uint32_t f = flags;

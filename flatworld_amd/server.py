@@ -101,12 +101,19 @@ class SimServer:
         self.busy_time = 0.0
         self.elem_substeps = 0
 
+    def _progress(self, sid, job):
+        """(substeps done after the warm-up, finished) of a resident job, from the
+        substeps launched since its admission (the first is its warm-up), as the
+        arena counts them."""
+        ran = self._launched.get(sid, 0)
+        done = min(max(ran - 1, 0), job.spec.horizon)
+        return done, ran - 1 >= job.spec.horizon
+
     def _next_frame(self, resident):
         """The longest frame length that does not carry the job closest to its horizon past it."""
-        b = self.batch
         rem = None
         for sid, (job, _) in resident.items():
-            done = b.progress(sid)[0]
+            done = self._progress(sid, job)[0]
             r = job.spec.horizon - done + (1 if done == 0 else 0)  # + its warm-up substep
             rem = r if rem is None else min(rem, r)
         for n in self.frame_lengths:
@@ -131,6 +138,7 @@ class SimServer:
             if job.spec.horizon <= 0:
                 raise ValueError(f"job {job.job_id}: served jobs need a positive horizon")
 
+        self._launched = {}  # scene id -> substeps launched since admission
         pool = ThreadPoolExecutor(self.prep_workers) if self.prep_workers > 0 else None
         prepared = {}  # job id -> Future of the job's prepared scene
 
@@ -175,15 +183,17 @@ class SimServer:
                 waiting.popleft()
                 prepared.pop(job.job_id, None)
                 resident[sid] = (job, self.clock() - t0)
+                self._launched[sid] = 0
                 free = b.free_bytes()
 
         def retire_queue():
             # jobs the launched frames finish: their readbacks queued behind those frames
             out = []
             for sid in list(resident.keys()):
-                if not b.progress(sid)[1]:
+                if not self._progress(sid, resident[sid][0])[1]:
                     continue
                 job, adm = resident.pop(sid)
+                self._launched.pop(sid, None)
                 x, v = b.retire(sid, self.keep_results)
                 out.append((job, adm, x, v, self.clock() - t0))
             return out
@@ -201,10 +211,11 @@ class SimServer:
 
         def retire():
             for sid in list(resident.keys()):
-                steps, fin = b.progress(sid)
+                steps, fin = self._progress(sid, resident[sid][0])
                 if not fin:
                     continue
                 job, adm = resident.pop(sid)
+                self._launched.pop(sid, None)
                 x = b.positions(sid)
                 v = b.velocities(sid) if self.keep_results else None
                 ok = bool(np.isfinite(x).all())
@@ -235,6 +246,8 @@ class SimServer:
     def _launch(self, resident):
         n = self._next_frame(resident)
         self.batch.step(n, graph=True, frame=n)
+        for sid in resident:
+            self._launched[sid] += n
         self.frames += 1
         self.substeps += n
 

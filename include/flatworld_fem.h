@@ -281,6 +281,12 @@ int32_t fw_scene_prepare(const fw_scene_desc* desc, const fw_tuning* tuning, fw_
 int32_t fw_prepared_footprint(const fw_prepared* p, int64_t* bytes);
 int32_t fw_scene_add_prepared(fw_arena* a, fw_prepared* p, int32_t* scene_id);
 int32_t fw_prepared_free(fw_prepared* p);
+/* Move a prepared scene's image into page-locked memory (on the worker thread that
+ * prepared it), so that fw_scene_add_prepared issues the upload without a host
+ * copy.  Needs the GPU runtime; without page-locked memory it changes nothing.  The
+ * buffers are pooled per process and reused once their upload has completed.  No
+ * reference counterpart. */
+int32_t fw_prepared_stage(fw_prepared* p);
 int32_t fw_scene_remove(fw_arena* a, int32_t scene_id);
 /* fw_scene_remove with the scene's final state kept: the copies of its positions
  * (and velocities, when v_out is not NULL) are queued on the arena stream behind
