@@ -372,15 +372,21 @@ __device__ __forceinline__ bool ground_force(const GroundDev& g, float px, float
 
 // ------------------------------------------------------------------ tile loop helpers
 
-__device__ __forceinline__ int tile_of_block(int round, int ntiles_total_grid) {
-    // XCD-aware mapping: hardware places block b on XCD b % 8; give each XCD a
-    // contiguous run of tiles so halo elements and nodes shared by neighbouring
-    // tiles are reused from that XCD's L2.
+// The tile block b processes in its round-th pass, or a value >= ntiles once it has
+// none left.  XCD-aware: hardware places block b on XCD b % 8, and each XCD takes
+// one contiguous eighth of the live tiles (its blocks striding through it), so halo
+// elements and nodes shared by neighbouring tiles are reused from that XCD's L2 and
+// every XCD gets work whatever the grid: a graph's grid covers the tile table's
+// capacity, often twice the live tiles, and an eighth of the grid per XCD would
+// leave the upper XCDs with nothing but exits.
+__device__ __forceinline__ int tile_of_block(int round, int ntiles) {
     const int G = gridDim.x;
     const int b = blockIdx.x;
-    int v = b;
-    if ((G & 7) == 0) v = (b & 7) * (G >> 3) + (b >> 3);
-    return v + round * G;
+    if ((G & 7) != 0) return b + round * G;
+    const int chunk = (ntiles + 7) >> 3;         // tiles per XCD
+    const int j = (b >> 3) + round * (G >> 3);   // this block's position in its XCD's run
+    if (j >= chunk) return ntiles;
+    return (b & 7) * chunk + j;                  // >= ntiles only in the last XCD's short run
 }
 
 // Workgroup barrier that orders LDS only.  Threads of a tile never exchange data
