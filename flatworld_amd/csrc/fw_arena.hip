@@ -110,31 +110,57 @@ uint32_t spread16(uint32_t v) {
 // Corners per element: three for a triangle, two for a spring.
 inline int corners_of(const fw_scene_desc* d) { return d->mat_type == FW_MAT_SPRING ? 2 : 3; }
 
+// Temporaries of one tiling, kept per thread: a server prepares a stream of jobs on
+// worker threads, and reusing warm buffers spares every job the page faults (and
+// the cross-thread unmapping) of a dozen freshly allocated multi-megabyte arrays.
+struct TileScratch {
+    std::vector<int32_t> inc_ptr, inc, iptr, iinc, cint, stamp, seen, nstamp, nseen, tile_start, node_tile, owner,
+        own_count, slot_of, lid, nst, slots, hl, hn, src_cnt;
+    std::vector<uint64_t> kv, kv_tmp;
+    std::vector<uint32_t> cnt;
+    std::vector<uint16_t> rank;
+};
+
+TileScratch& tile_scratch() {
+    static thread_local TileScratch s;
+    return s;
+}
+
 // K corners per element, a compile-time constant: the incidence codes K * e + c are
 // divided by K in every loop below.
+//
+// Node-indexed work runs in the internal (Morton) order: incidence lists are
+// regrouped by internal node and connectivity is translated to internal node ids
+// once, so the tile cut, the halo search and the slot records walk memory in the
+// order they are produced instead of jumping through the caller's numbering.
 template <int K>
 int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int node_cap, Tiling& T) {
     const int N = d->n_nodes, E = d->n_elems;
+    TileScratch& S = tile_scratch();
     // incidence (caller ids): node -> (element, corner), element-major so lists are ascending
-    std::vector<int32_t> inc_ptr(N + 1, 0);
+    std::vector<int32_t>& inc_ptr = S.inc_ptr;
+    inc_ptr.assign(N + 1, 0);
     for (int64_t i = 0; i < K * (int64_t)E; ++i) {
         int32_t n = d->conn[i];
         if (n < 0 || n >= N) return fail(FW_E_INVALID, "connectivity references node " + std::to_string(n) + " outside [0, n_nodes)");
         inc_ptr[n + 1]++;
     }
     for (int i = 0; i < N; ++i) inc_ptr[i + 1] += inc_ptr[i];
-    std::vector<int32_t> inc(K * (int64_t)E);
+    std::vector<int32_t>& inc = S.inc;
+    inc.resize(K * (size_t)E);
     {
-        std::vector<int32_t> fillp(inc_ptr.begin(), inc_ptr.end() - 1);
+        std::vector<int32_t>& fillp = S.nseen;  // reused below, re-initialised there
+        fillp.assign(inc_ptr.begin(), inc_ptr.end() - 1);
         for (int32_t e = 0; e < E; ++e)
             for (int c = 0; c < K; ++c) inc[fillp[d->conn[K * e + c]]++] = K * e + c;
     }
 
     // node order
     T.node_perm.resize(N);
-    for (int i = 0; i < N; ++i) T.node_perm[i] = i;
     const bool single = N <= tile_nodes && E <= tile_slots && N <= node_cap;
-    if (!single) {
+    if (single) {
+        for (int i = 0; i < N; ++i) T.node_perm[i] = i;
+    } else {
         float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
         for (int i = 0; i < N; ++i)
             for (int k = 0; k < 2; ++k) {
@@ -144,12 +170,17 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
                     hi[k] = std::max(hi[k], v);
                 }
             }
-        std::vector<uint32_t> key(N);
         double scale[2];
         for (int k = 0; k < 2; ++k) {
             const double span = (double)hi[k] - (double)lo[k];
             scale[k] = span > 0 ? 65535.0 / span : 0.0;
         }
+        // (Morton key << 32 | node id), radix-sorted on the key by three stable 11-bit
+        // passes: equal keys keep ascending node ids
+        std::vector<uint64_t>& kv = S.kv;
+        std::vector<uint64_t>& tmp = S.kv_tmp;
+        kv.resize(N);
+        tmp.resize(N);
         for (int i = 0; i < N; ++i) {
             uint32_t q[2];
             for (int k = 0; k < 2; ++k) {
@@ -158,47 +189,71 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
                 if (t > 65535.0) t = 65535.0;
                 q[k] = (uint32_t)t;
             }
-            key[i] = spread16(q[0]) | (spread16(q[1]) << 1);
+            const uint32_t key = spread16(q[0]) | (spread16(q[1]) << 1);
+            kv[i] = ((uint64_t)key << 32) | (uint32_t)i;
         }
-        // stable LSD radix sort of the node ids by key, three 11-bit passes
-        std::vector<int32_t> tmp(N);
-        std::vector<uint32_t> cnt(2049);
+        std::vector<uint32_t>& cnt = S.cnt;
+        cnt.resize(2049);
         for (int pass = 0; pass < 3; ++pass) {
-            const int sh = 11 * pass;
+            const int sh = 32 + 11 * pass;
             std::fill(cnt.begin(), cnt.end(), 0u);
-            for (int i = 0; i < N; ++i) cnt[((key[T.node_perm[i]] >> sh) & 0x7FFu) + 1]++;
+            for (int i = 0; i < N; ++i) cnt[((kv[i] >> sh) & 0x7FFu) + 1]++;
             for (int b = 0; b < 2048; ++b) cnt[b + 1] += cnt[b];
-            for (int i = 0; i < N; ++i) tmp[cnt[(key[T.node_perm[i]] >> sh) & 0x7FFu]++] = T.node_perm[i];
-            T.node_perm.swap(tmp);
+            for (int i = 0; i < N; ++i) tmp[cnt[(kv[i] >> sh) & 0x7FFu]++] = kv[i];
+            kv.swap(tmp);
         }
+        for (int i = 0; i < N; ++i) T.node_perm[i] = (int32_t)(uint32_t)kv[i];
     }
-    T.node_inv.assign(N, 0);
+    T.node_inv.resize(N);
     for (int i = 0; i < N; ++i) T.node_inv[T.node_perm[i]] = i;
+
+    // incidence lists regrouped by internal node, and connectivity in internal node ids
+    std::vector<int32_t>& iptr = S.iptr;
+    std::vector<int32_t>& iinc = S.iinc;
+    iptr.resize(N + 1);
+    iinc.resize(K * (size_t)E);
+    iptr[0] = 0;
+    for (int p = 0; p < N; ++p) {
+        const int n = T.node_perm[p];
+        const int a = inc_ptr[n], b = inc_ptr[n + 1];
+        std::memcpy(iinc.data() + iptr[p], inc.data() + a, 4 * (size_t)(b - a));
+        iptr[p + 1] = iptr[p] + (b - a);
+    }
+    std::vector<int32_t>& cint = S.cint;
+    cint.resize(K * (size_t)E);
+    for (int64_t i = 0; i < K * (int64_t)E; ++i) cint[i] = T.node_inv[d->conn[i]];
 
     // greedy tiles over internal node order: at most tile_nodes owned nodes, tile_slots
     // incident elements and node_cap nodes in LDS (owned + nodes of incident elements)
-    std::vector<int32_t> stamp(E, -1), seen(E, -1), nstamp(N, -1), nseen(N, -1);
-    std::vector<int32_t> tile_start;  // internal node index where each tile starts
+    std::vector<int32_t>& stamp = S.stamp;
+    std::vector<int32_t>& seen = S.seen;
+    std::vector<int32_t>& nstamp = S.nstamp;
+    std::vector<int32_t>& nseen = S.nseen;
+    stamp.assign(E, -1);
+    seen.assign(E, -1);
+    nstamp.assign(N, -1);  // indexed by internal node
+    nseen.assign(N, -1);
+    std::vector<int32_t>& tile_start = S.tile_start;  // internal node index where each tile starts
+    tile_start.clear();
     {
         int tid = 0, cur_nodes = 0, cur_slots = 0, cur_lds = 0;
         if (N > 0) tile_start.push_back(0);
         for (int p = 0; p < N; ++p) {
-            const int n = T.node_perm[p];
-            // what adding node n costs, relative to the current tile (add*) and to an empty one (all*)
+            // what adding node p costs, relative to the current tile (add*) and to an empty one (all*)
             int add = 0, all = 0, addn = 0, alln = 0;
-            if (nseen[n] != p) {
-                nseen[n] = p;
+            if (nseen[p] != p) {
+                nseen[p] = p;
                 ++alln;
-                if (nstamp[n] != tid) ++addn;
+                if (nstamp[p] != tid) ++addn;
             }
-            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
-                const int e = inc[j] / K;
+            for (int j = iptr[p]; j < iptr[p + 1]; ++j) {
+                const int e = iinc[j] / K;
                 if (seen[e] == p) continue;
                 seen[e] = p;
                 ++all;
                 if (stamp[e] != tid) ++add;
                 for (int c = 0; c < K; ++c) {
-                    const int q = d->conn[K * e + c];
+                    const int q = cint[K * e + c];
                     if (nseen[q] == p) continue;
                     nseen[q] = p;
                     ++alln;
@@ -216,13 +271,13 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
                 addn = alln;
             }
             if (add > tile_slots || addn > node_cap)
-                return fail(FW_E_INVALID, "node " + std::to_string(n) + " touches " + std::to_string(add) +
+                return fail(FW_E_INVALID, "node " + std::to_string(T.node_perm[p]) + " touches " + std::to_string(add) +
                                               " elements, more than one tile holds");
-            nstamp[n] = tid;
-            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
-                const int e = inc[j] / K;
+            nstamp[p] = tid;
+            for (int j = iptr[p]; j < iptr[p + 1]; ++j) {
+                const int e = iinc[j] / K;
                 stamp[e] = tid;
-                for (int c = 0; c < K; ++c) nstamp[d->conn[K * e + c]] = tid;
+                for (int c = 0; c < K; ++c) nstamp[cint[K * e + c]] = tid;
             }
             cur_nodes += 1;
             cur_slots += add;
@@ -230,41 +285,58 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         }
     }
     const int ntiles = (int)tile_start.size();
-    std::vector<int32_t> node_tile(N);
+    std::vector<int32_t>& node_tile = S.node_tile;
+    node_tile.resize(N);
     for (int t = 0; t < ntiles; ++t) {
         int p1 = t + 1 < ntiles ? tile_start[t + 1] : N;
         for (int p = tile_start[t]; p < p1; ++p) node_tile[p] = t;
     }
 
     // element owner = tile of its lowest internal node; store grouped by owner
-    std::vector<int32_t> owner(E);
-    std::vector<int32_t> own_count(ntiles + 1, 0);
+    std::vector<int32_t>& owner = S.owner;
+    std::vector<int32_t>& own_count = S.own_count;
+    owner.resize(E);
+    own_count.assign(ntiles + 1, 0);
     for (int e = 0; e < E; ++e) {
-        int m = T.node_inv[d->conn[K * e]];
-        for (int c = 1; c < K; ++c) m = std::min(m, T.node_inv[d->conn[K * e + c]]);
+        int m = cint[K * e];
+        for (int c = 1; c < K; ++c) m = std::min(m, cint[K * e + c]);
         owner[e] = node_tile[m];
         own_count[owner[e] + 1]++;
     }
     for (int t = 0; t < ntiles; ++t) own_count[t + 1] += own_count[t];
     T.elem_perm.resize(E);
     {
-        std::vector<int32_t> fillp(own_count.begin(), own_count.end() - 1);
+        std::vector<int32_t>& fillp = S.slot_of;  // re-initialised below
+        fillp.assign(own_count.begin(), own_count.end() - 1);
         for (int e = 0; e < E; ++e) T.elem_perm[fillp[owner[e]]++] = e;  // ascending caller id inside a tile
     }
-    T.elem_inv.assign(E, 0);
+    T.elem_inv.resize(E);
     for (int i = 0; i < E; ++i) T.elem_inv[T.elem_perm[i]] = i;
 
     // per tile: slot records, halo node list, node CSR
     T.tiles.resize(ntiles);
     std::fill(stamp.begin(), stamp.end(), -1);
-    std::vector<int32_t> slot_of(E, -1), lid(N, -1), nst(N, -1);
+    std::vector<int32_t>& slot_of = S.slot_of;
+    std::vector<int32_t>& lid = S.lid;  // by internal node
+    std::vector<int32_t>& nst = S.nst;  // by internal node
+    slot_of.assign(E, -1);
+    lid.assign(N, -1);
+    nst.assign(N, -1);
     // scratch reused by every tile, and the outputs sized for a typical halo up front
-    std::vector<int32_t> slots, hl, hn;
-    std::vector<uint16_t> rank;
+    std::vector<int32_t>& slots = S.slots;
+    std::vector<int32_t>& hl = S.hl;
+    std::vector<int32_t>& hn = S.hn;
+    std::vector<uint16_t>& rank = S.rank;
+    T.rec.clear();
+    T.hel.clear();
+    T.csr_ptr.clear();
+    T.hnode.clear();
     T.rec.reserve(2 * ((size_t)E + E / 4 + 64));
     T.hel.reserve((size_t)E / 4 + 64);
     T.csr_ptr.reserve((size_t)N + ntiles);
     T.hnode.reserve((size_t)N / 4 + 64);
+    T.halo_slots = 0;
+    const int max_rank = K == 3 ? FW_REC_MAX_RANK_TRI : FW_REC_MAX_RANK;
     for (int t = 0; t < ntiles; ++t) {
         TileDev& td = T.tiles[t];
         std::memset(&td, 0, sizeof(td));
@@ -283,14 +355,11 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         slots.clear();  // internal element index per slot
         for (int i = td.elem0; i < td.elem0 + td.n_own; ++i) slots.push_back(i);
         hl.clear();
-        for (int p = p0; p < p1; ++p) {
-            const int n = T.node_perm[p];
-            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {
-                int e = inc[j] / K;
-                if (owner[e] != t && stamp[e] != t) {
-                    stamp[e] = t;
-                    hl.push_back(T.elem_inv[e]);
-                }
+        for (int j = iptr[p0]; j < iptr[p1]; ++j) {
+            int e = iinc[j] / K;
+            if (owner[e] != t && stamp[e] != t) {
+                stamp[e] = t;
+                hl.push_back(T.elem_inv[e]);
             }
         }
         std::sort(hl.begin(), hl.end());
@@ -301,21 +370,20 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         if (nslots > tile_slots) return fail(FW_E_INVALID, "internal tiling error: slot overflow");
         for (int s = 0; s < nslots; ++s) slot_of[T.elem_perm[slots[s]]] = s;
         // local node ids: owned nodes first (p - p0), then halo nodes in ascending storage order
-        for (int p = p0; p < p1; ++p) lid[T.node_perm[p]] = p - p0;
+        for (int p = p0; p < p1; ++p) lid[p] = p - p0;
         hn.clear();
         for (int s = 0; s < nslots; ++s) {
             const int e = T.elem_perm[slots[s]];
             for (int c = 0; c < K; ++c) {
-                const int n = d->conn[K * e + c];
-                const int q = T.node_inv[n];
-                if ((q < p0 || q >= p1) && nst[n] != t) {
-                    nst[n] = t;
+                const int q = cint[K * e + c];
+                if ((q < p0 || q >= p1) && nst[q] != t) {
+                    nst[q] = t;
                     hn.push_back(q);
                 }
             }
         }
         std::sort(hn.begin(), hn.end());
-        for (size_t k = 0; k < hn.size(); ++k) lid[T.node_perm[hn[k]]] = td.n_nodes + (int)k;
+        for (size_t k = 0; k < hn.size(); ++k) lid[hn[k]] = td.n_nodes + (int)k;
         if (td.n_nodes + (int)hn.size() > node_cap) return fail(FW_E_INVALID, "internal tiling error: node overflow");
         td.hnode_off = (int32_t)T.hnode.size();
         td.n_hnodes = (int32_t)hn.size();
@@ -328,15 +396,13 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         int cnt = 0;
         T.csr_ptr.push_back(0);
         for (int p = p0; p < p1; ++p) {
-            const int n = T.node_perm[p];
-            const int max_rank = K == 3 ? FW_REC_MAX_RANK_TRI : FW_REC_MAX_RANK;
-            if (inc_ptr[n + 1] - inc_ptr[n] > max_rank + 1)
-                return fail(FW_E_UNSUPPORTED, "node " + std::to_string(n) + " has " +
-                                                  std::to_string(inc_ptr[n + 1] - inc_ptr[n]) + " element corners; the substep kernel "
+            if (iptr[p + 1] - iptr[p] > max_rank + 1)
+                return fail(FW_E_UNSUPPORTED, "node " + std::to_string(T.node_perm[p]) + " has " +
+                                                  std::to_string(iptr[p + 1] - iptr[p]) + " element corners; the substep kernel "
                                                   "supports at most " + std::to_string(max_rank + 1));
-            for (int j = inc_ptr[n]; j < inc_ptr[n + 1]; ++j) {  // ascending caller element id, then corner
-                int e = inc[j] / K, c = inc[j] % K;
-                rank[K * (size_t)slot_of[e] + c] = (uint16_t)(j - inc_ptr[n]);
+            for (int j = iptr[p]; j < iptr[p + 1]; ++j) {  // ascending caller element id, then corner
+                int e = iinc[j] / K, c = iinc[j] % K;
+                rank[K * (size_t)slot_of[e] + c] = (uint16_t)(j - iptr[p]);
                 ++cnt;
             }
             if (cnt > 65535) return fail(FW_E_INVALID, "internal tiling error: CSR overflow");
@@ -353,7 +419,7 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
             uint32_t w[3] = {0u, 0u, 0u};
             uint32_t hi = 0u;  // triangles: rank bits 5..9 of each corner
             for (int c = 0; c < K; ++c) {
-                const uint32_t l = (uint32_t)lid[d->conn[K * e + c]];
+                const uint32_t l = (uint32_t)lid[cint[K * e + c]];
                 if (l > FW_REC_ID_MASK) return fail(FW_E_INVALID, "internal tiling error: local id overflow");
                 const uint32_t rk = rank[K * (size_t)s + c];
                 w[c] = l | ((rk & 31u) << FW_REC_ID_BITS);
@@ -367,22 +433,26 @@ int build_tiling_k(const fw_scene_desc* d, int tile_nodes, int tile_slots, int n
         for (int s = td.n_own; s < nslots; ++s) T.hel.push_back(slots[s]);
     }
     // push lists: every halo copy is written by the tile owning its node, right after that
-    // tile computes the node's new position
-    std::vector<std::vector<int32_t>> by_src(ntiles);
+    // tile computes the node's new position; grouped by that tile, in (reading tile,
+    // halo copy) order
+    std::vector<int32_t>& src_cnt = S.src_cnt;
+    src_cnt.assign(ntiles + 1, 0);
+    for (size_t j = 0; j < T.hnode.size(); ++j) src_cnt[node_tile[T.hnode[j]] + 1]++;
+    for (int t = 0; t < ntiles; ++t) src_cnt[t + 1] += src_cnt[t];
+    T.push.resize(2 * T.hnode.size());
+    for (int t = 0; t < ntiles; ++t) {
+        T.tiles[t].push_off = src_cnt[t];
+        T.tiles[t].n_push = src_cnt[t + 1] - src_cnt[t];
+    }
     for (int t = 0; t < ntiles; ++t) {
         const TileDev& td = T.tiles[t];
         for (int h = 0; h < td.n_hnodes; ++h) {
             const int q = T.hnode[td.hnode_off + h];
             const int o = node_tile[q];
-            by_src[o].push_back(q - T.tiles[o].node0);
-            by_src[o].push_back(td.hnode_off + h);
+            const int at = src_cnt[o]++;
+            T.push[2 * (size_t)at] = q - T.tiles[o].node0;
+            T.push[2 * (size_t)at + 1] = td.hnode_off + h;
         }
-    }
-    for (int t = 0; t < ntiles; ++t) {
-        TileDev& td = T.tiles[t];
-        td.push_off = (int32_t)(T.push.size() / 2);
-        td.n_push = (int32_t)(by_src[t].size() / 2);
-        T.push.insert(T.push.end(), by_src[t].begin(), by_src[t].end());
     }
     return FW_OK;
 }
