@@ -96,7 +96,7 @@ class SimServer:
     """
 
     def __init__(self, batch=None, frame=60, keep_results=True, clock=time.perf_counter, prep_workers=None,
-                 min_frame=8, **batch_kw):
+                 min_frame=15, **batch_kw):
         self.batch = batch or FemBatch(frame=frame, **batch_kw)
         self.frame = int(frame)
         min_frame = max(1, min(int(min_frame), self.frame))
@@ -140,10 +140,11 @@ class SimServer:
     def run(self, jobs, max_wall=None):
         """Serve ``jobs`` (arrival times relative to the start of this call); returns the results.
 
-        One frame is in flight at a time.  While it runs, the host admits the
-        jobs that arrived meanwhile (tiling, packing, uploads queued behind the
-        frame on the arena stream), so admission work overlaps device work; the
-        admitted jobs join at the next frame.
+        Up to two frames are queued on the device.  While they run, the host
+        admits the jobs that arrived meanwhile (tiling and packing on the
+        preparation threads, uploads queued behind the frames on the arena
+        stream), so admission work overlaps device work; the admitted jobs join at
+        the next frame.
         """
         pending = collections.deque(sorted(jobs, key=lambda j: j.arrival))
         waiting = collections.deque()
