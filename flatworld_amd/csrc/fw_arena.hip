@@ -2348,8 +2348,7 @@ int32_t fw_flex_add(fw_arena* a, const fw_flex_desc* d, int32_t* group_id) {
         (W > 0 && (!d->item_body || !d->item_node || !d->item_target || !d->item_pair || !d->item_penalty ||
                    !d->item_friction)))
         return fail(FW_E_INVALID, "group arrays are required");
-    if (B > 256 || S >= (1 << 21))
-        return fail(FW_E_UNSUPPORTED, "a group holds at most 256 bodies and 2^21 - 1 boundary segments");
+    if (S >= (1 << 29)) return fail(FW_E_UNSUPPORTED, "a group holds at most 2^29 - 1 boundary segments");
     if (d->max_cells < 1 || d->max_query < 1 || d->max_pairs < 1) return fail(FW_E_INVALID, "hash limits must be >= 1");
     HIPCHK(hipSetDevice(a->device));
 

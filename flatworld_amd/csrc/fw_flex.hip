@@ -361,7 +361,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_hash_kernel(const FlexDev* gr
         for (int i = b; i < e; ++i) {
             const int sg = F.sorted[i];
             const int4 r = F.seg_cells[sg];
-            F.cell_ent[i] = sg << 10 | F.seg_body[sg] << 2 | (cx == r.x) << 1 | (cy == r.y);
+            F.cell_ent[i] = sg << 2 | (cx == r.x) << 1 | (cy == r.y);
         }
     }
     if (tid == 0) {
@@ -418,13 +418,13 @@ struct Best {
 
 // Walk the query cells lane, lane + stride, ... (x outer, y inner), ranges loaded
 // WALK_BATCH at a time, and keep the best qualifying candidate among the segments
-// of body tb met for the first time; `met` counts those first meetings.  A single
+// [tlo, thi) (the target body's) met for the first time; `met` counts those first meetings.  A single
 // lane (stride 1) stops evaluating once max_query segments were met, as the
 // reference's query list stops growing.  The candidate rule is the contact
 // kernel's (deepest penetration below 1e-4) or, with NEAREST, the tie
 // initialisation's (smallest |penetration|, flatworld/mixedcontact.py:757-762).
 template <bool NEAREST>
-__device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, float2 p, int tb, const float2* tpos,
+__device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, float2 p, int tlo, int thi, const float2* tpos,
                      float qbuf, int lane, int stride, bool exact, int max_query, int& met, Best& best) {
     const int nyq = q.uy - q.ly + 1, ncell = (q.ux - q.lx + 1) * nyq;
     for (int i0 = 0; i0 * stride + lane < ncell; i0 += WALK_BATCH) {
@@ -442,8 +442,8 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
             const int e = cr[u].y;
             for (int pp = cr[u].x; pp < e; ++pp) {
                 const int ce = F.cell_ent[pp];
-                if (((ce >> 2) & 0xFF) != tb) continue;
-                const int s = ce >> 10;
+                const int s = ce >> 2;
+                if (s < tlo || s >= thi) continue;  // a segment of another body
                 // first meeting of s in the walk: the first cell of the walk inside the
                 // segment's box, i.e. its first column or the walk's, and its first row
                 // or the walk's
@@ -562,7 +562,7 @@ __global__ __launch_bounds__(FLEX_BLOCK) void flex_ties_kernel(const FlexDev* gr
     const CellRange q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
     int met = 0;
     Best b;
-    walk<true>(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, b);
+    walk<true>(F, g, q, p, F.body_seg_off[tb], F.body_seg_off[tb + 1], tpos, qbuf, 0, 1, g.exact, F.max_query, met, b);
     if (!b.found) return;
     const int2 sn = F.seg_node[b.s];
     const float2 a = tpos[sn.x], c = tpos[sn.y];
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
                 if (be_cnt <= 2 * ncell)
                     scan_segments(F, q, p, be_off, be_cnt, tpos, qbuf, sub, FLEX_LPI, met, lb);
                 else
-                    walk<false>(F, g, q, p, tb, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
+                    walk<false>(F, g, q, p, be_off, be_off + be_cnt, tpos, qbuf, sub, FLEX_LPI, true, F.max_query, met, lb);
 #pragma unroll
                 for (int off = 1; off < FLEX_LPI; off <<= 1) met += __shfl_xor(met, off, FLEX_LPI);
                 if (met > F.max_query) {
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             if (serial) {
                 int met = 0;
                 Best sb;
-                walk<false>(F, g, q, p, tb, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
+                walk<false>(F, g, q, p, be_off, be_off + be_cnt, tpos, qbuf, 0, 1, g.exact, F.max_query, met, sb);
                 if (sb.found) {
                     b = sb;
                     cache_new = sb.s;

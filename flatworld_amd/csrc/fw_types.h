@@ -118,8 +118,9 @@ struct FlexDev {
     int2* cell_rng;                // [max_cells] (start, end) together: one load per cell in a query
     int32_t* sorted;               // [max_pairs] segment ids, ascending within a cell
     int32_t* cell_ent;             // [max_pairs] the same entries packed for the query walk:
-                                   // segment << 10 | body << 2 | (cell is the segment's first
-                                   // column) << 1 | (first row)
+                                   // segment << 2 | (cell is the segment's first column) << 1
+                                   // | (first row); a body's segments are one id range
+                                   // (body_seg_off), so the walk tests the body by range
     // per-substep scratch of flex_contact_kernel
     int2* res_i;                   // [W] (hit, segment)
     float4* res_f;                 // [W] (force x, y, w0, w1)

@@ -520,3 +520,55 @@ def test_spring_fem_pairs_gate_fem_fem_contact_bitexact():
     o = oracle_of(loop)
     contact = run_frames(loop, o, 3, check_every=1, what="FEM-spring gate")
     assert contact > 0
+
+
+def test_group_of_300_bodies_bitexact():
+    """One FEM-FEM group of 300 bodies (75 stacks of four small blocks on a plane,
+    each block paired with the one above it): more bodies than eight bits of a
+    packed hash entry could name; the query walk tells the target body by its
+    segment range."""
+    from flatworld_amd import workloads as W
+    from flatworld_amd.definitions import ContactType
+    from flatworld_amd.femcontact import ContactFlexFlex, contactHelper
+    from flatworld_amd.flexcontact import build_flex_group
+    from flatworld_amd.scene import add_flex_record, pack_domain
+
+    mat = fw.Elastic(2e4, 0.3, 100.0)
+    plane = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    bodies, pairs = [], []
+    for col in range(75):
+        x0 = 0.02 * col
+        for lvl in range(4):
+            b = ccw_grid(2, (x0 + 0.002 * (lvl % 2), 0.0161 * lvl), 0.016, mat, [fw.Gravity([0, -9.8])],
+                         initials=[fw.InitialVel("ALL", [0.0, -0.05 * lvl])], friction=0.1 * (lvl % 2))
+            if lvl:
+                pairs.append(ContactFlexFlex(b, bodies[-1], ContactType.FLEXFLEX))
+            bodies.append(b)
+    group = build_flex_group(bodies, pairs, True, sum(b.nnodes for b in bodies))
+    assert group.n_bodies == 300 and group.n_pairs == 225
+    grounds = [(plane, contactHelper(plane, b, plane.type | b.type)) for b in bodies]
+    dt = 0.2 * min([1e-3] + [b.stableTime for b in bodies] + [p.stableTime for p in pairs] + [c.stableTime for _, c in grounds])
+    specs = [add_flex_record(pack_domain(b, [grounds[k]], dt=dt), group.flex_cap(k)) for k, b in enumerate(bodies)]
+    arena = _native.Arena(tile_nodes=256, tile_slots=512, block=256, grid_per_cu=0)
+    sids = [arena.add(s) for s in specs]
+    gid = arena.flex_add(group, sids)
+    log(f"[case] 300-body group, dt={dt:.3e}, items={group.n_items}")
+    arena.step(1, -1.0, 0)  # warm-up
+    arena.flex_rebuild(gid)
+    o = OracleLoop(specs, group)
+    elapsed = 0.0
+    for k in range(300):
+        elapsed += dt
+        if elapsed >= group.rebuild_interval:
+            arena.flex_rebuild(gid)
+            elapsed = 0.0
+        arena.step(1, -1.0, 0)
+        o.substep(dt)
+    hits = int((o.flex.cache >= 0).sum())
+    assert all(np.isfinite(b.x).all() for b in o.bodies)
+    assert np.array_equal(arena.flex_read(gid, _native.FLEX_CACHE), o.flex.cache)
+    for k, (sid, body) in enumerate(zip(sids, o.bodies)):
+        x = arena.read(sid, _native.FIELD_X)
+        assert np.array_equal(x, body.x), f"300-body group: body {k} differs (max {np.abs(x - body.x).max():.3e})"
+    assert hits > 0
+    arena.close()
