@@ -38,7 +38,9 @@ Printed JSON line:
                    its roofline fraction and the oracle on the same scene as CPU baseline
   cpu_baseline     the oracle (serial C restatement) on a bounded sample
 roofline.frac uses the live kernel time (HIP events); roofline.frac_wall uses the
-wall-clock time per substep of the timed region.
+wall-clock time per substep of the timed region; roofline.copy_ceiling is the fastest
+read + write stream the device sustains (a tuned copy), the practical ceiling beside
+the 8 TB/s peak.
 """
 
 import argparse
@@ -628,7 +630,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is not part of it", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from flatworld_amd import autotune
+    from flatworld_amd import _native, autotune
     from flatworld_amd import workloads as W
 
     t_setup = time.perf_counter()
@@ -694,6 +696,8 @@ def main():
 
     shape = {k: tuning[k] for k in ("tile_nodes", "tile_slots", "block")}
     traffic, traffic_src = traffic_from_profile(shape, f"{args.config}-{len(specs)}")
+    ceiling = _native.bandwidth_ceiling(0, 1 << 30, 5)
+    log(f"[bench] copy ceiling: {ceiling}")
     out = {
         "metric": "element-substeps/s, batched vs per-scene; p50/p99 request latency @1e-4 tol",
         "value": value,
@@ -726,8 +730,14 @@ def main():
             "frac_wall": algo / (total_s / args.steps) / 1e9 / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": traffic_src,
+            "traffic_GBps": traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None,
             "kernel_ms": kernel_ms,
             "algo_bytes_per_substep": algo,
+            # the fastest read + write stream this device sustains (tuned copy), the
+            # practical ceiling beside the 8 TB/s peak
+            "copy_ceiling": ceiling,
+            "frac_of_copy_ceiling": achieved / ceiling["GBps"],
+            "traffic_frac_of_copy_ceiling": traffic / (kernel_ms * 1e-3) / 1e9 / ceiling["GBps"] if traffic else None,
         },
         "per_scene": per_scene,
         "serving": serving,

@@ -190,6 +190,9 @@ SIGNATURES = {
     "fw_scene_write": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_tiling_check": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "fw_bandwidth_probe": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "fw_bandwidth_ceiling": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_int32)]),
     "fw_time_steps": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "fw_scene_prepare": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_void_p)]),
     "fw_prepared_footprint": (ctypes.c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
@@ -342,6 +345,16 @@ def tiling_check(spec, tile_nodes=512, tile_slots=1024):
     stats = (ctypes.c_int64 * 4)()
     check(load().fw_tiling_check(ctypes.byref(d), int(tile_nodes), int(tile_slots), stats), "fw_tiling_check")
     return dict(tiles=stats[0], halo_slots=stats[1], max_slots=stats[2], csr_entries=stats[3])
+
+
+def bandwidth_ceiling(device=0, nbytes=1 << 30, reps=10):
+    """Fastest streaming copy over loads in flight per thread and grid size:
+    dict(GBps, kernel_ms, grid_per_cu, ilp); GBps counts bytes read + written."""
+    g, k = ctypes.c_double(), ctypes.c_double()
+    gp, il = ctypes.c_int32(), ctypes.c_int32()
+    check(load().fw_bandwidth_ceiling(int(device), int(nbytes), int(reps), ctypes.byref(g), ctypes.byref(k),
+                                      ctypes.byref(gp), ctypes.byref(il)), "fw_bandwidth_ceiling")
+    return dict(GBps=g.value, kernel_ms=k.value, grid_per_cu=gp.value, ilp=il.value)
 
 
 def bandwidth_probe(device=0, nbytes=1 << 30, reps=20):

@@ -41,6 +41,8 @@ extern "C" hipError_t fw_launch_prepare(const SceneDev* scenes, const TileDev* t
 extern "C" hipError_t fw_launch_advance_clock(ArenaCtl* ctl, int n, hipStream_t stream);
 extern "C" hipError_t fw_substep_occupancy(int block, int slots, int npt, int* per_cu);
 extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n_float4, int grid, hipStream_t stream);
+extern "C" hipError_t fw_launch_stream_copy(const void* src, void* dst, int64_t n_float4, int grid, int ilp,
+                                            hipStream_t stream);
 extern "C" hipError_t fw_launch_mass_scaling(const SceneDev* scenes, const TileDev* tiles, int tile_count, int buf,
                                              float dt_target, int block, int slots, hipStream_t stream);
 extern "C" hipError_t fw_launch_flex_hash(const FlexDev* groups, int n_groups, const SceneDev* scenes,
@@ -2743,6 +2745,54 @@ int32_t fw_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double* 
     (void)hipFree(src);
     (void)hipFree(dst);
     return FW_OK;
+}
+
+int32_t fw_bandwidth_ceiling(int32_t device, int64_t bytes, int32_t reps, double* gbs, double* kernel_ms,
+                             int32_t* grid_per_cu, int32_t* ilp) {
+    if (bytes < 4096 || reps < 1 || !gbs || !kernel_ms || !grid_per_cu || !ilp) return fail(FW_E_INVALID, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    const int64_t n4 = bytes / 16;
+    void *src = nullptr, *dst = nullptr;
+    HIPCHK(hipMalloc(&src, n4 * 16));
+    HIPCHK(hipMalloc(&dst, n4 * 16));
+    HIPCHK(hipMemset(src, 0, n4 * 16));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    *gbs = 0.0;
+    *kernel_ms = 0.0;
+    int rc = FW_OK;
+    for (int g : {2, 4, 8, 16, 32}) {
+        for (int u : {1, 2, 4, 8}) {
+            const int grid = prop.multiProcessorCount * g;
+            hipError_t e = fw_launch_stream_copy(src, dst, n4, grid, u, nullptr);  // first touch / warm
+            if (e == hipSuccess) e = hipEventRecord(t0, nullptr);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = fw_launch_stream_copy(src, dst, n4, grid, u, nullptr);
+            if (e == hipSuccess) e = hipEventRecord(t1, nullptr);
+            if (e == hipSuccess) e = hipEventSynchronize(t1);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+            if (e != hipSuccess) {
+                rc = fail(FW_E_HIP, std::string("stream copy: ") + hipGetErrorString(e));
+                break;
+            }
+            const double k = ms / reps, rate = 2.0 * (double)(n4 * 16) / (k * 1e-3) / 1e9;
+            if (rate > *gbs) {
+                *gbs = rate;
+                *kernel_ms = k;
+                *grid_per_cu = g;
+                *ilp = u;
+            }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return rc;
 }
 
 }  // extern "C"

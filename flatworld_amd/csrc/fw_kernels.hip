@@ -793,6 +793,24 @@ __global__ __launch_bounds__(256) void copy_probe_kernel(const float4* __restric
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// The same copy with ILP float4 loads per thread issued before their stores, so a
+// wave keeps ILP x 1 KiB in flight; fw_bandwidth_ceiling sweeps ILP and the grid
+// for the fastest read + write stream the device sustains.
+template <int ILP>
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                          int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (ILP - 1) * stride < n; i += ILP * stride) {
+        float4 v[ILP];
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < ILP; ++u) dst[i + u * stride] = v[u];
+    }
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void advance_clock_kernel(ArenaCtl* ctl, int n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl->clock += n;
 }
@@ -867,6 +885,22 @@ extern "C" hipError_t fw_launch_copy_probe(const void* src, void* dst, int64_t n
     hipLaunchKernelGGL(fw::copy_probe_kernel, dim3(grid), dim3(256), 0, stream, (const float4*)src, (float4*)dst,
                        n_float4);
     return hipGetLastError();
+}
+
+extern "C" hipError_t fw_launch_stream_copy(const void* src, void* dst, int64_t n_float4, int grid, int ilp,
+                                            hipStream_t stream) {
+#define FW_SC(U)                                                                                              \
+    if (ilp == U) {                                                                                           \
+        hipLaunchKernelGGL(fw::stream_copy_kernel<U>, dim3(grid), dim3(256), 0, stream, (const float4*)src,    \
+                           (float4*)dst, n_float4);                                                            \
+        return hipGetLastError();                                                                             \
+    }
+    FW_SC(1)
+    FW_SC(2)
+    FW_SC(4)
+    FW_SC(8)
+#undef FW_SC
+    return hipErrorInvalidValue;
 }
 
 extern "C" hipError_t fw_launch_rigid_pass(const SceneDev* scenes, const int2* rows, int n_rows, const ArenaCtl* ctl,

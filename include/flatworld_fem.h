@@ -416,6 +416,14 @@ int32_t fw_time_steps(fw_arena* a, int32_t n, int32_t mode, double* total_ms, do
  * it measures the achievable HBM bandwidth the substep is compared against. */
 int32_t fw_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double* gbs, double* kernel_ms);
 
+/* Calibration: the fastest read+write stream of `bytes` bytes the device sustains,
+ * over a sweep of float4 loads in flight per thread (1, 2, 4, 8) and workgroups per
+ * CU (2 to 32), each `reps` times; reports that bandwidth, its kernel time and the
+ * winning grid and loads per thread.  No reference counterpart: it is the ceiling
+ * the substep's HBM rate is read against. */
+int32_t fw_bandwidth_ceiling(int32_t device, int64_t bytes, int32_t reps, double* gbs, double* kernel_ms,
+                             int32_t* grid_per_cu, int32_t* ilp);
+
 #ifdef __cplusplus
 }
 #endif
