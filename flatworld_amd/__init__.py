@@ -15,7 +15,7 @@ from .domains import FemDomain, GroundDomain, HeightFieldDomain, VoxelGridDomain
 from .femesher import FEMesher, structured_grid
 from .materials import Elastic, HyperElastic, MisesPlastic, SolidProp
 from .mesh import Mesh
-from .rigid import BallRigid, BoxRigid, CapsuleRigid, RigidBodyDomain
+from .rigid import BallRigid, BoxRigid, CapsuleRigid, MeshRigid, RigidBodyDomain
 from .scene import GroundSpec, SceneSpec, pack_domain, pack_fem_domain, pack_spring_domain
 from .springmass import SpringMassDomain
 

@@ -82,6 +82,29 @@ class GroundDesc(ctypes.Structure):
         ("rigid_omega", ctypes.c_float),
         ("rigid_friction", ctypes.c_float),
         ("rigid_tied", ctypes.c_int32),
+        ("mesh_coords", _f32p),
+        ("mesh_conn", _i32p),
+        ("mesh_elb", _f32p),
+        ("mesh_eub", _f32p),
+        ("mesh_margin", _f32p),
+        ("mesh_reg", _i32p),
+        ("mesh_cell", _i32p),
+        ("mesh_sorted", _i32p),
+        ("mesh_nodes", ctypes.c_int32),
+        ("mesh_edges", ctypes.c_int32),
+        ("mesh_nreg", ctypes.c_int32),
+        ("mesh_tc", ctypes.c_int32),
+        ("mesh_npairs", ctypes.c_int32),
+        ("mesh_cap_reg", ctypes.c_int32),
+        ("mesh_cap_cells", ctypes.c_int32),
+        ("mesh_cap_pairs", ctypes.c_int32),
+        ("mesh_glb", ctypes.c_float * 2),
+        ("mesh_gs", ctypes.c_float * 2),
+        ("mesh_cn", ctypes.c_int32 * 2),
+        ("mesh_did", ctypes.c_int32),
+        ("mesh_subcycle", ctypes.c_int32),
+        ("mesh_reset_at", ctypes.c_int32),
+        ("mesh_update", ctypes.c_int32),
     ]
 
 
@@ -270,6 +293,33 @@ def fill_rigid(d, g):
     d.rigid_tied = int(g.rigid_tied)
 
 
+RIGID_MESH = 16
+MESH_UPDATE_COORDS, MESH_UPDATE_EDGES, MESH_UPDATE_HASH = 1, 2, 4
+
+
+def fill_mesh(d, g, keep, update=MESH_UPDATE_COORDS | MESH_UPDATE_EDGES | MESH_UPDATE_HASH):
+    """The mesh_* fields of a GroundDesc from a mesh rigid GroundSpec: the current
+    arrays of its RigidMeshSet and the capacities the set reserves."""
+    m = g.mesh_set
+    arrs = [np.ascontiguousarray(m.coords, np.float32), np.ascontiguousarray(m.conn, np.int32),
+            np.ascontiguousarray(m.elb, np.float32), np.ascontiguousarray(m.eub, np.float32),
+            np.ascontiguousarray(m.margin, np.float32),
+            np.ascontiguousarray(m.reg if m.reg.size else np.zeros((1, 2), np.int32), np.int32),
+            np.ascontiguousarray(m.cell if m.cell.size else np.zeros((1, 2), np.int32), np.int32),
+            np.ascontiguousarray(m.sorted if m.sorted.size else np.zeros(1, np.int32), np.int32)]
+    keep.extend(arrs)
+    d.mesh_coords, d.mesh_elb, d.mesh_eub, d.mesh_margin = (_ptr(arrs[k], _f32p) for k in (0, 2, 3, 4))
+    d.mesh_conn, d.mesh_reg, d.mesh_cell, d.mesh_sorted = (_ptr(arrs[k], _i32p) for k in (1, 5, 6, 7))
+    d.mesh_nodes, d.mesh_edges = int(m.n_nodes), int(m.n_elems)
+    d.mesh_nreg, d.mesh_tc, d.mesh_npairs = int(m.reg.shape[0]), int(m.tc), int(m.sorted.shape[0])
+    d.mesh_cap_reg, d.mesh_cap_cells, d.mesh_cap_pairs = int(m.n_elems), int(m.max_cells), int(m.max_pairs)
+    d.mesh_glb[:] = [float(v) for v in m.glb]
+    d.mesh_gs[:] = [float(v) for v in m.gs]
+    d.mesh_cn[:] = [int(v) for v in m.cn]
+    d.mesh_did, d.mesh_subcycle, d.mesh_reset_at = int(g.mesh_did), int(g.mesh_subcycle), int(g.mesh_reset_at)
+    d.mesh_update = int(update)
+
+
 def scene_desc(spec, track_disp=True):
     """Build the C description of a SceneSpec; returns (desc, keepalive list)."""
     keep = []
@@ -308,6 +358,8 @@ def scene_desc(spec, track_disp=True):
             d.flex_cap = int(g.flex_cap)
         elif g.kind == GROUND_RIGID:
             fill_rigid(d, g)
+            if int(g.rigid_type) == RIGID_MESH:
+                fill_mesh(d, g, keep)
         gs.append(d)
     garr = (GroundDesc * max(len(gs), 1))(*gs)
     keep.append(garr)
@@ -513,15 +565,19 @@ class Arena:
         """Selective mass scaling of one scene at its current positions (asynchronous)."""
         check(self._L.fw_scene_mass_scaling(self.h, int(sid), float(np.float32(dt_target))), "fw_scene_mass_scaling")
 
-    def set_rigid(self, sid, ground_index, g):
+    def set_rigid(self, sid, ground_index, g, mesh_update=MESH_UPDATE_COORDS | MESH_UPDATE_EDGES | MESH_UPDATE_HASH):
         """Move rigid record ``ground_index`` of scene ``sid`` to the pose of GroundSpec ``g``
-        (stream-ordered: later substeps see it)."""
+        (stream-ordered: later substeps see it); a mesh record also takes the parts of
+        its mesh set that ``mesh_update`` selects, and its header."""
         d = GroundDesc()
         d.kind = int(g.kind)
         d.penalty = float(g.penalty)
         d.aabb_lo[:] = [float(v) for v in g.aabb_lo]
         d.aabb_hi[:] = [float(v) for v in g.aabb_hi]
         fill_rigid(d, g)
+        keep = []
+        if int(g.rigid_type) == RIGID_MESH:
+            fill_mesh(d, g, keep, mesh_update)
         check(self._L.fw_scene_set_rigid(self.h, int(sid), int(ground_index), ctypes.byref(d)), "fw_scene_set_rigid")
 
     def step_graph(self, n, flags=0):

@@ -517,6 +517,80 @@ float hf_skip_level(const float* h, int nx, bool reversed) {
 
 // ------------------------------------------------------------------ scene layout
 
+// A mesh rigid record's data block (offsets from the block start): the rigid pass's
+// float4 per node and the boundary-node ids as for a primitive, then the MeshDev
+// header, the per-item state and the mesh arrays at the capacities reserved at
+// admission.
+struct MeshLayout {
+    int64_t hdr, cache, skip, fric_i, fric_f, fric_pen, tie;
+    int64_t coords, conn, elb, eub, margin, reg, cell, sorted;
+    int64_t total;
+};
+
+MeshLayout mesh_layout(int64_t N, const fw_ground_desc& gd) {
+    MeshLayout M;
+    int64_t o = (20 * N + 15) / 16 * 16;
+    auto take = [&](int64_t bytes) {
+        int64_t r = o;
+        o += (bytes + 15) / 16 * 16;
+        return r;
+    };
+    M.hdr = take((int64_t)sizeof(MeshDev));
+    M.cache = take(8 * N);
+    M.skip = take(4 * N);
+    M.fric_i = take(8 * N);
+    M.fric_f = take(16 * N);
+    M.fric_pen = take(4 * N);
+    M.tie = take(16 * N);
+    M.coords = take(8 * (int64_t)gd.mesh_nodes);
+    M.conn = take(8 * (int64_t)gd.mesh_edges);
+    M.elb = take(8 * (int64_t)gd.mesh_edges);
+    M.eub = take(8 * (int64_t)gd.mesh_edges);
+    M.margin = take(4 * (int64_t)gd.mesh_edges);
+    M.reg = take(8 * (int64_t)std::max(gd.mesh_cap_reg, 1));
+    M.cell = take(8 * (int64_t)std::max(gd.mesh_cap_cells, 1));
+    M.sorted = take(4 * (int64_t)std::max(gd.mesh_cap_pairs, 1));
+    M.total = o;
+    return M;
+}
+
+// What the host keeps of a scene's mesh rigid record: its layout, counts and capacities.
+struct MeshHost {
+    bool used = false;
+    MeshLayout ml{};
+    int32_t nodes = 0, edges = 0, cap_reg = 0, cap_cells = 0, cap_pairs = 0, n_items = 0;
+};
+
+MeshDev mesh_header(const fw_ground_desc& gd, const MeshLayout& M, int32_t n_items) {
+    MeshDev H;
+    std::memset(&H, 0, sizeof(H));
+    for (int k = 0; k < 2; ++k) {
+        H.glb[k] = gd.mesh_glb[k];
+        H.gs[k] = gd.mesh_gs[k];
+        H.cn[k] = gd.mesh_cn[k];
+    }
+    H.tc = gd.mesh_tc;
+    H.did = gd.mesh_did;
+    H.subcycle = gd.mesh_subcycle;
+    H.reset_at = gd.mesh_reset_at;
+    H.n_items = n_items;
+    H.o_coords = M.coords;
+    H.o_conn = M.conn;
+    H.o_elb = M.elb;
+    H.o_eub = M.eub;
+    H.o_margin = M.margin;
+    H.o_reg = M.reg;
+    H.o_cell = M.cell;
+    H.o_sorted = M.sorted;
+    H.o_cache = M.cache;
+    H.o_skip = M.skip;
+    H.o_fric_i = M.fric_i;
+    H.o_fric_f = M.fric_f;
+    H.o_fric_pen = M.fric_pen;
+    H.o_tie = M.tie;
+    return H;
+}
+
 struct Layout {
     int64_t pos0, pos1, rest, vel, mass, flag, bcv, disp, force, cflag;
     int64_t jinv, area, j2a, j2b, stress, strain, yflag;
@@ -563,7 +637,8 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
         if (gd.kind == FW_GROUND_HEIGHTFIELD) L.gdata[g] = take(hf_bytes(gd.hf_nx));
         if (gd.kind == FW_GROUND_VOXEL) L.gdata[g] = take(8 * (int64_t)gd.vox_nx * gd.vox_ny);
         // rigid: the rigid pass's float4 per node, then the boundary nodes' internal ids
-        if (gd.kind == FW_GROUND_RIGID) L.gdata[g] = take(16 * N + 4 * (int64_t)N);
+        if (gd.kind == FW_GROUND_RIGID)
+            L.gdata[g] = take(gd.rigid_type == FW_RIGID_MESH ? mesh_layout(N, gd).total : 16 * N + 4 * (int64_t)N);
     }
     L.host_bytes = o;
     // FEM-FEM contributions: int2 (first, count) per node, then the float2 entries;
@@ -595,9 +670,39 @@ Layout make_layout(const fw_scene_desc* d, int64_t n_slots, int64_t n_hslots, in
     return L;
 }
 
+// A mesh record's arrays, checked so that no index the rigid pass follows leaves them.
+int validate_mesh(const fw_ground_desc& gd) {
+    auto bad = [](const char* what) { return fail(FW_E_INVALID, std::string("mesh rigid record: ") + what); };
+    if (gd.mesh_nodes < 1 || gd.mesh_edges < 1) return bad("needs boundary nodes and edges");
+    if (!gd.mesh_coords || !gd.mesh_conn || !gd.mesh_elb || !gd.mesh_eub || !gd.mesh_margin || !gd.mesh_reg ||
+        !gd.mesh_cell || !gd.mesh_sorted)
+        return bad("arrays are required");
+    if (gd.mesh_cap_reg < gd.mesh_nreg || gd.mesh_cap_cells < gd.mesh_tc || gd.mesh_cap_pairs < gd.mesh_npairs ||
+        gd.mesh_nreg < 0 || gd.mesh_tc < 0 || gd.mesh_npairs < 0)
+        return bad("hash sizes exceed the capacities");
+    if (gd.mesh_cn[0] < 1 || gd.mesh_cn[1] < 1 || (int64_t)gd.mesh_cn[0] * gd.mesh_cn[1] < gd.mesh_tc)
+        return bad("cell counts do not cover the cells");
+    if (!(gd.mesh_gs[0] > 0.0f) || !(gd.mesh_gs[1] > 0.0f)) return bad("cell size must be positive");
+    if (gd.mesh_subcycle < 1) return bad("subcycle must be >= 1");
+    for (int64_t e = 0; e < 2 * (int64_t)gd.mesh_edges; ++e)
+        if (gd.mesh_conn[e] < 0 || gd.mesh_conn[e] >= gd.mesh_nodes) return bad("edge node outside the nodes");
+    for (int64_t r = 0; r < gd.mesh_nreg; ++r)
+        if (gd.mesh_reg[2 * r + 1] < 0 || gd.mesh_reg[2 * r + 1] >= gd.mesh_edges) return bad("registry edge outside");
+    for (int64_t c = 0; c < gd.mesh_tc; ++c)
+        if (gd.mesh_cell[2 * c] < 0 || gd.mesh_cell[2 * c] > gd.mesh_cell[2 * c + 1] || gd.mesh_cell[2 * c + 1] > gd.mesh_npairs)
+            return bad("cell range outside the entries");
+    for (int64_t q = 0; q < gd.mesh_npairs; ++q)
+        if (gd.mesh_sorted[q] >= gd.mesh_nreg) return bad("cell entry outside the registry");
+    return FW_OK;
+}
+
 int validate_rigid(const fw_ground_desc& gd) {
+    if (gd.rigid_type == FW_RIGID_MESH) {
+        if (gd.rigid_tied != 0 && gd.rigid_tied != 1) return fail(FW_E_INVALID, "rigid record: rigid_tied must be 0 or 1");
+        return validate_mesh(gd);
+    }
     if (gd.rigid_type != FW_RIGID_BALL && gd.rigid_type != FW_RIGID_BOX && gd.rigid_type != FW_RIGID_CAPSULE)
-        return fail(FW_E_INVALID, "rigid record: type must be FW_RIGID_BALL, FW_RIGID_BOX or FW_RIGID_CAPSULE");
+        return fail(FW_E_INVALID, "rigid record: type must be FW_RIGID_BALL, FW_RIGID_BOX, FW_RIGID_CAPSULE or FW_RIGID_MESH");
     if (gd.rigid_type == FW_RIGID_CAPSULE && gd.rigid_prim[0] == 0.0f && gd.rigid_prim[1] == 0.0f)
         return fail(FW_E_INVALID, "rigid record: a capsule needs a nonzero axis");
     if (gd.rigid_tied != 0 && gd.rigid_tied != 1) return fail(FW_E_INVALID, "rigid record: rigid_tied must be 0 or 1");
@@ -743,6 +848,7 @@ struct SceneHost {
     std::vector<TileDev> tiles;  // scene-relative, scene id filled in
     int tile_base = 0;           // position in the arena tile table
     SceneDev dev;
+    MeshHost mesh[FW_MAX_GROUNDS];  // mesh rigid records
 };
 
 // A scene prepared on the host for an arena tuning (fw_scene_prepare).
@@ -803,6 +909,7 @@ struct fw_prepared {
     PinnedBuf* pin = nullptr;  // the same bytes in page-locked memory, after fw_prepared_stage
     SceneDev D{};           // scene entry without device addresses
     int32_t n_nodes = 0, n_elems = 0, mat_type = 0, horizon = 0;
+    MeshHost mesh[FW_MAX_GROUNDS];
     bool consumed = false;
 };
 
@@ -1002,7 +1109,8 @@ int ensure_tiles(fw_arena* a) {
         const auto& sp = a->scenes[id];
         if (!sp || !sp->live) continue;
         for (int g = 0; g < sp->dev.hot.n_grounds; ++g)
-            if (sp->dev.ground[g].kind == FW_GROUND_RIGID) rigid_rows.push_back(make_int2((int)id, g));
+            if (sp->dev.ground[g].kind == FW_GROUND_RIGID || sp->dev.ground[g].kind == FW_GROUND_RIGID_MESH_DEV)
+                rigid_rows.push_back(make_int2((int)id, g));
     }
     if ((int)rigid_rows.size() > a->rigid_cap) {
         const int cap = std::max((int)rigid_rows.size(), std::max(64, 2 * a->rigid_cap));
@@ -1290,6 +1398,41 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             for (int i = 0; i < N; ++i)
                 if (flag[i] & FW_NODE_BOUNDARY) bn[nb++] = i;
             G.reverse = nb;
+            if (gd.rigid_type == FW_RIGID_MESH) {
+                // header, per-item state at the reference's initial values
+                // (flatworld/mixedcontact.py:2052-2093) and the mesh arrays
+                G.kind = FW_GROUND_RIGID_MESH_DEV;
+                const MeshLayout M = mesh_layout(N, gd);
+                char* b = img + L.gdata[g];
+                const MeshDev H = mesh_header(gd, M, nb);
+                std::memcpy(b + M.hdr, &H, sizeof(H));
+                int32_t* cache = (int32_t*)(b + M.cache);
+                int32_t* skip = (int32_t*)(b + M.skip);
+                int32_t* fi = (int32_t*)(b + M.fric_i);
+                for (int i = 0; i < N; ++i) {
+                    cache[2 * i] = cache[2 * i + 1] = -1;
+                    skip[i] = -1;
+                    fi[2 * i] = -1;
+                    fi[2 * i + 1] = 0;
+                }
+                std::memcpy(b + M.coords, gd.mesh_coords, 8 * (size_t)gd.mesh_nodes);
+                std::memcpy(b + M.conn, gd.mesh_conn, 8 * (size_t)gd.mesh_edges);
+                std::memcpy(b + M.elb, gd.mesh_elb, 8 * (size_t)gd.mesh_edges);
+                std::memcpy(b + M.eub, gd.mesh_eub, 8 * (size_t)gd.mesh_edges);
+                std::memcpy(b + M.margin, gd.mesh_margin, 4 * (size_t)gd.mesh_edges);
+                std::memcpy(b + M.reg, gd.mesh_reg, 8 * (size_t)gd.mesh_nreg);
+                std::memcpy(b + M.cell, gd.mesh_cell, 8 * (size_t)gd.mesh_tc);
+                std::memcpy(b + M.sorted, gd.mesh_sorted, 4 * (size_t)gd.mesh_npairs);
+                MeshHost& mh = P->mesh[g];
+                mh.used = true;
+                mh.ml = M;
+                mh.nodes = gd.mesh_nodes;
+                mh.edges = gd.mesh_edges;
+                mh.cap_reg = gd.mesh_cap_reg;
+                mh.cap_cells = gd.mesh_cap_cells;
+                mh.cap_pairs = gd.mesh_cap_pairs;
+                mh.n_items = nb;
+            }
         } else {
             G.a[0] = gd.vox_lb[0];
             G.a[1] = gd.vox_lb[1];
@@ -1403,6 +1546,7 @@ int commit_scene(fw_arena* a, fw_prepared* p, int32_t* scene_id) {
     sh->horizon = D.horizon;
     sh->tiles = T.tiles;
     sh->dev = D;
+    for (int g = 0; g < FW_MAX_GROUNDS; ++g) sh->mesh[g] = p->mesh[g];
     sh->tiling = std::move(T);
     p->consumed = true;
     p->img.clear();
@@ -1700,7 +1844,8 @@ int32_t fw_scene_set_rigid(fw_arena* a, int32_t id, int32_t gi, const fw_ground_
     if (!a || !g) return fail(FW_E_INVALID, "null argument");
     SceneHost* s = get_scene(a, id);
     if (!s) return fail(FW_E_NOT_FOUND, "no scene " + std::to_string(id));
-    if (gi < 0 || gi >= s->dev.hot.n_grounds || s->dev.ground[gi].kind != FW_GROUND_RIGID)
+    if (gi < 0 || gi >= s->dev.hot.n_grounds ||
+        (s->dev.ground[gi].kind != FW_GROUND_RIGID && s->dev.ground[gi].kind != FW_GROUND_RIGID_MESH_DEV))
         return fail(FW_E_INVALID, "ground " + std::to_string(gi) + " of scene " + std::to_string(id) +
                                       " is not a rigid record");
     if (g->kind != FW_GROUND_RIGID) return fail(FW_E_INVALID, "descriptor kind must be FW_GROUND_RIGID");
@@ -1712,6 +1857,30 @@ int32_t fw_scene_set_rigid(fw_arena* a, int32_t id, int32_t gi, const fw_ground_
     pack_aabb(G, *g);
     pack_rigid(G, *g);
     HIPCHK(hipSetDevice(a->device));
+    if (G.kind == FW_GROUND_RIGID_MESH_DEV) {
+        // the header always, the arrays mesh_update selects; stream-ordered behind
+        // the substeps launched so far
+        const MeshHost& mh = s->mesh[gi];
+        if (g->mesh_nodes != mh.nodes || g->mesh_edges != mh.edges || g->mesh_cap_reg != mh.cap_reg ||
+            g->mesh_cap_cells != mh.cap_cells || g->mesh_cap_pairs != mh.cap_pairs)
+            return fail(FW_E_INVALID, "a mesh rigid record keeps its node and edge counts and hash capacities");
+        char* b = static_cast<char*>(const_cast<void*>(G.data));
+        const MeshLayout& M = mh.ml;
+        const MeshDev H = mesh_header(*g, M, mh.n_items);
+        rc = upload(a, b + M.hdr, &H, (int64_t)sizeof(H));
+        if (!rc && (g->mesh_update & 1)) rc = upload(a, b + M.coords, g->mesh_coords, 8 * (int64_t)g->mesh_nodes);
+        if (!rc && (g->mesh_update & 2)) {
+            rc = upload(a, b + M.elb, g->mesh_elb, 8 * (int64_t)g->mesh_edges);
+            if (!rc) rc = upload(a, b + M.eub, g->mesh_eub, 8 * (int64_t)g->mesh_edges);
+            if (!rc) rc = upload(a, b + M.margin, g->mesh_margin, 4 * (int64_t)g->mesh_edges);
+        }
+        if (!rc && (g->mesh_update & 4)) {
+            rc = upload(a, b + M.reg, g->mesh_reg, 8 * (int64_t)g->mesh_nreg);
+            if (!rc) rc = upload(a, b + M.cell, g->mesh_cell, 8 * (int64_t)g->mesh_tc);
+            if (!rc) rc = upload(a, b + M.sorted, g->mesh_sorted, 4 * (int64_t)g->mesh_npairs);
+        }
+        if (rc) return rc;
+    }
     rc = upload(a, &a->d_scenes[id].ground[gi], &G, (int64_t)sizeof(GroundDev));
     if (rc) return rc;
     s->dev.ground[gi] = G;

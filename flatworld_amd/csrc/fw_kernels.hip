@@ -433,8 +433,10 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                 const GroundDev gd = fw_ldc(&Sp->ground[gi]);
                 const float* lo = pad ? gd.plo : gd.lo;
                 const float* hi = pad ? gd.phi : gd.hi;
-                // the FEM-FEM record is gated per pair by the contact pass itself
-                if (gd.kind == FW_GROUND_FLEX_DEV || (lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1]))
+                // the FEM-FEM record is gated per pair by the contact pass itself, a mesh
+                // rigid record per work item by the rigid pass
+                if (gd.kind == FW_GROUND_FLEX_DEV || gd.kind == FW_GROUND_RIGID_MESH_DEV ||
+                    (lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1]))
                     active |= 1u << gi;
             }
         }
@@ -481,7 +483,7 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
                             fy = fy + c.y;
                         }
                     } else {
-                        if (gd.kind == FW_GROUND_RIGID_DEV) {
+                        if (gd.kind >= FW_GROUND_RIGID_DEV) {
                             // the rigid pass evaluated this node's force against the primitive
                             const float4 rf = fw_ld(static_cast<const float4*>(gd.data) + n);
                             if (rf.z != 0.0f) {
@@ -626,6 +628,325 @@ __global__ __launch_bounds__(BLOCK) void substep_kernel(StepArgs A) {
 // v_n, into the record's per-node slots, which the substep adds at the record's place
 // among the node's ground forces when the pair gate is active.  Kept out of the
 // substep kernel so that its registers do not pay for the primitive queries.
+// ---------------------------------------------------------------- mesh rigids
+// The FEM side of _batched_rigid_mesh_kernel_sh, its workset filter and the mesh tie
+// kernels (flatworld/mixedcontact.py:519-588, 1119-1565): one lane per work item.
+
+__device__ __forceinline__ float mdot(float ax, float ay, float bx, float by) { return (0.0f + ax * bx) + ay * by; }
+__device__ __forceinline__ float mlen(float x, float y) { return sqrtf(mdot(x, y, x, y)); }
+__device__ __forceinline__ float mmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float mmax(float a, float b) { return a > b ? a : b; }
+
+// _point_outside_aabb (mixedcontact.py:113-119)
+__device__ __forceinline__ bool m_outside(float px, float py, float l0, float l1, float u0, float u1) {
+    return mmin(px - l0, py - l1) < 0.0f || mmin(u0 - px, u1 - py) < 0.0f;
+}
+
+// _point_aabb_distance_sq (mixedcontact.py:122-132)
+__device__ __forceinline__ float m_dist_sq(float px, float py, float l0, float l1, float u0, float u1) {
+    float d2 = 0.0f, v = 0.0f;
+    if (px < l0) v = l0 - px;
+    else if (px > u0) v = px - u0;
+    d2 = d2 + v * v;
+    v = 0.0f;
+    if (py < l1) v = l1 - py;
+    else if (py > u1) v = py - u1;
+    return d2 + v * v;
+}
+
+// detectPointToMeshBoundary over pointToEdgeContact (flatworld/contact_detection.py:15-90):
+// the penetration when p projects inside the edge within `limit`, else 1e9; weights either way.
+__device__ __forceinline__ float m_detect(float px, float py, float2 a, float2 b, float limit, float& nx, float& ny,
+                                          float& w0, float& w1) {
+    const float ex = b.x - a.x, ey = b.y - a.y;
+    const float len = mlen(ex, ey);
+    float pen = 1e9f, n0 = 0.0f, n1 = 0.0f;
+    bool inside = false;
+    w0 = w1 = 0.0f;
+    if (len > 1e-9f) {
+        const float dx = ex / len, dy = ey / len;
+        n0 = dy;
+        n1 = -dx;
+        const float tx = px - a.x, ty = py - a.y;
+        float t = mdot(tx, ty, dx, dy);
+        t = t / len;
+        if (0.0f <= t && t <= 1.0f) {
+            inside = true;
+            pen = mdot(tx, ty, n0, n1);
+            w0 = 1.0f - t;
+            w1 = t;
+        }
+    }
+    if (fabsf(pen) < limit && inside) {
+        nx = n0;
+        ny = n1;
+        return pen;
+    }
+    nx = ny = 0.0f;
+    return 1e9f;
+}
+
+struct MeshView {
+    const float2* coords;
+    const int2* conn;
+    const float2 *elb, *eub;
+    const float* margin;
+    const int2* reg;
+    const int2* cell;
+    const int32_t* sorted;
+    int2* cache;
+    int32_t* skip;
+    int2* fric_i;
+    float4* fric_f;
+    float* fric_pen;
+    float4* tie;
+};
+
+struct MeshBest {
+    float pen, nx, ny, w0, w1;
+    int elem, did;
+    bool found, has_cand;
+};
+
+__device__ __forceinline__ void m_take(MeshBest& b, float pen, float nx, float ny, float w0, float w1, int e, int did) {
+    b.pen = pen;
+    b.nx = nx;
+    b.ny = ny;
+    b.w0 = w0;
+    b.w1 = w1;
+    b.elem = e;
+    b.did = did;
+    b.found = true;
+}
+
+// _process_rigid_sh_cell (mixedcontact.py:382-467): the edges of rigid `target` in
+// cell cid; every improvement is written to the item's cache.
+__device__ void m_cell(const MeshDev& H, const MeshView& V, int cid, float px, float py, int target, int item,
+                       MeshBest& b) {
+    if (!(cid >= 0 && cid < H.tc)) return;
+    const float best_abs = fabsf(b.pen);
+    if (b.found && best_abs < 1e8f) {
+        const int nxc = H.cn[0], ix = cid % nxc, iy = cid / nxc;
+        const float l0 = H.glb[0] + (float)ix * H.gs[0], l1 = H.glb[1] + (float)iy * H.gs[1];
+        const float u0 = l0 + H.gs[0], u1 = l1 + H.gs[1];
+        if (m_dist_sq(px, py, l0, l1, u0, u1) > best_abs * best_abs) return;
+    }
+    const int2 r = fw_ld(V.cell + cid);
+    for (int q = r.x; q < r.y; ++q) {
+        const int ei = fw_ld(V.sorted + q);
+        if (ei < 0) continue;
+        const int2 re = fw_ld(V.reg + ei);
+        if (re.x != target) continue;
+        b.has_cand = true;
+        const int e = re.y;
+        const int2 cn = fw_ld(V.conn + e);
+        const float2 a = fw_ld(V.coords + cn.x), c = fw_ld(V.coords + cn.y);
+        float l0 = 1e30f, l1 = 1e30f, u0 = -1e30f, u1 = -1e30f;
+        l0 = mmin(l0, a.x);
+        l1 = mmin(l1, a.y);
+        u0 = mmax(u0, a.x);
+        u1 = mmax(u1, a.y);
+        l0 = mmin(l0, c.x);
+        l1 = mmin(l1, c.y);
+        u0 = mmax(u0, c.x);
+        u1 = mmax(u1, c.y);
+        const float margin = mmax(1e-4f, fw_ld(V.margin + e));
+        l0 = l0 - margin;
+        l1 = l1 - margin;
+        u0 = u0 + margin;
+        u1 = u1 + margin;
+        const float best_abs2 = fabsf(b.pen);
+        if (b.found && best_abs2 < 1e8f && m_dist_sq(px, py, l0, l1, u0, u1) > best_abs2 * best_abs2) continue;
+        if (!m_outside(px, py, l0, l1, u0, u1)) {
+            float nx, ny, w0, w1;
+            const float pen = m_detect(px, py, a, c, margin, nx, ny, w0, w1);
+            if (fabsf(pen) < fabsf(b.pen)) {
+                m_take(b, pen, nx, ny, w0, w1, e, re.x);
+                fw_st(V.cache + item, make_int2(e, re.x));
+            }
+        }
+    }
+}
+
+// query cells of (px, py) widened by qb, clamped (mixedcontact.py:1226-1243)
+__device__ __forceinline__ int4 m_cells(const MeshDev& H, float px, float py, float qb) {
+    const float d0 = (px - qb) - H.glb[0], d1 = (py - qb) - H.glb[1];
+    const float e0 = (px + qb) - H.glb[0], e1 = (py + qb) - H.glb[1];
+    const int cx = H.cn[0] - 1, cy = H.cn[1] - 1;
+    int lx = (int)floorf(d0 / H.gs[0]), ly = (int)floorf(d1 / H.gs[1]);
+    int ux = (int)floorf(e0 / H.gs[0]), uy = (int)floorf(e1 / H.gs[1]);
+    lx = min(max(lx, 0), cx);
+    ly = min(max(ly, 0), cy);
+    ux = min(max(ux, 0), cx);
+    uy = min(max(uy, 0), cy);
+    return make_int4(lx, ly, ux, uy);
+}
+
+// _clear_rigid_mesh_friction_state (mixedcontact.py:347-361)
+__device__ __forceinline__ void m_clear(const MeshView& V, int k) {
+    fw_st(V.fric_i + k, make_int2(-1, 0));
+    fw_st(V.fric_f + k, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    fw_st(V.fric_pen + k, 0.0f);
+}
+
+// Force of a mesh record on work item k at p; false when it applies none.
+__device__ bool mesh_item(const GroundDev& g, const MeshDev& H, const MeshView& V, int k, float px, float py, int local,
+                          bool active, float& fx, float& fy) {
+    const float penalty = g.penalty;
+    fx = fy = 0.0f;
+    if (g.ny == 1) {
+        // _initialize_fem_rigid_mesh_ties_once_kernel at the first contact substep, then
+        // _apply_fem_rigid_mesh_ties_fixed_kernel every substep, whatever the gate
+        float4 tie = fw_ld(V.tie + k);
+        if (local == 0 && tie.x == 0.0f && H.tc > 0) {
+            const int4 r = m_cells(H, px, py, mmax(1e-4f, mmax(H.gs[0], H.gs[1])));
+            MeshBest b{1e9f, 0.0f, 0.0f, 0.0f, 0.0f, -1, -1, false, false};
+            for (int ix = r.x; ix <= r.z; ++ix)
+                for (int iy = r.y; iy <= r.w; ++iy) m_cell(H, V, ix + iy * H.cn[0], px, py, H.did, k, b);
+            if (b.found) {
+                fw_st(V.cache + k, make_int2(b.elem, b.did));
+                tie = make_float4(1.0f, b.w0, b.w1, 0.0f);
+                fw_st(V.tie + k, tie);
+            }
+        }
+        if (tie.x == 0.0f) return false;
+        const int e = fw_ld(V.cache + k).x;
+        const int2 cn = fw_ld(V.conn + e);
+        const float2 a = fw_ld(V.coords + cn.x), c = fw_ld(V.coords + cn.y);
+        // _eval_fixed_tie_point: (0, 0) + a w0 + c w1
+        const float cp0 = (0.0f + a.x * tie.y) + c.x * tie.z;
+        const float cp1 = (0.0f + a.y * tie.y) + c.y * tie.z;
+        fx = penalty * (cp0 - px);
+        fy = penalty * (cp1 - py);
+        return true;
+    }
+    if (local == H.reset_at) fw_st(V.skip + k, -1);  // flatworld/explicitloop.py:543-553
+    const int skip = fw_ld(V.skip + k);
+    // _build_active_rigid_mesh_workset_kernel (mixedcontact.py:519-588)
+    if (!active || skip == 1 || m_outside(px, py, g.lo[0] - 1e-4f, g.lo[1] - 1e-4f, g.hi[0] + 1e-4f, g.hi[1] + 1e-4f)) {
+        m_clear(V, k);
+        return false;
+    }
+    const int target = H.did;
+    MeshBest b{1e9f, 0.0f, 0.0f, 0.0f, 0.0f, -1, 0, false, false};
+    const int2 cached = fw_ld(V.cache + k);
+    if (cached.x >= 0 && cached.y == target) {
+        const float margin = mmax(1e-4f, fw_ld(V.margin + cached.x));
+        const float2 lb = fw_ld(V.elb + cached.x), ub = fw_ld(V.eub + cached.x);
+        if (!m_outside(px, py, lb.x - margin, lb.y - margin, ub.x + margin, ub.y + margin)) {
+            const int2 cn = fw_ld(V.conn + cached.x);
+            float nx, ny, w0, w1;
+            const float pen = m_detect(px, py, fw_ld(V.coords + cn.x), fw_ld(V.coords + cn.y), margin, nx, ny, w0, w1);
+            if (fabsf(pen) < fabsf(b.pen)) m_take(b, pen, nx, ny, w0, w1, cached.x, cached.y);
+        }
+    }
+    const bool run_full = !(H.subcycle > 1 && local % H.subcycle != 0);
+    if (!b.found && (skip == 1 || !run_full)) {
+        m_clear(V, k);
+        return false;
+    }
+    if (!b.found) {
+        if (H.tc > 0) {
+            const int4 r = m_cells(H, px, py, mmax(1e-4f, mmax(H.gs[0], H.gs[1])));
+            bool local_first = false;
+            int4 h = make_int4(0, 0, 0, 0);
+            if (cached.x >= 0 && cached.y == target) {
+                // _point_to_cell_coord of the cached edge's box centre (mixedcontact.py:322-344)
+                local_first = true;
+                const float2 lb = fw_ld(V.elb + cached.x), ub = fw_ld(V.eub + cached.x);
+                const float c0 = 0.5f * (lb.x + ub.x), c1 = 0.5f * (lb.y + ub.y);
+                int rx = (int)floorf((c0 - H.glb[0]) / H.gs[0]), ry = (int)floorf((c1 - H.glb[1]) / H.gs[1]);
+                rx = min(max(rx, 0), H.cn[0] - 1);
+                ry = min(max(ry, 0), H.cn[1] - 1);
+                h = make_int4(max(rx, r.x), max(ry, r.y), min(rx, r.z), min(ry, r.w));
+                for (int ix = h.x; ix <= h.z; ++ix)
+                    for (int iy = h.y; iy <= h.w; ++iy) m_cell(H, V, ix + iy * H.cn[0], px, py, target, k, b);
+            }
+            for (int ix = r.x; ix <= r.z; ++ix)
+                for (int iy = r.y; iy <= r.w; ++iy) {
+                    if (local_first && ix >= h.x && ix <= h.z && iy >= h.y && iy <= h.w) continue;
+                    m_cell(H, V, ix + iy * H.cn[0], px, py, target, k, b);
+                }
+        }
+        if (!b.has_cand) fw_st(V.skip + k, 1);
+    }
+    if (!(b.found && b.pen < 0.0f)) {
+        m_clear(V, k);
+        return false;
+    }
+    const float nf0 = ((-b.nx) * penalty) * b.pen, nf1 = ((-b.ny) * penalty) * b.pen;
+    float t0 = nf0, t1 = nf1;
+    const float mu = g.b[7];
+    if (mu > 1e-9f) {
+        const float Fy = mu * mlen(nf0, nf1);
+        const float kt = penalty * 1.0f;
+        float pf0 = 0.0f, pf1 = 0.0f, s0 = 0.0f, s1 = 0.0f;
+        const int2 fi = fw_ld(V.fric_i + k);
+        if (fi.y == 1) {
+            const float4 ff = fw_ld(V.fric_f + k);
+            const int2 cc = fw_ld(V.conn + b.elem), pc = fw_ld(V.conn + fi.x);
+            const float2 ca = fw_ld(V.coords + cc.x), cb = fw_ld(V.coords + cc.y);
+            const float2 pa = fw_ld(V.coords + pc.x), pb = fw_ld(V.coords + pc.y);
+            s0 = (ca.x * b.w0 + cb.x * b.w1) - (pa.x * ff.x + pb.x * ff.y);
+            s1 = (ca.y * b.w0 + cb.y * b.w1) - (pa.y * ff.x + pb.y * ff.y);
+            const float sd = mdot(s0, s1, b.nx, b.ny);
+            s0 = s0 - sd * b.nx;
+            s1 = s1 - sd * b.ny;
+            pf0 = ff.z;
+            pf1 = ff.w;
+            const float pd = mdot(pf0, pf1, b.nx, b.ny);
+            pf0 = pf0 - pd * b.nx;
+            pf1 = pf1 - pd * b.ny;
+        }
+        const float ft0 = pf0 + kt * s0, ft1 = pf1 + kt * s1;
+        const float fn = mlen(ft0, ft1);
+        float fr0 = ft0, fr1 = ft1;
+        if (!(fn <= Fy)) {
+            fr0 = (ft0 / fn) * Fy;
+            fr1 = (ft1 / fn) * Fy;
+        }
+        fw_st(V.fric_i + k, make_int2(b.elem, 1));
+        fw_st(V.fric_f + k, make_float4(b.w0, b.w1, fr0, fr1));
+        fw_st(V.fric_pen + k, b.pen);
+        t0 = t0 + fr0;
+        t1 = t1 + fr1;
+    } else {
+        m_clear(V, k);
+    }
+    fx = t0;
+    fy = t1;
+    return true;
+}
+
+// The pair gate of a record at `local` (the substep kernel's, flatworld/explicitloop.py:477-487,
+// bvh.py:39-46): the body's boundary AABB of the previous substep, padded with the
+// ground's on BVH rebuild steps, against the record's AABB.
+__device__ bool record_gate(const SceneDev* S, const GroundDev& g, int local) {
+    const int32_t* sub = S->hot.aabb;
+    const int kk = S->hot.aabb_k;
+    const int32_t* e = sub + mod3(local) * kk * FW_AABB_SUB_INTS;
+    int32_t r[4] = {fw_ld(e + 0), fw_ld(e + 1), fw_ld(e + 2), fw_ld(e + 3)};
+    for (int j = 1; j < kk; ++j) {
+        const int32_t* f = e + j * FW_AABB_SUB_INTS;
+        r[0] = min(r[0], fw_ld(f + 0));
+        r[1] = min(r[1], fw_ld(f + 1));
+        r[2] = max(r[2], fw_ld(f + 2));
+        r[3] = max(r[3], fw_ld(f + 3));
+    }
+    float lbx = fw_o2f(r[0]), lby = fw_o2f(r[1]), ubx = fw_o2f(r[2]), uby = fw_o2f(r[3]);
+    const bool pad = ((local - S->hot.gate_base) % 100) == 0;
+    if (pad) {
+        const float bx = 0.1f * (ubx - lbx), by = 0.1f * (uby - lby);
+        lbx = lbx - bx;
+        ubx = ubx + bx;
+        lby = lby - by;
+        uby = uby + by;
+    }
+    const float* lo = pad ? g.plo : g.lo;
+    const float* hi = pad ? g.phi : g.hi;
+    return lbx <= hi[0] && ubx >= lo[0] && lby <= hi[1] && uby >= lo[1];
+}
+
 __global__ __launch_bounds__(256) void rigid_pass_kernel(const SceneDev* scenes, const int2* rows, const ArenaCtl* ctl,
                                                          int k, uint32_t flags) {
     const int2 row = rows[blockIdx.x];
@@ -637,6 +958,35 @@ __global__ __launch_bounds__(256) void rigid_pass_kernel(const SceneDev* scenes,
     const float2* __restrict__ vel = S->vel;
     float4* __restrict__ out = static_cast<float4*>(const_cast<void*>(g.data));
     const int32_t* __restrict__ bn = reinterpret_cast<const int32_t*>(out + S->n_nodes);
+    if (g.kind == FW_GROUND_RIGID_MESH_DEV) {
+        char* blk = static_cast<char*>(const_cast<void*>(g.data));
+        const int64_t hoff = ((int64_t)20 * S->n_nodes + 15) / 16 * 16;
+        const MeshDev H = *reinterpret_cast<const MeshDev*>(blk + hoff);
+        MeshView V;
+        V.coords = reinterpret_cast<const float2*>(blk + H.o_coords);
+        V.conn = reinterpret_cast<const int2*>(blk + H.o_conn);
+        V.elb = reinterpret_cast<const float2*>(blk + H.o_elb);
+        V.eub = reinterpret_cast<const float2*>(blk + H.o_eub);
+        V.margin = reinterpret_cast<const float*>(blk + H.o_margin);
+        V.reg = reinterpret_cast<const int2*>(blk + H.o_reg);
+        V.cell = reinterpret_cast<const int2*>(blk + H.o_cell);
+        V.sorted = reinterpret_cast<const int32_t*>(blk + H.o_sorted);
+        V.cache = reinterpret_cast<int2*>(blk + H.o_cache);
+        V.skip = reinterpret_cast<int32_t*>(blk + H.o_skip);
+        V.fric_i = reinterpret_cast<int2*>(blk + H.o_fric_i);
+        V.fric_f = reinterpret_cast<float4*>(blk + H.o_fric_f);
+        V.fric_pen = reinterpret_cast<float*>(blk + H.o_fric_pen);
+        V.tie = reinterpret_cast<float4*>(blk + H.o_tie);
+        const bool active = record_gate(S, g, local);
+        for (int j = threadIdx.x; j < H.n_items; j += blockDim.x) {
+            const int n = fw_ld(bn + j);
+            const float2 p = fw_ld(pos + n);
+            float fx = 0.0f, fy = 0.0f;
+            const bool hit = mesh_item(g, H, V, j, p.x, p.y, local, active, fx, fy);
+            fw_st(out + n, make_float4(fx, fy, hit ? 1.0f : 0.0f, 0.0f));
+        }
+        return;
+    }
     for (int j = threadIdx.x; j < g.reverse; j += blockDim.x) {
         const int n = fw_ld(bn + j);
         const float2 p = fw_ld(pos + n), v = fw_ld(vel + n);

@@ -26,6 +26,36 @@
 #define FW_MAT_SPRING_DEV 4  // FW_MAT_SPRING of the public header
 #define FW_GROUND_FLEX_DEV 4 // FW_GROUND_FLEX: the scene's FEM-FEM contact contributions
 #define FW_GROUND_RIGID_DEV 5 // FW_GROUND_RIGID: a rigid primitive at the pose the host set
+#define FW_GROUND_RIGID_MESH_DEV 6 // FW_GROUND_RIGID of type FW_RIGID_MESH: never gated by the
+                                   // substep (the rigid pass gates its items itself)
+#define FW_RIGID_MESH_TYPE 16  // FW_RIGID_MESH
+
+// A mesh rigid record's header, at the start of its data block after the per-node
+// force array and the boundary-node ids: the loop's mesh rigids and the spatial
+// hash over their boundary edges (built on the host), and the offsets, from the
+// block start, of the arrays the rigid pass reads and writes.
+struct MeshDev {
+    float glb[2], gs[2];  // grid origin and cell size
+    int32_t cn[2], tc;    // cells per axis, cell count
+    int32_t did;          // the record's rigid id
+    int32_t subcycle;     // full hash queries on local steps % subcycle == 0
+    int32_t reset_at;     // local step before which the skip epochs reset (-1: none)
+    int32_t n_items;      // work items (the scene's boundary nodes)
+    int32_t pad;
+    int64_t o_coords;     // float2 [mesh nodes] boundary node positions
+    int64_t o_conn;       // int2 [edges] node indices
+    int64_t o_elb, o_eub; // float2 [edges] edge AABBs
+    int64_t o_margin;     // float [edges] margin bases
+    int64_t o_reg;        // int2 [registered] (rigid id, edge)
+    int64_t o_cell;       // int2 [tc] (start, end)
+    int64_t o_sorted;     // int32 [pairs] registry indices
+    int64_t o_cache;      // int2 [items] cached (edge, rigid id)
+    int64_t o_skip;       // int32 [items] skip epoch
+    int64_t o_fric_i;     // int2 [items] friction history: (edge, valid)
+    int64_t o_fric_f;     // float4 [items] (weight 0, weight 1, force x, force y)
+    int64_t o_fric_pen;   // float [items] penetration
+    int64_t o_tie;        // float4 [items] (resolved, weight 0, weight 1, 0)
+};
 
 struct GroundDev {
     int32_t kind;
@@ -320,6 +350,7 @@ __device__ inline void fw_st(float4* p, float4 a) { *(FW_GLOBAL fw_v4f*)p = fw_v
 __device__ inline void fw_st(float* p, float a) { *(FW_GLOBAL float*)p = a; }
 __device__ inline void fw_st(int32_t* p, int32_t a) { *(FW_GLOBAL int32_t*)p = a; }
 __device__ inline void fw_st(int4* p, int4 a) { *(FW_GLOBAL fw_v4i*)p = fw_v4i{a.x, a.y, a.z, a.w}; }
+__device__ inline void fw_st(int2* p, int2 a) { *(FW_GLOBAL fw_v2i*)p = fw_v2i{a.x, a.y}; }
 __device__ inline void fw_st(uint8_t* p, uint8_t a) { *(FW_GLOBAL uint8_t*)p = a; }
 
 // Tile and scene tables are never written while a kernel runs (the host edits

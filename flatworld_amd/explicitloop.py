@@ -69,7 +69,7 @@ class ExplicitLoop:
             if d.type not in _BODY_TYPES + _GROUND_TYPES:
                 raise NotImplementedError(f"domain type {d.type:#b} is not part of this package")
             if d.type == DomainType.RIGID and not hasattr(d, "record"):
-                raise NotImplementedError("rigid domains are kinematic flatworld_amd.rigid.RigidBodyDomain primitives here")
+                raise NotImplementedError("rigid domains are kinematic flatworld_amd.rigid.RigidBodyDomain bodies here")
         self.domains = domains
         self.d = domains[0].d
         self.numDomains = len(domains)
@@ -138,6 +138,15 @@ class ExplicitLoop:
                 "unbatched and apply no force; they lower the stable time and turn the BVH gate on"
             )
             self._unbatched_contacts += spring_fem
+        # rigid records as the reference launches their kernels: primitives, then the
+        # mesh ties, then the mesh contacts, each in pair order (mixedcontact.py:1711-1742)
+        def rig_order(cp):
+            c = cp[0]
+            if not getattr(c.domain2, "is_mesh", False):
+                return 0
+            return 1 if getattr(c, "tied", False) else 2
+        rig = sorted(rig, key=rig_order)
+        mesh_set = self._mesh_rigids(domains, rig, mass_scaling_dt)
         for c, _ in anal + rig + hf + vox:
             fem_idx = self.domains.index(c.domain1)
             grounds_of.setdefault(fem_idx, []).append((c.domain2, c))
@@ -167,7 +176,7 @@ class ExplicitLoop:
             flex = build_flex_group(fem_bodies, flex_contacts, gated, count_nodes)
 
         self.femSpringManager = FemSpringManager(self.d, domains, grounds_of=grounds_of, device=device,
-                                                 arena_kwargs=arena_kwargs, flex=flex)
+                                                 arena_kwargs=arena_kwargs, flex=flex, mesh_set=mesh_set)
         self.stableTime = min(self.femSpringManager.stableTime, self.stableTime)
         self.femSpringManager.setGlobalAABB(None)
         if mass_scaling_dt > 0.0:
@@ -179,12 +188,50 @@ class ExplicitLoop:
         self.numContacts = len(self.contacts)
         for c in self.contacts:
             self.stableTime = min(self.stableTime, c.stableTime)
+        if mesh_set is not None:
+            # initial rigid spatial hash over the elements near the FEM nodes in mesh
+            # contact (flatworld/explicitloop.py:386-396)
+            mesh_set.initial_build(*mesh_set.fem_contact_aabb(self._mesh_node_sets))
+        self.mesh_set = mesh_set
 
         if useAdapativeDT:
             self.dt = self.stableTime
         else:
             self.dt = dt
         self.default_dt = dt
+
+    # ------------------------------------------------------------ mesh rigids
+    def _mesh_rigids(self, domains, rig, mass_scaling_dt):
+        """The loop's RigidMeshSet, when a FEM body or spring network touches a mesh
+        rigid; its contacts learn the set, their rigid id and the full-query subcycle
+        max(2, min(8, rigid-contact stable time / domain stable time))
+        (flatworld/explicitloop.py:176-182, 308-327)."""
+        from .rigid import RigidMeshSet
+
+        mesh_contacts = [c for c, _ in rig if getattr(c.domain2, "is_mesh", False)]
+        if not mesh_contacts:
+            return None
+        rigid_domains = [d for d in domains if d.type == DomainType.RIGID]
+        meshes = [d for d in rigid_domains if getattr(d, "is_mesh", False) and d.considerContact]
+        mesh_set = RigidMeshSet(meshes, rigid_domains)
+        managed = [d for d in domains if d.type in _BODY_TYPES]
+        dom_min = min([1.0] + ([1.0 / 1000.0] if any(d.type in (DomainType.ANALYTICAL, DomainType.RIGID) for d in domains)
+                               else []) + [d.stableTime for d in managed])
+        if mass_scaling_dt > 0.0:
+            dom_min = max(dom_min, mass_scaling_dt)
+        rigid_min = min(c.stableTime for c, _ in rig)
+        subcycle = int(max(2, min(8, rigid_min / max(dom_min, 1e-12))))
+        self._mesh_node_sets = []
+        for c in mesh_contacts:
+            c.mesh_set = mesh_set
+            c.mesh_did = mesh_set.did(c.domain2)
+            c.mesh_subcycle = subcycle
+            body = c.domain1
+            if body.type == DomainType.SPRINGMASS:
+                self._mesh_node_sets.append((np.asarray(body.coords, np.float32), np.arange(body.nnodes)))
+            else:
+                self._mesh_node_sets.append((np.asarray(body.mesh.coords, np.float32), np.asarray(body.mesh.boundaryNodes)))
+        return mesh_set
 
     # ------------------------------------------------------------ ties
     def _parse_tie_entry(self, tie):

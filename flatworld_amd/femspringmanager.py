@@ -14,6 +14,7 @@ import numpy as np
 from . import _native
 from .definitions import DomainType
 from .flexcontact import REBUILD_INTERVAL
+from .rigid import KinematicRigids
 from .scene import GROUND_RIGID, add_flex_record, pack_domain, update_rigid_spec
 
 
@@ -56,7 +57,7 @@ class FemSpringManager:
     """
 
     def __init__(self, d, domains, spatial_hash=None, grounds_of=None, device=0, track_disp=True, arena_kwargs=None,
-                 flex=None):
+                 flex=None, mesh_set=None):
         if d != 2:
             raise ValueError(f"FemSpringManager supports d=2 only (got {d})")
         self.d = d
@@ -167,12 +168,34 @@ class FemSpringManager:
         for _, _, g in self._rigid_records:
             if all(g.rigid_domain is not r for r in self._rigids):
                 self._rigids.append(g.rigid_domain)
+        # the host schedule of the kinematic rigids, and the mesh set versions each
+        # mesh record last received
+        self._mesh_set = mesh_set
+        self._kin = KinematicRigids(self._rigids, mesh_set)
+        self._mesh_pushed = {}
 
-    def sync_rigids(self):
-        """Push every rigid domain's current pose to its contact records (stream-ordered)."""
+    def sync_rigids(self, reset_at=-1):
+        """Push every rigid domain's current pose to its contact records (stream-ordered);
+        a mesh record also takes what changed in its mesh set since the last push and
+        the step before which its skip epochs reset."""
         for sid, gi, g in self._rigid_records:
-            update_rigid_spec(g)
-            self._arena.set_rigid(sid, gi, g)
+            if g.mesh_set is None:
+                update_rigid_spec(g)
+                self._arena.set_rigid(sid, gi, g)
+                continue
+            m = g.mesh_set
+            r = g.rigid_domain
+            g.rigid_center = r.center.astype(np.float32)
+            g.rigid_vel = r.velocity.astype(np.float32)
+            g.rigid_omega = np.float32(r.omega)
+            g.aabb_lo, g.aabb_hi = m.aabb_of(r)
+            g.mesh_reset_at = int(reset_at)
+            last = self._mesh_pushed.get((sid, gi), (-1, -1, -1))
+            upd = ((_native.MESH_UPDATE_COORDS if last[0] != m.v_coords else 0)
+                   | (_native.MESH_UPDATE_EDGES if last[1] != m.v_edges else 0)
+                   | (_native.MESH_UPDATE_HASH if last[2] != m.v_hash else 0))
+            self._arena.set_rigid(sid, gi, g, mesh_update=upd)
+            self._mesh_pushed[(sid, gi)] = (m.v_coords, m.v_edges, m.v_hash)
 
     def _moving_rigids(self):
         return [r for r in self._rigids if r.omega != 0.0 or np.any(r.velocity != 0.0)]
@@ -197,18 +220,21 @@ class FemSpringManager:
         With FEM-FEM contact, ``dt64`` (one float64 per substep, default ``dt``)
         advances the hash schedule: a rebuild due at a substep is enqueued before
         it, as run_batched_contacts does (flatworld/mixedcontact.py:1672-1678).
-        Moving kinematic rigids are stepped one substep at a time: their pose is
-        pushed before the substep and advanced after it."""
-        moving = self._moving_rigids()
-        if not moving:
-            self._advance_block(n, dt, damping, diag, dt64)
-            return
+        Loops with moving kinematic rigids or mesh rigids are stepped one substep at
+        a time: poses (and the mesh set's positions, edge boxes, hash and skip reset)
+        are pushed before the substep, and the rigids advance after it
+        (``rigid.KinematicRigids``)."""
         seq = [float(dt)] * int(n) if dt64 is None else [float(v) for v in dt64]
+        if not self._moving_rigids() and self._mesh_set is None:
+            self._advance_block(n, dt, damping, diag, dt64)
+            self._kin.counter += int(n)
+            if seq:
+                self._kin.prev_dt = seq[-1]
+            return
         for k in range(int(n)):
-            self.sync_rigids()
+            self.sync_rigids(self._kin.before_substep())
             self._advance_block(1, dt, damping, diag and k == int(n) - 1, [seq[k]])
-            for r in moving:
-                r.advance(seq[k])
+            self._kin.after_substep(seq[k])
 
     def _advance_block(self, n, dt, damping, diag=True, dt64=None):
         flags = _native.FW_STEP_DIAG if diag else 0

@@ -78,6 +78,13 @@ class GroundSpec:
     rigid_friction: np.float32 = np.float32(0.0)
     rigid_tied: int = 0
     rigid_domain: object = None  # the RigidBodyDomain the record follows
+    # mesh rigid record: the loop's RigidMeshSet (positions, edges, spatial hash), the
+    # record's rigid id in it, substeps between full hash queries, and the loop step
+    # before which the skip epochs reset (-1: none pending)
+    mesh_set: object = None
+    mesh_did: int = 0
+    mesh_subcycle: int = 2
+    mesh_reset_at: int = -1
 
 
 @dataclass
@@ -213,7 +220,8 @@ def rigid_spec(rigid, contact):
     (flatworld/mixedcontact.py:1960-2043: penalty, the rigid's friction, tied flag)."""
     g = GroundSpec(GROUND_RIGID, np.float32(float(contact.penalty)), np.zeros(2, np.float32), np.zeros(2, np.float32),
                    rigid_friction=np.float32(float(rigid.friction)), rigid_tied=1 if getattr(contact, "tied", False) else 0,
-                   rigid_domain=rigid)
+                   rigid_domain=rigid, mesh_set=getattr(contact, "mesh_set", None),
+                   mesh_did=int(getattr(contact, "mesh_did", 0)), mesh_subcycle=int(getattr(contact, "mesh_subcycle", 2)))
     update_rigid_spec(g)
     return g
 

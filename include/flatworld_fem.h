@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 6
+#define FW_ABI_VERSION 7
 #define FW_MAX_GROUNDS 8
 
 /* status codes */
@@ -88,10 +88,21 @@ extern "C" {
  * impulses are not accumulated. */
 #define FW_GROUND_RIGID 5
 
-/* rigid primitive types (flatworld/definitions.py:50-54) */
+/* rigid types (flatworld/definitions.py:50-55) */
 #define FW_RIGID_BALL 1
 #define FW_RIGID_BOX 2
 #define FW_RIGID_CAPSULE 8
+/* A rigid triangle mesh touched through its boundary edges: the FEM side of
+ * _batched_rigid_mesh_kernel_sh, its workset filter and the mesh tie kernels
+ * (flatworld/mixedcontact.py:519-588, 1119-1565).  The record carries the loop's
+ * mesh rigids (boundary node positions at the pose, edges, edge AABBs and margin
+ * bases) and the RigidManager's spatial hash over their edges, all built on the
+ * host; per work item (the scene's boundary nodes) the device keeps the cached
+ * edge, the skip epoch, the friction history and the tie.  Its aabb is the mesh's
+ * AABB at the pose.  Non-tied items are gated by the pair gate inside the rigid
+ * pass; tied items are resolved once at local step 0 and pulled to their edge
+ * point every substep whatever the gate. */
+#define FW_RIGID_MESH 16
 
 /* node flag word: bits 0..8 BC bits (flatworld/definitions.py:95-104),
  * bit 15 boundary node, bits 16..31 row of the scene's BC value table */
@@ -157,6 +168,30 @@ typedef struct {
     float rigid_omega;
     float rigid_friction;
     int32_t rigid_tied;
+    /* FW_RIGID_MESH: boundary node positions [2 x mesh_nodes], edges as node index
+       pairs [2 x mesh_edges], edge AABBs [2 x mesh_edges each] and margin bases
+       [mesh_edges]; the hash: registry (rigid id, edge) [2 x mesh_nreg], per cell
+       (start, end) [2 x mesh_tc], entries [mesh_npairs], grid origin, cell size and
+       cells per axis; the capacities reserved at admission for the registry, the
+       cells and the entries; the record's rigid id, the substeps between full hash
+       queries and the local step before which the skip epochs reset (-1: none).
+       mesh_update (fw_scene_set_rigid) selects what is copied: 1 positions, 2 edge
+       AABBs and margins, 4 hash; the header always. */
+    const float* mesh_coords;
+    const int32_t* mesh_conn;
+    const float* mesh_elb;
+    const float* mesh_eub;
+    const float* mesh_margin;
+    const int32_t* mesh_reg;
+    const int32_t* mesh_cell;
+    const int32_t* mesh_sorted;
+    int32_t mesh_nodes, mesh_edges, mesh_nreg, mesh_tc, mesh_npairs;
+    int32_t mesh_cap_reg, mesh_cap_cells, mesh_cap_pairs;
+    float mesh_glb[2];
+    float mesh_gs[2];
+    int32_t mesh_cn[2];
+    int32_t mesh_did, mesh_subcycle, mesh_reset_at;
+    int32_t mesh_update;
 } fw_ground_desc;
 
 /* One FEM or spring-mass body; arrays in the caller's node/element order.

@@ -472,6 +472,7 @@ static float rdot(float ax, float ay, float bx, float by) { return (0.0f + ax * 
 static float rlen(float x, float y) { return sqrtf(rdot(x, y, x, y)); }
 static float rmaxf(float a, float b) { return a > b ? a : b; }
 static float rminf(float a, float b) { return a < b ? a : b; }
+static float rmaxf_(float a, float b) { return a > b ? a : b; }
 
 /* sphereSDFQuery (flatworld/utils.py:34-40) */
 static float sphere_sdf(const float p[2], const float c[2], float radius, float n[2]) {
@@ -642,6 +643,296 @@ static int32_t rigid_force(const ora_ground* g, const float p[2], const float ve
     return 1;
 }
 
+/* ---------------------------------------------------------------- rigid meshes
+ * _batched_rigid_mesh_kernel_sh with its workset filter, and the mesh tie kernels
+ * (flatworld/mixedcontact.py:519-588, 1119-1401, 1402-1565), serially in work-item
+ * order, for the FEM node's side (the reaction impulses on the rigid are not
+ * formed).  The loop's mesh rigids and their spatial hash come from the host. */
+
+static int32_t rm_outside(const float p[2], float lb0, float lb1, float ub0, float ub1) {
+    /* _point_outside_aabb (mixedcontact.py:113-119) */
+    return rminf(p[0] - lb0, p[1] - lb1) < 0.0f || rminf(ub0 - p[0], ub1 - p[1]) < 0.0f;
+}
+
+static float rm_dist_sq(const float p[2], float lb0, float lb1, float ub0, float ub1) {
+    /* _point_aabb_distance_sq (mixedcontact.py:122-132) */
+    float d2 = 0.0f, v = 0.0f;
+    if (p[0] < lb0) v = lb0 - p[0];
+    else if (p[0] > ub0) v = p[0] - ub0;
+    d2 = d2 + v * v;
+    v = 0.0f;
+    if (p[1] < lb1) v = lb1 - p[1];
+    else if (p[1] > ub1) v = p[1] - ub1;
+    d2 = d2 + v * v;
+    return d2;
+}
+
+typedef struct {
+    float pen, nrm[2], w[2];
+    int32_t elem, did, found, has_cand;
+} rm_best;
+
+/* detectPointToMeshBoundary on edge e (contact_detection.py:64-90). */
+static float rm_detect(const ora_ground* g, const float p[2], int32_t e, float limit, float nrm[2], float w[2]) {
+    const float* a = g->mesh_coords + 2 * g->mesh_conn[2 * e];
+    const float* b = g->mesh_coords + 2 * g->mesh_conn[2 * e + 1];
+    float pen, n[2];
+    const int32_t inside = ora_point_edge(p, a, b, &pen, n, w);
+    if (fabsf(pen) < limit && inside) {
+        nrm[0] = n[0];
+        nrm[1] = n[1];
+        return pen;
+    }
+    nrm[0] = nrm[1] = 0.0f;
+    return 1e9f;
+}
+
+/* _process_rigid_sh_cell (mixedcontact.py:382-467): every edge of rigid `target` in
+ * cell cid, each improvement written to the item's cache. */
+static void rm_cell(const ora_ground* g, int32_t cid, const float p[2], int32_t target, int32_t item, rm_best* b) {
+    if (!(cid >= 0 && cid < g->mesh_tc)) return;
+    const float best_abs = fabsf(b->pen);
+    if (b->found == 1 && best_abs < 1e8f) {
+        const int32_t nx = g->mesh_cn[0], ix = cid % nx, iy = cid / nx;
+        const float l0 = g->mesh_glb[0] + (float)ix * g->mesh_gs[0], l1 = g->mesh_glb[1] + (float)iy * g->mesh_gs[1];
+        const float u0 = l0 + g->mesh_gs[0], u1 = l1 + g->mesh_gs[1];
+        if (rm_dist_sq(p, l0, l1, u0, u1) > best_abs * best_abs) return;
+    }
+    for (int32_t q = g->mesh_cell[2 * cid]; q < g->mesh_cell[2 * cid + 1]; ++q) {
+        const int32_t ei = g->mesh_sorted[q];
+        if (ei < 0) continue;
+        const int32_t did = g->mesh_reg[2 * ei], e = g->mesh_reg[2 * ei + 1];
+        if (did != target) continue;
+        b->has_cand = 1;
+        float l0 = 1e30f, l1 = 1e30f, u0 = -1e30f, u1 = -1e30f;
+        for (int vj = 0; vj < 2; ++vj) {
+            const float* c = g->mesh_coords + 2 * g->mesh_conn[2 * e + vj];
+            l0 = rminf(l0, c[0]);
+            l1 = rminf(l1, c[1]);
+            u0 = c[0] > u0 ? c[0] : u0;
+            u1 = c[1] > u1 ? c[1] : u1;
+        }
+        const float margin = rmaxf_(1e-4f, g->mesh_margin[e]);
+        l0 = l0 - margin;
+        l1 = l1 - margin;
+        u0 = u0 + margin;
+        u1 = u1 + margin;
+        const float best_abs2 = fabsf(b->pen);
+        if (b->found == 1 && best_abs2 < 1e8f && rm_dist_sq(p, l0, l1, u0, u1) > best_abs2 * best_abs2) continue;
+        if (!rm_outside(p, l0, l1, u0, u1)) {
+            float nrm[2], w[2];
+            const float pen = rm_detect(g, p, e, margin, nrm, w);
+            if (fabsf(pen) < fabsf(b->pen)) {
+                b->pen = pen;
+                b->nrm[0] = nrm[0];
+                b->nrm[1] = nrm[1];
+                b->w[0] = w[0];
+                b->w[1] = w[1];
+                b->elem = e;
+                b->did = did;
+                b->found = 1;
+                g->st_cache[2 * item] = e;
+                g->st_cache[2 * item + 1] = did;
+            }
+        }
+    }
+}
+
+/* Query cells of p with buffer qb, clamped (mixedcontact.py:1226-1243). */
+static void rm_cells(const ora_ground* g, const float p[2], float qb, int32_t r[4]) {
+    const float d0 = (p[0] - qb) - g->mesh_glb[0], d1 = (p[1] - qb) - g->mesh_glb[1];
+    const float e0 = (p[0] + qb) - g->mesh_glb[0], e1 = (p[1] + qb) - g->mesh_glb[1];
+    int32_t lx = (int32_t)floorf(d0 / g->mesh_gs[0]), ly = (int32_t)floorf(d1 / g->mesh_gs[1]);
+    int32_t ux = (int32_t)floorf(e0 / g->mesh_gs[0]), uy = (int32_t)floorf(e1 / g->mesh_gs[1]);
+    const int32_t cx = g->mesh_cn[0] - 1, cy = g->mesh_cn[1] - 1;
+    lx = lx > 0 ? lx : 0;
+    ly = ly > 0 ? ly : 0;
+    lx = lx < cx ? lx : cx;
+    ly = ly < cy ? ly : cy;
+    ux = ux > 0 ? ux : 0;
+    uy = uy > 0 ? uy : 0;
+    ux = ux < cx ? ux : cx;
+    uy = uy < cy ? uy : cy;
+    r[0] = lx;
+    r[1] = ly;
+    r[2] = ux;
+    r[3] = uy;
+}
+
+static void rm_clear(const ora_ground* g, int32_t k) {
+    /* _clear_rigid_mesh_friction_state (mixedcontact.py:347-361) */
+    g->st_pelem[k] = -1;
+    g->st_pvalid[k] = 0;
+    g->st_pw[2 * k] = 0.0f;
+    g->st_pw[2 * k + 1] = 0.0f;
+    g->st_pf[2 * k] = 0.0f;
+    g->st_pf[2 * k + 1] = 0.0f;
+    g->st_ppen[k] = 0.0f;
+}
+
+static void rm_surface(const ora_ground* g, int32_t e, const float w[2], float out[2]) {
+    /* mesh_boundary_coords[c0] * w0 + mesh_boundary_coords[c1] * w1 */
+    const float* a = g->mesh_coords + 2 * g->mesh_conn[2 * e];
+    const float* b = g->mesh_coords + 2 * g->mesh_conn[2 * e + 1];
+    out[0] = a[0] * w[0] + b[0] * w[1];
+    out[1] = a[1] * w[0] + b[1] * w[1];
+}
+
+/* Force of the mesh record on work item k (node n at p); returns 1 when it applies one. */
+static int32_t rigid_mesh_item(const ora_ground* g, int32_t k, const float p[2], int32_t counter, int32_t active,
+                               float force[2]) {
+    const float penalty = g->penalty;
+    force[0] = force[1] = 0.0f;
+    if (g->rigid_tied == 1) {
+        /* _initialize_fem_rigid_mesh_ties_once_kernel at the first contact substep, then
+           _apply_fem_rigid_mesh_ties_fixed_kernel every substep, whatever the gate */
+        if (counter == 0 && g->st_tres[k] == 0 && g->mesh_tc > 0) {
+            const float qb = rmaxf_(1e-4f, rmaxf_(g->mesh_gs[0], g->mesh_gs[1]));
+            int32_t r[4];
+            rm_cells(g, p, qb, r);
+            rm_best b = {1e9f, {0.0f, 0.0f}, {0.0f, 0.0f}, -1, -1, 0, 0};
+            for (int32_t ix = r[0]; ix <= r[2]; ++ix)
+                for (int32_t iy = r[1]; iy <= r[3]; ++iy) rm_cell(g, ix + iy * g->mesh_cn[0], p, g->mesh_did, k, &b);
+            if (b.found == 1) {
+                g->st_tres[k] = 1;
+                g->st_cache[2 * k] = b.elem;
+                g->st_cache[2 * k + 1] = b.did;
+                g->st_tw[2 * k] = b.w[0];
+                g->st_tw[2 * k + 1] = b.w[1];
+            }
+        }
+        if (g->st_tres[k] == 0) return 0;
+        const int32_t e = g->st_cache[2 * k];
+        const float* a = g->mesh_coords + 2 * g->mesh_conn[2 * e];
+        const float* c = g->mesh_coords + 2 * g->mesh_conn[2 * e + 1];
+        /* _eval_fixed_tie_point: (0, 0) + a * w0 + c * w1 */
+        const float cp0 = (0.0f + a[0] * g->st_tw[2 * k]) + c[0] * g->st_tw[2 * k + 1];
+        const float cp1 = (0.0f + a[1] * g->st_tw[2 * k]) + c[1] * g->st_tw[2 * k + 1];
+        force[0] = penalty * (cp0 - p[0]);
+        force[1] = penalty * (cp1 - p[1]);
+        return 1;
+    }
+    if (counter == g->mesh_reset_at) g->st_skip[k] = -1;  /* explicitloop.py:543-553 */
+    /* _build_active_rigid_mesh_workset_kernel (mixedcontact.py:519-588) */
+    int32_t keep = active != 0 && g->st_skip[k] != 1 &&
+                   !rm_outside(p, g->aabb_lo[0] - 1e-4f, g->aabb_lo[1] - 1e-4f, g->aabb_hi[0] + 1e-4f,
+                               g->aabb_hi[1] + 1e-4f);
+    if (!keep) {
+        rm_clear(g, k);
+        return 0;
+    }
+    const int32_t target = g->mesh_did;
+    const float qbs = rmaxf_(1e-4f, rmaxf_(g->mesh_gs[0], g->mesh_gs[1]));
+    rm_best b = {1e9f, {0.0f, 0.0f}, {0.0f, 0.0f}, -1, 0, 0, 0};
+    const int32_t cel = g->st_cache[2 * k], cdid = g->st_cache[2 * k + 1];
+    if (cel >= 0 && cdid == target) {
+        const float margin = rmaxf_(1e-4f, g->mesh_margin[cel]);
+        const float l0 = g->mesh_elb[2 * cel] - margin, l1 = g->mesh_elb[2 * cel + 1] - margin;
+        const float u0 = g->mesh_eub[2 * cel] + margin, u1 = g->mesh_eub[2 * cel + 1] + margin;
+        if (!rm_outside(p, l0, l1, u0, u1)) {
+            float nrm[2], w[2];
+            const float pen = rm_detect(g, p, cel, margin, nrm, w);
+            if (fabsf(pen) < fabsf(b.pen)) {
+                b.pen = pen;
+                b.nrm[0] = nrm[0];
+                b.nrm[1] = nrm[1];
+                b.w[0] = w[0];
+                b.w[1] = w[1];
+                b.elem = cel;
+                b.did = cdid;
+                b.found = 1;
+            }
+        }
+    }
+    const int32_t run_full = !(g->mesh_subcycle > 1 && counter % g->mesh_subcycle != 0);
+    if (b.found == 0 && (g->st_skip[k] == 1 || run_full == 0)) {
+        rm_clear(g, k);
+        return 0;
+    }
+    if (b.found == 0) {
+        if (g->mesh_tc > 0) {
+            int32_t r[4];
+            rm_cells(g, p, qbs, r);
+            int32_t local_first = 0, h[4] = {0, 0, 0, 0};
+            if (cel >= 0 && cdid == target) {
+                local_first = 1;
+                /* _point_to_cell_coord of the cached edge's box centre (mixedcontact.py:322-344) */
+                const float c0 = 0.5f * (g->mesh_elb[2 * cel] + g->mesh_eub[2 * cel]);
+                const float c1 = 0.5f * (g->mesh_elb[2 * cel + 1] + g->mesh_eub[2 * cel + 1]);
+                int32_t rx = (int32_t)floorf((c0 - g->mesh_glb[0]) / g->mesh_gs[0]);
+                int32_t ry = (int32_t)floorf((c1 - g->mesh_glb[1]) / g->mesh_gs[1]);
+                if (rx < 0) rx = 0;
+                if (ry < 0) ry = 0;
+                if (rx > g->mesh_cn[0] - 1) rx = g->mesh_cn[0] - 1;
+                if (ry > g->mesh_cn[1] - 1) ry = g->mesh_cn[1] - 1;
+                h[0] = rx > r[0] ? rx : r[0];
+                h[1] = ry > r[1] ? ry : r[1];
+                h[2] = rx < r[2] ? rx : r[2];
+                h[3] = ry < r[3] ? ry : r[3];
+                for (int32_t ix = h[0]; ix <= h[2]; ++ix)
+                    for (int32_t iy = h[1]; iy <= h[3]; ++iy) rm_cell(g, ix + iy * g->mesh_cn[0], p, target, k, &b);
+            }
+            for (int32_t ix = r[0]; ix <= r[2]; ++ix)
+                for (int32_t iy = r[1]; iy <= r[3]; ++iy) {
+                    if (local_first && ix >= h[0] && ix <= h[2] && iy >= h[1] && iy <= h[3]) continue;
+                    rm_cell(g, ix + iy * g->mesh_cn[0], p, target, k, &b);
+                }
+        }
+        if (b.has_cand == 0) g->st_skip[k] = 1;
+    }
+    if (!(b.found == 1 && b.pen < 0.0f)) {
+        rm_clear(g, k);
+        return 0;
+    }
+    const float nf0 = ((-b.nrm[0]) * penalty) * b.pen, nf1 = ((-b.nrm[1]) * penalty) * b.pen;
+    float t0 = nf0, t1 = nf1;
+    const float mu = g->rigid_friction;
+    if (mu > 1e-9f) {
+        const float Fy = mu * rlen(nf0, nf1);
+        const float kt = penalty * 1.0f;
+        float pf0 = 0.0f, pf1 = 0.0f, s0 = 0.0f, s1 = 0.0f;
+        if (g->st_pvalid[k] == 1) {
+            float cur[2], prev[2];
+            rm_surface(g, b.elem, b.w, cur);
+            rm_surface(g, g->st_pelem[k], g->st_pw + 2 * k, prev);
+            s0 = cur[0] - prev[0];
+            s1 = cur[1] - prev[1];
+            const float sd = rdot(s0, s1, b.nrm[0], b.nrm[1]);
+            s0 = s0 - sd * b.nrm[0];
+            s1 = s1 - sd * b.nrm[1];
+            pf0 = g->st_pf[2 * k];
+            pf1 = g->st_pf[2 * k + 1];
+            const float pd = rdot(pf0, pf1, b.nrm[0], b.nrm[1]);
+            pf0 = pf0 - pd * b.nrm[0];
+            pf1 = pf1 - pd * b.nrm[1];
+        }
+        const float ft0 = pf0 + kt * s0, ft1 = pf1 + kt * s1;
+        const float fn = rlen(ft0, ft1);
+        float fr0, fr1;
+        if (fn <= Fy) {
+            fr0 = ft0;
+            fr1 = ft1;
+        } else {
+            fr0 = (ft0 / fn) * Fy;
+            fr1 = (ft1 / fn) * Fy;
+        }
+        g->st_pelem[k] = b.elem;
+        g->st_pvalid[k] = 1;
+        g->st_pw[2 * k] = b.w[0];
+        g->st_pw[2 * k + 1] = b.w[1];
+        g->st_pf[2 * k] = fr0;
+        g->st_pf[2 * k + 1] = fr1;
+        g->st_ppen[k] = b.pen;
+        t0 = t0 + fr0;
+        t1 = t1 + fr1;
+    } else {
+        rm_clear(g, k);
+    }
+    force[0] = t0;
+    force[1] = t1;
+    return 1;
+}
+
 /* Pair activation: inclusive AABB overlap of the FEM boundary AABB and the
  * ground AABB, both widened by 0.1 * extent on full BVH rebuilds
  * (counter % 100 == 0) and raw on refits.
@@ -726,6 +1017,21 @@ void ora_substep(ora_scene* s, float dt, float damping, int32_t counter, int32_t
                     for (int32_t j = g->flex_off[n]; j < g->flex_off[n + 1]; ++j) {
                         fx[2 * n] += g->flex_f[2 * j];
                         fx[2 * n + 1] += g->flex_f[2 * j + 1];
+                    }
+                }
+                continue;
+            }
+            if (g->kind == ORA_RIGID && g->rigid_type == ORA_RIGID_MESH) {
+                /* the mesh kernels gate their non-tied items themselves and apply ties
+                   whatever the gate (mixedcontact.py:1716-1742) */
+                const int32_t act = ora_gate(s->aabb, s->aabb + 2, g, counter);
+                for (int32_t k = 0; k < s->n_bnodes; ++k) {
+                    int32_t n = s->bnodes[k];
+                    float f[2];
+                    if (rigid_mesh_item(g, k, s->x + 2 * n, counter, act, f)) {
+                        fx[2 * n] += f[0];
+                        fx[2 * n + 1] += f[1];
+                        if (s->contact_flag) s->contact_flag[n] = 1;
                     }
                 }
                 continue;

@@ -31,8 +31,8 @@ enum { ORA_ELASTIC = 1, ORA_NEOHOOKEAN = 2, ORA_MISES = 3, ORA_SPRING = 4 };
  * FEM-FEM kernel runs: after the plane kernel, before the height-field and voxel
  * kernels (flatworld/mixedcontact.py:1654-1754).  It is never gated. */
 enum { ORA_PLANE = 1, ORA_HEIGHTFIELD = 2, ORA_VOXEL = 3, ORA_FLEX = 4, ORA_RIGID = 5 };
-/* rigid primitive types (flatworld/definitions.py:50-55) */
-enum { ORA_RIGID_BALL = 1, ORA_RIGID_BOX = 2, ORA_RIGID_CAPSULE = 8 };
+/* rigid types (flatworld/definitions.py:50-55) */
+enum { ORA_RIGID_BALL = 1, ORA_RIGID_BOX = 2, ORA_RIGID_CAPSULE = 8, ORA_RIGID_MESH = 16 };
 
 typedef struct {
     int32_t kind;
@@ -71,6 +71,41 @@ typedef struct {
     float rigid_omega;
     float rigid_friction;
     int32_t rigid_tied;
+    /* ORA_RIGID of type ORA_RIGID_MESH: the loop's mesh rigids as the RigidManager
+       keeps them and its spatial hash over their boundary edges, built on the host:
+       boundary node positions [2 x nodes]; edges as two node indices [2 x elems];
+       each edge's AABB and margin base [2 x elems each, elems]; the hash registry
+       (rigid id, edge) [2 x registered], per cell (start, end) [2 x tc], the cell
+       entries (registry indices) and the grid (origin, cell size, cells per axis,
+       cell count).  mesh_did is the record's rigid; full queries run on substeps
+       counter % mesh_subcycle == 0 and the skip epochs reset before the substep
+       counter == mesh_reset_at (the host's schedule).  Per work item (the body's boundary nodes in order) the state
+       the reference keeps in arrays: cached (edge, rigid), skip epoch, friction
+       history (edge, valid, weights, force, penetration), tie (resolved, weights). */
+    const float* mesh_coords;
+    const int32_t* mesh_conn;
+    const float* mesh_elb;
+    const float* mesh_eub;
+    const float* mesh_margin;
+    const int32_t* mesh_reg;
+    const int32_t* mesh_cell;
+    const int32_t* mesh_sorted;
+    float mesh_glb[2];
+    float mesh_gs[2];
+    int32_t mesh_cn[2];
+    int32_t mesh_tc;
+    int32_t mesh_did;
+    int32_t mesh_subcycle;
+    int32_t mesh_reset_at;
+    int32_t* st_cache;   /* [2 x items] (edge, rigid), -1 when empty */
+    int32_t* st_skip;    /* [items] skip epoch, -1 or 1 */
+    int32_t* st_pelem;   /* [items] friction: previous edge (-1) */
+    int32_t* st_pvalid;  /* [items] */
+    float* st_pw;        /* [2 x items] */
+    float* st_pf;        /* [2 x items] */
+    float* st_ppen;      /* [items] */
+    int32_t* st_tres;    /* [items] tie resolved */
+    float* st_tw;        /* [2 x items] tie weights */
 } ora_ground;
 
 typedef struct {

@@ -61,6 +61,30 @@ class OraGround(ctypes.Structure):
         ("rigid_omega", ctypes.c_float),
         ("rigid_friction", ctypes.c_float),
         ("rigid_tied", ctypes.c_int32),
+        ("mesh_coords", _f32p),
+        ("mesh_conn", _i32p),
+        ("mesh_elb", _f32p),
+        ("mesh_eub", _f32p),
+        ("mesh_margin", _f32p),
+        ("mesh_reg", _i32p),
+        ("mesh_cell", _i32p),
+        ("mesh_sorted", _i32p),
+        ("mesh_glb", ctypes.c_float * 2),
+        ("mesh_gs", ctypes.c_float * 2),
+        ("mesh_cn", ctypes.c_int32 * 2),
+        ("mesh_tc", ctypes.c_int32),
+        ("mesh_did", ctypes.c_int32),
+        ("mesh_subcycle", ctypes.c_int32),
+        ("mesh_reset_at", ctypes.c_int32),
+        ("st_cache", _i32p),
+        ("st_skip", _i32p),
+        ("st_pelem", _i32p),
+        ("st_pvalid", _i32p),
+        ("st_pw", _f32p),
+        ("st_pf", _f32p),
+        ("st_ppen", _f32p),
+        ("st_tres", _i32p),
+        ("st_tw", _f32p),
     ]
 
 
@@ -221,6 +245,52 @@ def make_ground(gs, keep):
     return g
 
 
+RIGID_MESH = 16
+
+
+class MeshItemState:
+    """Per-work-item state of one mesh rigid record (the arrays the reference's mesh
+    kernels keep per item): initial values as _build_batched_contacts sets them
+    (flatworld/mixedcontact.py:2052-2093)."""
+
+    def __init__(self, n):
+        self.cache = np.full((n, 2), -1, np.int32)
+        self.skip = np.full(n, -1, np.int32)
+        self.pelem = np.full(n, -1, np.int32)
+        self.pvalid = np.zeros(n, np.int32)
+        self.pw = np.zeros((n, 2), np.float32)
+        self.pf = np.zeros((n, 2), np.float32)
+        self.ppen = np.zeros(n, np.float32)
+        self.tres = np.zeros(n, np.int32)
+        self.tw = np.zeros((n, 2), np.float32)
+
+    def attach(self, g):
+        g.st_cache, g.st_skip, g.st_pelem = _p(self.cache, _i32p), _p(self.skip, _i32p), _p(self.pelem, _i32p)
+        g.st_pvalid, g.st_tres = _p(self.pvalid, _i32p), _p(self.tres, _i32p)
+        g.st_pw, g.st_pf, g.st_ppen, g.st_tw = (_p(a, _f32p) for a in (self.pw, self.pf, self.ppen, self.tw))
+
+
+def set_mesh(g, gs, keep):
+    """Point an OraGround at the current arrays of a mesh record's RigidMeshSet."""
+    m = gs.mesh_set
+    arrs = dict(coords=np.ascontiguousarray(m.coords, np.float32), conn=np.ascontiguousarray(m.conn, np.int32),
+                elb=np.ascontiguousarray(m.elb, np.float32), eub=np.ascontiguousarray(m.eub, np.float32),
+                margin=np.ascontiguousarray(m.margin, np.float32),
+                reg=np.ascontiguousarray(m.reg if m.reg.size else np.zeros((1, 2), np.int32), np.int32),
+                cell=np.ascontiguousarray(m.cell if m.cell.size else np.zeros((1, 2), np.int32), np.int32),
+                sorted=np.ascontiguousarray(m.sorted if m.sorted.size else np.zeros(1, np.int32), np.int32))
+    keep.append(arrs)
+    g.mesh_coords, g.mesh_elb, g.mesh_eub, g.mesh_margin = (_p(arrs[k], _f32p) for k in ("coords", "elb", "eub", "margin"))
+    g.mesh_conn, g.mesh_reg, g.mesh_cell, g.mesh_sorted = (_p(arrs[k], _i32p) for k in ("conn", "reg", "cell", "sorted"))
+    g.mesh_glb[:] = [float(v) for v in m.glb]
+    g.mesh_gs[:] = [float(v) for v in m.gs]
+    g.mesh_cn[:] = [int(v) for v in m.cn]
+    g.mesh_tc = int(m.tc)
+    g.mesh_did = int(gs.mesh_did)
+    g.mesh_subcycle = int(gs.mesh_subcycle)
+    g.mesh_reset_at = int(gs.mesh_reset_at)
+
+
 def set_rigid(g, gs):
     """Copy a rigid record's pose (flatworld_amd.scene.GroundSpec of kind 5) into an OraGround."""
     g.aabb_lo[:] = [float(v) for v in gs.aabb_lo]
@@ -265,6 +335,14 @@ class OracleScene:
         self._keep = []
         gs = [make_ground(g, self._keep) for g in spec.grounds]
         self._grounds = (OraGround * max(len(gs), 1))(*gs) if gs else (OraGround * 1)()
+        # mesh rigid records: the mesh set's arrays and per-item state
+        self.mesh_state = {}
+        for gi, g in enumerate(spec.grounds):
+            if g.kind == 5 and int(g.rigid_type) == RIGID_MESH:
+                st = MeshItemState(self.bnodes.shape[0])
+                st.attach(self._grounds[gi])
+                set_mesh(self._grounds[gi], g, self._keep)
+                self.mesh_state[gi] = st
         s = OraScene()
         s.n_nodes, s.n_elems, s.n_bnodes, s.n_grounds = N, E, self.bnodes.shape[0], len(gs)
         s.x, s.v, s.u, s.fext, s.mass = (_p(a, _f32p) for a in (self.x, self.v, self.u, self.fext, self.mass))
@@ -289,9 +367,15 @@ class OracleScene:
         return np.array(self._s.aabb[:], dtype=np.float32)
 
     def set_rigid(self, ground_index, gs):
-        """Move rigid record ``ground_index`` to the pose of GroundSpec ``gs``."""
+        """Move rigid record ``ground_index`` to the pose of GroundSpec ``gs`` (a mesh
+        record also takes its mesh set's current positions and hash)."""
         assert self._grounds[ground_index].kind == 5
         set_rigid(self._grounds[ground_index], gs)
+        if int(gs.rigid_type) == RIGID_MESH:
+            self._keep = [k for k in self._keep if not isinstance(k, dict)]
+            for gi, g in enumerate(self.spec.grounds):
+                if g.kind == 5 and int(g.rigid_type) == RIGID_MESH:
+                    set_mesh(self._grounds[gi], g, self._keep)
 
     def warmup(self):
         """Zero-dt femStep run by setGlobalAABB (flatworld/femspringmanager.py:872-879)."""

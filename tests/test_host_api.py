@@ -176,7 +176,7 @@ def test_library_exports_declared_symbols():
     for name in sorted(declared):
         assert hasattr(L, name), name
         assert name in _native.SIGNATURES, f"{name} not bound in _native.SIGNATURES"
-    assert L.fw_abi_version() == 6
+    assert L.fw_abi_version() == 7
 
 
 @pytest.mark.parametrize("tile_nodes,tile_slots", [(512, 1024), (64, 1024), (1024, 2048), (7, 1024)])
