@@ -86,7 +86,7 @@ def primitive_aabb(rtype, center, rot, prim, radius):
         lo = np.array([f32(f32(min(p00, p10) - perp0) - ax0), f32(f32(min(p01, p11) - perp1) - ax1)], np.float32)
         hi = np.array([f32(f32(max(p00, p10) + perp0) + ax0), f32(f32(max(p01, p11) + perp1) + ax1)], np.float32)
         return lo, hi
-    raise NotImplementedError(f"rigid type {rtype:#b}: only ball, box and capsule primitives are supported")
+    raise NotImplementedError(f"rigid type {rtype:#b} has no primitive AABB (ball, box and capsule do; a mesh rigid's comes from its placed nodes)")
 
 
 def _angle(angle, d):
