@@ -3,7 +3,7 @@
 // flatworld/femspringmanager.py:421-487, 1035-1073) and the batched contact pass
 // (flatworld/mixedcontact.py:886-1025).  The hash build and the contribution
 // lists take one workgroup per contact group; the work-item pass spreads the work
-// items of every group over the GPU, four lanes per item.
+// items of every group over the GPU, two lanes per item.
 //
 // The reference runs both with atomics (the hash scatter, the Fext updates) and
 // with one query buffer shared by every thread.  Here every result that order
@@ -395,7 +395,7 @@ __device__ __forceinline__ bool seen_before(const FlexDev& F, const FlexGrid& g,
 }
 
 constexpr int WALK_BATCH = 8;  // cells whose ranges are loaded together in a query walk
-constexpr int FLEX_LPI = 4;    // lanes per work item in the contact pass
+constexpr int FLEX_LPI = 2;    // lanes per work item in the contact pass
 
 // The best candidate so far: the most negative penetration, the earliest walk
 // position on ties (key = walk cell << 32 | entry).
