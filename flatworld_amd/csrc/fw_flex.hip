@@ -47,6 +47,22 @@ __device__ __forceinline__ int mod3_(int k) {
 
 __device__ __forceinline__ void phase_barrier() { __syncthreads(); }
 
+// A load through a pointer read from a table, made a global-memory load: left to
+// itself the compiler cannot tell such a pointer from an LDS one and emits a flat
+// load (64-bit address arithmetic, and a wait on the LDS counter as well).
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+    return fw_ld(p);
+}
+template <typename T>
+__device__ __forceinline__ T* gld(T* const* p) {
+    return reinterpret_cast<T*>(*(const FW_GLOBAL uint64_t*)p);
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    fw_st(p, v);
+}
+
 // The per-substep scratch of a group's contact pass: the group's own arrays (scr
 // null), or its part of the compact layout at scr (fw_step_noalloc).
 struct FlexScratch {
@@ -131,13 +147,13 @@ __device__ __forceinline__ CellRange cell_range(const FlexGrid& g, float lbx, fl
 __device__ __forceinline__ void scene_aabb(const FlexBody& B, int ar3, float bb[4]) {
     const int kk = B.aabb_k;
     const int32_t* e = B.aabb + ar3 * kk * FW_AABB_SUB_INTS;
-    int32_t r[4] = {e[0], e[1], e[2], e[3]};
+    int32_t r[4] = {gld(e + 0), gld(e + 1), gld(e + 2), gld(e + 3)};
     for (int j = 1; j < kk; ++j) {
         const int32_t* f = e + j * FW_AABB_SUB_INTS;
-        r[0] = imin_(r[0], f[0]);
-        r[1] = imin_(r[1], f[1]);
-        r[2] = imax_(r[2], f[2]);
-        r[3] = imax_(r[3], f[3]);
+        r[0] = imin_(r[0], gld(f + 0));
+        r[1] = imin_(r[1], gld(f + 1));
+        r[2] = imax_(r[2], gld(f + 2));
+        r[3] = imax_(r[3], gld(f + 3));
     }
     for (int k = 0; k < 4; ++k) bb[k] = fw_o2f(r[k]);
 }
@@ -433,7 +449,7 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
         for (int u = 0; u < WALK_BATCH; ++u) {
             const int t = (i0 + u) * stride + lane;
             const int cid = (q.lx + t / nyq) + (q.ly + t % nyq) * g.cn[0];
-            cr[u] = (t < ncell && cid >= 0 && cid < g.tc) ? F.cell_rng[cid] : make_int2(0, 0);
+            cr[u] = (t < ncell && cid >= 0 && cid < g.tc) ? gld(F.cell_rng + cid) : make_int2(0, 0);
         }
         for (int u = 0; u < WALK_BATCH; ++u) {
             const int t = (i0 + u) * stride + lane;
@@ -441,7 +457,7 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
             const int ix = q.lx + t / nyq, iy = q.ly + t % nyq;
             const int e = cr[u].y;
             for (int pp = cr[u].x; pp < e; ++pp) {
-                const int ce = F.cell_ent[pp];
+                const int ce = gld(F.cell_ent + pp);
                 const int s = ce >> 2;
                 if (s < tlo || s >= thi) continue;  // a segment of another body
                 // first meeting of s in the walk: the first cell of the walk inside the
@@ -452,9 +468,9 @@ __device__ void walk(const FlexDev& F, const FlexGrid& g, const CellRange& q, fl
                 if (seen) continue;
                 if (stride == 1 && met >= max_query) continue;
                 ++met;
-                const int2 sn = F.seg_node[s];
+                const int2 sn = gld(F.seg_node + s);
                 float nx, ny, w0, w1;
-                const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
+                const float pen = detect(p.x, p.y, gld(tpos + sn.x), gld(tpos + sn.y), qbuf, nx, ny, w0, w1);
                 const bool better = NEAREST ? (fabsf(pen) < fabsf(best.pen) && fabsf(pen) < qbuf)
                                             : (pen < best.pen && pen < 1e-4f && fabsf(pen) < qbuf);
                 if (better)
@@ -480,7 +496,7 @@ __device__ void scan_segments(const FlexDev& F, const CellRange& q, float2 p, in
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
             const int j = j0 + u * stride;
-            rb[u] = j < be_cnt ? F.seg_cells[be_off + j] : make_int4(1, 1, 0, 0);
+            rb[u] = j < be_cnt ? gld(F.seg_cells + be_off + j) : make_int4(1, 1, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
@@ -490,9 +506,9 @@ __device__ void scan_segments(const FlexDev& F, const CellRange& q, float2 p, in
             ++met;
             const int t = (imax_(r.x, q.lx) - q.lx) * nyq + (imax_(r.y, q.ly) - q.ly);
             const long long key = ((long long)t << 32) | (long long)s;
-            const int2 sn = F.seg_node[s];
+            const int2 sn = gld(F.seg_node + s);
             float nx, ny, w0, w1;
-            const float pen = detect(p.x, p.y, tpos[sn.x], tpos[sn.y], qbuf, nx, ny, w0, w1);
+            const float pen = detect(p.x, p.y, gld(tpos + sn.x), gld(tpos + sn.y), qbuf, nx, ny, w0, w1);
             if (pen < 1e-4f && fabsf(pen) < qbuf && (pen < best.pen || (pen == best.pen && key < best.key)))
                 best.take(pen, nx, ny, w0, w1, s, key);
         }
@@ -505,7 +521,7 @@ __device__ __forceinline__ bool pair_active(const FlexDev& F, const SceneDev* sc
     if (!F.gated) return true;
     const bool pad = ((local - scenes[F.body_scene[0]].hot.gate_base) % 100) == 0;
     const int ar3 = mod3_(local);
-    const int2 pb = F.pair_body[q];
+    const int2 pb = gld(F.pair_body + q);
     float a[4], b[4];
     scene_aabb(F.bodies[pb.x], ar3, a);
     scene_aabb(F.bodies[pb.y], ar3, b);
@@ -600,50 +616,50 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
     const FlexScratch X = flex_scratch(F, scr);
     const int rd = (local + 1) & 1;
     int2 ri = make_int2(0, -1);
-    if (F.n_tied > 0 && F.item_tied[w]) {
+    if (F.n_tied > 0 && gld(F.item_tied + w)) {
         // _apply_fem_fem_ties_fixed_kernel (flatworld/mixedcontact.py:773-811): a
         // spring toward the resolved edge point at the initial normal gap, every
         // substep, whatever the pair gate; tied items never slide
-        const int ts = F.tie_seg[w];
+        const int ts = gld(F.tie_seg + w);
         if (sub != 0) return;
         if (ts >= 0) {
-            const float2 x = F.bodies[F.item_body[w]].pos[rd][F.item_node[w]];
-            const float2* tpos = F.bodies[F.item_target[w]].pos[rd];
-            const int2 sn = F.seg_node[ts];
-            const float2 a = tpos[sn.x], c = tpos[sn.y];
-            const float4 wg = F.tie_wg[w];
+            const float2 x = gld(gld(&F.bodies[gld(F.item_body + w)].pos[rd]) + gld(F.item_node + w));
+            const float2* tpos = gld(&F.bodies[gld(F.item_target + w)].pos[rd]);
+            const int2 sn = gld(F.seg_node + ts);
+            const float2 a = gld(tpos + sn.x), c = gld(tpos + sn.y);
+            const float4 wg = gld(F.tie_wg + w);
             const float2 cp = tie_point(a, c, wg.x, wg.y);
             const float2 n = tie_normal(a, c);
-            const float k = F.item_penalty[w];
+            const float k = gld(F.item_penalty + w);
             const float gx = cp.x + wg.z * n.x, gy = cp.y + wg.z * n.y;
-            X.res_f[w] = make_float4(k * (gx - x.x), k * (gy - x.y), wg.x, wg.y);
-            const int2 ss = F.seg_slot[ts];
-            atomicAdd(&X.slot_start[F.item_slot[w]], 1);
+            gst(X.res_f + w, make_float4(k * (gx - x.x), k * (gy - x.y), wg.x, wg.y));
+            const int2 ss = gld(F.seg_slot + ts);
+            atomicAdd(&X.slot_start[gld(F.item_slot + w)], 1);
             atomicAdd(&X.slot_start[ss.x], 1);
             atomicAdd(&X.slot_start[ss.y], 1);
             ri = make_int2(1, ts);
         }
-        X.res_i[w] = ri;
+        gst(X.res_i + w, ri);
         return;
     }
     const FlexGrid g = *F.grid;
     const float qbuf = fmax_(fmax_(1e-4f, fmax_(g.gs[0], g.gs[1])), 0.016f);
-    if (pair_active(F, scenes, F.item_pair[w], local)) {
-        const FlexBody& SB = F.bodies[F.item_body[w]];
-        const int nid = F.item_node[w];
-        const float2 p = SB.pos[rd][nid];
-        const float2 vel = SB.vel[nid];
-        const int tb = F.item_target[w];
-        const FlexBody& ST = F.bodies[tb];
-        const float2* tpos = ST.pos[rd];
-        const int be_off = F.body_seg_off[tb], be_cnt = F.body_seg_off[tb + 1] - be_off;
-        const float penalty = F.item_penalty[w], friction = F.item_friction[w];
+    if (pair_active(F, scenes, gld(F.item_pair + w), local)) {
+        const FlexBody* SB = F.bodies + gld(F.item_body + w);
+        const int nid = gld(F.item_node + w);
+        const float2 p = gld(gld(&SB->pos[rd]) + nid);
+        const float2 vel = gld(gld(&SB->vel) + nid);
+        const int tb = gld(F.item_target + w);
+        const FlexBody* ST = F.bodies + tb;
+        const float2* tpos = gld(&ST->pos[rd]);
+        const int be_off = gld(F.body_seg_off + tb), be_cnt = gld(F.body_seg_off + tb + 1) - be_off;
+        const float penalty = gld(F.item_penalty + w), friction = gld(F.item_friction + w);
 
         Best b;
-        const int cached = F.cache[w];
+        const int cached = gld(F.cache + w);
         if (cached >= be_off && cached < be_off + be_cnt) {
-            const int2 sn = F.seg_node[cached];
-            const float2 a = tpos[sn.x], c = tpos[sn.y];
+            const int2 sn = gld(F.seg_node + cached);
+            const float2 a = gld(tpos + sn.x), c = gld(tpos + sn.y);
             float lb0 = 1e30f, lb1 = 1e30f, ub0 = -1e30f, ub1 = -1e30f;
             lb0 = fmin_(lb0, a.x);
             lb1 = fmin_(lb1, a.y);
@@ -673,7 +689,7 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
         bool reach = false;
         if (g.tc > 0) {
             q = cell_range(g, p.x - qbuf, p.y - qbuf, p.x + qbuf, p.y + qbuf);
-            const int4 tbc = F.body_cells[tb];
+            const int4 tbc = gld(F.body_cells + tb);
             reach = !(tbc.x > q.ux || tbc.z < q.lx || tbc.y > q.uy || tbc.w < q.ly);
         }
         if (!b.found && reach) {
@@ -730,14 +746,15 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             }
         }
         if (!b.found) cache_new = -1;
-        if (sub == 0) F.cache[w] = cache_new;
+        if (sub == 0) gst(F.cache + w, cache_new);
         if (b.found && b.pen < 0.0f) {
             const float nfx = ((-b.nx) * penalty) * b.pen;
             const float nfy = ((-b.ny) * penalty) * b.pen;
             float tx = nfx, ty = nfy;
             if (friction > 1e-9f) {
-                const int2 sn = F.seg_node[b.s];
-                const float2 v0 = ST.vel[sn.x], v1 = ST.vel[sn.y];
+                const int2 sn = gld(F.seg_node + b.s);
+                const float2* tv = gld(&ST->vel);
+                const float2 v0 = gld(tv + sn.x), v1 = gld(tv + sn.y);
                 const float svx = v0.x * b.w0 + v1.x * b.w1;
                 const float svy = v0.y * b.w0 + v1.y * b.w1;
                 const float rvx = vel.x - svx, rvy = vel.y - svy;
@@ -753,15 +770,15 @@ __global__ __launch_bounds__(FLEX_BLOCK, 4) void flex_items_kernel(const FlexDev
             }
             ri = make_int2(1, b.s);
             if (sub == 0) {
-                X.res_f[w] = make_float4(tx, ty, b.w0, b.w1);
-                const int2 ss = F.seg_slot[b.s];
-                atomicAdd(&X.slot_start[F.item_slot[w]], 1);
+                gst(X.res_f + w, make_float4(tx, ty, b.w0, b.w1));
+                const int2 ss = gld(F.seg_slot + b.s);
+                atomicAdd(&X.slot_start[gld(F.item_slot + w)], 1);
                 atomicAdd(&X.slot_start[ss.x], 1);
                 atomicAdd(&X.slot_start[ss.y], 1);
             }
         }
     }
-    if (sub == 0) X.res_i[w] = ri;
+    if (sub == 0) gst(X.res_i + w, ri);
 }
 
 // The node-major contribution lists of one group (one workgroup): slot ranges by
