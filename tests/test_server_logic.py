@@ -11,7 +11,8 @@ from types import SimpleNamespace
 import numpy as np
 import pytest
 
-from flatworld_amd.server import Job, SimServer, poisson_arrivals
+from flatworld_amd import server as server_mod
+from flatworld_amd.server import Job, SimServer, default_prep_workers, poisson_arrivals
 
 
 class FakeBatch:
@@ -157,6 +158,19 @@ def test_frames_shorten_near_a_horizon():
     # warm-up + 100 substeps = 101: 60 + 30 + 8 + 8 (overshoot 5 < 8)
     assert b.frames == [60, 30, 8, 8]
     assert sum(b.frames) - 101 < 8
+
+
+def test_default_frames_and_prep_workers(monkeypatch):
+    """By default frames halve down to 15 substeps (60, 30, 15), so a job waits at most
+    14 substeps past its horizon; preparation takes half the CPU share the process may
+    run on, at most 8 threads and at least one, and OMP_NUM_THREADS caps that share."""
+    srv = SimServer(batch=FakeBatch(capacity=100), frame=60)
+    assert srv.frame_lengths == [60, 30, 15]
+    assert SimServer(batch=FakeBatch(capacity=100), frame=10).frame_lengths == [10]
+    for share, omp, want in ((16, "", 8), (32, "", 8), (6, "", 3), (1, "", 1), (16, "4", 2), (16, "x", 8)):
+        monkeypatch.setattr(server_mod.os, "sched_getaffinity", lambda pid, n=share: set(range(n)))
+        monkeypatch.setenv("OMP_NUM_THREADS", omp)
+        assert default_prep_workers() == want, (share, omp)
 
 
 class FakeAsyncBatch(FakePrepBatch):
