@@ -150,7 +150,7 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs, tune_resident=24):
     dozen jobs at a time, a different regime from the 4096-scene arena ``arena_kw``
     was tuned for (there, one workgroup per tile wins; here a persistent grid does).
     """
-    from flatworld_amd import autotune
+    from flatworld_amd import _native, autotune
     from flatworld_amd import workloads as W
     from flatworld_amd.server import Job, SimServer, poisson_arrivals
 
@@ -167,19 +167,25 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs, tune_resident=24):
         arena_kw = autotune.arena_kwargs(dict(res, frame=frame))
     log(f"[bench] serving: tuning {tuning}")
 
+    # the tiling cache starts empty for the burst and again for the stream (the stream
+    # re-serves the burst's payloads): only connectivities repeated within a run hit it
     nb = min(burst_jobs, n_jobs)
+    _native.tiling_cache_clear()
     srv = SimServer(frame=frame, keep_results=False, **arena_kw)
     srv.run([Job(i, specs[i], 0.0) for i in range(nb)])
     burst = srv.stats()
+    cache_burst = _native.tiling_cache_stats()
     srv.batch.close()
     capacity = burst["jobs"] / burst["wall_s"]
     rate = 0.8 * capacity
     log(f"[bench] serving: burst of {nb} jobs in {burst['wall_s']:.2f}s -> capacity {capacity:.1f} jobs/s")
 
     arrivals = poisson_arrivals(n_jobs, rate, rng)
+    _native.tiling_cache_clear()
     srv = SimServer(frame=frame, keep_results=False, **arena_kw)
     srv.run([Job(i, specs[i], float(arrivals[i])) for i in range(n_jobs)])
     st = srv.stats()
+    cache_stream = _native.tiling_cache_stats()
     srv.batch.close()
     # algorithmic bytes the served stream moved per second (each job: its per-substep bytes x horizon)
     done = {r.job_id for r in srv.results}
@@ -200,6 +206,8 @@ def serving_bench(n_jobs, seed, arena_kw, frame, burst_jobs, tune_resident=24):
         "gpu_busy_frac": st["gpu_busy_frac"],
         "frame_substeps": frame,
         "tuning": tuning,
+        "tiling_cache": {"burst": cache_burst, "stream": cache_stream,
+                         "note": "tilings reused by connectivity (jobs of one grid side share it); empty at the start of each"},
         "workload": "Poisson stream: grid side U{16..256}, Elastic/Neo-Hookean/J2, plane/height field/voxel ground, "
                     "horizon U{60..600}; latency = arrival -> final positions on the host",
     }

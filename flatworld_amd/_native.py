@@ -212,6 +212,8 @@ SIGNATURES = {
     "fw_scene_read": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_scene_write": (ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64]),
     "fw_tiling_check": (ctypes.c_int32, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    "fw_tiling_cache_stats": (ctypes.c_int32, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "fw_tiling_cache_clear": (ctypes.c_int32, []),
     "fw_bandwidth_probe": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "fw_bandwidth_ceiling": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32),
@@ -397,6 +399,20 @@ def tiling_check(spec, tile_nodes=512, tile_slots=1024):
     stats = (ctypes.c_int64 * 4)()
     check(load().fw_tiling_check(ctypes.byref(d), int(tile_nodes), int(tile_slots), stats), "fw_tiling_check")
     return dict(tiles=stats[0], halo_slots=stats[1], max_slots=stats[2], csr_entries=stats[3])
+
+
+
+
+def tiling_cache_stats():
+    """Tilings reused by connectivity in fw_scene_prepare: hits, misses, bytes kept, entries."""
+    h, m, b, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+    check(load().fw_tiling_cache_stats(ctypes.byref(h), ctypes.byref(m), ctypes.byref(b), ctypes.byref(e)),
+          "fw_tiling_cache_stats")
+    return {"hits": h.value, "misses": m.value, "bytes": b.value, "entries": e.value}
+
+
+def tiling_cache_clear():
+    check(load().fw_tiling_cache_clear(), "fw_tiling_cache_clear")
 
 
 def bandwidth_ceiling(device=0, nbytes=1 << 30, reps=10):

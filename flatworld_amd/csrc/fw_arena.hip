@@ -465,6 +465,110 @@ int build_tiling(const fw_scene_desc* d, int tile_nodes, int tile_slots, int nod
 }
 
 // Height-field block: nx float heights, then (8-byte aligned) nx float2 unit normals.
+// Tilings of prepared scenes, kept by connectivity.  A tiling is valid for a scene
+// exactly when the scene has the tiling's connectivity (the slot, LDS and rank limits
+// depend on it alone) and, for springs, its rest lengths (they ride in the slot
+// records); the node positions only choose the Morton order, i.e. which valid tiling
+// is built.  Results do not depend on the tiling, so a job whose connectivity was seen
+// before reuses that tiling and skips the tile cut.  Keys are compared in full; the
+// least recently used entries go once the cache holds more than TC_MAX_BYTES.
+struct TilingKey {
+    uint64_t hash;
+    int32_t n_nodes, n_elems, corners, tile_nodes, tile_slots, node_cap;
+    std::vector<int32_t> conn;
+    std::vector<float> rest;
+};
+struct TilingCacheEntry {
+    std::shared_ptr<const TilingKey> key;
+    std::shared_ptr<const Tiling> T;
+    int64_t bytes;
+    uint64_t used;
+};
+constexpr int64_t TC_MAX_BYTES = int64_t(512) << 20;
+std::mutex g_tc_mu;
+std::vector<TilingCacheEntry> g_tc;
+int64_t g_tc_bytes = 0, g_tc_hits = 0, g_tc_misses = 0;
+uint64_t g_tc_clock = 0;
+
+int64_t tiling_bytes(const Tiling& T) {
+    return 4 * (int64_t)(T.node_perm.size() + T.node_inv.size() + T.elem_perm.size() + T.elem_inv.size() +
+                         T.rec.size() + T.hel.size() + T.hnode.size() + T.push.size()) +
+           (int64_t)sizeof(TileDev) * (int64_t)T.tiles.size() + 2 * (int64_t)T.csr_ptr.size();
+}
+
+uint64_t tiling_key_hash(const fw_scene_desc* d) {
+    uint64_t h = 1469598103934665603ull;  // FNV-1a over the connectivity words (and rest lengths)
+    const int64_t n = corners_of(d) * (int64_t)d->n_elems;
+    for (int64_t i = 0; i < n; ++i) h = (h ^ (uint32_t)d->conn[i]) * 1099511628211ull;
+    if (d->mat_type == FW_MAT_SPRING)
+        for (int64_t i = 0; i < d->n_elems; ++i) {
+            uint32_t b;
+            std::memcpy(&b, d->rest_len + i, 4);
+            h = (h ^ b) * 1099511628211ull;
+        }
+    return h;
+}
+
+// The key's scalar fields (cheap, under the lock) and its arrays (outside it).
+bool tiling_key_head(const TilingKey& k, uint64_t h, const fw_scene_desc* d, int tn, int ts, int cap) {
+    return k.hash == h && k.n_nodes == d->n_nodes && k.n_elems == d->n_elems && k.corners == corners_of(d) &&
+           k.tile_nodes == tn && k.tile_slots == ts && k.node_cap == cap;
+}
+bool tiling_key_arrays(const TilingKey& k, const fw_scene_desc* d) {
+    if (std::memcmp(k.conn.data(), d->conn, sizeof(int32_t) * k.corners * (size_t)d->n_elems) != 0) return false;
+    return k.corners != 2 || std::memcmp(k.rest.data(), d->rest_len, sizeof(float) * (size_t)d->n_elems) == 0;
+}
+
+// The tiling of d for shape (tn, ts, cap): a copy of a kept one, or built and kept.
+int cached_tiling(const fw_scene_desc* d, int tn, int ts, int cap, Tiling& out) {
+    const uint64_t h = tiling_key_hash(d);
+    std::vector<TilingCacheEntry> cand;
+    {
+        std::lock_guard<std::mutex> lock(g_tc_mu);
+        for (const auto& c : g_tc)
+            if (tiling_key_head(*c.key, h, d, tn, ts, cap)) cand.push_back(c);
+    }
+    for (const auto& c : cand)
+        if (tiling_key_arrays(*c.key, d)) {
+            out = *c.T;
+            std::lock_guard<std::mutex> lock(g_tc_mu);
+            ++g_tc_hits;
+            for (auto& e : g_tc)
+                if (e.T == c.T) e.used = ++g_tc_clock;
+            return FW_OK;
+        }
+    const int rc = build_tiling(d, tn, ts, cap, out);
+    if (rc) return rc;
+    auto key = std::make_shared<TilingKey>();
+    key->hash = h;
+    key->n_nodes = d->n_nodes;
+    key->n_elems = d->n_elems;
+    key->corners = corners_of(d);
+    key->tile_nodes = tn;
+    key->tile_slots = ts;
+    key->node_cap = cap;
+    key->conn.assign(d->conn, d->conn + key->corners * (size_t)d->n_elems);
+    if (key->corners == 2) key->rest.assign(d->rest_len, d->rest_len + d->n_elems);
+    TilingCacheEntry e;
+    e.bytes = tiling_bytes(out) + 4 * (int64_t)(key->conn.size() + key->rest.size());
+    e.key = std::move(key);
+    e.T = std::make_shared<const Tiling>(out);
+    std::lock_guard<std::mutex> lock(g_tc_mu);
+    ++g_tc_misses;
+    if (e.bytes > TC_MAX_BYTES) return FW_OK;
+    while (!g_tc.empty() && g_tc_bytes + e.bytes > TC_MAX_BYTES) {
+        size_t lru = 0;
+        for (size_t i = 1; i < g_tc.size(); ++i)
+            if (g_tc[i].used < g_tc[lru].used) lru = i;
+        g_tc_bytes -= g_tc[lru].bytes;
+        g_tc.erase(g_tc.begin() + (std::ptrdiff_t)lru);
+    }
+    e.used = ++g_tc_clock;
+    g_tc_bytes += e.bytes;
+    g_tc.push_back(std::move(e));
+    return FW_OK;
+}
+
 int64_t hf_normals_offset(int nx) { return (4 * (int64_t)nx + 7) / 8 * 8; }
 int64_t hf_bytes(int nx) { return hf_normals_offset(nx) + 8 * (int64_t)nx; }
 
@@ -1267,7 +1371,7 @@ int prepare_scene(const fw_scene_desc* d, const fw_tuning& tune, fw_arena* cache
             rc = tiling_for(cache_arena, d, &Tp);
         } else {
             Tp = std::make_unique<Tiling>();
-            rc = build_tiling(d, tune.tile_nodes, tune.tile_slots, node_cap_of(tune, d), *Tp);
+            rc = cached_tiling(d, tune.tile_nodes, tune.tile_slots, node_cap_of(tune, d), *Tp);
         }
         if (rc) return rc;
         P->T = std::move(*Tp);
@@ -1607,6 +1711,24 @@ extern "C" {
 const char* fw_last_error(void) { return g_err_static ? g_err_static : g_err.c_str(); }
 
 int32_t fw_abi_version(void) { return FW_ABI_VERSION; }
+
+int32_t fw_tiling_cache_stats(int64_t* hits, int64_t* misses, int64_t* bytes, int32_t* entries) {
+    std::lock_guard<std::mutex> lock(g_tc_mu);
+    if (hits) *hits = g_tc_hits;
+    if (misses) *misses = g_tc_misses;
+    if (bytes) *bytes = g_tc_bytes;
+    if (entries) *entries = (int32_t)g_tc.size();
+    return FW_OK;
+}
+
+int32_t fw_tiling_cache_clear(void) {
+    std::lock_guard<std::mutex> lock(g_tc_mu);
+    g_tc.clear();
+    g_tc_bytes = 0;
+    g_tc_hits = 0;
+    g_tc_misses = 0;
+    return FW_OK;
+}
 
 int32_t fw_arena_create(int32_t device, int64_t capacity_bytes, const fw_tuning* tuning, fw_arena** out) {
     if (!out) return fail(FW_E_INVALID, "out is null");

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 7
+#define FW_ABI_VERSION 8
 #define FW_MAX_GROUNDS 8
 
 /* status codes */
@@ -436,6 +436,16 @@ int32_t fw_wait_launches(fw_arena* a, int32_t keep);
  * stats[0] tiles, [1] halo slots, [2] largest tile slot count, [3] CSR entries.
  * Returns FW_E_INVALID with a message if any invariant fails. */
 int32_t fw_tiling_check(const fw_scene_desc* desc, int32_t tile_nodes, int32_t tile_slots, int64_t* stats);
+
+/* fw_scene_prepare keeps the tilings it builds, by connectivity (and, for springs,
+ * rest lengths) and kernel shape: a later scene with the same connectivity reuses the
+ * tiling, whatever its node positions, since results do not depend on the tiling.
+ * Keys are compared in full; at most 512 MiB of host memory, least recently used
+ * first out.  No reference counterpart (the reference builds no tiling): the
+ * served form's admission cost.  Counters since the last clear; clear also empties
+ * the cache. */
+int32_t fw_tiling_cache_stats(int64_t* hits, int64_t* misses, int64_t* bytes, int32_t* entries);
+int32_t fw_tiling_cache_clear(void);
 
 /* Synchronous copies between a scene and host memory (caller's order). */
 int32_t fw_scene_read(fw_arena* a, int32_t scene_id, int32_t field, void* out, int64_t out_bytes);

@@ -176,7 +176,7 @@ def test_library_exports_declared_symbols():
     for name in sorted(declared):
         assert hasattr(L, name), name
         assert name in _native.SIGNATURES, f"{name} not bound in _native.SIGNATURES"
-    assert L.fw_abi_version() == 7
+    assert L.fw_abi_version() == 8
 
 
 @pytest.mark.parametrize("tile_nodes,tile_slots", [(512, 1024), (64, 1024), (1024, 2048), (7, 1024)])
@@ -439,3 +439,33 @@ def test_tied_contact_flags_its_work_items():
     g = build_flex_group(b, cs, False, sum(x.nnodes for x in b))
     assert g.n_tied == int(np.sum(g.item_pair == 1)) > 0
     assert np.array_equal(g.tied_flags(), (g.item_pair == 1).astype(np.int32))
+
+
+def test_tiling_cache_reuses_by_connectivity():
+    """fw_scene_prepare reuses a tiling for a scene with the same connectivity (and
+    kernel shape) whatever its positions and material; another connectivity, another
+    shape, or springs with other rest lengths build their own."""
+    ground = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+
+    def block(side, lower, mat):
+        return W.one_body_spec(W.grid_domain(side, lower, 0.4, mat, [fw.Gravity([0.0, -9.8])]), ground, horizon=10)
+
+    tun = dict(tile_nodes=128, tile_slots=512, block=256, grid_per_cu=0)
+    _native.tiling_cache_clear()
+    a = _native.Prepared(block(40, (0.3, 0.0), fw.Elastic(1e4, 0.3, 10.0)), tun)
+    b = _native.Prepared(block(40, (1.7, 0.2), fw.MisesPlastic(2e4, 0.3, 10.0, 50.0, 1e2)), tun)
+    st = _native.tiling_cache_stats()
+    assert (st["hits"], st["misses"], st["entries"]) == (1, 1, 1) and st["bytes"] > 0
+    c = _native.Prepared(block(40, (0.9, 0.5), fw.Elastic(3e4, 0.25, 20.0)), tun)
+    assert a.footprint == c.footprint
+    _native.Prepared(block(41, (0.3, 0.0), fw.Elastic(1e4, 0.3, 10.0)), tun)
+    _native.Prepared(block(40, (0.3, 0.0), fw.Elastic(1e4, 0.3, 10.0)), dict(tun, tile_nodes=256))
+    st = _native.tiling_cache_stats()
+    assert (st["hits"], st["misses"], st["entries"]) == (2, 3, 3)
+    lat = [W.spring_lattice(12, (0.1, 0.1), h, [200.0, 0.3, 0.01], [fw.Gravity([0.0, -9.8])]) for h in (0.01, 0.01, 0.02)]
+    for d in lat:
+        _native.Prepared(W.one_body_spec(d, ground, horizon=10), dict(tile_nodes=64, tile_slots=512, block=256, grid_per_cu=0))
+    st = _native.tiling_cache_stats()
+    assert (st["hits"], st["misses"]) == (3, 5)  # the second lattice hits, the third (other rest lengths) misses
+    _native.tiling_cache_clear()
+    assert _native.tiling_cache_stats() == {"hits": 0, "misses": 0, "bytes": 0, "entries": 0}
