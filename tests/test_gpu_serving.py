@@ -111,6 +111,38 @@ def test_prepared_scenes_equal_direct_adds():
     b.close()
 
 
+def test_reused_tiling_matches_oracle():
+    """A scene admitted with a tiling fw_scene_prepare kept from another scene of the same
+    connectivity (other positions, material and ground) gives the oracle's bits: the
+    tiling is valid whatever the positions, and results do not depend on it."""
+    from flatworld_amd import _native
+    from oracle.oracle import OracleScene
+
+    hf = W.sinusoid_heightfield(W.rng_for(3))
+    plane = fw.GroundDomain(2, [0.0, 0.0], [0.0, 1.0])
+    first = W.one_body_spec(W.grid_domain(40, (1.2, 0.5), 0.3, fw.Elastic(1e4, 0.3, 10.0), [fw.Gravity([0.0, -9.8])]),
+                            plane, horizon=120)
+    second = W.one_body_spec(W.grid_domain(40, (0.3, W.ground_top(hf)), 0.4, fw.MisesPlastic(2e4, 0.3, 10.0, 60.0, 1e2),
+                                           [fw.Gravity([0.0, -9.8])], jitter=0.1, seed=4), hf, horizon=120)
+    b = FemBatch(tile_nodes=128, tile_slots=1024)
+    _native.tiling_cache_clear()
+    p1 = b.prepare(first)
+    p2 = b.prepare(second)
+    st = _native.tiling_cache_stats()
+    log(f"[case] reused tiling: cache {st}")
+    assert (st["hits"], st["misses"]) == (1, 1)
+    i1, i2 = b.add_prepared(p1), b.add_prepared(p2)
+    assert b.arena.info()["n_tiles"] > 20
+    b.step(121, graph=False)
+    for sid, spec in ((i1, first), (i2, second)):
+        o = OracleScene(spec)
+        o.warmup()
+        o.run(120)
+        assert np.isfinite(o.x).all()
+        assert np.array_equal(b.positions(sid), o.x) and np.array_equal(b.velocities(sid), o.v)
+    b.close()
+
+
 @pytest.mark.timeout(400)
 def test_served_c5_stream_matches_oracle():
     """A served stream shaped like the serving configuration (grid side 16..256, horizon
